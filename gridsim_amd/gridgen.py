@@ -1,0 +1,245 @@
+"""Seeded synthetic meshed grids for the N-1 / batching / serving workloads.
+
+The reference has no such generator (its synthetic network, src/GridSim.Core/Synthetic/SyntheticGb.cs, is GB-shaped
+and unrelated). This one builds the unit-square grids the workloads are specified on:
+
+* buses uniform in [0, 1]^2 from the seed;
+* a random spanning tree over the 3-nearest-neighbour graph (Kruskal on random edge weights; components the 3-NN
+  graph leaves apart are joined by their shortest connecting edge), then the shortest remaining nearest-neighbour
+  edges up to the branch count;
+* r ~ U[a, b] pu, x = r * U[3, 10], line charging b ~ U[0, 0.05] pu; a fixed fraction of branches are transformers
+  with tap ~ U[0.95, 1.05] (shift 0), placed on radial (bridge) branches first, the way step-up transformers connect
+  units, so that no off-nominal tap sits inside a loop;
+* every branch's series impedance (r and x together, so x/r is kept) is multiplied by one per-grid factor
+  kappa = min(1, theta_max / theta_dc), where theta_dc is the largest bus angle of the DC power flow of the drawn
+  grid and theta_max = 1 rad;
+* one slack (the bus nearest the centre), ``n_pv`` PV buses drawn at random, the rest PQ;
+* PQ loads P ~ U[0, 1] pu at power factor 0.95 (or lognormal(mu, sigma) pu at pf ~ U[pf_lo, pf_hi]);
+* PV generation totals 1.03 x the load, split evenly; voltage set points ~ U[1.00, 1.05]; Q limits +-inf.
+
+Everything is drawn from one ``numpy.random.default_rng(seed)`` stream in a fixed order, so a (sizes, seed) pair
+names one network on every machine.
+
+Why the two adjustments: drawn literally, these grids have no AC solution. A random spanning tree plus short
+nearest-neighbour chords is electrically long (about 200 hops across 2,000 buses, a mean effective reactance of
+12 pu from the slack with impedances drawn independently of length), so the DC angles span about 194 rad at the
+drawn loading; and a 5 % off-nominal tap on a 0.003 pu branch inside a loop drives about 17 pu of circulating
+reactive power. Scaling impedance (not load) keeps every MW and MVAr figure as drawn.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+from scipy.spatial import cKDTree
+
+from .network import Network, PQ, PV, SLACK, bridges
+
+BASE_SEED = 20261203
+
+
+class _Dsu:
+    def __init__(self, n):
+        self.p = np.arange(n)
+
+    def find(self, a):
+        p = self.p
+        root = a
+        while p[root] != root:
+            root = p[root]
+        while p[a] != root:
+            p[a], a = root, p[a]
+        return root
+
+    def union(self, a, b):
+        ra, rb = self.find(a), self.find(b)
+        if ra == rb:
+            return False
+        if ra < rb:
+            self.p[rb] = ra
+        else:
+            self.p[ra] = rb
+        return True
+
+
+def _knn_edges(tree: cKDTree, pts: np.ndarray, k: int):
+    n = len(pts)
+    kk = min(k + 1, n)
+    d, idx = tree.query(pts, k=kk)
+    i = np.repeat(np.arange(n), kk - 1)
+    j = idx[:, 1:].reshape(-1)
+    a = np.minimum(i, j)
+    b = np.maximum(i, j)
+    key = np.unique(a.astype(np.int64) * n + b)
+    a = (key // n).astype(np.int64)
+    b = (key % n).astype(np.int64)
+    length = np.hypot(pts[a, 0] - pts[b, 0], pts[a, 1] - pts[b, 1])
+    return a, b, length
+
+
+def _topology(n: int, n_branch: int, rng: np.random.Generator):
+    if n_branch < n - 1:
+        raise ValueError(f"{n_branch} branches cannot connect {n} buses")
+    pts = rng.random((n, 2))
+    tree = cKDTree(pts)
+    a, b, length = _knn_edges(tree, pts, 3)
+    w = rng.random(len(a))
+    dsu = _Dsu(n)
+    chosen = []
+    in_tree = np.zeros(len(a), dtype=bool)
+    for e in np.argsort(w, kind="stable"):
+        if dsu.union(int(a[e]), int(b[e])):
+            chosen.append((int(a[e]), int(b[e])))
+            in_tree[e] = True
+    # join components the 3-NN graph left apart: shortest edge out of the component holding the lowest bus index
+    while len(chosen) < n - 1:
+        roots = np.array([dsu.find(i) for i in range(n)])
+        r0 = roots[0]
+        inside = np.nonzero(roots == r0)[0]
+        outside_mask = roots != r0
+        out_idx = np.nonzero(outside_mask)[0]
+        sub = cKDTree(pts[out_idx])
+        d, j = sub.query(pts[inside], k=1)
+        best = int(np.argmin(d))
+        u, v = int(inside[best]), int(out_idx[j[best]])
+        dsu.union(u, v)
+        chosen.append((min(u, v), max(u, v)))
+    have = set(chosen)
+    k = 3
+    extra_needed = n_branch - len(chosen)
+    extras = []
+    while extra_needed > 0:
+        a, b, length = _knn_edges(tree, pts, k)
+        order = np.lexsort((b, a, length))
+        for e in order:
+            key = (int(a[e]), int(b[e]))
+            if key in have:
+                continue
+            have.add(key)
+            extras.append(key)
+            extra_needed -= 1
+            if extra_needed == 0:
+                break
+        k += 2
+        if k > n + 2:
+            raise ValueError(f"cannot place {n_branch} distinct branches on {n} buses")
+    edges = np.array(chosen + extras, dtype=np.int64)
+    return pts, edges
+
+
+def _bridge_mask(n: int, edges: np.ndarray) -> np.ndarray:
+    from .network import bridges as _bridges
+    nb = len(edges)
+    probe = Network(base_mva=100.0, bus_type=np.zeros(n), pd=np.zeros(n), qd=np.zeros(n), gs=np.zeros(n),
+                    bs=np.zeros(n), vm=np.ones(n), va_deg=np.zeros(n), br_from=edges[:, 0], br_to=edges[:, 1],
+                    br_r=np.zeros(nb), br_x=np.ones(nb), br_b=np.zeros(nb), br_tap=np.zeros(nb),
+                    br_shift_deg=np.zeros(nb), br_in_service=np.ones(nb), gen_bus=np.zeros(0), gen_pg=np.zeros(0),
+                    gen_qg=np.zeros(0), gen_vg=np.zeros(0))
+    return _bridges(probe)
+
+
+def _dc_angle_range(n: int, edges: np.ndarray, x: np.ndarray, pinj: np.ndarray, slack: int) -> float:
+    """Largest |bus angle| (rad) of the DC power flow B' theta = P with the slack angle at 0."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+    f, t = edges[:, 0], edges[:, 1]
+    bx = 1.0 / x
+    bmat = sp.coo_matrix((np.concatenate([bx, bx, -bx, -bx]),
+                          (np.concatenate([f, t, f, t]), np.concatenate([f, t, t, f]))), shape=(n, n)).tocsc()
+    keep = np.arange(n) != slack
+    theta = spl.spsolve(bmat[keep][:, keep].tocsc(), pinj[keep])
+    return float(np.abs(theta).max()) if len(theta) else 0.0
+
+
+def synthetic_grid(n_bus: int, n_branch: int, n_pv: int, seed: int, *, xfmr_frac: float = 0.10,
+                   r_range=(0.001, 0.05), load_model: str = "uniform", pf: float = 0.95,
+                   pf_range=(0.9, 0.98), lognormal=(-1.0, 1.0), base_mva: float = 100.0,
+                   theta_max: float = 1.0, name: str | None = None) -> Network:
+    """One seeded synthetic grid (see the module docstring for the recipe)."""
+    if n_pv > n_bus - 1:
+        raise ValueError("too many PV buses")
+    rng = np.random.default_rng(seed)
+    pts, edges = _topology(n_bus, n_branch, rng)
+    nb = len(edges)
+    r = rng.uniform(r_range[0], r_range[1], nb)
+    x = r * rng.uniform(3.0, 10.0, nb)
+    bch = rng.uniform(0.0, 0.05, nb)
+    n_x = int(round(xfmr_frac * nb))
+    tap = np.zeros(nb)
+    radial = _bridge_mask(n_bus, edges)
+    cand = np.nonzero(radial)[0]
+    if len(cand) >= n_x:
+        xf = rng.choice(cand, size=n_x, replace=False)
+    else:
+        rest = np.nonzero(~radial)[0]
+        xf = np.concatenate([cand, rng.choice(rest, size=n_x - len(cand), replace=False)])
+    tap[xf] = rng.uniform(0.95, 1.05, n_x)
+
+    slack = int(np.argmin(np.hypot(pts[:, 0] - 0.5, pts[:, 1] - 0.5)))
+    others = np.array([i for i in range(n_bus) if i != slack], dtype=np.int64)
+    pv = np.sort(rng.choice(others, size=n_pv, replace=False))
+    bus_type = np.full(n_bus, PQ, dtype=np.int32)
+    bus_type[slack] = SLACK
+    bus_type[pv] = PV
+    pq = np.nonzero(bus_type == PQ)[0]
+
+    pd = np.zeros(n_bus)
+    qd = np.zeros(n_bus)
+    if load_model == "uniform":
+        p = rng.uniform(0.0, 1.0, len(pq))
+        pfs = np.full(len(pq), pf)
+    elif load_model == "lognormal":
+        p = rng.lognormal(lognormal[0], lognormal[1], len(pq))
+        pfs = rng.uniform(pf_range[0], pf_range[1], len(pq))
+    else:
+        raise ValueError(load_model)
+    pd[pq] = p * base_mva
+    qd[pq] = pd[pq] * np.tan(np.arccos(pfs))
+
+    total = float(pd.sum())
+    gen_bus = np.concatenate([[slack], pv]).astype(np.int32)
+    gen_pg = np.concatenate([[0.0], np.full(n_pv, 1.03 * total / max(n_pv, 1))])
+    gen_vg = rng.uniform(1.00, 1.05, len(gen_bus))
+    gen_qg = np.zeros(len(gen_bus))
+    pinj = -pd / base_mva
+    np.add.at(pinj, gen_bus, gen_pg / base_mva)
+    theta = _dc_angle_range(n_bus, edges, x, pinj, slack)
+    kappa = min(1.0, theta_max / theta) if theta > 0 else 1.0
+    r = r * kappa
+    x = x * kappa
+    return Network(
+        base_mva=base_mva, bus_type=bus_type, pd=pd, qd=qd, gs=np.zeros(n_bus), bs=np.zeros(n_bus),
+        vm=np.ones(n_bus), va_deg=np.zeros(n_bus),
+        br_from=edges[:, 0], br_to=edges[:, 1], br_r=r, br_x=x, br_b=bch, br_tap=tap,
+        br_shift_deg=np.zeros(nb), br_in_service=np.ones(nb, dtype=np.uint8),
+        gen_bus=gen_bus, gen_pg=gen_pg, gen_qg=gen_qg, gen_vg=gen_vg,
+        name=name or f"synthetic-{n_bus}b-{nb}br-s{seed}", seed=seed)
+
+
+# ---- the five workloads -------------------------------------------------------------------------------------------
+
+def config1_grid(seed: int = BASE_SEED + 1) -> Network:
+    """118 buses (1 slack / 53 PV / 64 PQ), 186 branches, flat start."""
+    return synthetic_grid(118, 186, 53, seed, name="synthetic-118")
+
+
+def config2_grid(seed: int = BASE_SEED + 2) -> Network:
+    """2,000 buses (1 slack / 300 PV / 1,699 PQ), 2,700 branches: the N-1 screening grid."""
+    return synthetic_grid(2000, 2700, 300, seed, name="synthetic-2000")
+
+
+def config2_outages(net: Network) -> np.ndarray:
+    """Every non-bridge in-service branch, in branch order."""
+    return np.nonzero((net.br_in_service != 0) & ~bridges(net))[0].astype(np.int32)
+
+
+def sized_grid(n_bus: int, seed: int, pv_frac: float = 0.15, branch_ratio: float = 1.35) -> Network:
+    """Uneven-size workloads: branches 1.35 n, PV fraction 15 %."""
+    return synthetic_grid(n_bus, int(round(branch_ratio * n_bus)), max(1, int(round(pv_frac * n_bus))), seed,
+                          name=f"synthetic-{n_bus}")
+
+
+def config4_grid(seed: int = BASE_SEED + 4) -> Network:
+    """70,000 buses (8,000 PV), 88,000 branches, 12 % transformers, lognormal loads."""
+    return synthetic_grid(70000, 88000, 8000, seed, xfmr_frac=0.12, load_model="lognormal",
+                          name="synthetic-70000")

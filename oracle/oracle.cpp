@@ -1,0 +1,1074 @@
+// CPU oracle: a C++ restatement of GridSim's CPU sparse Newton-Raphson load flow (TEST INFRASTRUCTURE ONLY).
+//
+// Used by tests/, the smoke check and the benchmark's CPU-baseline leg as the checker; the product never links it.
+// Build: oracle/Makefile (g++ -O2 -ffp-contract=off; no FMA anywhere, as the reference forbids it,
+// MultiCaseSparseLu.cs:10-16).
+//
+// Reference files restated here (paths relative to the reference root):
+//   src/GridSim.Core/PowerFlow/SparseNewtonRaphsonPowerFlow.cs  (BuildY :382-402, ComputeInjections :269-289,
+//       RunNewton :168-265, BuildJacobianStructure :296-332, RefillJacobian :342-380)
+//   src/GridSim.Core/PowerFlow/NewtonShell.cs  (SolveStep :74-120, ApplyStep :128-136, DivergedBeyondLimit :177-178,
+//       EquationPositions :327-336, BuildBusSpec :433-486, AssembleResult :649-697)
+//   src/GridSim.Core/PowerFlow/AdmittanceStamp.cs:35-75, CpuSparseLuSolver.cs:30-48
+//   src/GridSim.Core/Numerics/Sparse.cs  (CsrMatrix.Builder :33-124, ComplexCsr.Builder :145-222, Ilu0 :230-300,
+//       BiCgStab :302-405)
+//   src/GridSim.Core/Numerics/SparseLu.cs  (ToCsc :63-91, ToCscKeepingZeros :97-117)
+//   src/GridSim.Core/Numerics/SparseLuFactorization.cs  (Factor :69-195, Refactor :246-306, Solve :338-371,
+//       SpSolve :753-772, Reach :775-786, Dfs :789-821)
+//   src/GridSim.Core/Numerics/MinimumDegreeOrdering.cs:23-136
+//   src/GridSim.Core/Numerics/NumericTolerances.cs  (RelativePivotFloor :36, row floors :99-115, MaxAbs :165,
+//       Ilu0PivotGuard :226-234, UnderflowGuard :242, Krylov cap :256-260)
+//   src/GridSim.Core/Analysis/ContingencyAnalysis.cs  (ScreenBranchesParallel :323-347, WithoutMany :1839-1845)
+#include "oracle.h"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <thread>
+#include <vector>
+
+#include "fdlibm.h"
+
+namespace {
+
+using std::vector;
+constexpr double kPi = 3.141592653589793;  // Math.PI
+constexpr double kRelativePivotFloor = 1e-12;
+constexpr double kMaxUndampedStep = 0.4;
+constexpr double kVmFloorPu = 0.01;
+constexpr double kUnderflowGuard = 1e-300;
+
+// .NET Math.Max(double, double): NaN if either argument is NaN.
+inline double net_max(double a, double b) {
+  if (std::isnan(a) || std::isnan(b)) return std::numeric_limits<double>::quiet_NaN();
+  return a > b ? a : b;
+}
+
+// ---------------------------------------------------------------- System.Numerics.Complex arithmetic
+struct Cx { double re, im; };
+inline Cx cx_add(Cx a, Cx b) { return {a.re + b.re, a.im + b.im}; }
+inline Cx cx_neg(Cx a) { return {-a.re, -a.im}; }
+inline Cx cx_conj(Cx a) { return {a.re, -a.im}; }
+// Complex / Complex: Smith's formula (the .NET BCL operator).
+inline Cx cx_div(Cx l, Cx r) {
+  double a = l.re, b = l.im, c = r.re, d = r.im;
+  if (std::fabs(d) < std::fabs(c)) {
+    double doc = d / c;
+    return {(a + b * doc) / (c + d * doc), (b - a * doc) / (c + d * doc)};
+  }
+  double cod = c / d;
+  return {(b + a * cod) / (d + c * cod), (-a + b * cod) / (d + c * cod)};
+}
+// Complex / double (finite operands: component-wise division).
+inline Cx cx_div_real(Cx l, double r) {
+  if (r == 0) return {std::nan(""), std::nan("")};
+  return {l.re / r, l.im / r};
+}
+
+struct Stamp { Cx yff, ytt, yft, ytf; };
+
+// AdmittanceStamp.Branch + FromAdmittance (AdmittanceStamp.cs:41-75) at h = 1, rFactor = 1.
+Stamp branch_stamp(double r, double x, double b, double tap_ratio, double shift_deg) {
+  Cx ys = cx_div(Cx{1.0, 0.0}, Cx{r * 1.0, x * 1.0});
+  Cx bc = {0.0, (b * 1.0) / 2.0};
+  double a = tap_ratio == 0 ? 1.0 : tap_ratio;
+  double shift = shift_deg * kPi / 180.0;
+  double s, c;
+  orc_fdlibm::sincos(shift, &s, &c);
+  Cx tap = {a * c, a * s};  // DetMath.FromPolar
+  Stamp st;
+  st.yff = cx_div_real(cx_add(ys, bc), a * a);
+  st.ytt = cx_add(ys, bc);
+  st.yft = cx_div(cx_neg(ys), cx_conj(tap));
+  st.ytf = cx_div(cx_neg(ys), tap);
+  return st;
+}
+
+// ---------------------------------------------------------------- ComplexCsr.Builder (Sparse.cs:145-222)
+struct Ycsr {
+  int n = 0;
+  vector<int> rp, ci, diag;
+  vector<double> g, b;
+};
+
+struct YBuilder {
+  int n;
+  vector<int> r, c;
+  vector<double> g, b;
+  explicit YBuilder(int n_) : n(n_) {}
+  void add(int row, int col, double gv, double bv) {
+    if (gv == 0.0 && bv == 0.0) return;
+    r.push_back(row); c.push_back(col); g.push_back(gv); b.push_back(bv);
+  }
+  Ycsr build() const {
+    int nnz = (int)g.size();
+    vector<int> row_start(n + 1, 0);
+    for (int t = 0; t < nnz; t++) row_start[r[t] + 1]++;
+    for (int i = 0; i < n; i++) row_start[i + 1] += row_start[i];
+    vector<int> col(nnz), fill(row_start.begin(), row_start.end() - 1);
+    vector<double> gv(nnz), bv(nnz);
+    for (int t = 0; t < nnz; t++) { int p = fill[r[t]]++; col[p] = c[t]; gv[p] = g[t]; bv[p] = b[t]; }
+    Ycsr y;
+    y.n = n;
+    y.rp.assign(n + 1, 0);
+    y.diag.assign(n, -1);
+    vector<int> order;
+    for (int i = 0; i < n; i++) {
+      int s = row_start[i], e = row_start[i + 1], len = e - s;
+      order.resize(len);
+      for (int k = 0; k < len; k++) order[k] = s + k;
+      // stable by (column, insertion index)
+      std::stable_sort(order.begin(), order.end(), [&](int x, int z) { return col[x] < col[z]; });
+      y.rp[i] = (int)y.ci.size();
+      int prev = std::numeric_limits<int>::min();
+      bool diag_written = false;
+      for (int k = 0; k < len; k++) {
+        int cc = col[order[k]];
+        double gg = gv[order[k]], bb = bv[order[k]];
+        if (cc == prev) { y.g.back() += gg; y.b.back() += bb; continue; }
+        if (!diag_written && prev < i && cc > i) {
+          y.diag[i] = (int)y.ci.size(); y.ci.push_back(i); y.g.push_back(0); y.b.push_back(0); diag_written = true;
+        }
+        y.ci.push_back(cc); y.g.push_back(0.0 + gg); y.b.push_back(0.0 + bb); prev = cc;
+        if (cc == i) { y.diag[i] = (int)y.ci.size() - 1; diag_written = true; }
+      }
+      if (!diag_written) { y.diag[i] = (int)y.ci.size(); y.ci.push_back(i); y.g.push_back(0); y.b.push_back(0); }
+    }
+    y.rp[n] = (int)y.ci.size();
+    return y;
+  }
+};
+
+// BuildY (SparseNewtonRaphsonPowerFlow.cs:382-402); `skip` removes one branch (WithoutMany keeps model order).
+Ycsr build_y(const orc_network& net, int skip) {
+  YBuilder bld(net.n_bus);
+  for (int k = 0; k < net.n_branch; k++) {
+    if (k == skip || !net.br_in_service[k]) continue;
+    int f = net.br_from[k], t = net.br_to[k];
+    Stamp s = branch_stamp(net.br_r[k], net.br_x[k], net.br_b[k], net.br_tap[k], net.br_shift_deg[k]);
+    bld.add(f, f, s.yff.re, s.yff.im);
+    bld.add(t, t, s.ytt.re, s.ytt.im);
+    bld.add(f, t, s.yft.re, s.yft.im);
+    bld.add(t, f, s.ytf.re, s.ytf.im);
+  }
+  for (int i = 0; i < net.n_bus; i++) {
+    if (net.gs[i] == 0 && net.bs[i] == 0) continue;
+    bld.add(i, i, net.gs[i] / net.base_mva, net.bs[i] / net.base_mva);
+  }
+  return bld.build();
+}
+
+// ---------------------------------------------------------------- CsrMatrix (Sparse.cs:10-124)
+struct Csr {
+  int n = 0;
+  vector<int> rp, ci, diag;
+  vector<double> val;
+};
+
+struct CsrBuilder {
+  int n;
+  vector<int> r, c;
+  vector<double> v;
+  explicit CsrBuilder(int n_) : n(n_) {}
+  void add(int row, int col, double value) {
+    if (value == 0.0) return;
+    r.push_back(row); c.push_back(col); v.push_back(value);
+  }
+  Csr build() const {
+    int nnz_raw = (int)v.size();
+    vector<int> count(n + 1, 0);
+    for (int t = 0; t < nnz_raw; t++) count[r[t] + 1]++;
+    vector<int> row_start(n + 1, 0);
+    for (int i = 0; i < n; i++) row_start[i + 1] = row_start[i] + count[i + 1];
+    vector<int> col_tmp(nnz_raw);
+    vector<double> val_tmp(nnz_raw);
+    vector<int> fill(row_start.begin(), row_start.end() - 1);
+    for (int t = 0; t < nnz_raw; t++) { int p = fill[r[t]]++; col_tmp[p] = c[t]; val_tmp[p] = v[t]; }
+    Csr a;
+    a.n = n;
+    a.rp.assign(n + 1, 0);
+    a.diag.assign(n, -1);
+    vector<int> idx;
+    for (int i = 0; i < n; i++) {
+      int s = row_start[i], e = row_start[i + 1], len = e - s;
+      idx.resize(len);
+      for (int k = 0; k < len; k++) idx[k] = s + k;
+      std::stable_sort(idx.begin(), idx.end(), [&](int x, int z) { return col_tmp[x] < col_tmp[z]; });
+      a.rp[i] = (int)a.ci.size();
+      int prev = std::numeric_limits<int>::min();
+      bool diag_written = false;
+      for (int k = 0; k < len; k++) {
+        int cc = col_tmp[idx[k]];
+        double vv = val_tmp[idx[k]];
+        if (cc == prev) { a.val.back() += vv; continue; }
+        if (!diag_written && prev < i && cc > i) {
+          a.diag[i] = (int)a.ci.size(); a.ci.push_back(i); a.val.push_back(0.0); diag_written = true;
+        }
+        a.ci.push_back(cc); a.val.push_back(vv); prev = cc;
+        if (cc == i) { a.diag[i] = (int)a.ci.size() - 1; diag_written = true; }
+      }
+      if (!diag_written) { a.diag[i] = (int)a.ci.size(); a.ci.push_back(i); a.val.push_back(0.0); }
+    }
+    a.rp[n] = (int)a.ci.size();
+    return a;
+  }
+};
+
+void csr_multiply(const Csr& a, const double* x, double* y) {
+  for (int i = 0; i < a.n; i++) {
+    double s = 0;
+    for (int k = a.rp[i]; k < a.rp[i + 1]; k++) s += a.val[k] * x[a.ci[k]];
+    y[i] = s;
+  }
+}
+
+// ---------------------------------------------------------------- bus spec, numbering, injections, Jacobian
+struct BusSpec {
+  vector<int> type0;
+  vector<double> vm, va, pspec, qspec;
+};
+
+// NewtonShell.BuildBusSpec (NewtonShell.cs:433-486), constant-power buses.
+BusSpec build_bus_spec(const orc_network& net, bool flat_start) {
+  int n = net.n_bus;
+  BusSpec s;
+  s.type0.assign(n, ORC_PQ);
+  s.vm.assign(n, 0); s.va.assign(n, 0); s.pspec.assign(n, 0); s.qspec.assign(n, 0);
+  vector<double> pg(n, 0), qg(n, 0), vg(n, 0);
+  vector<char> has_gen(n, 0);
+  for (int g = 0; g < net.n_gen; g++) {
+    int i = net.gen_bus[g];
+    pg[i] += net.gen_pg[g];
+    qg[i] += net.gen_qg[g];
+    vg[i] = net.gen_vg[g];
+    has_gen[i] = 1;
+  }
+  double base = net.base_mva;
+  for (int i = 0; i < n; i++) {
+    int bt = net.bus_type[i];
+    s.type0[i] = (bt == ORC_PV && !has_gen[i]) ? ORC_PQ : bt;
+    bool vc = s.type0[i] == ORC_SLACK || s.type0[i] == ORC_PV;
+    s.vm[i] = vc && has_gen[i] ? vg[i] : s.type0[i] == ORC_SLACK ? net.vm[i] : flat_start ? 1.0 : net.vm[i];
+    s.va[i] = bt == ORC_SLACK ? net.va_deg[i] * kPi / 180.0 : flat_start ? 0.0 : net.va_deg[i] * kPi / 180.0;
+    s.pspec[i] = (pg[i] - net.pd[i]) / base;
+    s.qspec[i] = (qg[i] - net.qd[i]) / base;
+  }
+  return s;
+}
+
+// NewtonShell.EquationPositions (NewtonShell.cs:327-336).
+int equation_positions(const vector<int>& type, vector<int>& ppos, vector<int>& qpos) {
+  int n = (int)type.size();
+  ppos.assign(n, -1);
+  qpos.assign(n, -1);
+  int m = 0;
+  for (int i = 0; i < n; i++) if (type[i] != ORC_SLACK) ppos[i] = m++;
+  for (int i = 0; i < n; i++) if (type[i] == ORC_PQ) qpos[i] = m++;
+  return m;
+}
+
+// ComputeInjections (SparseNewtonRaphsonPowerFlow.cs:269-289).
+void compute_injections(const Ycsr& y, const double* vm, const double* va, double* p, double* q) {
+  for (int i = 0; i < y.n; i++) {
+    double pi = 0, qi = 0, vai = va[i];
+    for (int k = y.rp[i]; k < y.rp[i + 1]; k++) {
+      int j = y.ci[k];
+      double th = vai - va[j];
+      double sn, cs;
+      orc_fdlibm::sincos(th, &sn, &cs);
+      double vmj = vm[j];
+      pi += vmj * (y.g[k] * cs + y.b[k] * sn);
+      qi += vmj * (y.g[k] * sn - y.b[k] * cs);
+    }
+    p[i] = vm[i] * pi;
+    q[i] = vm[i] * qi;
+  }
+}
+
+int locate(const Csr& a, int row, int col) {
+  auto b = a.ci.begin() + a.rp[row], e = a.ci.begin() + a.rp[row + 1];
+  auto it = std::lower_bound(b, e, col);
+  return (it != e && *it == col) ? (int)(it - a.ci.begin()) : -1;
+}
+
+// BuildJacobianStructure (SparseNewtonRaphsonPowerFlow.cs:296-332).
+Csr build_jacobian_structure(const Ycsr& y, const vector<int>& ppos, const vector<int>& qpos, int m,
+                             vector<int>& slots) {
+  int n = y.n;
+  CsrBuilder jb(m);
+  for (int i = 0; i < n; i++) {
+    int pi = ppos[i], qi = qpos[i];
+    if (pi < 0 && qi < 0) continue;
+    for (int k = y.rp[i]; k < y.rp[i + 1]; k++) {
+      int j = y.ci[k];
+      int pj = ppos[j], qj = qpos[j];
+      if (pi >= 0 && pj >= 0) jb.add(pi, pj, 1.0);
+      if (pi >= 0 && qj >= 0) jb.add(pi, qj, 1.0);
+      if (qi >= 0 && pj >= 0) jb.add(qi, pj, 1.0);
+      if (qi >= 0 && qj >= 0) jb.add(qi, qj, 1.0);
+    }
+  }
+  Csr jac = jb.build();
+  slots.assign(4 * (size_t)y.rp[n], -1);
+  for (int i = 0; i < n; i++) {
+    int pi = ppos[i], qi = qpos[i];
+    for (int k = y.rp[i]; k < y.rp[i + 1]; k++) {
+      int j = y.ci[k];
+      int pj = ppos[j], qj = qpos[j];
+      size_t b = 4 * (size_t)k;
+      slots[b + 0] = pi >= 0 && pj >= 0 ? locate(jac, pi, pj) : -1;
+      slots[b + 1] = pi >= 0 && qj >= 0 ? locate(jac, pi, qj) : -1;
+      slots[b + 2] = qi >= 0 && pj >= 0 ? locate(jac, qi, pj) : -1;
+      slots[b + 3] = qi >= 0 && qj >= 0 ? locate(jac, qi, qj) : -1;
+    }
+  }
+  return jac;
+}
+
+// RefillJacobian (SparseNewtonRaphsonPowerFlow.cs:342-380).
+void refill_jacobian(Csr& jac, const vector<int>& slots, const Ycsr& y, const double* vm, const double* va,
+                     const double* p, const double* q) {
+  std::fill(jac.val.begin(), jac.val.end(), 0.0);
+  for (int i = 0; i < y.n; i++) {
+    for (int k = y.rp[i]; k < y.rp[i + 1]; k++) {
+      int j = y.ci[k];
+      double gij = y.g[k], bij = y.b[k];
+      double h, nn, mm, l;
+      if (i == j) {
+        h = -q[i] - bij * vm[i] * vm[i];
+        nn = p[i] / vm[i] + gij * vm[i];
+        mm = p[i] - gij * vm[i] * vm[i];
+        l = q[i] / vm[i] - bij * vm[i];
+      } else {
+        double th = va[i] - va[j];
+        double sn, cs;
+        orc_fdlibm::sincos(th, &sn, &cs);
+        h = vm[i] * vm[j] * (gij * sn - bij * cs);
+        nn = vm[i] * (gij * cs + bij * sn);
+        mm = -vm[i] * vm[j] * (gij * cs + bij * sn);
+        l = vm[i] * (gij * sn - bij * cs);
+      }
+      size_t b = 4 * (size_t)k;
+      int s;
+      if ((s = slots[b + 0]) >= 0) jac.val[s] = h;
+      if ((s = slots[b + 1]) >= 0) jac.val[s] = nn;
+      if ((s = slots[b + 2]) >= 0) jac.val[s] = mm;
+      if ((s = slots[b + 3]) >= 0) jac.val[s] = l;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- minimum degree (MinimumDegreeOrdering.cs:23-136)
+vector<int> md_order(int n, const int* rowptr, const int* colidx) {
+  vector<int> order(n);
+  if (n == 0) return order;
+  vector<int> cnt(n, 0);
+  for (int i = 0; i < n; i++)
+    for (int k = rowptr[i]; k < rowptr[i + 1]; k++) {
+      int j = colidx[k];
+      if (j != i) { cnt[i]++; cnt[j]++; }
+    }
+  vector<vector<int>> adj(n);
+  for (int i = 0; i < n; i++) adj[i].assign(cnt[i], 0);
+  vector<int> len(n, 0);
+  for (int i = 0; i < n; i++)
+    for (int k = rowptr[i]; k < rowptr[i + 1]; k++) {
+      int j = colidx[k];
+      if (j == i) continue;
+      adj[i][len[i]++] = j;
+      adj[j][len[j]++] = i;
+    }
+  vector<int> stamp(n, 0);
+  int st = 0;
+  for (int i = 0; i < n; i++) {
+    st++;
+    int w = 0;
+    for (int p = 0; p < len[i]; p++) {
+      int j = adj[i][p];
+      if (stamp[j] != st) { stamp[j] = st; adj[i][w++] = j; }
+    }
+    len[i] = w;
+  }
+  vector<int> head(n + 1, -1), next(n), prev(n), deg(n);
+  auto insert = [&](int v, int d) {
+    deg[v] = d; prev[v] = -1; next[v] = head[d];
+    if (head[d] >= 0) prev[head[d]] = v;
+    head[d] = v;
+  };
+  auto remove = [&](int v) {
+    int d = deg[v];
+    if (prev[v] >= 0) next[prev[v]] = next[v]; else head[d] = next[v];
+    if (next[v] >= 0) prev[next[v]] = prev[v];
+  };
+  for (int i = n - 1; i >= 0; i--) insert(i, len[i]);
+  vector<char> dead(n, 0);
+  vector<int> nb(n);
+  int min_deg = 0;
+  for (int s = 0; s < n; s++) {
+    while (head[min_deg] < 0) min_deg++;
+    int v = head[min_deg];
+    remove(v);
+    dead[v] = 1;
+    order[s] = v;
+    int nb_count = 0;
+    for (int p = 0; p < len[v]; p++) {
+      int u = adj[v][p];
+      if (!dead[u]) nb[nb_count++] = u;
+    }
+    st++;
+    for (int t = 0; t < nb_count; t++) stamp[nb[t]] = st;
+    int nb_stamp = st;
+    for (int t = 0; t < nb_count; t++) {
+      int u = nb[t];
+      remove(u);
+      vector<int>& au = adj[u];
+      int w = 0;
+      for (int p = 0; p < len[u]; p++) {
+        int x = au[p];
+        if (x != v && !dead[x] && stamp[x] != nb_stamp) au[w++] = x;
+      }
+      int need = w + nb_count - 1;
+      if ((int)au.size() < need) au.resize(std::max(need, (int)au.size() * 2));
+      for (int qq = 0; qq < nb_count; qq++)
+        if (nb[qq] != u) au[w++] = nb[qq];
+      len[u] = w;
+      insert(u, w);
+    }
+    adj[v].clear();
+    len[v] = 0;
+    if (min_deg > 0) min_deg--;
+  }
+  return order;
+}
+
+// ---------------------------------------------------------------- CSR -> CSC (SparseLu.cs:63-117)
+struct Csc {
+  int n = 0;
+  vector<int> ap, ai;
+  vector<double> ax;
+};
+
+Csc to_csc(int n, const int* rowptr, const int* colidx, const double* val, bool keep_zeros) {
+  Csc c;
+  c.n = n;
+  c.ap.assign(n + 1, 0);
+  int stored = rowptr[n];
+  for (int k = 0; k < stored; k++)
+    if (keep_zeros || val[k] != 0.0) c.ap[colidx[k] + 1]++;
+  for (int j = 0; j < n; j++) c.ap[j + 1] += c.ap[j];
+  int nnz = c.ap[n];
+  c.ai.assign(nnz, 0);
+  c.ax.assign(nnz, 0.0);
+  vector<int> fill(c.ap.begin(), c.ap.end() - 1);
+  for (int i = 0; i < n; i++)
+    for (int k = rowptr[i]; k < rowptr[i + 1]; k++) {
+      double v = val[k];
+      if (!keep_zeros && v == 0.0) continue;
+      int dst = fill[colidx[k]]++;
+      c.ai[dst] = i;
+      c.ax[dst] = v;
+    }
+  return c;
+}
+
+// ---------------------------------------------------------------- Gilbert-Peierls LU (SparseLuFactorization.cs)
+}  // namespace
+
+struct orc_lu {
+  int n = 0;
+  vector<int> lp, li, up, ui, pinv, q;
+  vector<double> lx, ux;
+  // plan (keepPlan): reach per column in DFS topological order, pivot row per step, L rows as original rows
+  vector<int> reach_ptr, reach, pivot_row, li_original;
+  vector<int> plan_ap, plan_ai;
+  double diagonal_preference = 0.0;
+};
+
+namespace {
+
+// NumericTolerances.MaxAbs (:165) and RowPivotFloorsCsc (:99-115) with ToRowPivotFloors.
+bool row_floors(const Csc& a, vector<double>& floors, double& anorm) {
+  anorm = 0.0;
+  int nnz = a.ap[a.n];
+  for (int p = 0; p < nnz; p++) {
+    double v = a.ax[p];
+    if (std::isnan(v)) { anorm = NAN; return false; }
+    double av = std::fabs(v);
+    if (av > anorm) anorm = av;
+  }
+  if (std::isinf(anorm)) return false;
+  floors.assign(a.n, 0.0);
+  for (int p = 0; p < nnz; p++) {
+    double av = std::fabs(a.ax[p]);
+    if (av > floors[a.ai[p]]) floors[a.ai[p]] = av;
+  }
+  for (int r = 0; r < a.n; r++) floors[r] = floors[r] * kRelativePivotFloor;
+  return true;
+}
+
+int dfs(int j, const vector<int>& lp, const vector<int>& li, const vector<int>& pinv, int top, vector<int>& xi,
+        vector<int>& pstack, vector<char>& marked) {
+  int head = 0;
+  xi[0] = j;
+  while (head >= 0) {
+    j = xi[head];
+    int jcol = pinv[j];
+    if (!marked[j]) {
+      marked[j] = 1;
+      pstack[head] = jcol < 0 ? 0 : lp[jcol] + 1;
+    }
+    bool done = true;
+    int pend = jcol < 0 ? 0 : lp[jcol + 1];
+    for (int p = pstack[head]; p < pend; p++) {
+      int i = li[p];
+      if (marked[i]) continue;
+      pstack[head] = p + 1;
+      xi[++head] = i;
+      done = false;
+      break;
+    }
+    if (done) {
+      head--;
+      xi[--top] = j;
+    }
+  }
+  return top;
+}
+
+int reach(int col, const Csc& a, const vector<int>& lp, const vector<int>& li, const vector<int>& pinv, int n,
+          vector<int>& xi, vector<int>& pstack, vector<char>& marked) {
+  int top = n;
+  for (int p = a.ap[col]; p < a.ap[col + 1]; p++) {
+    int j = a.ai[p];
+    if (!marked[j]) top = dfs(j, lp, li, pinv, top, xi, pstack, marked);
+  }
+  for (int p = top; p < n; p++) marked[xi[p]] = 0;
+  return top;
+}
+
+// Factor (SparseLuFactorization.cs:69-195) with keepPlan; returns null when singular. Non-finite entries are refused
+// by the reference with an ArgumentException; here they return null with *nonfinite set.
+orc_lu* gp_factor(const Csc& a, const int* q, double diag_pref, bool* nonfinite) {
+  int n = a.n;
+  vector<double> floors;
+  double anorm;
+  *nonfinite = false;
+  if (!row_floors(a, floors, anorm)) { *nonfinite = true; return nullptr; }
+  auto* f = new orc_lu();
+  f->n = n;
+  f->diagonal_preference = diag_pref;
+  f->lp.assign(n + 1, 0);
+  f->up.assign(n + 1, 0);
+  f->pinv.assign(n, -1);
+  f->q.assign(q, q + n);
+  f->reach_ptr.assign(n + 1, 0);
+  f->pivot_row.assign(n, -1);
+  vector<double> x(n, 0.0);
+  vector<int> xi(n), pstack(n);
+  vector<char> marked(n, 0);
+  vector<int>& li = f->li;
+  vector<double>& lx = f->lx;
+  for (int k = 0; k < n; k++) {
+    f->lp[k] = (int)li.size();
+    f->up[k] = (int)f->ui.size();
+    int col = q[k];
+    int top = reach(col, a, f->lp, li, f->pinv, n, xi, pstack, marked);
+    // SpSolve: clear, scatter, update in topological order
+    for (int p = top; p < n; p++) x[xi[p]] = 0.0;
+    for (int p = a.ap[col]; p < a.ap[col + 1]; p++) x[a.ai[p]] = a.ax[p];
+    for (int px = top; px < n; px++) {
+      int j = xi[px];
+      int jcol = f->pinv[j];
+      if (jcol < 0) continue;
+      double xj = x[j];
+      for (int p = f->lp[jcol] + 1; p < f->lp[jcol + 1]; p++) x[li[p]] -= lx[p] * xj;
+    }
+    for (int p = top; p < n; p++) f->reach.push_back(xi[p]);
+    f->reach_ptr[k + 1] = (int)f->reach.size();
+    int ipiv = -1;
+    double best = -1.0;
+    for (int px = top; px < n; px++) {
+      int i = xi[px];
+      if (f->pinv[i] < 0) {
+        double t = std::fabs(x[i]);
+        if (t > best) { best = t; ipiv = i; }
+      } else {
+        f->ui.push_back(f->pinv[i]);
+        f->ux.push_back(x[i]);
+      }
+    }
+    if (ipiv < 0 || best <= floors[ipiv]) { delete f; return nullptr; }
+    if (diag_pref > 0.0) {
+      // PreferDiagonal (SparseLuFactorization.cs:201-213)
+      if (!(ipiv == col || f->pinv[col] >= 0)) {
+        for (int px = top; px < n; px++) {
+          if (xi[px] != col) continue;
+          double td = std::fabs(x[col]);
+          if (td >= diag_pref * best && td > floors[col]) ipiv = col;
+          break;
+        }
+      }
+    }
+    double pivot = x[ipiv];
+    f->pivot_row[k] = ipiv;
+    f->ui.push_back(k);
+    f->ux.push_back(pivot);
+    f->pinv[ipiv] = k;
+    li.push_back(ipiv);
+    lx.push_back(1.0);
+    for (int px = top; px < n; px++) {
+      int i = xi[px];
+      if (f->pinv[i] < 0) {
+        li.push_back(i);
+        lx.push_back(x[i] / pivot);
+      }
+      x[i] = 0.0;
+    }
+  }
+  f->lp[n] = (int)li.size();
+  f->up[n] = (int)f->ui.size();
+  f->li_original = li;
+  for (size_t p = 0; p < li.size(); p++) li[p] = f->pinv[li[p]];
+  f->plan_ap = a.ap;
+  f->plan_ai = a.ai;
+  return f;
+}
+
+// Refactor (SparseLuFactorization.cs:246-306): replays the plan; null when the pivot would move or is singular.
+orc_lu* gp_refactor(const orc_lu& plan, const Csc& a) {
+  int n = plan.n;
+  vector<double> floors;
+  double anorm;
+  if (!row_floors(a, floors, anorm)) return nullptr;
+  auto* f = new orc_lu(plan);
+  f->lx.assign(plan.lp[n], 0.0);
+  f->ux.assign(plan.up[n], 0.0);
+  vector<double> x(n, 0.0);
+  const vector<int>& step = plan.pinv;
+  int lnz = 0, unz = 0;
+  for (int k = 0; k < n; k++) {
+    int r0 = plan.reach_ptr[k], r1 = plan.reach_ptr[k + 1];
+    int col = plan.q[k];
+    for (int p = r0; p < r1; p++) x[plan.reach[p]] = 0.0;
+    for (int p = a.ap[col]; p < a.ap[col + 1]; p++) x[a.ai[p]] = a.ax[p];
+    for (int px = r0; px < r1; px++) {
+      int j = plan.reach[px];
+      int jcol = step[j];
+      if (jcol >= k) continue;
+      double xj = x[j];
+      for (int p = plan.lp[jcol] + 1; p < plan.lp[jcol + 1]; p++) x[plan.li_original[p]] -= f->lx[p] * xj;
+    }
+    int ipiv = -1;
+    double best = -1.0;
+    for (int px = r0; px < r1; px++) {
+      int i = plan.reach[px];
+      if (step[i] >= k) {
+        double t = std::fabs(x[i]);
+        if (t > best) { best = t; ipiv = i; }
+      } else {
+        f->ux[unz++] = x[i];
+      }
+    }
+    bool clears = ipiv >= 0 && best > floors[ipiv];
+    if (clears && plan.diagonal_preference > 0.0 && !(ipiv == col || step[col] < k)) {
+      for (int px = r0; px < r1; px++) {
+        if (plan.reach[px] != col) continue;
+        double td = std::fabs(x[col]);
+        if (td >= plan.diagonal_preference * best && td > floors[col]) ipiv = col;
+        break;
+      }
+    }
+    if (ipiv != plan.pivot_row[k] || !clears) { delete f; return nullptr; }
+    double pivot = x[ipiv];
+    f->ux[unz++] = pivot;
+    f->lx[lnz++] = 1.0;
+    for (int px = r0; px < r1; px++) {
+      int i = plan.reach[px];
+      if (step[i] > k) f->lx[lnz++] = x[i] / pivot;
+      x[i] = 0.0;
+    }
+  }
+  return f;
+}
+
+// Solve (SparseLuFactorization.cs:338-371).
+void lu_solve(const orc_lu& f, const double* b, double* out) {
+  int n = f.n;
+  vector<double> x(n);
+  for (int k = 0; k < n; k++) x[f.pinv[k]] = b[k];
+  for (int j = 0; j < n; j++) {
+    double xj = x[j];
+    for (int p = f.lp[j] + 1; p < f.lp[j + 1]; p++) x[f.li[p]] -= f.lx[p] * xj;
+  }
+  for (int j = n - 1; j >= 0; j--) {
+    double xj = x[j] / f.ux[f.up[j + 1] - 1];
+    x[j] = xj;
+    for (int p = f.up[j]; p < f.up[j + 1] - 1; p++) x[f.ui[p]] -= f.ux[p] * xj;
+  }
+  for (int k = 0; k < n; k++) out[f.q[k]] = x[k];
+}
+
+// ---------------------------------------------------------------- ILU(0) + BiCGSTAB (Sparse.cs:230-405)
+struct Ilu0 {
+  const Csr* a;
+  vector<double> lu;
+};
+
+bool ilu0_factor(const Csr& a, Ilu0& out) {
+  int n = a.n;
+  for (int i = 0; i < n; i++) if (a.diag[i] < 0) return false;
+  out.a = &a;
+  out.lu = a.val;
+  vector<int> iw(n, -1);
+  for (int i = 0; i < n; i++) {
+    for (int k = a.rp[i]; k < a.rp[i + 1]; k++) iw[a.ci[k]] = k;
+    for (int k = a.rp[i]; k < a.diag[i]; k++) {
+      int j = a.ci[k];
+      double piv = out.lu[a.diag[j]];
+      double lij = (piv != 0.0) ? out.lu[k] / piv : 0.0;
+      out.lu[k] = lij;
+      if (lij == 0.0) continue;
+      for (int kk = a.diag[j] + 1; kk < a.rp[j + 1]; kk++) {
+        int pos = iw[a.ci[kk]];
+        if (pos != -1) out.lu[pos] -= lij * out.lu[kk];
+      }
+    }
+    double row_scale = 0.0;
+    for (int k = a.rp[i]; k < a.rp[i + 1]; k++) { double av = std::fabs(a.val[k]); if (av > row_scale) row_scale = av; }
+    double guard = kRelativePivotFloor * (row_scale > 0.0 ? row_scale : 1.0);
+    double d = out.lu[a.diag[i]];
+    if (std::fabs(d) < guard) out.lu[a.diag[i]] = d < 0 ? -guard : guard;
+    for (int k = a.rp[i]; k < a.rp[i + 1]; k++) iw[a.ci[k]] = -1;
+  }
+  return true;
+}
+
+void ilu0_apply(const Ilu0& m, const double* r, double* z) {
+  const Csr& a = *m.a;
+  for (int i = 0; i < a.n; i++) {
+    double s = r[i];
+    for (int k = a.rp[i]; k < a.diag[i]; k++) s -= m.lu[k] * z[a.ci[k]];
+    z[i] = s;
+  }
+  for (int i = a.n - 1; i >= 0; i--) {
+    double s = z[i];
+    for (int k = a.diag[i] + 1; k < a.rp[i + 1]; k++) s -= m.lu[k] * z[a.ci[k]];
+    z[i] = s / m.lu[a.diag[i]];
+  }
+}
+
+double dot(const vector<double>& a, const vector<double>& b) {
+  double s = 0;
+  for (size_t i = 0; i < a.size(); i++) s += a[i] * b[i];
+  return s;
+}
+
+bool bicgstab(const Csr& a, const vector<double>& b, vector<double>& x, const Ilu0& pc, double tol, int max_iter) {
+  int n = a.n;
+  std::fill(x.begin(), x.end(), 0.0);
+  vector<double> r(b), rhat(b), p(n, 0), v(n, 0), s(n, 0), t(n, 0), ph(n, 0), sh(n, 0);
+  double bnorm = std::sqrt(dot(b, b));
+  if (bnorm == 0) {
+    for (double bi : b) if (bi != 0.0) return false;
+    return true;
+  }
+  auto true_residual = [&]() {
+    csr_multiply(a, x.data(), t.data());
+    double sum = 0.0;
+    for (int i = 0; i < n; i++) { double d = b[i] - t[i]; sum += d * d; }
+    return std::sqrt(sum) / bnorm < tol;
+  };
+  double rho = 1, alpha = 1, omega = 1;
+  for (int it = 0; it < max_iter; it++) {
+    double rho_new = dot(rhat, r);
+    if (std::fabs(rho_new) < kUnderflowGuard || std::isnan(rho_new)) return false;
+    double beta = (rho_new / rho) * (alpha / omega);
+    for (int i = 0; i < n; i++) p[i] = r[i] + beta * (p[i] - omega * v[i]);
+    ilu0_apply(pc, p.data(), ph.data());
+    csr_multiply(a, ph.data(), v.data());
+    double rhat_v = dot(rhat, v);
+    if (rhat_v == 0.0 || std::isnan(rhat_v)) return false;
+    alpha = rho_new / rhat_v;
+    for (int i = 0; i < n; i++) s[i] = r[i] - alpha * v[i];
+    if (std::sqrt(dot(s, s)) / bnorm < tol) {
+      for (int i = 0; i < n; i++) x[i] += alpha * ph[i];
+      return true_residual();
+    }
+    ilu0_apply(pc, s.data(), sh.data());
+    csr_multiply(a, sh.data(), t.data());
+    double tt = dot(t, t);
+    omega = tt > 0 ? dot(t, s) / tt : 0.0;
+    for (int i = 0; i < n; i++) { x[i] += alpha * ph[i] + omega * sh[i]; r[i] = s[i] - omega * t[i]; }
+    rho = rho_new;
+    if (std::sqrt(dot(r, r)) / bnorm < tol) return true_residual();
+    if (omega == 0) return false;
+  }
+  return false;
+}
+
+// ---------------------------------------------------------------- CpuSparseLuSolver (ordering cached per pattern)
+struct CpuSparseLuSolver {
+  vector<int> q, rp, ci;
+  bool have = false;
+  // 0 ok, 1 singular (null), 2 non-finite matrix or right-hand side (ArgumentException in the reference)
+  int solve(const Csr& a, const double* b, double* x) {
+    if (a.n == 0) return 0;
+    if (!have || rp != a.rp || ci != a.ci) {
+      q = md_order(a.n, a.rp.data(), a.ci.data());
+      rp = a.rp;
+      ci = a.ci;
+      have = true;
+    }
+    Csc c = to_csc(a.n, a.rp.data(), a.ci.data(), a.val.data(), false);
+    bool nonfinite = false;
+    orc_lu* f = gp_factor(c, q.data(), 0.0, &nonfinite);
+    if (nonfinite) return 2;
+    if (!f) return 1;
+    for (int k = 0; k < a.n; k++)
+      if (!std::isfinite(b[k])) { delete f; return 2; }
+    lu_solve(*f, b, x);
+    delete f;
+    return 0;
+  }
+};
+
+// ---------------------------------------------------------------- the Newton loop (RunNewton, Q limits off)
+struct NewtonOut {
+  bool converged = false, diverged = false;
+  int iters = 0;
+  double max_mis = INFINITY;
+  int step_failures = 0;
+};
+
+NewtonOut run_newton(const Ycsr& y, const vector<int>& type, vector<double>& vm, vector<double>& va,
+                     const vector<double>& pspec, const vector<double>& qspec, vector<double>& p, vector<double>& q,
+                     const orc_options& opt) {
+  int n = y.n;
+  vector<int> ppos, qpos;
+  int m = equation_positions(type, ppos, qpos);
+  vector<double> f(m), dx(m);
+  vector<int> slots;
+  Csr jac = build_jacobian_structure(y, ppos, qpos, m, slots);
+  CpuSparseLuSolver solver;
+  NewtonOut out;
+  double start_mis = NAN;
+  int iter = 0;
+  for (;; iter++) {
+    compute_injections(y, vm.data(), va.data(), p.data(), q.data());
+    for (int i = 0; i < n; i++) {
+      if (ppos[i] >= 0) f[ppos[i]] = pspec[i] - p[i];
+      if (qpos[i] >= 0) f[qpos[i]] = qspec[i] - q[i];
+    }
+    out.max_mis = 0;
+    for (int k = 0; k < m; k++) out.max_mis = net_max(out.max_mis, std::fabs(f[k]));
+    if (iter == 0) start_mis = out.max_mis;
+    if (out.max_mis < opt.tolerance) { out.converged = true; break; }
+    if (out.max_mis > net_max(opt.diverge_limit, start_mis)) { out.diverged = true; break; }
+    if (iter >= opt.max_iterations) break;
+    refill_jacobian(jac, slots, y, vm.data(), va.data(), p.data(), q.data());
+    // NewtonShell.SolveStep
+    bool ok = true;
+    for (double jv : jac.val) if (!std::isfinite(jv)) { ok = false; break; }
+    if (ok) for (double fv : f) if (!std::isfinite(fv)) { ok = false; break; }
+    if (!ok) break;
+    int rc = solver.solve(jac, f.data(), dx.data());
+    if (rc == 2) break;
+    if (rc == 1) {
+      out.step_failures++;
+      Ilu0 pc;
+      vector<double> xs(m, 0.0), fv(f);
+      if (!ilu0_factor(jac, pc)) break;
+      double tol = std::min(1e-10, opt.tolerance * 1e-3);
+      if (!bicgstab(jac, fv, xs, pc, tol, 4 * m + 200)) break;
+      dx = xs;
+    }
+    double max_abs = 0;
+    bool finite = true;
+    for (int k = 0; k < m; k++) {
+      if (!std::isfinite(dx[k])) { finite = false; break; }
+      double av = std::fabs(dx[k]);
+      if (av > max_abs) max_abs = av;
+    }
+    if (!finite) break;
+    // NewtonShell.ApplyStep
+    double damp = max_abs > kMaxUndampedStep ? kMaxUndampedStep / max_abs : 1.0;
+    for (int i = 0; i < n; i++) {
+      if (ppos[i] >= 0) va[i] += damp * dx[ppos[i]];
+      if (qpos[i] >= 0) { vm[i] += damp * dx[qpos[i]]; if (vm[i] < kVmFloorPu) vm[i] = kVmFloorPu; }
+    }
+  }
+  out.iters = iter;
+  return out;
+}
+
+void solve_one(const orc_network& net, int skip, const orc_options& opt, double* vm_out, double* va_out,
+               orc_result* res) {
+  auto t0 = std::chrono::steady_clock::now();
+  Ycsr y = build_y(net, skip);
+  BusSpec spec = build_bus_spec(net, opt.flat_start != 0);
+  int n = net.n_bus;
+  vector<double> p(n), q(n);
+  NewtonOut o = run_newton(y, spec.type0, spec.vm, spec.va, spec.pspec, spec.qspec, p, q, opt);
+  compute_injections(y, spec.vm.data(), spec.va.data(), p.data(), q.data());
+  bool finite = std::isfinite(o.max_mis);
+  for (int i = 0; i < n && finite; i++)
+    finite = std::isfinite(spec.vm[i]) && std::isfinite(spec.va[i]) && std::isfinite(p[i] * net.base_mva) &&
+             std::isfinite(q[i] * net.base_mva);
+  res->status = !finite ? ORC_NONFINITE : o.converged ? ORC_CONVERGED : o.diverged ? ORC_DIVERGED
+                                                                                   : ORC_MAX_ITERATIONS;
+  res->iterations = o.iters;
+  res->max_mismatch = o.max_mis;
+  res->step_failures = o.step_failures;
+  res->pad = 0;
+  if (vm_out) std::memcpy(vm_out, spec.vm.data(), sizeof(double) * n);
+  if (va_out) std::memcpy(va_out, spec.va.data(), sizeof(double) * n);
+  res->solve_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace
+
+// ================================================================ C ABI
+extern "C" {
+
+void orc_sincos(const double* x, int64_t n, double* s, double* c) {
+  for (int64_t i = 0; i < n; i++) orc_fdlibm::sincos(x[i], &s[i], &c[i]);
+}
+
+void orc_complex_div(double a, double b, double c, double d, double* re, double* im) {
+  Cx z = cx_div(Cx{a, b}, Cx{c, d});
+  *re = z.re;
+  *im = z.im;
+}
+
+void orc_branch_stamp(double r, double x, double b, double tap, double shift_deg, double* out8) {
+  Stamp s = branch_stamp(r, x, b, tap, shift_deg);
+  Cx v[4] = {s.yff, s.ytt, s.yft, s.ytf};
+  for (int i = 0; i < 4; i++) { out8[2 * i] = v[i].re; out8[2 * i + 1] = v[i].im; }
+}
+
+int32_t orc_ybus(const orc_network* net, int32_t skip, int32_t* rowptr, int32_t* colidx, double* g, double* b) {
+  Ycsr y = build_y(*net, skip);
+  if (rowptr) std::memcpy(rowptr, y.rp.data(), sizeof(int32_t) * (y.n + 1));
+  if (colidx) std::memcpy(colidx, y.ci.data(), sizeof(int32_t) * y.ci.size());
+  if (g) std::memcpy(g, y.g.data(), sizeof(double) * y.g.size());
+  if (b) std::memcpy(b, y.b.data(), sizeof(double) * y.b.size());
+  return (int32_t)y.ci.size();
+}
+
+void orc_bus_spec(const orc_network* net, int32_t flat_start, int32_t* type, double* vm, double* va, double* pspec,
+                  double* qspec) {
+  BusSpec s = build_bus_spec(*net, flat_start != 0);
+  for (int i = 0; i < net->n_bus; i++) {
+    type[i] = s.type0[i]; vm[i] = s.vm[i]; va[i] = s.va[i]; pspec[i] = s.pspec[i]; qspec[i] = s.qspec[i];
+  }
+}
+
+int32_t orc_mismatch(const orc_network* net, int32_t skip, const double* vm, const double* va, double* p, double* q,
+                     double* f) {
+  Ycsr y = build_y(*net, skip);
+  BusSpec s = build_bus_spec(*net, false);
+  vector<int> ppos, qpos;
+  int m = equation_positions(s.type0, ppos, qpos);
+  compute_injections(y, vm, va, p, q);
+  for (int i = 0; i < net->n_bus; i++) {
+    if (ppos[i] >= 0) f[ppos[i]] = s.pspec[i] - p[i];
+    if (qpos[i] >= 0) f[qpos[i]] = s.qspec[i] - q[i];
+  }
+  return m;
+}
+
+int32_t orc_jacobian(const orc_network* net, int32_t skip, const double* vm, const double* va, int32_t* m_out,
+                     int32_t* rowptr, int32_t* colidx, double* val) {
+  Ycsr y = build_y(*net, skip);
+  BusSpec s = build_bus_spec(*net, false);
+  vector<int> ppos, qpos, slots;
+  int m = equation_positions(s.type0, ppos, qpos);
+  Csr jac = build_jacobian_structure(y, ppos, qpos, m, slots);
+  if (m_out) *m_out = m;
+  if (val) {
+    vector<double> p(net->n_bus), q(net->n_bus);
+    compute_injections(y, vm, va, p.data(), q.data());
+    refill_jacobian(jac, slots, y, vm, va, p.data(), q.data());
+    std::memcpy(val, jac.val.data(), sizeof(double) * jac.val.size());
+  }
+  if (rowptr) std::memcpy(rowptr, jac.rp.data(), sizeof(int32_t) * (m + 1));
+  if (colidx) std::memcpy(colidx, jac.ci.data(), sizeof(int32_t) * jac.ci.size());
+  return (int32_t)jac.ci.size();
+}
+
+void orc_md_order(int32_t n, const int32_t* rowptr, const int32_t* colidx, int32_t* q) {
+  vector<int> o = md_order(n, rowptr, colidx);
+  std::memcpy(q, o.data(), sizeof(int32_t) * n);
+}
+
+orc_lu* orc_lu_factor(int32_t n, const int32_t* rowptr, const int32_t* colidx, const double* val, const int32_t* q,
+                      int32_t keep_zeros, double diagonal_preference) {
+  Csc c = to_csc(n, rowptr, colidx, val, keep_zeros != 0);
+  bool nonfinite = false;
+  return gp_factor(c, q, diagonal_preference, &nonfinite);
+}
+
+void orc_lu_sizes(const orc_lu* h, int32_t* lnz, int32_t* unz, int32_t* reach_nnz) {
+  *lnz = h->lp[h->n];
+  *unz = h->up[h->n];
+  *reach_nnz = (int32_t)h->reach.size();
+}
+
+void orc_lu_get(const orc_lu* h, int32_t* lp, int32_t* li, double* lx, int32_t* up, int32_t* ui, double* ux,
+                int32_t* pinv, int32_t* q, int32_t* pivot_row, int32_t* reach_ptr, int32_t* reach) {
+  int n = h->n;
+  auto cp = [](auto* dst, const auto& src) { if (dst) std::memcpy(dst, src.data(), sizeof(src[0]) * src.size()); };
+  cp(lp, h->lp); cp(li, h->li); cp(lx, h->lx); cp(up, h->up); cp(ui, h->ui); cp(ux, h->ux);
+  cp(pinv, h->pinv); cp(q, h->q); cp(pivot_row, h->pivot_row); cp(reach_ptr, h->reach_ptr); cp(reach, h->reach);
+  (void)n;
+}
+
+void orc_lu_solve(const orc_lu* h, const double* b, double* x) { lu_solve(*h, b, x); }
+
+orc_lu* orc_lu_refactor(const orc_lu* plan, int32_t n, const int32_t* rowptr, const int32_t* colidx,
+                        const double* val) {
+  Csc c = to_csc(n, rowptr, colidx, val, true);
+  if (c.ap != plan->plan_ap || c.ai != plan->plan_ai) return nullptr;  // SparseLuPlan.SamePattern
+  return gp_refactor(*plan, c);
+}
+
+void orc_lu_free(orc_lu* h) { delete h; }
+
+int32_t orc_sparse_solve(int32_t n, const int32_t* rowptr, const int32_t* colidx, const double* val,
+                         const double* b, double* x) {
+  Csr a;
+  a.n = n;
+  a.rp.assign(rowptr, rowptr + n + 1);
+  a.ci.assign(colidx, colidx + rowptr[n]);
+  a.val.assign(val, val + rowptr[n]);
+  CpuSparseLuSolver s;
+  return s.solve(a, b, x);
+}
+
+void orc_solve(const orc_network* net, int32_t skip, const orc_options* opt, double* vm_out, double* va_out,
+               orc_result* res) {
+  solve_one(*net, skip, *opt, vm_out, va_out, res);
+}
+
+void orc_screen(const orc_network* net, const int32_t* outages, int32_t count, const orc_options* opt,
+                int32_t threads, orc_result* res, double* vm_out, double* va_out) {
+  if (threads < 1) threads = 1;
+  std::atomic<int> next{0};
+  auto worker = [&]() {
+    for (;;) {
+      int k = next.fetch_add(1);
+      if (k >= count) break;
+      size_t off = (size_t)k * net->n_bus;
+      solve_one(*net, outages[k], *opt, vm_out ? vm_out + off : nullptr, va_out ? va_out + off : nullptr, &res[k]);
+    }
+  };
+  vector<std::thread> pool;
+  for (int t = 0; t < threads; t++) pool.emplace_back(worker);
+  for (auto& t : pool) t.join();
+}
+
+}  // extern "C"
