@@ -1,0 +1,876 @@
+// libgridsim_hip.so host engine: device-resident networks and plans, batched and plain Newton-Raphson, the sparse
+// linear-solver seams, and the C ABI of include/gridsim_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "fdlibm.h"
+#include "gridsim_hip.h"
+#include "model.h"
+#include "nr_kernels.h"
+
+namespace gs {
+hipError_t launch_nr_batch(int lu_fp32, int x_in_lds, const InstDesc* descs, int count, const NrParams& prm,
+                           size_t lds_bytes, hipStream_t stream);
+hipError_t launch_seam(const SeamDesc* descs, int count, size_t lds_bytes, hipStream_t stream);
+hipError_t launch_sincos(const double* x, double* s, double* c, int64_t n, hipStream_t stream);
+hipError_t launch_fp64_probe(double* out, hipStream_t stream);
+}  // namespace gs
+
+using namespace gs;
+
+namespace {
+
+constexpr size_t kLdsLimit = 64 * 1024;  // dynamic LDS a single-wave work-group may take for the solve workspace
+
+struct HipError {
+  gs_status code;
+  std::string what;
+};
+
+[[noreturn]] void fail(gs_status code, const std::string& what) { throw HipError{code, what}; }
+
+void check(hipError_t e, const char* what) {
+  if (e == hipSuccess) return;
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+    fail(GS_OUT_OF_MEMORY, std::string(what) + ": " + hipGetErrorString(e));
+  fail(GS_DEVICE_ERROR, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Device allocation owned by a handle.
+struct DevMem {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevMem() = default;
+  DevMem(const DevMem&) = delete;
+  DevMem& operator=(const DevMem&) = delete;
+  ~DevMem() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  void ensure(size_t n) {
+    if (n <= bytes && p) return;
+    release();
+    check(hipMalloc(&p, n ? n : 16), "hipMalloc");
+    bytes = n ? n : 16;
+  }
+  template <typename T> T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+template <typename T>
+void upload(DevMem& dst, const std::vector<T>& v, hipStream_t s) {
+  dst.ensure(sizeof(T) * v.size());
+  if (!v.empty()) check(hipMemcpyAsync(dst.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, s), "upload");
+}
+
+}  // namespace
+
+// ================================================================================================ context
+struct SeamPlan;
+
+struct gs_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::mutex mu;
+  std::string err;
+  size_t budget = 0;
+  std::map<std::pair<std::vector<int>, std::vector<int>>, std::vector<int>> seam_orders;  // pattern -> MD order
+  std::map<std::vector<int>, std::shared_ptr<SeamPlan>> seam_plans;                       // nonzero positions -> plan
+};
+
+namespace {
+
+// A plan and its network pattern uploaded to the device.
+struct DevicePlan {
+  DevMem y_rp, y_ci, y_diag, y_g, y_b, slots, ppos, qpos, j_rp, j_ci, vm0, va0;
+  DevMem col_order, level_end, col_a_ptr, a_src, a_slot, upd_ptr, upd_slot, lp, li, up, ui, pinv, q, prow;
+  DevMem piv_pos;
+  DevMem d_struct;
+  DevPlan h{};
+  DevicePlanHost dph;
+};
+
+void upload_plan(DevicePlan& dp, const YBus* y, const JacobianPattern& jp, const BusSpec* spec, const Plan& plan,
+                 double base_mva, hipStream_t s) {
+  compile_plan(plan, dp.dph);
+  const DevicePlanHost& d = dp.dph;
+  DevPlan& h = dp.h;
+  h = DevPlan{};
+  h.m = jp.m;
+  h.nnz_j = jp.rp[jp.m];
+  h.nnz_l = d.nnz_l;
+  h.nnz_u = d.nnz_u;
+  h.max_col = d.max_col;
+  h.n_levels = d.n_levels;
+  h.base_mva = base_mva;
+  if (y) {
+    h.n = y->n;
+    h.nnz_y = y->rp[y->n];
+    upload(dp.y_rp, y->rp, s); upload(dp.y_ci, y->ci, s); upload(dp.y_diag, y->diag, s);
+    upload(dp.y_g, y->g, s); upload(dp.y_b, y->b, s);
+    upload(dp.slots, jp.slots, s); upload(dp.ppos, jp.ppos, s); upload(dp.qpos, jp.qpos, s);
+    h.y_rp = dp.y_rp.as<int>(); h.y_ci = dp.y_ci.as<int>(); h.y_diag = dp.y_diag.as<int>();
+    h.y_g = dp.y_g.as<double>(); h.y_b = dp.y_b.as<double>();
+    h.slots = dp.slots.as<int>(); h.ppos = dp.ppos.as<int>(); h.qpos = dp.qpos.as<int>();
+  }
+  if (spec) {
+    upload(dp.vm0, spec->vm, s); upload(dp.va0, spec->va, s);
+    h.vm0 = dp.vm0.as<double>(); h.va0 = dp.va0.as<double>();
+  }
+  upload(dp.j_rp, jp.rp, s); upload(dp.j_ci, jp.ci, s);
+  h.j_rp = dp.j_rp.as<int>(); h.j_ci = dp.j_ci.as<int>();
+  upload(dp.col_order, d.col_order, s); upload(dp.level_end, d.level_end, s);
+  upload(dp.col_a_ptr, d.col_a_ptr, s); upload(dp.a_src, d.a_src, s); upload(dp.a_slot, d.a_slot, s);
+  upload(dp.upd_ptr, d.upd_ptr, s); upload(dp.upd_slot, d.upd_slot, s);
+  upload(dp.lp, d.lp, s); upload(dp.li, d.li, s); upload(dp.up, d.up, s); upload(dp.ui, d.ui, s);
+  upload(dp.pinv, d.pinv, s); upload(dp.q, d.q, s); upload(dp.prow, d.prow, s); upload(dp.piv_pos, d.piv_pos, s);
+  h.col_order = dp.col_order.as<int>(); h.level_end = dp.level_end.as<int>();
+  h.col_a_ptr = dp.col_a_ptr.as<int>(); h.a_src = dp.a_src.as<int>(); h.a_slot = dp.a_slot.as<int>();
+  h.upd_ptr = dp.upd_ptr.as<int>(); h.upd_slot = dp.upd_slot.as<int>();
+  h.lp = dp.lp.as<int>(); h.li = dp.li.as<int>(); h.up = dp.up.as<int>(); h.ui = dp.ui.as<int>();
+  h.pinv = dp.pinv.as<int>(); h.q = dp.q.as<int>(); h.prow = dp.prow.as<int>();
+  dp.d_struct.ensure(sizeof(DevPlan));
+  check(hipMemcpyAsync(dp.d_struct.p, &h, sizeof(DevPlan), hipMemcpyHostToDevice, s), "upload plan");
+}
+
+// The symbolic pass: minimum-degree order of the Jacobian pattern and the Gilbert-Peierls pivot sequence of the
+// Jacobian at (vm, va), with every structural cell kept and the fixed-pattern solver's diagonal preference
+// (FixedPatternSparseLuSolver.cs:52,75-99).
+constexpr double kDiagonalPreference = 0.1;
+
+bool symbolic_pass(const YBus& y, const JacobianPattern& jp, const double* vm, const double* va, Plan& plan,
+                   std::string& why) {
+  std::vector<double> p(y.n), q(y.n), jv(jp.rp[jp.m]);
+  host_injections(y, vm, va, p.data(), q.data());
+  host_jacobian(y, jp, vm, va, p.data(), q.data(), jv.data());
+  std::vector<int> order = minimum_degree(jp.m, jp.rp.data(), jp.ci.data());
+  return gp_plan(jp.m, jp.rp, jp.ci, jv.data(), order, kDiagonalPreference, true, plan, why);
+}
+
+gs_nr_options normalise(const gs_nr_options* o) {
+  gs_nr_options r{};
+  r.tolerance = 1e-8;
+  r.max_iterations = 30;
+  r.flat_start = 0;
+  r.diverge_limit = 10.0;
+  r.lu_precision = GS_LU_FP32;
+  r.refine_steps = 1;
+  r.growth_tol = 1e-3;
+  r.schedule = GS_SCHED_AUTO;
+  if (o) r = *o;
+  if (!(r.tolerance > 0)) fail(GS_INVALID_ARGUMENT, "tolerance must be positive");
+  if (r.max_iterations < 0) fail(GS_INVALID_ARGUMENT, "max_iterations must be >= 0");
+  if (r.refine_steps < 0 || r.refine_steps > 4) fail(GS_INVALID_ARGUMENT, "refine_steps must lie in 0..4");
+  if (!(r.growth_tol >= 0.0 && r.growth_tol <= 1.0)) fail(GS_INVALID_ARGUMENT, "growth_tol must lie in [0, 1]");
+  return r;
+}
+
+}  // namespace
+
+struct gs_net {
+  gs_ctx* ctx = nullptr;
+  Network net;
+  YBus y;
+  BusSpec spec;
+  bool flat_start = false;
+  JacobianPattern jp;
+  Plan plan;
+  DevicePlan dev;
+  // per-call device arenas
+  DevMem inst_buf, desc_buf, state_buf, aux_buf;
+  double last_total_ms = 0, last_kernel_ms = 0;
+  int last_launches = 0;
+};
+
+namespace {
+
+struct Layout {
+  size_t vm, va, p, q, f, dx, r, jv, lx, ux, xg, spec, start, stride;
+};
+
+Layout instance_layout(const DevPlan& h, bool fp32, bool x_lds) {
+  auto al = [](size_t v) { return (v + 255) & ~size_t(255); };
+  size_t t = fp32 ? 4 : 8;
+  Layout L{};
+  size_t o = 0;
+  L.vm = o; o += al(8 * (size_t)h.n);
+  L.va = o; o += al(8 * (size_t)h.n);
+  L.p = o; o += al(8 * (size_t)h.n);
+  L.q = o; o += al(8 * (size_t)h.n);
+  L.f = o; o += al(8 * (size_t)h.m);
+  L.dx = o; o += al(8 * (size_t)h.m);
+  L.r = o; o += al(8 * (size_t)h.m);
+  L.jv = o; o += al(8 * (size_t)h.nnz_j);
+  L.lx = o; o += al(t * (size_t)h.nnz_l);
+  L.ux = o; o += al(t * (size_t)h.nnz_u);
+  L.xg = o; o += x_lds ? 0 : al(t * (size_t)h.m);
+  L.spec = o; o += al(16 * (size_t)h.n);
+  L.start = o; o += al(16 * (size_t)h.n);
+  L.stride = o;
+  return L;
+}
+
+// Re-sums the cells an outage touches, skipping the outaged branch: bit-identical to the rebuilt Y-bus.
+int outage_patches(const YBus& y, const Network& net, int branch, Patch* out) {
+  if (branch < 0) return 0;
+  int f = net.br_from[branch], t = net.br_to[branch];
+  int cells[4] = {y.find(f, f), y.find(t, t), y.find(f, t), y.find(t, f)};
+  int np = 0;
+  for (int c = 0; c < 4; c++) {
+    int k = cells[c];
+    if (k < 0) continue;
+    bool dup = false;
+    for (int e = 0; e < np; e++) dup |= out[e].idx == k;
+    if (dup) continue;
+    double g = 0.0, b = 0.0;
+    bool any = false;
+    for (int e = y.contrib_ptr[k]; e < y.contrib_ptr[k + 1]; e++) {
+      if (y.contrib_branch[e] == branch) continue;
+      if (!any) { g = 0.0 + y.contrib_g[e]; b = 0.0 + y.contrib_b[e]; any = true; }
+      else { g += y.contrib_g[e]; b += y.contrib_b[e]; }
+    }
+    out[np].idx = k;
+    out[np].pad = 0;
+    out[np].g = any ? g : 0.0;
+    out[np].b = any ? b : 0.0;
+    np++;
+  }
+  return np;
+}
+
+struct BatchRun {
+  bool fp32 = true, x_lds = true;
+  size_t lds_bytes = 0;
+};
+
+BatchRun plan_run(const DevPlan& h, const gs_nr_options& o, bool force_fp64) {
+  BatchRun r;
+  r.fp32 = !force_fp64 && o.lu_precision == GS_LU_FP32;
+  size_t t = r.fp32 ? 4 : 8;
+  size_t ws = t * (((size_t)h.max_col + 7) & ~size_t(7));
+  size_t x = t * (size_t)h.m;
+  r.x_lds = ws + x <= kLdsLimit;
+  r.lds_bytes = ws + (r.x_lds ? x : 0);
+  return r;
+}
+
+// Runs `count` instances on one plan. Fills results, optional vm/va. Handles paused instances (pivot growth) by a
+// fresh host symbolic pass on the instance's own reduced network and a resumed solve on that plan.
+void run_batch(gs_net* net, int count, const gs_instance* inst, const gs_nr_options& o, gs_nr_result* results,
+               double* vm_out, double* va_out, bool plain);
+
+void fill_result(const InstState& s, gs_nr_result& r, int refreshes) {
+  r.status = s.nonfinite ? GS_PF_NONFINITE : s.converged ? GS_PF_CONVERGED : s.diverged ? GS_PF_DIVERGED
+                                                                                       : GS_PF_MAX_ITERATIONS;
+  r.iterations = s.iter;
+  r.max_mismatch = s.max_mis;
+  r.symbolic_refreshes = refreshes;
+  r.step_failures = s.step_failures;
+  r.min_pivot_ratio = s.min_ratio;
+}
+
+struct Prepared {
+  std::vector<InstDesc> descs;
+  std::vector<char> staging;   // per-instance spec / start uploads
+  std::vector<size_t> staging_dst;
+};
+
+}  // namespace
+
+namespace {
+
+// Core launcher for instances sharing one device plan. `states` may be pre-filled (resume) when init == 0.
+double launch_instances(gs_ctx* ctx, DevicePlan& dp, DevMem& inst_buf, DevMem& desc_buf, DevMem& state_buf,
+                        DevMem& aux_buf, int count, const gs_instance* inst, const YBus* y, const Network* netw,
+                        const BusSpec& base_spec, const gs_nr_options& o, bool plain,
+                        std::vector<InstState>& states, bool init,
+                        const std::vector<std::vector<double>>* spec_override,
+                        const std::vector<std::vector<double>>* start_override, int* launches) {
+  hipStream_t s = ctx->stream;
+  const DevPlan& h = dp.h;
+  BatchRun run = plan_run(h, o, plain);
+  Layout L = instance_layout(h, run.fp32, run.x_lds);
+  size_t need = L.stride * (size_t)count;
+  if (ctx->budget && need > ctx->budget) fail(GS_OUT_OF_MEMORY, "batch exceeds the HBM budget");
+  inst_buf.ensure(need);
+  desc_buf.ensure(sizeof(InstDesc) * (size_t)count);
+  state_buf.ensure(sizeof(InstState) * (size_t)count);
+  char* base = inst_buf.as<char>();
+  InstState* dstate = state_buf.as<InstState>();
+  std::vector<InstDesc> descs(count);
+  // staged per-instance host data: specs and starts, uploaded with one copy each
+  std::vector<double> stage(4 * (size_t)h.n * count, 0.0);
+  bool any_stage = false;
+  for (int i = 0; i < count; i++) {
+    InstDesc& d = descs[i];
+    std::memset(&d, 0, sizeof(d));
+    char* b = base + L.stride * i;
+    d.plan = dp.d_struct.as<DevPlan>();
+    d.state = dstate + i;
+    d.vm = (double*)(b + L.vm); d.va = (double*)(b + L.va); d.p = (double*)(b + L.p); d.q = (double*)(b + L.q);
+    d.f = (double*)(b + L.f); d.dx = (double*)(b + L.dx); d.r = (double*)(b + L.r); d.jv = (double*)(b + L.jv);
+    d.lx = b + L.lx; d.ux = b + L.ux; d.xg = run.x_lds ? nullptr : b + L.xg;
+    const gs_instance& in = inst[i];
+    d.n_patch = (y && netw) ? outage_patches(*y, *netw, in.outage_branch, d.patch) : 0;
+    // specified injections: the network's, or a load-scaled copy
+    const std::vector<double>* sp = spec_override ? &(*spec_override)[i] : nullptr;
+    std::vector<double> scaled;
+    if (!sp && netw && in.load_scale != 1.0) {
+      BusSpec bs = build_bus_spec(*netw, o.flat_start != 0, in.load_scale);
+      scaled.resize(2 * (size_t)h.n);
+      std::copy(bs.pspec.begin(), bs.pspec.end(), scaled.begin());
+      std::copy(bs.qspec.begin(), bs.qspec.end(), scaled.begin() + h.n);
+      sp = &scaled;
+    }
+    if (sp) {
+      std::copy(sp->begin(), sp->end(), stage.begin() + 4 * (size_t)h.n * i);
+      d.pspec = (const double*)(b + L.spec);
+      d.qspec = d.pspec + h.n;
+      any_stage = true;
+    } else {
+      d.pspec = nullptr;  // patched below to the network spec
+      d.qspec = nullptr;
+    }
+    const std::vector<double>* st = start_override ? &(*start_override)[i] : nullptr;
+    std::vector<double> own;
+    if (!st && in.vm0 && in.va0) {
+      own.assign(in.vm0, in.vm0 + h.n);
+      own.insert(own.end(), in.va0, in.va0 + h.n);
+      st = &own;
+    }
+    if (st) {
+      std::copy(st->begin(), st->end(), stage.begin() + 4 * (size_t)h.n * i + 2 * (size_t)h.n);
+      d.vm_start = (const double*)(b + L.start);
+      d.va_start = d.vm_start + h.n;
+      any_stage = true;
+    } else {
+      d.vm_start = h.vm0;
+      d.va_start = h.va0;
+    }
+  }
+  // the network's own specification (load scale 1) lives in the aux buffer
+  {
+    std::vector<double> sp(base_spec.pspec);
+    sp.insert(sp.end(), base_spec.qspec.begin(), base_spec.qspec.end());
+    upload(aux_buf, sp, s);
+    for (auto& d : descs)
+      if (!d.pspec) { d.pspec = aux_buf.as<double>(); d.qspec = aux_buf.as<double>() + h.n; }
+  }
+  if (any_stage) {
+    for (int i = 0; i < count; i++) {
+      char* b = base + L.stride * i;
+      const double* src = stage.data() + 4 * (size_t)h.n * i;
+      if (descs[i].pspec == (const double*)(b + L.spec))
+        check(hipMemcpyAsync(b + L.spec, src, 16 * (size_t)h.n, hipMemcpyHostToDevice, s), "stage spec");
+      if (descs[i].vm_start == (const double*)(b + L.start))
+        check(hipMemcpyAsync(b + L.start, src + 2 * h.n, 16 * (size_t)h.n, hipMemcpyHostToDevice, s), "stage start");
+    }
+  }
+  check(hipMemcpyAsync(desc_buf.p, descs.data(), sizeof(InstDesc) * count, hipMemcpyHostToDevice, s), "descs");
+  if (!init) check(hipMemcpyAsync(dstate, states.data(), sizeof(InstState) * count, hipMemcpyHostToDevice, s),
+                   "states");
+  else check(hipMemsetAsync(dstate, 0, sizeof(InstState) * count, s), "state clear");
+  NrParams prm{};
+  prm.tolerance = o.tolerance;
+  prm.diverge_limit = o.diverge_limit;
+  prm.growth_tol = o.growth_tol;
+  prm.max_iterations = o.max_iterations;
+  prm.refine_steps = run.fp32 ? o.refine_steps : 0;
+  prm.x_in_lds = run.x_lds;
+  prm.load_start = start_override ? 1 : 0;
+  hipEvent_t e0, e1;
+  check(hipEventCreate(&e0), "event");
+  check(hipEventCreate(&e1), "event");
+  check(hipEventRecord(e0, s), "event");
+  int nl = 0;
+  if (!plain) {
+    prm.steps_this_launch = 1 << 30;
+    prm.init = init ? 1 : 0;
+    check(launch_nr_batch(run.fp32, run.x_lds, desc_buf.as<InstDesc>(), count, prm, run.lds_bytes, s), "nr kernel");
+    nl = 1;
+  } else {
+    // one instance per launch, one Newton step per launch, the host reading the verdict after each
+    prm.steps_this_launch = 1;
+    const int load_start = prm.load_start;
+    for (int i = 0; i < count; i++) {
+      bool first = true;
+      for (;;) {
+        prm.init = (first && init) ? 1 : 0;
+        prm.load_start = first ? load_start : 0;
+        first = false;
+        check(launch_nr_batch(0, run.x_lds, desc_buf.as<InstDesc>() + i, 1, prm, run.lds_bytes, s), "plain kernel");
+        nl++;
+        InstState hs;
+        check(hipMemcpyAsync(&hs, dstate + i, sizeof(InstState), hipMemcpyDeviceToHost, s), "plain state");
+        check(hipStreamSynchronize(s), "plain sync");
+        if (hs.done) break;
+      }
+    }
+  }
+  check(hipEventRecord(e1, s), "event");
+  states.resize(count);
+  check(hipMemcpyAsync(states.data(), dstate, sizeof(InstState) * count, hipMemcpyDeviceToHost, s), "states back");
+  check(hipStreamSynchronize(s), "sync");
+  float ms = 0;
+  check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (launches) *launches += nl;
+  (void)L;
+  return ms;
+}
+
+void copy_state_out(gs_ctx* ctx, const DevMem& inst_buf, const Layout& L, int i, int n, double* vm, double* va) {
+  const char* b = inst_buf.as<char>() + L.stride * i;
+  if (vm) check(hipMemcpyAsync(vm, b + L.vm, 8 * (size_t)n, hipMemcpyDeviceToHost, ctx->stream), "vm out");
+  if (va) check(hipMemcpyAsync(va, b + L.va, 8 * (size_t)n, hipMemcpyDeviceToHost, ctx->stream), "va out");
+}
+
+void run_batch(gs_net* net, int count, const gs_instance* inst, const gs_nr_options& o, gs_nr_result* results,
+               double* vm_out, double* va_out, bool plain) {
+  gs_ctx* ctx = net->ctx;
+  const int n = net->net.n;
+  std::vector<InstState> states;
+  auto t0 = std::chrono::steady_clock::now();
+  int launches = 0;
+  double kms = launch_instances(ctx, net->dev, net->inst_buf, net->desc_buf, net->state_buf, net->aux_buf, count,
+                                inst, &net->y, &net->net, net->spec, o, plain, states, true, nullptr, nullptr,
+                                &launches);
+  BatchRun run = plan_run(net->dev.h, o, plain);
+  Layout L = instance_layout(net->dev.h, run.fp32, run.x_lds);
+  for (int i = 0; i < count; i++) {
+    if (states[i].done == 1) {
+      fill_result(states[i], results[i], 0);
+      copy_state_out(ctx, net->inst_buf, L, i, n, vm_out ? vm_out + (size_t)n * i : nullptr,
+                     va_out ? va_out + (size_t)n * i : nullptr);
+    }
+  }
+  check(hipStreamSynchronize(ctx->stream), "sync out");
+  // Paused instances: a static pivot failed its growth check. Fresh CPU symbolic pass on the instance's own network
+  // at its current state, then resume on that plan (the resumed run keeps its iteration count and start mismatch).
+  for (int i = 0; i < count; i++) {
+    if (states[i].done == 1) continue;
+    int refreshes = 0;
+    std::vector<double> vm(n), va(n);
+    copy_state_out(ctx, net->inst_buf, L, i, n, vm.data(), va.data());
+    check(hipStreamSynchronize(ctx->stream), "sync pause");
+    InstState st = states[i];
+    Network own = net->net;
+    if (inst[i].outage_branch >= 0) own.br_in[inst[i].outage_branch] = 0;
+    if (inst[i].load_scale != 1.0) {
+      for (int b = 0; b < own.n; b++) { own.pd[b] *= inst[i].load_scale; own.qd[b] *= inst[i].load_scale; }
+      for (int g = 0; g < own.ng; g++) own.gen_pg[g] *= inst[i].load_scale;
+    }
+    YBus y = build_ybus(own, -1);
+    BusSpec spec = build_bus_spec(own, o.flat_start != 0, 1.0);
+    JacobianPattern jp = build_jacobian_pattern(y, spec.type);
+    DevicePlan dp;
+    DevMem ib, db, sb, ab;
+    bool finished = false;
+    while (!finished) {
+      refreshes++;
+      Plan plan;
+      std::string why;
+      if (!symbolic_pass(y, jp, vm.data(), va.data(), plan, why) || refreshes > 8) {
+        // singular at this state: the step cannot be solved and the run stops (NewtonShell.SolveStep false)
+        st.done = 1;
+        st.step_failures++;
+        break;
+      }
+      upload_plan(dp, &y, jp, &spec, plan, own.base, ctx->stream);
+      gs_instance one = inst[i];
+      one.outage_branch = -1;
+      one.load_scale = 1.0;
+      std::vector<std::vector<double>> start(1);
+      start[0] = vm;
+      start[0].insert(start[0].end(), va.begin(), va.end());
+      std::vector<InstState> rs(1, st);
+      rs[0].done = 0;
+      // resumed: the device keeps iter / start mismatch / counters from the paused state
+      kms += launch_instances(ctx, dp, ib, db, sb, ab, 1, &one, nullptr, nullptr, spec, o, plain, rs, false, nullptr,
+                              &start, &launches);
+      st = rs[0];
+      BatchRun r2 = plan_run(dp.h, o, plain);
+      Layout L2 = instance_layout(dp.h, r2.fp32, r2.x_lds);
+      copy_state_out(ctx, ib, L2, 0, n, vm.data(), va.data());
+      check(hipStreamSynchronize(ctx->stream), "sync resume");
+      finished = st.done == 1;
+    }
+    fill_result(st, results[i], refreshes);
+    if (vm_out) std::copy(vm.begin(), vm.end(), vm_out + (size_t)n * i);
+    if (va_out) std::copy(va.begin(), va.end(), va_out + (size_t)n * i);
+  }
+  net->last_kernel_ms = kms;
+  net->last_launches = launches;
+  net->last_total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace
+
+// ================================================================================================ seam solves
+struct SeamPlan {
+  DevicePlan dev;
+  std::vector<int> nonzero_pos;
+};
+
+namespace {
+
+gs_status seam_solve_one(gs_ctx* ctx, int n, const int* rowptr, const int* colidx, const double* val,
+                         const double* b, double* x) {
+  if (n < 0 || !rowptr || (n > 0 && (!colidx || !val || !b || !x))) fail(GS_INVALID_ARGUMENT, "null argument");
+  if (n == 0) return GS_OK;
+  if (rowptr[0] != 0) fail(GS_INVALID_ARGUMENT, "rowptr[0] must be 0");
+  for (int i = 0; i < n; i++) {
+    if (rowptr[i + 1] < rowptr[i]) fail(GS_INVALID_ARGUMENT, "rowptr must be non-decreasing");
+    for (int k = rowptr[i]; k < rowptr[i + 1]; k++) {
+      if (colidx[k] < 0 || colidx[k] >= n) fail(GS_INVALID_ARGUMENT, "column index out of range");
+      if (k > rowptr[i] && colidx[k] <= colidx[k - 1]) fail(GS_INVALID_ARGUMENT, "columns must be sorted per row");
+    }
+  }
+  int nnz = rowptr[n];
+  for (int k = 0; k < nnz; k++)
+    if (!std::isfinite(val[k])) {
+      ctx->err = "Sparse LU: the matrix contains a non-finite entry, so it cannot be factorised.";
+      return GS_NONFINITE_INPUT;
+    }
+  std::vector<int> rp(rowptr, rowptr + n + 1), ci(colidx, colidx + nnz);
+  auto key = std::make_pair(rp, ci);
+  auto it = ctx->seam_orders.find(key);
+  if (it == ctx->seam_orders.end()) it = ctx->seam_orders.emplace(key, minimum_degree(n, rowptr, colidx)).first;
+  const std::vector<int>& order = it->second;
+  std::vector<int> nz;
+  nz.reserve(nnz);
+  for (int k = 0; k < nnz; k++) if (val[k] != 0.0) nz.push_back(k);
+  hipStream_t s = ctx->stream;
+  JacobianPattern jp;
+  jp.m = n;
+  jp.rp = rp;
+  jp.ci = ci;
+  std::vector<double> xt(n);
+  auto run_device = [&](SeamPlan& sp) -> int {
+    DevMem a, bb, xx, lx, ux, xs, rf, stt, desc;
+    a.ensure(8 * (size_t)nnz); bb.ensure(8 * (size_t)n); xx.ensure(8 * (size_t)n);
+    lx.ensure(8 * (size_t)sp.dev.h.nnz_l); ux.ensure(8 * (size_t)sp.dev.h.nnz_u); xs.ensure(8 * (size_t)n);
+    rf.ensure(8 * (size_t)n); stt.ensure(4);
+    check(hipMemcpyAsync(a.p, val, 8 * (size_t)nnz, hipMemcpyHostToDevice, s), "seam a");
+    check(hipMemcpyAsync(bb.p, b, 8 * (size_t)n, hipMemcpyHostToDevice, s), "seam b");
+    SeamDesc d{sp.dev.d_struct.as<DevPlan>(), a.as<double>(), bb.as<double>(), xx.as<double>(), lx.as<double>(),
+               ux.as<double>(), xs.as<double>(), rf.as<double>(), stt.as<int>(), sp.dev.piv_pos.as<int>()};
+    desc.ensure(sizeof(SeamDesc));
+    check(hipMemcpyAsync(desc.p, &d, sizeof(SeamDesc), hipMemcpyHostToDevice, s), "seam desc");
+    size_t lds = 8 * (((size_t)sp.dev.h.max_col + 7) & ~size_t(7));
+    check(launch_seam(desc.as<SeamDesc>(), 1, lds, s), "seam kernel");
+    int status = 1;
+    check(hipMemcpyAsync(&status, stt.p, 4, hipMemcpyDeviceToHost, s), "seam status");
+    check(hipMemcpyAsync(xt.data(), xx.p, 8 * (size_t)n, hipMemcpyDeviceToHost, s), "seam x");
+    check(hipStreamSynchronize(s), "seam sync");
+    return status;
+  };
+  auto pit = ctx->seam_plans.find(nz);
+  if (pit != ctx->seam_plans.end()) {
+    if (run_device(*pit->second) == 0) {
+      for (int k = 0; k < n; k++)
+        if (!std::isfinite(b[k])) {
+          ctx->err = "Sparse LU: the right-hand side has a non-finite entry.";
+          return GS_NONFINITE_INPUT;
+        }
+      std::copy(xt.begin(), xt.end(), x);
+      return GS_OK;
+    }
+  }
+  Plan plan;
+  std::string why;
+  if (!gp_plan(n, rp, ci, val, order, 0.0, false, plan, why)) {
+    ctx->err = why;
+    return GS_SINGULAR;
+  }
+  for (int k = 0; k < n; k++)
+    if (!std::isfinite(b[k])) {
+      ctx->err = "Sparse LU: the right-hand side has a non-finite entry.";
+      return GS_NONFINITE_INPUT;
+    }
+  auto sp = std::make_shared<SeamPlan>();
+  upload_plan(sp->dev, nullptr, jp, nullptr, plan, 100.0, s);
+  sp->nonzero_pos = nz;
+  int st = run_device(*sp);
+  if (st != 0) fail(GS_DEVICE_ERROR, "the device refused the replay of its own factorisation");
+  ctx->seam_plans[nz] = sp;
+  std::copy(xt.begin(), xt.end(), x);
+  return GS_OK;
+}
+
+}  // namespace
+
+// ================================================================================================ C ABI
+namespace {
+template <typename F>
+gs_status guarded(gs_ctx* ctx, F&& f) {
+  if (!ctx) return GS_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  try {
+    return f();
+  } catch (const HipError& e) {
+    ctx->err = e.what;
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    ctx->err = "host allocation failed";
+    return GS_OUT_OF_MEMORY;
+  } catch (const std::exception& e) {
+    ctx->err = e.what();
+    return GS_DEVICE_ERROR;
+  }
+}
+}  // namespace
+
+extern "C" {
+
+GS_API const char* gs_version(void) { return "gridsim-hip 0.1 (gfx950)"; }
+
+GS_API gs_status gs_init(int32_t device, size_t hbm_budget_bytes, gs_ctx** out) {
+  if (!out) return GS_INVALID_ARGUMENT;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return GS_NO_DEVICE;
+  if (device < 0 || device >= count) return GS_INVALID_ARGUMENT;
+  if (hipSetDevice(device) != hipSuccess) return GS_DEVICE_ERROR;
+  auto* ctx = new gs_ctx();
+  ctx->device = device;
+  ctx->budget = hbm_budget_bytes;
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return GS_DEVICE_ERROR;
+  }
+  *out = ctx;
+  return GS_OK;
+}
+
+GS_API void gs_shutdown(gs_ctx* ctx) {
+  if (!ctx) return;
+  ctx->seam_plans.clear();
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+GS_API const char* gs_last_error(const gs_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+GS_API gs_status gs_device_info_get(gs_ctx* ctx, gs_device_info* out) {
+  return guarded(ctx, [&]() -> gs_status {
+    if (!out) return GS_INVALID_ARGUMENT;
+    std::memset(out, 0, sizeof(*out));
+    hipDeviceProp_t p;
+    check(hipGetDeviceProperties(&p, ctx->device), "properties");
+    std::snprintf(out->name, sizeof(out->name), "%s", p.name);
+    std::snprintf(out->arch, sizeof(out->arch), "%s", p.gcnArchName);
+    out->compute_units = p.multiProcessorCount;
+    out->hbm_bytes = p.totalGlobalMem;
+    DevMem d;
+    d.ensure(3 * sizeof(double));
+    check(launch_fp64_probe(d.as<double>(), ctx->stream), "probe");
+    double h[3];
+    check(hipMemcpyAsync(h, d.p, sizeof(h), hipMemcpyDeviceToHost, ctx->stream), "probe back");
+    check(hipStreamSynchronize(ctx->stream), "probe sync");
+    out->fp64_ok = h[0] == 9.094947017729282e-13;
+    double s, c;
+    gs_fdlibm::sincos(3.141592653589793 / 7.0, &s, &c);
+    out->fdlibm_ok = (h[1] == s && h[2] == c);
+    return GS_OK;
+  });
+}
+
+GS_API gs_status gs_sparse_solve(gs_ctx* ctx, int32_t n, const int32_t* rowptr, const int32_t* colidx,
+                                 const double* val, const double* b, double* x) {
+  return guarded(ctx, [&]() { return seam_solve_one(ctx, n, rowptr, colidx, val, b, x); });
+}
+
+GS_API gs_status gs_sparse_solve_batch(gs_ctx* ctx, int32_t count, const gs_csr* systems, const double* const* b,
+                                       double* const* x, gs_status* per_system) {
+  return guarded(ctx, [&]() -> gs_status {
+    if (count < 0 || (count > 0 && (!systems || !b || !x || !per_system))) fail(GS_INVALID_ARGUMENT, "null argument");
+    for (int s = 0; s < count; s++) {
+      try {
+        per_system[s] = seam_solve_one(ctx, systems[s].n, systems[s].rowptr, systems[s].colidx, systems[s].val,
+                                       b[s], x[s]);
+      } catch (const HipError& e) {
+        if (e.code == GS_DEVICE_ERROR) throw;
+        per_system[s] = e.code;
+      }
+    }
+    return GS_OK;
+  });
+}
+
+GS_API gs_status gs_network_upload(gs_ctx* ctx, const gs_network* net, const gs_nr_options* opts, gs_net** out) {
+  return guarded(ctx, [&]() -> gs_status {
+    if (!net || !out || net->n_bus <= 0 || net->n_branch < 0 || net->n_gen < 0) fail(GS_INVALID_ARGUMENT, "network");
+    *out = nullptr;
+    gs_nr_options o = normalise(opts);
+    auto h = std::make_unique<gs_net>();
+    h->ctx = ctx;
+    h->net = copy_network(*net);
+    for (int k = 0; k < h->net.nb; k++)
+      if (h->net.br_from[k] < 0 || h->net.br_from[k] >= h->net.n || h->net.br_to[k] < 0 || h->net.br_to[k] >= h->net.n)
+        fail(GS_INVALID_ARGUMENT, "branch end outside the bus range");
+    for (int g = 0; g < h->net.ng; g++)
+      if (h->net.gen_bus[g] < 0 || h->net.gen_bus[g] >= h->net.n) fail(GS_INVALID_ARGUMENT, "generator bus");
+    h->flat_start = o.flat_start != 0;
+    h->y = build_ybus(h->net, -1);
+    h->spec = build_bus_spec(h->net, h->flat_start, 1.0);
+    h->jp = build_jacobian_pattern(h->y, h->spec.type);
+    std::string why;
+    if (!symbolic_pass(h->y, h->jp, h->spec.vm.data(), h->spec.va.data(), h->plan, why)) {
+      ctx->err = why;
+      return GS_SINGULAR;
+    }
+    upload_plan(h->dev, &h->y, h->jp, &h->spec, h->plan, h->net.base, ctx->stream);
+    check(hipStreamSynchronize(ctx->stream), "upload sync");
+    *out = h.release();
+    return GS_OK;
+  });
+}
+
+GS_API void gs_network_free(gs_net* net) { delete net; }
+
+GS_API gs_status gs_nr_solve_batch(gs_net* net, int32_t count, const gs_instance* inst, const gs_nr_options* opts,
+                                   gs_nr_result* results, double* vm_out, double* va_out) {
+  if (!net) return GS_INVALID_ARGUMENT;
+  return guarded(net->ctx, [&]() -> gs_status {
+    if (count < 0 || (count > 0 && (!inst || !results))) fail(GS_INVALID_ARGUMENT, "null argument");
+    gs_nr_options o = normalise(opts);
+    if ((o.flat_start != 0) != net->flat_start) fail(GS_INVALID_ARGUMENT, "flat_start differs from the upload");
+    for (int i = 0; i < count; i++)
+      if (inst[i].outage_branch < -1 || inst[i].outage_branch >= net->net.nb || !(inst[i].load_scale > 0))
+        fail(GS_INVALID_ARGUMENT, "instance " + std::to_string(i));
+    if (count == 0) return GS_OK;
+    run_batch(net, count, inst, o, results, vm_out, va_out, false);
+    return GS_OK;
+  });
+}
+
+GS_API gs_status gs_nr_solve_plain(gs_net* net, int32_t count, const gs_instance* inst, const gs_nr_options* opts,
+                                   gs_nr_result* results, double* vm_out, double* va_out) {
+  if (!net) return GS_INVALID_ARGUMENT;
+  return guarded(net->ctx, [&]() -> gs_status {
+    if (count < 0 || (count > 0 && (!inst || !results))) fail(GS_INVALID_ARGUMENT, "null argument");
+    gs_nr_options o = normalise(opts);
+    o.lu_precision = GS_LU_FP64;
+    if ((o.flat_start != 0) != net->flat_start) fail(GS_INVALID_ARGUMENT, "flat_start differs from the upload");
+    if (count == 0) return GS_OK;
+    run_batch(net, count, inst, o, results, vm_out, va_out, true);
+    return GS_OK;
+  });
+}
+
+GS_API gs_status gs_network_plan_sizes(const gs_net* net, int32_t* m, int32_t* nnz_j, int32_t* nnz_l,
+                                       int32_t* nnz_u) {
+  if (!net) return GS_INVALID_ARGUMENT;
+  if (m) *m = net->jp.m;
+  if (nnz_j) *nnz_j = net->jp.rp[net->jp.m];
+  if (nnz_l) *nnz_l = net->plan.lp[net->plan.m];
+  if (nnz_u) *nnz_u = net->plan.up[net->plan.m];
+  return GS_OK;
+}
+
+GS_API gs_status gs_network_plan_get(const gs_net* net, int32_t* q, int32_t* pivot_row, int32_t* lp, int32_t* li,
+                                     int32_t* up, int32_t* ui, int32_t* j_rowptr, int32_t* j_colidx) {
+  if (!net) return GS_INVALID_ARGUMENT;
+  auto cp = [](int32_t* dst, const std::vector<int>& v) { if (dst) std::copy(v.begin(), v.end(), dst); };
+  cp(q, net->plan.q); cp(pivot_row, net->plan.pivot_row); cp(lp, net->plan.lp); cp(li, net->plan.li);
+  cp(up, net->plan.up); cp(ui, net->plan.ui); cp(j_rowptr, net->jp.rp); cp(j_colidx, net->jp.ci);
+  return GS_OK;
+}
+
+}  // extern "C"
+
+struct gs_plan {
+  Network net;
+  YBus y;
+  BusSpec spec;
+  JacobianPattern jp;
+  Plan plan;
+  DevicePlanHost dph;
+};
+
+extern "C" {
+
+GS_API gs_status gs_plan_build(const gs_network* net, const gs_nr_options* opts, gs_plan** out) {
+  if (!net || !out || net->n_bus <= 0) return GS_INVALID_ARGUMENT;
+  *out = nullptr;
+  try {
+    gs_nr_options o = normalise(opts);
+    auto p = std::make_unique<gs_plan>();
+    p->net = copy_network(*net);
+    p->y = build_ybus(p->net, -1);
+    p->spec = build_bus_spec(p->net, o.flat_start != 0, 1.0);
+    p->jp = build_jacobian_pattern(p->y, p->spec.type);
+    std::string why;
+    if (!symbolic_pass(p->y, p->jp, p->spec.vm.data(), p->spec.va.data(), p->plan, why)) return GS_SINGULAR;
+    compile_plan(p->plan, p->dph);
+    *out = p.release();
+    return GS_OK;
+  } catch (const HipError& e) {
+    return e.code;
+  } catch (...) {
+    return GS_INVALID_ARGUMENT;
+  }
+}
+
+GS_API gs_status gs_plan_sizes(const gs_plan* p, int32_t* m, int32_t* nnz_j, int32_t* nnz_l, int32_t* nnz_u,
+                               int32_t* n_levels) {
+  if (!p) return GS_INVALID_ARGUMENT;
+  if (m) *m = p->jp.m;
+  if (nnz_j) *nnz_j = p->jp.rp[p->jp.m];
+  if (nnz_l) *nnz_l = p->plan.lp[p->plan.m];
+  if (nnz_u) *nnz_u = p->plan.up[p->plan.m];
+  if (n_levels) *n_levels = p->dph.n_levels;
+  return GS_OK;
+}
+
+GS_API gs_status gs_plan_get(const gs_plan* p, int32_t* q, int32_t* pivot_row, int32_t* lp, int32_t* li,
+                             int32_t* up, int32_t* ui, int32_t* j_rowptr, int32_t* j_colidx) {
+  if (!p) return GS_INVALID_ARGUMENT;
+  auto cp = [](int32_t* dst, const std::vector<int>& v) { if (dst) std::copy(v.begin(), v.end(), dst); };
+  cp(q, p->plan.q); cp(pivot_row, p->plan.pivot_row); cp(lp, p->plan.lp); cp(li, p->plan.li);
+  cp(up, p->plan.up); cp(ui, p->plan.ui); cp(j_rowptr, p->jp.rp); cp(j_colidx, p->jp.ci);
+  return GS_OK;
+}
+
+GS_API void gs_plan_free(gs_plan* p) { delete p; }
+
+GS_API gs_status gs_last_timing(const gs_net* net, double* total_ms, double* kernel_ms, int32_t* launches) {
+  if (!net) return GS_INVALID_ARGUMENT;
+  if (total_ms) *total_ms = net->last_total_ms;
+  if (kernel_ms) *kernel_ms = net->last_kernel_ms;
+  if (launches) *launches = net->last_launches;
+  return GS_OK;
+}
+
+GS_API gs_status gs_device_sincos(gs_ctx* ctx, int64_t n, const double* x, double* s, double* c) {
+  return guarded(ctx, [&]() -> gs_status {
+    if (n < 0 || (n > 0 && (!x || !s || !c))) fail(GS_INVALID_ARGUMENT, "null argument");
+    if (n == 0) return GS_OK;
+    DevMem dx, ds, dc;
+    dx.ensure(8 * (size_t)n); ds.ensure(8 * (size_t)n); dc.ensure(8 * (size_t)n);
+    check(hipMemcpyAsync(dx.p, x, 8 * (size_t)n, hipMemcpyHostToDevice, ctx->stream), "sincos in");
+    check(launch_sincos(dx.as<double>(), ds.as<double>(), dc.as<double>(), n, ctx->stream), "sincos kernel");
+    check(hipMemcpyAsync(s, ds.p, 8 * (size_t)n, hipMemcpyDeviceToHost, ctx->stream), "sincos s");
+    check(hipMemcpyAsync(c, dc.p, 8 * (size_t)n, hipMemcpyDeviceToHost, ctx->stream), "sincos c");
+    check(hipStreamSynchronize(ctx->stream), "sincos sync");
+    return GS_OK;
+  });
+}
+
+GS_API void gs_host_sincos(int64_t n, const double* x, double* s, double* c) {
+  for (int64_t i = 0; i < n; i++) gs_fdlibm::sincos(x[i], &s[i], &c[i]);
+}
+
+}  // extern "C"

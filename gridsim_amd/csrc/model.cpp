@@ -1,0 +1,548 @@
+// Host-side model and symbolic analysis (see model.h). Reference files restated:
+//   AdmittanceStamp.cs:35-75 (stamp), SparseNewtonRaphsonPowerFlow.cs:382-402 (BuildY), Sparse.cs:145-222
+//   (ComplexCsr.Builder: stable (column, insertion) order, duplicates summed from +0.0, zero diagonal injected),
+//   NewtonShell.cs:433-486 (BuildBusSpec), :327-336 (EquationPositions), SparseNewtonRaphsonPowerFlow.cs:269-380
+//   (injections, Jacobian structure and refill), MinimumDegreeOrdering.cs:23-136, SparseLuFactorization.cs:69-195
+//   and :753-821 (Gilbert-Peierls with the plan), NumericTolerances.cs:36,99-115 (row pivot floors).
+#include "model.h"
+
+#include <algorithm>
+#include <cmath>
+#include <limits>
+#include <stdexcept>
+
+#include "fdlibm.h"
+
+namespace gs {
+
+namespace {
+constexpr double kPi = 3.141592653589793;
+
+inline Cx add(Cx a, Cx b) { return {a.re + b.re, a.im + b.im}; }
+// System.Numerics.Complex division (Smith's formula, as the .NET BCL operator computes it).
+inline Cx cdiv(Cx l, Cx r) {
+  double a = l.re, b = l.im, c = r.re, d = r.im;
+  if (std::fabs(d) < std::fabs(c)) {
+    double doc = d / c;
+    return {(a + b * doc) / (c + d * doc), (b - a * doc) / (c + d * doc)};
+  }
+  double cod = c / d;
+  return {(b + a * cod) / (d + c * cod), (-a + b * cod) / (d + c * cod)};
+}
+inline Cx cdiv_real(Cx l, double r) {
+  if (r == 0) return {std::nan(""), std::nan("")};
+  return {l.re / r, l.im / r};
+}
+}  // namespace
+
+Stamp branch_stamp(double r, double x, double b, double tap_ratio, double shift_deg) {
+  Cx ys = cdiv(Cx{1.0, 0.0}, Cx{r * 1.0, x * 1.0});
+  Cx bc{0.0, (b * 1.0) / 2.0};
+  double a = tap_ratio == 0 ? 1.0 : tap_ratio;
+  double shift = shift_deg * kPi / 180.0;
+  double s, c;
+  gs_fdlibm::sincos(shift, &s, &c);
+  Cx tap{a * c, a * s};
+  Stamp st;
+  st.yff = cdiv_real(add(ys, bc), a * a);
+  st.ytt = add(ys, bc);
+  st.yft = cdiv(Cx{-ys.re, -ys.im}, Cx{tap.re, -tap.im});
+  st.ytf = cdiv(Cx{-ys.re, -ys.im}, tap);
+  return st;
+}
+
+int YBus::find(int row, int col) const {
+  auto b = ci.begin() + rp[row], e = ci.begin() + rp[row + 1];
+  auto it = std::lower_bound(b, e, col);
+  return (it != e && *it == col) ? (int)(it - ci.begin()) : -1;
+}
+
+Network copy_network(const gs_network& s) {
+  Network n;
+  n.n = s.n_bus;
+  n.nb = s.n_branch;
+  n.ng = s.n_gen;
+  n.base = s.base_mva;
+  auto cp = [](auto& dst, const auto* src, int cnt) { dst.assign(src, src + cnt); };
+  cp(n.bus_type, s.bus_type, n.n);
+  cp(n.pd, s.pd, n.n); cp(n.qd, s.qd, n.n); cp(n.gs, s.gs, n.n); cp(n.bs, s.bs, n.n);
+  cp(n.vm, s.vm, n.n); cp(n.va_deg, s.va_deg, n.n);
+  cp(n.br_from, s.br_from, n.nb); cp(n.br_to, s.br_to, n.nb);
+  cp(n.br_r, s.br_r, n.nb); cp(n.br_x, s.br_x, n.nb); cp(n.br_b, s.br_b, n.nb);
+  cp(n.br_tap, s.br_tap, n.nb); cp(n.br_shift, s.br_shift_deg, n.nb); cp(n.br_in, s.br_in_service, n.nb);
+  cp(n.gen_bus, s.gen_bus, n.ng); cp(n.gen_pg, s.gen_pg, n.ng); cp(n.gen_qg, s.gen_qg, n.ng);
+  cp(n.gen_vg, s.gen_vg, n.ng);
+  return n;
+}
+
+YBus build_ybus(const Network& net, int skip) {
+  struct Trip { int row, col, branch; double g, b; };
+  vector<Trip> trips;
+  trips.reserve(4 * (size_t)net.nb + net.n);
+  auto add_trip = [&](int r, int c, int br, double g, double b) {
+    if (g == 0.0 && b == 0.0) return;  // ComplexCsr.Builder.Add skips an all-zero stamp
+    trips.push_back({r, c, br, g, b});
+  };
+  for (int k = 0; k < net.nb; k++) {
+    if (k == skip || !net.br_in[k]) continue;
+    int f = net.br_from[k], t = net.br_to[k];
+    Stamp s = branch_stamp(net.br_r[k], net.br_x[k], net.br_b[k], net.br_tap[k], net.br_shift[k]);
+    add_trip(f, f, k, s.yff.re, s.yff.im);
+    add_trip(t, t, k, s.ytt.re, s.ytt.im);
+    add_trip(f, t, k, s.yft.re, s.yft.im);
+    add_trip(t, f, k, s.ytf.re, s.ytf.im);
+  }
+  for (int i = 0; i < net.n; i++) {
+    if (net.gs[i] == 0 && net.bs[i] == 0) continue;
+    add_trip(i, i, -1, net.gs[i] / net.base, net.bs[i] / net.base);
+  }
+  // Bucket by row keeping insertion order, then a stable sort by column inside each row.
+  int n = net.n;
+  vector<int> start(n + 1, 0);
+  for (auto& t : trips) start[t.row + 1]++;
+  for (int i = 0; i < n; i++) start[i + 1] += start[i];
+  vector<int> order(trips.size()), fill(start.begin(), start.end() - 1);
+  for (size_t t = 0; t < trips.size(); t++) order[fill[trips[t].row]++] = (int)t;
+  YBus y;
+  y.n = n;
+  y.rp.assign(n + 1, 0);
+  y.diag.assign(n, -1);
+  y.contrib_ptr.push_back(0);
+  auto open_cell = [&](int col) {
+    y.ci.push_back(col);
+    y.g.push_back(0.0);
+    y.b.push_back(0.0);
+    y.contrib_ptr.push_back(y.contrib_ptr.back());
+  };
+  for (int i = 0; i < n; i++) {
+    std::stable_sort(order.begin() + start[i], order.begin() + start[i + 1],
+                     [&](int a, int b) { return trips[a].col < trips[b].col; });
+    y.rp[i] = (int)y.ci.size();
+    int prev = std::numeric_limits<int>::min();
+    for (int p = start[i]; p < start[i + 1]; p++) {
+      const Trip& t = trips[order[p]];
+      if (t.col != prev) {
+        if (y.diag[i] < 0 && prev < i && t.col > i) { y.diag[i] = (int)y.ci.size(); open_cell(i); }
+        open_cell(t.col);
+        if (t.col == i) y.diag[i] = (int)y.ci.size() - 1;
+        prev = t.col;
+      }
+      // duplicates accumulate left to right from +0.0 (0.0 + first stamp, then += each further stamp)
+      y.g.back() += t.g;
+      y.b.back() += t.b;
+      y.contrib_branch.push_back(t.branch);
+      y.contrib_g.push_back(t.g);
+      y.contrib_b.push_back(t.b);
+      y.contrib_ptr.back()++;
+    }
+    if (y.diag[i] < 0) { y.diag[i] = (int)y.ci.size(); open_cell(i); }
+  }
+  y.rp[n] = (int)y.ci.size();
+  return y;
+}
+
+BusSpec build_bus_spec(const Network& net, bool flat_start, double load_scale) {
+  int n = net.n;
+  BusSpec s;
+  s.type.assign(n, GS_BUS_PQ);
+  s.vm.assign(n, 0.0); s.va.assign(n, 0.0); s.pspec.assign(n, 0.0); s.qspec.assign(n, 0.0);
+  vector<double> pg(n, 0.0), qg(n, 0.0), vg(n, 0.0);
+  vector<char> has(n, 0);
+  for (int g = 0; g < net.ng; g++) {
+    int i = net.gen_bus[g];
+    pg[i] += net.gen_pg[g] * load_scale;
+    qg[i] += net.gen_qg[g];
+    vg[i] = net.gen_vg[g];
+    has[i] = 1;
+  }
+  for (int i = 0; i < n; i++) {
+    int bt = net.bus_type[i];
+    s.type[i] = (bt == GS_BUS_PV && !has[i]) ? GS_BUS_PQ : bt;
+    bool vc = s.type[i] == GS_BUS_SLACK || s.type[i] == GS_BUS_PV;
+    s.vm[i] = vc && has[i] ? vg[i] : s.type[i] == GS_BUS_SLACK ? net.vm[i] : flat_start ? 1.0 : net.vm[i];
+    s.va[i] = bt == GS_BUS_SLACK ? net.va_deg[i] * kPi / 180.0 : flat_start ? 0.0 : net.va_deg[i] * kPi / 180.0;
+    s.pspec[i] = (pg[i] - net.pd[i] * load_scale) / net.base;
+    s.qspec[i] = (qg[i] - net.qd[i] * load_scale) / net.base;
+  }
+  return s;
+}
+
+JacobianPattern build_jacobian_pattern(const YBus& y, const vector<int>& type) {
+  JacobianPattern jp;
+  int n = y.n;
+  jp.ppos.assign(n, -1);
+  jp.qpos.assign(n, -1);
+  int m = 0;
+  for (int i = 0; i < n; i++) if (type[i] != GS_BUS_SLACK) jp.ppos[i] = m++;
+  for (int i = 0; i < n; i++) if (type[i] == GS_BUS_PQ) jp.qpos[i] = m++;
+  jp.m = m;
+  // Cells per Jacobian row: every (Y cell, block) maps to a distinct cell, so no duplicates arise; sort by column.
+  vector<vector<int>> rows(m);
+  for (int i = 0; i < n; i++) {
+    int pi = jp.ppos[i], qi = jp.qpos[i];
+    if (pi < 0 && qi < 0) continue;
+    for (int k = y.rp[i]; k < y.rp[i + 1]; k++) {
+      int j = y.ci[k], pj = jp.ppos[j], qj = jp.qpos[j];
+      if (pi >= 0 && pj >= 0) rows[pi].push_back(pj);
+      if (pi >= 0 && qj >= 0) rows[pi].push_back(qj);
+      if (qi >= 0 && pj >= 0) rows[qi].push_back(pj);
+      if (qi >= 0 && qj >= 0) rows[qi].push_back(qj);
+    }
+  }
+  jp.rp.assign(m + 1, 0);
+  for (int r = 0; r < m; r++) {
+    std::sort(rows[r].begin(), rows[r].end());
+    rows[r].erase(std::unique(rows[r].begin(), rows[r].end()), rows[r].end());
+    jp.rp[r + 1] = jp.rp[r] + (int)rows[r].size();
+  }
+  jp.ci.reserve(jp.rp[m]);
+  for (int r = 0; r < m; r++) jp.ci.insert(jp.ci.end(), rows[r].begin(), rows[r].end());
+  auto locate = [&](int r, int c) {
+    auto b = jp.ci.begin() + jp.rp[r], e = jp.ci.begin() + jp.rp[r + 1];
+    return (int)(std::lower_bound(b, e, c) - jp.ci.begin());
+  };
+  jp.slots.assign(4 * (size_t)y.rp[n], -1);
+  for (int i = 0; i < n; i++) {
+    int pi = jp.ppos[i], qi = jp.qpos[i];
+    for (int k = y.rp[i]; k < y.rp[i + 1]; k++) {
+      int j = y.ci[k], pj = jp.ppos[j], qj = jp.qpos[j];
+      size_t b = 4 * (size_t)k;
+      if (pi >= 0 && pj >= 0) jp.slots[b + 0] = locate(pi, pj);
+      if (pi >= 0 && qj >= 0) jp.slots[b + 1] = locate(pi, qj);
+      if (qi >= 0 && pj >= 0) jp.slots[b + 2] = locate(qi, pj);
+      if (qi >= 0 && qj >= 0) jp.slots[b + 3] = locate(qi, qj);
+    }
+  }
+  return jp;
+}
+
+void host_injections(const YBus& y, const double* vm, const double* va, double* p, double* q) {
+  for (int i = 0; i < y.n; i++) {
+    double pi = 0, qi = 0, vai = va[i];
+    for (int k = y.rp[i]; k < y.rp[i + 1]; k++) {
+      int j = y.ci[k];
+      double sn, cs;
+      gs_fdlibm::sincos(vai - va[j], &sn, &cs);
+      double vmj = vm[j];
+      pi += vmj * (y.g[k] * cs + y.b[k] * sn);
+      qi += vmj * (y.g[k] * sn - y.b[k] * cs);
+    }
+    p[i] = vm[i] * pi;
+    q[i] = vm[i] * qi;
+  }
+}
+
+void host_jacobian(const YBus& y, const JacobianPattern& jp, const double* vm, const double* va, const double* p,
+                   const double* q, double* val) {
+  std::fill(val, val + jp.rp[jp.m], 0.0);
+  for (int i = 0; i < y.n; i++) {
+    for (int k = y.rp[i]; k < y.rp[i + 1]; k++) {
+      int j = y.ci[k];
+      double g = y.g[k], b = y.b[k], h, nn, mm, l;
+      if (i == j) {
+        h = -q[i] - b * vm[i] * vm[i];
+        nn = p[i] / vm[i] + g * vm[i];
+        mm = p[i] - g * vm[i] * vm[i];
+        l = q[i] / vm[i] - b * vm[i];
+      } else {
+        double sn, cs;
+        gs_fdlibm::sincos(va[i] - va[j], &sn, &cs);
+        h = vm[i] * vm[j] * (g * sn - b * cs);
+        nn = vm[i] * (g * cs + b * sn);
+        mm = -vm[i] * vm[j] * (g * cs + b * sn);
+        l = vm[i] * (g * sn - b * cs);
+      }
+      const int* s = &jp.slots[4 * (size_t)k];
+      if (s[0] >= 0) val[s[0]] = h;
+      if (s[1] >= 0) val[s[1]] = nn;
+      if (s[2] >= 0) val[s[2]] = mm;
+      if (s[3] >= 0) val[s[3]] = l;
+    }
+  }
+}
+
+// Exact minimum degree on the explicit elimination graph with a bucket queue: LIFO among equal degrees, first
+// insertion in descending index (the lowest index wins the first ties), neighbours reinserted in adjacency order.
+vector<int> minimum_degree(int n, const int* rowptr, const int* colidx) {
+  vector<int> order(n);
+  if (n == 0) return order;
+  vector<vector<int>> adj(n);
+  {
+    vector<int> cnt(n, 0);
+    for (int i = 0; i < n; i++)
+      for (int k = rowptr[i]; k < rowptr[i + 1]; k++)
+        if (colidx[k] != i) { cnt[i]++; cnt[colidx[k]]++; }
+    for (int i = 0; i < n; i++) adj[i].reserve(cnt[i]);
+    for (int i = 0; i < n; i++)
+      for (int k = rowptr[i]; k < rowptr[i + 1]; k++) {
+        int j = colidx[k];
+        if (j == i) continue;
+        adj[i].push_back(j);
+        adj[j].push_back(i);
+      }
+  }
+  vector<int> mark(n, 0);
+  int tick = 0;
+  for (int i = 0; i < n; i++) {  // de-duplicate keeping first occurrences
+    tick++;
+    auto& a = adj[i];
+    size_t w = 0;
+    for (size_t p = 0; p < a.size(); p++)
+      if (mark[a[p]] != tick) { mark[a[p]] = tick; a[w++] = a[p]; }
+    a.resize(w);
+  }
+  vector<int> head(n + 1, -1), nxt(n, -1), prv(n, -1), deg(n, 0);
+  auto push = [&](int v, int d) {
+    deg[v] = d; prv[v] = -1; nxt[v] = head[d];
+    if (head[d] >= 0) prv[head[d]] = v;
+    head[d] = v;
+  };
+  auto pop = [&](int v) {
+    if (prv[v] >= 0) nxt[prv[v]] = nxt[v]; else head[deg[v]] = nxt[v];
+    if (nxt[v] >= 0) prv[nxt[v]] = prv[v];
+  };
+  for (int i = n - 1; i >= 0; i--) push(i, (int)adj[i].size());
+  vector<char> gone(n, 0);
+  vector<int> nb;
+  nb.reserve(n);
+  int lo = 0;
+  for (int s = 0; s < n; s++) {
+    while (head[lo] < 0) lo++;
+    int v = head[lo];
+    pop(v);
+    gone[v] = 1;
+    order[s] = v;
+    nb.clear();
+    for (int u : adj[v]) if (!gone[u]) nb.push_back(u);
+    tick++;
+    for (int u : nb) mark[u] = tick;
+    for (int u : nb) {
+      pop(u);
+      auto& au = adj[u];
+      size_t w = 0;
+      for (size_t p = 0; p < au.size(); p++) {
+        int x = au[p];
+        if (x != v && !gone[x] && mark[x] != tick) au[w++] = x;
+      }
+      au.resize(w);
+      for (int x : nb) if (x != u) au.push_back(x);
+      push(u, (int)au.size());
+    }
+    adj[v].clear();
+    adj[v].shrink_to_fit();
+    if (lo > 0) lo--;
+  }
+  return order;
+}
+
+bool gp_plan(int m, const vector<int>& rowptr, const vector<int>& colidx, const double* val, const vector<int>& q,
+             double diag_pref, bool keep_zeros, Plan& plan, std::string& why) {
+  plan = Plan();
+  plan.m = m;
+  plan.diagonal_preference = diag_pref;
+  // CSC of A: every structural cell (ToCscKeepingZeros) or the non-zero ones (ToCsc); a_src maps back to CSR.
+  plan.ap.assign(m + 1, 0);
+  int stored = rowptr[m];
+  for (int k = 0; k < stored; k++)
+    if (keep_zeros || val[k] != 0.0) plan.ap[colidx[k] + 1]++;
+  for (int c = 0; c < m; c++) plan.ap[c + 1] += plan.ap[c];
+  int nnz = plan.ap[m];
+  plan.ai.assign(nnz, 0);
+  plan.a_src.assign(nnz, 0);
+  vector<double> ax(nnz);
+  {
+    vector<int> fill(plan.ap.begin(), plan.ap.end() - 1);
+    for (int i = 0; i < m; i++)
+      for (int k = rowptr[i]; k < rowptr[i + 1]; k++) {
+        if (!keep_zeros && val[k] == 0.0) continue;
+        int d = fill[colidx[k]]++;
+        plan.ai[d] = i;
+        plan.a_src[d] = k;
+        ax[d] = val[k];
+      }
+  }
+  // row floors: 1e-12 x the largest |entry| of each original row; NaN / Inf refused
+  vector<double> floor_row(m, 0.0);
+  for (int e = 0; e < nnz; e++) {
+    double v = ax[e];
+    if (!std::isfinite(v)) { why = "the Jacobian holds a non-finite entry"; return false; }
+    double a = std::fabs(v);
+    if (a > floor_row[plan.ai[e]]) floor_row[plan.ai[e]] = a;
+  }
+  for (double& f : floor_row) f *= 1e-12;
+  plan.q = q;
+  plan.pinv.assign(m, -1);
+  plan.pivot_row.assign(m, -1);
+  plan.lp.assign(m + 1, 0);
+  plan.up.assign(m + 1, 0);
+  plan.reach_ptr.assign(m + 1, 0);
+  vector<double> x(m, 0.0);
+  vector<int> xi(m), stack_pos(m);
+  vector<char> seen(m, 0);
+  vector<int>& li = plan.li_orig;
+  vector<double> lx;
+  for (int k = 0; k < m; k++) {
+    plan.lp[k] = (int)li.size();
+    plan.up[k] = (int)plan.ui.size();
+    int col = q[k];
+    // reach of A(:,col) in the graph of L, by iterative depth-first search; xi[top..m) in topological order
+    int top = m;
+    for (int p = plan.ap[col]; p < plan.ap[col + 1]; p++) {
+      int start = plan.ai[p];
+      if (seen[start]) continue;
+      int head = 0;
+      xi[0] = start;
+      while (head >= 0) {
+        int j = xi[head];
+        int jcol = plan.pinv[j];
+        if (!seen[j]) { seen[j] = 1; stack_pos[head] = jcol < 0 ? 0 : plan.lp[jcol] + 1; }
+        bool done = true;
+        int end = jcol < 0 ? 0 : plan.lp[jcol + 1];
+        for (int r = stack_pos[head]; r < end; r++) {
+          int i = li[r];
+          if (seen[i]) continue;
+          stack_pos[head] = r + 1;
+          xi[++head] = i;
+          done = false;
+          break;
+        }
+        if (done) { head--; xi[--top] = j; }
+      }
+    }
+    for (int p = top; p < m; p++) seen[xi[p]] = 0;
+    // numeric column: scatter A(:,col), update in topological order
+    for (int p = top; p < m; p++) x[xi[p]] = 0.0;
+    for (int p = plan.ap[col]; p < plan.ap[col + 1]; p++) x[plan.ai[p]] = ax[p];
+    for (int p = top; p < m; p++) {
+      int j = xi[p], jcol = plan.pinv[j];
+      if (jcol < 0) continue;
+      double xj = x[j];
+      for (int r = plan.lp[jcol] + 1; r < plan.lp[jcol + 1]; r++) x[li[r]] -= lx[r] * xj;
+    }
+    plan.reach.insert(plan.reach.end(), xi.begin() + top, xi.begin() + m);
+    plan.reach_ptr[k + 1] = (int)plan.reach.size();
+    int ipiv = -1;
+    double best = -1.0;
+    for (int p = top; p < m; p++) {
+      int i = xi[p];
+      if (plan.pinv[i] < 0) {
+        double t = std::fabs(x[i]);
+        if (t > best) { best = t; ipiv = i; }
+      } else {
+        plan.ui.push_back(plan.pinv[i]);
+      }
+    }
+    if (ipiv < 0 || best <= floor_row[ipiv]) {
+      why = "singular Jacobian: no pivot above its row floor at step " + std::to_string(k);
+      return false;
+    }
+    if (diag_pref > 0.0 && !(ipiv == col || plan.pinv[col] >= 0)) {
+      for (int p = top; p < m; p++) {
+        if (xi[p] != col) continue;
+        double td = std::fabs(x[col]);
+        if (td >= diag_pref * best && td > floor_row[col]) ipiv = col;
+        break;
+      }
+    }
+    double pivot = x[ipiv];
+    plan.pivot_row[k] = ipiv;
+    plan.ui.push_back(k);
+    plan.pinv[ipiv] = k;
+    li.push_back(ipiv);
+    lx.push_back(1.0);
+    for (int p = top; p < m; p++) {
+      int i = xi[p];
+      if (plan.pinv[i] < 0) { li.push_back(i); lx.push_back(x[i] / pivot); }
+      x[i] = 0.0;
+    }
+  }
+  plan.lp[m] = (int)li.size();
+  plan.up[m] = (int)plan.ui.size();
+  plan.li.resize(li.size());
+  for (size_t p = 0; p < li.size(); p++) plan.li[p] = plan.pinv[li[p]];
+  return true;
+}
+
+void compile_plan(const Plan& plan, DevicePlanHost& d) {
+  int m = plan.m;
+  d = DevicePlanHost();
+  d.m = m;
+  d.lp = plan.lp;
+  d.up = plan.up;
+  d.li = plan.li;
+  d.ui = plan.ui;
+  d.pinv = plan.pinv;
+  d.q = plan.q;
+  d.nnz_l = plan.lp[m];
+  d.nnz_u = plan.up[m];
+  // Column k's workspace: slots [0, nU_k - 1) the off-diagonal U rows in storage (= topological) order, slot nU_k - 1
+  // the pivot, slots [nU_k, nU_k + nL_k - 1) the off-diagonal L rows in storage order.
+  vector<int> slot_of(m, -1);
+  d.col_a_ptr.assign(m + 1, 0);
+  d.pivot_slot.assign(m, 0);
+  d.upd_ptr.assign(d.nnz_u + 1, 0);
+  for (int k = 0; k < m; k++) {
+    int nu = plan.up[k + 1] - plan.up[k], nl = plan.lp[k + 1] - plan.lp[k];
+    d.max_col = std::max(d.max_col, nu + nl - 1);
+    for (int t = 0; t < nu - 1; t++) slot_of[plan.ui[plan.up[k] + t]] = t;
+    slot_of[k] = nu - 1;
+    d.pivot_slot[k] = nu - 1;
+    for (int t = 1; t < nl; t++) slot_of[plan.li[plan.lp[k] + t]] = nu - 1 + t;
+    int col = plan.q[k];
+    for (int e = plan.ap[col]; e < plan.ap[col + 1]; e++) {
+      int s = slot_of[plan.pinv[plan.ai[e]]];
+      if (s < 0) throw std::runtime_error("plan: an entry of A lies outside its column's pattern");
+      d.a_src.push_back(plan.a_src[e]);
+      d.a_slot.push_back(s);
+    }
+    d.col_a_ptr[k + 1] = (int)d.a_src.size();
+    for (int t = 0; t < nu; t++) {
+      int ue = plan.up[k] + t;
+      d.upd_ptr[ue] = (int)d.upd_slot.size();
+      if (t < nu - 1) {
+        int j = plan.ui[ue];
+        for (int r = plan.lp[j] + 1; r < plan.lp[j + 1]; r++) {
+          int s = slot_of[plan.li[r]];
+          if (s < 0) throw std::runtime_error("plan: an update lies outside its column's pattern");
+          d.upd_slot.push_back(s);
+        }
+      }
+    }
+    for (int t = 0; t < nu - 1; t++) slot_of[plan.ui[plan.up[k] + t]] = -1;
+    slot_of[k] = -1;
+    for (int t = 1; t < nl; t++) slot_of[plan.li[plan.lp[k] + t]] = -1;
+  }
+  d.upd_ptr[d.nnz_u] = (int)d.upd_slot.size();
+  d.n_upd = (int)d.upd_slot.size();
+  d.prow = plan.pivot_row;
+  // Dependency levels: column k reads L(:, j) for every off-diagonal U(j, k), so level(k) = 1 + max level(j).
+  vector<int> level(m, 0);
+  int n_levels = 0;
+  for (int k = 0; k < m; k++) {
+    int lv = 0;
+    for (int p = plan.up[k]; p < plan.up[k + 1] - 1; p++) lv = std::max(lv, level[plan.ui[p]] + 1);
+    level[k] = lv;
+    n_levels = std::max(n_levels, lv + 1);
+  }
+  vector<int> count(n_levels + 1, 0);
+  for (int k = 0; k < m; k++) count[level[k] + 1]++;
+  for (int l = 0; l < n_levels; l++) count[l + 1] += count[l];
+  d.level_end.assign(count.begin() + 1, count.end());
+  d.level_end.push_back(m + 1);  // sentinel past the last level
+  d.col_order.assign(m, 0);
+  for (int k = 0; k < m; k++) d.col_order[count[level[k]]++] = k;
+  d.n_levels = n_levels;
+  // For the exact pivot check: how many candidates (rows not yet pivotal at step k) precede the pivot in reach order.
+  d.piv_pos.assign(m, 0);
+  for (int k = 0; k < m; k++) {
+    int before = 0;
+    for (int p = plan.reach_ptr[k]; p < plan.reach_ptr[k + 1]; p++) {
+      int row = plan.reach[p];
+      if (row == plan.pivot_row[k]) break;
+      if (plan.pinv[row] > k) before++;
+    }
+    d.piv_pos[k] = before;
+  }
+}
+
+}  // namespace gs

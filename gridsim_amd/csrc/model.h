@@ -1,0 +1,118 @@
+// Host-side model and symbolic analysis of the sparse Newton-Raphson engine.
+//
+// Everything here runs once per network pattern (or once per fallback instance): the Y-bus with its per-cell
+// contribution lists, the bus specification, the equation numbering, the Jacobian's structural pattern, the
+// minimum-degree column ordering and the Gilbert-Peierls plan (pivot order, reach order, L/U patterns). The
+// per-iteration numeric work runs on the GPU (nr_kernels.hip).
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "gridsim_hip.h"
+
+namespace gs {
+
+using std::vector;
+
+struct Cx { double re, im; };
+
+// The pi-model stamp of one branch (AdmittanceStamp.cs:41-75).
+struct Stamp { Cx yff, ytt, yft, ytf; };
+Stamp branch_stamp(double r, double x, double b, double tap_ratio, double shift_deg);
+
+// A Y-bus in CSR with sorted columns and an explicit diagonal in every row (ComplexCsr, Sparse.cs:134-222),
+// plus, per stored cell, the list of stamps summed into it in insertion order (branches in model order, then the bus
+// shunt). The list lets one cell be re-summed without one branch, which is bit-identical to rebuilding the Y-bus
+// without that branch (ContingencyAnalysis.WithoutMany rebuilds Y from scratch).
+struct YBus {
+  int n = 0;
+  vector<int> rp, ci, diag;
+  vector<double> g, b;
+  vector<int> contrib_ptr;     // [nnz + 1]
+  vector<int> contrib_branch;  // branch index, or -1 for the bus shunt
+  vector<double> contrib_g, contrib_b;
+  int find(int row, int col) const;
+};
+
+// Network copied from the C ABI description (only in-service branches matter to the solve).
+struct Network {
+  int n = 0, nb = 0, ng = 0;
+  double base = 100.0;
+  vector<int> bus_type;
+  vector<double> pd, qd, gs, bs, vm, va_deg;
+  vector<int> br_from, br_to;
+  vector<double> br_r, br_x, br_b, br_tap, br_shift;
+  vector<uint8_t> br_in;
+  vector<int> gen_bus;
+  vector<double> gen_pg, gen_qg, gen_vg;
+};
+
+Network copy_network(const gs_network& src);
+// The Y-bus of `net`; `skip` (or -1) removes one branch as an outage does.
+YBus build_ybus(const Network& net, int skip);
+
+// NewtonShell.BuildBusSpec (NewtonShell.cs:433-486) for constant-power buses.
+struct BusSpec {
+  vector<int> type;                       // effective types (a PV bus without a generator reads PQ)
+  vector<double> vm, va, pspec, qspec;    // start magnitudes, start angles (rad), specified injections (pu)
+};
+BusSpec build_bus_spec(const Network& net, bool flat_start, double load_scale);
+
+// Equation numbering (NewtonShell.cs:327-336) and the Jacobian's structural pattern with the map from each Y cell's
+// four blocks (H, N, M, L) to their CSR positions, or -1 (SparseNewtonRaphsonPowerFlow.cs:296-332).
+struct JacobianPattern {
+  int m = 0;
+  vector<int> ppos, qpos;
+  vector<int> rp, ci;
+  vector<int> slots;   // [4 * nnzY]
+};
+JacobianPattern build_jacobian_pattern(const YBus& y, const vector<int>& type);
+
+// Injections and Jacobian values at a state (ComputeInjections :269-289, RefillJacobian :342-380), on the host:
+// the numbers the symbolic pass factors.
+void host_injections(const YBus& y, const double* vm, const double* va, double* p, double* q);
+void host_jacobian(const YBus& y, const JacobianPattern& jp, const double* vm, const double* va, const double* p,
+                   const double* q, double* val);
+
+// Minimum-degree ordering of a square CSR pattern (MinimumDegreeOrdering.cs:23-136).
+vector<int> minimum_degree(int n, const int* rowptr, const int* colidx);
+
+// The Gilbert-Peierls plan of one factorisation (SparseLuFactorization.Factor with keepPlan, SparseLuPlan):
+// CSC pattern of A with every structural cell kept (SparseLu.ToCscKeepingZeros), the ordering, each column's reach in
+// DFS topological order, each step's pivot row, and the L and U patterns (rows as pivot steps).
+struct Plan {
+  int m = 0;
+  double diagonal_preference = 0.0;
+  vector<int> ap, ai, a_src;     // CSC of A; a_src[e] = CSR position of CSC entry e
+  vector<int> q, pinv, pivot_row;
+  vector<int> reach_ptr, reach;
+  vector<int> lp, li, up, ui;    // li, ui: pivot steps
+  vector<int> li_orig;           // L rows as original rows
+};
+
+// Factor A (CSR values `val`) with ordering q and partial pivoting under the given diagonal preference, on the CSC
+// that keeps every structural cell (keep_zeros, the fixed-pattern solver's) or drops exact zeros (the default
+// solver's). Returns false when a pivot is at or below its row's floor (singular) or an entry is not finite.
+bool gp_plan(int m, const vector<int>& rowptr, const vector<int>& colidx, const double* val, const vector<int>& q,
+             double diagonal_preference, bool keep_zeros, Plan& plan, std::string& why);
+
+// The plan compiled into the flat arrays the refactor and solve kernels walk (see nr_kernels.hip).
+struct DevicePlanHost {
+  int m = 0, nnz_l = 0, nnz_u = 0, max_col = 0, n_upd = 0;
+  vector<int> col_a_ptr;     // [m+1] into a_src/a_slot, per pivot step
+  vector<int> a_src;         // J CSR position of each A entry of the column
+  vector<int> a_slot;        // workspace slot of that entry
+  vector<int> upd_ptr;       // [nnz_u + 1]: per U entry, range of update slots (L(:,j) off-diagonals)
+  vector<int> upd_slot;      // slot in column k of each L(i,j) row
+  vector<int> lp, up;        // L, U column pointers (L unit diagonal first, U pivot last)
+  vector<int> li, ui;        // pivot-step rows
+  vector<int> pivot_slot;    // [m] slot of the pivot (= nU_k - 1)
+  vector<int> pinv, q, prow;
+  vector<int> col_order, level_end;  // columns by dependency level (a column needs the L columns its U row set names)
+  vector<int> piv_pos;       // per step: candidates before the pivot in reach order (the exact pivot-search check)
+  int n_levels = 0;
+};
+void compile_plan(const Plan& plan, DevicePlanHost& out);
+
+}  // namespace gs

@@ -1,0 +1,413 @@
+"""Python bindings of libgridsim_hip.so and a mirror of the reference's solver seams on top of it.
+
+The classes follow the reference interfaces the library replaces, with the same names, argument meaning and error
+behaviour:
+
+* ``HipSparseLinearSolver``        - ISparseLinearSolver (src/GridSim.Core/Abstractions/ISparseLinearSolver.cs:12-16):
+  ``solve(a, b)`` returns the solution or ``None`` when the matrix is singular, raises ``ValueError`` (the
+  reference's ArgumentException) for a NaN/Inf matrix or right-hand side.
+* ``HipBatchedSparseLinearSolver`` - IBatchedSparseLinearSolver (IBatchedSparseLinearSolver.cs:20-37): one entry
+  per system, a throwing system stores its exception in ``errors[s]`` and does not abort the others.
+* ``HipNewtonPowerFlow``           - IPowerFlowSolver (IPowerFlowSolver.cs:10-12) on a device-resident network.
+* ``screen_branches_batched``      - ContingencyAnalysis.ScreenBranchesBatched (ContingencyAnalysis.Batched.cs:55).
+
+The library must be built (``python -m gridsim_amd.build``); nothing here falls back to a CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+from .network import Network
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libgridsim_hip.so")
+
+GS_OK, GS_SINGULAR, GS_NONFINITE_INPUT, GS_OUT_OF_MEMORY, GS_INVALID_ARGUMENT, GS_DEVICE_ERROR, GS_NO_DEVICE, \
+    GS_BUSY = range(8)
+CONVERGED, MAX_ITERATIONS, DIVERGED, NONFINITE = 0, 1, 2, 3
+LU_FP64, LU_FP32 = 0, 1
+STATUS_NAMES = {0: "Converged", 1: "MaxIterations", 2: "Diverged", 3: "NonFinite"}
+
+
+class GsError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"gridsim-hip status {status}: {message}")
+        self.status = status
+
+
+class GsNetwork(C.Structure):
+    _fields_ = [("n_bus", C.c_int32), ("n_branch", C.c_int32), ("n_gen", C.c_int32), ("base_mva", C.c_double),
+                ("bus_type", C.c_void_p), ("pd", C.c_void_p), ("qd", C.c_void_p), ("gs", C.c_void_p),
+                ("bs", C.c_void_p), ("vm", C.c_void_p), ("va_deg", C.c_void_p),
+                ("br_from", C.c_void_p), ("br_to", C.c_void_p), ("br_r", C.c_void_p), ("br_x", C.c_void_p),
+                ("br_b", C.c_void_p), ("br_tap", C.c_void_p), ("br_shift_deg", C.c_void_p),
+                ("br_in_service", C.c_void_p), ("gen_bus", C.c_void_p), ("gen_pg", C.c_void_p),
+                ("gen_qg", C.c_void_p), ("gen_vg", C.c_void_p)]
+
+
+class GsNrOptions(C.Structure):
+    _fields_ = [("tolerance", C.c_double), ("max_iterations", C.c_int32), ("flat_start", C.c_int32),
+                ("diverge_limit", C.c_double), ("lu_precision", C.c_int32), ("refine_steps", C.c_int32),
+                ("growth_tol", C.c_double), ("schedule", C.c_int32), ("reserved", C.c_int32)]
+
+
+class GsInstance(C.Structure):
+    _fields_ = [("outage_branch", C.c_int32), ("reserved", C.c_int32), ("load_scale", C.c_double),
+                ("vm0", C.c_void_p), ("va0", C.c_void_p)]
+
+
+RESULT_DTYPE = np.dtype([("status", np.int32), ("iterations", np.int32), ("max_mismatch", np.float64),
+                         ("symbolic_refreshes", np.int32), ("step_failures", np.int32),
+                         ("min_pivot_ratio", np.float64)])
+
+
+class GsCsr(C.Structure):
+    _fields_ = [("n", C.c_int32), ("reserved", C.c_int32), ("rowptr", C.c_void_p), ("colidx", C.c_void_p),
+                ("val", C.c_void_p)]
+
+
+class GsDeviceInfo(C.Structure):
+    _fields_ = [("name", C.c_char * 64), ("arch", C.c_char * 32), ("compute_units", C.c_int32),
+                ("fp64_ok", C.c_int32), ("hbm_bytes", C.c_size_t), ("fdlibm_ok", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+_lib = None
+
+
+def lib():
+    """The loaded library; raises when it has not been built (the engine has no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m gridsim_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        L.gs_last_error.restype = C.c_char_p
+        L.gs_version.restype = C.c_char_p
+        for name in ("gs_init", "gs_device_info_get", "gs_sparse_solve", "gs_sparse_solve_batch",
+                     "gs_network_upload", "gs_nr_solve_batch", "gs_nr_solve_plain", "gs_network_plan_sizes",
+                     "gs_network_plan_get", "gs_last_timing", "gs_device_sincos", "gs_plan_build",
+                     "gs_plan_sizes", "gs_plan_get"):
+            getattr(L, name).restype = C.c_int32
+        L.gs_host_sincos.restype = None
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class NetView:
+    """A ``gs_network`` pointing at the numpy arrays of a Network (kept alive with it)."""
+
+    def __init__(self, net: Network):
+        self.net = net
+        self.s = GsNetwork(net.n_bus, net.n_branch, net.n_gen, float(net.base_mva), _p(net.bus_type), _p(net.pd),
+                           _p(net.qd), _p(net.gs), _p(net.bs), _p(net.vm), _p(net.va_deg), _p(net.br_from),
+                           _p(net.br_to), _p(net.br_r), _p(net.br_x), _p(net.br_b), _p(net.br_tap),
+                           _p(net.br_shift_deg), _p(net.br_in_service), _p(net.gen_bus), _p(net.gen_pg),
+                           _p(net.gen_qg), _p(net.gen_vg))
+
+    @property
+    def ref(self):
+        return C.byref(self.s)
+
+
+@dataclass
+class NrOptions:
+    """PowerFlowOptions for the engine (PowerFlowOptions.cs): Q limits and the DC-start retry are off."""
+    tolerance: float = 1e-8
+    max_iterations: int = 30
+    flat_start: bool = False
+    diverge_limit: float = 10.0
+    lu_precision: int = LU_FP32
+    refine_steps: int = 1
+    growth_tol: float = 1e-3
+    schedule: int = 0
+
+    def c(self) -> GsNrOptions:
+        return GsNrOptions(self.tolerance, self.max_iterations, int(self.flat_start), self.diverge_limit,
+                           self.lu_precision, self.refine_steps, self.growth_tol, self.schedule, 0)
+
+
+def host_sincos(x):
+    """The product's FDLIBM compiled for the host (no device needed)."""
+    x = np.ascontiguousarray(x, np.float64)
+    s, c = np.empty_like(x), np.empty_like(x)
+    lib().gs_host_sincos(C.c_int64(x.size), _p(x), _p(s), _p(c))
+    return s, c
+
+
+class HostPlan:
+    """The symbolic plan the engine would upload for a network (gs_plan_build), computed on the host."""
+
+    def __init__(self, net: Network, options: NrOptions | None = None):
+        opts = (options or NrOptions()).c()
+        h = C.c_void_p()
+        st = lib().gs_plan_build(NetView(net).ref, C.byref(opts), C.byref(h))
+        if st != GS_OK:
+            raise GsError(st, "gs_plan_build")
+        self.h = h
+        m, nj, nl, nu, lv = (C.c_int32() for _ in range(5))
+        lib().gs_plan_sizes(h, C.byref(m), C.byref(nj), C.byref(nl), C.byref(nu), C.byref(lv))
+        self.m, self.nnz_j, self.nnz_l, self.nnz_u, self.n_levels = m.value, nj.value, nl.value, nu.value, lv.value
+        self.q = np.empty(self.m, np.int32); self.pivot_row = np.empty(self.m, np.int32)
+        self.lp = np.empty(self.m + 1, np.int32); self.li = np.empty(self.nnz_l, np.int32)
+        self.up = np.empty(self.m + 1, np.int32); self.ui = np.empty(self.nnz_u, np.int32)
+        self.j_rowptr = np.empty(self.m + 1, np.int32); self.j_colidx = np.empty(self.nnz_j, np.int32)
+        lib().gs_plan_get(h, _p(self.q), _p(self.pivot_row), _p(self.lp), _p(self.li), _p(self.up), _p(self.ui),
+                          _p(self.j_rowptr), _p(self.j_colidx))
+
+    def __del__(self):
+        try:
+            lib().gs_plan_free(self.h)
+        except Exception:
+            pass
+
+
+class Context:
+    """A device context (gs_init). One per thread or shared: calls are serialised inside the library."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0):
+        h = C.c_void_p()
+        st = lib().gs_init(C.c_int32(device), C.c_size_t(hbm_budget), C.byref(h))
+        if st != GS_OK:
+            raise GsError(st, "gs_init: no usable GPU" if st == GS_NO_DEVICE else "gs_init")
+        self.h = h
+
+    def error(self) -> str:
+        return lib().gs_last_error(self.h).decode()
+
+    def check(self, st, what):
+        if st != GS_OK:
+            raise GsError(st, f"{what}: {self.error()}")
+
+    def device_info(self) -> dict:
+        info = GsDeviceInfo()
+        self.check(lib().gs_device_info_get(self.h, C.byref(info)), "gs_device_info_get")
+        return {"name": info.name.decode(), "arch": info.arch.decode(), "compute_units": info.compute_units,
+                "fp64_ok": bool(info.fp64_ok), "fdlibm_ok": bool(info.fdlibm_ok), "hbm_bytes": info.hbm_bytes}
+
+    def device_sincos(self, x):
+        x = np.ascontiguousarray(x, np.float64)
+        s, c = np.empty_like(x), np.empty_like(x)
+        self.check(lib().gs_device_sincos(self.h, C.c_int64(x.size), _p(x), _p(s), _p(c)), "gs_device_sincos")
+        return s, c
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().gs_shutdown(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------- the sparse linear-solver seams
+
+class HipSparseLinearSolver:
+    """ISparseLinearSolver on the GPU: CpuSparseLuSolver's contract (CpuSparseLuSolver.cs:30-48)."""
+
+    name = "sparse-lu (hip, gfx950)"
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+
+    def solve(self, rowptr, colidx, val, b):
+        rowptr = np.ascontiguousarray(rowptr, np.int32)
+        colidx = np.ascontiguousarray(colidx, np.int32)
+        val = np.ascontiguousarray(val, np.float64)
+        b = np.ascontiguousarray(b, np.float64)
+        n = len(rowptr) - 1
+        if len(b) != n:
+            raise ValueError(f"The right-hand side has {len(b)} entries; the system has {n}.")
+        x = np.empty(n)
+        st = lib().gs_sparse_solve(self.ctx.h, C.c_int32(n), _p(rowptr), _p(colidx), _p(val), _p(b), _p(x))
+        if st == GS_SINGULAR:
+            return None
+        if st == GS_NONFINITE_INPUT:
+            raise ValueError(self.ctx.error())
+        self.ctx.check(st, "gs_sparse_solve")
+        return x
+
+
+class HipBatchedSparseLinearSolver:
+    """IBatchedSparseLinearSolver on the GPU (IBatchedSparseLinearSolver.cs:20-37)."""
+
+    name = "sparse-lu batched (hip, gfx950)"
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+
+    def solve_batch(self, matrices, rhs, errors):
+        if len(matrices) != len(rhs) or len(errors) != len(matrices):
+            raise ValueError("matrices, right-hand sides and errors must have one entry per system")
+        count = len(matrices)
+        keep = []
+        systems = (GsCsr * max(count, 1))()
+        bptr = (C.c_void_p * max(count, 1))()
+        xptr = (C.c_void_p * max(count, 1))()
+        xs = []
+        for s, ((rp, ci, v), b) in enumerate(zip(matrices, rhs)):
+            rp = np.ascontiguousarray(rp, np.int32); ci = np.ascontiguousarray(ci, np.int32)
+            v = np.ascontiguousarray(v, np.float64); b = np.ascontiguousarray(b, np.float64)
+            if len(b) != len(rp) - 1:
+                raise ValueError(f"system {s}: right-hand side length {len(b)} != {len(rp) - 1}")
+            x = np.empty(len(rp) - 1)
+            keep += [rp, ci, v, b]
+            xs.append(x)
+            systems[s] = GsCsr(len(rp) - 1, 0, _p(rp), _p(ci), _p(v))
+            bptr[s] = b.ctypes.data
+            xptr[s] = x.ctypes.data
+        status = np.zeros(max(count, 1), np.int32)
+        self.ctx.check(lib().gs_sparse_solve_batch(self.ctx.h, C.c_int32(count), systems, bptr, xptr, _p(status)),
+                       "gs_sparse_solve_batch")
+        out = []
+        for s in range(count):
+            if status[s] == GS_OK:
+                out.append(xs[s])
+            else:
+                if status[s] == GS_NONFINITE_INPUT:
+                    errors[s] = ValueError(f"system {s}: non-finite input")
+                elif status[s] != GS_SINGULAR:
+                    errors[s] = GsError(int(status[s]), f"system {s}")
+                out.append(None)
+        return out
+
+
+# ---------------------------------------------------------------- whole Newton-Raphson
+
+@dataclass
+class BatchResult:
+    results: np.ndarray            # RESULT_DTYPE per instance
+    vm: np.ndarray | None
+    va: np.ndarray | None          # radians
+    kernel_ms: float
+    total_ms: float
+    launches: int
+
+    @property
+    def status(self):
+        return self.results["status"]
+
+
+class DeviceNetwork:
+    """A network uploaded to HBM with its symbolic plan (gs_network_upload)."""
+
+    def __init__(self, ctx: Context, net: Network, options: NrOptions | None = None):
+        self.ctx = ctx
+        self.net = net
+        self.options = options or NrOptions()
+        self._view = NetView(net)
+        h = C.c_void_p()
+        ctx.check(lib().gs_network_upload(ctx.h, self._view.ref, C.byref(self.options.c()), C.byref(h)),
+                  "gs_network_upload")
+        self.h = h
+
+    def plan(self) -> dict:
+        m, nj, nl, nu = (C.c_int32() for _ in range(4))
+        lib().gs_network_plan_sizes(self.h, C.byref(m), C.byref(nj), C.byref(nl), C.byref(nu))
+        out = {k: np.empty(s, np.int32) for k, s in (("q", m.value), ("pivot_row", m.value), ("lp", m.value + 1),
+                                                       ("li", nl.value), ("up", m.value + 1), ("ui", nu.value),
+                                                       ("j_rowptr", m.value + 1), ("j_colidx", nj.value))}
+        lib().gs_network_plan_get(self.h, *(_p(out[k]) for k in ("q", "pivot_row", "lp", "li", "up", "ui",
+                                                                 "j_rowptr", "j_colidx")))
+        return out
+
+    def _instances(self, outages=None, load_scale=None, count=None, starts=None):
+        count = count if count is not None else (len(outages) if outages is not None else len(load_scale))
+        arr = (GsInstance * max(count, 1))()
+        keep = []
+        for i in range(count):
+            o = int(outages[i]) if outages is not None else -1
+            s = float(load_scale[i]) if load_scale is not None else 1.0
+            vm0 = va0 = None
+            if starts is not None:
+                vm0 = np.ascontiguousarray(starts[0][i], np.float64)
+                va0 = np.ascontiguousarray(starts[1][i], np.float64)
+                keep += [vm0, va0]
+            arr[i] = GsInstance(o, 0, s, _p(vm0), _p(va0))
+        return arr, count, keep
+
+    def solve(self, outages=None, load_scale=None, options: NrOptions | None = None, keep_state=True,
+              plain=False, starts=None, count=None) -> BatchResult:
+        opts = options or self.options
+        arr, count, keep = self._instances(outages, load_scale, count, starts)
+        res = np.zeros(count, RESULT_DTYPE)
+        n = self.net.n_bus
+        vm = np.empty((count, n)) if keep_state else None
+        va = np.empty((count, n)) if keep_state else None
+        fn = lib().gs_nr_solve_plain if plain else lib().gs_nr_solve_batch
+        self.ctx.check(fn(self.h, C.c_int32(count), arr, C.byref(opts.c()), _p(res), _p(vm), _p(va)),
+                       "gs_nr_solve_plain" if plain else "gs_nr_solve_batch")
+        tot, ker, nl = C.c_double(), C.c_double(), C.c_int32()
+        lib().gs_last_timing(self.h, C.byref(tot), C.byref(ker), C.byref(nl))
+        return BatchResult(res, vm, va, ker.value, tot.value, nl.value)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().gs_network_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class PowerFlowResult:
+    """The fields of the reference's PowerFlowResult (PowerFlowResult.cs:5-21) the engine reports."""
+    status: int
+    converged: bool
+    iterations: int
+    max_mismatch: float
+    vm: np.ndarray
+    va_deg: np.ndarray
+    symbolic_refreshes: int
+
+
+class HipNewtonPowerFlow:
+    """IPowerFlowSolver on the GPU: SparseNewtonRaphsonPowerFlow.Solve with Q limits and the DC retry off."""
+
+    name = "newton-raphson (hip, gfx950)"
+
+    def __init__(self, ctx: Context, options: NrOptions | None = None):
+        self.ctx = ctx
+        self.options = options or NrOptions()
+
+    def solve(self, net: Network, options: NrOptions | None = None) -> PowerFlowResult:
+        opts = options or self.options
+        dn = DeviceNetwork(self.ctx, net, opts)
+        try:
+            r = dn.solve(count=1, options=opts)
+        finally:
+            dn.close()
+        rec = r.results[0]
+        return PowerFlowResult(int(rec["status"]), int(rec["status"]) == CONVERGED, int(rec["iterations"]),
+                               float(rec["max_mismatch"]), r.vm[0], np.degrees(r.va[0]),
+                               int(rec["symbolic_refreshes"]))
+
+
+def screen_branches_batched(ctx: Context, net: Network, branch_indices=None, options: NrOptions | None = None,
+                            keep_state=False) -> BatchResult:
+    """N-1 screen of every in-service branch (or the given ones) in one batched launch
+    (ContingencyAnalysis.ScreenBranchesBatched, ContingencyAnalysis.Batched.cs:55-136)."""
+    idx = np.nonzero(net.br_in_service)[0] if branch_indices is None else np.asarray(branch_indices)
+    for i in idx:
+        if not net.br_in_service[int(i)]:
+            raise ValueError(f"branch {int(i)} is out of service: removing it is not an outage.")
+    dn = DeviceNetwork(ctx, net, options)
+    try:
+        return dn.solve(outages=idx, options=options, keep_state=keep_state)
+    finally:
+        dn.close()

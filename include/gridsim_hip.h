@@ -1,0 +1,188 @@
+/* gridsim_hip.h - C ABI of the MI355X sparse Newton-Raphson load-flow engine (libgridsim_hip.so).
+ *
+ * Flat C: plain pointers and sizes, caller-owned host arrays, opaque handles for device-resident state. Status codes
+ * follow the reference's P/Invoke convention (int, 0 = success; RocmInterop.cs:37-96) and its solver seams:
+ *
+ *   ISparseLinearSolver.Solve(CsrMatrix, double[]) -> double[]?          src/GridSim.Core/Abstractions/ISparseLinearSolver.cs:12-16
+ *       -> gs_sparse_solve
+ *   IBatchedSparseLinearSolver.SolveBatch(matrices, rhs, errors)          src/GridSim.Core/Abstractions/IBatchedSparseLinearSolver.cs:20-37
+ *       -> gs_sparse_solve_batch
+ *   IPowerFlowSolver.Solve(GridModel, PowerFlowOptions?)                  src/GridSim.Core/Abstractions/IPowerFlowSolver.cs:10-12
+ *       -> gs_network_upload + gs_nr_solve_batch (one or many instances)
+ *   ContingencyAnalysis.ScreenBranchesBatched / ScreenBranchesParallel    src/GridSim.Core/Analysis/ContingencyAnalysis.Batched.cs:55-136,
+ *       -> gs_nr_solve_batch with one outage per instance                 ContingencyAnalysis.cs:323-347
+ *   RpcCommand.Solve (gridsim-rpc/1 "solve", sequential)                  src/GridSim.Cli/RpcCommand.cs:668-700
+ *       -> gs_server_* (continuous batching in front of the same contract)
+ *
+ * Threading: a gs_ctx serialises its own calls with an internal lock (the reference serialises device access with a
+ * process-global lock, RocmContext.cs:21). Handles belong to the context that made them.
+ */
+#ifndef GRIDSIM_HIP_H
+#define GRIDSIM_HIP_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GS_API __attribute__((visibility("default")))
+
+typedef int32_t gs_status;
+enum {
+  GS_OK = 0,
+  GS_SINGULAR = 1,          /* the sparse LU refused the matrix: the seam's null (caller's CPU fallback) */
+  GS_NONFINITE_INPUT = 2,   /* NaN / Inf in the matrix or right-hand side: the seam's ArgumentException */
+  GS_OUT_OF_MEMORY = 3,     /* device allocation failed: the only status the reference falls back on */
+  GS_INVALID_ARGUMENT = 4,  /* malformed call (bad sizes, pattern, handle) */
+  GS_DEVICE_ERROR = 5,      /* any other HIP failure */
+  GS_NO_DEVICE = 6,         /* no usable GPU */
+  GS_BUSY = 7               /* the server's admission queue is full */
+};
+
+/* Bus types in the reference enum order (src/GridSim.Core/Model/Model.cs:4-12). */
+enum { GS_BUS_SLACK = 0, GS_BUS_PV = 1, GS_BUS_PQ = 2 };
+
+/* PowerFlowStatus (src/GridSim.Core/PowerFlow/PowerFlowResult.cs; precedence NewtonShell.cs:671-674). */
+enum { GS_PF_CONVERGED = 0, GS_PF_MAX_ITERATIONS = 1, GS_PF_DIVERGED = 2, GS_PF_NONFINITE = 3 };
+
+/* Arithmetic of the Jacobian factorisation and triangular solves. The mismatch, convergence test, voltage update and
+ * refinement residual are always FP64. */
+enum { GS_LU_FP64 = 0, GS_LU_FP32 = 1 };
+
+/* Work schedule of the batched kernel (GS_SCHED_AUTO lets the autotuner choose). */
+enum { GS_SCHED_AUTO = 0, GS_SCHED_WAVE = 1, GS_SCHED_WAVE_LEVELS = 2 };
+
+/* A network as the reference's GridModel holds it (Model.cs:20-35 Bus, :91-115 Generator, :253-276 Branch), with
+ * buses identified by position. Loads and shunts in MW / MVAr, angles in degrees, impedances in pu. */
+typedef struct gs_network {
+  int32_t n_bus, n_branch, n_gen;
+  double base_mva;
+  const int32_t* bus_type;
+  const double *pd, *qd, *gs, *bs, *vm, *va_deg;
+  const int32_t *br_from, *br_to;
+  const double *br_r, *br_x, *br_b, *br_tap, *br_shift_deg;
+  const uint8_t* br_in_service;
+  const int32_t* gen_bus;
+  const double *gen_pg, *gen_qg, *gen_vg;
+} gs_network;
+
+/* PowerFlowOptions (src/GridSim.Core/PowerFlow/PowerFlowOptions.cs:6-170) for the engine. Q limits and the
+ * DC-start retry are off in every mode this engine runs (EnforceQLimits = false, RetryFromDcStart = false). */
+typedef struct gs_nr_options {
+  double tolerance;         /* PowerFlowOptions.Tolerance, pu on ||mismatch||_inf (1e-8) */
+  int32_t max_iterations;   /* PowerFlowOptions.MaxIterations (30) */
+  int32_t flat_start;       /* PowerFlowOptions.FlatStart */
+  double diverge_limit;     /* PowerFlowOptions.DivergeLimitPu (10) */
+  int32_t lu_precision;     /* GS_LU_FP64 or GS_LU_FP32 */
+  int32_t refine_steps;     /* FP64 iterative-refinement steps after each FP32 solve (0..4) */
+  double growth_tol;        /* a static pivot is kept while |pivot| >= growth_tol * max |candidate| */
+  int32_t schedule;         /* GS_SCHED_* */
+  int32_t reserved;
+} gs_nr_options;
+
+/* One solve of a batch: the uploaded network with one branch out of service (or none) and every load and generator
+ * P scaled by load_scale. vm0 / va0 (radians), when non-null, replace the start point BuildBusSpec would give. */
+typedef struct gs_instance {
+  int32_t outage_branch;    /* index into the network's branches, or -1 */
+  int32_t reserved;
+  double load_scale;        /* 1.0 = as uploaded */
+  const double* vm0;
+  const double* va0;
+} gs_instance;
+
+typedef struct gs_nr_result {
+  int32_t status;               /* GS_PF_* */
+  int32_t iterations;           /* Newton steps taken */
+  double max_mismatch;          /* ||f||_inf at the final state, pu */
+  int32_t symbolic_refreshes;   /* fresh CPU symbolic passes the pivot-growth check asked for */
+  int32_t step_failures;        /* steps whose factorisation was refused (singular) */
+  double min_pivot_ratio;       /* smallest |pivot| / max |candidate| seen (1 = the pivot was the largest) */
+} gs_nr_result;
+
+typedef struct gs_device_info {
+  char name[64];
+  char arch[32];
+  int32_t compute_units;
+  int32_t fp64_ok;              /* (1 + 2^-40) - 1 computed on the device equals 2^-40 (GpuContext.cs:227-256) */
+  size_t hbm_bytes;
+  int32_t fdlibm_ok;            /* a device sincos of pi/7 matches the host's FDLIBM bits */
+  int32_t reserved;
+} gs_device_info;
+
+typedef struct gs_ctx gs_ctx;
+typedef struct gs_net gs_net;
+
+/* ---- context ------------------------------------------------------------------------------------------------ */
+GS_API gs_status gs_init(int32_t device, size_t hbm_budget_bytes, gs_ctx** out);
+GS_API void gs_shutdown(gs_ctx* ctx);
+GS_API const char* gs_last_error(const gs_ctx* ctx);
+GS_API gs_status gs_device_info_get(gs_ctx* ctx, gs_device_info* out);
+GS_API const char* gs_version(void);
+
+/* ---- ISparseLinearSolver / IBatchedSparseLinearSolver --------------------------------------------------------
+ * Solve A x = b for a square CSR matrix (columns sorted per row). Same contract as CpuSparseLuSolver.Solve
+ * (src/GridSim.Core/PowerFlow/CpuSparseLuSolver.cs:30-48): minimum-degree ordering cached per pattern, Gilbert-
+ * Peierls LU with partial pivoting (explicit zeros dropped), GS_SINGULAR for the seam's null, GS_NONFINITE_INPUT
+ * for a NaN/Inf matrix or right-hand side. Arguments are read only; x (length n) is written on GS_OK only. */
+GS_API gs_status gs_sparse_solve(gs_ctx* ctx, int32_t n, const int32_t* rowptr, const int32_t* colidx,
+                                 const double* val, const double* b, double* x);
+
+typedef struct gs_csr {
+  int32_t n;
+  int32_t reserved;
+  const int32_t* rowptr;
+  const int32_t* colidx;
+  const double* val;
+} gs_csr;
+
+/* One status per system in per_system (a failing system never aborts the others); returns GS_OK unless the call
+ * itself is malformed (GS_INVALID_ARGUMENT) or the device failed. */
+GS_API gs_status gs_sparse_solve_batch(gs_ctx* ctx, int32_t count, const gs_csr* systems, const double* const* b,
+                                       double* const* x, gs_status* per_system);
+
+/* ---- whole Newton-Raphson on the device --------------------------------------------------------------------- */
+/* Upload a network: Y-bus, branch stamps, bus specification, Jacobian pattern and the symbolic plan (minimum-degree
+ * order and Gilbert-Peierls pivot sequence of the Jacobian at the network's start point under `opts`), resident in
+ * HBM until gs_network_free. */
+GS_API gs_status gs_network_upload(gs_ctx* ctx, const gs_network* net, const gs_nr_options* opts, gs_net** out);
+GS_API void gs_network_free(gs_net* net);
+
+/* Solve `count` instances of the network in one persistent launch per Newton iteration group. results[count];
+ * vm_out / va_out (radians), when non-null, receive count * n_bus values. */
+GS_API gs_status gs_nr_solve_batch(gs_net* net, int32_t count, const gs_instance* inst, const gs_nr_options* opts,
+                                   gs_nr_result* results, double* vm_out, double* va_out);
+
+/* The plain path: one instance per launch, FP64 LU, the host driving each Newton iteration (the baseline the batched
+ * engine is measured against). Same arguments as gs_nr_solve_batch; opts->lu_precision is ignored. */
+GS_API gs_status gs_nr_solve_plain(gs_net* net, int32_t count, const gs_instance* inst, const gs_nr_options* opts,
+                                   gs_nr_result* results, double* vm_out, double* va_out);
+
+/* The symbolic plan of an uploaded network, for parity checks: sizes first (any pointer may be null). */
+GS_API gs_status gs_network_plan_sizes(const gs_net* net, int32_t* m, int32_t* nnz_j, int32_t* nnz_l, int32_t* nnz_u);
+GS_API gs_status gs_network_plan_get(const gs_net* net, int32_t* q, int32_t* pivot_row, int32_t* lp, int32_t* li,
+                                     int32_t* up, int32_t* ui, int32_t* j_rowptr, int32_t* j_colidx);
+
+/* The same symbolic pass on the host alone (no device): the plan gs_network_upload would upload. For callers that
+ * prepare plans on CPU threads, and for checking the ordering and pivot sequence without a GPU. */
+typedef struct gs_plan gs_plan;
+GS_API gs_status gs_plan_build(const gs_network* net, const gs_nr_options* opts, gs_plan** out);
+GS_API gs_status gs_plan_sizes(const gs_plan* plan, int32_t* m, int32_t* nnz_j, int32_t* nnz_l, int32_t* nnz_u,
+                               int32_t* n_levels);
+GS_API gs_status gs_plan_get(const gs_plan* plan, int32_t* q, int32_t* pivot_row, int32_t* lp, int32_t* li,
+                             int32_t* up, int32_t* ui, int32_t* j_rowptr, int32_t* j_colidx);
+GS_API void gs_plan_free(gs_plan* plan);
+
+/* Kernel timing of the last gs_nr_solve_batch on this network (milliseconds, HIP events on its stream). */
+GS_API gs_status gs_last_timing(const gs_net* net, double* total_ms, double* kernel_ms, int32_t* launches);
+
+/* Device FDLIBM sincos over n host values (for the bit-parity test against the StrictMath vectors). */
+GS_API gs_status gs_device_sincos(gs_ctx* ctx, int64_t n, const double* x, double* s, double* c);
+
+/* Host FDLIBM sincos (the same source compiled for the host). */
+GS_API void gs_host_sincos(int64_t n, const double* x, double* s, double* c);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,106 @@
+"""CPU-side checks of libgridsim_hip.so: the C ABI is complete, the host half of the library (FDLIBM, the symbolic
+plan) matches the oracle bit for bit. No GPU needed."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT, matpower_case
+import strictmath
+
+
+@pytest.fixture(scope="module")
+def eng():
+    from gridsim_amd import engine
+    engine.lib()
+    return engine
+
+
+def _declared_symbols():
+    text = open(os.path.join(ROOT, "include", "gridsim_hip.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return re.findall(r"GS_API\s+[\w\s\*]+?\b(gs_\w+)\s*\(", text)
+
+
+def test_library_exports_every_declared_symbol(eng):
+    names = _declared_symbols()
+    assert len(names) >= 20
+    lib = C.CDLL(eng.LIB_PATH)
+    for name in names:
+        assert hasattr(lib, name), name
+
+
+def test_struct_sizes_match_the_header(eng):
+    """The ctypes mirrors of the C structs have the C layout (the P/Invoke shape test of RocmInteropShapeTests)."""
+    assert C.sizeof(eng.GsNrOptions) == 48
+    assert C.sizeof(eng.GsInstance) == 32
+    assert eng.RESULT_DTYPE.itemsize == 32
+    assert C.sizeof(eng.GsCsr) == 32
+    assert C.sizeof(eng.GsNetwork) == 4 * 3 + 4 + 8 + 19 * 8
+
+
+def test_no_device_status_without_a_gpu(eng):
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU is present")
+    h = C.c_void_p()
+    assert eng.lib().gs_init(0, 0, C.byref(h)) == eng.GS_NO_DEVICE
+
+
+def test_host_fdlibm_matches_strictmath(eng):
+    vec, dig = strictmath.load(os.path.join(GOLDEN, "strictmath_sincos.txt"))
+    xs = np.array([strictmath.from_bits(v[1]) for v in vec])
+    s, c = eng.host_sincos(xs)
+    got = np.where([v[0] == "sin" for v in vec], s, c)
+    want = np.array([strictmath.from_bits(v[2]) for v in vec])
+    assert np.array_equal(got.view(np.uint64)[~np.isnan(want)], want.view(np.uint64)[~np.isnan(want)])
+    assert np.all(np.isnan(got[np.isnan(want)]))
+    for fn, cls, seed, count, d in dig:
+        x = strictmath.draw(cls, seed, count)
+        s, c = eng.host_sincos(x)
+        assert strictmath.digest(s if fn == "sin" else c) == d, (fn, cls)
+
+
+def _oracle_plan(oracle, net, flat):
+    t, vm, va, _, _ = oracle.bus_spec(net, flat_start=flat)
+    rp, ci, val = oracle.jacobian(net, vm, va)
+    q = oracle.md_order(rp, ci)
+    lu = oracle.lu_factor(rp, ci, val, q, keep_zeros=True, diagonal_preference=0.1)
+    return rp, ci, lu
+
+
+@pytest.mark.parametrize("case,flat", [("case9", True), ("case14", False), ("case118", True), ("case300", True),
+                                       ("synthetic-118", True), ("synthetic-2000", True)])
+def test_product_plan_equals_oracle_plan(eng, oracle, case, flat):
+    """The engine's symbolic pass (minimum degree + Gilbert-Peierls pivot sequence, zeros kept, diagonal preference
+    0.1) reproduces the oracle's integer arrays exactly."""
+    from gridsim_amd import gridgen
+    if case == "synthetic-118":
+        net = gridgen.config1_grid()
+    elif case == "synthetic-2000":
+        net = gridgen.config2_grid()
+    else:
+        net = matpower_case(case)
+    plan = eng.HostPlan(net, eng.NrOptions(flat_start=flat))
+    rp, ci, lu = _oracle_plan(oracle, net, flat)
+    np.testing.assert_array_equal(plan.j_rowptr, rp)
+    np.testing.assert_array_equal(plan.j_colidx, ci)
+    np.testing.assert_array_equal(plan.q, lu.q)
+    np.testing.assert_array_equal(plan.pivot_row, lu.pivot_row)
+    np.testing.assert_array_equal(plan.lp, lu.lp)
+    np.testing.assert_array_equal(plan.li, lu.li)
+    np.testing.assert_array_equal(plan.up, lu.up)
+    np.testing.assert_array_equal(plan.ui, lu.ui)
+
+
+def test_plan_of_warm_started_outage_grid(eng, oracle):
+    """The N-1 grid's plan is taken at the base solution (warm start), as the screening run uses it."""
+    from gridsim_amd import gridgen
+    net = gridgen.config1_grid()
+    rec, vm, va = oracle.solve(net, oracle.options(flat_start=True))
+    warm = net.with_start(vm, np.degrees(va))
+    plan = eng.HostPlan(warm, eng.NrOptions(flat_start=False))
+    rp, ci, lu = _oracle_plan(oracle, warm, False)
+    np.testing.assert_array_equal(plan.pivot_row, lu.pivot_row)
+    np.testing.assert_array_equal(plan.li, lu.li)
