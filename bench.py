@@ -1,0 +1,168 @@
+"""Converged N-1 Newton-Raphson solves per second on the 2,000-bus synthetic grid, one MI355X.
+
+Workload: the 2,000-bus / 2,700-branch synthetic grid (1 slack, 300 PV), base case solved from flat start, then every
+non-bridge single-branch outage solved from the base-case voltages in one batched launch: max 10 Newton iterations,
+tolerance 1e-8 pu, FP32 LU with FP64 mismatch, refinement and update. One step = one full N-1 sweep.
+
+Printed as one JSON line: throughput (converged solves/s), the sweep latency, the plain FP64 one-instance-per-launch
+path on a sample (the baseline the batched engine is measured against), the roofline of the batched kernel, and the
+CPU oracle (a C++ restatement of the reference's CPU sparse NR) timed on this host's cores on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(plan: dict, n_bus: int, nnz_y: int, iterations: np.ndarray, refine: int, lu_bytes: int) -> int:
+    """Bytes one batched launch must move at minimum: per Newton step and instance, the mismatch (Y values and
+    pattern, state), the Jacobian fill, the LU refactor (A values in, L/U indices and values), the triangular solves
+    and refinement residuals, and the update; plus the final mismatch evaluation of every instance."""
+    m = len(plan["q"])
+    nj = len(plan["j_colidx"])
+    nlu = len(plan["li"]) + len(plan["ui"])
+    mis = 20 * nnz_y + 44 * n_bus
+    jac = 16 * nnz_y + 8 * nj
+    lu = 8 * nj + (4 + lu_bytes) * nlu
+    solve = (4 + lu_bytes) * nlu + 12 * m
+    refine_b = 12 * nj + 24 * m + solve
+    upd = 8 * m + 32 * n_bus
+    per_step = mis + jac + lu + solve + refine * refine_b + upd
+    steps = int(iterations.sum())
+    return per_step * steps + mis * len(iterations)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--plain-sample", type=int, default=48)
+    ap.add_argument("--cpu-sample", type=int, default=800)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one MI355X: this build has no multi-GPU path (run with --gpus 1).", file=sys.stderr)
+        sys.exit(2)
+
+    from gridsim_amd import engine, gridgen
+
+    t0 = time.time()
+    net = gridgen.config2_grid()
+    ctx = engine.Context(0)
+    info = ctx.device_info()
+    # base case from flat start on the device (FP64), then the warm-start grid the outages run from
+    base = engine.DeviceNetwork(ctx, net, engine.NrOptions(flat_start=True, max_iterations=60,
+                                                           lu_precision=engine.LU_FP64))
+    b = base.solve(count=1)
+    assert b.status[0] == engine.CONVERGED, "base case did not converge"
+    base.close()
+    warm = net.with_start(b.vm[0], np.degrees(b.va[0]))
+    outages = gridgen.config2_outages(warm)
+    opts = engine.NrOptions(max_iterations=10, tolerance=1e-8, lu_precision=engine.LU_FP32, refine_steps=1)
+    dn = engine.DeviceNetwork(ctx, warm, opts)
+    plan = dn.plan()
+    print(f"# setup {time.time() - t0:.1f}s: {net.n_bus} buses, {net.n_branch} branches, {len(outages)} outages, "
+          f"m={len(plan['q'])}, nnz(J)={len(plan['j_colidx'])}, nnz(L+U)={len(plan['li']) + len(plan['ui'])}",
+          file=sys.stderr, flush=True)
+
+    for w in range(args.warmup):
+        r = dn.solve(outages=outages, keep_state=False)
+    sweep_ms, kernel_ms = [], []
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        ts = time.perf_counter()
+        r = dn.solve(outages=outages, keep_state=False)
+        sweep_ms.append(1000 * (time.perf_counter() - ts))
+        kernel_ms.append(r.kernel_ms)
+        print(f"# step {k}: sweep {sweep_ms[-1]:.2f} ms, kernel {r.kernel_ms:.2f} ms", file=sys.stderr, flush=True)
+    elapsed = time.perf_counter() - t_start
+    res = r.results
+    converged = int((res["status"] == engine.CONVERGED).sum())
+    ms_per_step = 1000 * elapsed / args.steps
+    value = converged * args.steps / elapsed
+
+    # roofline of the batched kernel: algorithmic bytes of one launch / its average duration (HIP events)
+    nnz_y = net.n_bus + 2 * net.n_branch
+    alg = algorithmic_bytes(plan, net.n_bus, nnz_y, res["iterations"], opts.refine_steps, 4)
+    kavg = float(np.mean(kernel_ms))
+    achieved = alg / (kavg / 1000) / 1e9
+
+    # the plain FP64 one-instance-per-launch path on a sample of the same outages
+    sample = outages[:: max(1, len(outages) // args.plain_sample)][: args.plain_sample]
+    tp = time.perf_counter()
+    pr = dn.solve(outages=sample, plain=True, keep_state=False,
+                  options=engine.NrOptions(max_iterations=10, lu_precision=engine.LU_FP64))
+    plain_s = time.perf_counter() - tp
+    plain_conv = int((pr.status == engine.CONVERGED).sum())
+    plain_rate = plain_conv / plain_s
+    plain_lat = plain_s * 1000 / len(sample)
+
+    cpu = None
+    verdicts_match = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        threads = min(os.cpu_count() or 1, 16)
+        cs = outages[:: max(1, len(outages) // args.cpu_sample)][: args.cpu_sample]
+        tc = time.perf_counter()
+        cres, _, _ = O.screen(warm, cs, O.options(max_iterations=10), threads=threads, keep_state=False)
+        csec = time.perf_counter() - tc
+        cpu = {"value": round(int((cres["status"] == 0).sum()) / csec, 1), "unit": "converged solves/s",
+               "cores": threads, "kind": "port",
+               "sample": f"{len(cs)} of the {len(outages)} outages, {threads} threads, one solver per thread "
+                         f"(C++ restatement of the reference's CPU sparse NR; per-solve p50 "
+                         f"{1000 * np.median(cres['solve_seconds']):.1f} ms single-thread)"}
+        idx = np.searchsorted(outages, cs)
+        verdicts_match = bool(np.array_equal(res["status"][idx], cres["status"]))
+
+    line = {
+        "metric": "N-1 NR solves/s at 2k buses (1e-8 pu) + p50/p99 solve latency, 1 MI355X",
+        "value": round(value, 1),
+        "unit": "converged solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-lu+f64",
+        "data": "synthetic",
+        "config": {"workload": "batched N-1 screen, 2000-bus synthetic grid (1 slack / 300 PV), every non-bridge "
+                               "single-branch outage from the base-case voltages, max 10 iterations, 1e-8 pu",
+                   "n_bus": net.n_bus, "n_branch": net.n_branch, "outages": int(len(outages)),
+                   "seed": int(net.seed), "lu": "fp32 + 1 fp64 refinement step", "device": info["name"]},
+        "latency_ms": {"p50": round(float(np.percentile(sweep_ms, 50)), 3),
+                       "p99": round(float(np.percentile(sweep_ms, 99)), 3),
+                       "note": "every solve of a sweep completes when its batched launch does: per-solve latency "
+                               "is the sweep time"},
+        "verdicts": {"converged": converged, "max_iterations": int((res["status"] == 1).sum()),
+                     "diverged": int((res["status"] == 2).sum()), "nonfinite": int((res["status"] == 3).sum()),
+                     "symbolic_refreshes": int(res["symbolic_refreshes"].sum()),
+                     "match_cpu_oracle_on_sample": verdicts_match},
+        "plain_fp64_one_instance_per_launch": {"value": round(plain_rate, 2), "unit": "converged solves/s",
+                                               "sample": int(len(sample)), "latency_ms_per_solve": round(plain_lat, 2),
+                                               "speedup_of_batched": round(value / plain_rate, 1)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "kernel": "nr_batch_kernel<float,true>", "kernel_ms": round(kavg, 3),
+                     "algorithmic_bytes": int(alg)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+    dn.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()
