@@ -95,6 +95,8 @@ struct DevicePlan {
   DevMem y_rp, y_ci, y_diag, y_g, y_b, slots, ppos, qpos, j_rp, j_ci, vm0, va0;
   DevMem col_order, level_end, col_a_ptr, a_src, a_slot, upd_ptr, upd_slot, lp, li, up, ui, pinv, q, prow;
   DevMem piv_pos;
+  DevMem ch_track_ptr, ch_row_ptr, ch_a_ptr, ch_fence, tr_col, ch_a_ws, ch_a_src, row_rec, row_lsrc;
+  DevMem fw_order, fw_level_end, lr_ptr, lr_col, lr_idx, bw_order, bw_level_end, ur_ptr, ur_col, ur_idx;
   DevMem d_struct;
   DevPlan h{};
   DevicePlanHost dph;
@@ -139,6 +141,27 @@ void upload_plan(DevicePlan& dp, const YBus* y, const JacobianPattern& jp, const
   h.upd_ptr = dp.upd_ptr.as<int>(); h.upd_slot = dp.upd_slot.as<int>();
   h.lp = dp.lp.as<int>(); h.li = dp.li.as<int>(); h.up = dp.up.as<int>(); h.ui = dp.ui.as<int>();
   h.pinv = dp.pinv.as<int>(); h.q = dp.q.as<int>(); h.prow = dp.prow.as<int>();
+  h.piv_pos = dp.piv_pos.as<int>();
+  h.wstride = d.wstride;
+  h.n_chunks = d.n_chunks;
+  upload(dp.ch_track_ptr, d.ch_track_ptr, s); upload(dp.ch_row_ptr, d.ch_row_ptr, s);
+  upload(dp.ch_a_ptr, d.ch_a_ptr, s); upload(dp.ch_fence, d.ch_fence, s); upload(dp.tr_col, d.tr_col, s);
+  upload(dp.ch_a_ws, d.ch_a_ws, s); upload(dp.ch_a_src, d.ch_a_src, s);
+  upload(dp.row_rec, d.row_rec, s); upload(dp.row_lsrc, d.row_lsrc, s);
+  h.ch_track_ptr = dp.ch_track_ptr.as<int>(); h.ch_row_ptr = dp.ch_row_ptr.as<int>();
+  h.ch_a_ptr = dp.ch_a_ptr.as<int>(); h.ch_fence = dp.ch_fence.as<int>(); h.tr_col = dp.tr_col.as<int>();
+  h.ch_a_ws = dp.ch_a_ws.as<int>(); h.ch_a_src = dp.ch_a_src.as<int>();
+  h.row_rec = dp.row_rec.as<int>(); h.row_lsrc = dp.row_lsrc.as<int>();
+  h.fw_levels = d.fw_levels;
+  h.bw_levels = d.bw_levels;
+  upload(dp.fw_order, d.fw_order, s); upload(dp.fw_level_end, d.fw_level_end, s);
+  upload(dp.lr_ptr, d.lr_ptr, s); upload(dp.lr_col, d.lr_col, s); upload(dp.lr_idx, d.lr_idx, s);
+  upload(dp.bw_order, d.bw_order, s); upload(dp.bw_level_end, d.bw_level_end, s);
+  upload(dp.ur_ptr, d.ur_ptr, s); upload(dp.ur_col, d.ur_col, s); upload(dp.ur_idx, d.ur_idx, s);
+  h.fw_order = dp.fw_order.as<int>(); h.fw_level_end = dp.fw_level_end.as<int>();
+  h.lr_ptr = dp.lr_ptr.as<int>(); h.lr_col = dp.lr_col.as<int>(); h.lr_idx = dp.lr_idx.as<int>();
+  h.bw_order = dp.bw_order.as<int>(); h.bw_level_end = dp.bw_level_end.as<int>();
+  h.ur_ptr = dp.ur_ptr.as<int>(); h.ur_col = dp.ur_col.as<int>(); h.ur_idx = dp.ur_idx.as<int>();
   dp.d_struct.ensure(sizeof(DevPlan));
   check(hipMemcpyAsync(dp.d_struct.p, &h, sizeof(DevPlan), hipMemcpyHostToDevice, s), "upload plan");
 }
@@ -257,7 +280,7 @@ BatchRun plan_run(const DevPlan& h, const gs_nr_options& o, bool force_fp64) {
   BatchRun r;
   r.fp32 = !force_fp64 && o.lu_precision == GS_LU_FP32;
   size_t t = r.fp32 ? 4 : 8;
-  size_t ws = t * (((size_t)h.max_col + 7) & ~size_t(7));
+  size_t ws = t * (size_t)kTracks * (size_t)h.wstride;
   size_t x = t * (size_t)h.m;
   r.x_lds = ws + x <= kLdsLimit;
   r.lds_bytes = ws + (r.x_lds ? x : 0);
@@ -568,7 +591,7 @@ gs_status seam_solve_one(gs_ctx* ctx, int n, const int* rowptr, const int* colid
                ux.as<double>(), xs.as<double>(), rf.as<double>(), stt.as<int>(), sp.dev.piv_pos.as<int>()};
     desc.ensure(sizeof(SeamDesc));
     check(hipMemcpyAsync(desc.p, &d, sizeof(SeamDesc), hipMemcpyHostToDevice, s), "seam desc");
-    size_t lds = 8 * (((size_t)sp.dev.h.max_col + 7) & ~size_t(7));
+    size_t lds = 8 * (size_t)kTracks * (size_t)sp.dev.h.wstride;
     check(launch_seam(desc.as<SeamDesc>(), 1, lds, s), "seam kernel");
     int status = 1;
     check(hipMemcpyAsync(&status, stt.p, 4, hipMemcpyDeviceToHost, s), "seam status");
@@ -845,6 +868,33 @@ GS_API gs_status gs_plan_get(const gs_plan* p, int32_t* q, int32_t* pivot_row, i
 }
 
 GS_API void gs_plan_free(gs_plan* p) { delete p; }
+
+GS_API gs_status gs_plan_selfcheck(const gs_plan* p, double* rel_residual, char* why) {
+  if (!p || !rel_residual) return GS_INVALID_ARGUMENT;
+  const int n = p->net.n, m = p->jp.m;
+  std::vector<double> pp(n), qq(n), jv(p->jp.rp[m]), f(m), x(m);
+  host_injections(p->y, p->spec.vm.data(), p->spec.va.data(), pp.data(), qq.data());
+  host_jacobian(p->y, p->jp, p->spec.vm.data(), p->spec.va.data(), pp.data(), qq.data(), jv.data());
+  for (int i = 0; i < n; i++) {
+    if (p->jp.ppos[i] >= 0) f[p->jp.ppos[i]] = p->spec.pspec[i] - pp[i];
+    if (p->jp.qpos[i] >= 0) f[p->jp.qpos[i]] = p->spec.qspec[i] - qq[i];
+  }
+  std::string w;
+  if (!emulate_device_lu(p->dph, p->jp.rp[m], jv.data(), f.data(), x.data(), w)) {
+    if (why) std::snprintf(why, 128, "%s", w.c_str());
+    return GS_INVALID_ARGUMENT;
+  }
+  double rmax = 0, fmax = 0;
+  for (int i = 0; i < m; i++) {
+    double s = 0;
+    for (int k = p->jp.rp[i]; k < p->jp.rp[i + 1]; k++) s += jv[k] * x[p->jp.ci[k]];
+    rmax = std::max(rmax, std::fabs(s - f[i]));
+    fmax = std::max(fmax, std::fabs(f[i]));
+  }
+  *rel_residual = fmax > 0 ? rmax / fmax : rmax;
+  if (why) why[0] = 0;
+  return GS_OK;
+}
 
 GS_API gs_status gs_last_timing(const gs_net* net, double* total_ms, double* kernel_ms, int32_t* launches) {
   if (!net) return GS_INVALID_ARGUMENT;

@@ -532,6 +532,121 @@ void compile_plan(const Plan& plan, DevicePlanHost& d) {
   d.col_order.assign(m, 0);
   for (int k = 0; k < m; k++) d.col_order[count[level[k]]++] = k;
   d.n_levels = n_levels;
+  // Level chunks: within each level, columns by decreasing chain length, kTracks at a time.
+  const int G = DevicePlanHost::kTracks, TL = DevicePlanHost::kTrackLanes;
+  d.wstride = (d.max_col + 7) & ~7;
+  if (d.wstride * G > 0xFFFF) throw std::runtime_error("plan: column workspace too large for 16-bit slot indices");
+  d.ch_track_ptr.assign(1, 0);
+  d.ch_row_ptr.assign(1, 0);
+  d.ch_a_ptr.assign(1, 0);
+  for (int lv = 0; lv < n_levels; lv++) {
+    int b0 = lv == 0 ? 0 : d.level_end[lv - 1], b1 = d.level_end[lv];
+    vector<int> cols(d.col_order.begin() + b0, d.col_order.begin() + b1);
+    std::stable_sort(cols.begin(), cols.end(), [&](int a, int b) {
+      return (plan.up[a + 1] - plan.up[a]) > (plan.up[b + 1] - plan.up[b]);
+    });
+    for (size_t c0 = 0; c0 < cols.size(); c0 += G) {
+      size_t c1 = std::min(cols.size(), c0 + G);
+      // rows of each track: one per (U entry t, group of TL updates)
+      vector<vector<std::pair<int, int>>> trows(c1 - c0);  // per track: list of (t, first update r) sub-steps
+      size_t nrows = 0;
+      for (size_t g = 0; g < c1 - c0; g++) {
+        int k = cols[c0 + g];
+        d.tr_col.push_back(k);
+        int nu = plan.up[k + 1] - plan.up[k];
+        for (int t = 0; t < nu - 1; t++) {
+          int ue = plan.up[k] + t, r0 = d.upd_ptr[ue], r1 = d.upd_ptr[ue + 1];
+          for (int r = r0; r < r1; r += TL) trows[g].push_back({t, r});
+        }
+        nrows = std::max(nrows, trows[g].size());
+        int col = plan.q[k];
+        for (int e = d.col_a_ptr[k]; e < d.col_a_ptr[k + 1]; e++) {
+          d.ch_a_ws.push_back((int)g * d.wstride + d.a_slot[e]);
+          d.ch_a_src.push_back(d.a_src[e]);
+        }
+        (void)col;
+      }
+      for (size_t rho = 0; rho < nrows; rho++) {
+        for (int lane = 0; lane < 64; lane++) {
+          size_t g = lane / TL;
+          int idx = lane % TL;
+          int rec = 0xFFFF, lsrc = 0;
+          if (g < c1 - c0 && rho < trows[g].size()) {
+            int k = d.tr_col[d.tr_col.size() - (c1 - c0) + g];
+            auto [t, rstart] = trows[g][rho];
+            int ue = plan.up[k] + t;
+            int r = rstart + idx;
+            if (r < d.upd_ptr[ue + 1]) {
+              int j = plan.ui[ue];
+              int base = (int)g * d.wstride;
+              rec = (base + d.upd_slot[r]) | ((base + t) << 16);
+              lsrc = plan.lp[j] + 1 + (r - d.upd_ptr[ue]);
+            }
+          }
+          d.row_rec.push_back(rec);
+          d.row_lsrc.push_back(lsrc);
+        }
+      }
+      d.ch_track_ptr.push_back((int)d.tr_col.size());
+      d.ch_row_ptr.push_back(d.ch_row_ptr.back() + (int)nrows);
+      d.ch_a_ptr.push_back((int)d.ch_a_ws.size());
+      d.ch_fence.push_back(c1 == cols.size() ? 1 : 0);
+    }
+  }
+  d.n_chunks = (int)d.ch_fence.size();
+  d.n_rows = d.ch_row_ptr.back();
+  // Forward solve rows: L(i, j) for j < i, columns ascending; level(i) = 1 + max level(j).
+  {
+    vector<vector<std::pair<int, int>>> rows(m);
+    for (int j = 0; j < m; j++)
+      for (int p = plan.lp[j] + 1; p < plan.lp[j + 1]; p++) rows[plan.li[p]].push_back({j, p});
+    vector<int> lvl(m, 0);
+    int nlv = 0;
+    d.lr_ptr.assign(m + 1, 0);
+    for (int i = 0; i < m; i++) {
+      std::sort(rows[i].begin(), rows[i].end());
+      int lv = 0;
+      for (auto& e : rows[i]) lv = std::max(lv, lvl[e.first] + 1);
+      lvl[i] = lv;
+      nlv = std::max(nlv, lv + 1);
+      d.lr_ptr[i + 1] = d.lr_ptr[i] + (int)rows[i].size();
+      for (auto& e : rows[i]) { d.lr_col.push_back(e.first); d.lr_idx.push_back(e.second); }
+    }
+    vector<int> cnt(nlv + 1, 0);
+    for (int i = 0; i < m; i++) cnt[lvl[i] + 1]++;
+    for (int l = 0; l < nlv; l++) cnt[l + 1] += cnt[l];
+    d.fw_level_end.assign(cnt.begin() + 1, cnt.end());
+    d.fw_order.assign(m, 0);
+    for (int i = 0; i < m; i++) d.fw_order[cnt[lvl[i]]++] = i;
+    d.fw_levels = nlv;
+  }
+  // Back solve rows: U(i, j) for j > i, columns descending; level(i) = 1 + max level(j).
+  {
+    vector<vector<std::pair<int, int>>> rows(m);
+    for (int j = 0; j < m; j++)
+      for (int p = plan.up[j]; p < plan.up[j + 1] - 1; p++) rows[plan.ui[p]].push_back({j, p});
+    vector<int> lvl(m, 0);
+    int nlv = 0;
+    d.ur_ptr.assign(m + 1, 0);
+    for (int i = m - 1; i >= 0; i--) {
+      std::sort(rows[i].begin(), rows[i].end(), [](auto& a, auto& b) { return a.first > b.first; });
+      int lv = 0;
+      for (auto& e : rows[i]) lv = std::max(lv, lvl[e.first] + 1);
+      lvl[i] = lv;
+      nlv = std::max(nlv, lv + 1);
+    }
+    for (int i = 0; i < m; i++) {
+      d.ur_ptr[i + 1] = d.ur_ptr[i] + (int)rows[i].size();
+      for (auto& e : rows[i]) { d.ur_col.push_back(e.first); d.ur_idx.push_back(e.second); }
+    }
+    vector<int> cnt(nlv + 1, 0);
+    for (int i = 0; i < m; i++) cnt[lvl[i] + 1]++;
+    for (int l = 0; l < nlv; l++) cnt[l + 1] += cnt[l];
+    d.bw_level_end.assign(cnt.begin() + 1, cnt.end());
+    d.bw_order.assign(m, 0);
+    for (int i = 0; i < m; i++) d.bw_order[cnt[lvl[i]]++] = i;
+    d.bw_levels = nlv;
+  }
   // For the exact pivot check: how many candidates (rows not yet pivotal at step k) precede the pivot in reach order.
   d.piv_pos.assign(m, 0);
   for (int k = 0; k < m; k++) {
@@ -543,6 +658,87 @@ void compile_plan(const Plan& plan, DevicePlanHost& d) {
     }
     d.piv_pos[k] = before;
   }
+}
+
+}  // namespace gs
+
+namespace gs {
+
+bool emulate_device_lu(const DevicePlanHost& d, int nnz_j, const double* jv, const double* rhs, double* out,
+                       std::string& why) {
+  const int m = d.m, G = DevicePlanHost::kTracks, TL = DevicePlanHost::kTrackLanes;
+  const int wsz = G * d.wstride;
+  vector<double> ws(wsz), lx(d.nnz_l, std::nan("")), ux(d.nnz_u, std::nan(""));
+  auto bad = [&](const std::string& w) { why = w; return false; };
+  for (int c = 0; c < d.n_chunks; c++) {
+    int t0 = d.ch_track_ptr[c], ntr = d.ch_track_ptr[c + 1] - t0;
+    if (ntr < 1 || ntr > G) return bad("chunk track count");
+    std::fill(ws.begin(), ws.end(), 0.0);
+    for (int e = d.ch_a_ptr[c]; e < d.ch_a_ptr[c + 1]; e++) {
+      if (d.ch_a_ws[e] < 0 || d.ch_a_ws[e] >= wsz) return bad("A workspace index");
+      if (d.ch_a_src[e] < 0 || d.ch_a_src[e] >= nnz_j) return bad("A source index");
+      ws[d.ch_a_ws[e]] = jv[d.ch_a_src[e]];
+    }
+    for (int rho = d.ch_row_ptr[c]; rho < d.ch_row_ptr[c + 1]; rho++) {
+      for (int lane = 0; lane < 64; lane++) {
+        int rec = d.row_rec[(size_t)rho * 64 + lane];
+        int dst = rec & 0xFFFF, xj = rec >> 16;
+        if (dst == 0xFFFF) continue;
+        int lsrc = d.row_lsrc[(size_t)rho * 64 + lane];
+        if (dst >= wsz || xj >= wsz || dst / d.wstride != lane / TL || xj / d.wstride != lane / TL)
+          return bad("row record outside its track's workspace");
+        if (lsrc < 0 || lsrc >= d.nnz_l || std::isnan(lx[lsrc])) return bad("row reads an L value not yet written");
+        ws[dst] = ws[dst] - lx[lsrc] * ws[xj];
+      }
+    }
+    for (int g = 0; g < ntr; g++) {
+      int k = d.tr_col[t0 + g];
+      int u0 = d.up[k], nu = d.up[k + 1] - u0, l0 = d.lp[k], nl = d.lp[k + 1] - l0;
+      if (nu + nl - 1 > d.wstride) return bad("column larger than the workspace stride");
+      const double* w = &ws[g * d.wstride];
+      double pivot = w[nu - 1];
+      if (pivot == 0.0) return bad("zero pivot");
+      for (int t = 0; t < nu; t++) ux[u0 + t] = w[t];
+      lx[l0] = 1.0;
+      for (int t = 1; t < nl; t++) lx[l0 + t] = w[nu - 1 + t] / pivot;
+    }
+  }
+  vector<double> x(m, std::nan(""));
+  for (int r = 0; r < m; r++) x[d.pinv[r]] = rhs[r];
+  vector<char> done(m, 0);
+  int b0 = 0;
+  for (int lv = 0; lv < d.fw_levels; lv++) {
+    int b1 = d.fw_level_end[lv];
+    for (int idx = b0; idx < b1; idx++) {
+      int i = d.fw_order[idx];
+      double s = x[i];
+      for (int e = d.lr_ptr[i]; e < d.lr_ptr[i + 1]; e++) {
+        if (!done[d.lr_col[e]]) return bad("forward solve reads a row of a later level");
+        s = s - lx[d.lr_idx[e]] * x[d.lr_col[e]];
+      }
+      x[i] = s;
+    }
+    for (int idx = b0; idx < b1; idx++) done[d.fw_order[idx]] = 1;
+    b0 = b1;
+  }
+  std::fill(done.begin(), done.end(), 0);
+  b0 = 0;
+  for (int lv = 0; lv < d.bw_levels; lv++) {
+    int b1 = d.bw_level_end[lv];
+    for (int idx = b0; idx < b1; idx++) {
+      int i = d.bw_order[idx];
+      double s = x[i];
+      for (int e = d.ur_ptr[i]; e < d.ur_ptr[i + 1]; e++) {
+        if (!done[d.ur_col[e]]) return bad("back solve reads a row of a later level");
+        s = s - ux[d.ur_idx[e]] * x[d.ur_col[e]];
+      }
+      x[i] = s / ux[d.up[i + 1] - 1];
+    }
+    for (int idx = b0; idx < b1; idx++) done[d.bw_order[idx]] = 1;
+    b0 = b1;
+  }
+  for (int k = 0; k < m; k++) out[d.q[k]] = x[k];
+  return true;
 }
 
 }  // namespace gs

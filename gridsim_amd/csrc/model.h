@@ -112,7 +112,28 @@ struct DevicePlanHost {
   vector<int> col_order, level_end;  // columns by dependency level (a column needs the L columns its U row set names)
   vector<int> piv_pos;       // per step: candidates before the pivot in reach order (the exact pivot-search check)
   int n_levels = 0;
+
+  // Level chunks of the refactor: up to kTracks columns of one dependency level factored side by side by one
+  // wavefront, kTrackLanes lanes per column. Each chunk's updates are laid out in rows of 64 records (lane l of a row
+  // serves track l / kTrackLanes), so every lane knows its record's address without a search.
+  static constexpr int kTracks = 8, kTrackLanes = 8;
+  int wstride = 0, n_chunks = 0, n_rows = 0;
+  vector<int> ch_track_ptr, ch_row_ptr, ch_a_ptr, ch_fence;  // per chunk; ch_fence: last chunk of its level
+  vector<int> tr_col;                                        // column of each track, chunk order
+  vector<int> ch_a_ws, ch_a_src;                             // A entries: workspace index, J CSR position
+  vector<int> row_rec;                                       // (dst ws) | (xj ws << 16), 0xFFFF dst = no update
+  vector<int> row_lsrc;                                      // L value index of the record
+  // Level-scheduled triangular solves, rows pulled in the reference's update order.
+  int fw_levels = 0, bw_levels = 0;
+  vector<int> fw_order, fw_level_end, lr_ptr, lr_col, lr_idx;  // L rows: columns ascending
+  vector<int> bw_order, bw_level_end, ur_ptr, ur_col, ur_idx;  // U rows: columns descending, diagonal excluded
 };
 void compile_plan(const Plan& plan, DevicePlanHost& out);
+
+// Host emulation of the device refactor (level chunks, rows of 64 records) and level-scheduled solves, with every
+// index checked against its buffer: out = A^{-1} rhs for the CSR values jv. Returns false (and why) on an
+// out-of-range index or a zero pivot. The check the kernels' indexing gets on the CPU.
+bool emulate_device_lu(const DevicePlanHost& d, int nnz_j, const double* jv, const double* rhs, double* out,
+                       std::string& why);
 
 }  // namespace gs

@@ -22,7 +22,18 @@ struct DevPlan {
   const int32_t *upd_ptr, *upd_slot;
   const int32_t *lp, *li, *up, *ui;
   const int32_t *pinv, *q, *prow;     // row permutation, column permutation, pivot row of each step
+  // level chunks of the refactor (model.h DevicePlanHost)
+  int32_t wstride, n_chunks;
+  const int32_t *ch_track_ptr, *ch_row_ptr, *ch_a_ptr, *ch_fence, *tr_col, *ch_a_ws, *ch_a_src;
+  const int32_t *row_rec, *row_lsrc;
+  // level-scheduled triangular solves
+  int32_t fw_levels, bw_levels;
+  const int32_t *fw_order, *fw_level_end, *lr_ptr, *lr_col, *lr_idx;
+  const int32_t *bw_order, *bw_level_end, *ur_ptr, *ur_col, *ur_idx;
+  const int32_t* piv_pos;             // per step: candidates before the pivot in reach order
 };
+
+constexpr int kTracks = 8, kTrackLanes = 8;
 
 constexpr int kMaxPatch = 8;
 

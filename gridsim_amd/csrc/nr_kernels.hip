@@ -4,13 +4,15 @@
 //   1. injections, mismatch and Jacobian in one pass over the Y-bus (FDLIBM sincos once per Y cell, FP64):
 //      ComputeInjections :269-289, residual :218-233, RefillJacobian :342-380;
 //   2. the verdicts: converged / diverged / iteration cap (:235-238, NewtonShell.DivergedBeyondLimit :177-178);
-//   3. Gilbert-Peierls refactor on the plan (SparseLuFactorization.Refactor :246-306): columns in dependency-level
-//      order, each column staged in an LDS workspace, static pivots with a growth check (a failed check pauses the
-//      instance for a fresh CPU symbolic pass);
-//   4. triangular solves (Solve :338-371) in the LU precision with the right-hand side in LDS, then FP64 iterative
-//      refinement r = f - J dx on the FP64 Jacobian;
+//   3. Gilbert-Peierls refactor on the plan (SparseLuFactorization.Refactor :246-306), level-parallel: the columns of
+//      one dependency level are independent, so up to 8 of them are factored side by side (8 lanes each), each in its
+//      own LDS workspace, with static pivots and a growth check (a failed check pauses the instance for a fresh CPU
+//      symbolic pass);
+//   4. triangular solves (Solve :338-371) in the LU precision, level-scheduled over rows with the vector in LDS, each
+//      row pulling its updates in the order the reference's column sweeps apply them; then FP64 iterative refinement
+//      r = f - J dx on the FP64 Jacobian;
 //   5. the damped update (NewtonShell.ApplyStep :128-136).
-// Compiled with -ffp-contract=off: in FP64 every step is the reference's arithmetic, operation for operation.
+// Compiled with -ffp-contract=off: in FP64 every value is the reference's arithmetic, operation for operation.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -33,6 +35,17 @@ __device__ inline double wave_max(double v) {
   return v;
 }
 
+// maximum over the kTrackLanes lanes of one track (aligned groups of 8)
+__device__ inline double track_max(double v) {
+  for (int off = kTrackLanes / 2; off > 0; off >>= 1) v = nan_max(v, __shfl_xor(v, off, kWave));
+  return v;
+}
+
+__device__ inline int track_any(int v) {
+  for (int off = kTrackLanes / 2; off > 0; off >>= 1) v |= __shfl_xor(v, off, kWave);
+  return v;
+}
+
 __device__ inline int wave_any(int v) { return __any(v) ? 1 : 0; }
 
 // Ordering of LDS traffic between the lanes of one wavefront (the hardware executes a wave's LDS instructions in
@@ -40,9 +53,6 @@ __device__ inline int wave_any(int v) { return __any(v) ? 1 : 0; }
 #define WAVE_SYNC() __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront")
 // Global-memory writes of this wavefront made visible to its own later reads (work-group scope: same CU, shared L1).
 #define WG_SYNC() __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup")
-
-template <typename T>
-__device__ inline T tabs(T v) { return v < T(0) ? -v : v; }
 
 // ------------------------------------------------------------------------------------------------ mismatch + J
 // Returns max |f| (NaN-propagating) over the wave; sets *bad when a Jacobian value is not finite.
@@ -105,83 +115,129 @@ __device__ double mismatch_jacobian(const DevPlan& P, const InstDesc& d, int lan
 }
 
 // ------------------------------------------------------------------------------------------------ refactor
-// Left-looking Gilbert-Peierls on the plan, columns in dependency-level order (a column reads only L columns of
-// earlier levels). Returns -1 on success, else the failing pivot step.
-template <typename T>
+// Level-chunked left-looking Gilbert-Peierls on the plan. Track g of a chunk (lanes 8g..8g+7) owns one column and the
+// workspace ws[g * wstride, ...): slots [0, nu - 1) its off-diagonal U rows in topological order, slot nu - 1 the
+// pivot, slots [nu, nu + nl - 1) its L rows. Every workspace slot receives its updates in the reference's order.
+//   EXACT = false (Newton): a static pivot is kept while |pivot| >= growth_tol * max |candidate|.
+//   EXACT = true (seam): the reference's own pivot search must pick the plan's row (first strict maximum over the
+//   candidates in reach order), above that row's floor; anything else refuses, as Refactor returns null.
+// Returns -1 on success, else a failing pivot step.
+template <typename T, bool EXACT>
 __device__ int refactor(const DevPlan& P, const double* __restrict__ jv, T* lx, T* ux, T* ws, double growth_tol,
-                        int lane, double* min_ratio) {
-  const int m = P.m;
-  int level = 0;
-  int level_end = P.level_end[0];
+                        const double* rowfloor, int lane, double* min_ratio) {
+  const int g = lane / kTrackLanes, tl = lane % kTrackLanes;
+  const int wsz = kTracks * P.wstride;
   double mratio = *min_ratio;
-  for (int idx = 0; idx < m; idx++) {
-    if (idx == level_end) {
-      WG_SYNC();
-      level++;
-      level_end = P.level_end[level];
+  for (int c = 0; c < P.n_chunks; c++) {
+    const int t0 = P.ch_track_ptr[c], ntr = P.ch_track_ptr[c + 1] - t0;
+    const int r0 = P.ch_row_ptr[c], nrows = P.ch_row_ptr[c + 1] - r0;
+    for (int s = lane; s < wsz; s += kWave) ws[s] = T(0);
+    WAVE_SYNC();
+    for (int e = P.ch_a_ptr[c] + lane; e < P.ch_a_ptr[c + 1]; e += kWave) ws[P.ch_a_ws[e]] = (T)jv[P.ch_a_src[e]];
+    WAVE_SYNC();
+    // updates, two rows of records in flight ahead of the row being applied
+    int rec_a = 0xFFFF, rec_b = 0xFFFF;
+    T val_a = T(0);
+    if (nrows > 0) {
+      rec_a = P.row_rec[(size_t)r0 * kWave + lane];
+      if ((rec_a & 0xFFFF) != 0xFFFF) val_a = lx[P.row_lsrc[(size_t)r0 * kWave + lane]];
     }
-    const int k = P.col_order[idx];
-    const int u0 = P.up[k], u1 = P.up[k + 1], l0 = P.lp[k], l1 = P.lp[k + 1];
-    const int nu = u1 - u0, nl = l1 - l0, c = nu + nl - 1;
-    for (int s = lane; s < c; s += kWave) ws[s] = T(0);
-    WAVE_SYNC();
-    for (int e = P.col_a_ptr[k] + lane; e < P.col_a_ptr[k + 1]; e += kWave) ws[P.a_slot[e]] = (T)jv[P.a_src[e]];
-    WAVE_SYNC();
-    for (int t = 0; t < nu - 1; t++) {
-      const T xj = ws[t];
-      const int j = P.ui[u0 + t];
-      const int r0 = P.upd_ptr[u0 + t], r1 = P.upd_ptr[u0 + t + 1];
-      const int lbase = P.lp[j] + 1 - r0;
-      for (int r = r0 + lane; r < r1; r += kWave) {
-        const int s = P.upd_slot[r];
-        ws[s] = ws[s] - lx[lbase + r] * xj;
+    if (nrows > 1) rec_b = P.row_rec[(size_t)(r0 + 1) * kWave + lane];
+    for (int rho = 0; rho < nrows; rho++) {
+      const int rec = rec_a;
+      const T val = val_a;
+      rec_a = rec_b;
+      val_a = T(0);
+      if (rho + 1 < nrows && (rec_a & 0xFFFF) != 0xFFFF) val_a = lx[P.row_lsrc[(size_t)(r0 + rho + 1) * kWave + lane]];
+      rec_b = (rho + 2 < nrows) ? P.row_rec[(size_t)(r0 + rho + 2) * kWave + lane] : 0xFFFF;
+      const int dst = rec & 0xFFFF;
+      if (dst != 0xFFFF) {
+        const T xj = ws[rec >> 16];
+        ws[dst] = ws[dst] - val * xj;
       }
       WAVE_SYNC();
     }
-    const T pivot = ws[nu - 1];
-    double cand = fabs((double)pivot);
-    for (int s = nu + lane; s < c; s += kWave) cand = nan_max(cand, fabs((double)ws[s]));
-    const double best = wave_max(cand);
-    const double ratio = fabs((double)pivot) / best;
-    if (!(ratio >= growth_tol) || !(fabs((double)pivot) > 0.0) || !isfinite((double)pivot)) return k;
-    mratio = ratio < mratio ? ratio : mratio;
-    for (int t = lane; t < nu; t += kWave) ux[u0 + t] = ws[t];
-    if (lane == 0) lx[l0] = T(1);
-    for (int t = 1 + lane; t < nl; t += kWave) lx[l0 + t] = ws[nu - 1 + t] / pivot;
+    // pivots, growth checks and write-out, one track per 8 lanes
+    int fail = -1;
+    if (g < ntr) {
+      const int k = P.tr_col[t0 + g];
+      const int u0 = P.up[k], nu = P.up[k + 1] - u0, l0 = P.lp[k], nl = P.lp[k + 1] - l0;
+      T* w = ws + g * P.wstride;
+      const T pivot = w[nu - 1];
+      const double pabs = fabs((double)pivot);
+      double cand = pabs;
+      int moved = 0;
+      const int before = EXACT ? P.piv_pos[k] : 0;
+      for (int t = 1 + tl; t < nl; t += kTrackLanes) {
+        const double cabs = fabs((double)w[nu - 1 + t]);
+        cand = nan_max(cand, cabs);
+        if (EXACT) moved |= (t - 1 < before) ? !(cabs < pabs) : (cabs > pabs);
+      }
+      const double best = track_max(cand);
+      bool bad;
+      if (EXACT) {
+        bad = track_any(moved) || !(pabs > rowfloor[P.prow[k]]);
+      } else {
+        const double ratio = pabs / best;
+        bad = !(ratio >= growth_tol) || !(pabs > 0.0) || !isfinite(pabs);
+        mratio = ratio < mratio ? ratio : mratio;
+      }
+      if (bad) {
+        fail = k;
+      } else {
+        for (int t = tl; t < nu; t += kTrackLanes) ux[u0 + t] = w[t];
+        if (tl == 0) lx[l0] = T(1);
+        for (int t = 1 + tl; t < nl; t += kTrackLanes) lx[l0 + t] = w[nu - 1 + t] / pivot;
+      }
+    }
+    // any failing track fails the factorisation (lowest lane's column reported)
+    unsigned long long fmask = __ballot(fail >= 0);
+    if (fmask) {
+      const int first = __ffsll((long long)fmask) - 1;
+      return __shfl(fail, first, kWave);
+    }
     WAVE_SYNC();
+    if (P.ch_fence[c]) WG_SYNC();
   }
   WG_SYNC();
+  mratio = -wave_max(-mratio);
   *min_ratio = mratio;
   return -1;
 }
 
 // ------------------------------------------------------------------------------------------------ solves
-// out = A^{-1} rhs with the factors (column sweeps in the reference's order); x is the workspace (LDS or global).
+// out = A^{-1} rhs with the factors. Rows of one level are independent: each lane pulls one row's updates in the order
+// the reference's column sweeps apply them (forward: columns ascending; back: columns descending, then the division
+// by the pivot), so every entry is the reference's value. x is the workspace (LDS or global).
 template <typename T, bool XLDS>
 __device__ void lu_solve(const DevPlan& P, const T* __restrict__ lx, const T* __restrict__ ux, const double* rhs,
                          double* out, T* x, int lane) {
   const int m = P.m;
   for (int r = lane; r < m; r += kWave) x[P.pinv[r]] = (T)rhs[r];
   if (XLDS) WAVE_SYNC(); else WG_SYNC();
-  for (int j = 0; j < m; j++) {
-    const T xj = x[j];
-    const int p1 = P.lp[j + 1];
-    for (int p = P.lp[j] + 1 + lane; p < p1; p += kWave) {
-      const int i = P.li[p];
-      x[i] = x[i] - lx[p] * xj;
+  int b0 = 0;
+  for (int lv = 0; lv < P.fw_levels; lv++) {
+    const int b1 = P.fw_level_end[lv];
+    for (int idx = b0 + lane; idx < b1; idx += kWave) {
+      const int i = P.fw_order[idx];
+      T s = x[i];
+      for (int e = P.lr_ptr[i]; e < P.lr_ptr[i + 1]; e++) s = s - lx[P.lr_idx[e]] * x[P.lr_col[e]];
+      x[i] = s;
     }
     if (XLDS) WAVE_SYNC(); else WG_SYNC();
+    b0 = b1;
   }
-  for (int j = m - 1; j >= 0; j--) {
-    const int p0 = P.up[j], p1 = P.up[j + 1] - 1;
-    const T xj = x[j] / ux[p1];
-    if (XLDS) WAVE_SYNC(); else WG_SYNC();
-    if (lane == 0) x[j] = xj;
-    for (int p = p0 + lane; p < p1; p += kWave) {
-      const int i = P.ui[p];
-      x[i] = x[i] - ux[p] * xj;
+  b0 = 0;
+  for (int lv = 0; lv < P.bw_levels; lv++) {
+    const int b1 = P.bw_level_end[lv];
+    for (int idx = b0 + lane; idx < b1; idx += kWave) {
+      const int i = P.bw_order[idx];
+      T s = x[i];
+      for (int e = P.ur_ptr[i]; e < P.ur_ptr[i + 1]; e++) s = s - ux[P.ur_idx[e]] * x[P.ur_col[e]];
+      x[i] = s / ux[P.up[i + 1] - 1];
     }
     if (XLDS) WAVE_SYNC(); else WG_SYNC();
+    b0 = b1;
   }
   for (int k = lane; k < m; k += kWave) out[P.q[k]] = (double)x[k];
   WG_SYNC();
@@ -207,7 +263,7 @@ __global__ __launch_bounds__(kWave) void nr_batch_kernel(const InstDesc* __restr
   InstState* st = d.state;
   const int n = P.n, m = P.m;
   T* ws = reinterpret_cast<T*>(smem);
-  T* x = XLDS ? ws + ((P.max_col + 7) & ~7) : reinterpret_cast<T*>(d.xg);
+  T* x = XLDS ? ws + kTracks * P.wstride : reinterpret_cast<T*>(d.xg);
   T* lx = reinterpret_cast<T*>(d.lx);
   T* ux = reinterpret_cast<T*>(d.ux);
 
@@ -215,7 +271,7 @@ __global__ __launch_bounds__(kWave) void nr_batch_kernel(const InstDesc* __restr
     for (int i = lane; i < n; i += kWave) { d.vm[i] = d.vm_start[i]; d.va[i] = d.va_start[i]; }
     WG_SYNC();
   }
-  if (st->done) return;
+  if (!prm.init && st->done) return;
   int iter = prm.init ? 0 : st->iter;
   double start_mis = prm.init ? __builtin_nan("") : st->start_mis;
   double min_ratio = prm.init ? 1.0 : st->min_ratio;
@@ -232,7 +288,7 @@ __global__ __launch_bounds__(kWave) void nr_batch_kernel(const InstDesc* __restr
     if (iter >= prm.max_iterations) break;
     if (steps >= prm.steps_this_launch) { paused = 1; break; }
     if (bad || !isfinite(max_mis)) break;  // NewtonShell.SolveStep refuses a non-finite J or f
-    fail_col = refactor<T>(P, d.jv, lx, ux, ws, prm.growth_tol, lane, &min_ratio);
+    fail_col = refactor<T, false>(P, d.jv, lx, ux, ws, prm.growth_tol, nullptr, lane, &min_ratio);
     if (fail_col >= 0) { paused = 2; break; }
     lu_solve<T, XLDS>(P, lx, ux, d.f, d.dx, x, lane);
     for (int it = 0; it < prm.refine_steps; it++) {
@@ -299,46 +355,12 @@ __global__ __launch_bounds__(kWave) void seam_solve_kernel(const SeamDesc* __res
     d.rowfloor[i] = mx * 1e-12;
   }
   WG_SYNC();
-  int level = 0, level_end = P.level_end[0];
-  for (int idx = 0; idx < m; idx++) {
-    if (idx == level_end) { WG_SYNC(); level++; level_end = P.level_end[level]; }
-    const int k = P.col_order[idx];
-    const int u0 = P.up[k], u1 = P.up[k + 1], l0 = P.lp[k], l1 = P.lp[k + 1];
-    const int nu = u1 - u0, nl = l1 - l0, c = nu + nl - 1;
-    for (int s = lane; s < c; s += kWave) ws[s] = 0.0;
-    WAVE_SYNC();
-    for (int e = P.col_a_ptr[k] + lane; e < P.col_a_ptr[k + 1]; e += kWave) ws[P.a_slot[e]] = d.a[P.a_src[e]];
-    WAVE_SYNC();
-    for (int t = 0; t < nu - 1; t++) {
-      const double xj = ws[t];
-      const int j = P.ui[u0 + t];
-      const int r0 = P.upd_ptr[u0 + t], r1 = P.upd_ptr[u0 + t + 1];
-      const int lbase = P.lp[j] + 1 - r0;
-      for (int r = r0 + lane; r < r1; r += kWave) {
-        const int s = P.upd_slot[r];
-        ws[s] = ws[s] - d.lx[lbase + r] * xj;
-      }
-      WAVE_SYNC();
-    }
-    // the reference's pivot search: first strict maximum over the candidates in reach order
-    const double pabs = fabs(ws[nu - 1]);
-    const int before = d.piv_pos[k];
-    int moved = 0;
-    for (int t = 1 + lane; t < nl; t += kWave) {
-      const double cabs = fabs(ws[nu - 1 + t]);
-      moved |= (t - 1 < before) ? !(cabs < pabs) : (cabs > pabs);
-    }
-    if (wave_any(moved) || !(pabs > d.rowfloor[P.prow[k]])) {
-      if (lane == 0) *d.status = 1;
-      return;
-    }
-    const double pivot = ws[nu - 1];
-    for (int t = lane; t < nu; t += kWave) d.ux[u0 + t] = ws[t];
-    if (lane == 0) d.lx[l0] = 1.0;
-    for (int t = 1 + lane; t < nl; t += kWave) d.lx[l0 + t] = ws[nu - 1 + t] / pivot;
-    WAVE_SYNC();
+  double ratio = 1.0;
+  const int fail = refactor<double, true>(P, d.a, d.lx, d.ux, ws, 0.0, d.rowfloor, lane, &ratio);
+  if (fail >= 0) {
+    if (lane == 0) *d.status = 1;
+    return;
   }
-  WG_SYNC();
   lu_solve<double, false>(P, d.lx, d.ux, d.b, d.x, d.xs, lane);
   if (lane == 0) *d.status = 0;
 }

@@ -172,6 +172,10 @@ GS_API gs_status gs_plan_sizes(const gs_plan* plan, int32_t* m, int32_t* nnz_j, 
 GS_API gs_status gs_plan_get(const gs_plan* plan, int32_t* q, int32_t* pivot_row, int32_t* lp, int32_t* li,
                              int32_t* up, int32_t* ui, int32_t* j_rowptr, int32_t* j_colidx);
 GS_API void gs_plan_free(gs_plan* plan);
+/* Runs the device factorisation and solves of the plan in a host emulation with every index checked, on the
+ * Jacobian at the start point, and returns ||J x - f||_inf / ||f||_inf. GS_INVALID_ARGUMENT (with the reason in
+ * *why, 128 bytes) when an index falls outside its buffer. */
+GS_API gs_status gs_plan_selfcheck(const gs_plan* plan, double* rel_residual, char* why);
 
 /* Kernel timing of the last gs_nr_solve_batch on this network (milliseconds, HIP events on its stream). */
 GS_API gs_status gs_last_timing(const gs_net* net, double* total_ms, double* kernel_ms, int32_t* launches);

@@ -94,6 +94,25 @@ def test_product_plan_equals_oracle_plan(eng, oracle, case, flat):
     np.testing.assert_array_equal(plan.ui, lu.ui)
 
 
+@pytest.mark.parametrize("case", ["case9", "case118", "case300", "synthetic-2000", "synthetic-10000"])
+def test_device_lu_indexing_emulated_on_host(eng, case):
+    """The device refactor (level chunks, 64-record rows) and level-scheduled solves, emulated on the host with every
+    index checked against its buffer, solve J dx = f of the start point to FP64 accuracy."""
+    from gridsim_amd import gridgen
+    if case == "synthetic-2000":
+        net = gridgen.config2_grid()
+    elif case == "synthetic-10000":
+        net = gridgen.sized_grid(10000, gridgen.BASE_SEED + 3)
+    else:
+        net = matpower_case(case)
+    plan = eng.HostPlan(net, eng.NrOptions(flat_start=True))
+    res = C.c_double()
+    why = C.create_string_buffer(128)
+    st = eng.lib().gs_plan_selfcheck(plan.h, C.byref(res), why)
+    assert st == eng.GS_OK, why.value.decode()
+    assert res.value < 1e-12
+
+
 def test_plan_of_warm_started_outage_grid(eng, oracle):
     """The N-1 grid's plan is taken at the base solution (warm start), as the screening run uses it."""
     from gridsim_amd import gridgen
