@@ -88,6 +88,9 @@ def main():
         print(f"# step {k}: sweep {sweep_ms[-1]:.2f} ms, kernel {r.kernel_ms:.2f} ms", file=sys.stderr, flush=True)
     elapsed = time.perf_counter() - t_start
     res = r.results
+    phase = res["phase_us"].mean(axis=0)
+    print("# mean device time per instance (us): " +
+          ", ".join(f"{n} {v:.0f}" for n, v in zip(engine.PHASES, phase)), file=sys.stderr, flush=True)
     converged = int((res["status"] == engine.CONVERGED).sum())
     ms_per_step = 1000 * elapsed / args.steps
     value = converged * args.steps / elapsed
@@ -156,7 +159,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
                      "kernel": "nr_batch_kernel<float,true>", "kernel_ms": round(kavg, 3),
-                     "algorithmic_bytes": int(alg)},
+                     "algorithmic_bytes": int(alg),
+                     "phase_us_per_instance": {n: round(float(v), 1) for n, v in zip(engine.PHASES, phase)}},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -84,6 +84,7 @@ struct gs_ctx {
   std::mutex mu;
   std::string err;
   size_t budget = 0;
+  double us_per_tick = 0.01;  // device wall clock (wall_clock64) period in microseconds
   std::map<std::pair<std::vector<int>, std::vector<int>>, std::vector<int>> seam_orders;  // pattern -> MD order
   std::map<std::vector<int>, std::shared_ptr<SeamPlan>> seam_plans;                       // nonzero positions -> plan
 };
@@ -292,7 +293,7 @@ BatchRun plan_run(const DevPlan& h, const gs_nr_options& o, bool force_fp64) {
 void run_batch(gs_net* net, int count, const gs_instance* inst, const gs_nr_options& o, gs_nr_result* results,
                double* vm_out, double* va_out, bool plain);
 
-void fill_result(const InstState& s, gs_nr_result& r, int refreshes) {
+void fill_result(const InstState& s, gs_nr_result& r, int refreshes, double us_per_tick) {
   r.status = s.nonfinite ? GS_PF_NONFINITE : s.converged ? GS_PF_CONVERGED : s.diverged ? GS_PF_DIVERGED
                                                                                        : GS_PF_MAX_ITERATIONS;
   r.iterations = s.iter;
@@ -300,6 +301,10 @@ void fill_result(const InstState& s, gs_nr_result& r, int refreshes) {
   r.symbolic_refreshes = refreshes;
   r.step_failures = s.step_failures;
   r.min_pivot_ratio = s.min_ratio;
+  r.phase_us[0] = s.t_mismatch * us_per_tick;
+  r.phase_us[1] = s.t_refactor * us_per_tick;
+  r.phase_us[2] = s.t_solve * us_per_tick;
+  r.phase_us[3] = s.t_update * us_per_tick;
 }
 
 struct Prepared {
@@ -331,9 +336,8 @@ double launch_instances(gs_ctx* ctx, DevicePlan& dp, DevMem& inst_buf, DevMem& d
   char* base = inst_buf.as<char>();
   InstState* dstate = state_buf.as<InstState>();
   std::vector<InstDesc> descs(count);
-  // staged per-instance host data: specs and starts, uploaded with one copy each
-  std::vector<double> stage(4 * (size_t)h.n * count, 0.0);
-  bool any_stage = false;
+  // staged per-instance host data (only instances with their own spec or start): [pspec | qspec | vm0 | va0]
+  std::vector<std::pair<int, std::vector<double>>> stage;
   for (int i = 0; i < count; i++) {
     InstDesc& d = descs[i];
     std::memset(&d, 0, sizeof(d));
@@ -355,15 +359,6 @@ double launch_instances(gs_ctx* ctx, DevicePlan& dp, DevMem& inst_buf, DevMem& d
       std::copy(bs.qspec.begin(), bs.qspec.end(), scaled.begin() + h.n);
       sp = &scaled;
     }
-    if (sp) {
-      std::copy(sp->begin(), sp->end(), stage.begin() + 4 * (size_t)h.n * i);
-      d.pspec = (const double*)(b + L.spec);
-      d.qspec = d.pspec + h.n;
-      any_stage = true;
-    } else {
-      d.pspec = nullptr;  // patched below to the network spec
-      d.qspec = nullptr;
-    }
     const std::vector<double>* st = start_override ? &(*start_override)[i] : nullptr;
     std::vector<double> own;
     if (!st && in.vm0 && in.va0) {
@@ -371,11 +366,22 @@ double launch_instances(gs_ctx* ctx, DevicePlan& dp, DevMem& inst_buf, DevMem& d
       own.insert(own.end(), in.va0, in.va0 + h.n);
       st = &own;
     }
+    if (sp || st) {
+      stage.emplace_back(i, std::vector<double>(4 * (size_t)h.n, 0.0));
+      std::vector<double>& buf = stage.back().second;
+      if (sp) std::copy(sp->begin(), sp->end(), buf.begin());
+      if (st) std::copy(st->begin(), st->end(), buf.begin() + 2 * (size_t)h.n);
+    }
+    if (sp) {
+      d.pspec = (const double*)(b + L.spec);
+      d.qspec = d.pspec + h.n;
+    } else {
+      d.pspec = nullptr;  // patched below to the network spec
+      d.qspec = nullptr;
+    }
     if (st) {
-      std::copy(st->begin(), st->end(), stage.begin() + 4 * (size_t)h.n * i + 2 * (size_t)h.n);
       d.vm_start = (const double*)(b + L.start);
       d.va_start = d.vm_start + h.n;
-      any_stage = true;
     } else {
       d.vm_start = h.vm0;
       d.va_start = h.va0;
@@ -389,15 +395,13 @@ double launch_instances(gs_ctx* ctx, DevicePlan& dp, DevMem& inst_buf, DevMem& d
     for (auto& d : descs)
       if (!d.pspec) { d.pspec = aux_buf.as<double>(); d.qspec = aux_buf.as<double>() + h.n; }
   }
-  if (any_stage) {
-    for (int i = 0; i < count; i++) {
-      char* b = base + L.stride * i;
-      const double* src = stage.data() + 4 * (size_t)h.n * i;
-      if (descs[i].pspec == (const double*)(b + L.spec))
-        check(hipMemcpyAsync(b + L.spec, src, 16 * (size_t)h.n, hipMemcpyHostToDevice, s), "stage spec");
-      if (descs[i].vm_start == (const double*)(b + L.start))
-        check(hipMemcpyAsync(b + L.start, src + 2 * h.n, 16 * (size_t)h.n, hipMemcpyHostToDevice, s), "stage start");
-    }
+  for (auto& [i, buf] : stage) {
+    char* b = base + L.stride * i;
+    if (descs[i].pspec == (const double*)(b + L.spec))
+      check(hipMemcpyAsync(b + L.spec, buf.data(), 16 * (size_t)h.n, hipMemcpyHostToDevice, s), "stage spec");
+    if (descs[i].vm_start == (const double*)(b + L.start))
+      check(hipMemcpyAsync(b + L.start, buf.data() + 2 * h.n, 16 * (size_t)h.n, hipMemcpyHostToDevice, s),
+            "stage start");
   }
   check(hipMemcpyAsync(desc_buf.p, descs.data(), sizeof(InstDesc) * count, hipMemcpyHostToDevice, s), "descs");
   if (!init) check(hipMemcpyAsync(dstate, states.data(), sizeof(InstState) * count, hipMemcpyHostToDevice, s),
@@ -473,7 +477,7 @@ void run_batch(gs_net* net, int count, const gs_instance* inst, const gs_nr_opti
   Layout L = instance_layout(net->dev.h, run.fp32, run.x_lds);
   for (int i = 0; i < count; i++) {
     if (states[i].done == 1) {
-      fill_result(states[i], results[i], 0);
+      fill_result(states[i], results[i], 0, ctx->us_per_tick);
       copy_state_out(ctx, net->inst_buf, L, i, n, vm_out ? vm_out + (size_t)n * i : nullptr,
                      va_out ? va_out + (size_t)n * i : nullptr);
     }
@@ -529,7 +533,7 @@ void run_batch(gs_net* net, int count, const gs_instance* inst, const gs_nr_opti
       check(hipStreamSynchronize(ctx->stream), "sync resume");
       finished = st.done == 1;
     }
-    fill_result(st, results[i], refreshes);
+    fill_result(st, results[i], refreshes, ctx->us_per_tick);
     if (vm_out) std::copy(vm.begin(), vm.end(), vm_out + (size_t)n * i);
     if (va_out) std::copy(va.begin(), va.end(), va_out + (size_t)n * i);
   }
@@ -669,6 +673,9 @@ GS_API gs_status gs_init(int32_t device, size_t hbm_budget_bytes, gs_ctx** out) 
   auto* ctx = new gs_ctx();
   ctx->device = device;
   ctx->budget = hbm_budget_bytes;
+  int khz = 0;
+  if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
+    ctx->us_per_tick = 1000.0 / khz;
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
     delete ctx;
     return GS_DEVICE_ERROR;

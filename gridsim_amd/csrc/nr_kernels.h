@@ -51,6 +51,8 @@ struct InstState {
   int32_t nonfinite, step_failures;
   int32_t fail_column, pad;
   double start_mis, max_mis, min_ratio, pad2;
+  // time in each phase (device wall-clock ticks): mismatch + Jacobian, refactor, solves + refinement, update
+  uint64_t t_mismatch, t_refactor, t_solve, t_update;
 };
 
 struct InstDesc {

@@ -278,10 +278,19 @@ __global__ __launch_bounds__(kWave) void nr_batch_kernel(const InstDesc* __restr
   int step_failures = prm.init ? 0 : st->step_failures;
   int converged = 0, diverged = 0, paused = 0, fail_col = -1, steps = 0;
   double max_mis;
+  uint64_t t_mis = prm.init ? 0 : st->t_mismatch, t_fac = prm.init ? 0 : st->t_refactor;
+  uint64_t t_sol = prm.init ? 0 : st->t_solve, t_upd = prm.init ? 0 : st->t_update;
+  uint64_t clk = wall_clock64();
+  auto tick = [&](uint64_t& acc) {
+    const uint64_t now = wall_clock64();
+    acc += now - clk;
+    clk = now;
+  };
   for (;;) {
     int bad;
     max_mis = mismatch_jacobian(P, d, lane, &bad);
     WG_SYNC();
+    tick(t_mis);
     if (iter == 0) start_mis = max_mis;
     if (max_mis < prm.tolerance) { converged = 1; break; }
     if (max_mis > nan_max(prm.diverge_limit, start_mis)) { diverged = 1; break; }
@@ -289,6 +298,7 @@ __global__ __launch_bounds__(kWave) void nr_batch_kernel(const InstDesc* __restr
     if (steps >= prm.steps_this_launch) { paused = 1; break; }
     if (bad || !isfinite(max_mis)) break;  // NewtonShell.SolveStep refuses a non-finite J or f
     fail_col = refactor<T, false>(P, d.jv, lx, ux, ws, prm.growth_tol, nullptr, lane, &min_ratio);
+    tick(t_fac);
     if (fail_col >= 0) { paused = 2; break; }
     lu_solve<T, XLDS>(P, lx, ux, d.f, d.dx, x, lane);
     for (int it = 0; it < prm.refine_steps; it++) {
@@ -297,6 +307,7 @@ __global__ __launch_bounds__(kWave) void nr_batch_kernel(const InstDesc* __restr
       for (int k = lane; k < m; k += kWave) d.dx[k] = d.dx[k] + d.r[k];
       WG_SYNC();
     }
+    tick(t_sol);
     double lmax = 0.0;
     int lbad = 0;
     for (int k = lane; k < m; k += kWave) {
@@ -316,6 +327,7 @@ __global__ __launch_bounds__(kWave) void nr_batch_kernel(const InstDesc* __restr
       }
     }
     WG_SYNC();
+    tick(t_upd);
     iter++;
     steps++;
   }
@@ -334,6 +346,10 @@ __global__ __launch_bounds__(kWave) void nr_batch_kernel(const InstDesc* __restr
     st->converged = converged;
     st->diverged = diverged;
     st->nonfinite = nonfinite;
+    st->t_mismatch = t_mis;
+    st->t_refactor = t_fac;
+    st->t_solve = t_sol;
+    st->t_update = t_upd;
     st->done = paused == 1 ? 0 : paused == 2 ? 2 : 1;
   }
 }

@@ -62,7 +62,8 @@ class GsInstance(C.Structure):
 
 RESULT_DTYPE = np.dtype([("status", np.int32), ("iterations", np.int32), ("max_mismatch", np.float64),
                          ("symbolic_refreshes", np.int32), ("step_failures", np.int32),
-                         ("min_pivot_ratio", np.float64)])
+                         ("min_pivot_ratio", np.float64), ("phase_us", np.float64, (4,))])
+PHASES = ("mismatch+jacobian", "refactor", "solve+refine", "update")
 
 
 class GsCsr(C.Structure):

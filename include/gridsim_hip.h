@@ -98,6 +98,7 @@ typedef struct gs_nr_result {
   int32_t symbolic_refreshes;   /* fresh CPU symbolic passes the pivot-growth check asked for */
   int32_t step_failures;        /* steps whose factorisation was refused (singular) */
   double min_pivot_ratio;       /* smallest |pivot| / max |candidate| seen (1 = the pivot was the largest) */
+  double phase_us[4];           /* device time of this instance: mismatch + Jacobian, refactor, solves, update */
 } gs_nr_result;
 
 typedef struct gs_device_info {

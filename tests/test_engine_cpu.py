@@ -36,7 +36,7 @@ def test_struct_sizes_match_the_header(eng):
     """The ctypes mirrors of the C structs have the C layout (the P/Invoke shape test of RocmInteropShapeTests)."""
     assert C.sizeof(eng.GsNrOptions) == 48
     assert C.sizeof(eng.GsInstance) == 32
-    assert eng.RESULT_DTYPE.itemsize == 32
+    assert eng.RESULT_DTYPE.itemsize == 64
     assert C.sizeof(eng.GsCsr) == 32
     assert C.sizeof(eng.GsNetwork) == 4 * 3 + 4 + 8 + 19 * 8
 
