@@ -18,9 +18,9 @@
 #include "nr_kernels.h"
 
 namespace gs {
-hipError_t launch_nr_batch(int lu_fp32, int x_in_lds, const InstDesc* descs, int count, const NrParams& prm,
-                           size_t lds_bytes, hipStream_t stream);
-hipError_t launch_seam(const SeamDesc* descs, int count, size_t lds_bytes, hipStream_t stream);
+hipError_t launch_nr_batch(int lu_fp32, int x_in_lds, const InstDesc* descs, const DevPlan& plan, int count,
+                           const NrParams& prm, size_t lds_bytes, hipStream_t stream);
+hipError_t launch_seam(const SeamDesc* descs, const DevPlan& plan, int count, size_t lds_bytes, hipStream_t stream);
 hipError_t launch_sincos(const double* x, double* s, double* c, int64_t n, hipStream_t stream);
 hipError_t launch_fp64_probe(double* out, hipStream_t stream);
 }  // namespace gs
@@ -93,13 +93,11 @@ namespace {
 
 // A plan and its network pattern uploaded to the device.
 struct DevicePlan {
-  DevMem y_rp, y_ci, y_diag, y_g, y_b, slots, ppos, qpos, j_rp, j_ci, vm0, va0;
-  DevMem col_order, level_end, col_a_ptr, a_src, a_slot, upd_ptr, upd_slot, lp, li, up, ui, pinv, q, prow;
-  DevMem piv_pos;
-  DevMem ch_track_ptr, ch_row_ptr, ch_a_ptr, ch_fence, tr_col, ch_a_ws, ch_a_src, row_rec, row_lsrc;
-  DevMem fw_order, fw_level_end, lr_ptr, lr_col, lr_idx, bw_order, bw_level_end, ur_ptr, ur_col, ur_idx;
-  DevMem d_struct;
-  DevPlan h{};
+  DevMem y_rp, y_ci, y_row, y_diag, y_g, y_b, slots, aslots, ppos, qpos, j_rp, j_ci, vm0, va0;
+  DevMem pinv, q, prow, piv_pos;
+  DevMem ch_meta, ch_a_ws, ch_a_src, tr_meta, tr_col, row_rl, row_meta;
+  DevMem fs_rec, fs_flags, bs_rec, bs_flags;
+  DevPlan h{};   // passed to the kernels by value
   DevicePlanHost dph;
 };
 
@@ -116,15 +114,24 @@ void upload_plan(DevicePlan& dp, const YBus* y, const JacobianPattern& jp, const
   h.max_col = d.max_col;
   h.n_levels = d.n_levels;
   h.base_mva = base_mva;
+  // the refactor record of each Jacobian CSR entry (every structural cell is kept, so every entry has one)
+  std::vector<int> j2a(d.j2a);
+  j2a.resize(std::max<size_t>((size_t)h.nnz_j, j2a.size()), -1);
   if (y) {
     h.n = y->n;
     h.nnz_y = y->rp[y->n];
-    upload(dp.y_rp, y->rp, s); upload(dp.y_ci, y->ci, s); upload(dp.y_diag, y->diag, s);
-    upload(dp.y_g, y->g, s); upload(dp.y_b, y->b, s);
-    upload(dp.slots, jp.slots, s); upload(dp.ppos, jp.ppos, s); upload(dp.qpos, jp.qpos, s);
-    h.y_rp = dp.y_rp.as<int>(); h.y_ci = dp.y_ci.as<int>(); h.y_diag = dp.y_diag.as<int>();
-    h.y_g = dp.y_g.as<double>(); h.y_b = dp.y_b.as<double>();
-    h.slots = dp.slots.as<int>(); h.ppos = dp.ppos.as<int>(); h.qpos = dp.qpos.as<int>();
+    std::vector<int> y_row(h.nnz_y), aslots(jp.slots.size());
+    for (int i = 0; i < y->n; i++)
+      for (int k = y->rp[i]; k < y->rp[i + 1]; k++) y_row[k] = i;
+    for (size_t e = 0; e < jp.slots.size(); e++) aslots[e] = jp.slots[e] >= 0 ? j2a[jp.slots[e]] : -1;
+    upload(dp.y_rp, y->rp, s); upload(dp.y_ci, y->ci, s); upload(dp.y_row, y_row, s);
+    upload(dp.y_diag, y->diag, s); upload(dp.y_g, y->g, s); upload(dp.y_b, y->b, s);
+    upload(dp.slots, jp.slots, s); upload(dp.aslots, aslots, s);
+    upload(dp.ppos, jp.ppos, s); upload(dp.qpos, jp.qpos, s);
+    h.y_rp = dp.y_rp.as<int>(); h.y_ci = dp.y_ci.as<int>(); h.y_row = dp.y_row.as<int>();
+    h.y_diag = dp.y_diag.as<int>(); h.y_g = dp.y_g.as<double>(); h.y_b = dp.y_b.as<double>();
+    h.slots = dp.slots.as<int4>(); h.aslots = dp.aslots.as<int4>();
+    h.ppos = dp.ppos.as<int>(); h.qpos = dp.qpos.as<int>();
   }
   if (spec) {
     upload(dp.vm0, spec->vm, s); upload(dp.va0, spec->va, s);
@@ -132,39 +139,68 @@ void upload_plan(DevicePlan& dp, const YBus* y, const JacobianPattern& jp, const
   }
   upload(dp.j_rp, jp.rp, s); upload(dp.j_ci, jp.ci, s);
   h.j_rp = dp.j_rp.as<int>(); h.j_ci = dp.j_ci.as<int>();
-  upload(dp.col_order, d.col_order, s); upload(dp.level_end, d.level_end, s);
-  upload(dp.col_a_ptr, d.col_a_ptr, s); upload(dp.a_src, d.a_src, s); upload(dp.a_slot, d.a_slot, s);
-  upload(dp.upd_ptr, d.upd_ptr, s); upload(dp.upd_slot, d.upd_slot, s);
-  upload(dp.lp, d.lp, s); upload(dp.li, d.li, s); upload(dp.up, d.up, s); upload(dp.ui, d.ui, s);
   upload(dp.pinv, d.pinv, s); upload(dp.q, d.q, s); upload(dp.prow, d.prow, s); upload(dp.piv_pos, d.piv_pos, s);
-  h.col_order = dp.col_order.as<int>(); h.level_end = dp.level_end.as<int>();
-  h.col_a_ptr = dp.col_a_ptr.as<int>(); h.a_src = dp.a_src.as<int>(); h.a_slot = dp.a_slot.as<int>();
-  h.upd_ptr = dp.upd_ptr.as<int>(); h.upd_slot = dp.upd_slot.as<int>();
-  h.lp = dp.lp.as<int>(); h.li = dp.li.as<int>(); h.up = dp.up.as<int>(); h.ui = dp.ui.as<int>();
-  h.pinv = dp.pinv.as<int>(); h.q = dp.q.as<int>(); h.prow = dp.prow.as<int>();
-  h.piv_pos = dp.piv_pos.as<int>();
+  h.pinv = dp.pinv.as<int>(); h.q = dp.q.as<int>(); h.prow = dp.prow.as<int>(); h.piv_pos = dp.piv_pos.as<int>();
+  // the refactor's chunks and row stream
   h.wstride = d.wstride;
   h.n_chunks = d.n_chunks;
-  upload(dp.ch_track_ptr, d.ch_track_ptr, s); upload(dp.ch_row_ptr, d.ch_row_ptr, s);
-  upload(dp.ch_a_ptr, d.ch_a_ptr, s); upload(dp.ch_fence, d.ch_fence, s); upload(dp.tr_col, d.tr_col, s);
-  upload(dp.ch_a_ws, d.ch_a_ws, s); upload(dp.ch_a_src, d.ch_a_src, s);
-  upload(dp.row_rec, d.row_rec, s); upload(dp.row_lsrc, d.row_lsrc, s);
-  h.ch_track_ptr = dp.ch_track_ptr.as<int>(); h.ch_row_ptr = dp.ch_row_ptr.as<int>();
-  h.ch_a_ptr = dp.ch_a_ptr.as<int>(); h.ch_fence = dp.ch_fence.as<int>(); h.tr_col = dp.tr_col.as<int>();
-  h.ch_a_ws = dp.ch_a_ws.as<int>(); h.ch_a_src = dp.ch_a_src.as<int>();
-  h.row_rec = dp.row_rec.as<int>(); h.row_lsrc = dp.row_lsrc.as<int>();
-  h.fw_levels = d.fw_levels;
-  h.bw_levels = d.bw_levels;
-  upload(dp.fw_order, d.fw_order, s); upload(dp.fw_level_end, d.fw_level_end, s);
-  upload(dp.lr_ptr, d.lr_ptr, s); upload(dp.lr_col, d.lr_col, s); upload(dp.lr_idx, d.lr_idx, s);
-  upload(dp.bw_order, d.bw_order, s); upload(dp.bw_level_end, d.bw_level_end, s);
-  upload(dp.ur_ptr, d.ur_ptr, s); upload(dp.ur_col, d.ur_col, s); upload(dp.ur_idx, d.ur_idx, s);
-  h.fw_order = dp.fw_order.as<int>(); h.fw_level_end = dp.fw_level_end.as<int>();
-  h.lr_ptr = dp.lr_ptr.as<int>(); h.lr_col = dp.lr_col.as<int>(); h.lr_idx = dp.lr_idx.as<int>();
-  h.bw_order = dp.bw_order.as<int>(); h.bw_level_end = dp.bw_level_end.as<int>();
-  h.ur_ptr = dp.ur_ptr.as<int>(); h.ur_col = dp.ur_col.as<int>(); h.ur_idx = dp.ur_idx.as<int>();
-  dp.d_struct.ensure(sizeof(DevPlan));
-  check(hipMemcpyAsync(dp.d_struct.p, &h, sizeof(DevPlan), hipMemcpyHostToDevice, s), "upload plan");
+  h.n_rows = d.n_rows;
+  h.n_a = (int)d.ch_a_src.size();
+  // (every stream padded past its end: the kernels load ahead without bounds checks)
+  const int n_tracks = (int)d.tr_col.size();
+  std::vector<int> row_rl(2 * ((size_t)d.n_rows + kStreamPad) * 64), row_meta((size_t)d.n_rows + kStreamPad, 0);
+  for (size_t e = 0; e < row_rl.size() / 2; e++) {
+    const bool in = e < d.row_rec.size();
+    row_rl[2 * e] = in ? d.row_rec[e] : 0xFFFF;
+    row_rl[2 * e + 1] = in ? d.row_lsrc[e] : 0;
+  }
+  for (int r = 0; r < d.n_rows; r++) row_meta[r] = (d.row_level[r] << kRowLevelShift) | d.row_flags[r];
+  std::vector<int> ch_meta(4 * ((size_t)d.n_chunks + 2));
+  for (int c = 0; c < d.n_chunks + 2; c++) {
+    const bool in = c < d.n_chunks;
+    ch_meta[4 * c] = in ? d.ch_a_ptr[c] : h.n_a;
+    ch_meta[4 * c + 1] = in ? d.ch_a_ptr[c + 1] : h.n_a;
+    ch_meta[4 * c + 2] = in ? d.ch_track_ptr[c] : n_tracks;
+    ch_meta[4 * c + 3] = in ? d.ch_track_ptr[c + 1] - d.ch_track_ptr[c] : 0;
+  }
+  std::vector<int> ch_a_ws(d.ch_a_ws), ch_a_src(d.ch_a_src);
+  ch_a_ws.resize(ch_a_ws.size() + kAPad, 0);
+  ch_a_src.resize(ch_a_src.size() + kAPad, 0);
+  std::vector<int> tr_meta(4 * ((size_t)n_tracks + kTracks), 0), tr_col(d.tr_col);
+  tr_col.resize((size_t)n_tracks + kTracks, 0);
+  for (int t = 0; t < n_tracks; t++) {
+    tr_meta[4 * t] = d.tr_u0[t];
+    tr_meta[4 * t + 1] = d.tr_nu[t];
+    tr_meta[4 * t + 2] = d.tr_l0[t];
+    tr_meta[4 * t + 3] = d.tr_nl[t];
+  }
+  upload(dp.ch_meta, ch_meta, s); upload(dp.ch_a_ws, ch_a_ws, s); upload(dp.ch_a_src, ch_a_src, s);
+  upload(dp.tr_meta, tr_meta, s); upload(dp.tr_col, tr_col, s);
+  upload(dp.row_rl, row_rl, s); upload(dp.row_meta, row_meta, s);
+  h.ch_meta = dp.ch_meta.as<int4>(); h.ch_a_ws = dp.ch_a_ws.as<int>(); h.ch_a_src = dp.ch_a_src.as<int>();
+  h.tr_meta = dp.tr_meta.as<int4>(); h.tr_col = dp.tr_col.as<int>();
+  h.row_rl = dp.row_rl.as<int2>(); h.row_meta = dp.row_meta.as<int>();
+  // the solves' step streams (padded with steps that touch nothing)
+  h.fs_steps = d.fs_steps;
+  h.bs_steps = d.bs_steps;
+  auto pad_steps = [](const std::vector<int>& rec, const std::vector<int>& flags, std::vector<int>& prec,
+                      std::vector<int>& pflags) {
+    prec = rec;
+    pflags = flags;
+    for (int t = 0; t < kStreamPad; t++) {
+      pflags.push_back(0);
+      for (int l = 0; l < 64; l++) { prec.push_back(-1); prec.push_back(0); prec.push_back(-1); prec.push_back(0); }
+    }
+  };
+  std::vector<int> fs_rec, fs_flags, bs_rec, bs_flags;
+  pad_steps(d.fs_rec, d.fs_flags, fs_rec, fs_flags);
+  pad_steps(d.bs_rec, d.bs_flags, bs_rec, bs_flags);
+  upload(dp.fs_rec, fs_rec, s); upload(dp.fs_flags, fs_flags, s);
+  upload(dp.bs_rec, bs_rec, s); upload(dp.bs_flags, bs_flags, s);
+  h.fs_rec = dp.fs_rec.as<int4>(); h.fs_flags = dp.fs_flags.as<int>();
+  h.bs_rec = dp.bs_rec.as<int4>(); h.bs_flags = dp.bs_flags.as<int>();
+  // the staging vectors above are locals: finish the copies before they go
+  check(hipStreamSynchronize(s), "upload plan");
 }
 
 // The symbolic pass: minimum-degree order of the Jacobian pattern and the Gilbert-Peierls pivot sequence of the
@@ -219,7 +255,7 @@ struct gs_net {
 namespace {
 
 struct Layout {
-  size_t vm, va, p, q, f, dx, r, jv, lx, ux, xg, spec, start, stride;
+  size_t vm, va, p, q, f, dx, r, jv, tpq, lx, ux, aval, xg, spec, start, stride;
 };
 
 Layout instance_layout(const DevPlan& h, bool fp32, bool x_lds) {
@@ -235,8 +271,10 @@ Layout instance_layout(const DevPlan& h, bool fp32, bool x_lds) {
   L.dx = o; o += al(8 * (size_t)h.m);
   L.r = o; o += al(8 * (size_t)h.m);
   L.jv = o; o += al(8 * (size_t)h.nnz_j);
+  L.tpq = o; o += al(16 * (size_t)h.nnz_y);
   L.lx = o; o += al(t * (size_t)h.nnz_l);
   L.ux = o; o += al(t * (size_t)h.nnz_u);
+  L.aval = o; o += al(t * ((size_t)h.n_a + kAPad));
   L.xg = o; o += x_lds ? 0 : al(t * (size_t)h.m);
   L.spec = o; o += al(16 * (size_t)h.n);
   L.start = o; o += al(16 * (size_t)h.n);
@@ -342,11 +380,11 @@ double launch_instances(gs_ctx* ctx, DevicePlan& dp, DevMem& inst_buf, DevMem& d
     InstDesc& d = descs[i];
     std::memset(&d, 0, sizeof(d));
     char* b = base + L.stride * i;
-    d.plan = dp.d_struct.as<DevPlan>();
     d.state = dstate + i;
     d.vm = (double*)(b + L.vm); d.va = (double*)(b + L.va); d.p = (double*)(b + L.p); d.q = (double*)(b + L.q);
     d.f = (double*)(b + L.f); d.dx = (double*)(b + L.dx); d.r = (double*)(b + L.r); d.jv = (double*)(b + L.jv);
-    d.lx = b + L.lx; d.ux = b + L.ux; d.xg = run.x_lds ? nullptr : b + L.xg;
+    d.tpq = (double*)(b + L.tpq);
+    d.lx = b + L.lx; d.ux = b + L.ux; d.aval = b + L.aval; d.xg = run.x_lds ? nullptr : b + L.xg;
     const gs_instance& in = inst[i];
     d.n_patch = (y && netw) ? outage_patches(*y, *netw, in.outage_branch, d.patch) : 0;
     // specified injections: the network's, or a load-scaled copy
@@ -387,9 +425,10 @@ double launch_instances(gs_ctx* ctx, DevicePlan& dp, DevMem& inst_buf, DevMem& d
       d.va_start = h.va0;
     }
   }
-  // the network's own specification (load scale 1) lives in the aux buffer
+  // the network's own specification (load scale 1) lives in the aux buffer (the host copy outlives the upload)
+  std::vector<double> aux_host(base_spec.pspec);
   {
-    std::vector<double> sp(base_spec.pspec);
+    std::vector<double>& sp = aux_host;
     sp.insert(sp.end(), base_spec.qspec.begin(), base_spec.qspec.end());
     upload(aux_buf, sp, s);
     for (auto& d : descs)
@@ -413,7 +452,6 @@ double launch_instances(gs_ctx* ctx, DevicePlan& dp, DevMem& inst_buf, DevMem& d
   prm.growth_tol = o.growth_tol;
   prm.max_iterations = o.max_iterations;
   prm.refine_steps = run.fp32 ? o.refine_steps : 0;
-  prm.x_in_lds = run.x_lds;
   prm.load_start = start_override ? 1 : 0;
   hipEvent_t e0, e1;
   check(hipEventCreate(&e0), "event");
@@ -423,7 +461,8 @@ double launch_instances(gs_ctx* ctx, DevicePlan& dp, DevMem& inst_buf, DevMem& d
   if (!plain) {
     prm.steps_this_launch = 1 << 30;
     prm.init = init ? 1 : 0;
-    check(launch_nr_batch(run.fp32, run.x_lds, desc_buf.as<InstDesc>(), count, prm, run.lds_bytes, s), "nr kernel");
+    check(launch_nr_batch(run.fp32, run.x_lds, desc_buf.as<InstDesc>(), h, count, prm, run.lds_bytes, s),
+          "nr kernel");
     nl = 1;
   } else {
     // one instance per launch, one Newton step per launch, the host reading the verdict after each
@@ -435,7 +474,8 @@ double launch_instances(gs_ctx* ctx, DevicePlan& dp, DevMem& inst_buf, DevMem& d
         prm.init = (first && init) ? 1 : 0;
         prm.load_start = first ? load_start : 0;
         first = false;
-        check(launch_nr_batch(0, run.x_lds, desc_buf.as<InstDesc>() + i, 1, prm, run.lds_bytes, s), "plain kernel");
+        check(launch_nr_batch(0, run.x_lds, desc_buf.as<InstDesc>() + i, h, 1, prm, run.lds_bytes, s),
+              "plain kernel");
         nl++;
         InstState hs;
         check(hipMemcpyAsync(&hs, dstate + i, sizeof(InstState), hipMemcpyDeviceToHost, s), "plain state");
@@ -585,18 +625,18 @@ gs_status seam_solve_one(gs_ctx* ctx, int n, const int* rowptr, const int* colid
   jp.ci = ci;
   std::vector<double> xt(n);
   auto run_device = [&](SeamPlan& sp) -> int {
-    DevMem a, bb, xx, lx, ux, xs, rf, stt, desc;
+    DevMem a, bb, xx, lx, ux, xs, rf, av, stt, desc;
     a.ensure(8 * (size_t)nnz); bb.ensure(8 * (size_t)n); xx.ensure(8 * (size_t)n);
     lx.ensure(8 * (size_t)sp.dev.h.nnz_l); ux.ensure(8 * (size_t)sp.dev.h.nnz_u); xs.ensure(8 * (size_t)n);
-    rf.ensure(8 * (size_t)n); stt.ensure(4);
+    rf.ensure(8 * (size_t)n); av.ensure(8 * ((size_t)sp.dev.h.n_a + kAPad)); stt.ensure(4);
     check(hipMemcpyAsync(a.p, val, 8 * (size_t)nnz, hipMemcpyHostToDevice, s), "seam a");
     check(hipMemcpyAsync(bb.p, b, 8 * (size_t)n, hipMemcpyHostToDevice, s), "seam b");
-    SeamDesc d{sp.dev.d_struct.as<DevPlan>(), a.as<double>(), bb.as<double>(), xx.as<double>(), lx.as<double>(),
-               ux.as<double>(), xs.as<double>(), rf.as<double>(), stt.as<int>(), sp.dev.piv_pos.as<int>()};
+    SeamDesc d{a.as<double>(), bb.as<double>(), xx.as<double>(), lx.as<double>(), ux.as<double>(), xs.as<double>(),
+               rf.as<double>(), av.as<double>(), stt.as<int>()};
     desc.ensure(sizeof(SeamDesc));
     check(hipMemcpyAsync(desc.p, &d, sizeof(SeamDesc), hipMemcpyHostToDevice, s), "seam desc");
     size_t lds = 8 * (size_t)kTracks * (size_t)sp.dev.h.wstride;
-    check(launch_seam(desc.as<SeamDesc>(), 1, lds, s), "seam kernel");
+    check(launch_seam(desc.as<SeamDesc>(), sp.dev.h, 1, lds, s), "seam kernel");
     int status = 1;
     check(hipMemcpyAsync(&status, stt.p, 4, hipMemcpyDeviceToHost, s), "seam status");
     check(hipMemcpyAsync(xt.data(), xx.p, 8 * (size_t)n, hipMemcpyDeviceToHost, s), "seam x");

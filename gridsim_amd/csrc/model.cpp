@@ -566,7 +566,25 @@ void compile_plan(const Plan& plan, DevicePlanHost& d) {
         }
         (void)col;
       }
+      for (size_t g = 0; g < c1 - c0; g++) {
+        int k = d.tr_col[d.tr_col.size() - (c1 - c0) + g];
+        d.tr_u0.push_back(plan.up[k]);
+        d.tr_nu.push_back(plan.up[k + 1] - plan.up[k]);
+        d.tr_l0.push_back(plan.lp[k]);
+        d.tr_nl.push_back(plan.lp[k + 1] - plan.lp[k]);
+      }
+      const bool level_last = c1 == cols.size();
+      if (nrows == 0) {  // a chunk without updates still gets one (empty) row to carry its events
+        for (int lane = 0; lane < 64; lane++) { d.row_rec.push_back(0xFFFF); d.row_lsrc.push_back(0); }
+        d.row_flags.push_back(DevicePlanHost::kRowBegin | DevicePlanHost::kRowEnd |
+                              (level_last ? DevicePlanHost::kRowLevelEnd : 0));
+        d.row_level.push_back(lv);
+      }
       for (size_t rho = 0; rho < nrows; rho++) {
+        d.row_flags.push_back((rho == 0 ? DevicePlanHost::kRowBegin : 0) |
+                              (rho + 1 == nrows ? DevicePlanHost::kRowEnd : 0) |
+                              (rho + 1 == nrows && level_last ? DevicePlanHost::kRowLevelEnd : 0));
+        d.row_level.push_back(lv);
         for (int lane = 0; lane < 64; lane++) {
           size_t g = lane / TL;
           int idx = lane % TL;
@@ -588,13 +606,15 @@ void compile_plan(const Plan& plan, DevicePlanHost& d) {
         }
       }
       d.ch_track_ptr.push_back((int)d.tr_col.size());
-      d.ch_row_ptr.push_back(d.ch_row_ptr.back() + (int)nrows);
+      d.ch_row_ptr.push_back(d.ch_row_ptr.back() + (int)std::max<size_t>(nrows, 1));
       d.ch_a_ptr.push_back((int)d.ch_a_ws.size());
       d.ch_fence.push_back(c1 == cols.size() ? 1 : 0);
     }
   }
   d.n_chunks = (int)d.ch_fence.size();
   d.n_rows = d.ch_row_ptr.back();
+  d.j2a.assign(plan.a_src.empty() ? 0 : (size_t)(*std::max_element(plan.a_src.begin(), plan.a_src.end()) + 1), -1);
+  for (size_t e = 0; e < d.ch_a_src.size(); e++) d.j2a[d.ch_a_src[e]] = (int)e;
   // Forward solve rows: L(i, j) for j < i, columns ascending; level(i) = 1 + max level(j).
   {
     vector<vector<std::pair<int, int>>> rows(m);
@@ -647,6 +667,43 @@ void compile_plan(const Plan& plan, DevicePlanHost& d) {
     for (int i = 0; i < m; i++) d.bw_order[cnt[lvl[i]]++] = i;
     d.bw_levels = nlv;
   }
+  // Solve step streams: per level, rows by decreasing entry count, 64 rows per pass, one step per entry index.
+  auto build_stream = [&](const vector<int>& order, const vector<int>& level_end, int nlv, const vector<int>& rptr,
+                          const vector<int>& rcol, const vector<int>& ridx, bool back, vector<int>& rec,
+                          vector<int>& flags) {
+    int b0 = 0;
+    for (int lv = 0; lv < nlv; lv++) {
+      int b1 = level_end[lv];
+      vector<int> rows;
+      for (int t = b0; t < b1; t++) {
+        int i = order[t];
+        if (back || rptr[i + 1] > rptr[i]) rows.push_back(i);  // forward rows without entries need no work
+      }
+      std::stable_sort(rows.begin(), rows.end(),
+                       [&](int a, int b) { return rptr[a + 1] - rptr[a] > rptr[b + 1] - rptr[b]; });
+      for (size_t p0 = 0; p0 < rows.size(); p0 += 64) {
+        size_t p1 = std::min(rows.size(), p0 + 64);
+        int len = std::max(1, rptr[rows[p0] + 1] - rptr[rows[p0]]);
+        for (int e = 0; e < len; e++) {
+          flags.push_back((e == 0 ? DevicePlanHost::kStepBegin : 0) | (e + 1 == len ? DevicePlanHost::kStepEnd : 0));
+          for (int lane = 0; lane < 64; lane++) {
+            int row = -1, col = -1, idx = 0, didx = 0;
+            if (p0 + lane < p1) {
+              row = rows[p0 + lane];
+              if (rptr[row] + e < rptr[row + 1]) { col = rcol[rptr[row] + e]; idx = ridx[rptr[row] + e]; }
+              if (back) didx = plan.up[row + 1] - 1;
+            }
+            rec.push_back(col); rec.push_back(idx); rec.push_back(row); rec.push_back(didx);
+          }
+        }
+      }
+      b0 = b1;
+    }
+  };
+  build_stream(d.fw_order, d.fw_level_end, d.fw_levels, d.lr_ptr, d.lr_col, d.lr_idx, false, d.fs_rec, d.fs_flags);
+  build_stream(d.bw_order, d.bw_level_end, d.bw_levels, d.ur_ptr, d.ur_col, d.ur_idx, true, d.bs_rec, d.bs_flags);
+  d.fs_steps = (int)d.fs_flags.size();
+  d.bs_steps = (int)d.bs_flags.size();
   // For the exact pivot check: how many candidates (rows not yet pivotal at step k) precede the pivot in reach order.
   d.piv_pos.assign(m, 0);
   for (int k = 0; k < m; k++) {
@@ -670,75 +727,98 @@ bool emulate_device_lu(const DevicePlanHost& d, int nnz_j, const double* jv, con
   const int wsz = G * d.wstride;
   vector<double> ws(wsz), lx(d.nnz_l, std::nan("")), ux(d.nnz_u, std::nan(""));
   auto bad = [&](const std::string& w) { why = w; return false; };
-  for (int c = 0; c < d.n_chunks; c++) {
-    int t0 = d.ch_track_ptr[c], ntr = d.ch_track_ptr[c + 1] - t0;
-    if (ntr < 1 || ntr > G) return bad("chunk track count");
-    std::fill(ws.begin(), ws.end(), 0.0);
-    for (int e = d.ch_a_ptr[c]; e < d.ch_a_ptr[c + 1]; e++) {
-      if (d.ch_a_ws[e] < 0 || d.ch_a_ws[e] >= wsz) return bad("A workspace index");
-      if (d.ch_a_src[e] < 0 || d.ch_a_src[e] >= nnz_j) return bad("A source index");
-      ws[d.ch_a_ws[e]] = jv[d.ch_a_src[e]];
-    }
-    for (int rho = d.ch_row_ptr[c]; rho < d.ch_row_ptr[c + 1]; rho++) {
-      for (int lane = 0; lane < 64; lane++) {
-        int rec = d.row_rec[(size_t)rho * 64 + lane];
-        int dst = rec & 0xFFFF, xj = rec >> 16;
-        if (dst == 0xFFFF) continue;
-        int lsrc = d.row_lsrc[(size_t)rho * 64 + lane];
-        if (dst >= wsz || xj >= wsz || dst / d.wstride != lane / TL || xj / d.wstride != lane / TL)
-          return bad("row record outside its track's workspace");
-        if (lsrc < 0 || lsrc >= d.nnz_l || std::isnan(lx[lsrc])) return bad("row reads an L value not yet written");
-        ws[dst] = ws[dst] - lx[lsrc] * ws[xj];
+  // A values in record order, as the mismatch kernel writes them through j2a
+  const int n_a = (int)d.ch_a_src.size();
+  vector<double> aval(n_a, std::nan(""));
+  for (int s = 0; s < nnz_j && s < (int)d.j2a.size(); s++) {
+    int e = d.j2a[s];
+    if (e < -1 || e >= n_a) return bad("j2a index");
+    if (e >= 0) aval[e] = jv[s];
+  }
+  if ((int)d.row_flags.size() != d.n_rows || (int)d.row_rec.size() != 64 * d.n_rows) return bad("row stream size");
+  int chunk = -1, cur_level = 0;
+  for (int r = 0; r < d.n_rows; r++) {
+    const int fl = d.row_flags[r];
+    if (d.row_level[r] != cur_level) return bad("row stream level out of order");
+    if (fl & DevicePlanHost::kRowBegin) {
+      chunk++;
+      if (chunk >= d.n_chunks) return bad("more chunk begins than chunks");
+      int ntr = d.ch_track_ptr[chunk + 1] - d.ch_track_ptr[chunk];
+      if (ntr < 1 || ntr > G) return bad("chunk track count");
+      std::fill(ws.begin(), ws.end(), 0.0);
+      for (int e = d.ch_a_ptr[chunk]; e < d.ch_a_ptr[chunk + 1]; e++) {
+        if (d.ch_a_ws[e] < 0 || d.ch_a_ws[e] >= wsz) return bad("A workspace index");
+        if (std::isnan(aval[e])) return bad("A value not written by the Jacobian fill");
+        ws[d.ch_a_ws[e]] = aval[e];
       }
     }
-    for (int g = 0; g < ntr; g++) {
-      int k = d.tr_col[t0 + g];
-      int u0 = d.up[k], nu = d.up[k + 1] - u0, l0 = d.lp[k], nl = d.lp[k + 1] - l0;
-      if (nu + nl - 1 > d.wstride) return bad("column larger than the workspace stride");
-      const double* w = &ws[g * d.wstride];
-      double pivot = w[nu - 1];
-      if (pivot == 0.0) return bad("zero pivot");
-      for (int t = 0; t < nu; t++) ux[u0 + t] = w[t];
-      lx[l0] = 1.0;
-      for (int t = 1; t < nl; t++) lx[l0 + t] = w[nu - 1 + t] / pivot;
+    for (int lane = 0; lane < 64; lane++) {
+      int rec = d.row_rec[(size_t)r * 64 + lane];
+      int dst = rec & 0xFFFF, xj = rec >> 16;
+      if (dst == 0xFFFF) continue;
+      int lsrc = d.row_lsrc[(size_t)r * 64 + lane];
+      if (dst >= wsz || xj >= wsz || dst / d.wstride != lane / TL || xj / d.wstride != lane / TL)
+        return bad("row record outside its track's workspace");
+      if (lsrc < 0 || lsrc >= d.nnz_l || std::isnan(lx[lsrc])) return bad("row reads an L value not yet written");
+      ws[dst] = ws[dst] - lx[lsrc] * ws[xj];
     }
+    if (fl & DevicePlanHost::kRowEnd) {
+      int t0 = d.ch_track_ptr[chunk], ntr = d.ch_track_ptr[chunk + 1] - t0;
+      for (int g = 0; g < ntr; g++) {
+        int u0 = d.tr_u0[t0 + g], nu = d.tr_nu[t0 + g], l0 = d.tr_l0[t0 + g], nl = d.tr_nl[t0 + g];
+        int k = d.tr_col[t0 + g];
+        if (u0 != d.up[k] || nu != d.up[k + 1] - d.up[k] || l0 != d.lp[k] || nl != d.lp[k + 1] - d.lp[k])
+          return bad("track metadata");
+        if (nu + nl - 1 > d.wstride) return bad("column larger than the workspace stride");
+        const double* w = &ws[g * d.wstride];
+        double pivot = w[nu - 1];
+        if (pivot == 0.0) return bad("zero pivot");
+        for (int t = 0; t < nu; t++) ux[u0 + t] = w[t];
+        lx[l0] = 1.0;
+        for (int t = 1; t < nl; t++) lx[l0 + t] = w[nu - 1 + t] / pivot;
+      }
+    }
+    if (fl & DevicePlanHost::kRowLevelEnd) cur_level++;
   }
+  if (chunk + 1 != d.n_chunks) return bad("chunk count");
   vector<double> x(m, std::nan(""));
   for (int r = 0; r < m; r++) x[d.pinv[r]] = rhs[r];
-  vector<char> done(m, 0);
-  int b0 = 0;
-  for (int lv = 0; lv < d.fw_levels; lv++) {
-    int b1 = d.fw_level_end[lv];
-    for (int idx = b0; idx < b1; idx++) {
-      int i = d.fw_order[idx];
-      double s = x[i];
-      for (int e = d.lr_ptr[i]; e < d.lr_ptr[i + 1]; e++) {
-        if (!done[d.lr_col[e]]) return bad("forward solve reads a row of a later level");
-        s = s - lx[d.lr_idx[e]] * x[d.lr_col[e]];
+  // the step streams, with the rows a step may read checked against the rows already final
+  auto run_stream = [&](const vector<int>& rec, const vector<int>& flags, const vector<double>& vals, bool back,
+                        vector<char>& final_rows) -> bool {
+    vector<double> acc(64, 0.0);
+    vector<int> row(64, -1);
+    for (size_t t = 0; t < flags.size(); t++) {
+      for (int lane = 0; lane < 64; lane++) {
+        const int* r4 = &rec[(t * 64 + lane) * 4];
+        if (flags[t] & DevicePlanHost::kStepBegin) {
+          row[lane] = r4[2];
+          acc[lane] = row[lane] >= 0 ? x[row[lane]] : 0.0;
+        }
+        if (r4[0] >= 0) {
+          if (r4[0] >= m || !final_rows[r4[0]]) { why = "stream reads a row not yet final"; return false; }
+          if (r4[1] < 0 || r4[1] >= (int)vals.size()) { why = "stream value index"; return false; }
+          acc[lane] = acc[lane] - vals[r4[1]] * x[r4[0]];
+        }
+        if ((flags[t] & DevicePlanHost::kStepEnd) && row[lane] >= 0) {
+          x[row[lane]] = back ? acc[lane] / vals[r4[3]] : acc[lane];
+        }
       }
-      x[i] = s;
+      if (flags[t] & DevicePlanHost::kStepEnd)
+        for (int lane = 0; lane < 64; lane++) if (row[lane] >= 0) final_rows[row[lane]] = 1;
     }
-    for (int idx = b0; idx < b1; idx++) done[d.fw_order[idx]] = 1;
-    b0 = b1;
+    return true;
+  };
+  {
+    // forward: rows without L entries are final from the start; back: a row is final once divided
+    vector<char> fin(m, 0);
+    for (int i = 0; i < m; i++) fin[i] = d.lr_ptr[i + 1] == d.lr_ptr[i];
+    if (!run_stream(d.fs_rec, d.fs_flags, lx, false, fin)) return false;
+    vector<char> fin_b(m, 0);
+    if (!run_stream(d.bs_rec, d.bs_flags, ux, true, fin_b)) return false;
+    for (int k = 0; k < m; k++) out[d.q[k]] = x[k];
+    return true;
   }
-  std::fill(done.begin(), done.end(), 0);
-  b0 = 0;
-  for (int lv = 0; lv < d.bw_levels; lv++) {
-    int b1 = d.bw_level_end[lv];
-    for (int idx = b0; idx < b1; idx++) {
-      int i = d.bw_order[idx];
-      double s = x[i];
-      for (int e = d.ur_ptr[i]; e < d.ur_ptr[i + 1]; e++) {
-        if (!done[d.ur_col[e]]) return bad("back solve reads a row of a later level");
-        s = s - ux[d.ur_idx[e]] * x[d.ur_col[e]];
-      }
-      x[i] = s / ux[d.up[i + 1] - 1];
-    }
-    for (int idx = b0; idx < b1; idx++) done[d.bw_order[idx]] = 1;
-    b0 = b1;
-  }
-  for (int k = 0; k < m; k++) out[d.q[k]] = x[k];
-  return true;
 }
 
 }  // namespace gs

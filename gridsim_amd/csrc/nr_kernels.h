@@ -2,36 +2,49 @@
 #pragma once
 #include <cstdint>
 
+#include <hip/hip_runtime.h>
+
 namespace gs {
 
-// One network pattern + symbolic plan, resident in HBM. Every array is a device pointer.
+// One network pattern + symbolic plan, resident in HBM. Every array is a device pointer. The struct itself is passed
+// to the kernels by value (kernel argument), so its fields sit in scalar registers and its pointers are known to
+// address global memory.
 struct DevPlan {
   int32_t n, m, nnz_y, nnz_j, nnz_l, nnz_u, max_col, n_levels;
   // network pattern
-  const int32_t *y_rp, *y_ci, *y_diag;
+  const int32_t *y_rp, *y_ci, *y_row, *y_diag;
   const double *y_g, *y_b;            // base Y values (instances patch at most kMaxPatch cells)
-  const int32_t* slots;               // [4 nnz_y] J CSR position of H, N, M, L of each Y cell, or -1
+  const int4* slots;                  // per Y cell: J CSR position of its H, N, M, L entries, or -1
+  const int4* aslots;                 // per Y cell: refactor record (aval index) of the same four entries, or -1
   const int32_t *ppos, *qpos;         // equation numbering
   const int32_t *j_rp, *j_ci;         // Jacobian CSR pattern (for the FP64 refinement residual)
   // base bus specification
   const double *vm0, *va0;            // start point (BuildBusSpec under the upload options)
   double base_mva;
-  // plan (see model.h DevicePlanHost)
-  const int32_t *col_order, *level_end;     // columns sorted by dependency level; per level one-past-end index
-  const int32_t *col_a_ptr, *a_src, *a_slot;
-  const int32_t *upd_ptr, *upd_slot;
-  const int32_t *lp, *li, *up, *ui;
+  // plan (model.h DevicePlanHost)
   const int32_t *pinv, *q, *prow;     // row permutation, column permutation, pivot row of each step
-  // level chunks of the refactor (model.h DevicePlanHost)
-  int32_t wstride, n_chunks;
-  const int32_t *ch_track_ptr, *ch_row_ptr, *ch_a_ptr, *ch_fence, *tr_col, *ch_a_ws, *ch_a_src;
-  const int32_t *row_rec, *row_lsrc;
-  // level-scheduled triangular solves
-  int32_t fw_levels, bw_levels;
-  const int32_t *fw_order, *fw_level_end, *lr_ptr, *lr_col, *lr_idx;
-  const int32_t *bw_order, *bw_level_end, *ur_ptr, *ur_col, *ur_idx;
   const int32_t* piv_pos;             // per step: candidates before the pivot in reach order
+  // The refactor: level chunks of up to kTracks columns, run as one stream of rows of 64 records. Every stream is
+  // padded past its end (rows, steps, chunk and track metadata, A entries) so the kernels can load ahead without
+  // bounds checks.
+  int32_t wstride, n_chunks, n_rows, n_a;
+  const int4* ch_meta;                // per chunk: (first A entry, end of its A entries, first track, tracks)
+  const int32_t *ch_a_ws, *ch_a_src;  // per A entry: workspace slot, J CSR position
+  const int4* tr_meta;                // per track: (first U entry, U count, first L entry, L count)
+  const int32_t* tr_col;              // per track: its pivot step
+  const int2* row_rl;                 // per row and lane: (record, L value index)
+  const int32_t* row_meta;            // per row: level * 8 | flags (kRowBegin, kRowEnd, kRowLevelEnd)
+  // triangular-solve step streams: per lane and step (col, value index, row, diagonal index)
+  int32_t fs_steps, bs_steps;
+  const int4 *fs_rec, *bs_rec;
+  const int32_t *fs_flags, *bs_flags;
 };
+
+constexpr int kRowBegin = 1, kRowEnd = 2, kRowLevelEnd = 4, kRowLevelShift = 3;
+constexpr int kStepBegin = 1, kStepEnd = 2;
+// load-ahead distance of the row and step streams, and the padding each stream carries
+constexpr int kStreamPad = 8;
+constexpr int kAPad = 128;            // A entries a lane may load ahead per chunk (two per lane)
 
 constexpr int kTracks = 8, kTrackLanes = 8;
 
@@ -56,7 +69,6 @@ struct InstState {
 };
 
 struct InstDesc {
-  const DevPlan* plan;
   InstState* state;
   const double *pspec, *qspec;        // specified injections: the network's, or this instance's load-scaled copy
   const double *vm_start, *va_start;  // start point: the network's, or this instance's own
@@ -65,28 +77,28 @@ struct InstDesc {
   // per-instance buffers (device)
   double *vm, *va, *p, *q;
   double *f, *dx, *r, *jv;            // f, dx, residual, FP64 Jacobian values (CSR order)
+  double* tpq;                        // per Y cell: the cell's two injection terms (P, Q)
   void *lx, *ux;                      // L / U values in the LU precision
+  void* aval;                         // Jacobian values in the refactor's record order, LU precision
   void* xg;                           // global triangular-solve workspace (used when LDS cannot hold x)
 };
 
 // One CSR system of the sparse-solver seam, solved on a plan made from its own factorisation.
 struct SeamDesc {
-  const DevPlan* plan;
   const double* a;         // CSR values
   const double* b;
   double* x;
-  double *lx, *ux, *xs, *rowfloor;
+  double *lx, *ux, *xs, *rowfloor, *aval;
   int32_t* status;         // 0 solved, 1 refused (a pivot would move, or is at or below its row floor)
-  const int32_t* piv_pos;  // per step: candidates (in reach order) before the pivot
 };
 
 struct NrParams {
   double tolerance, diverge_limit, growth_tol;
   int32_t max_iterations, refine_steps;
   int32_t steps_this_launch;          // Newton steps one launch may take (the plain path takes 1)
-  int32_t x_in_lds;                   // 1: triangular-solve workspace in LDS
   int32_t init;                       // 1: reset the instance's counters before stepping
   int32_t load_start;                 // 1: copy the start point into the iterate before stepping
+  int32_t pad;
 };
 
 }  // namespace gs

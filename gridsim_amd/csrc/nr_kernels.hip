@@ -1,8 +1,9 @@
 // Batched sparse Newton-Raphson on gfx950: one wavefront per instance runs the whole Newton loop.
 //
 // Per Newton step, for each instance (reference path: SparseNewtonRaphsonPowerFlow.RunNewton, :213-262):
-//   1. injections, mismatch and Jacobian in one pass over the Y-bus (FDLIBM sincos once per Y cell, FP64):
-//      ComputeInjections :269-289, residual :218-233, RefillJacobian :342-380;
+//   1. injections, mismatch and Jacobian (FP64, FDLIBM sincos once per Y cell): ComputeInjections :269-289, residual
+//      :218-233, RefillJacobian :342-380. First every Y cell's two injection terms and its off-diagonal Jacobian
+//      entries, one cell per lane; then per bus the ordered sums, the diagonal entries and the mismatch;
 //   2. the verdicts: converged / diverged / iteration cap (:235-238, NewtonShell.DivergedBeyondLimit :177-178);
 //   3. Gilbert-Peierls refactor on the plan (SparseLuFactorization.Refactor :246-306), level-parallel: the columns of
 //      one dependency level are independent, so up to 8 of them are factored side by side (8 lanes each), each in its
@@ -13,6 +14,11 @@
 //      r = f - J dx on the FP64 Jacobian;
 //   5. the damped update (NewtonShell.ApplyStep :128-136).
 // Compiled with -ffp-contract=off: in FP64 every value is the reference's arithmetic, operation for operation.
+//
+// Every phase is a chain of dependent memory reads (pattern, then the value it points at), so the loops keep the
+// reads for later iterations in flight in registers (software pipelining) instead of waiting on each in turn. All
+// global pointers are cast to the global address space: generic (flat) loads would also count against the LDS wait
+// counter, and every LDS wait would then drain the reads in flight.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -21,32 +27,47 @@
 #include "gridsim_hip.h"
 #include "nr_kernels.h"
 
+#define GLOBAL __attribute__((address_space(1)))
+#define LDSP __attribute__((address_space(3)))
+
 namespace gs {
 
 constexpr int kWave = 64;
 
-__device__ inline double nan_max(double a, double b) {
+// native vector types (HIP's vector classes do not carry an address space)
+typedef int v2i __attribute__((ext_vector_type(2)));
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef double v2d __attribute__((ext_vector_type(2)));
+
+template <typename T>
+__device__ __forceinline__ GLOBAL T* glob(T* p) { return (GLOBAL T*)p; }
+template <typename V>
+__device__ __forceinline__ GLOBAL const V* globv(const void* p) { return (GLOBAL const V*)p; }
+template <typename T>
+__device__ __forceinline__ LDSP T* lds(T* p) { return (LDSP T*)p; }
+
+__device__ __forceinline__ double nan_max(double a, double b) {
   if (a != a || b != b) return __builtin_nan("");
   return a > b ? a : b;
 }
 
-__device__ inline double wave_max(double v) {
+__device__ __forceinline__ double wave_max(double v) {
   for (int off = kWave / 2; off > 0; off >>= 1) v = nan_max(v, __shfl_xor(v, off, kWave));
   return v;
 }
 
 // maximum over the kTrackLanes lanes of one track (aligned groups of 8)
-__device__ inline double track_max(double v) {
+__device__ __forceinline__ double track_max(double v) {
   for (int off = kTrackLanes / 2; off > 0; off >>= 1) v = nan_max(v, __shfl_xor(v, off, kWave));
   return v;
 }
 
-__device__ inline int track_any(int v) {
+__device__ __forceinline__ int track_any(int v) {
   for (int off = kTrackLanes / 2; off > 0; off >>= 1) v |= __shfl_xor(v, off, kWave);
   return v;
 }
 
-__device__ inline int wave_any(int v) { return __any(v) ? 1 : 0; }
+__device__ __forceinline__ int wave_any(int v) { return __any(v) ? 1 : 0; }
 
 // Ordering of LDS traffic between the lanes of one wavefront (the hardware executes a wave's LDS instructions in
 // order; this stops the compiler from moving them).
@@ -54,61 +75,145 @@ __device__ inline int wave_any(int v) { return __any(v) ? 1 : 0; }
 // Global-memory writes of this wavefront made visible to its own later reads (work-group scope: same CU, shared L1).
 #define WG_SYNC() __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup")
 
+// One instance's buffers, as global-memory pointers.
+template <typename T>
+struct Inst {
+  GLOBAL const double *pspec, *qspec, *vm_start, *va_start;
+  GLOBAL double *vm, *va, *p, *q, *f, *dx, *r, *jv;
+  GLOBAL v2d* tpq;
+  GLOBAL T *lx, *ux, *aval, *xg;
+  GLOBAL const Patch* patch;
+  int n_patch;
+  int pidx[kMaxPatch];   // patched Y cells (-1: unused)
+};
+
+// Y value of cell k: the base value, or this instance's patch of it.
+template <typename T>
+__device__ __forceinline__ void y_value(const DevPlan& P, const Inst<T>& I, int k, double g0, double b0, double* g,
+                                        double* b) {
+  *g = g0;
+  *b = b0;
+#pragma unroll
+  for (int t = 0; t < kMaxPatch; t++)
+    if (I.pidx[t] == k) { *g = I.patch[t].g; *b = I.patch[t].b; }
+}
+
+template <typename T>
+__device__ __forceinline__ void put_j(const Inst<T>& I, int s, int a, double v) {
+  if (s >= 0) I.jv[s] = v;
+  if (a >= 0) I.aval[a] = (T)v;
+}
+
 // ------------------------------------------------------------------------------------------------ mismatch + J
-// Returns max |f| (NaN-propagating) over the wave; sets *bad when a Jacobian value is not finite.
-__device__ double mismatch_jacobian(const DevPlan& P, const InstDesc& d, int lane, int* bad) {
-  const int n = P.n;
-  double lmax = 0.0;
+// Returns max |f| (NaN-propagating) over the wave; sets *bad when a Jacobian value is not finite. Each Jacobian value
+// goes to the FP64 CSR copy (the refinement residual) and, in the LU precision, to its refactor record (aval).
+template <typename T>
+__device__ double mismatch_jacobian(const DevPlan& P, const Inst<T>& I, int lane, int* bad) {
+  GLOBAL const int* y_row = glob(P.y_row);
+  GLOBAL const int* y_ci = glob(P.y_ci);
+  GLOBAL const double* y_g = glob(P.y_g);
+  GLOBAL const double* y_b = glob(P.y_b);
+  GLOBAL const v4i* slots = globv<v4i>(P.slots);
+  GLOBAL const v4i* aslots = globv<v4i>(P.aslots);
+  const int nnz = P.nnz_y;
   int lbad = 0;
-  for (int i = lane; i < n; i += kWave) {
-    const double vai = d.va[i], vmi = d.vm[i];
-    double pi = 0.0, qi = 0.0;
-    const int k0 = P.y_rp[i], k1 = P.y_rp[i + 1];
-    for (int k = k0; k < k1; k++) {
-      const int j = P.y_ci[k];
-      double g = P.y_g[k], b = P.y_b[k];
-      for (int t = 0; t < d.n_patch; t++)
-        if (d.patch[t].idx == k) { g = d.patch[t].g; b = d.patch[t].b; }
-      double sn, cs;
-      gs_fdlibm::sincos(vai - d.va[j], &sn, &cs);
-      const double vmj = d.vm[j];
-      pi += vmj * (g * cs + b * sn);
-      qi += vmj * (g * sn - b * cs);
-      if (j != i) {
-        const int* s = &P.slots[4 * (size_t)k];
-        double h = vmi * vmj * (g * sn - b * cs);
-        double nn = vmi * (g * cs + b * sn);
-        double mm = -vmi * vmj * (g * cs + b * sn);
-        double l = vmi * (g * sn - b * cs);
-        if (s[0] >= 0) d.jv[s[0]] = h;
-        if (s[1] >= 0) d.jv[s[1]] = nn;
-        if (s[2] >= 0) d.jv[s[2]] = mm;
-        if (s[3] >= 0) d.jv[s[3]] = l;
-        lbad |= !(isfinite(h) && isfinite(nn) && isfinite(mm) && isfinite(l));
+  // cells: stage 0 holds the pattern of cell t + 2 (lane-strided), stage 1 the gathered state of cell t + 1
+  {
+    const int iters = (nnz + kWave - 1) / kWave;
+    int i1 = 0, j1 = 0, i2 = 0, j2 = 0;
+    double g1 = 0, b1 = 0, g2 = 0, b2 = 0;
+    v4i s1 = (v4i)(-1), a1 = s1, s2 = s1, a2 = s1;
+    double vai1 = 0, vaj1 = 0, vmi1 = 0, vmj1 = 0;
+    auto load_pattern = [&](int t, int& i, int& j, double& g, double& b, v4i& s, v4i& a) {
+      const int k = lane + t * kWave;
+      const int kk = k < nnz ? k : 0;
+      i = y_row[kk]; j = y_ci[kk]; g = y_g[kk]; b = y_b[kk]; s = slots[kk]; a = aslots[kk];
+    };
+    // (cells past the end read cell 0: every load is in bounds and unconditional, so the waits stay partial)
+    load_pattern(0, i1, j1, g1, b1, s1, a1);
+    load_pattern(1, i2, j2, g2, b2, s2, a2);
+    vai1 = I.va[i1]; vaj1 = I.va[j1]; vmi1 = I.vm[i1]; vmj1 = I.vm[j1];
+    for (int t = 0; t < iters; t++) {
+      // this cell
+      const int k = lane + t * kWave;
+      const int i = i1, j = j1;
+      const double g0 = g1, b0 = b1, vai = vai1, vaj = vaj1, vmi = vmi1, vmj = vmj1;
+      const v4i s = s1, a = a1;
+      // advance the pipeline: gather for cell t + 1, pattern for cell t + 2
+      i1 = i2; j1 = j2; g1 = g2; b1 = b2; s1 = s2; a1 = a2;
+      vai1 = I.va[i1]; vaj1 = I.va[j1]; vmi1 = I.vm[i1]; vmj1 = I.vm[j1];
+      load_pattern(t + 2, i2, j2, g2, b2, s2, a2);
+      if (k < nnz) {
+        double g, b;
+        y_value(P, I, k, g0, b0, &g, &b);
+        double sn, cs;
+        gs_fdlibm::sincos(vai - vaj, &sn, &cs);
+        const double tp = vmj * (g * cs + b * sn);
+        const double tq = vmj * (g * sn - b * cs);
+        I.tpq[k] = (v2d){tp, tq};
+        if (j != i) {
+          const double h = vmi * vmj * (g * sn - b * cs);
+          const double nn = vmi * (g * cs + b * sn);
+          const double mm = -vmi * vmj * (g * cs + b * sn);
+          const double l = vmi * (g * sn - b * cs);
+          put_j(I, s.x, a.x, h);
+          put_j(I, s.y, a.y, nn);
+          put_j(I, s.z, a.z, mm);
+          put_j(I, s.w, a.w, l);
+          lbad |= !(isfinite(h) && isfinite(nn) && isfinite(mm) && isfinite(l));
+        }
       }
     }
-    const double P_i = vmi * pi, Q_i = vmi * qi;
-    d.p[i] = P_i;
-    d.q[i] = Q_i;
-    {
-      const int k = P.y_diag[i];
-      double g = P.y_g[k], b = P.y_b[k];
-      for (int t = 0; t < d.n_patch; t++)
-        if (d.patch[t].idx == k) { g = d.patch[t].g; b = d.patch[t].b; }
-      const int* s = &P.slots[4 * (size_t)k];
-      double h = -Q_i - b * vmi * vmi;
-      double nn = P_i / vmi + g * vmi;
-      double mm = P_i - g * vmi * vmi;
-      double l = Q_i / vmi - b * vmi;
-      if (s[0] >= 0) d.jv[s[0]] = h;
-      if (s[1] >= 0) d.jv[s[1]] = nn;
-      if (s[2] >= 0) d.jv[s[2]] = mm;
-      if (s[3] >= 0) d.jv[s[3]] = l;
+  }
+  WG_SYNC();
+  // buses: ordered sums of the row's terms, diagonal entries, mismatch; the next bus's scalars are loaded ahead
+  GLOBAL const int* y_rp = glob(P.y_rp);
+  GLOBAL const int* y_diag = glob(P.y_diag);
+  GLOBAL const int* ppos = glob(P.ppos);
+  GLOBAL const int* qpos = glob(P.qpos);
+  const int n = P.n;
+  double lmax = 0.0;
+  {
+    int i = lane;
+    int nk0 = 0, nk1 = 0, nkd = 0, npp = -1, nqq = -1;
+    double nvm = 0;
+    auto load_bus = [&](int ii) {
+      const int c = ii < n ? ii : 0;
+      nk0 = y_rp[c]; nk1 = y_rp[c + 1]; nkd = y_diag[c]; npp = ppos[c]; nqq = qpos[c]; nvm = I.vm[c];
+    };
+    load_bus(i);
+    for (; i < n; i += kWave) {
+      const int k0 = nk0, k1 = nk1, kd = nkd, pp = npp, qq = nqq;
+      const double vmi = nvm;
+      load_bus(i + kWave);
+      const double gd0 = y_g[kd], bd0 = y_b[kd];
+      const v4i s = slots[kd], a = aslots[kd];
+      const double ps = I.pspec[i], qs = I.qspec[i];
+      double pi = 0.0, qi = 0.0;
+      int k = k0;
+      for (; k + 2 <= k1; k += 2) {
+        const v2d u = I.tpq[k], v = I.tpq[k + 1];
+        pi += u.x; qi += u.y;
+        pi += v.x; qi += v.y;
+      }
+      if (k < k1) { const v2d u = I.tpq[k]; pi += u.x; qi += u.y; }
+      const double P_i = vmi * pi, Q_i = vmi * qi;
+      I.p[i] = P_i;
+      I.q[i] = Q_i;
+      double g, b;
+      y_value(P, I, kd, gd0, bd0, &g, &b);
+      const double h = -Q_i - b * vmi * vmi;
+      const double nn = P_i / vmi + g * vmi;
+      const double mm = P_i - g * vmi * vmi;
+      const double l = Q_i / vmi - b * vmi;
+      put_j(I, s.x, a.x, h);
+      put_j(I, s.y, a.y, nn);
+      put_j(I, s.z, a.z, mm);
+      put_j(I, s.w, a.w, l);
       lbad |= !(isfinite(h) && isfinite(nn) && isfinite(mm) && isfinite(l));
+      if (pp >= 0) { const double fv = ps - P_i; I.f[pp] = fv; lmax = nan_max(lmax, fabs(fv)); }
+      if (qq >= 0) { const double fv = qs - Q_i; I.f[qq] = fv; lmax = nan_max(lmax, fabs(fv)); }
     }
-    const int pp = P.ppos[i], qq = P.qpos[i];
-    if (pp >= 0) { double fv = d.pspec[i] - P_i; d.f[pp] = fv; lmax = nan_max(lmax, fabs(fv)); }
-    if (qq >= 0) { double fv = d.qspec[i] - Q_i; d.f[qq] = fv; lmax = nan_max(lmax, fabs(fv)); }
   }
   *bad = wave_any(lbad);
   return wave_max(lmax);
@@ -122,82 +227,150 @@ __device__ double mismatch_jacobian(const DevPlan& P, const InstDesc& d, int lan
 //   EXACT = true (seam): the reference's own pivot search must pick the plan's row (first strict maximum over the
 //   candidates in reach order), above that row's floor; anything else refuses, as Refactor returns null.
 // Returns -1 on success, else a failing pivot step.
+//
+// The chunks' rows run as one stream through a ring of registers: each row's record (workspace slots) is loaded kRing
+// rows ahead and its L value kHalf rows ahead. An L value loaded before its row's dependency level started may be
+// stale, so at every level end (after the fence that publishes the level's L columns) the values of all rows in the
+// ring are loaded again: every row's value is then loaded after its level started. Chunk metadata runs two chunks
+// ahead, a chunk's A values and track metadata one chunk ahead. The streams are padded so that every load is in
+// bounds without a branch: the wait counts before each use are then known to the compiler, and loads issued ahead
+// stay in flight instead of being drained at every use.
+constexpr int kRing = kStreamPad, kHalf = kRing / 2;
+
 template <typename T, bool EXACT>
-__device__ int refactor(const DevPlan& P, const double* __restrict__ jv, T* lx, T* ux, T* ws, double growth_tol,
-                        const double* rowfloor, int lane, double* min_ratio) {
+__device__ __forceinline__ int refactor(const DevPlan& P, GLOBAL const T* aval, GLOBAL T* lx, GLOBAL T* ux,
+                                        LDSP T* ws, double growth_tol, GLOBAL const double* rowfloor, int lane,
+                                        double* min_ratio) {
+  GLOBAL const v2i* row_rl = globv<v2i>(P.row_rl);
+  GLOBAL const int* row_meta = glob(P.row_meta);
+  GLOBAL const v4i* ch_meta = globv<v4i>(P.ch_meta);
+  GLOBAL const int* ch_a_ws = glob(P.ch_a_ws);
+  GLOBAL const v4i* tr_meta = globv<v4i>(P.tr_meta);
+  GLOBAL const int* tr_col = glob(P.tr_col);
+  GLOBAL const int* piv_pos = glob(P.piv_pos);
+  GLOBAL const int* prow = glob(P.prow);
   const int g = lane / kTrackLanes, tl = lane % kTrackLanes;
-  const int wsz = kTracks * P.wstride;
+  const int wstride = P.wstride, wsz = kTracks * wstride;
+  const int nrows = P.n_rows;
   double mratio = *min_ratio;
-  for (int c = 0; c < P.n_chunks; c++) {
-    const int t0 = P.ch_track_ptr[c], ntr = P.ch_track_ptr[c + 1] - t0;
-    const int r0 = P.ch_row_ptr[c], nrows = P.ch_row_ptr[c + 1] - r0;
-    for (int s = lane; s < wsz; s += kWave) ws[s] = T(0);
-    WAVE_SYNC();
-    for (int e = P.ch_a_ptr[c] + lane; e < P.ch_a_ptr[c + 1]; e += kWave) ws[P.ch_a_ws[e]] = (T)jv[P.ch_a_src[e]];
-    WAVE_SYNC();
-    // updates, two rows of records in flight ahead of the row being applied
-    int rec_a = 0xFFFF, rec_b = 0xFFFF;
-    T val_a = T(0);
-    if (nrows > 0) {
-      rec_a = P.row_rec[(size_t)r0 * kWave + lane];
-      if ((rec_a & 0xFFFF) != 0xFFFF) val_a = lx[P.row_lsrc[(size_t)r0 * kWave + lane]];
-    }
-    if (nrows > 1) rec_b = P.row_rec[(size_t)(r0 + 1) * kWave + lane];
-    for (int rho = 0; rho < nrows; rho++) {
-      const int rec = rec_a;
-      const T val = val_a;
-      rec_a = rec_b;
-      val_a = T(0);
-      if (rho + 1 < nrows && (rec_a & 0xFFFF) != 0xFFFF) val_a = lx[P.row_lsrc[(size_t)(r0 + rho + 1) * kWave + lane]];
-      rec_b = (rho + 2 < nrows) ? P.row_rec[(size_t)(r0 + rho + 2) * kWave + lane] : 0xFFFF;
-      const int dst = rec & 0xFFFF;
-      if (dst != 0xFFFF) {
-        const T xj = ws[rec >> 16];
-        ws[dst] = ws[dst] - val * xj;
+  // ring of upcoming rows: (record, L value index), L value, level * 8 | flags. A row without an update carries L
+  // index 0, so its value load is in bounds too.
+  v2i rrl[kRing];
+  int rmeta[kRing];
+  T rval[kRing];
+#pragma unroll
+  for (int s = 0; s < kRing; s++) {
+    rrl[s] = row_rl[(size_t)s * kWave + lane];
+    rmeta[s] = row_meta[s];
+  }
+#pragma unroll
+  for (int s = 0; s < kRing; s++) rval[s] = lx[rrl[s].y];
+  // chunks: metadata (first A entry, end of A entries, first track, tracks) of the chunk after next; A entries (two
+  // per lane) and track metadata of the next chunk
+  int chunk = -1;
+  v4i cm_after = ch_meta[1];
+  v4i nx_cm = ch_meta[0];
+  int nx_ws0, nx_ws1, nx_k;
+  T nx_v0, nx_v1;
+  v4i nx_tr;
+  auto load_chunk = [&](const v4i& cm) {
+    const int e0 = cm.x + lane, e1 = cm.x + kWave + lane;
+    nx_ws0 = ch_a_ws[e0]; nx_v0 = aval[e0];
+    nx_ws1 = ch_a_ws[e1]; nx_v1 = aval[e1];
+    nx_tr = tr_meta[cm.z + g];
+    nx_k = tr_col[cm.z + g];
+    nx_cm = cm;
+  };
+  load_chunk(nx_cm);
+  v4i cur_cm = nx_cm, cur_tr = nx_tr;
+  int cur_k = 0;
+  for (int base = 0; base < nrows; base += kRing) {
+#pragma unroll
+    for (int s = 0; s < kRing; s++) {
+      const int r = base + s;
+      if (r >= nrows) break;
+      const int fl = rmeta[s] & 7;
+      if (fl & kRowBegin) {
+        chunk++;
+        cur_cm = nx_cm;
+        cur_tr = nx_tr;
+        cur_k = nx_k;
+        const int a_ws0 = nx_ws0, a_ws1 = nx_ws1;
+        const T a_v0 = nx_v0, a_v1 = nx_v1;
+        load_chunk(cm_after);
+        cm_after = ch_meta[chunk + 2];
+        for (int q = lane; q < wsz; q += kWave) ws[q] = T(0);
+        WAVE_SYNC();
+        if (cur_cm.x + lane < cur_cm.y) ws[a_ws0] = a_v0;
+        if (cur_cm.x + kWave + lane < cur_cm.y) ws[a_ws1] = a_v1;
+        for (int e = cur_cm.x + 2 * kWave + lane; e < cur_cm.y; e += kWave) ws[ch_a_ws[e]] = aval[e];
+        WAVE_SYNC();
       }
-      WAVE_SYNC();
-    }
-    // pivots, growth checks and write-out, one track per 8 lanes
-    int fail = -1;
-    if (g < ntr) {
-      const int k = P.tr_col[t0 + g];
-      const int u0 = P.up[k], nu = P.up[k + 1] - u0, l0 = P.lp[k], nl = P.lp[k + 1] - l0;
-      T* w = ws + g * P.wstride;
-      const T pivot = w[nu - 1];
-      const double pabs = fabs((double)pivot);
-      double cand = pabs;
-      int moved = 0;
-      const int before = EXACT ? P.piv_pos[k] : 0;
-      for (int t = 1 + tl; t < nl; t += kTrackLanes) {
-        const double cabs = fabs((double)w[nu - 1 + t]);
-        cand = nan_max(cand, cabs);
-        if (EXACT) moved |= (t - 1 < before) ? !(cabs < pabs) : (cabs > pabs);
+      {
+        const int rec = rrl[s].x;
+        const int dst = rec & 0xFFFF;
+        if (dst != 0xFFFF) {
+          const T xj = ws[rec >> 16];
+          ws[dst] = ws[dst] - rval[s] * xj;
+        }
+        WAVE_SYNC();
       }
-      const double best = track_max(cand);
-      bool bad;
-      if (EXACT) {
-        bad = track_any(moved) || !(pabs > rowfloor[P.prow[k]]);
-      } else {
-        const double ratio = pabs / best;
-        bad = !(ratio >= growth_tol) || !(pabs > 0.0) || !isfinite(pabs);
-        mratio = ratio < mratio ? ratio : mratio;
+      if (fl & kRowEnd) {
+        // pivots, growth checks and write-out, one track per 8 lanes
+        int fail = -1;
+        if (g < cur_cm.w) {
+          const int k = cur_k, u0 = cur_tr.x, nu = cur_tr.y, l0 = cur_tr.z, nl = cur_tr.w;
+          LDSP T* w = ws + g * wstride;
+          const T pivot = w[nu - 1];
+          const double pabs = fabs((double)pivot);
+          double cand = pabs;
+          int moved = 0;
+          const int before = EXACT ? piv_pos[k] : 0;
+          for (int t = 1 + tl; t < nl; t += kTrackLanes) {
+            const double cabs = fabs((double)w[nu - 1 + t]);
+            cand = nan_max(cand, cabs);
+            if (EXACT) moved |= (t - 1 < before) ? !(cabs < pabs) : (cabs > pabs);
+          }
+          const double best = track_max(cand);
+          bool bad;
+          if (EXACT) {
+            bad = track_any(moved) || !(pabs > rowfloor[prow[k]]);
+          } else {
+            const double ratio = pabs / best;
+            bad = !(ratio >= growth_tol) || !(pabs > 0.0) || !isfinite(pabs);
+            mratio = ratio < mratio ? ratio : mratio;
+          }
+          if (bad) {
+            fail = k;
+          } else {
+            for (int t = tl; t < nu; t += kTrackLanes) ux[u0 + t] = w[t];
+            if (tl == 0) lx[l0] = T(1);
+            for (int t = 1 + tl; t < nl; t += kTrackLanes) lx[l0 + t] = w[nu - 1 + t] / pivot;
+          }
+        }
+        // any failing track fails the factorisation (lowest lane's column reported)
+        const unsigned long long fmask = __ballot(fail >= 0);
+        if (fmask) {
+          const int first = __ffsll((long long)fmask) - 1;
+          return __shfl(fail, first, kWave);
+        }
+        WAVE_SYNC();
+        if (fl & kRowLevelEnd) {
+          // the level's L columns are written: publish them, then reload the values of the rows in the ring
+          WG_SYNC();
+#pragma unroll
+          for (int t = 0; t < kRing; t++) rval[t] = lx[rrl[t].y];
+        }
       }
-      if (bad) {
-        fail = k;
-      } else {
-        for (int t = tl; t < nu; t += kTrackLanes) ux[u0 + t] = w[t];
-        if (tl == 0) lx[l0] = T(1);
-        for (int t = 1 + tl; t < nl; t += kTrackLanes) lx[l0 + t] = w[nu - 1 + t] / pivot;
+      // refill this slot with row r + kRing (the stream is padded by kRing rows); load the value of row r + kHalf
+      {
+        const int rn = r + kRing;
+        rrl[s] = row_rl[(size_t)rn * kWave + lane];
+        rmeta[s] = row_meta[rn];
+        const int hs = (s + kHalf) % kRing;
+        rval[hs] = lx[rrl[hs].y];
       }
     }
-    // any failing track fails the factorisation (lowest lane's column reported)
-    unsigned long long fmask = __ballot(fail >= 0);
-    if (fmask) {
-      const int first = __ffsll((long long)fmask) - 1;
-      return __shfl(fail, first, kWave);
-    }
-    WAVE_SYNC();
-    if (P.ch_fence[c]) WG_SYNC();
   }
   WG_SYNC();
   mratio = -wave_max(-mratio);
@@ -206,69 +379,123 @@ __device__ int refactor(const DevPlan& P, const double* __restrict__ jv, T* lx, 
 }
 
 // ------------------------------------------------------------------------------------------------ solves
-// out = A^{-1} rhs with the factors. Rows of one level are independent: each lane pulls one row's updates in the order
-// the reference's column sweeps apply them (forward: columns ascending; back: columns descending, then the division
-// by the pivot), so every entry is the reference's value. x is the workspace (LDS or global).
-template <typename T, bool XLDS>
-__device__ void lu_solve(const DevPlan& P, const T* __restrict__ lx, const T* __restrict__ ux, const double* rhs,
-                         double* out, T* x, int lane) {
+// out = A^{-1} rhs with the factors. Rows of one level are independent: a pass takes up to 64 rows of one level, one
+// per lane, and each lane pulls its row's updates, one per step, in the order the reference's column sweeps apply them
+// (forward: columns ascending; back: columns descending, then the division by the pivot), so every entry is the
+// reference's value. The step records are loaded kRing steps ahead and the factor values kHalf steps ahead (the
+// factors are final before a solve starts; the streams are padded by kRing steps); x is read at the step itself.
+// XP: the workspace pointer (LDS or global).
+template <typename T, bool BACK, bool XLDS, typename XP>
+__device__ __forceinline__ void solve_stream(GLOBAL const v4i* rec, GLOBAL const int* flags, int steps,
+                                             GLOBAL const T* vals, XP x, int lane) {
+  v4i rr[kRing];
+  int rf[kRing];
+  T rv[kRing], rd[kRing];
+#pragma unroll
+  for (int s = 0; s < kRing; s++) {
+    rr[s] = rec[(size_t)s * kWave + lane];
+    rf[s] = flags[s];
+  }
+#pragma unroll
+  for (int s = 0; s < kRing; s++) {
+    rv[s] = vals[rr[s].y];
+    rd[s] = BACK ? vals[rr[s].w] : T(1);
+  }
+  T acc = T(0);
+  for (int base = 0; base < steps; base += kRing) {
+#pragma unroll
+    for (int s = 0; s < kRing; s++) {
+      const int t = base + s;
+      if (t >= steps) break;
+      const v4i r4 = rr[s];
+      const int fl = rf[s];
+      if (fl & kStepBegin) acc = r4.z >= 0 ? x[r4.z] : T(0);
+      if (r4.x >= 0) acc = acc - rv[s] * x[r4.x];
+      if (fl & kStepEnd) {
+        if (r4.z >= 0) x[r4.z] = BACK ? acc / rd[s] : acc;
+        if (XLDS) WAVE_SYNC(); else WG_SYNC();
+      }
+      const int tn = t + kRing;
+      rr[s] = rec[(size_t)tn * kWave + lane];
+      rf[s] = flags[tn];
+      const int hs = (s + kHalf) % kRing;
+      rv[hs] = vals[rr[hs].y];
+      if (BACK) rd[hs] = vals[rr[hs].w];
+    }
+  }
+}
+
+template <typename T, bool XLDS, typename XP>
+__device__ __forceinline__ void lu_solve(const DevPlan& P, GLOBAL const T* lx, GLOBAL const T* ux,
+                                         GLOBAL const double* rhs, GLOBAL double* out, XP x, int lane) {
   const int m = P.m;
-  for (int r = lane; r < m; r += kWave) x[P.pinv[r]] = (T)rhs[r];
+  GLOBAL const int* pinv = glob(P.pinv);
+  GLOBAL const int* q = glob(P.q);
+  for (int r = lane; r < m; r += kWave) x[pinv[r]] = (T)rhs[r];
   if (XLDS) WAVE_SYNC(); else WG_SYNC();
-  int b0 = 0;
-  for (int lv = 0; lv < P.fw_levels; lv++) {
-    const int b1 = P.fw_level_end[lv];
-    for (int idx = b0 + lane; idx < b1; idx += kWave) {
-      const int i = P.fw_order[idx];
-      T s = x[i];
-      for (int e = P.lr_ptr[i]; e < P.lr_ptr[i + 1]; e++) s = s - lx[P.lr_idx[e]] * x[P.lr_col[e]];
-      x[i] = s;
-    }
-    if (XLDS) WAVE_SYNC(); else WG_SYNC();
-    b0 = b1;
-  }
-  b0 = 0;
-  for (int lv = 0; lv < P.bw_levels; lv++) {
-    const int b1 = P.bw_level_end[lv];
-    for (int idx = b0 + lane; idx < b1; idx += kWave) {
-      const int i = P.bw_order[idx];
-      T s = x[i];
-      for (int e = P.ur_ptr[i]; e < P.ur_ptr[i + 1]; e++) s = s - ux[P.ur_idx[e]] * x[P.ur_col[e]];
-      x[i] = s / ux[P.up[i + 1] - 1];
-    }
-    if (XLDS) WAVE_SYNC(); else WG_SYNC();
-    b0 = b1;
-  }
-  for (int k = lane; k < m; k += kWave) out[P.q[k]] = (double)x[k];
+  solve_stream<T, false, XLDS>(globv<v4i>(P.fs_rec), glob(P.fs_flags), P.fs_steps, lx, x, lane);
+  solve_stream<T, true, XLDS>(globv<v4i>(P.bs_rec), glob(P.bs_flags), P.bs_steps, ux, x, lane);
+  for (int k = lane; k < m; k += kWave) out[q[k]] = (double)x[k];
   WG_SYNC();
 }
 
-// r = f - J dx (FP64, CSR rows).
-__device__ void residual(const DevPlan& P, const double* jv, const double* f, const double* dx, double* r, int lane) {
-  for (int i = lane; i < P.m; i += kWave) {
+// r = f - J dx (FP64, CSR rows; each row's products summed in column order).
+__device__ __forceinline__ void residual(const DevPlan& P, GLOBAL const double* jv, GLOBAL const double* f,
+                                         GLOBAL const double* dx, GLOBAL double* r, int lane) {
+  GLOBAL const int* j_rp = glob(P.j_rp);
+  GLOBAL const int* j_ci = glob(P.j_ci);
+  const int m = P.m;
+  int i = lane;
+  int nk0 = 0, nk1 = 0;
+  double nf = 0;
+  auto load_row = [&](int ii) {
+    const int c = ii < m ? ii : 0;
+    nk0 = j_rp[c]; nk1 = j_rp[c + 1]; nf = f[c];
+  };
+  load_row(i);
+  for (; i < m; i += kWave) {
+    const int k0 = nk0, k1 = nk1;
+    const double fi = nf;
+    load_row(i + kWave);
     double s = 0.0;
-    for (int k = P.j_rp[i]; k < P.j_rp[i + 1]; k++) s += jv[k] * dx[P.j_ci[k]];
-    r[i] = f[i] - s;
+    int k = k0;
+    for (; k + 4 <= k1; k += 4) {
+      const int c0 = j_ci[k], c1 = j_ci[k + 1], c2 = j_ci[k + 2], c3 = j_ci[k + 3];
+      const double v0 = jv[k], v1 = jv[k + 1], v2 = jv[k + 2], v3 = jv[k + 3];
+      const double x0 = dx[c0], x1 = dx[c1], x2 = dx[c2], x3 = dx[c3];
+      s += v0 * x0; s += v1 * x1; s += v2 * x2; s += v3 * x3;
+    }
+    for (; k < k1; k++) s += jv[k] * dx[j_ci[k]];
+    r[i] = fi - s;
   }
   WG_SYNC();
 }
 
 // ------------------------------------------------------------------------------------------------ the kernel
 template <typename T, bool XLDS>
-__global__ __launch_bounds__(kWave) void nr_batch_kernel(const InstDesc* __restrict__ descs, NrParams prm) {
+// At least two waves per SIMD (registers capped at 256): 8 instances resident per CU.
+__global__ __launch_bounds__(kWave, 2) void nr_batch_kernel(const InstDesc* __restrict__ descs, const DevPlan P,
+                                                         const NrParams prm) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int lane = threadIdx.x;
   const InstDesc& d = descs[blockIdx.x];
-  const DevPlan& P = *d.plan;
+  Inst<T> I;
+  I.pspec = glob(d.pspec); I.qspec = glob(d.qspec); I.vm_start = glob(d.vm_start); I.va_start = glob(d.va_start);
+  I.vm = glob(d.vm); I.va = glob(d.va); I.p = glob(d.p); I.q = glob(d.q);
+  I.f = glob(d.f); I.dx = glob(d.dx); I.r = glob(d.r); I.jv = glob(d.jv);
+  I.tpq = (GLOBAL v2d*)d.tpq;
+  I.lx = glob(reinterpret_cast<T*>(d.lx)); I.ux = glob(reinterpret_cast<T*>(d.ux));
+  I.aval = glob(reinterpret_cast<T*>(d.aval)); I.xg = glob(reinterpret_cast<T*>(d.xg));
+  I.patch = glob(&descs[blockIdx.x].patch[0]);
+  I.n_patch = d.n_patch;
+#pragma unroll
+  for (int t = 0; t < kMaxPatch; t++) I.pidx[t] = t < I.n_patch ? d.patch[t].idx : -1;
   InstState* st = d.state;
   const int n = P.n, m = P.m;
-  T* ws = reinterpret_cast<T*>(smem);
-  T* x = XLDS ? ws + kTracks * P.wstride : reinterpret_cast<T*>(d.xg);
-  T* lx = reinterpret_cast<T*>(d.lx);
-  T* ux = reinterpret_cast<T*>(d.ux);
+  LDSP T* ws = lds(reinterpret_cast<T*>(smem));
 
   if (prm.init || prm.load_start) {
-    for (int i = lane; i < n; i += kWave) { d.vm[i] = d.vm_start[i]; d.va[i] = d.va_start[i]; }
+    for (int i = lane; i < n; i += kWave) { I.vm[i] = I.vm_start[i]; I.va[i] = I.va_start[i]; }
     WG_SYNC();
   }
   if (!prm.init && st->done) return;
@@ -286,9 +513,13 @@ __global__ __launch_bounds__(kWave) void nr_batch_kernel(const InstDesc* __restr
     acc += now - clk;
     clk = now;
   };
+  auto solve = [&](GLOBAL const double* rhs, GLOBAL double* out) {
+    if constexpr (XLDS) lu_solve<T, true>(P, I.lx, I.ux, rhs, out, ws + kTracks * P.wstride, lane);
+    else lu_solve<T, false>(P, I.lx, I.ux, rhs, out, I.xg, lane);
+  };
   for (;;) {
     int bad;
-    max_mis = mismatch_jacobian(P, d, lane, &bad);
+    max_mis = mismatch_jacobian<T>(P, I, lane, &bad);
     WG_SYNC();
     tick(t_mis);
     if (iter == 0) start_mis = max_mis;
@@ -297,33 +528,35 @@ __global__ __launch_bounds__(kWave) void nr_batch_kernel(const InstDesc* __restr
     if (iter >= prm.max_iterations) break;
     if (steps >= prm.steps_this_launch) { paused = 1; break; }
     if (bad || !isfinite(max_mis)) break;  // NewtonShell.SolveStep refuses a non-finite J or f
-    fail_col = refactor<T, false>(P, d.jv, lx, ux, ws, prm.growth_tol, nullptr, lane, &min_ratio);
+    fail_col = refactor<T, false>(P, I.aval, I.lx, I.ux, ws, prm.growth_tol, nullptr, lane, &min_ratio);
     tick(t_fac);
     if (fail_col >= 0) { paused = 2; break; }
-    lu_solve<T, XLDS>(P, lx, ux, d.f, d.dx, x, lane);
+    solve(I.f, I.dx);
     for (int it = 0; it < prm.refine_steps; it++) {
-      residual(P, d.jv, d.f, d.dx, d.r, lane);
-      lu_solve<T, XLDS>(P, lx, ux, d.r, d.r, x, lane);
-      for (int k = lane; k < m; k += kWave) d.dx[k] = d.dx[k] + d.r[k];
+      residual(P, I.jv, I.f, I.dx, I.r, lane);
+      solve(I.r, I.r);
+      for (int k = lane; k < m; k += kWave) I.dx[k] = I.dx[k] + I.r[k];
       WG_SYNC();
     }
     tick(t_sol);
     double lmax = 0.0;
     int lbad = 0;
     for (int k = lane; k < m; k += kWave) {
-      const double v = d.dx[k];
+      const double v = I.dx[k];
       lbad |= !isfinite(v);
       lmax = fabs(v) > lmax ? fabs(v) : lmax;
     }
     if (wave_any(lbad)) { step_failures++; break; }
     const double max_abs = wave_max(lmax);
     const double damp = max_abs > 0.4 ? 0.4 / max_abs : 1.0;
+    GLOBAL const int* ppos = glob(P.ppos);
+    GLOBAL const int* qpos = glob(P.qpos);
     for (int i = lane; i < n; i += kWave) {
-      const int pp = P.ppos[i], qq = P.qpos[i];
-      if (pp >= 0) d.va[i] = d.va[i] + damp * d.dx[pp];
+      const int pp = ppos[i], qq = qpos[i];
+      if (pp >= 0) I.va[i] = I.va[i] + damp * I.dx[pp];
       if (qq >= 0) {
-        double v = d.vm[i] + damp * d.dx[qq];
-        d.vm[i] = v < 0.01 ? 0.01 : v;
+        double v = I.vm[i] + damp * I.dx[qq];
+        I.vm[i] = v < 0.01 ? 0.01 : v;
       }
     }
     WG_SYNC();
@@ -334,7 +567,8 @@ __global__ __launch_bounds__(kWave) void nr_batch_kernel(const InstDesc* __restr
   // final verdict (NewtonShell.AssembleResult precedence: NonFinite > Converged > Diverged > MaxIterations)
   int lnf = 0;
   for (int i = lane; i < n; i += kWave)
-    lnf |= !(isfinite(d.vm[i]) && isfinite(d.va[i]) && isfinite(d.p[i] * P.base_mva) && isfinite(d.q[i] * P.base_mva));
+    lnf |= !(isfinite(I.vm[i]) && isfinite(I.va[i]) && isfinite(I.p[i] * P.base_mva) &&
+             isfinite(I.q[i] * P.base_mva));
   const int nonfinite = wave_any(lnf) || !isfinite(max_mis);
   if (lane == 0) {
     st->iter = iter;
@@ -355,29 +589,34 @@ __global__ __launch_bounds__(kWave) void nr_batch_kernel(const InstDesc* __restr
 }
 
 // ------------------------------------------------------------------------------------------------ seam solves
-// One CSR system per work-group on a plan made from that system's own factorisation: the numeric refactor replays
-// the reference's arithmetic (SparseLuFactorization.Refactor), refusing exactly where it would (a pivot that moves,
-// or one at or below its row floor), then the reference's solve.
-__global__ __launch_bounds__(kWave) void seam_solve_kernel(const SeamDesc* __restrict__ descs) {
+// One CSR system on a plan made from that system's own factorisation: the numeric refactor replays the reference's
+// arithmetic (SparseLuFactorization.Refactor), refusing exactly where it would (a pivot that moves, or one at or below
+// its row floor), then the reference's solve.
+__global__ __launch_bounds__(kWave) void seam_solve_kernel(const SeamDesc* __restrict__ descs, const DevPlan P) {
   extern __shared__ __align__(16) unsigned char smem[];
   const SeamDesc& d = descs[blockIdx.x];
-  const DevPlan& P = *d.plan;
   const int lane = threadIdx.x, m = P.m;
-  double* ws = reinterpret_cast<double*>(smem);
+  LDSP double* ws = lds(reinterpret_cast<double*>(smem));
+  GLOBAL const double* a = glob(d.a);
+  GLOBAL double* rowfloor = glob(d.rowfloor);
+  GLOBAL double* aval = glob(d.aval);
+  GLOBAL const int* j_rp = glob(P.j_rp);
+  GLOBAL const int* ch_a_src = glob(P.ch_a_src);
   // row floors of the matrix as given (1e-12 x the largest |entry| of each row)
   for (int i = lane; i < m; i += kWave) {
     double mx = 0.0;
-    for (int k = P.j_rp[i]; k < P.j_rp[i + 1]; k++) mx = fabs(d.a[k]) > mx ? fabs(d.a[k]) : mx;
-    d.rowfloor[i] = mx * 1e-12;
+    for (int k = j_rp[i]; k < j_rp[i + 1]; k++) mx = fabs(a[k]) > mx ? fabs(a[k]) : mx;
+    rowfloor[i] = mx * 1e-12;
   }
+  for (int e = lane; e < P.n_a; e += kWave) aval[e] = a[ch_a_src[e]];
   WG_SYNC();
   double ratio = 1.0;
-  const int fail = refactor<double, true>(P, d.a, d.lx, d.ux, ws, 0.0, d.rowfloor, lane, &ratio);
+  const int fail = refactor<double, true>(P, aval, glob(d.lx), glob(d.ux), ws, 0.0, rowfloor, lane, &ratio);
   if (fail >= 0) {
     if (lane == 0) *d.status = 1;
     return;
   }
-  lu_solve<double, false>(P, d.lx, d.ux, d.b, d.x, d.xs, lane);
+  lu_solve<double, false>(P, glob(d.lx), glob(d.ux), glob(d.b), glob(d.x), glob(d.xs), lane);
   if (lane == 0) *d.status = 0;
 }
 
@@ -398,21 +637,25 @@ __global__ void fp64_probe_kernel(double* out, double a) {
 }
 
 // host-side launchers (engine.cpp)
-hipError_t launch_nr_batch(int lu_fp32, int x_in_lds, const InstDesc* descs, int count, const NrParams& prm,
-                           size_t lds_bytes, hipStream_t stream) {
+hipError_t launch_nr_batch(int lu_fp32, int x_in_lds, const InstDesc* descs, const DevPlan& plan, int count,
+                           const NrParams& prm, size_t lds_bytes, hipStream_t stream) {
   dim3 grid(count), block(kWave);
   if (lu_fp32) {
-    if (x_in_lds) hipLaunchKernelGGL((nr_batch_kernel<float, true>), grid, block, lds_bytes, stream, descs, prm);
-    else hipLaunchKernelGGL((nr_batch_kernel<float, false>), grid, block, lds_bytes, stream, descs, prm);
+    if (x_in_lds)
+      hipLaunchKernelGGL((nr_batch_kernel<float, true>), grid, block, lds_bytes, stream, descs, plan, prm);
+    else
+      hipLaunchKernelGGL((nr_batch_kernel<float, false>), grid, block, lds_bytes, stream, descs, plan, prm);
   } else {
-    if (x_in_lds) hipLaunchKernelGGL((nr_batch_kernel<double, true>), grid, block, lds_bytes, stream, descs, prm);
-    else hipLaunchKernelGGL((nr_batch_kernel<double, false>), grid, block, lds_bytes, stream, descs, prm);
+    if (x_in_lds)
+      hipLaunchKernelGGL((nr_batch_kernel<double, true>), grid, block, lds_bytes, stream, descs, plan, prm);
+    else
+      hipLaunchKernelGGL((nr_batch_kernel<double, false>), grid, block, lds_bytes, stream, descs, plan, prm);
   }
   return hipGetLastError();
 }
 
-hipError_t launch_seam(const SeamDesc* descs, int count, size_t lds_bytes, hipStream_t stream) {
-  hipLaunchKernelGGL(seam_solve_kernel, dim3(count), dim3(kWave), lds_bytes, stream, descs);
+hipError_t launch_seam(const SeamDesc* descs, const DevPlan& plan, int count, size_t lds_bytes, hipStream_t stream) {
+  hipLaunchKernelGGL(seam_solve_kernel, dim3(count), dim3(kWave), lds_bytes, stream, descs, plan);
   return hipGetLastError();
 }
 
