@@ -50,12 +50,24 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=48)
     ap.add_argument("--cpu-sample", type=int, default=800)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="first copy this many MiB device-to-device (a kernel of known traffic, for scaling the "
+                         "HBM counters of a profiler run)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path (run with --gpus 1).", file=sys.stderr)
         sys.exit(2)
 
     from gridsim_amd import engine, gridgen
+
+    if args.copy_probe_mib:
+        import torch
+        src = torch.ones(args.copy_probe_mib * 2**18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"# copy probe: {args.copy_probe_mib} MiB read + {args.copy_probe_mib} MiB written", file=sys.stderr)
+        del src, dst
 
     t0 = time.time()
     net = gridgen.config2_grid()

@@ -23,6 +23,7 @@ hipError_t launch_nr_batch(int lu_fp32, int x_in_lds, const InstDesc* descs, con
 hipError_t launch_seam(const SeamDesc* descs, const DevPlan& plan, int count, size_t lds_bytes, hipStream_t stream);
 hipError_t launch_sincos(const double* x, double* s, double* c, int64_t n, hipStream_t stream);
 hipError_t launch_fp64_probe(double* out, hipStream_t stream);
+size_t refactor_stash_bytes();
 }  // namespace gs
 
 using namespace gs;
@@ -95,8 +96,8 @@ namespace {
 struct DevicePlan {
   DevMem y_rp, y_ci, y_row, y_diag, y_g, y_b, slots, aslots, ppos, qpos, j_rp, j_ci, vm0, va0;
   DevMem pinv, q, prow, piv_pos;
-  DevMem ch_meta, ch_a_ws, ch_a_src, tr_meta, tr_col, row_rl, row_meta;
-  DevMem fs_rec, fs_flags, bs_rec, bs_flags;
+  DevMem ch_meta, ch_a_ws, ch_a_src, tr_meta, tr_col, tr_wpos, fpos, bpos, row_rl, row_meta;
+  DevMem fs_rec, bs_rec;
   DevPlan h{};   // passed to the kernels by value
   DevicePlanHost dph;
 };
@@ -144,11 +145,13 @@ void upload_plan(DevicePlan& dp, const YBus* y, const JacobianPattern& jp, const
   // the refactor's chunks and row stream
   h.wstride = d.wstride;
   h.n_chunks = d.n_chunks;
-  h.n_rows = d.n_rows;
+  // rows and steps rounded up to whole rings of no-ops, plus one ring of look-ahead padding
+  auto ring_up = [](int v) { return (v + kStreamPad - 1) / kStreamPad * kStreamPad; };
+  h.n_rows = ring_up(d.n_rows);
   h.n_a = (int)d.ch_a_src.size();
   // (every stream padded past its end: the kernels load ahead without bounds checks)
   const int n_tracks = (int)d.tr_col.size();
-  std::vector<int> row_rl(2 * ((size_t)d.n_rows + kStreamPad) * 64), row_meta((size_t)d.n_rows + kStreamPad, 0);
+  std::vector<int> row_rl(2 * ((size_t)h.n_rows + kStreamPad) * 64), row_meta((size_t)h.n_rows + kStreamPad, 0);
   for (size_t e = 0; e < row_rl.size() / 2; e++) {
     const bool in = e < d.row_rec.size();
     row_rl[2 * e] = in ? d.row_rec[e] : 0xFFFF;
@@ -175,30 +178,29 @@ void upload_plan(DevicePlan& dp, const YBus* y, const JacobianPattern& jp, const
     tr_meta[4 * t + 3] = d.tr_nl[t];
   }
   upload(dp.ch_meta, ch_meta, s); upload(dp.ch_a_ws, ch_a_ws, s); upload(dp.ch_a_src, ch_a_src, s);
-  upload(dp.tr_meta, tr_meta, s); upload(dp.tr_col, tr_col, s);
+  std::vector<int> tr_wpos(d.tr_wpos);
+  tr_wpos.resize(((size_t)n_tracks + kTracks) * kTrackLanes * 4, -1);
+  upload(dp.tr_meta, tr_meta, s); upload(dp.tr_col, tr_col, s); upload(dp.tr_wpos, tr_wpos, s);
+  upload(dp.fpos, d.fpos, s); upload(dp.bpos, d.bpos, s);
   upload(dp.row_rl, row_rl, s); upload(dp.row_meta, row_meta, s);
   h.ch_meta = dp.ch_meta.as<int4>(); h.ch_a_ws = dp.ch_a_ws.as<int>(); h.ch_a_src = dp.ch_a_src.as<int>();
-  h.tr_meta = dp.tr_meta.as<int4>(); h.tr_col = dp.tr_col.as<int>();
+  h.tr_meta = dp.tr_meta.as<int4>(); h.tr_col = dp.tr_col.as<int>(); h.tr_wpos = dp.tr_wpos.as<int4>();
+  h.fpos = dp.fpos.as<int>(); h.bpos = dp.bpos.as<int>();
   h.row_rl = dp.row_rl.as<int2>(); h.row_meta = dp.row_meta.as<int>();
-  // the solves' step streams (padded with steps that touch nothing)
-  h.fs_steps = d.fs_steps;
-  h.bs_steps = d.bs_steps;
-  auto pad_steps = [](const std::vector<int>& rec, const std::vector<int>& flags, std::vector<int>& prec,
-                      std::vector<int>& pflags) {
-    prec = rec;
-    pflags = flags;
-    for (int t = 0; t < kStreamPad; t++) {
-      pflags.push_back(0);
+  // the solves' step streams (padded with steps that touch nothing) and the factors' stream-order sizes
+  h.fs_steps = ring_up(d.fs_steps);
+  h.bs_steps = ring_up(d.bs_steps);
+  h.n_fval = d.n_fval;
+  h.n_bval = d.n_bval;
+  auto pad_steps = [](const std::vector<int>& rec, int steps) {
+    std::vector<int> prec(rec);
+    for (int t = (int)(rec.size() / 256); t < steps + kStreamPad; t++)
       for (int l = 0; l < 64; l++) { prec.push_back(-1); prec.push_back(0); prec.push_back(-1); prec.push_back(0); }
-    }
+    return prec;
   };
-  std::vector<int> fs_rec, fs_flags, bs_rec, bs_flags;
-  pad_steps(d.fs_rec, d.fs_flags, fs_rec, fs_flags);
-  pad_steps(d.bs_rec, d.bs_flags, bs_rec, bs_flags);
-  upload(dp.fs_rec, fs_rec, s); upload(dp.fs_flags, fs_flags, s);
-  upload(dp.bs_rec, bs_rec, s); upload(dp.bs_flags, bs_flags, s);
-  h.fs_rec = dp.fs_rec.as<int4>(); h.fs_flags = dp.fs_flags.as<int>();
-  h.bs_rec = dp.bs_rec.as<int4>(); h.bs_flags = dp.bs_flags.as<int>();
+  std::vector<int> fs_rec = pad_steps(d.fs_rec, h.fs_steps), bs_rec = pad_steps(d.bs_rec, h.bs_steps);
+  upload(dp.fs_rec, fs_rec, s); upload(dp.bs_rec, bs_rec, s);
+  h.fs_rec = dp.fs_rec.as<int4>(); h.bs_rec = dp.bs_rec.as<int4>();
   // the staging vectors above are locals: finish the copies before they go
   check(hipStreamSynchronize(s), "upload plan");
 }
@@ -255,7 +257,7 @@ struct gs_net {
 namespace {
 
 struct Layout {
-  size_t vm, va, p, q, f, dx, r, jv, tpq, lx, ux, aval, xg, spec, start, stride;
+  size_t vm, va, p, q, f, dx, r, jv, tpq, lx, fval, bval, dval, aval, xg, spec, start, stride;
 };
 
 Layout instance_layout(const DevPlan& h, bool fp32, bool x_lds) {
@@ -273,7 +275,9 @@ Layout instance_layout(const DevPlan& h, bool fp32, bool x_lds) {
   L.jv = o; o += al(8 * (size_t)h.nnz_j);
   L.tpq = o; o += al(16 * (size_t)h.nnz_y);
   L.lx = o; o += al(t * (size_t)h.nnz_l);
-  L.ux = o; o += al(t * (size_t)h.nnz_u);
+  L.fval = o; o += al(t * (size_t)std::max(1, h.n_fval));
+  L.bval = o; o += al(t * (size_t)std::max(1, h.n_bval));
+  L.dval = o; o += al(t * (size_t)h.m);
   L.aval = o; o += al(t * ((size_t)h.n_a + kAPad));
   L.xg = o; o += x_lds ? 0 : al(t * (size_t)h.m);
   L.spec = o; o += al(16 * (size_t)h.n);
@@ -321,8 +325,9 @@ BatchRun plan_run(const DevPlan& h, const gs_nr_options& o, bool force_fp64) {
   size_t t = r.fp32 ? 4 : 8;
   size_t ws = t * (size_t)kTracks * (size_t)h.wstride;
   size_t x = t * (size_t)h.m;
-  r.x_lds = ws + x <= kLdsLimit;
-  r.lds_bytes = ws + (r.x_lds ? x : 0);
+  const size_t stash = refactor_stash_bytes();
+  r.x_lds = ws + stash + x <= kLdsLimit;
+  r.lds_bytes = ws + stash + (r.x_lds ? x : 0);
   return r;
 }
 
@@ -384,7 +389,7 @@ double launch_instances(gs_ctx* ctx, DevicePlan& dp, DevMem& inst_buf, DevMem& d
     d.vm = (double*)(b + L.vm); d.va = (double*)(b + L.va); d.p = (double*)(b + L.p); d.q = (double*)(b + L.q);
     d.f = (double*)(b + L.f); d.dx = (double*)(b + L.dx); d.r = (double*)(b + L.r); d.jv = (double*)(b + L.jv);
     d.tpq = (double*)(b + L.tpq);
-    d.lx = b + L.lx; d.ux = b + L.ux; d.aval = b + L.aval; d.xg = run.x_lds ? nullptr : b + L.xg;
+    d.lx = b + L.lx; d.fval = b + L.fval; d.bval = b + L.bval; d.dval = b + L.dval; d.aval = b + L.aval; d.xg = run.x_lds ? nullptr : b + L.xg;
     const gs_instance& in = inst[i];
     d.n_patch = (y && netw) ? outage_patches(*y, *netw, in.outage_branch, d.patch) : 0;
     // specified injections: the network's, or a load-scaled copy
@@ -625,17 +630,18 @@ gs_status seam_solve_one(gs_ctx* ctx, int n, const int* rowptr, const int* colid
   jp.ci = ci;
   std::vector<double> xt(n);
   auto run_device = [&](SeamPlan& sp) -> int {
-    DevMem a, bb, xx, lx, ux, xs, rf, av, stt, desc;
+    DevMem a, bb, xx, lv, fv, bv, dv, xs, rf, av, stt, desc;
     a.ensure(8 * (size_t)nnz); bb.ensure(8 * (size_t)n); xx.ensure(8 * (size_t)n);
-    lx.ensure(8 * (size_t)sp.dev.h.nnz_l); ux.ensure(8 * (size_t)sp.dev.h.nnz_u); xs.ensure(8 * (size_t)n);
+    lv.ensure(8 * (size_t)sp.dev.h.nnz_l); fv.ensure(8 * (size_t)std::max(1, sp.dev.h.n_fval)); bv.ensure(8 * (size_t)std::max(1, sp.dev.h.n_bval));
+    dv.ensure(8 * (size_t)n); xs.ensure(8 * (size_t)n);
     rf.ensure(8 * (size_t)n); av.ensure(8 * ((size_t)sp.dev.h.n_a + kAPad)); stt.ensure(4);
     check(hipMemcpyAsync(a.p, val, 8 * (size_t)nnz, hipMemcpyHostToDevice, s), "seam a");
     check(hipMemcpyAsync(bb.p, b, 8 * (size_t)n, hipMemcpyHostToDevice, s), "seam b");
-    SeamDesc d{a.as<double>(), bb.as<double>(), xx.as<double>(), lx.as<double>(), ux.as<double>(), xs.as<double>(),
-               rf.as<double>(), av.as<double>(), stt.as<int>()};
+    SeamDesc d{a.as<double>(),  bb.as<double>(), xx.as<double>(), lv.as<double>(), fv.as<double>(),
+               bv.as<double>(), dv.as<double>(), xs.as<double>(), rf.as<double>(), av.as<double>(), stt.as<int>()};
     desc.ensure(sizeof(SeamDesc));
     check(hipMemcpyAsync(desc.p, &d, sizeof(SeamDesc), hipMemcpyHostToDevice, s), "seam desc");
-    size_t lds = 8 * (size_t)kTracks * (size_t)sp.dev.h.wstride;
+    size_t lds = 8 * (size_t)kTracks * (size_t)sp.dev.h.wstride + refactor_stash_bytes();
     check(launch_seam(desc.as<SeamDesc>(), sp.dev.h, 1, lds, s), "seam kernel");
     int status = 1;
     check(hipMemcpyAsync(&status, stt.p, 4, hipMemcpyDeviceToHost, s), "seam status");

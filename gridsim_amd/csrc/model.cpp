@@ -667,11 +667,14 @@ void compile_plan(const Plan& plan, DevicePlanHost& d) {
     for (int i = 0; i < m; i++) d.bw_order[cnt[lvl[i]]++] = i;
     d.bw_levels = nlv;
   }
-  // Solve step streams: per level, rows by decreasing entry count, 64 rows per pass, one step per entry index.
+  // Solve step streams: per level, rows by decreasing entry count, 64 rows per pass, one step per entry index. Each
+  // entry's value gets the next position of its value array, step by step, so that a step's values are contiguous.
   auto build_stream = [&](const vector<int>& order, const vector<int>& level_end, int nlv, const vector<int>& rptr,
                           const vector<int>& rcol, const vector<int>& ridx, bool back, vector<int>& rec,
-                          vector<int>& flags) {
+                          vector<int>& pos, int& n_val, int& steps) {
     int b0 = 0;
+    n_val = 0;
+    steps = 0;
     for (int lv = 0; lv < nlv; lv++) {
       int b1 = level_end[lv];
       vector<int> rows;
@@ -685,25 +688,46 @@ void compile_plan(const Plan& plan, DevicePlanHost& d) {
         size_t p1 = std::min(rows.size(), p0 + 64);
         int len = std::max(1, rptr[rows[p0] + 1] - rptr[rows[p0]]);
         for (int e = 0; e < len; e++) {
-          flags.push_back((e == 0 ? DevicePlanHost::kStepBegin : 0) | (e + 1 == len ? DevicePlanHost::kStepEnd : 0));
+          int flags = (e == 0 ? DevicePlanHost::kStepBegin : 0) | (e + 1 == len ? DevicePlanHost::kStepEnd : 0);
           for (int lane = 0; lane < 64; lane++) {
-            int row = -1, col = -1, idx = 0, didx = 0;
+            int row = -1, col = -1, idx = 0;
             if (p0 + lane < p1) {
               row = rows[p0 + lane];
-              if (rptr[row] + e < rptr[row + 1]) { col = rcol[rptr[row] + e]; idx = ridx[rptr[row] + e]; }
-              if (back) didx = plan.up[row + 1] - 1;
+              if (rptr[row] + e < rptr[row + 1]) {
+                col = rcol[rptr[row] + e];
+                idx = n_val++;
+                pos[ridx[rptr[row] + e]] = idx;
+              }
             }
-            rec.push_back(col); rec.push_back(idx); rec.push_back(row); rec.push_back(didx);
+            rec.push_back(col); rec.push_back(idx); rec.push_back(row); rec.push_back(flags);
           }
+          steps++;
         }
       }
       b0 = b1;
     }
   };
-  build_stream(d.fw_order, d.fw_level_end, d.fw_levels, d.lr_ptr, d.lr_col, d.lr_idx, false, d.fs_rec, d.fs_flags);
-  build_stream(d.bw_order, d.bw_level_end, d.bw_levels, d.ur_ptr, d.ur_col, d.ur_idx, true, d.bs_rec, d.bs_flags);
-  d.fs_steps = (int)d.fs_flags.size();
-  d.bs_steps = (int)d.bs_flags.size();
+  d.fpos.assign(d.nnz_l, -1);
+  d.bpos.assign(d.nnz_u, -1);
+  build_stream(d.fw_order, d.fw_level_end, d.fw_levels, d.lr_ptr, d.lr_col, d.lr_idx, false, d.fs_rec, d.fpos,
+               d.n_fval, d.fs_steps);
+  build_stream(d.bw_order, d.bw_level_end, d.bw_levels, d.ur_ptr, d.ur_col, d.ur_idx, true, d.bs_rec, d.bpos,
+               d.n_bval, d.bs_steps);
+  // (the refactor reads its L values from the column-ordered copy lx: a track's lanes read consecutive entries)
+  // per track and lane: where its first two U slots and first two L slots are written (bval position, -2 for the
+  // pivot, which goes to dval; fval position; -1 for none)
+  d.tr_wpos.assign(d.tr_col.size() * TL * 4, -1);
+  for (size_t tr = 0; tr < d.tr_col.size(); tr++) {
+    const int k = d.tr_col[tr], u0 = plan.up[k], nu = plan.up[k + 1] - u0, l0 = plan.lp[k], nl = plan.lp[k + 1] - l0;
+    for (int tl = 0; tl < TL; tl++) {
+      int* wp = &d.tr_wpos[(tr * TL + tl) * 4];
+      for (int i = 0; i < 2; i++) {
+        const int tu = tl + i * TL, tlw = 1 + tl + i * TL;
+        wp[i] = tu < nu - 1 ? d.bpos[u0 + tu] : tu == nu - 1 ? -2 : -1;
+        wp[2 + i] = tlw < nl ? d.fpos[l0 + tlw] : -1;
+      }
+    }
+  }
   // For the exact pivot check: how many candidates (rows not yet pivotal at step k) precede the pivot in reach order.
   d.piv_pos.assign(m, 0);
   for (int k = 0; k < m; k++) {
@@ -725,7 +749,8 @@ bool emulate_device_lu(const DevicePlanHost& d, int nnz_j, const double* jv, con
                        std::string& why) {
   const int m = d.m, G = DevicePlanHost::kTracks, TL = DevicePlanHost::kTrackLanes;
   const int wsz = G * d.wstride;
-  vector<double> ws(wsz), lx(d.nnz_l, std::nan("")), ux(d.nnz_u, std::nan(""));
+  vector<double> ws(wsz), fval(std::max(1, d.n_fval), std::nan("")), bval(std::max(1, d.n_bval), std::nan(""));
+  vector<double> dval(m, std::nan("")), lx(d.nnz_l, std::nan(""));
   auto bad = [&](const std::string& w) { why = w; return false; };
   // A values in record order, as the mismatch kernel writes them through j2a
   const int n_a = (int)d.ch_a_src.size();
@@ -773,9 +798,21 @@ bool emulate_device_lu(const DevicePlanHost& d, int nnz_j, const double* jv, con
         const double* w = &ws[g * d.wstride];
         double pivot = w[nu - 1];
         if (pivot == 0.0) return bad("zero pivot");
-        for (int t = 0; t < nu; t++) ux[u0 + t] = w[t];
-        lx[l0] = 1.0;
-        for (int t = 1; t < nl; t++) lx[l0 + t] = w[nu - 1 + t] / pivot;
+        // each lane writes its slots where the solves read them: the first two through its prefetched codes
+        for (int tl = 0; tl < TL; tl++) {
+          const int* wp = &d.tr_wpos[((size_t)(t0 + g) * TL + tl) * 4];
+          for (int t = tl, i = 0; t < nu; t += TL, i++) {
+            int code = i == 0 ? wp[0] : i == 1 ? wp[1] : (t == nu - 1 ? -2 : d.bpos[u0 + t]);
+            if (code >= d.n_bval || code == -1) return bad("U write position");
+            if (code >= 0) bval[code] = w[t]; else dval[k] = w[t];
+            if ((code < 0) != (t == nu - 1)) return bad("U diagonal write");
+          }
+          for (int t = 1 + tl, i = 0; t < nl; t += TL, i++) {
+            int code = i == 0 ? wp[2] : i == 1 ? wp[3] : d.fpos[l0 + t];
+            if (code < 0 || code >= d.n_fval) return bad("L write position");
+            fval[code] = lx[l0 + t] = w[nu - 1 + t] / pivot;
+          }
+        }
       }
     }
     if (fl & DevicePlanHost::kRowLevelEnd) cur_level++;
@@ -784,14 +821,17 @@ bool emulate_device_lu(const DevicePlanHost& d, int nnz_j, const double* jv, con
   vector<double> x(m, std::nan(""));
   for (int r = 0; r < m; r++) x[d.pinv[r]] = rhs[r];
   // the step streams, with the rows a step may read checked against the rows already final
-  auto run_stream = [&](const vector<int>& rec, const vector<int>& flags, const vector<double>& vals, bool back,
+  auto run_stream = [&](const vector<int>& rec, int steps, const vector<double>& vals, bool back,
                         vector<char>& final_rows) -> bool {
     vector<double> acc(64, 0.0);
     vector<int> row(64, -1);
-    for (size_t t = 0; t < flags.size(); t++) {
+    if ((int)rec.size() != steps * 64 * 4) { why = "stream size"; return false; }
+    for (int t = 0; t < steps; t++) {
+      const int fl = rec[((size_t)t * 64) * 4 + 3];
       for (int lane = 0; lane < 64; lane++) {
-        const int* r4 = &rec[(t * 64 + lane) * 4];
-        if (flags[t] & DevicePlanHost::kStepBegin) {
+        const int* r4 = &rec[((size_t)t * 64 + lane) * 4];
+        if (r4[3] != fl) { why = "stream flags differ across lanes"; return false; }
+        if (fl & DevicePlanHost::kStepBegin) {
           row[lane] = r4[2];
           acc[lane] = row[lane] >= 0 ? x[row[lane]] : 0.0;
         }
@@ -800,11 +840,12 @@ bool emulate_device_lu(const DevicePlanHost& d, int nnz_j, const double* jv, con
           if (r4[1] < 0 || r4[1] >= (int)vals.size()) { why = "stream value index"; return false; }
           acc[lane] = acc[lane] - vals[r4[1]] * x[r4[0]];
         }
-        if ((flags[t] & DevicePlanHost::kStepEnd) && row[lane] >= 0) {
-          x[row[lane]] = back ? acc[lane] / vals[r4[3]] : acc[lane];
+        if ((fl & DevicePlanHost::kStepEnd) && row[lane] >= 0) {
+          if (r4[2] != row[lane]) { why = "stream row changes inside a pass"; return false; }
+          x[row[lane]] = back ? acc[lane] / dval[row[lane]] : acc[lane];
         }
       }
-      if (flags[t] & DevicePlanHost::kStepEnd)
+      if (fl & DevicePlanHost::kStepEnd)
         for (int lane = 0; lane < 64; lane++) if (row[lane] >= 0) final_rows[row[lane]] = 1;
     }
     return true;
@@ -813,9 +854,9 @@ bool emulate_device_lu(const DevicePlanHost& d, int nnz_j, const double* jv, con
     // forward: rows without L entries are final from the start; back: a row is final once divided
     vector<char> fin(m, 0);
     for (int i = 0; i < m; i++) fin[i] = d.lr_ptr[i + 1] == d.lr_ptr[i];
-    if (!run_stream(d.fs_rec, d.fs_flags, lx, false, fin)) return false;
+    if (!run_stream(d.fs_rec, d.fs_steps, fval, false, fin)) return false;
     vector<char> fin_b(m, 0);
-    if (!run_stream(d.bs_rec, d.bs_flags, ux, true, fin_b)) return false;
+    if (!run_stream(d.bs_rec, d.bs_steps, bval, true, fin_b)) return false;
     for (int k = 0; k < m; k++) out[d.q[k]] = x[k];
     return true;
   }

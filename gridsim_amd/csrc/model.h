@@ -122,19 +122,25 @@ struct DevicePlanHost {
   vector<int> tr_col;                                        // column of each track, chunk order
   vector<int> ch_a_ws, ch_a_src;                             // A entries: workspace index, J CSR position
   vector<int> row_rec;                                       // (dst ws) | (xj ws << 16), 0xFFFF dst = no update
-  vector<int> row_lsrc;                                      // L value index of the record
+  vector<int> row_lsrc;                                      // L value index (CSC) of the record
   // The refactor as one stream of rows (every chunk has at least one row) with per-row flags, so the kernel can
   // keep loads for the next rows in flight: kRowBegin / kRowEnd mark a chunk's first / last row, kRowLevelEnd the
   // last row of a dependency level. Per-track metadata and the A values' order are laid out for one-hop loads.
   static constexpr int kRowBegin = 1, kRowEnd = 2, kRowLevelEnd = 4;
   vector<int> row_flags, row_level;                 // [n_rows]
   vector<int> tr_u0, tr_nu, tr_l0, tr_nl;           // per track (chunk order)
+  vector<int> tr_wpos;                              // per track and lane: 4 write positions (see compile_plan)
   vector<int> j2a;                                  // [nnz_j]: A record of each Jacobian CSR entry, or -1
-  // Triangular solves as step streams: a pass is up to 64 rows of one level, one lane per row, one step per entry;
-  // record (col, value index, row, diagonal index) per lane and step, col = -1 for no entry.
+  // Triangular solves as step streams: a pass is up to 64 rows of one level, one lane per row (rows by decreasing
+  // length), one step per entry; record (col, value index, row, flags) per lane and step, col = -1 for no entry. The
+  // factors are stored in the order the streams read them: fval holds L's off-diagonal entries in forward-stream
+  // order, bval U's off-diagonal entries in back-stream order (step by step, the active lanes of a step being a prefix,
+  // so a step's values are contiguous), dval U's diagonal by pivot step. fpos / bpos map an L / U entry (CSC index) to
+  // its position, -1 for a diagonal. The refactor keeps L in column order too (lx, CSC), for its own updates.
   static constexpr int kStepBegin = 1, kStepEnd = 2;
-  int fs_steps = 0, bs_steps = 0;
-  vector<int> fs_rec, fs_flags, bs_rec, bs_flags;   // rec: 4 ints per lane per step
+  int fs_steps = 0, bs_steps = 0, n_fval = 0, n_bval = 0;
+  vector<int> fs_rec, bs_rec;                       // 4 ints per lane per step
+  vector<int> fpos, bpos;                           // [nnz_l], [nnz_u]
   // Level-scheduled triangular solves, rows pulled in the reference's update order.
   int fw_levels = 0, bw_levels = 0;
   vector<int> fw_order, fw_level_end, lr_ptr, lr_col, lr_idx;  // L rows: columns ascending

@@ -32,18 +32,21 @@ struct DevPlan {
   const int32_t *ch_a_ws, *ch_a_src;  // per A entry: workspace slot, J CSR position
   const int4* tr_meta;                // per track: (first U entry, U count, first L entry, L count)
   const int32_t* tr_col;              // per track: its pivot step
+  const int4* tr_wpos;                // per track and lane: write positions of its first two U and two L slots
+  const int32_t *fpos, *bpos;         // per L / U entry (CSC index): position in fval / bval (-1: diagonal)
   const int2* row_rl;                 // per row and lane: (record, L value index)
   const int32_t* row_meta;            // per row: level * 8 | flags (kRowBegin, kRowEnd, kRowLevelEnd)
-  // triangular-solve step streams: per lane and step (col, value index, row, diagonal index)
-  int32_t fs_steps, bs_steps;
+  // Triangular-solve step streams: per lane and step (col, value index, row, flags). The factors live in the order
+  // the streams read them: fval (L off-diagonals, forward order), bval (U off-diagonals, back order), dval (U's
+  // diagonal by pivot step).
+  int32_t fs_steps, bs_steps, n_fval, n_bval;
   const int4 *fs_rec, *bs_rec;
-  const int32_t *fs_flags, *bs_flags;
 };
 
 constexpr int kRowBegin = 1, kRowEnd = 2, kRowLevelEnd = 4, kRowLevelShift = 3;
 constexpr int kStepBegin = 1, kStepEnd = 2;
 // load-ahead distance of the row and step streams, and the padding each stream carries
-constexpr int kStreamPad = 8;
+constexpr int kStreamPad = 16;
 constexpr int kAPad = 128;            // A entries a lane may load ahead per chunk (two per lane)
 
 constexpr int kTracks = 8, kTrackLanes = 8;
@@ -78,7 +81,8 @@ struct InstDesc {
   double *vm, *va, *p, *q;
   double *f, *dx, *r, *jv;            // f, dx, residual, FP64 Jacobian values (CSR order)
   double* tpq;                        // per Y cell: the cell's two injection terms (P, Q)
-  void *lx, *ux;                      // L / U values in the LU precision
+  void *lx;                           // L in the LU precision, column order (CSC)
+  void *fval, *bval, *dval;           // the factors in the LU precision (stream order)
   void* aval;                         // Jacobian values in the refactor's record order, LU precision
   void* xg;                           // global triangular-solve workspace (used when LDS cannot hold x)
 };
@@ -88,7 +92,7 @@ struct SeamDesc {
   const double* a;         // CSR values
   const double* b;
   double* x;
-  double *lx, *ux, *xs, *rowfloor, *aval;
+  double *lx, *fval, *bval, *dval, *xs, *rowfloor, *aval;
   int32_t* status;         // 0 solved, 1 refused (a pivot would move, or is at or below its row floor)
 };
 

@@ -69,6 +69,13 @@ __device__ __forceinline__ int track_any(int v) {
 
 __device__ __forceinline__ int wave_any(int v) { return __any(v) ? 1 : 0; }
 
+// The lane index, opaque to the optimiser: passed to each phase of the Newton loop so that lane-dependent addresses
+// are computed inside the phase rather than hoisted out of the loop (where they would hold registers throughout).
+__device__ __forceinline__ int opaque_lane(int lane) {
+  asm volatile("" : "+v"(lane));
+  return lane;
+}
+
 // Ordering of LDS traffic between the lanes of one wavefront (the hardware executes a wave's LDS instructions in
 // order; this stops the compiler from moving them).
 #define WAVE_SYNC() __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront")
@@ -81,7 +88,7 @@ struct Inst {
   GLOBAL const double *pspec, *qspec, *vm_start, *va_start;
   GLOBAL double *vm, *va, *p, *q, *f, *dx, *r, *jv;
   GLOBAL v2d* tpq;
-  GLOBAL T *lx, *ux, *aval, *xg;
+  GLOBAL T *lx, *fval, *bval, *dval, *aval, *xg;
   GLOBAL const Patch* patch;
   int n_patch;
   int pidx[kMaxPatch];   // patched Y cells (-1: unused)
@@ -232,29 +239,40 @@ __device__ double mismatch_jacobian(const DevPlan& P, const Inst<T>& I, int lane
 // rows ahead and its L value kHalf rows ahead. An L value loaded before its row's dependency level started may be
 // stale, so at every level end (after the fence that publishes the level's L columns) the values of all rows in the
 // ring are loaded again: every row's value is then loaded after its level started. Chunk metadata runs two chunks
-// ahead, a chunk's A values and track metadata one chunk ahead. The streams are padded so that every load is in
-// bounds without a branch: the wait counts before each use are then known to the compiler, and loads issued ahead
-// stay in flight instead of being drained at every use.
+// ahead, a chunk's A values, track metadata and write positions one chunk ahead. The streams are padded so that every
+// load is in bounds without a branch: the wait counts before each use are then known to the compiler, and loads
+// issued ahead stay in flight instead of being drained at every use.
+//
+// The factors are written where the triangular solves read them: L's off-diagonal entries to fval, U's off-diagonal
+// entries to bval, the pivots to dval. L is also kept in column order (lx, CSC) for the later columns' updates: the 8
+// lanes of a track then read 8 consecutive values of one column, a single memory segment.
 constexpr int kRing = kStreamPad, kHalf = kRing / 2;
 
+// LDS the refactor needs beside its workspace: the stash of the current chunk's track metadata.
+constexpr int kStashBytes = kWave * (16 + 16 + 4);
+
 template <typename T, bool EXACT>
-__device__ __forceinline__ int refactor(const DevPlan& P, GLOBAL const T* aval, GLOBAL T* lx, GLOBAL T* ux,
-                                        LDSP T* ws, double growth_tol, GLOBAL const double* rowfloor, int lane,
+__device__ __forceinline__ int refactor(const DevPlan& P, GLOBAL const T* aval, GLOBAL T* lx, GLOBAL T* fval,
+                                        GLOBAL T* bval, GLOBAL T* dval, LDSP T* ws, LDSP void* stash,
+                                        double growth_tol, GLOBAL const double* rowfloor, int lane,
                                         double* min_ratio) {
   GLOBAL const v2i* row_rl = globv<v2i>(P.row_rl);
   GLOBAL const int* row_meta = glob(P.row_meta);
   GLOBAL const v4i* ch_meta = globv<v4i>(P.ch_meta);
   GLOBAL const int* ch_a_ws = glob(P.ch_a_ws);
   GLOBAL const v4i* tr_meta = globv<v4i>(P.tr_meta);
+  GLOBAL const v4i* tr_wpos = globv<v4i>(P.tr_wpos);
   GLOBAL const int* tr_col = glob(P.tr_col);
+  GLOBAL const int* fpos = glob(P.fpos);
+  GLOBAL const int* bpos = glob(P.bpos);
   GLOBAL const int* piv_pos = glob(P.piv_pos);
   GLOBAL const int* prow = glob(P.prow);
   const int g = lane / kTrackLanes, tl = lane % kTrackLanes;
   const int wstride = P.wstride, wsz = kTracks * wstride;
   const int nrows = P.n_rows;
   double mratio = *min_ratio;
-  // ring of upcoming rows: (record, L value index), L value, level * 8 | flags. A row without an update carries L
-  // index 0, so its value load is in bounds too.
+  // ring of upcoming rows: (record, L value index), L value, level * 8 | flags. A row without an update carries index
+  // 0, so its value load is in bounds too.
   v2i rrl[kRing];
   int rmeta[kRing];
   T rval[kRing];
@@ -266,45 +284,48 @@ __device__ __forceinline__ int refactor(const DevPlan& P, GLOBAL const T* aval, 
 #pragma unroll
   for (int s = 0; s < kRing; s++) rval[s] = lx[rrl[s].y];
   // chunks: metadata (first A entry, end of A entries, first track, tracks) of the chunk after next; A entries (two
-  // per lane) and track metadata of the next chunk
+  // per lane), track metadata and write positions of the next chunk. At a chunk's first row its prefetched data is
+  // consumed at once (A values into the workspace, track metadata into the LDS stash that its last row reads back), so
+  // the next chunk's loads can land in the same registers: a value in flight is never copied (a copy would wait for
+  // it, and with it for every load issued before it).
   int chunk = -1;
   v4i cm_after = ch_meta[1];
   v4i nx_cm = ch_meta[0];
   int nx_ws0, nx_ws1, nx_k;
   T nx_v0, nx_v1;
-  v4i nx_tr;
+  v4i nx_tr, nx_wp;
   auto load_chunk = [&](const v4i& cm) {
     const int e0 = cm.x + lane, e1 = cm.x + kWave + lane;
     nx_ws0 = ch_a_ws[e0]; nx_v0 = aval[e0];
     nx_ws1 = ch_a_ws[e1]; nx_v1 = aval[e1];
     nx_tr = tr_meta[cm.z + g];
     nx_k = tr_col[cm.z + g];
+    nx_wp = tr_wpos[(cm.z + g) * kTrackLanes + tl];
     nx_cm = cm;
   };
   load_chunk(nx_cm);
-  v4i cur_cm = nx_cm, cur_tr = nx_tr;
-  int cur_k = 0;
-  for (int base = 0; base < nrows; base += kRing) {
+  LDSP v4i* st_tr = reinterpret_cast<LDSP v4i*>(stash);
+  LDSP v4i* st_wp = st_tr + kWave;
+  LDSP int* st_k = reinterpret_cast<LDSP int*>(st_wp + kWave);
+  for (int base = 0; base < nrows; base += kRing) {  // nrows: a multiple of kRing (the stream is padded with no-ops)
 #pragma unroll
     for (int s = 0; s < kRing; s++) {
       const int r = base + s;
-      if (r >= nrows) break;
       const int fl = rmeta[s] & 7;
       if (fl & kRowBegin) {
         chunk++;
-        cur_cm = nx_cm;
-        cur_tr = nx_tr;
-        cur_k = nx_k;
-        const int a_ws0 = nx_ws0, a_ws1 = nx_ws1;
-        const T a_v0 = nx_v0, a_v1 = nx_v1;
-        load_chunk(cm_after);
-        cm_after = ch_meta[chunk + 2];
+        const v4i cm = nx_cm;
         for (int q = lane; q < wsz; q += kWave) ws[q] = T(0);
         WAVE_SYNC();
-        if (cur_cm.x + lane < cur_cm.y) ws[a_ws0] = a_v0;
-        if (cur_cm.x + kWave + lane < cur_cm.y) ws[a_ws1] = a_v1;
-        for (int e = cur_cm.x + 2 * kWave + lane; e < cur_cm.y; e += kWave) ws[ch_a_ws[e]] = aval[e];
+        if (cm.x + lane < cm.y) ws[nx_ws0] = nx_v0;
+        if (cm.x + kWave + lane < cm.y) ws[nx_ws1] = nx_v1;
+        for (int e = cm.x + 2 * kWave + lane; e < cm.y; e += kWave) ws[ch_a_ws[e]] = aval[e];
+        st_tr[lane] = nx_tr;
+        st_wp[lane] = nx_wp;
+        st_k[lane] = g < cm.w ? nx_k : -1;
         WAVE_SYNC();
+        load_chunk(cm_after);
+        cm_after = ch_meta[chunk + 2];
       }
       {
         const int rec = rrl[s].x;
@@ -318,8 +339,10 @@ __device__ __forceinline__ int refactor(const DevPlan& P, GLOBAL const T* aval, 
       if (fl & kRowEnd) {
         // pivots, growth checks and write-out, one track per 8 lanes
         int fail = -1;
-        if (g < cur_cm.w) {
-          const int k = cur_k, u0 = cur_tr.x, nu = cur_tr.y, l0 = cur_tr.z, nl = cur_tr.w;
+        const int k = st_k[lane];
+        if (k >= 0) {
+          const v4i tr = st_tr[lane], cur_wp = st_wp[lane];
+          const int u0 = tr.x, nu = tr.y, l0 = tr.z, nl = tr.w;
           LDSP T* w = ws + g * wstride;
           const T pivot = w[nu - 1];
           const double pabs = fabs((double)pivot);
@@ -343,9 +366,19 @@ __device__ __forceinline__ int refactor(const DevPlan& P, GLOBAL const T* aval, 
           if (bad) {
             fail = k;
           } else {
-            for (int t = tl; t < nu; t += kTrackLanes) ux[u0 + t] = w[t];
-            if (tl == 0) lx[l0] = T(1);
-            for (int t = 1 + tl; t < nl; t += kTrackLanes) lx[l0 + t] = w[nu - 1 + t] / pivot;
+            // the lane's first two U and L slots go through the prefetched positions, any further ones (long columns)
+            // through the position maps
+            for (int t = tl, i = 0; t < nu; t += kTrackLanes, i++) {
+              const int code = i == 0 ? cur_wp.x : i == 1 ? cur_wp.y : (t == nu - 1 ? -2 : bpos[u0 + t]);
+              if (code >= 0) bval[code] = w[t];
+              else dval[k] = w[t];
+            }
+            for (int t = 1 + tl, i = 0; t < nl; t += kTrackLanes, i++) {
+              const int code = i == 0 ? cur_wp.z : i == 1 ? cur_wp.w : fpos[l0 + t];
+              const T v = w[nu - 1 + t] / pivot;
+              lx[l0 + t] = v;
+              fval[code] = v;
+            }
           }
         }
         // any failing track fails the factorisation (lowest lane's column reported)
@@ -383,58 +416,53 @@ __device__ __forceinline__ int refactor(const DevPlan& P, GLOBAL const T* aval, 
 // per lane, and each lane pulls its row's updates, one per step, in the order the reference's column sweeps apply them
 // (forward: columns ascending; back: columns descending, then the division by the pivot), so every entry is the
 // reference's value. The step records are loaded kRing steps ahead and the factor values kHalf steps ahead (the
-// factors are final before a solve starts; the streams are padded by kRing steps); x is read at the step itself.
+// factors are final before a solve starts; the streams are padded by kRing steps); a step's values are contiguous
+// (its active lanes are a prefix), so each value load is one coalesced segment. x is read at the step itself.
 // XP: the workspace pointer (LDS or global).
 template <typename T, bool BACK, bool XLDS, typename XP>
-__device__ __forceinline__ void solve_stream(GLOBAL const v4i* rec, GLOBAL const int* flags, int steps,
-                                             GLOBAL const T* vals, XP x, int lane) {
+__device__ __forceinline__ void solve_stream(GLOBAL const v4i* rec, int steps, GLOBAL const T* vals,
+                                             GLOBAL const T* dval, XP x, int lane) {
   v4i rr[kRing];
-  int rf[kRing];
   T rv[kRing], rd[kRing];
 #pragma unroll
-  for (int s = 0; s < kRing; s++) {
-    rr[s] = rec[(size_t)s * kWave + lane];
-    rf[s] = flags[s];
-  }
+  for (int s = 0; s < kRing; s++) rr[s] = rec[(size_t)s * kWave + lane];
 #pragma unroll
   for (int s = 0; s < kRing; s++) {
     rv[s] = vals[rr[s].y];
-    rd[s] = BACK ? vals[rr[s].w] : T(1);
+    rd[s] = BACK ? dval[rr[s].z < 0 ? 0 : rr[s].z] : T(1);
   }
   T acc = T(0);
-  for (int base = 0; base < steps; base += kRing) {
+  for (int base = 0; base < steps; base += kRing) {  // steps: a multiple of kRing (the stream is padded with no-ops)
 #pragma unroll
     for (int s = 0; s < kRing; s++) {
       const int t = base + s;
-      if (t >= steps) break;
       const v4i r4 = rr[s];
-      const int fl = rf[s];
+      const int fl = r4.w;
       if (fl & kStepBegin) acc = r4.z >= 0 ? x[r4.z] : T(0);
       if (r4.x >= 0) acc = acc - rv[s] * x[r4.x];
       if (fl & kStepEnd) {
         if (r4.z >= 0) x[r4.z] = BACK ? acc / rd[s] : acc;
         if (XLDS) WAVE_SYNC(); else WG_SYNC();
       }
-      const int tn = t + kRing;
-      rr[s] = rec[(size_t)tn * kWave + lane];
-      rf[s] = flags[tn];
+      rr[s] = rec[(size_t)(t + kRing) * kWave + lane];
       const int hs = (s + kHalf) % kRing;
       rv[hs] = vals[rr[hs].y];
-      if (BACK) rd[hs] = vals[rr[hs].w];
+      if (BACK) rd[hs] = dval[rr[hs].z < 0 ? 0 : rr[hs].z];
     }
   }
 }
 
 template <typename T, bool XLDS, typename XP>
-__device__ __forceinline__ void lu_solve(const DevPlan& P, GLOBAL const T* lx, GLOBAL const T* ux,
-                                         GLOBAL const double* rhs, GLOBAL double* out, XP x, int lane) {
+__device__ __forceinline__ void lu_solve(const DevPlan& P, GLOBAL const T* fval, GLOBAL const T* bval,
+                                         GLOBAL const T* dval, GLOBAL const double* rhs, GLOBAL double* out, XP x,
+                                         int lane) {
   const int m = P.m;
   GLOBAL const int* pinv = glob(P.pinv);
   GLOBAL const int* q = glob(P.q);
   for (int r = lane; r < m; r += kWave) x[pinv[r]] = (T)rhs[r];
   if (XLDS) WAVE_SYNC(); else WG_SYNC();
-  solve_stream<T, false, XLDS>(globv<v4i>(P.fs_rec), glob(P.fs_flags), P.fs_steps, lx, x, lane);
-  solve_stream<T, true, XLDS>(globv<v4i>(P.bs_rec), glob(P.bs_flags), P.bs_steps, ux, x, lane);
+  solve_stream<T, false, XLDS>(globv<v4i>(P.fs_rec), P.fs_steps, fval, dval, x, lane);
+  solve_stream<T, true, XLDS>(globv<v4i>(P.bs_rec), P.bs_steps, bval, dval, x, lane);
   for (int k = lane; k < m; k += kWave) out[q[k]] = (double)x[k];
   WG_SYNC();
 }
@@ -484,7 +512,9 @@ __global__ __launch_bounds__(kWave, 2) void nr_batch_kernel(const InstDesc* __re
   I.vm = glob(d.vm); I.va = glob(d.va); I.p = glob(d.p); I.q = glob(d.q);
   I.f = glob(d.f); I.dx = glob(d.dx); I.r = glob(d.r); I.jv = glob(d.jv);
   I.tpq = (GLOBAL v2d*)d.tpq;
-  I.lx = glob(reinterpret_cast<T*>(d.lx)); I.ux = glob(reinterpret_cast<T*>(d.ux));
+  I.lx = glob(reinterpret_cast<T*>(d.lx));
+  I.fval = glob(reinterpret_cast<T*>(d.fval)); I.bval = glob(reinterpret_cast<T*>(d.bval));
+  I.dval = glob(reinterpret_cast<T*>(d.dval));
   I.aval = glob(reinterpret_cast<T*>(d.aval)); I.xg = glob(reinterpret_cast<T*>(d.xg));
   I.patch = glob(&descs[blockIdx.x].patch[0]);
   I.n_patch = d.n_patch;
@@ -492,7 +522,10 @@ __global__ __launch_bounds__(kWave, 2) void nr_batch_kernel(const InstDesc* __re
   for (int t = 0; t < kMaxPatch; t++) I.pidx[t] = t < I.n_patch ? d.patch[t].idx : -1;
   InstState* st = d.state;
   const int n = P.n, m = P.m;
+  // LDS: the refactor's workspace, its stash, then (when it fits) the triangular-solve vector
   LDSP T* ws = lds(reinterpret_cast<T*>(smem));
+  LDSP void* stash = lds(reinterpret_cast<void*>(smem + sizeof(T) * kTracks * P.wstride));
+  LDSP T* xl = lds(reinterpret_cast<T*>(smem + sizeof(T) * kTracks * P.wstride + kStashBytes));
 
   if (prm.init || prm.load_start) {
     for (int i = lane; i < n; i += kWave) { I.vm[i] = I.vm_start[i]; I.va[i] = I.va_start[i]; }
@@ -514,12 +547,12 @@ __global__ __launch_bounds__(kWave, 2) void nr_batch_kernel(const InstDesc* __re
     clk = now;
   };
   auto solve = [&](GLOBAL const double* rhs, GLOBAL double* out) {
-    if constexpr (XLDS) lu_solve<T, true>(P, I.lx, I.ux, rhs, out, ws + kTracks * P.wstride, lane);
-    else lu_solve<T, false>(P, I.lx, I.ux, rhs, out, I.xg, lane);
+    if constexpr (XLDS) lu_solve<T, true>(P, I.fval, I.bval, I.dval, rhs, out, xl, opaque_lane(lane));
+    else lu_solve<T, false>(P, I.fval, I.bval, I.dval, rhs, out, I.xg, opaque_lane(lane));
   };
   for (;;) {
     int bad;
-    max_mis = mismatch_jacobian<T>(P, I, lane, &bad);
+    max_mis = mismatch_jacobian<T>(P, I, opaque_lane(lane), &bad);
     WG_SYNC();
     tick(t_mis);
     if (iter == 0) start_mis = max_mis;
@@ -528,12 +561,13 @@ __global__ __launch_bounds__(kWave, 2) void nr_batch_kernel(const InstDesc* __re
     if (iter >= prm.max_iterations) break;
     if (steps >= prm.steps_this_launch) { paused = 1; break; }
     if (bad || !isfinite(max_mis)) break;  // NewtonShell.SolveStep refuses a non-finite J or f
-    fail_col = refactor<T, false>(P, I.aval, I.lx, I.ux, ws, prm.growth_tol, nullptr, lane, &min_ratio);
+    fail_col = refactor<T, false>(P, I.aval, I.lx, I.fval, I.bval, I.dval, ws, stash, prm.growth_tol, nullptr,
+                                  opaque_lane(lane), &min_ratio);
     tick(t_fac);
     if (fail_col >= 0) { paused = 2; break; }
     solve(I.f, I.dx);
     for (int it = 0; it < prm.refine_steps; it++) {
-      residual(P, I.jv, I.f, I.dx, I.r, lane);
+      residual(P, I.jv, I.f, I.dx, I.r, opaque_lane(lane));
       solve(I.r, I.r);
       for (int k = lane; k < m; k += kWave) I.dx[k] = I.dx[k] + I.r[k];
       WG_SYNC();
@@ -597,6 +631,7 @@ __global__ __launch_bounds__(kWave) void seam_solve_kernel(const SeamDesc* __res
   const SeamDesc& d = descs[blockIdx.x];
   const int lane = threadIdx.x, m = P.m;
   LDSP double* ws = lds(reinterpret_cast<double*>(smem));
+  LDSP void* stash = lds(reinterpret_cast<void*>(smem + sizeof(double) * kTracks * P.wstride));
   GLOBAL const double* a = glob(d.a);
   GLOBAL double* rowfloor = glob(d.rowfloor);
   GLOBAL double* aval = glob(d.aval);
@@ -611,12 +646,14 @@ __global__ __launch_bounds__(kWave) void seam_solve_kernel(const SeamDesc* __res
   for (int e = lane; e < P.n_a; e += kWave) aval[e] = a[ch_a_src[e]];
   WG_SYNC();
   double ratio = 1.0;
-  const int fail = refactor<double, true>(P, aval, glob(d.lx), glob(d.ux), ws, 0.0, rowfloor, lane, &ratio);
+  const int fail =
+      refactor<double, true>(P, aval, glob(d.lx), glob(d.fval), glob(d.bval), glob(d.dval), ws, stash, 0.0, rowfloor, lane,
+                             &ratio);
   if (fail >= 0) {
     if (lane == 0) *d.status = 1;
     return;
   }
-  lu_solve<double, false>(P, glob(d.lx), glob(d.ux), glob(d.b), glob(d.x), glob(d.xs), lane);
+  lu_solve<double, false>(P, glob(d.fval), glob(d.bval), glob(d.dval), glob(d.b), glob(d.x), glob(d.xs), lane);
   if (lane == 0) *d.status = 0;
 }
 
@@ -637,6 +674,8 @@ __global__ void fp64_probe_kernel(double* out, double a) {
 }
 
 // host-side launchers (engine.cpp)
+size_t refactor_stash_bytes() { return kStashBytes; }
+
 hipError_t launch_nr_batch(int lu_fp32, int x_in_lds, const InstDesc* descs, const DevPlan& plan, int count,
                            const NrParams& prm, size_t lds_bytes, hipStream_t stream) {
   dim3 grid(count), block(kWave);
