@@ -23,7 +23,6 @@ hipError_t launch_nr_batch(int lu_fp32, int x_in_lds, const InstDesc* descs, con
 hipError_t launch_seam(const SeamDesc* descs, const DevPlan& plan, int count, size_t lds_bytes, hipStream_t stream);
 hipError_t launch_sincos(const double* x, double* s, double* c, int64_t n, hipStream_t stream);
 hipError_t launch_fp64_probe(double* out, hipStream_t stream);
-size_t refactor_stash_bytes();
 }  // namespace gs
 
 using namespace gs;
@@ -96,7 +95,7 @@ namespace {
 struct DevicePlan {
   DevMem y_rp, y_ci, y_row, y_diag, y_g, y_b, slots, aslots, ppos, qpos, j_rp, j_ci, vm0, va0;
   DevMem pinv, q, prow, piv_pos;
-  DevMem ch_meta, ch_a_ws, ch_a_src, tr_meta, tr_col, tr_wpos, fpos, bpos, row_rl, row_meta;
+  DevMem rs_rec, ch_a_src;
   DevMem fs_rec, bs_rec;
   DevPlan h{};   // passed to the kernels by value
   DevicePlanHost dph;
@@ -142,51 +141,16 @@ void upload_plan(DevicePlan& dp, const YBus* y, const JacobianPattern& jp, const
   h.j_rp = dp.j_rp.as<int>(); h.j_ci = dp.j_ci.as<int>();
   upload(dp.pinv, d.pinv, s); upload(dp.q, d.q, s); upload(dp.prow, d.prow, s); upload(dp.piv_pos, d.piv_pos, s);
   h.pinv = dp.pinv.as<int>(); h.q = dp.q.as<int>(); h.prow = dp.prow.as<int>(); h.piv_pos = dp.piv_pos.as<int>();
-  // the refactor's chunks and row stream
-  h.wstride = d.wstride;
-  h.n_chunks = d.n_chunks;
-  // rows and steps rounded up to whole rings of no-ops, plus one ring of look-ahead padding
+  // the refactor's row stream, rounded up to whole rings of no-op rows plus one ring of look-ahead padding
   auto ring_up = [](int v) { return (v + kStreamPad - 1) / kStreamPad * kStreamPad; };
-  h.n_rows = ring_up(d.n_rows);
-  h.n_a = (int)d.ch_a_src.size();
-  // (every stream padded past its end: the kernels load ahead without bounds checks)
-  const int n_tracks = (int)d.tr_col.size();
-  std::vector<int> row_rl(2 * ((size_t)h.n_rows + kStreamPad) * 64), row_meta((size_t)h.n_rows + kStreamPad, 0);
-  for (size_t e = 0; e < row_rl.size() / 2; e++) {
-    const bool in = e < d.row_rec.size();
-    row_rl[2 * e] = in ? d.row_rec[e] : 0xFFFF;
-    row_rl[2 * e + 1] = in ? d.row_lsrc[e] : 0;
-  }
-  for (int r = 0; r < d.n_rows; r++) row_meta[r] = (d.row_level[r] << kRowLevelShift) | d.row_flags[r];
-  std::vector<int> ch_meta(4 * ((size_t)d.n_chunks + 2));
-  for (int c = 0; c < d.n_chunks + 2; c++) {
-    const bool in = c < d.n_chunks;
-    ch_meta[4 * c] = in ? d.ch_a_ptr[c] : h.n_a;
-    ch_meta[4 * c + 1] = in ? d.ch_a_ptr[c + 1] : h.n_a;
-    ch_meta[4 * c + 2] = in ? d.ch_track_ptr[c] : n_tracks;
-    ch_meta[4 * c + 3] = in ? d.ch_track_ptr[c + 1] - d.ch_track_ptr[c] : 0;
-  }
-  std::vector<int> ch_a_ws(d.ch_a_ws), ch_a_src(d.ch_a_src);
-  ch_a_ws.resize(ch_a_ws.size() + kAPad, 0);
-  ch_a_src.resize(ch_a_src.size() + kAPad, 0);
-  std::vector<int> tr_meta(4 * ((size_t)n_tracks + kTracks), 0), tr_col(d.tr_col);
-  tr_col.resize((size_t)n_tracks + kTracks, 0);
-  for (int t = 0; t < n_tracks; t++) {
-    tr_meta[4 * t] = d.tr_u0[t];
-    tr_meta[4 * t + 1] = d.tr_nu[t];
-    tr_meta[4 * t + 2] = d.tr_l0[t];
-    tr_meta[4 * t + 3] = d.tr_nl[t];
-  }
-  upload(dp.ch_meta, ch_meta, s); upload(dp.ch_a_ws, ch_a_ws, s); upload(dp.ch_a_src, ch_a_src, s);
-  std::vector<int> tr_wpos(d.tr_wpos);
-  tr_wpos.resize(((size_t)n_tracks + kTracks) * kTrackLanes * 4, -1);
-  upload(dp.tr_meta, tr_meta, s); upload(dp.tr_col, tr_col, s); upload(dp.tr_wpos, tr_wpos, s);
-  upload(dp.fpos, d.fpos, s); upload(dp.bpos, d.bpos, s);
-  upload(dp.row_rl, row_rl, s); upload(dp.row_meta, row_meta, s);
-  h.ch_meta = dp.ch_meta.as<int4>(); h.ch_a_ws = dp.ch_a_ws.as<int>(); h.ch_a_src = dp.ch_a_src.as<int>();
-  h.tr_meta = dp.tr_meta.as<int4>(); h.tr_col = dp.tr_col.as<int>(); h.tr_wpos = dp.tr_wpos.as<int4>();
-  h.fpos = dp.fpos.as<int>(); h.bpos = dp.bpos.as<int>();
-  h.row_rl = dp.row_rl.as<int2>(); h.row_meta = dp.row_meta.as<int>();
+  h.wstride = d.wstride;
+  h.n_a = d.n_a;
+  h.n_rs = ring_up(d.n_rs);
+  std::vector<int> rs_rec(d.rs_rec);
+  for (int r = d.n_rs; r < h.n_rs + kStreamPad; r++)
+    for (int l = 0; l < 64; l++) { rs_rec.push_back(0xFFFF); rs_rec.push_back(0); rs_rec.push_back(0); rs_rec.push_back(0); }
+  upload(dp.rs_rec, rs_rec, s); upload(dp.ch_a_src, d.ch_a_src, s);
+  h.rs_rec = dp.rs_rec.as<int4>(); h.ch_a_src = dp.ch_a_src.as<int>();
   // the solves' step streams (padded with steps that touch nothing) and the factors' stream-order sizes
   h.fs_steps = ring_up(d.fs_steps);
   h.bs_steps = ring_up(d.bs_steps);
@@ -257,7 +221,7 @@ struct gs_net {
 namespace {
 
 struct Layout {
-  size_t vm, va, p, q, f, dx, r, jv, tpq, lx, fval, bval, dval, aval, xg, spec, start, stride;
+  size_t vm, va, p, q, f, dx, r, jv, tpq, vals, fval, bval, dval, xg, spec, start, stride;
 };
 
 Layout instance_layout(const DevPlan& h, bool fp32, bool x_lds) {
@@ -274,11 +238,10 @@ Layout instance_layout(const DevPlan& h, bool fp32, bool x_lds) {
   L.r = o; o += al(8 * (size_t)h.m);
   L.jv = o; o += al(8 * (size_t)h.nnz_j);
   L.tpq = o; o += al(16 * (size_t)h.nnz_y);
-  L.lx = o; o += al(t * (size_t)h.nnz_l);
+  L.vals = o; o += al(t * ((size_t)h.n_a + h.nnz_l));
   L.fval = o; o += al(t * (size_t)std::max(1, h.n_fval));
   L.bval = o; o += al(t * (size_t)std::max(1, h.n_bval));
   L.dval = o; o += al(t * (size_t)h.m);
-  L.aval = o; o += al(t * ((size_t)h.n_a + kAPad));
   L.xg = o; o += x_lds ? 0 : al(t * (size_t)h.m);
   L.spec = o; o += al(16 * (size_t)h.n);
   L.start = o; o += al(16 * (size_t)h.n);
@@ -325,9 +288,8 @@ BatchRun plan_run(const DevPlan& h, const gs_nr_options& o, bool force_fp64) {
   size_t t = r.fp32 ? 4 : 8;
   size_t ws = t * (size_t)kTracks * (size_t)h.wstride;
   size_t x = t * (size_t)h.m;
-  const size_t stash = refactor_stash_bytes();
-  r.x_lds = ws + stash + x <= kLdsLimit;
-  r.lds_bytes = ws + stash + (r.x_lds ? x : 0);
+  r.x_lds = ws + x <= kLdsLimit;
+  r.lds_bytes = ws + (r.x_lds ? x : 0);
   return r;
 }
 
@@ -389,7 +351,7 @@ double launch_instances(gs_ctx* ctx, DevicePlan& dp, DevMem& inst_buf, DevMem& d
     d.vm = (double*)(b + L.vm); d.va = (double*)(b + L.va); d.p = (double*)(b + L.p); d.q = (double*)(b + L.q);
     d.f = (double*)(b + L.f); d.dx = (double*)(b + L.dx); d.r = (double*)(b + L.r); d.jv = (double*)(b + L.jv);
     d.tpq = (double*)(b + L.tpq);
-    d.lx = b + L.lx; d.fval = b + L.fval; d.bval = b + L.bval; d.dval = b + L.dval; d.aval = b + L.aval; d.xg = run.x_lds ? nullptr : b + L.xg;
+    d.vals = b + L.vals; d.fval = b + L.fval; d.bval = b + L.bval; d.dval = b + L.dval; d.xg = run.x_lds ? nullptr : b + L.xg;
     const gs_instance& in = inst[i];
     d.n_patch = (y && netw) ? outage_patches(*y, *netw, in.outage_branch, d.patch) : 0;
     // specified injections: the network's, or a load-scaled copy
@@ -630,18 +592,18 @@ gs_status seam_solve_one(gs_ctx* ctx, int n, const int* rowptr, const int* colid
   jp.ci = ci;
   std::vector<double> xt(n);
   auto run_device = [&](SeamPlan& sp) -> int {
-    DevMem a, bb, xx, lv, fv, bv, dv, xs, rf, av, stt, desc;
+    DevMem a, bb, xx, fv, bv, dv, xs, rf, av, stt, desc;
     a.ensure(8 * (size_t)nnz); bb.ensure(8 * (size_t)n); xx.ensure(8 * (size_t)n);
-    lv.ensure(8 * (size_t)sp.dev.h.nnz_l); fv.ensure(8 * (size_t)std::max(1, sp.dev.h.n_fval)); bv.ensure(8 * (size_t)std::max(1, sp.dev.h.n_bval));
+    fv.ensure(8 * (size_t)std::max(1, sp.dev.h.n_fval)); bv.ensure(8 * (size_t)std::max(1, sp.dev.h.n_bval));
     dv.ensure(8 * (size_t)n); xs.ensure(8 * (size_t)n);
-    rf.ensure(8 * (size_t)n); av.ensure(8 * ((size_t)sp.dev.h.n_a + kAPad)); stt.ensure(4);
+    rf.ensure(8 * (size_t)n); av.ensure(8 * ((size_t)sp.dev.h.n_a + sp.dev.h.nnz_l)); stt.ensure(4);
     check(hipMemcpyAsync(a.p, val, 8 * (size_t)nnz, hipMemcpyHostToDevice, s), "seam a");
     check(hipMemcpyAsync(bb.p, b, 8 * (size_t)n, hipMemcpyHostToDevice, s), "seam b");
-    SeamDesc d{a.as<double>(),  bb.as<double>(), xx.as<double>(), lv.as<double>(), fv.as<double>(),
-               bv.as<double>(), dv.as<double>(), xs.as<double>(), rf.as<double>(), av.as<double>(), stt.as<int>()};
+    SeamDesc d{a.as<double>(),  bb.as<double>(), xx.as<double>(), av.as<double>(), fv.as<double>(),
+               bv.as<double>(), dv.as<double>(), xs.as<double>(), rf.as<double>(), stt.as<int>()};
     desc.ensure(sizeof(SeamDesc));
     check(hipMemcpyAsync(desc.p, &d, sizeof(SeamDesc), hipMemcpyHostToDevice, s), "seam desc");
-    size_t lds = 8 * (size_t)kTracks * (size_t)sp.dev.h.wstride + refactor_stash_bytes();
+    size_t lds = 8 * (size_t)kTracks * (size_t)sp.dev.h.wstride;
     check(launch_seam(desc.as<SeamDesc>(), sp.dev.h, 1, lds, s), "seam kernel");
     int status = 1;
     check(hipMemcpyAsync(&status, stt.p, 4, hipMemcpyDeviceToHost, s), "seam status");

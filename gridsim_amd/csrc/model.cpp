@@ -713,20 +713,115 @@ void compile_plan(const Plan& plan, DevicePlanHost& d) {
                d.n_fval, d.fs_steps);
   build_stream(d.bw_order, d.bw_level_end, d.bw_levels, d.ur_ptr, d.ur_col, d.ur_idx, true, d.bs_rec, d.bpos,
                d.n_bval, d.bs_steps);
-  // (the refactor reads its L values from the column-ordered copy lx: a track's lanes read consecutive entries)
-  // per track and lane: where its first two U slots and first two L slots are written (bval position, -2 for the
-  // pivot, which goes to dval; fval position; -1 for none)
-  d.tr_wpos.assign(d.tr_col.size() * TL * 4, -1);
-  for (size_t tr = 0; tr < d.tr_col.size(); tr++) {
-    const int k = d.tr_col[tr], u0 = plan.up[k], nu = plan.up[k + 1] - u0, l0 = plan.lp[k], nl = plan.lp[k + 1] - l0;
-    for (int tl = 0; tl < TL; tl++) {
-      int* wp = &d.tr_wpos[(tr * TL + tl) * 4];
-      for (int i = 0; i < 2; i++) {
-        const int tu = tl + i * TL, tlw = 1 + tl + i * TL;
-        wp[i] = tu < nu - 1 ? d.bpos[u0 + tu] : tu == nu - 1 ? -2 : -1;
-        wp[2 + i] = tlw < nl ? d.fpos[l0 + tlw] : -1;
+  // The refactor's unified row stream (see DevicePlanHost): per chunk its ASSIGN rows, its UPDATE rows (no-op rows
+  // first where a row would read a level that ended fewer than kRsValueAhead rows before), a PIVOT row and its WRITE
+  // rows. The L values the updates read live in vals at n_a + CSC index.
+  {
+    using D = DevicePlanHost;
+    d.n_a = (int)d.ch_a_src.size();
+    vector<int> col_of_l(d.nnz_l);
+    for (int j = 0; j < m; j++)
+      for (int p = plan.lp[j]; p < plan.lp[j + 1]; p++) col_of_l[p] = j;
+    vector<int> level_end_row(n_levels, -1);
+    auto emit = [&](int kind, int lv, const int* rec4) {
+      d.rs_meta.push_back(kind);
+      d.rs_level.push_back(lv);
+      d.rs_rec.insert(d.rs_rec.end(), rec4, rec4 + 4 * 64);
+    };
+    if ((int64_t)d.n_a + d.nnz_l > D::kRsIndexMask) throw std::runtime_error("plan: too many values for the row stream");
+    vector<int> rec(4 * 64);
+    auto nop = [&](int lv) {
+      for (int l = 0; l < 64; l++) { rec[4 * l] = 0xFFFF; rec[4 * l + 1] = 0; rec[4 * l + 2] = 0; rec[4 * l + 3] = 0; }
+      emit(D::kRsNop, lv, rec.data());
+    };
+    for (int c = 0; c < d.n_chunks; c++) {
+      const int t0 = d.ch_track_ptr[c], ntr = d.ch_track_ptr[c + 1] - t0;
+      const int lv = level[d.tr_col[t0]];
+      // ASSIGN rows
+      const int a0 = d.ch_a_ptr[c], a1 = d.ch_a_ptr[c + 1];
+      for (int e0 = a0; e0 < std::max(a1, a0 + 1); e0 += 64) {
+        for (int l = 0; l < 64; l++) {
+          const int e = e0 + l;
+          const bool in = e < a1;
+          rec[4 * l] = in ? d.ch_a_ws[e] : 0xFFFF;
+          rec[4 * l + 1] = in ? e : 0;
+          rec[4 * l + 2] = 0;
+          rec[4 * l + 3] = 0;
+        }
+        emit(D::kRsAssign | (e0 == a0 ? D::kRsClear : 0), lv, rec.data());
+      }
+      // UPDATE rows
+      for (int r = d.ch_row_ptr[c]; r < d.ch_row_ptr[c + 1]; r++) {
+        int need = -1;
+        bool any = false;
+        for (int l = 0; l < 64; l++) {
+          const int rr = d.row_rec[(size_t)r * 64 + l];
+          if ((rr & 0xFFFF) == 0xFFFF) continue;
+          any = true;
+          need = std::max(need, level[col_of_l[d.row_lsrc[(size_t)r * 64 + l]]]);
+        }
+        if (!any) continue;
+        if (need >= lv || level_end_row[need] < 0) throw std::runtime_error("plan: an update reads an unfinished level");
+        while ((int)d.rs_meta.size() < level_end_row[need] + D::kRsValueAhead) nop(lv);
+        for (int l = 0; l < 64; l++) {
+          const int rr = d.row_rec[(size_t)r * 64 + l];
+          const bool in = (rr & 0xFFFF) != 0xFFFF;
+          rec[4 * l] = in ? rr : 0xFFFF;
+          rec[4 * l + 1] = in ? d.n_a + d.row_lsrc[(size_t)r * 64 + l] : 0;
+          rec[4 * l + 2] = 0;
+          rec[4 * l + 3] = 0;
+        }
+        emit(D::kRsUpdate, lv, rec.data());
+      }
+      // PIVOT row
+      int wrows = 0;
+      for (int l = 0; l < 64; l++) {
+        const int g = l / TL;
+        rec[4 * l] = 0; rec[4 * l + 1] = 0; rec[4 * l + 2] = 0; rec[4 * l + 3] = -1;
+        if (g < ntr) {
+          const int k = d.tr_col[t0 + g], nu = plan.up[k + 1] - plan.up[k], nl = plan.lp[k + 1] - plan.lp[k];
+          const int ps = g * d.wstride + nu - 1;
+          rec[4 * l] = ps;
+          rec[4 * l + 2] = (ps + 1) | ((ps + nl) << 16);
+          rec[4 * l + 3] = k;
+          wrows = std::max(wrows, (nu + nl - 1 + TL - 1) / TL);
+        }
+      }
+      emit(D::kRsPivot, lv, rec.data());
+      // WRITE rows
+      for (int w = 0; w < wrows; w++) {
+        for (int l = 0; l < 64; l++) {
+          const int g = l / TL, t = w * TL + l % TL;
+          rec[4 * l] = 0; rec[4 * l + 1] = 0; rec[4 * l + 2] = 0; rec[4 * l + 3] = 0;
+          if (g >= ntr) continue;
+          const int k = d.tr_col[t0 + g], u0 = plan.up[k], nu = plan.up[k + 1] - u0, l0 = plan.lp[k];
+          const int nl = plan.lp[k + 1] - l0;
+          if (t >= nu + nl - 1) continue;
+          const int slot = g * d.wstride + t;
+          if (t < nu - 1) {
+            rec[4 * l] = slot | (D::kWriteU << 16);
+            rec[4 * l + 2] = d.bpos[u0 + t];
+          } else if (t == nu - 1) {
+            rec[4 * l] = slot | (D::kWritePivot << 16);
+            rec[4 * l + 2] = k;
+          } else {
+            const int p = l0 + (t - (nu - 1));
+            rec[4 * l] = slot | (D::kWriteL << 16);
+            rec[4 * l + 2] = d.n_a + p;
+            rec[4 * l + 3] = d.fpos[p];
+          }
+        }
+        emit(D::kRsWrite, lv, rec.data());
+      }
+      if (d.ch_fence[c]) {
+        d.rs_meta.back() |= D::kRsLevelEnd;
+        level_end_row[lv] = (int)d.rs_meta.size() - 1;
       }
     }
+    d.n_rs = (int)d.rs_meta.size();
+    // fold each row's kind and flags into its records
+    for (int r = 0; r < d.n_rs; r++)
+      for (int l = 0; l < 64; l++) d.rs_rec[((size_t)r * 64 + l) * 4 + 1] |= d.rs_meta[r] << D::kRsMetaShift;
   }
   // For the exact pivot check: how many candidates (rows not yet pivotal at step k) precede the pivot in reach order.
   d.piv_pos.assign(m, 0);
@@ -750,74 +845,83 @@ bool emulate_device_lu(const DevicePlanHost& d, int nnz_j, const double* jv, con
   const int m = d.m, G = DevicePlanHost::kTracks, TL = DevicePlanHost::kTrackLanes;
   const int wsz = G * d.wstride;
   vector<double> ws(wsz), fval(std::max(1, d.n_fval), std::nan("")), bval(std::max(1, d.n_bval), std::nan(""));
-  vector<double> dval(m, std::nan("")), lx(d.nnz_l, std::nan(""));
+  vector<double> dval(m, std::nan(""));
   auto bad = [&](const std::string& w) { why = w; return false; };
-  // A values in record order, as the mismatch kernel writes them through j2a
-  const int n_a = (int)d.ch_a_src.size();
-  vector<double> aval(n_a, std::nan(""));
+  using D = DevicePlanHost;
+  // vals = [A values in record order, as the mismatch kernel writes them through j2a | L in column order]
+  const int n_a = d.n_a, n_vals = n_a + d.nnz_l;
+  vector<double> vals(n_vals, std::nan(""));
   for (int s = 0; s < nnz_j && s < (int)d.j2a.size(); s++) {
     int e = d.j2a[s];
     if (e < -1 || e >= n_a) return bad("j2a index");
-    if (e >= 0) aval[e] = jv[s];
+    if (e >= 0) vals[e] = jv[s];
   }
-  if ((int)d.row_flags.size() != d.n_rows || (int)d.row_rec.size() != 64 * d.n_rows) return bad("row stream size");
-  int chunk = -1, cur_level = 0;
-  for (int r = 0; r < d.n_rows; r++) {
-    const int fl = d.row_flags[r];
-    if (d.row_level[r] != cur_level) return bad("row stream level out of order");
-    if (fl & DevicePlanHost::kRowBegin) {
-      chunk++;
-      if (chunk >= d.n_chunks) return bad("more chunk begins than chunks");
-      int ntr = d.ch_track_ptr[chunk + 1] - d.ch_track_ptr[chunk];
-      if (ntr < 1 || ntr > G) return bad("chunk track count");
-      std::fill(ws.begin(), ws.end(), 0.0);
-      for (int e = d.ch_a_ptr[chunk]; e < d.ch_a_ptr[chunk + 1]; e++) {
-        if (d.ch_a_ws[e] < 0 || d.ch_a_ws[e] >= wsz) return bad("A workspace index");
-        if (std::isnan(aval[e])) return bad("A value not written by the Jacobian fill");
-        ws[d.ch_a_ws[e]] = aval[e];
-      }
-    }
+  if ((int)d.rs_meta.size() != d.n_rs || (int)d.rs_rec.size() != 4 * 64 * d.n_rs) return bad("row stream size");
+  // The device loads row r's value at the end of row r - kRsValueAhead (or before the first row). fence_row[v] is the
+  // row whose level-end fence publishes vals[v] (-1: final before the refactor starts).
+  vector<int> fence_row(n_vals, -1), written_row(n_vals, -1);
+  vector<int> pending;  // values written in the current level, published at its level end
+  vector<double> piv(64, 0.0);
+  for (int r = 0; r < d.n_rs; r++) {
+    const int meta = d.rs_meta[r], kind = meta & D::kRsKindMask;
+    if (kind == D::kRsAssign && (meta & D::kRsClear)) std::fill(ws.begin(), ws.end(), 0.0);
+    bool fail = false;
     for (int lane = 0; lane < 64; lane++) {
-      int rec = d.row_rec[(size_t)r * 64 + lane];
-      int dst = rec & 0xFFFF, xj = rec >> 16;
-      if (dst == 0xFFFF) continue;
-      int lsrc = d.row_lsrc[(size_t)r * 64 + lane];
-      if (dst >= wsz || xj >= wsz || dst / d.wstride != lane / TL || xj / d.wstride != lane / TL)
-        return bad("row record outside its track's workspace");
-      if (lsrc < 0 || lsrc >= d.nnz_l || std::isnan(lx[lsrc])) return bad("row reads an L value not yet written");
-      ws[dst] = ws[dst] - lx[lsrc] * ws[xj];
-    }
-    if (fl & DevicePlanHost::kRowEnd) {
-      int t0 = d.ch_track_ptr[chunk], ntr = d.ch_track_ptr[chunk + 1] - t0;
-      for (int g = 0; g < ntr; g++) {
-        int u0 = d.tr_u0[t0 + g], nu = d.tr_nu[t0 + g], l0 = d.tr_l0[t0 + g], nl = d.tr_nl[t0 + g];
-        int k = d.tr_col[t0 + g];
-        if (u0 != d.up[k] || nu != d.up[k + 1] - d.up[k] || l0 != d.lp[k] || nl != d.lp[k + 1] - d.lp[k])
-          return bad("track metadata");
-        if (nu + nl - 1 > d.wstride) return bad("column larger than the workspace stride");
-        const double* w = &ws[g * d.wstride];
-        double pivot = w[nu - 1];
-        if (pivot == 0.0) return bad("zero pivot");
-        // each lane writes its slots where the solves read them: the first two through its prefetched codes
-        for (int tl = 0; tl < TL; tl++) {
-          const int* wp = &d.tr_wpos[((size_t)(t0 + g) * TL + tl) * 4];
-          for (int t = tl, i = 0; t < nu; t += TL, i++) {
-            int code = i == 0 ? wp[0] : i == 1 ? wp[1] : (t == nu - 1 ? -2 : d.bpos[u0 + t]);
-            if (code >= d.n_bval || code == -1) return bad("U write position");
-            if (code >= 0) bval[code] = w[t]; else dval[k] = w[t];
-            if ((code < 0) != (t == nu - 1)) return bad("U diagonal write");
-          }
-          for (int t = 1 + tl, i = 0; t < nl; t += TL, i++) {
-            int code = i == 0 ? wp[2] : i == 1 ? wp[3] : d.fpos[l0 + t];
-            if (code < 0 || code >= d.n_fval) return bad("L write position");
-            fval[code] = lx[l0 + t] = w[nu - 1 + t] / pivot;
-          }
+      const int* rr = &d.rs_rec[((size_t)r * 64 + lane) * 4];
+      if ((int)((unsigned)rr[1] >> D::kRsMetaShift) != meta) return bad("row kind differs across lanes");
+      const int rc[4] = {rr[0], rr[1] & D::kRsIndexMask, rr[2], rr[3]};
+      if (rc[1] < 0 || rc[1] >= n_vals) return bad("row value index");
+      const int load_row = r - D::kRsValueAhead;  // the value is loaded at the end of this row
+      if (kind == D::kRsAssign) {
+        if (rc[0] == 0xFFFF) continue;
+        if (rc[0] < 0 || rc[0] >= wsz) return bad("A workspace index");  // any lane may assign any slot
+        if (rc[1] >= n_a || std::isnan(vals[rc[1]])) return bad("A value not written by the Jacobian fill");
+        ws[rc[0]] = vals[rc[1]];
+      } else if (kind == D::kRsUpdate) {
+        const int dst = rc[0] & 0xFFFF, xj = rc[0] >> 16;
+        if (dst == 0xFFFF) continue;
+        if (dst >= wsz || xj >= wsz || dst / d.wstride != lane / TL || xj / d.wstride != lane / TL)
+          return bad("row record outside its track's workspace");
+        if (rc[1] < n_a || written_row[rc[1]] < 0) return bad("row reads an L value not yet written");
+        if (fence_row[rc[1]] < 0 || fence_row[rc[1]] > load_row) return bad("row value loaded before its fence");
+        ws[dst] = ws[dst] - vals[rc[1]] * ws[xj];
+      } else if (kind == D::kRsPivot) {
+        if (rc[3] < 0) continue;
+        const int ps = rc[0], lo = rc[2] & 0xFFFF, hi = rc[2] >> 16;
+        if (ps < 0 || ps >= wsz || ps / d.wstride != lane / TL || lo != ps + 1 || hi < lo || hi > (lane / TL + 1) * d.wstride)
+          return bad("pivot record");
+        piv[lane] = ws[ps];
+        fail |= piv[lane] == 0.0;
+      } else if (kind == D::kRsWrite) {
+        const int wk = rc[0] >> 16, slot = rc[0] & 0xFFFF;
+        if (wk == 0) continue;
+        if (slot >= wsz || slot / d.wstride != lane / TL) return bad("write slot");
+        if (wk == D::kWriteU) {
+          if (rc[2] < 0 || rc[2] >= d.n_bval) return bad("U write position");
+          bval[rc[2]] = ws[slot];
+        } else if (wk == D::kWritePivot) {
+          if (rc[2] < 0 || rc[2] >= m) return bad("pivot write position");
+          dval[rc[2]] = ws[slot];
+        } else if (wk == D::kWriteL) {
+          if (rc[2] < n_a || rc[2] >= n_vals || rc[3] < 0 || rc[3] >= d.n_fval) return bad("L write position");
+          if (piv[lane] == 0.0) return bad("L write without a pivot");
+          vals[rc[2]] = fval[rc[3]] = ws[slot] / piv[lane];
+          written_row[rc[2]] = r;
+          pending.push_back(rc[2]);
+        } else {
+          return bad("write kind");
         }
+      } else if (kind != D::kRsNop) {
+        return bad("row kind");
       }
     }
-    if (fl & DevicePlanHost::kRowLevelEnd) cur_level++;
+    if (fail) return bad("zero pivot");
+    if (meta & D::kRsLevelEnd) {
+      for (int v : pending) fence_row[v] = r;
+      pending.clear();
+    }
   }
-  if (chunk + 1 != d.n_chunks) return bad("chunk count");
+  if (!pending.empty()) return bad("values written after the last level end");
   vector<double> x(m, std::nan(""));
   for (int r = 0; r < m; r++) x[d.pinv[r]] = rhs[r];
   // the step streams, with the rows a step may read checked against the rows already final

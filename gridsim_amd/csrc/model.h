@@ -129,8 +129,24 @@ struct DevicePlanHost {
   static constexpr int kRowBegin = 1, kRowEnd = 2, kRowLevelEnd = 4;
   vector<int> row_flags, row_level;                 // [n_rows]
   vector<int> tr_u0, tr_nu, tr_l0, tr_nl;           // per track (chunk order)
-  vector<int> tr_wpos;                              // per track and lane: 4 write positions (see compile_plan)
   vector<int> j2a;                                  // [nnz_j]: A record of each Jacobian CSR entry, or -1
+  // The refactor as the device runs it: one stream of rows, each row one operation of the wavefront with one record
+  // (4 ints) per lane, so the kernel loads every row's record and value a fixed number of rows ahead. The record's y
+  // holds the value index in its low 24 bits and the row's kind and flags (the same in every lane) in its high 8:
+  //   ASSIGN  ws[x] = vals[y]                     (a chunk's A entries; kRsClear on its first: zero the workspace)
+  //   UPDATE  ws[x & 0xFFFF] -= vals[y] * ws[x >> 16]   (x = 0xFFFF: no update in this lane)
+  //   PIVOT   x = pivot slot, z = candidate slots [z & 0xFFFF, z >> 16), w = pivot step (or -1: idle track)
+  //   WRITE   slot x & 0xFFFF, kind x >> 16: U (bval[z] = ws), PIVOT (dval[z] = ws), L (vals[z] = fval[w] = ws / pivot)
+  // vals = [A values (n_a) | L in column order (nnz_l)], so y and the L destinations index one array. kRsLevelEnd marks
+  // the last row of a dependency level (the device fences there). No-op rows are inserted so that an UPDATE row stands
+  // at least kRsValueAhead rows after the end of every level it reads: its value, loaded that far ahead, is then final.
+  static constexpr int kRsNop = 0, kRsAssign = 1, kRsUpdate = 2, kRsPivot = 3, kRsWrite = 4;
+  static constexpr int kRsKindMask = 7, kRsClear = 8, kRsLevelEnd = 16;
+  static constexpr int kWriteU = 1, kWritePivot = 2, kWriteL = 3;
+  static constexpr int kRsValueAhead = 8;
+  static constexpr int kRsMetaShift = 24, kRsIndexMask = (1 << kRsMetaShift) - 1;
+  int n_rs = 0, n_a = 0;
+  vector<int> rs_rec, rs_meta, rs_level;            // [4 * 64 * n_rs] (meta folded into y), [n_rs], [n_rs]
   // Triangular solves as step streams: a pass is up to 64 rows of one level, one lane per row (rows by decreasing
   // length), one step per entry; record (col, value index, row, flags) per lane and step, col = -1 for no entry. The
   // factors are stored in the order the streams read them: fval holds L's off-diagonal entries in forward-stream

@@ -24,18 +24,12 @@ struct DevPlan {
   // plan (model.h DevicePlanHost)
   const int32_t *pinv, *q, *prow;     // row permutation, column permutation, pivot row of each step
   const int32_t* piv_pos;             // per step: candidates before the pivot in reach order
-  // The refactor: level chunks of up to kTracks columns, run as one stream of rows of 64 records. Every stream is
-  // padded past its end (rows, steps, chunk and track metadata, A entries) so the kernels can load ahead without
-  // bounds checks.
-  int32_t wstride, n_chunks, n_rows, n_a;
-  const int4* ch_meta;                // per chunk: (first A entry, end of its A entries, first track, tracks)
-  const int32_t *ch_a_ws, *ch_a_src;  // per A entry: workspace slot, J CSR position
-  const int4* tr_meta;                // per track: (first U entry, U count, first L entry, L count)
-  const int32_t* tr_col;              // per track: its pivot step
-  const int4* tr_wpos;                // per track and lane: write positions of its first two U and two L slots
-  const int32_t *fpos, *bpos;         // per L / U entry (CSC index): position in fval / bval (-1: diagonal)
-  const int2* row_rl;                 // per row and lane: (record, L value index)
-  const int32_t* row_meta;            // per row: level * 8 | flags (kRowBegin, kRowEnd, kRowLevelEnd)
+  // The refactor: one stream of rows, each one operation of the wavefront with one int4 record per lane (ASSIGN,
+  // UPDATE, PIVOT, WRITE; model.h DevicePlanHost), padded with no-op rows to a multiple of kStreamPad rows plus one
+  // more kStreamPad, so the kernel loads ahead without bounds checks. n_a: A entries (the first part of vals).
+  int32_t wstride, n_rs, n_a, pad_rs;
+  const int4* rs_rec;                 // [n_rs + kStreamPad][64]; y = value index | (kind | flags) << kRsMetaShift
+  const int32_t* ch_a_src;            // per A entry: J CSR position (the seam fills vals from its matrix)
   // Triangular-solve step streams: per lane and step (col, value index, row, flags). The factors live in the order
   // the streams read them: fval (L off-diagonals, forward order), bval (U off-diagonals, back order), dval (U's
   // diagonal by pivot step).
@@ -43,11 +37,14 @@ struct DevPlan {
   const int4 *fs_rec, *bs_rec;
 };
 
-constexpr int kRowBegin = 1, kRowEnd = 2, kRowLevelEnd = 4, kRowLevelShift = 3;
+// refactor row kinds and flags, write kinds (model.h DevicePlanHost)
+constexpr int kRsNop = 0, kRsAssign = 1, kRsUpdate = 2, kRsPivot = 3, kRsWrite = 4;
+constexpr int kRsKindMask = 7, kRsClear = 8, kRsLevelEnd = 16;
+constexpr int kRsMetaShift = 24, kRsIndexMask = (1 << kRsMetaShift) - 1;
+constexpr int kWriteU = 1, kWritePivot = 2, kWriteL = 3;
 constexpr int kStepBegin = 1, kStepEnd = 2;
 // load-ahead distance of the row and step streams, and the padding each stream carries
 constexpr int kStreamPad = 16;
-constexpr int kAPad = 128;            // A entries a lane may load ahead per chunk (two per lane)
 
 constexpr int kTracks = 8, kTrackLanes = 8;
 
@@ -81,9 +78,8 @@ struct InstDesc {
   double *vm, *va, *p, *q;
   double *f, *dx, *r, *jv;            // f, dx, residual, FP64 Jacobian values (CSR order)
   double* tpq;                        // per Y cell: the cell's two injection terms (P, Q)
-  void *lx;                           // L in the LU precision, column order (CSC)
+  void* vals;                         // LU precision: [Jacobian in the refactor's record order | L, column order]
   void *fval, *bval, *dval;           // the factors in the LU precision (stream order)
-  void* aval;                         // Jacobian values in the refactor's record order, LU precision
   void* xg;                           // global triangular-solve workspace (used when LDS cannot hold x)
 };
 
@@ -92,7 +88,7 @@ struct SeamDesc {
   const double* a;         // CSR values
   const double* b;
   double* x;
-  double *lx, *fval, *bval, *dval, *xs, *rowfloor, *aval;
+  double *vals, *fval, *bval, *dval, *xs, *rowfloor;
   int32_t* status;         // 0 solved, 1 refused (a pivot would move, or is at or below its row floor)
 };
 

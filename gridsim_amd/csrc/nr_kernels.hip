@@ -88,7 +88,7 @@ struct Inst {
   GLOBAL const double *pspec, *qspec, *vm_start, *va_start;
   GLOBAL double *vm, *va, *p, *q, *f, *dx, *r, *jv;
   GLOBAL v2d* tpq;
-  GLOBAL T *lx, *fval, *bval, *dval, *aval, *xg;
+  GLOBAL T *vals, *fval, *bval, *dval, *xg;
   GLOBAL const Patch* patch;
   int n_patch;
   int pidx[kMaxPatch];   // patched Y cells (-1: unused)
@@ -108,12 +108,12 @@ __device__ __forceinline__ void y_value(const DevPlan& P, const Inst<T>& I, int 
 template <typename T>
 __device__ __forceinline__ void put_j(const Inst<T>& I, int s, int a, double v) {
   if (s >= 0) I.jv[s] = v;
-  if (a >= 0) I.aval[a] = (T)v;
+  if (a >= 0) I.vals[a] = (T)v;
 }
 
 // ------------------------------------------------------------------------------------------------ mismatch + J
 // Returns max |f| (NaN-propagating) over the wave; sets *bad when a Jacobian value is not finite. Each Jacobian value
-// goes to the FP64 CSR copy (the refinement residual) and, in the LU precision, to its refactor record (aval).
+// goes to the FP64 CSR copy (the refinement residual) and, in the LU precision, to its refactor record (vals).
 template <typename T>
 __device__ double mismatch_jacobian(const DevPlan& P, const Inst<T>& I, int lane, int* bad) {
   GLOBAL const int* y_row = glob(P.y_row);
@@ -235,124 +235,72 @@ __device__ double mismatch_jacobian(const DevPlan& P, const Inst<T>& I, int lane
 //   candidates in reach order), above that row's floor; anything else refuses, as Refactor returns null.
 // Returns -1 on success, else a failing pivot step.
 //
-// The chunks' rows run as one stream through a ring of registers: each row's record (workspace slots) is loaded kRing
-// rows ahead and its L value kHalf rows ahead. An L value loaded before its row's dependency level started may be
-// stale, so at every level end (after the fence that publishes the level's L columns) the values of all rows in the
-// ring are loaded again: every row's value is then loaded after its level started. Chunk metadata runs two chunks
-// ahead, a chunk's A values, track metadata and write positions one chunk ahead. The streams are padded so that every
-// load is in bounds without a branch: the wait counts before each use are then known to the compiler, and loads
-// issued ahead stay in flight instead of being drained at every use.
+// The refactor runs the plan's unified row stream: each row is one operation of the wavefront (ASSIGN a chunk's A
+// entries into the workspace, UPDATE, PIVOT check, WRITE the factors out) with one record per lane. Every row's record
+// is loaded kRing rows ahead and its value (an A entry or an L entry from vals) kHalf rows ahead; the plan places no-op
+// rows so that no row reads an L column whose level ended fewer than kHalf rows before it, so each value is final when
+// it is loaded and nothing is ever loaded twice. The ring registers are only rewritten at fixed points of the unrolled
+// loop, so the compiler's wait counts stay partial and the loads stay in flight. The stream is padded to whole rings.
 //
 // The factors are written where the triangular solves read them: L's off-diagonal entries to fval, U's off-diagonal
-// entries to bval, the pivots to dval. L is also kept in column order (lx, CSC) for the later columns' updates: the 8
-// lanes of a track then read 8 consecutive values of one column, a single memory segment.
+// entries to bval, the pivots to dval; L also stays in column order in vals, for the later columns' updates (the 8
+// lanes of a track then read 8 consecutive values of one column, a single memory segment).
 constexpr int kRing = kStreamPad, kHalf = kRing / 2;
 
-// LDS the refactor needs beside its workspace: the stash of the current chunk's track metadata.
-constexpr int kStashBytes = kWave * (16 + 16 + 4);
-
 template <typename T, bool EXACT>
-__device__ __forceinline__ int refactor(const DevPlan& P, GLOBAL const T* aval, GLOBAL T* lx, GLOBAL T* fval,
-                                        GLOBAL T* bval, GLOBAL T* dval, LDSP T* ws, LDSP void* stash,
-                                        double growth_tol, GLOBAL const double* rowfloor, int lane,
-                                        double* min_ratio) {
-  GLOBAL const v2i* row_rl = globv<v2i>(P.row_rl);
-  GLOBAL const int* row_meta = glob(P.row_meta);
-  GLOBAL const v4i* ch_meta = globv<v4i>(P.ch_meta);
-  GLOBAL const int* ch_a_ws = glob(P.ch_a_ws);
-  GLOBAL const v4i* tr_meta = globv<v4i>(P.tr_meta);
-  GLOBAL const v4i* tr_wpos = globv<v4i>(P.tr_wpos);
-  GLOBAL const int* tr_col = glob(P.tr_col);
-  GLOBAL const int* fpos = glob(P.fpos);
-  GLOBAL const int* bpos = glob(P.bpos);
+__device__ __forceinline__ int refactor(const DevPlan& P, GLOBAL T* vals, GLOBAL T* fval, GLOBAL T* bval,
+                                        GLOBAL T* dval, LDSP T* ws, double growth_tol, GLOBAL const double* rowfloor,
+                                        int lane, double* min_ratio) {
+  GLOBAL const v4i* rs_rec = globv<v4i>(P.rs_rec);
   GLOBAL const int* piv_pos = glob(P.piv_pos);
   GLOBAL const int* prow = glob(P.prow);
-  const int g = lane / kTrackLanes, tl = lane % kTrackLanes;
-  const int wstride = P.wstride, wsz = kTracks * wstride;
-  const int nrows = P.n_rows;
+  const int tl = lane % kTrackLanes;
+  const int wsz = kTracks * P.wstride;
+  const int nrows = P.n_rs;
   double mratio = *min_ratio;
-  // ring of upcoming rows: (record, L value index), L value, level * 8 | flags. A row without an update carries index
-  // 0, so its value load is in bounds too.
-  v2i rrl[kRing];
-  int rmeta[kRing];
-  T rval[kRing];
+  // The row's kind is read from the lane's own record (every lane holds the same), not from a per-row array: a value
+  // the compiler knows to be uniform would be moved to a scalar register as soon as it is loaded, which waits for it.
+  v4i rr[kRing];
+  T rv[kRing];
 #pragma unroll
-  for (int s = 0; s < kRing; s++) {
-    rrl[s] = row_rl[(size_t)s * kWave + lane];
-    rmeta[s] = row_meta[s];
-  }
+  for (int s = 0; s < kRing; s++) rr[s] = rs_rec[(size_t)s * kWave + lane];
 #pragma unroll
-  for (int s = 0; s < kRing; s++) rval[s] = lx[rrl[s].y];
-  // chunks: metadata (first A entry, end of A entries, first track, tracks) of the chunk after next; A entries (two
-  // per lane), track metadata and write positions of the next chunk. At a chunk's first row its prefetched data is
-  // consumed at once (A values into the workspace, track metadata into the LDS stash that its last row reads back), so
-  // the next chunk's loads can land in the same registers: a value in flight is never copied (a copy would wait for
-  // it, and with it for every load issued before it).
-  int chunk = -1;
-  v4i cm_after = ch_meta[1];
-  v4i nx_cm = ch_meta[0];
-  int nx_ws0, nx_ws1, nx_k;
-  T nx_v0, nx_v1;
-  v4i nx_tr, nx_wp;
-  auto load_chunk = [&](const v4i& cm) {
-    const int e0 = cm.x + lane, e1 = cm.x + kWave + lane;
-    nx_ws0 = ch_a_ws[e0]; nx_v0 = aval[e0];
-    nx_ws1 = ch_a_ws[e1]; nx_v1 = aval[e1];
-    nx_tr = tr_meta[cm.z + g];
-    nx_k = tr_col[cm.z + g];
-    nx_wp = tr_wpos[(cm.z + g) * kTrackLanes + tl];
-    nx_cm = cm;
-  };
-  load_chunk(nx_cm);
-  LDSP v4i* st_tr = reinterpret_cast<LDSP v4i*>(stash);
-  LDSP v4i* st_wp = st_tr + kWave;
-  LDSP int* st_k = reinterpret_cast<LDSP int*>(st_wp + kWave);
-  for (int base = 0; base < nrows; base += kRing) {  // nrows: a multiple of kRing (the stream is padded with no-ops)
+  for (int s = 0; s < kHalf; s++) rv[s] = vals[rr[s].y & kRsIndexMask];
+  T piv = T(1);
+  int failed = -1;
+  // (a failed pivot check ends the loop at the next ring boundary, not at once: an exit from inside the unrolled body
+  // would make the compiler drain every load in flight at each row; the rows run meanwhile only write factors that
+  // the caller discards)
+  for (int base = 0; base < nrows && failed < 0; base += kRing) {  // nrows: a multiple of kRing
 #pragma unroll
     for (int s = 0; s < kRing; s++) {
-      const int r = base + s;
-      const int fl = rmeta[s] & 7;
-      if (fl & kRowBegin) {
-        chunk++;
-        const v4i cm = nx_cm;
-        for (int q = lane; q < wsz; q += kWave) ws[q] = T(0);
-        WAVE_SYNC();
-        if (cm.x + lane < cm.y) ws[nx_ws0] = nx_v0;
-        if (cm.x + kWave + lane < cm.y) ws[nx_ws1] = nx_v1;
-        for (int e = cm.x + 2 * kWave + lane; e < cm.y; e += kWave) ws[ch_a_ws[e]] = aval[e];
-        st_tr[lane] = nx_tr;
-        st_wp[lane] = nx_wp;
-        st_k[lane] = g < cm.w ? nx_k : -1;
-        WAVE_SYNC();
-        load_chunk(cm_after);
-        cm_after = ch_meta[chunk + 2];
-      }
-      {
-        const int rec = rrl[s].x;
-        const int dst = rec & 0xFFFF;
-        if (dst != 0xFFFF) {
-          const T xj = ws[rec >> 16];
-          ws[dst] = ws[dst] - rval[s] * xj;
+      const v4i r4 = rr[s];
+      const int meta = (int)((unsigned)r4.y >> kRsMetaShift), kind = meta & kRsKindMask;
+      if (kind == kRsAssign) {
+        if (meta & kRsClear) {
+          for (int q = lane; q < wsz; q += kWave) ws[q] = T(0);
+          WAVE_SYNC();
         }
+        if (r4.x != 0xFFFF) ws[r4.x] = rv[s];
         WAVE_SYNC();
-      }
-      if (fl & kRowEnd) {
-        // pivots, growth checks and write-out, one track per 8 lanes
+      } else if (kind == kRsUpdate) {
+        const int dst = r4.x & 0xFFFF;
+        if (dst != 0xFFFF) ws[dst] = ws[dst] - rv[s] * ws[r4.x >> 16];
+        WAVE_SYNC();
+      } else if (kind == kRsPivot) {
+        // pivot and growth check, one track per 8 lanes (all 8 lanes of a track are active or idle together)
         int fail = -1;
-        const int k = st_k[lane];
-        if (k >= 0) {
-          const v4i tr = st_tr[lane], cur_wp = st_wp[lane];
-          const int u0 = tr.x, nu = tr.y, l0 = tr.z, nl = tr.w;
-          LDSP T* w = ws + g * wstride;
-          const T pivot = w[nu - 1];
-          const double pabs = fabs((double)pivot);
+        if (r4.w >= 0) {
+          const int k = r4.w, lo = r4.z & 0xFFFF, hi = r4.z >> 16;
+          piv = ws[r4.x];
+          const double pabs = fabs((double)piv);
           double cand = pabs;
           int moved = 0;
           const int before = EXACT ? piv_pos[k] : 0;
-          for (int t = 1 + tl; t < nl; t += kTrackLanes) {
-            const double cabs = fabs((double)w[nu - 1 + t]);
+          for (int c = lo + tl; c < hi; c += kTrackLanes) {
+            const double cabs = fabs((double)ws[c]);
             cand = nan_max(cand, cabs);
-            if (EXACT) moved |= (t - 1 < before) ? !(cabs < pabs) : (cabs > pabs);
+            if (EXACT) moved |= (c - lo < before) ? !(cabs < pabs) : (cabs > pabs);
           }
           const double best = track_max(cand);
           bool bad;
@@ -363,52 +311,38 @@ __device__ __forceinline__ int refactor(const DevPlan& P, GLOBAL const T* aval, 
             bad = !(ratio >= growth_tol) || !(pabs > 0.0) || !isfinite(pabs);
             mratio = ratio < mratio ? ratio : mratio;
           }
-          if (bad) {
-            fail = k;
+          if (bad) fail = k;
+        }
+        // any failing track fails the factorisation (the first failure's lowest lane's column reported)
+        const unsigned long long fmask = __ballot(fail >= 0);
+        if (fmask && failed < 0) failed = __shfl(fail, __ffsll((long long)fmask) - 1, kWave);
+      } else if (kind == kRsWrite) {
+        const int wk = r4.x >> 16;
+        if (wk != 0) {
+          const T v = ws[r4.x & 0xFFFF];
+          if (wk == kWriteU) {
+            bval[r4.z] = v;
+          } else if (wk == kWritePivot) {
+            dval[r4.z] = v;
           } else {
-            // the lane's first two U and L slots go through the prefetched positions, any further ones (long columns)
-            // through the position maps
-            for (int t = tl, i = 0; t < nu; t += kTrackLanes, i++) {
-              const int code = i == 0 ? cur_wp.x : i == 1 ? cur_wp.y : (t == nu - 1 ? -2 : bpos[u0 + t]);
-              if (code >= 0) bval[code] = w[t];
-              else dval[k] = w[t];
-            }
-            for (int t = 1 + tl, i = 0; t < nl; t += kTrackLanes, i++) {
-              const int code = i == 0 ? cur_wp.z : i == 1 ? cur_wp.w : fpos[l0 + t];
-              const T v = w[nu - 1 + t] / pivot;
-              lx[l0 + t] = v;
-              fval[code] = v;
-            }
+            const T lv = v / piv;
+            vals[r4.z] = lv;
+            fval[r4.w] = lv;
           }
         }
-        // any failing track fails the factorisation (lowest lane's column reported)
-        const unsigned long long fmask = __ballot(fail >= 0);
-        if (fmask) {
-          const int first = __ffsll((long long)fmask) - 1;
-          return __shfl(fail, first, kWave);
-        }
-        WAVE_SYNC();
-        if (fl & kRowLevelEnd) {
-          // the level's L columns are written: publish them, then reload the values of the rows in the ring
-          WG_SYNC();
-#pragma unroll
-          for (int t = 0; t < kRing; t++) rval[t] = lx[rrl[t].y];
-        }
       }
-      // refill this slot with row r + kRing (the stream is padded by kRing rows); load the value of row r + kHalf
-      {
-        const int rn = r + kRing;
-        rrl[s] = row_rl[(size_t)rn * kWave + lane];
-        rmeta[s] = row_meta[rn];
-        const int hs = (s + kHalf) % kRing;
-        rval[hs] = lx[rrl[hs].y];
-      }
+      // the level's L columns are written: publish them before any later row loads them
+      if (meta & kRsLevelEnd) WG_SYNC();
+      // refill this slot with row base + s + kRing; load the value of row base + s + kHalf
+      rr[s] = rs_rec[(size_t)(base + s + kRing) * kWave + lane];
+      const int hs = (s + kHalf) % kRing;
+      rv[hs] = vals[rr[hs].y & kRsIndexMask];
     }
   }
   WG_SYNC();
   mratio = -wave_max(-mratio);
   *min_ratio = mratio;
-  return -1;
+  return failed;
 }
 
 // ------------------------------------------------------------------------------------------------ solves
@@ -512,20 +446,19 @@ __global__ __launch_bounds__(kWave, 2) void nr_batch_kernel(const InstDesc* __re
   I.vm = glob(d.vm); I.va = glob(d.va); I.p = glob(d.p); I.q = glob(d.q);
   I.f = glob(d.f); I.dx = glob(d.dx); I.r = glob(d.r); I.jv = glob(d.jv);
   I.tpq = (GLOBAL v2d*)d.tpq;
-  I.lx = glob(reinterpret_cast<T*>(d.lx));
+  I.vals = glob(reinterpret_cast<T*>(d.vals));
   I.fval = glob(reinterpret_cast<T*>(d.fval)); I.bval = glob(reinterpret_cast<T*>(d.bval));
   I.dval = glob(reinterpret_cast<T*>(d.dval));
-  I.aval = glob(reinterpret_cast<T*>(d.aval)); I.xg = glob(reinterpret_cast<T*>(d.xg));
+  I.xg = glob(reinterpret_cast<T*>(d.xg));
   I.patch = glob(&descs[blockIdx.x].patch[0]);
   I.n_patch = d.n_patch;
 #pragma unroll
   for (int t = 0; t < kMaxPatch; t++) I.pidx[t] = t < I.n_patch ? d.patch[t].idx : -1;
   InstState* st = d.state;
   const int n = P.n, m = P.m;
-  // LDS: the refactor's workspace, its stash, then (when it fits) the triangular-solve vector
+  // LDS: the refactor's workspace, then (when it fits) the triangular-solve vector
   LDSP T* ws = lds(reinterpret_cast<T*>(smem));
-  LDSP void* stash = lds(reinterpret_cast<void*>(smem + sizeof(T) * kTracks * P.wstride));
-  LDSP T* xl = lds(reinterpret_cast<T*>(smem + sizeof(T) * kTracks * P.wstride + kStashBytes));
+  LDSP T* xl = ws + kTracks * P.wstride;
 
   if (prm.init || prm.load_start) {
     for (int i = lane; i < n; i += kWave) { I.vm[i] = I.vm_start[i]; I.va[i] = I.va_start[i]; }
@@ -561,7 +494,7 @@ __global__ __launch_bounds__(kWave, 2) void nr_batch_kernel(const InstDesc* __re
     if (iter >= prm.max_iterations) break;
     if (steps >= prm.steps_this_launch) { paused = 1; break; }
     if (bad || !isfinite(max_mis)) break;  // NewtonShell.SolveStep refuses a non-finite J or f
-    fail_col = refactor<T, false>(P, I.aval, I.lx, I.fval, I.bval, I.dval, ws, stash, prm.growth_tol, nullptr,
+    fail_col = refactor<T, false>(P, I.vals, I.fval, I.bval, I.dval, ws, prm.growth_tol, nullptr,
                                   opaque_lane(lane), &min_ratio);
     tick(t_fac);
     if (fail_col >= 0) { paused = 2; break; }
@@ -631,10 +564,9 @@ __global__ __launch_bounds__(kWave) void seam_solve_kernel(const SeamDesc* __res
   const SeamDesc& d = descs[blockIdx.x];
   const int lane = threadIdx.x, m = P.m;
   LDSP double* ws = lds(reinterpret_cast<double*>(smem));
-  LDSP void* stash = lds(reinterpret_cast<void*>(smem + sizeof(double) * kTracks * P.wstride));
   GLOBAL const double* a = glob(d.a);
   GLOBAL double* rowfloor = glob(d.rowfloor);
-  GLOBAL double* aval = glob(d.aval);
+  GLOBAL double* vals = glob(d.vals);
   GLOBAL const int* j_rp = glob(P.j_rp);
   GLOBAL const int* ch_a_src = glob(P.ch_a_src);
   // row floors of the matrix as given (1e-12 x the largest |entry| of each row)
@@ -643,12 +575,11 @@ __global__ __launch_bounds__(kWave) void seam_solve_kernel(const SeamDesc* __res
     for (int k = j_rp[i]; k < j_rp[i + 1]; k++) mx = fabs(a[k]) > mx ? fabs(a[k]) : mx;
     rowfloor[i] = mx * 1e-12;
   }
-  for (int e = lane; e < P.n_a; e += kWave) aval[e] = a[ch_a_src[e]];
+  for (int e = lane; e < P.n_a; e += kWave) vals[e] = a[ch_a_src[e]];
   WG_SYNC();
   double ratio = 1.0;
   const int fail =
-      refactor<double, true>(P, aval, glob(d.lx), glob(d.fval), glob(d.bval), glob(d.dval), ws, stash, 0.0, rowfloor, lane,
-                             &ratio);
+      refactor<double, true>(P, vals, glob(d.fval), glob(d.bval), glob(d.dval), ws, 0.0, rowfloor, lane, &ratio);
   if (fail >= 0) {
     if (lane == 0) *d.status = 1;
     return;
@@ -674,8 +605,6 @@ __global__ void fp64_probe_kernel(double* out, double a) {
 }
 
 // host-side launchers (engine.cpp)
-size_t refactor_stash_bytes() { return kStashBytes; }
-
 hipError_t launch_nr_batch(int lu_fp32, int x_in_lds, const InstDesc* descs, const DevPlan& plan, int count,
                            const NrParams& prm, size_t lds_bytes, hipStream_t stream) {
   dim3 grid(count), block(kWave);
