@@ -27,6 +27,9 @@ hipError_t launch_fp64_probe(double* out, hipStream_t stream);
 
 using namespace gs;
 
+static_assert(DevicePlanHost::kRsValueAhead == kRsValueAhead, "plan and kernel disagree on the refactor's look-ahead");
+static_assert(DevicePlanHost::kTracks == kTracks && DevicePlanHost::kTrackLanes == kTrackLanes, "track shape");
+
 namespace {
 
 constexpr size_t kLdsLimit = 64 * 1024;  // dynamic LDS a single-wave work-group may take for the solve workspace

@@ -43,8 +43,10 @@ constexpr int kRsKindMask = 7, kRsClear = 8, kRsLevelEnd = 16;
 constexpr int kRsMetaShift = 24, kRsIndexMask = (1 << kRsMetaShift) - 1;
 constexpr int kWriteU = 1, kWritePivot = 2, kWriteL = 3;
 constexpr int kStepBegin = 1, kStepEnd = 2;
-// load-ahead distance of the row and step streams, and the padding each stream carries
+// load-ahead distance of the row and step streams, and the padding each stream carries; rows of the refactor load
+// their value kRsValueAhead rows ahead (the plan places its rows for that distance)
 constexpr int kStreamPad = 16;
+constexpr int kRsValueAhead = 8;
 
 constexpr int kTracks = 8, kTrackLanes = 8;
 

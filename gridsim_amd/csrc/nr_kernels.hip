@@ -237,15 +237,16 @@ __device__ double mismatch_jacobian(const DevPlan& P, const Inst<T>& I, int lane
 //
 // The refactor runs the plan's unified row stream: each row is one operation of the wavefront (ASSIGN a chunk's A
 // entries into the workspace, UPDATE, PIVOT check, WRITE the factors out) with one record per lane. Every row's record
-// is loaded kRing rows ahead and its value (an A entry or an L entry from vals) kHalf rows ahead; the plan places no-op
-// rows so that no row reads an L column whose level ended fewer than kHalf rows before it, so each value is final when
+// is loaded kRing rows ahead and its value (an A entry or an L entry from vals) kAhead rows ahead; the plan places no-op
+// rows so that no row reads an L column whose level ended fewer than kAhead rows before it, so each value is final when
 // it is loaded and nothing is ever loaded twice. The ring registers are only rewritten at fixed points of the unrolled
 // loop, so the compiler's wait counts stay partial and the loads stay in flight. The stream is padded to whole rings.
 //
 // The factors are written where the triangular solves read them: L's off-diagonal entries to fval, U's off-diagonal
 // entries to bval, the pivots to dval; L also stays in column order in vals, for the later columns' updates (the 8
 // lanes of a track then read 8 consecutive values of one column, a single memory segment).
-constexpr int kRing = kStreamPad, kHalf = kRing / 2;
+constexpr int kRing = kStreamPad, kHalf = kRing / 2, kAhead = kRsValueAhead;
+static_assert(kAhead < kRing, "the value of a row is loaded after its record");
 
 template <typename T, bool EXACT>
 __device__ __forceinline__ int refactor(const DevPlan& P, GLOBAL T* vals, GLOBAL T* fval, GLOBAL T* bval,
@@ -265,7 +266,7 @@ __device__ __forceinline__ int refactor(const DevPlan& P, GLOBAL T* vals, GLOBAL
 #pragma unroll
   for (int s = 0; s < kRing; s++) rr[s] = rs_rec[(size_t)s * kWave + lane];
 #pragma unroll
-  for (int s = 0; s < kHalf; s++) rv[s] = vals[rr[s].y & kRsIndexMask];
+  for (int s = 0; s < kAhead; s++) rv[s] = vals[rr[s].y & kRsIndexMask];
   T piv = T(1);
   int failed = -1;
   // (a failed pivot check ends the loop at the next ring boundary, not at once: an exit from inside the unrolled body
@@ -335,7 +336,7 @@ __device__ __forceinline__ int refactor(const DevPlan& P, GLOBAL T* vals, GLOBAL
       if (meta & kRsLevelEnd) WG_SYNC();
       // refill this slot with row base + s + kRing; load the value of row base + s + kHalf
       rr[s] = rs_rec[(size_t)(base + s + kRing) * kWave + lane];
-      const int hs = (s + kHalf) % kRing;
+      const int hs = (s + kAhead) % kRing;
       rv[hs] = vals[rr[hs].y & kRsIndexMask];
     }
   }
