@@ -195,8 +195,9 @@ gs_nr_options normalise(const gs_nr_options* o) {
   r.lu_precision = GS_LU_FP32;
   r.refine_steps = 1;
   r.growth_tol = 1e-3;
-  r.schedule = GS_SCHED_AUTO;
+  r.schedule = 0;
   if (o) r = *o;
+  if (r.schedule < 0) fail(GS_INVALID_ARGUMENT, "schedule must be >= 0");
   if (!(r.tolerance > 0)) fail(GS_INVALID_ARGUMENT, "tolerance must be positive");
   if (r.max_iterations < 0) fail(GS_INVALID_ARGUMENT, "max_iterations must be >= 0");
   if (r.refine_steps < 0 || r.refine_steps > 4) fail(GS_INVALID_ARGUMENT, "refine_steps must lie in 0..4");
@@ -315,27 +316,40 @@ void fill_result(const InstState& s, gs_nr_result& r, int refreshes, double us_p
   r.phase_us[3] = s.t_update * us_per_tick;
 }
 
-struct Prepared {
+// Host copies that must outlive the asynchronous uploads of a prepared batch.
+struct Staging {
   std::vector<InstDesc> descs;
-  std::vector<char> staging;   // per-instance spec / start uploads
-  std::vector<size_t> staging_dst;
+  std::vector<double> aux;                                  // the network's own specification
+  std::vector<std::pair<int, std::vector<double>>> stage;   // per instance: [pspec | qspec | vm0 | va0]
+  std::vector<InstState> states;                            // resumed states (uploaded when not starting afresh)
+};
+
+struct PreparedRun {
+  BatchRun run;
+  Layout L;
+  NrParams prm;
+  int count = 0;
 };
 
 }  // namespace
 
 namespace {
 
-// Core launcher for instances sharing one device plan. `states` may be pre-filled (resume) when init == 0.
-double launch_instances(gs_ctx* ctx, DevicePlan& dp, DevMem& inst_buf, DevMem& desc_buf, DevMem& state_buf,
-                        DevMem& aux_buf, int count, const gs_instance* inst, const YBus* y, const Network* netw,
-                        const BusSpec& base_spec, const gs_nr_options& o, bool plain,
-                        std::vector<InstState>& states, bool init,
-                        const std::vector<std::vector<double>>* spec_override,
-                        const std::vector<std::vector<double>>* start_override, int* launches) {
-  hipStream_t s = ctx->stream;
+// Lays `count` instances sharing one device plan out in HBM and uploads their descriptors and own data on stream s.
+// `resume` (or null: start afresh) are paused states to continue from.
+PreparedRun prepare_instances(gs_ctx* ctx, DevicePlan& dp, DevMem& inst_buf, DevMem& desc_buf, DevMem& state_buf,
+                              DevMem& aux_buf, Staging& stg, int count, const gs_instance* inst, const YBus* y,
+                              const Network* netw, const BusSpec& base_spec, const gs_nr_options& o, bool plain,
+                              const std::vector<InstState>* resume,
+                              const std::vector<std::vector<double>>* spec_override,
+                              const std::vector<std::vector<double>>* start_override, hipStream_t s) {
   const DevPlan& h = dp.h;
-  BatchRun run = plan_run(h, o, plain);
-  Layout L = instance_layout(h, run.fp32, run.x_lds);
+  PreparedRun pr;
+  pr.count = count;
+  pr.run = plan_run(h, o, plain);
+  const BatchRun& run = pr.run;
+  pr.L = instance_layout(h, run.fp32, run.x_lds);
+  const Layout& L = pr.L;
   size_t need = L.stride * (size_t)count;
   if (ctx->budget && need > ctx->budget) fail(GS_OUT_OF_MEMORY, "batch exceeds the HBM budget");
   inst_buf.ensure(need);
@@ -343,9 +357,9 @@ double launch_instances(gs_ctx* ctx, DevicePlan& dp, DevMem& inst_buf, DevMem& d
   state_buf.ensure(sizeof(InstState) * (size_t)count);
   char* base = inst_buf.as<char>();
   InstState* dstate = state_buf.as<InstState>();
-  std::vector<InstDesc> descs(count);
-  // staged per-instance host data (only instances with their own spec or start): [pspec | qspec | vm0 | va0]
-  std::vector<std::pair<int, std::vector<double>>> stage;
+  std::vector<InstDesc>& descs = stg.descs;
+  descs.assign(count, InstDesc{});
+  stg.stage.clear();
   for (int i = 0; i < count; i++) {
     InstDesc& d = descs[i];
     std::memset(&d, 0, sizeof(d));
@@ -354,7 +368,8 @@ double launch_instances(gs_ctx* ctx, DevicePlan& dp, DevMem& inst_buf, DevMem& d
     d.vm = (double*)(b + L.vm); d.va = (double*)(b + L.va); d.p = (double*)(b + L.p); d.q = (double*)(b + L.q);
     d.f = (double*)(b + L.f); d.dx = (double*)(b + L.dx); d.r = (double*)(b + L.r); d.jv = (double*)(b + L.jv);
     d.tpq = (double*)(b + L.tpq);
-    d.vals = b + L.vals; d.fval = b + L.fval; d.bval = b + L.bval; d.dval = b + L.dval; d.xg = run.x_lds ? nullptr : b + L.xg;
+    d.vals = b + L.vals; d.fval = b + L.fval; d.bval = b + L.bval; d.dval = b + L.dval;
+    d.xg = run.x_lds ? nullptr : b + L.xg;
     const gs_instance& in = inst[i];
     d.n_patch = (y && netw) ? outage_patches(*y, *netw, in.outage_branch, d.patch) : 0;
     // specified injections: the network's, or a load-scaled copy
@@ -375,36 +390,22 @@ double launch_instances(gs_ctx* ctx, DevicePlan& dp, DevMem& inst_buf, DevMem& d
       st = &own;
     }
     if (sp || st) {
-      stage.emplace_back(i, std::vector<double>(4 * (size_t)h.n, 0.0));
-      std::vector<double>& buf = stage.back().second;
+      stg.stage.emplace_back(i, std::vector<double>(4 * (size_t)h.n, 0.0));
+      std::vector<double>& buf = stg.stage.back().second;
       if (sp) std::copy(sp->begin(), sp->end(), buf.begin());
       if (st) std::copy(st->begin(), st->end(), buf.begin() + 2 * (size_t)h.n);
     }
-    if (sp) {
-      d.pspec = (const double*)(b + L.spec);
-      d.qspec = d.pspec + h.n;
-    } else {
-      d.pspec = nullptr;  // patched below to the network spec
-      d.qspec = nullptr;
-    }
-    if (st) {
-      d.vm_start = (const double*)(b + L.start);
-      d.va_start = d.vm_start + h.n;
-    } else {
-      d.vm_start = h.vm0;
-      d.va_start = h.va0;
-    }
+    d.pspec = sp ? (const double*)(b + L.spec) : nullptr;  // null: the network's spec, set below
+    d.qspec = sp ? d.pspec + h.n : nullptr;
+    d.vm_start = st ? (const double*)(b + L.start) : h.vm0;
+    d.va_start = st ? d.vm_start + h.n : h.va0;
   }
-  // the network's own specification (load scale 1) lives in the aux buffer (the host copy outlives the upload)
-  std::vector<double> aux_host(base_spec.pspec);
-  {
-    std::vector<double>& sp = aux_host;
-    sp.insert(sp.end(), base_spec.qspec.begin(), base_spec.qspec.end());
-    upload(aux_buf, sp, s);
-    for (auto& d : descs)
-      if (!d.pspec) { d.pspec = aux_buf.as<double>(); d.qspec = aux_buf.as<double>() + h.n; }
-  }
-  for (auto& [i, buf] : stage) {
+  stg.aux = base_spec.pspec;
+  stg.aux.insert(stg.aux.end(), base_spec.qspec.begin(), base_spec.qspec.end());
+  upload(aux_buf, stg.aux, s);
+  for (auto& d : descs)
+    if (!d.pspec) { d.pspec = aux_buf.as<double>(); d.qspec = aux_buf.as<double>() + h.n; }
+  for (auto& [i, buf] : stg.stage) {
     char* b = base + L.stride * i;
     if (descs[i].pspec == (const double*)(b + L.spec))
       check(hipMemcpyAsync(b + L.spec, buf.data(), 16 * (size_t)h.n, hipMemcpyHostToDevice, s), "stage spec");
@@ -413,27 +414,57 @@ double launch_instances(gs_ctx* ctx, DevicePlan& dp, DevMem& inst_buf, DevMem& d
             "stage start");
   }
   check(hipMemcpyAsync(desc_buf.p, descs.data(), sizeof(InstDesc) * count, hipMemcpyHostToDevice, s), "descs");
-  if (!init) check(hipMemcpyAsync(dstate, states.data(), sizeof(InstState) * count, hipMemcpyHostToDevice, s),
-                   "states");
-  else check(hipMemsetAsync(dstate, 0, sizeof(InstState) * count, s), "state clear");
-  NrParams prm{};
+  // a fresh start needs no state upload: the kernel's init resets every counter it reads
+  if (resume) {
+    stg.states = *resume;
+    check(hipMemcpyAsync(dstate, stg.states.data(), sizeof(InstState) * count, hipMemcpyHostToDevice, s), "states");
+  }
+  NrParams& prm = pr.prm;
+  prm = NrParams{};
   prm.tolerance = o.tolerance;
   prm.diverge_limit = o.diverge_limit;
   prm.growth_tol = o.growth_tol;
   prm.max_iterations = o.max_iterations;
   prm.refine_steps = run.fp32 ? o.refine_steps : 0;
   prm.load_start = start_override ? 1 : 0;
+  prm.steps_this_launch = 1 << 30;
+  prm.init = resume ? 0 : 1;
+  return pr;
+}
+
+// The batched launch of a prepared set: the whole Newton loop of every instance in one kernel, or (schedule > 0) in
+// launches of at most `schedule` instances one after another on the stream.
+void launch_prepared(const DevicePlan& dp, const DevMem& desc_buf, const PreparedRun& pr, int schedule, hipStream_t s) {
+  const int slice = schedule > 0 ? schedule : pr.count;
+  for (int first = 0; first < pr.count; first += slice)
+    check(launch_nr_batch(pr.run.fp32, pr.run.x_lds, desc_buf.as<InstDesc>() + first, dp.h,
+                          std::min(slice, pr.count - first), pr.prm, pr.run.lds_bytes, s),
+          "nr kernel");
+}
+
+// Core launcher for instances sharing one device plan. `states` may be pre-filled (resume) when init == 0.
+double launch_instances(gs_ctx* ctx, DevicePlan& dp, DevMem& inst_buf, DevMem& desc_buf, DevMem& state_buf,
+                        DevMem& aux_buf, int count, const gs_instance* inst, const YBus* y, const Network* netw,
+                        const BusSpec& base_spec, const gs_nr_options& o, bool plain,
+                        std::vector<InstState>& states, bool init,
+                        const std::vector<std::vector<double>>* spec_override,
+                        const std::vector<std::vector<double>>* start_override, int* launches) {
+  hipStream_t s = ctx->stream;
+  const DevPlan& h = dp.h;
+  Staging stg;
+  PreparedRun pr = prepare_instances(ctx, dp, inst_buf, desc_buf, state_buf, aux_buf, stg, count, inst, y, netw,
+                                     base_spec, o, plain, init ? nullptr : &states, spec_override, start_override, s);
+  const BatchRun& run = pr.run;
+  NrParams prm = pr.prm;
+  InstState* dstate = state_buf.as<InstState>();
   hipEvent_t e0, e1;
   check(hipEventCreate(&e0), "event");
   check(hipEventCreate(&e1), "event");
   check(hipEventRecord(e0, s), "event");
   int nl = 0;
   if (!plain) {
-    prm.steps_this_launch = 1 << 30;
-    prm.init = init ? 1 : 0;
-    check(launch_nr_batch(run.fp32, run.x_lds, desc_buf.as<InstDesc>(), h, count, prm, run.lds_bytes, s),
-          "nr kernel");
-    nl = 1;
+    launch_prepared(dp, desc_buf, pr, o.schedule, s);
+    nl = o.schedule > 0 ? (count + o.schedule - 1) / o.schedule : 1;
   } else {
     // one instance per launch, one Newton step per launch, the host reading the verdict after each
     prm.steps_this_launch = 1;
@@ -463,7 +494,6 @@ double launch_instances(gs_ctx* ctx, DevicePlan& dp, DevMem& inst_buf, DevMem& d
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (launches) *launches += nl;
-  (void)L;
   return ms;
 }
 
@@ -473,10 +503,86 @@ void copy_state_out(gs_ctx* ctx, const DevMem& inst_buf, const Layout& L, int i,
   if (va) check(hipMemcpyAsync(va, b + L.va, 8 * (size_t)n, hipMemcpyDeviceToHost, ctx->stream), "va out");
 }
 
+// A paused instance (a static pivot failed its growth check): fresh CPU symbolic passes on the instance's own network
+// at its current state (vm, va: in and out), each followed by a resumed launch on that plan, until it finishes. The
+// resumed run keeps its iteration count and start mismatch.
+void finish_paused(gs_net* net, const gs_instance& in, const gs_nr_options& o, bool plain, InstState st,
+                   std::vector<double>& vm, std::vector<double>& va, gs_nr_result& result, double& kms, int& launches) {
+  gs_ctx* ctx = net->ctx;
+  const int n = net->net.n;
+  int refreshes = 0;
+  Network own = net->net;
+  if (in.outage_branch >= 0) own.br_in[in.outage_branch] = 0;
+  if (in.load_scale != 1.0) {
+    for (int b = 0; b < own.n; b++) { own.pd[b] *= in.load_scale; own.qd[b] *= in.load_scale; }
+    for (int g = 0; g < own.ng; g++) own.gen_pg[g] *= in.load_scale;
+  }
+  YBus y = build_ybus(own, -1);
+  BusSpec spec = build_bus_spec(own, o.flat_start != 0, 1.0);
+  JacobianPattern jp = build_jacobian_pattern(y, spec.type);
+  DevicePlan dp;
+  DevMem ib, db, sb, ab;
+  bool finished = false;
+  while (!finished) {
+    refreshes++;
+    Plan plan;
+    std::string why;
+    if (!symbolic_pass(y, jp, vm.data(), va.data(), plan, why) || refreshes > 8) {
+      // singular at this state: the step cannot be solved and the run stops (NewtonShell.SolveStep false)
+      st.done = 1;
+      st.step_failures++;
+      break;
+    }
+    upload_plan(dp, &y, jp, &spec, plan, own.base, ctx->stream);
+    gs_instance one = in;
+    one.outage_branch = -1;
+    one.load_scale = 1.0;
+    std::vector<std::vector<double>> start(1);
+    start[0] = vm;
+    start[0].insert(start[0].end(), va.begin(), va.end());
+    std::vector<InstState> rs(1, st);
+    rs[0].done = 0;
+    kms += launch_instances(ctx, dp, ib, db, sb, ab, 1, &one, nullptr, nullptr, spec, o, plain, rs, false, nullptr,
+                            &start, &launches);
+    st = rs[0];
+    BatchRun r2 = plan_run(dp.h, o, plain);
+    Layout L2 = instance_layout(dp.h, r2.fp32, r2.x_lds);
+    copy_state_out(ctx, ib, L2, 0, n, vm.data(), va.data());
+    check(hipStreamSynchronize(ctx->stream), "sync resume");
+    finished = st.done == 1;
+  }
+  fill_result(st, result, refreshes, ctx->us_per_tick);
+}
+
+// Results of a finished device run of `count` instances (states copied back): finished instances straight from the
+// device, paused ones through finish_paused.
+void collect_results(gs_net* net, int count, const gs_instance* inst, const gs_nr_options& o, bool plain,
+                     const std::vector<InstState>& states, const DevMem& inst_buf, const Layout& L,
+                     gs_nr_result* results, double* vm_out, double* va_out, double& kms, int& launches) {
+  gs_ctx* ctx = net->ctx;
+  const int n = net->net.n;
+  for (int i = 0; i < count; i++) {
+    if (states[i].done == 1) {
+      fill_result(states[i], results[i], 0, ctx->us_per_tick);
+      copy_state_out(ctx, inst_buf, L, i, n, vm_out ? vm_out + (size_t)n * i : nullptr,
+                     va_out ? va_out + (size_t)n * i : nullptr);
+    }
+  }
+  check(hipStreamSynchronize(ctx->stream), "sync out");
+  for (int i = 0; i < count; i++) {
+    if (states[i].done == 1) continue;
+    std::vector<double> vm(n), va(n);
+    copy_state_out(ctx, inst_buf, L, i, n, vm.data(), va.data());
+    check(hipStreamSynchronize(ctx->stream), "sync pause");
+    finish_paused(net, inst[i], o, plain, states[i], vm, va, results[i], kms, launches);
+    if (vm_out) std::copy(vm.begin(), vm.end(), vm_out + (size_t)n * i);
+    if (va_out) std::copy(va.begin(), va.end(), va_out + (size_t)n * i);
+  }
+}
+
 void run_batch(gs_net* net, int count, const gs_instance* inst, const gs_nr_options& o, gs_nr_result* results,
                double* vm_out, double* va_out, bool plain) {
   gs_ctx* ctx = net->ctx;
-  const int n = net->net.n;
   std::vector<InstState> states;
   auto t0 = std::chrono::steady_clock::now();
   int launches = 0;
@@ -485,68 +591,7 @@ void run_batch(gs_net* net, int count, const gs_instance* inst, const gs_nr_opti
                                 &launches);
   BatchRun run = plan_run(net->dev.h, o, plain);
   Layout L = instance_layout(net->dev.h, run.fp32, run.x_lds);
-  for (int i = 0; i < count; i++) {
-    if (states[i].done == 1) {
-      fill_result(states[i], results[i], 0, ctx->us_per_tick);
-      copy_state_out(ctx, net->inst_buf, L, i, n, vm_out ? vm_out + (size_t)n * i : nullptr,
-                     va_out ? va_out + (size_t)n * i : nullptr);
-    }
-  }
-  check(hipStreamSynchronize(ctx->stream), "sync out");
-  // Paused instances: a static pivot failed its growth check. Fresh CPU symbolic pass on the instance's own network
-  // at its current state, then resume on that plan (the resumed run keeps its iteration count and start mismatch).
-  for (int i = 0; i < count; i++) {
-    if (states[i].done == 1) continue;
-    int refreshes = 0;
-    std::vector<double> vm(n), va(n);
-    copy_state_out(ctx, net->inst_buf, L, i, n, vm.data(), va.data());
-    check(hipStreamSynchronize(ctx->stream), "sync pause");
-    InstState st = states[i];
-    Network own = net->net;
-    if (inst[i].outage_branch >= 0) own.br_in[inst[i].outage_branch] = 0;
-    if (inst[i].load_scale != 1.0) {
-      for (int b = 0; b < own.n; b++) { own.pd[b] *= inst[i].load_scale; own.qd[b] *= inst[i].load_scale; }
-      for (int g = 0; g < own.ng; g++) own.gen_pg[g] *= inst[i].load_scale;
-    }
-    YBus y = build_ybus(own, -1);
-    BusSpec spec = build_bus_spec(own, o.flat_start != 0, 1.0);
-    JacobianPattern jp = build_jacobian_pattern(y, spec.type);
-    DevicePlan dp;
-    DevMem ib, db, sb, ab;
-    bool finished = false;
-    while (!finished) {
-      refreshes++;
-      Plan plan;
-      std::string why;
-      if (!symbolic_pass(y, jp, vm.data(), va.data(), plan, why) || refreshes > 8) {
-        // singular at this state: the step cannot be solved and the run stops (NewtonShell.SolveStep false)
-        st.done = 1;
-        st.step_failures++;
-        break;
-      }
-      upload_plan(dp, &y, jp, &spec, plan, own.base, ctx->stream);
-      gs_instance one = inst[i];
-      one.outage_branch = -1;
-      one.load_scale = 1.0;
-      std::vector<std::vector<double>> start(1);
-      start[0] = vm;
-      start[0].insert(start[0].end(), va.begin(), va.end());
-      std::vector<InstState> rs(1, st);
-      rs[0].done = 0;
-      // resumed: the device keeps iter / start mismatch / counters from the paused state
-      kms += launch_instances(ctx, dp, ib, db, sb, ab, 1, &one, nullptr, nullptr, spec, o, plain, rs, false, nullptr,
-                              &start, &launches);
-      st = rs[0];
-      BatchRun r2 = plan_run(dp.h, o, plain);
-      Layout L2 = instance_layout(dp.h, r2.fp32, r2.x_lds);
-      copy_state_out(ctx, ib, L2, 0, n, vm.data(), va.data());
-      check(hipStreamSynchronize(ctx->stream), "sync resume");
-      finished = st.done == 1;
-    }
-    fill_result(st, results[i], refreshes, ctx->us_per_tick);
-    if (vm_out) std::copy(vm.begin(), vm.end(), vm_out + (size_t)n * i);
-    if (va_out) std::copy(va.begin(), va.end(), va_out + (size_t)n * i);
-  }
+  collect_results(net, count, inst, o, plain, states, net->inst_buf, L, results, vm_out, va_out, kms, launches);
   net->last_kernel_ms = kms;
   net->last_launches = launches;
   net->last_total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -646,6 +691,97 @@ gs_status seam_solve_one(gs_ctx* ctx, int n, const int* rowptr, const int* colid
   std::copy(xt.begin(), xt.end(), x);
   return GS_OK;
 }
+
+}  // namespace
+
+// ================================================================================================ multi-network batches
+struct gs_batch {
+  gs_ctx* ctx = nullptr;
+  struct Group {
+    gs_net* net = nullptr;
+    int count = 0;
+    std::vector<gs_instance> inst;
+    gs_nr_options opts{};
+    DevMem inst_buf, desc_buf, state_buf, aux_buf;
+    Staging stg;
+    PreparedRun pr;
+    hipStream_t stream = nullptr;
+    hipEvent_t joined = nullptr;
+    std::vector<InstState> states;
+    std::vector<gs_nr_result> results;
+    std::vector<double> vm, va;
+  };
+  std::vector<std::unique_ptr<Group>> groups;
+  bool use_graph = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t fork = nullptr, e0 = nullptr, e1 = nullptr;
+  bool ran = false;
+  ~gs_batch() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (auto& g : groups) {
+      if (g->joined) (void)hipEventDestroy(g->joined);
+      if (g->stream) (void)hipStreamDestroy(g->stream);
+    }
+    for (hipEvent_t e : {fork, e0, e1})
+      if (e) (void)hipEventDestroy(e);
+  }
+};
+
+namespace {
+
+// Every group's launches on its own stream, forked from and joined back into the context's stream.
+void enqueue_batch(gs_batch* b) {
+  hipStream_t s0 = b->ctx->stream;
+  check(hipEventRecord(b->fork, s0), "fork");
+  for (auto& g : b->groups) {
+    check(hipStreamWaitEvent(g->stream, b->fork, 0), "fork wait");
+    launch_prepared(g->net->dev, g->desc_buf, g->pr, g->opts.schedule, g->stream);
+    check(hipEventRecord(g->joined, g->stream), "join");
+    check(hipStreamWaitEvent(s0, g->joined, 0), "join wait");
+  }
+}
+
+gs_status batch_run(gs_batch* b, double* device_ms) {
+  hipStream_t s0 = b->ctx->stream;
+  if (b->use_graph && !b->exec) {
+    check(hipStreamBeginCapture(s0, hipStreamCaptureModeGlobal), "capture");
+    enqueue_batch(b);
+    check(hipStreamEndCapture(s0, &b->graph), "end capture");
+    check(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "instantiate");
+  }
+  check(hipEventRecord(b->e0, s0), "event");
+  if (b->use_graph) check(hipGraphLaunch(b->exec, s0), "graph launch");
+  else enqueue_batch(b);
+  check(hipEventRecord(b->e1, s0), "event");
+  for (auto& g : b->groups) {
+    g->states.resize(g->count);
+    check(hipMemcpyAsync(g->states.data(), g->state_buf.p, sizeof(InstState) * g->count, hipMemcpyDeviceToHost, s0),
+          "states back");
+  }
+  check(hipStreamSynchronize(s0), "batch sync");
+  float ms = 0;
+  check(hipEventElapsedTime(&ms, b->e0, b->e1), "elapsed");
+  if (device_ms) *device_ms = ms;
+  for (auto& g : b->groups) {
+    const int n = g->net->net.n;
+    g->results.assign(g->count, gs_nr_result{});
+    g->vm.assign((size_t)n * g->count, 0.0);
+    g->va.assign((size_t)n * g->count, 0.0);
+    double kms = 0;
+    int launches = 0;
+    collect_results(g->net, g->count, g->inst.data(), g->opts, false, g->states, g->inst_buf, g->pr.L,
+                    g->results.data(), g->vm.data(), g->va.data(), kms, launches);
+  }
+  b->ran = true;
+  return GS_OK;
+}
+
+// The autotuner's candidates: arithmetic (LU precision, refinement steps) and launch slicing. A candidate must keep
+// every converged voltage within kAutotuneDv (pu, rad) of the FP64 LU's.
+struct Candidate { int32_t lu, refine, schedule; };
+constexpr double kAutotuneDv = 1e-9;
 
 }  // namespace
 
@@ -939,6 +1075,124 @@ GS_API gs_status gs_device_sincos(gs_ctx* ctx, int64_t n, const double* x, doubl
 
 GS_API void gs_host_sincos(int64_t n, const double* x, double* s, double* c) {
   for (int64_t i = 0; i < n; i++) gs_fdlibm::sincos(x[i], &s[i], &c[i]);
+}
+
+GS_API gs_status gs_batch_create(gs_ctx* ctx, int32_t n_groups, gs_net* const* nets, const int32_t* counts,
+                                 const gs_instance* const* inst, const gs_nr_options* opts, int32_t use_graph,
+                                 gs_batch** out) {
+  return guarded(ctx, [&]() -> gs_status {
+    if (!out || n_groups <= 0 || !nets || !counts || !inst) fail(GS_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    auto b = std::make_unique<gs_batch>();
+    b->ctx = ctx;
+    b->use_graph = use_graph != 0;
+    check(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming), "event");
+    check(hipEventCreate(&b->e0), "event");
+    check(hipEventCreate(&b->e1), "event");
+    for (int gi = 0; gi < n_groups; gi++) {
+      gs_net* net = nets[gi];
+      if (!net || net->ctx != ctx) fail(GS_INVALID_ARGUMENT, "group " + std::to_string(gi) + ": network");
+      if (counts[gi] <= 0 || !inst[gi]) fail(GS_INVALID_ARGUMENT, "group " + std::to_string(gi) + ": instances");
+      gs_nr_options o = normalise(opts ? &opts[gi] : nullptr);
+      if ((o.flat_start != 0) != net->flat_start) fail(GS_INVALID_ARGUMENT, "flat_start differs from the upload");
+      for (int i = 0; i < counts[gi]; i++)
+        if (inst[gi][i].outage_branch < -1 || inst[gi][i].outage_branch >= net->net.nb ||
+            !(inst[gi][i].load_scale > 0))
+          fail(GS_INVALID_ARGUMENT, "group " + std::to_string(gi) + ", instance " + std::to_string(i));
+      auto g = std::make_unique<gs_batch::Group>();
+      g->net = net;
+      g->count = counts[gi];
+      g->inst.assign(inst[gi], inst[gi] + counts[gi]);
+      g->opts = o;
+      check(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking), "stream");
+      check(hipEventCreateWithFlags(&g->joined, hipEventDisableTiming), "event");
+      g->pr = prepare_instances(ctx, net->dev, g->inst_buf, g->desc_buf, g->state_buf, g->aux_buf, g->stg, g->count,
+                                g->inst.data(), &net->y, &net->net, net->spec, o, false, nullptr, nullptr, nullptr,
+                                ctx->stream);
+      for (auto& in : g->inst) in.vm0 = in.va0 = nullptr;  // start points now live in HBM
+      b->groups.push_back(std::move(g));
+    }
+    check(hipStreamSynchronize(ctx->stream), "batch upload");
+    *out = b.release();
+    return GS_OK;
+  });
+}
+
+GS_API gs_status gs_batch_run(gs_batch* batch, double* device_ms) {
+  if (!batch) return GS_INVALID_ARGUMENT;
+  return guarded(batch->ctx, [&]() { return batch_run(batch, device_ms); });
+}
+
+GS_API gs_status gs_batch_results(gs_batch* batch, int32_t group, gs_nr_result* results, double* vm_out,
+                                  double* va_out) {
+  if (!batch) return GS_INVALID_ARGUMENT;
+  return guarded(batch->ctx, [&]() -> gs_status {
+    if (group < 0 || group >= (int)batch->groups.size() || !results) fail(GS_INVALID_ARGUMENT, "group");
+    if (!batch->ran) fail(GS_INVALID_ARGUMENT, "the batch has not run");
+    const auto& g = *batch->groups[group];
+    std::copy(g.results.begin(), g.results.end(), results);
+    if (vm_out) std::copy(g.vm.begin(), g.vm.end(), vm_out);
+    if (va_out) std::copy(g.va.begin(), g.va.end(), va_out);
+    return GS_OK;
+  });
+}
+
+GS_API void gs_batch_free(gs_batch* batch) {
+  if (!batch) return;
+  std::lock_guard<std::mutex> lock(batch->ctx->mu);
+  delete batch;
+}
+
+GS_API gs_status gs_nr_autotune(gs_net* net, int32_t count, const gs_instance* sample, gs_nr_options* opts,
+                                double* best_ms) {
+  if (!net) return GS_INVALID_ARGUMENT;
+  return guarded(net->ctx, [&]() -> gs_status {
+    if (count <= 0 || !sample || !opts) fail(GS_INVALID_ARGUMENT, "null argument");
+    gs_nr_options base = normalise(opts);
+    if ((base.flat_start != 0) != net->flat_start) fail(GS_INVALID_ARGUMENT, "flat_start differs from the upload");
+    // verdicts of the FP64 LU: the reference every candidate must reproduce
+    const size_t nv = (size_t)count * net->net.n;
+    std::vector<gs_nr_result> ref(count), res(count);
+    std::vector<double> ref_vm(nv), ref_va(nv), vm(nv), va(nv);
+    gs_nr_options o = base;
+    o.lu_precision = GS_LU_FP64;
+    o.refine_steps = 0;
+    o.schedule = 0;
+    run_batch(net, count, sample, o, ref.data(), ref_vm.data(), ref_va.data(), false);
+    std::vector<Candidate> cands;
+    for (int32_t sched : {0, 512, 256})
+      for (Candidate c : {Candidate{GS_LU_FP32, 1, 0}, Candidate{GS_LU_FP32, 0, 0}, Candidate{GS_LU_FP32, 2, 0},
+                          Candidate{GS_LU_FP64, 0, 0}})
+        if (sched == 0 || sched < count) cands.push_back({c.lu, c.refine, sched});
+    double best = 1e300;
+    gs_nr_options chosen = base;
+    for (const Candidate& c : cands) {
+      o = base;
+      o.lu_precision = c.lu;
+      o.refine_steps = c.refine;
+      o.schedule = c.schedule;
+      run_batch(net, count, sample, o, res.data(), vm.data(), va.data(), false);  // warm-up, and the states
+      const double kernel_ms = (run_batch(net, count, sample, o, res.data(), nullptr, nullptr, false),
+                                net->last_kernel_ms);
+      // the parity bar: same verdicts, converged instances within the tolerance and within kAutotuneDv of the FP64
+      // LU's voltages
+      bool ok = true;
+      for (int i = 0; i < count && ok; i++) {
+        ok = res[i].status == ref[i].status &&
+             (res[i].status != GS_PF_CONVERGED || res[i].max_mismatch < base.tolerance);
+        if (ok && res[i].status == GS_PF_CONVERGED)
+          for (size_t b = (size_t)i * net->net.n; b < (size_t)(i + 1) * net->net.n && ok; b++)
+            ok = std::fabs(vm[b] - ref_vm[b]) <= kAutotuneDv && std::fabs(va[b] - ref_va[b]) <= kAutotuneDv;
+      }
+      if (ok && kernel_ms < best) {
+        best = kernel_ms;
+        chosen = o;
+      }
+    }
+    *opts = chosen;
+    if (best_ms) *best_ms = best;
+    return GS_OK;
+  });
 }
 
 }  // extern "C"

@@ -92,7 +92,8 @@ def lib():
         for name in ("gs_init", "gs_device_info_get", "gs_sparse_solve", "gs_sparse_solve_batch",
                      "gs_network_upload", "gs_nr_solve_batch", "gs_nr_solve_plain", "gs_network_plan_sizes",
                      "gs_network_plan_get", "gs_last_timing", "gs_device_sincos", "gs_plan_build",
-                     "gs_plan_sizes", "gs_plan_get"):
+                     "gs_plan_sizes", "gs_plan_get", "gs_batch_create", "gs_batch_run", "gs_batch_results",
+                     "gs_nr_autotune"):
             getattr(L, name).restype = C.c_int32
         L.gs_host_sincos.restype = None
         _lib = L
@@ -412,3 +413,83 @@ def screen_branches_batched(ctx: Context, net: Network, branch_indices=None, opt
         return dn.solve(outages=idx, options=options, keep_state=keep_state)
     finally:
         dn.close()
+
+
+# ---------------------------------------------------------------- multi-network batches and the autotuner
+
+@dataclass
+class Group:
+    """One group of a Batch: instances of one uploaded network."""
+    net: "DeviceNetwork"
+    outages: object = None
+    load_scale: object = None
+    options: NrOptions | None = None
+    starts: object = None          # (vm[count, n], va[count, n]) or None
+    count: int | None = None
+
+
+class Batch:
+    """Instances of several networks solved together (gs_batch_*): each run launches every group's kernel on its own
+    stream, or replays one captured hipGraph of those launches (use_graph)."""
+
+    def __init__(self, ctx: Context, groups: list[Group], use_graph: bool = True):
+        self.ctx = ctx
+        self.groups = groups
+        k = len(groups)
+        nets = (C.c_void_p * k)(*[g.net.h.value for g in groups])
+        counts = (C.c_int32 * k)()
+        insts = (C.c_void_p * k)()
+        opts = (GsNrOptions * k)()
+        self._keep = []
+        for i, g in enumerate(groups):
+            arr, count, keep = g.net._instances(g.outages, g.load_scale, g.count, g.starts)
+            self._keep += [arr, keep]
+            counts[i] = count
+            insts[i] = C.cast(arr, C.c_void_p)
+            opts[i] = (g.options or g.net.options).c()
+        self.counts = [int(c) for c in counts]
+        h = C.c_void_p()
+        ctx.check(lib().gs_batch_create(ctx.h, C.c_int32(k), nets, counts, insts, opts, C.c_int32(int(use_graph)),
+                                        C.byref(h)), "gs_batch_create")
+        self.h = h
+        self._keep = []   # start points are in HBM now
+
+    def run(self) -> float:
+        """One launch of the whole batch; returns the device time in ms (HIP events)."""
+        ms = C.c_double()
+        self.ctx.check(lib().gs_batch_run(self.h, C.byref(ms)), "gs_batch_run")
+        return ms.value
+
+    def results(self, group: int, keep_state: bool = False) -> BatchResult:
+        count = self.counts[group]
+        n = self.groups[group].net.net.n_bus
+        res = np.zeros(count, RESULT_DTYPE)
+        vm = np.empty((count, n)) if keep_state else None
+        va = np.empty((count, n)) if keep_state else None
+        self.ctx.check(lib().gs_batch_results(self.h, C.c_int32(group), _p(res), _p(vm), _p(va)), "gs_batch_results")
+        return BatchResult(res, vm, va, 0.0, 0.0, 1)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().gs_batch_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def autotune(dn: "DeviceNetwork", outages=None, load_scale=None, options: NrOptions | None = None, starts=None,
+             count=None) -> tuple[NrOptions, float]:
+    """gs_nr_autotune on a sample of instances: the fastest arithmetic / launch slicing whose verdicts equal the FP64
+    LU's. Returns the chosen options and their kernel time on the sample (ms)."""
+    base = options or dn.options
+    arr, count, keep = dn._instances(outages, load_scale, count, starts)
+    o = base.c()
+    ms = C.c_double()
+    dn.ctx.check(lib().gs_nr_autotune(dn.h, C.c_int32(count), arr, C.byref(o), C.byref(ms)), "gs_nr_autotune")
+    chosen = NrOptions(o.tolerance, o.max_iterations, bool(o.flat_start), o.diverge_limit, o.lu_precision,
+                       o.refine_steps, o.growth_tol, o.schedule)
+    return chosen, ms.value

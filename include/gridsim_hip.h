@@ -50,9 +50,6 @@ enum { GS_PF_CONVERGED = 0, GS_PF_MAX_ITERATIONS = 1, GS_PF_DIVERGED = 2, GS_PF_
  * refinement residual are always FP64. */
 enum { GS_LU_FP64 = 0, GS_LU_FP32 = 1 };
 
-/* Work schedule of the batched kernel (GS_SCHED_AUTO lets the autotuner choose). */
-enum { GS_SCHED_AUTO = 0, GS_SCHED_WAVE = 1, GS_SCHED_WAVE_LEVELS = 2 };
-
 /* A network as the reference's GridModel holds it (Model.cs:20-35 Bus, :91-115 Generator, :253-276 Branch), with
  * buses identified by position. Loads and shunts in MW / MVAr, angles in degrees, impedances in pu. */
 typedef struct gs_network {
@@ -77,7 +74,8 @@ typedef struct gs_nr_options {
   int32_t lu_precision;     /* GS_LU_FP64 or GS_LU_FP32 */
   int32_t refine_steps;     /* FP64 iterative-refinement steps after each FP32 solve (0..4) */
   double growth_tol;        /* a static pivot is kept while |pivot| >= growth_tol * max |candidate| */
-  int32_t schedule;         /* GS_SCHED_* */
+  int32_t schedule;         /* launch slicing: 0 = all instances of a call in one launch, k > 0 = launches of at
+                               most k instances one after another (gs_nr_autotune chooses) */
   int32_t reserved;
 } gs_nr_options;
 
@@ -177,6 +175,30 @@ GS_API void gs_plan_free(gs_plan* plan);
  * Jacobian at the start point, and returns ||J x - f||_inf / ||f||_inf. GS_INVALID_ARGUMENT (with the reason in
  * *why, 128 bytes) when an index falls outside its buffer. */
 GS_API gs_status gs_plan_selfcheck(const gs_plan* plan, double* rel_residual, char* why);
+
+/* ---- multi-network batches ------------------------------------------------------------------------------------
+ * Instances of several uploaded networks (groups) solved together, as ContingencyAnalysis.ScreenBranchesBatched
+ * drives its lanes in lock-step (src/GridSim.Core/Analysis/ContingencyAnalysis.Batched.cs:55-266, BatchedStepHub):
+ * gs_batch_create lays every group's instances out in HBM once (outages, load-scaled specifications, start points);
+ * each gs_batch_run launches every group's kernel on its own stream, forked from and joined into the context's
+ * stream, or with use_graph replays one captured hipGraph of those launches. opts: one per group (null: defaults).
+ * Results of the last run per group: results[count], vm_out / va_out (radians, count * n_bus, may be null). */
+typedef struct gs_batch gs_batch;
+GS_API gs_status gs_batch_create(gs_ctx* ctx, int32_t n_groups, gs_net* const* nets, const int32_t* counts,
+                                 const gs_instance* const* inst, const gs_nr_options* opts, int32_t use_graph,
+                                 gs_batch** out);
+GS_API gs_status gs_batch_run(gs_batch* batch, double* device_ms);
+GS_API gs_status gs_batch_results(gs_batch* batch, int32_t group, gs_nr_result* results, double* vm_out,
+                                  double* va_out);
+GS_API void gs_batch_free(gs_batch* batch);
+
+/* Autotuner (no counterpart in the reference, whose CPU path has nothing to tune): solves the `count` sample
+ * instances under each candidate - FP32 LU with 0, 1 or 2 FP64 refinement steps, or FP64 LU; all instances in one
+ * launch or slices of 512 or 256 - and writes into *opts the fastest whose verdicts equal the FP64 LU's, whose
+ * converged instances meet the tolerance and whose converged voltages lie within 1e-9 (pu, rad) of the FP64 LU's.
+ * *best_ms: that candidate's kernel time on the sample. */
+GS_API gs_status gs_nr_autotune(gs_net* net, int32_t count, const gs_instance* sample, gs_nr_options* opts,
+                                double* best_ms);
 
 /* Kernel timing of the last gs_nr_solve_batch on this network (milliseconds, HIP events on its stream). */
 GS_API gs_status gs_last_timing(const gs_net* net, double* total_ms, double* kernel_ms, int32_t* launches);

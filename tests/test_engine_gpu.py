@@ -176,3 +176,68 @@ def test_sparse_solve_seam_errors(ctx, eng):
     assert out[0] is None and errs[0] is None
     np.testing.assert_allclose(out[1], np.linalg.solve([[1, 2], [3, 4]], [1, 1]))
     assert out[2] is None and isinstance(errs[2], ValueError)
+
+
+@pytest.fixture(scope="module")
+def two_networks(oracle):
+    """The 118-bus grid and a 300-bus sized grid, each warm-started from its solved base case."""
+    from gridsim_amd import gridgen
+    out = []
+    for net in (gridgen.config1_grid(), gridgen.sized_grid(300, gridgen.BASE_SEED + 3)):
+        rec, vm, va = oracle.solve(net, oracle.options(flat_start=True, max_iterations=60))
+        assert rec["status"] == 0
+        out.append(net.with_start(vm, np.degrees(va)))
+    return out
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_batch_of_two_networks_equals_separate_solves(ctx, eng, two_networks, use_graph):
+    """A multi-network batch (one stream per group, or one hipGraph replayed twice) gives every instance exactly the
+    result of solving its network's instances on their own."""
+    from gridsim_amd import gridgen
+    rng = np.random.default_rng(7)
+    groups, singles = [], []
+    for net in two_networks:
+        dn = eng.DeviceNetwork(ctx, net, eng.NrOptions(max_iterations=10))
+        outs = rng.choice(gridgen.config2_outages(net), 12, replace=False)
+        scales = rng.uniform(0.8, 1.2, 12)
+        groups.append(eng.Group(dn, outages=outs, load_scale=scales))
+        singles.append(dn.solve(outages=outs, load_scale=scales))
+    b = eng.Batch(ctx, groups, use_graph=use_graph)
+    for rep in range(2):
+        ms = b.run()
+        print(f"batch (graph={use_graph}) run {rep}: {ms:.3f} ms", flush=True)
+        for g, single in enumerate(singles):
+            r = b.results(g, keep_state=True)
+            assert np.array_equal(r.status, single.status)
+            assert np.array_equal(r.results["iterations"], single.results["iterations"])
+            assert np.array_equal(r.vm, single.vm) and np.array_equal(r.va, single.va)
+    b.close()
+
+
+def test_autotune_choice_keeps_the_oracle_verdicts(ctx, eng, oracle, n1_118):
+    warm, outs, (rres, rvm, rva) = n1_118
+    dn = eng.DeviceNetwork(ctx, warm, eng.NrOptions(max_iterations=10))
+    chosen, ms = eng.autotune(dn, outages=outs)
+    print(f"autotune: lu {chosen.lu_precision} refine {chosen.refine_steps} schedule {chosen.schedule}, "
+          f"{ms:.3f} ms", flush=True)
+    assert chosen.lu_precision in (eng.LU_FP32, eng.LU_FP64) and 0 <= chosen.refine_steps <= 2
+    r = dn.solve(outages=outs, options=chosen)
+    _compare(oracle, r.results, r.vm, r.va, rres, rvm, rva, "N-1 118 autotuned")
+
+
+def test_10k_bus_grid_solve_vector_in_global_memory(ctx, eng, oracle):
+    """A network whose triangular-solve vector does not fit in LDS (the kernel keeps it in global memory)."""
+    from gridsim_amd import gridgen
+    net = gridgen.sized_grid(10000, gridgen.BASE_SEED + 3)
+    rec, vm, va = oracle.solve(net, oracle.options(flat_start=True, max_iterations=60))
+    assert rec["status"] == 0
+    warm = net.with_start(vm, np.degrees(va))
+    outs = gridgen.config2_outages(warm)[::1500][:6]
+    rres, rvm, rva = oracle.screen(warm, outs, oracle.options(max_iterations=10), threads=6)
+    dn = eng.DeviceNetwork(ctx, warm, eng.NrOptions(max_iterations=10))
+    m = len(dn.plan()["q"])
+    assert 4 * m > 64 * 1024, "the solve vector should not fit in LDS for this test"
+    r = dn.solve(outages=outs)
+    print(f"10k N-1 sample: m={m}, {len(outs)} outages, kernel {r.kernel_ms:.2f} ms", flush=True)
+    _compare(oracle, r.results, r.vm, r.va, rres, rvm, rva, "N-1 10k sample")
