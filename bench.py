@@ -68,7 +68,9 @@ def run_config3(args):
         dn = engine.DeviceNetwork(ctx, warm, engine.NrOptions(max_iterations=10))
         opts, ms = engine.autotune(dn, outages=outs, load_scale=scales)
         tuned.append({"n_bus": n, "lu": "fp32" if opts.lu_precision == engine.LU_FP32 else "fp64",
-                      "refine_steps": opts.refine_steps, "schedule": opts.schedule, "sample_kernel_ms": round(ms, 3)})
+                      "refine_steps": opts.refine_steps, "schedule": opts.schedule,
+                      "solve_vector": "global" if opts.flags & engine.FLAG_X_GLOBAL else "lds when it fits",
+                      "sample_kernel_ms": round(ms, 3)})
         print(f"# {n} buses: m={len(dn.plan()['q'])}, autotuned {tuned[-1]}", file=sys.stderr, flush=True)
         groups.append(engine.Group(dn, outages=outs, load_scale=scales, options=opts))
         nets.append(warm)
@@ -148,6 +150,8 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=48)
     ap.add_argument("--cpu-sample", type=int, default=800)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--lu", default="auto", choices=("auto", "fp32", "fp64"),
+                    help="LU arithmetic of the batched sweep: auto = the autotuner's choice on the sweep itself")
     ap.add_argument("--config", type=int, default=2, choices=(2, 3),
                     help="2: N-1 sweep of the 2,000-bus grid (the headline); 3: 512 uneven instances, hipGraph")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -184,20 +188,31 @@ def main():
     base.close()
     warm = net.with_start(b.vm[0], np.degrees(b.va[0]))
     outages = gridgen.config2_outages(warm)
-    opts = engine.NrOptions(max_iterations=10, tolerance=1e-8, lu_precision=engine.LU_FP32, refine_steps=1)
-    dn = engine.DeviceNetwork(ctx, warm, opts)
+    fp32 = engine.NrOptions(max_iterations=10, tolerance=1e-8, lu_precision=engine.LU_FP32, refine_steps=1)
+    fp64 = engine.NrOptions(max_iterations=10, tolerance=1e-8, lu_precision=engine.LU_FP64, refine_steps=0)
+    dn = engine.DeviceNetwork(ctx, warm, fp32)
     plan = dn.plan()
+    tuned_ms = None
+    if args.lu == "auto":
+        opts, tuned_ms = engine.autotune(dn, outages=outages, options=fp32)
+    else:
+        opts = fp32 if args.lu == "fp32" else fp64
+    lu_name = ("fp32 + %d fp64 refinement step(s)" % opts.refine_steps if opts.lu_precision == engine.LU_FP32
+               else "fp64")
+    x_place = "global" if opts.flags & engine.FLAG_X_GLOBAL else "lds when it fits"
+    print(f"# arithmetic: {lu_name}, schedule {opts.schedule}, solve vector {x_place}" +
+          (" (autotuned)" if tuned_ms else ""), file=sys.stderr, flush=True)
     print(f"# setup {time.time() - t0:.1f}s: {net.n_bus} buses, {net.n_branch} branches, {len(outages)} outages, "
           f"m={len(plan['q'])}, nnz(J)={len(plan['j_colidx'])}, nnz(L+U)={len(plan['li']) + len(plan['ui'])}",
           file=sys.stderr, flush=True)
 
     for w in range(args.warmup):
-        r = dn.solve(outages=outages, keep_state=False)
+        r = dn.solve(outages=outages, options=opts, keep_state=False)
     sweep_ms, kernel_ms = [], []
     t_start = time.perf_counter()
     for k in range(args.steps):
         ts = time.perf_counter()
-        r = dn.solve(outages=outages, keep_state=False)
+        r = dn.solve(outages=outages, options=opts, keep_state=False)
         sweep_ms.append(1000 * (time.perf_counter() - ts))
         kernel_ms.append(r.kernel_ms)
         print(f"# step {k}: sweep {sweep_ms[-1]:.2f} ms, kernel {r.kernel_ms:.2f} ms", file=sys.stderr, flush=True)
@@ -212,7 +227,8 @@ def main():
 
     # roofline of the batched kernel: algorithmic bytes of one launch / its average duration (HIP events)
     nnz_y = net.n_bus + 2 * net.n_branch
-    alg = algorithmic_bytes(plan, net.n_bus, nnz_y, res["iterations"], opts.refine_steps, 4)
+    alg = algorithmic_bytes(plan, net.n_bus, nnz_y, res["iterations"], opts.refine_steps if opts.lu_precision ==
+                            engine.LU_FP32 else 0, 4 if opts.lu_precision == engine.LU_FP32 else 8)
     kavg = float(np.mean(kernel_ms))
     achieved = alg / (kavg / 1000) / 1e9
 
@@ -233,7 +249,7 @@ def main():
         threads = min(os.cpu_count() or 1, 16)
         cs = outages[:: max(1, len(outages) // args.cpu_sample)][: args.cpu_sample]
         tc = time.perf_counter()
-        cres, _, _ = O.screen(warm, cs, O.options(max_iterations=10), threads=threads, keep_state=False)
+        cres, cvm, cva = O.screen(warm, cs, O.options(max_iterations=10), threads=threads, keep_state=True)
         csec = time.perf_counter() - tc
         cpu = {"value": round(int((cres["status"] == 0).sum()) / csec, 1), "unit": "converged solves/s",
                "cores": threads, "kind": "port",
@@ -242,6 +258,11 @@ def main():
                          f"{1000 * np.median(cres['solve_seconds']):.1f} ms single-thread)"}
         idx = np.searchsorted(outages, cs)
         verdicts_match = bool(np.array_equal(res["status"][idx], cres["status"]))
+        # the parity bar on the sample: |dV| <= 1e-9 (pu, rad) of every converged instance against the oracle
+        chk = dn.solve(outages=cs, options=opts, keep_state=True)
+        conv = cres["status"] == 0
+        max_dv = float(max(np.abs(chk.vm[conv] - cvm[conv]).max(), np.abs(chk.va[conv] - cva[conv]).max()))
+        verdicts_match = verdicts_match and max_dv <= 1e-9
 
     line = {
         "metric": "N-1 NR solves/s at 2k buses (1e-8 pu) + p50/p99 solve latency, 1 MI355X",
@@ -254,12 +275,14 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32-lu+f64",
+        "dtype": "f32-lu+f64" if opts.lu_precision == engine.LU_FP32 else "f64",
         "data": "synthetic",
         "config": {"workload": "batched N-1 screen, 2000-bus synthetic grid (1 slack / 300 PV), every non-bridge "
                                "single-branch outage from the base-case voltages, max 10 iterations, 1e-8 pu",
                    "n_bus": net.n_bus, "n_branch": net.n_branch, "outages": int(len(outages)),
-                   "seed": int(net.seed), "lu": "fp32 + 1 fp64 refinement step", "device": info["name"]},
+                   "seed": int(net.seed), "lu": lu_name, "schedule": opts.schedule, "solve_vector": x_place,
+                   "autotuned": tuned_ms is not None,
+                   "device": info["name"] or info["arch"]},
         "latency_ms": {"p50": round(float(np.percentile(sweep_ms, 50)), 3),
                        "p99": round(float(np.percentile(sweep_ms, 99)), 3),
                        "note": "every solve of a sweep completes when its batched launch does: per-solve latency "
@@ -267,13 +290,15 @@ def main():
         "verdicts": {"converged": converged, "max_iterations": int((res["status"] == 1).sum()),
                      "diverged": int((res["status"] == 2).sum()), "nonfinite": int((res["status"] == 3).sum()),
                      "symbolic_refreshes": int(res["symbolic_refreshes"].sum()),
-                     "match_cpu_oracle_on_sample": verdicts_match},
+                     "match_cpu_oracle_on_sample": verdicts_match,
+                     "max_dv_vs_oracle_on_sample": None if cpu is None else max_dv},
         "plain_fp64_one_instance_per_launch": {"value": round(plain_rate, 2), "unit": "converged solves/s",
                                                "sample": int(len(sample)), "latency_ms_per_solve": round(plain_lat, 2),
                                                "speedup_of_batched": round(value / plain_rate, 1)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                     "kernel": "nr_batch_kernel<float,true>", "kernel_ms": round(kavg, 3),
+                     "kernel": "nr_batch_kernel<%s,true>" % ("float" if opts.lu_precision == engine.LU_FP32
+                                                             else "double"), "kernel_ms": round(kavg, 3),
                      "algorithmic_bytes": int(alg),
                      "phase_us_per_instance": {n: round(float(v), 1) for n, v in zip(engine.PHASES, phase)}},
         "cpu_baseline": cpu,

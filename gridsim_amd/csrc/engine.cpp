@@ -196,7 +196,9 @@ gs_nr_options normalise(const gs_nr_options* o) {
   r.refine_steps = 1;
   r.growth_tol = 1e-3;
   r.schedule = 0;
+  r.flags = 0;
   if (o) r = *o;
+  if (r.flags & ~GS_FLAG_X_GLOBAL) fail(GS_INVALID_ARGUMENT, "unknown flags");
   if (r.schedule < 0) fail(GS_INVALID_ARGUMENT, "schedule must be >= 0");
   if (!(r.tolerance > 0)) fail(GS_INVALID_ARGUMENT, "tolerance must be positive");
   if (r.max_iterations < 0) fail(GS_INVALID_ARGUMENT, "max_iterations must be >= 0");
@@ -292,7 +294,7 @@ BatchRun plan_run(const DevPlan& h, const gs_nr_options& o, bool force_fp64) {
   size_t t = r.fp32 ? 4 : 8;
   size_t ws = t * (size_t)kTracks * (size_t)h.wstride;
   size_t x = t * (size_t)h.m;
-  r.x_lds = ws + x <= kLdsLimit;
+  r.x_lds = !(o.flags & GS_FLAG_X_GLOBAL) && ws + x <= kLdsLimit;
   r.lds_bytes = ws + (r.x_lds ? x : 0);
   return r;
 }
@@ -780,7 +782,7 @@ gs_status batch_run(gs_batch* b, double* device_ms) {
 
 // The autotuner's candidates: arithmetic (LU precision, refinement steps) and launch slicing. A candidate must keep
 // every converged voltage within kAutotuneDv (pu, rad) of the FP64 LU's.
-struct Candidate { int32_t lu, refine, schedule; };
+struct Candidate { int32_t lu, refine, schedule, flags; };
 constexpr double kAutotuneDv = 1e-9;
 
 }  // namespace
@@ -1160,10 +1162,11 @@ GS_API gs_status gs_nr_autotune(gs_net* net, int32_t count, const gs_instance* s
     o.schedule = 0;
     run_batch(net, count, sample, o, ref.data(), ref_vm.data(), ref_va.data(), false);
     std::vector<Candidate> cands;
-    for (int32_t sched : {0, 512, 256})
-      for (Candidate c : {Candidate{GS_LU_FP32, 1, 0}, Candidate{GS_LU_FP32, 0, 0}, Candidate{GS_LU_FP32, 2, 0},
-                          Candidate{GS_LU_FP64, 0, 0}})
-        if (sched == 0 || sched < count) cands.push_back({c.lu, c.refine, sched});
+    for (int32_t flags : {0, (int32_t)GS_FLAG_X_GLOBAL})
+      for (int32_t sched : {0, 512, 256})
+        for (Candidate c : {Candidate{GS_LU_FP32, 1, 0, 0}, Candidate{GS_LU_FP32, 0, 0, 0},
+                            Candidate{GS_LU_FP32, 2, 0, 0}, Candidate{GS_LU_FP64, 0, 0, 0}})
+          if (sched == 0 || sched < count) cands.push_back({c.lu, c.refine, sched, flags});
     double best = 1e300;
     gs_nr_options chosen = base;
     for (const Candidate& c : cands) {
@@ -1171,6 +1174,7 @@ GS_API gs_status gs_nr_autotune(gs_net* net, int32_t count, const gs_instance* s
       o.lu_precision = c.lu;
       o.refine_steps = c.refine;
       o.schedule = c.schedule;
+      o.flags = c.flags;
       run_batch(net, count, sample, o, res.data(), vm.data(), va.data(), false);  // warm-up, and the states
       const double kernel_ms = (run_batch(net, count, sample, o, res.data(), nullptr, nullptr, false),
                                 net->last_kernel_ms);

@@ -106,16 +106,17 @@ __device__ __forceinline__ void y_value(const DevPlan& P, const Inst<T>& I, int 
 }
 
 template <typename T>
-__device__ __forceinline__ void put_j(const Inst<T>& I, int s, int a, double v) {
-  if (s >= 0) I.jv[s] = v;
+__device__ __forceinline__ void put_j(const Inst<T>& I, int s, int a, double v, bool keep_jv) {
+  if (keep_jv && s >= 0) I.jv[s] = v;
   if (a >= 0) I.vals[a] = (T)v;
 }
 
 // ------------------------------------------------------------------------------------------------ mismatch + J
 // Returns max |f| (NaN-propagating) over the wave; sets *bad when a Jacobian value is not finite. Each Jacobian value
-// goes to the FP64 CSR copy (the refinement residual) and, in the LU precision, to its refactor record (vals).
+// goes, in the LU precision, to its refactor record (vals) and, when the solves are refined (keep_jv), to the FP64 CSR
+// copy the refinement residual reads.
 template <typename T>
-__device__ double mismatch_jacobian(const DevPlan& P, const Inst<T>& I, int lane, int* bad) {
+__device__ double mismatch_jacobian(const DevPlan& P, const Inst<T>& I, int lane, bool keep_jv, int* bad) {
   GLOBAL const int* y_row = glob(P.y_row);
   GLOBAL const int* y_ci = glob(P.y_ci);
   GLOBAL const double* y_g = glob(P.y_g);
@@ -163,10 +164,10 @@ __device__ double mismatch_jacobian(const DevPlan& P, const Inst<T>& I, int lane
           const double nn = vmi * (g * cs + b * sn);
           const double mm = -vmi * vmj * (g * cs + b * sn);
           const double l = vmi * (g * sn - b * cs);
-          put_j(I, s.x, a.x, h);
-          put_j(I, s.y, a.y, nn);
-          put_j(I, s.z, a.z, mm);
-          put_j(I, s.w, a.w, l);
+          put_j(I, s.x, a.x, h, keep_jv);
+          put_j(I, s.y, a.y, nn, keep_jv);
+          put_j(I, s.z, a.z, mm, keep_jv);
+          put_j(I, s.w, a.w, l, keep_jv);
           lbad |= !(isfinite(h) && isfinite(nn) && isfinite(mm) && isfinite(l));
         }
       }
@@ -213,10 +214,10 @@ __device__ double mismatch_jacobian(const DevPlan& P, const Inst<T>& I, int lane
       const double nn = P_i / vmi + g * vmi;
       const double mm = P_i - g * vmi * vmi;
       const double l = Q_i / vmi - b * vmi;
-      put_j(I, s.x, a.x, h);
-      put_j(I, s.y, a.y, nn);
-      put_j(I, s.z, a.z, mm);
-      put_j(I, s.w, a.w, l);
+      put_j(I, s.x, a.x, h, keep_jv);
+      put_j(I, s.y, a.y, nn, keep_jv);
+      put_j(I, s.z, a.z, mm, keep_jv);
+      put_j(I, s.w, a.w, l, keep_jv);
       lbad |= !(isfinite(h) && isfinite(nn) && isfinite(mm) && isfinite(l));
       if (pp >= 0) { const double fv = ps - P_i; I.f[pp] = fv; lmax = nan_max(lmax, fabs(fv)); }
       if (qq >= 0) { const double fv = qs - Q_i; I.f[qq] = fv; lmax = nan_max(lmax, fabs(fv)); }
@@ -486,7 +487,7 @@ __global__ __launch_bounds__(kWave, 2) void nr_batch_kernel(const InstDesc* __re
   };
   for (;;) {
     int bad;
-    max_mis = mismatch_jacobian<T>(P, I, opaque_lane(lane), &bad);
+    max_mis = mismatch_jacobian<T>(P, I, opaque_lane(lane), prm.refine_steps > 0, &bad);
     WG_SYNC();
     tick(t_mis);
     if (iter == 0) start_mis = max_mis;

@@ -30,6 +30,7 @@ GS_OK, GS_SINGULAR, GS_NONFINITE_INPUT, GS_OUT_OF_MEMORY, GS_INVALID_ARGUMENT, G
     GS_BUSY = range(8)
 CONVERGED, MAX_ITERATIONS, DIVERGED, NONFINITE = 0, 1, 2, 3
 LU_FP64, LU_FP32 = 0, 1
+FLAG_X_GLOBAL = 1
 STATUS_NAMES = {0: "Converged", 1: "MaxIterations", 2: "Diverged", 3: "NonFinite"}
 
 
@@ -52,7 +53,7 @@ class GsNetwork(C.Structure):
 class GsNrOptions(C.Structure):
     _fields_ = [("tolerance", C.c_double), ("max_iterations", C.c_int32), ("flat_start", C.c_int32),
                 ("diverge_limit", C.c_double), ("lu_precision", C.c_int32), ("refine_steps", C.c_int32),
-                ("growth_tol", C.c_double), ("schedule", C.c_int32), ("reserved", C.c_int32)]
+                ("growth_tol", C.c_double), ("schedule", C.c_int32), ("flags", C.c_int32)]
 
 
 class GsInstance(C.Structure):
@@ -131,10 +132,11 @@ class NrOptions:
     refine_steps: int = 1
     growth_tol: float = 1e-3
     schedule: int = 0
+    flags: int = 0                 # FLAG_X_GLOBAL: the solve vector in global memory
 
     def c(self) -> GsNrOptions:
         return GsNrOptions(self.tolerance, self.max_iterations, int(self.flat_start), self.diverge_limit,
-                           self.lu_precision, self.refine_steps, self.growth_tol, self.schedule, 0)
+                           self.lu_precision, self.refine_steps, self.growth_tol, self.schedule, self.flags)
 
 
 def host_sincos(x):
@@ -491,5 +493,5 @@ def autotune(dn: "DeviceNetwork", outages=None, load_scale=None, options: NrOpti
     ms = C.c_double()
     dn.ctx.check(lib().gs_nr_autotune(dn.h, C.c_int32(count), arr, C.byref(o), C.byref(ms)), "gs_nr_autotune")
     chosen = NrOptions(o.tolerance, o.max_iterations, bool(o.flat_start), o.diverge_limit, o.lu_precision,
-                       o.refine_steps, o.growth_tol, o.schedule)
+                       o.refine_steps, o.growth_tol, o.schedule, o.flags)
     return chosen, ms.value

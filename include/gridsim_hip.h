@@ -76,8 +76,12 @@ typedef struct gs_nr_options {
   double growth_tol;        /* a static pivot is kept while |pivot| >= growth_tol * max |candidate| */
   int32_t schedule;         /* launch slicing: 0 = all instances of a call in one launch, k > 0 = launches of at
                                most k instances one after another (gs_nr_autotune chooses) */
-  int32_t reserved;
+  int32_t flags;            /* GS_FLAG_*: placement choices (gs_nr_autotune chooses) */
 } gs_nr_options;
+
+/* gs_nr_options.flags: keep the triangular-solve vector in global memory even where it fits in LDS (smaller LDS per
+ * instance, so more instances resident per compute unit). */
+enum { GS_FLAG_X_GLOBAL = 1 };
 
 /* One solve of a batch: the uploaded network with one branch out of service (or none) and every load and generator
  * P scaled by load_scale. vm0 / va0 (radians), when non-null, replace the start point BuildBusSpec would give. */
@@ -194,7 +198,8 @@ GS_API void gs_batch_free(gs_batch* batch);
 
 /* Autotuner (no counterpart in the reference, whose CPU path has nothing to tune): solves the `count` sample
  * instances under each candidate - FP32 LU with 0, 1 or 2 FP64 refinement steps, or FP64 LU; all instances in one
- * launch or slices of 512 or 256 - and writes into *opts the fastest whose verdicts equal the FP64 LU's, whose
+ * launch or slices of 512 or 256; the solve vector in LDS or in global memory - and writes into *opts the fastest
+ * whose verdicts equal the FP64 LU's, whose
  * converged instances meet the tolerance and whose converged voltages lie within 1e-9 (pu, rad) of the FP64 LU's.
  * *best_ms: that candidate's kernel time on the sample. */
 GS_API gs_status gs_nr_autotune(gs_net* net, int32_t count, const gs_instance* sample, gs_nr_options* opts,
