@@ -2,7 +2,8 @@
 
 Workload: the 2,000-bus / 2,700-branch synthetic grid (1 slack, 300 PV), base case solved from flat start, then every
 non-bridge single-branch outage solved from the base-case voltages in one batched launch: max 10 Newton iterations,
-tolerance 1e-8 pu, FP32 LU with FP64 mismatch, refinement and update. One step = one full N-1 sweep.
+tolerance 1e-8 pu, FP64 mismatch and update, the LU arithmetic (FP32 with FP64 refinement, or FP64) and launch shape
+chosen by the autotuner on the sweep itself under the |dV| <= 1e-9 bar. One step = one full N-1 sweep.
 
 Printed as one JSON line: throughput (converged solves/s), the sweep latency, the plain FP64 one-instance-per-launch
 path on a sample (the baseline the batched engine is measured against), the roofline of the batched kernel, and the
@@ -40,6 +41,22 @@ def algorithmic_bytes(plan: dict, n_bus: int, nnz_y: int, iterations: np.ndarray
     per_step = mis + jac + lu + solve + refine * refine_b + upd
     steps = int(iterations.sum())
     return per_step * steps + mis * len(iterations)
+
+
+def measured_traffic(stats: dict, lu: str, flags: int, alg: int):
+    """HBM bytes of one launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this same workload, when
+    the plan, arithmetic and iteration counts here are the ones profiled; otherwise None (no stale figure)."""
+    path = os.path.join(ROOT, "profiles", "nr_batch_kernel_n1_2000bus_traffic.json")
+    if not os.path.exists(path) or flags != 0:
+        return None, None
+    with open(path) as fh:
+        rec = json.load(fh)
+    fp = rec["fingerprint"]
+    mine = {k: stats[k] for k in ("m", "nnz_j", "nnz_l", "nnz_u", "refactor_rows", "forward_steps", "back_steps")}
+    mine.update(lu=lu, algorithmic_bytes=int(alg))
+    if any(fp.get(k) != v for k, v in mine.items()):
+        return None, None
+    return int(rec["read_bytes"] + rec["write_bytes"]), rec["source"]
 
 
 def run_config3(args):
@@ -231,6 +248,9 @@ def main():
                             engine.LU_FP32 else 0, 4 if opts.lu_precision == engine.LU_FP32 else 8)
     kavg = float(np.mean(kernel_ms))
     achieved = alg / (kavg / 1000) / 1e9
+    stats = dn.plan_stats()
+    traffic, traffic_src = measured_traffic(stats, "fp32" if opts.lu_precision == engine.LU_FP32 else "fp64",
+                                            opts.flags, alg)
 
     # the plain FP64 one-instance-per-launch path on a sample of the same outages
     sample = outages[:: max(1, len(outages) // args.plain_sample)][: args.plain_sample]
@@ -296,7 +316,9 @@ def main():
                                                "sample": int(len(sample)), "latency_ms_per_solve": round(plain_lat, 2),
                                                "speedup_of_batched": round(value / plain_rate, 1)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                     "traffic_source": traffic_src,
+                     "plan_stream_bytes": stats["stream_bytes"],
                      "kernel": "nr_batch_kernel<%s,true>" % ("float" if opts.lu_precision == engine.LU_FP32
                                                              else "double"), "kernel_ms": round(kavg, 3),
                      "algorithmic_bytes": int(alg),

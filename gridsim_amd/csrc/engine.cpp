@@ -172,6 +172,28 @@ void upload_plan(DevicePlan& dp, const YBus* y, const JacobianPattern& jp, const
   check(hipStreamSynchronize(s), "upload plan");
 }
 
+// The shape of a compiled plan as upload_plan lays it out in HBM (record streams padded the same way).
+gs_plan_stats plan_stats(const DevicePlanHost& d, const YBus& y, const JacobianPattern& jp) {
+  auto ring_up = [](int v) { return (v + kStreamPad - 1) / kStreamPad * kStreamPad; };
+  gs_plan_stats st{};
+  st.m = jp.m;
+  st.nnz_j = jp.rp[jp.m];
+  st.nnz_l = d.nnz_l;
+  st.nnz_u = d.nnz_u;
+  st.levels = d.n_levels;
+  st.refactor_rows = ring_up(d.n_rs);
+  st.forward_steps = ring_up(d.fs_steps);
+  st.back_steps = ring_up(d.bs_steps);
+  st.workspace_stride = d.wstride;
+  const int64_t n = y.n, nnz_y = y.rp.empty() ? 0 : y.rp[y.n], m = jp.m, rec_row = 64 * 16;
+  int64_t b = 4 * (n + 1) + 4 * nnz_y * 2 + 4 * n + 16 * nnz_y + 2 * 16 * nnz_y + 2 * 4 * n;  // Y-bus, slots, pos
+  b += 4 * (m + 1) + 4 * (int64_t)st.nnz_j + 2 * 8 * n + 4 * 4 * m;                            // J pattern, start, pivots
+  b += rec_row * (int64_t)(st.refactor_rows + kStreamPad) + 4 * (int64_t)d.ch_a_src.size();
+  b += rec_row * (int64_t)(st.forward_steps + kStreamPad + st.back_steps + kStreamPad);
+  st.stream_bytes = b;
+  return st;
+}
+
 // The symbolic pass: minimum-degree order of the Jacobian pattern and the Gilbert-Peierls pivot sequence of the
 // Jacobian at (vm, va), with every structural cell kept and the fixed-pattern solver's diagonal preference
 // (FixedPatternSparseLuSolver.cs:52,75-99).
@@ -968,6 +990,12 @@ GS_API gs_status gs_network_plan_get(const gs_net* net, int32_t* q, int32_t* piv
   return GS_OK;
 }
 
+GS_API gs_status gs_network_plan_stats(const gs_net* net, gs_plan_stats* out) {
+  if (!net || !out) return GS_INVALID_ARGUMENT;
+  *out = plan_stats(net->dev.dph, net->y, net->jp);
+  return GS_OK;
+}
+
 }  // extern "C"
 
 struct gs_plan {
@@ -980,6 +1008,12 @@ struct gs_plan {
 };
 
 extern "C" {
+
+GS_API gs_status gs_plan_shape(const gs_plan* plan, gs_plan_stats* out) {
+  if (!plan || !out) return GS_INVALID_ARGUMENT;
+  *out = plan_stats(plan->dph, plan->y, plan->jp);
+  return GS_OK;
+}
 
 GS_API gs_status gs_plan_build(const gs_network* net, const gs_nr_options* opts, gs_plan** out) {
   if (!net || !out || net->n_bus <= 0) return GS_INVALID_ARGUMENT;

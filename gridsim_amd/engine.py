@@ -72,6 +72,13 @@ class GsCsr(C.Structure):
                 ("val", C.c_void_p)]
 
 
+class GsPlanStats(C.Structure):
+    _fields_ = [("m", C.c_int32), ("nnz_j", C.c_int32), ("nnz_l", C.c_int32), ("nnz_u", C.c_int32),
+                ("levels", C.c_int32), ("refactor_rows", C.c_int32), ("forward_steps", C.c_int32),
+                ("back_steps", C.c_int32), ("workspace_stride", C.c_int32), ("reserved", C.c_int32),
+                ("stream_bytes", C.c_int64)]
+
+
 class GsDeviceInfo(C.Structure):
     _fields_ = [("name", C.c_char * 64), ("arch", C.c_char * 32), ("compute_units", C.c_int32),
                 ("fp64_ok", C.c_int32), ("hbm_bytes", C.c_size_t), ("fdlibm_ok", C.c_int32),
@@ -94,7 +101,7 @@ def lib():
                      "gs_network_upload", "gs_nr_solve_batch", "gs_nr_solve_plain", "gs_network_plan_sizes",
                      "gs_network_plan_get", "gs_last_timing", "gs_device_sincos", "gs_plan_build",
                      "gs_plan_sizes", "gs_plan_get", "gs_batch_create", "gs_batch_run", "gs_batch_results",
-                     "gs_nr_autotune"):
+                     "gs_nr_autotune", "gs_network_plan_stats", "gs_plan_shape"):
             getattr(L, name).restype = C.c_int32
         L.gs_host_sincos.restype = None
         _lib = L
@@ -166,6 +173,13 @@ class HostPlan:
         self.j_rowptr = np.empty(self.m + 1, np.int32); self.j_colidx = np.empty(self.nnz_j, np.int32)
         lib().gs_plan_get(h, _p(self.q), _p(self.pivot_row), _p(self.lp), _p(self.li), _p(self.up), _p(self.ui),
                           _p(self.j_rowptr), _p(self.j_colidx))
+
+    def stats(self) -> dict:
+        """The compiled device plan's shape (gs_plan_shape): record streams, levels, shared bytes."""
+        st = GsPlanStats()
+        if lib().gs_plan_shape(self.h, C.byref(st)) != GS_OK:
+            raise GsError(GS_INVALID_ARGUMENT, "gs_plan_shape")
+        return {k: getattr(st, k) for k, _ in GsPlanStats._fields_ if k != "reserved"}
 
     def __del__(self):
         try:
@@ -325,6 +339,12 @@ class DeviceNetwork:
         lib().gs_network_plan_get(self.h, *(_p(out[k]) for k in ("q", "pivot_row", "lp", "li", "up", "ui",
                                                                  "j_rowptr", "j_colidx")))
         return out
+
+    def plan_stats(self) -> dict:
+        """Shape of the compiled device plan (record streams, levels, shared bytes)."""
+        st = GsPlanStats()
+        self.ctx.check(lib().gs_network_plan_stats(self.h, C.byref(st)), "gs_network_plan_stats")
+        return {k: getattr(st, k) for k, _ in GsPlanStats._fields_ if k != "reserved"}
 
     def _instances(self, outages=None, load_scale=None, count=None, starts=None):
         count = count if count is not None else (len(outages) if outages is not None else len(load_scale))

@@ -166,6 +166,19 @@ GS_API gs_status gs_network_plan_sizes(const gs_net* net, int32_t* m, int32_t* n
 GS_API gs_status gs_network_plan_get(const gs_net* net, int32_t* q, int32_t* pivot_row, int32_t* lp, int32_t* li,
                                      int32_t* up, int32_t* ui, int32_t* j_rowptr, int32_t* j_colidx);
 
+/* Shape of the compiled device plan (no counterpart in the reference, which has no device plan): what every instance
+ * of a batch walks. stream_bytes counts the refactor and solve record streams and the per-network index arrays the
+ * kernel reads; they are shared by the instances of a launch, so their HBM cost per instance depends on cache reuse. */
+typedef struct gs_plan_stats {
+  int32_t m, nnz_j, nnz_l, nnz_u, levels;
+  int32_t refactor_rows;        /* rows of the refactor stream (64 lanes each, padding included) */
+  int32_t forward_steps, back_steps;
+  int32_t workspace_stride;     /* refactor workspace slots per track */
+  int32_t reserved;
+  int64_t stream_bytes;
+} gs_plan_stats;
+GS_API gs_status gs_network_plan_stats(const gs_net* net, gs_plan_stats* out);
+
 /* The same symbolic pass on the host alone (no device): the plan gs_network_upload would upload. For callers that
  * prepare plans on CPU threads, and for checking the ordering and pivot sequence without a GPU. */
 typedef struct gs_plan gs_plan;
@@ -175,6 +188,8 @@ GS_API gs_status gs_plan_sizes(const gs_plan* plan, int32_t* m, int32_t* nnz_j, 
 GS_API gs_status gs_plan_get(const gs_plan* plan, int32_t* q, int32_t* pivot_row, int32_t* lp, int32_t* li,
                              int32_t* up, int32_t* ui, int32_t* j_rowptr, int32_t* j_colidx);
 GS_API void gs_plan_free(gs_plan* plan);
+/* gs_network_plan_stats of the plan gs_network_upload would upload. */
+GS_API gs_status gs_plan_shape(const gs_plan* plan, gs_plan_stats* out);
 /* Runs the device factorisation and solves of the plan in a host emulation with every index checked, on the
  * Jacobian at the start point, and returns ||J x - f||_inf / ||f||_inf. GS_INVALID_ARGUMENT (with the reason in
  * *why, 128 bytes) when an index falls outside its buffer. */
