@@ -159,6 +159,187 @@ def run_config3(args):
     ctx.close()
 
 
+def _serve_stream(submit, wait, reqs, arrivals):
+    """Open loop: submit request i at arrivals[i] (seconds from the start) from this thread while a collector thread
+    waits for the tickets in submission order. Returns (results, wall seconds from the first arrival to the last
+    result)."""
+    import queue
+    import threading
+    q = queue.Queue()
+    out = [None] * len(reqs)
+
+    def collect():
+        for _ in range(len(reqs)):
+            i, t = q.get()
+            out[i] = wait(t, reqs[i][0])
+
+    th = threading.Thread(target=collect)
+    th.start()
+    t0 = time.perf_counter()
+    for i, r in enumerate(reqs):
+        while True:
+            dt = arrivals[i] - (time.perf_counter() - t0)
+            if dt <= 0:
+                break
+            time.sleep(min(dt, 0.0005) if dt > 0.0002 else 0)
+        q.put((i, submit(*r)))
+    th.join()
+    return out, time.perf_counter() - t0
+
+
+def _plain_loop(dns, opts, reqs, arrivals):
+    """The one-solve-per-launch loop the server is measured against: requests served one at a time in arrival order
+    (gs_nr_solve_batch of one instance each, the network's own options), latency = completion - arrival."""
+    lat, conv = [], 0
+    t0 = time.perf_counter()
+    for (k, o, sc), a in zip(reqs, arrivals):
+        now = time.perf_counter() - t0
+        if now < a:
+            time.sleep(a - now)
+        r = dns[k].solve(outages=[o], load_scale=[sc], options=opts[k], keep_state=True)
+        done = time.perf_counter() - t0
+        lat.append(1000 * (done - a))
+        conv += int(r.status[0] == 0)
+    wall = time.perf_counter() - t0
+    return lat, conv, wall
+
+
+def run_config5(args):
+    """Serving: an open-loop Poisson stream of power-flow and single-contingency requests over 8 resident networks
+    (bus counts log-uniform in [100, 25,000]), load multiplier U[0.8, 1.2], served by the continuous-batching loop
+    (gs_server_*) at offered rates from 100 to 5,000 requests/s; the same streams through a plain one-solve-per-launch
+    loop; the oracle on a sample of requests."""
+    from gridsim_amd import engine, gridgen
+    t0 = time.time()
+    ctx = engine.Context(0)
+    info = ctx.device_info()
+    seed = gridgen.BASE_SEED + 5
+    rng = np.random.default_rng(seed)
+    sizes = gridgen.config5_sizes(seed)
+    nets, warm, dns, opts, tuned, outs, seeds = [], [], [], [], [], [], []
+    for i, n_bus in enumerate(sizes):
+        # a grid whose base case does not converge from flat start is redrawn with the next seed (a resident
+        # network is served from its solved base case)
+        gseed = seed + 1 + i
+        while True:
+            net = gridgen.sized_grid(n_bus, gseed)
+            base = engine.DeviceNetwork(ctx, net, engine.NrOptions(flat_start=True, max_iterations=100,
+                                                                   lu_precision=engine.LU_FP64))
+            b = base.solve(count=1)
+            base.close()
+            if b.status[0] == engine.CONVERGED:
+                break
+            print(f"# {n_bus}-bus grid, seed {gseed}: base case did not converge, redrawn", file=sys.stderr,
+                  flush=True)
+            gseed += 1000
+        nets.append(net)
+        seeds.append(gseed)
+        w = net.with_start(b.vm[0], np.degrees(b.va[0]))
+        dn = engine.DeviceNetwork(ctx, w, engine.NrOptions(max_iterations=10, lu_precision=engine.LU_FP32,
+                                                           refine_steps=1))
+        o_all = gridgen.config2_outages(w)
+        so = np.where(rng.random(16) < 0.5, rng.choice(o_all, 16), -1)
+        o, ms = engine.autotune(dn, outages=so, load_scale=rng.uniform(0.8, 1.2, 16), options=dn.options)
+        tuned.append({"n_bus": w.n_bus, "lu": "fp32" if o.lu_precision == engine.LU_FP32 else "fp64",
+                      "refine_steps": o.refine_steps, "schedule": o.schedule})
+        print(f"# {w.n_bus} buses: {tuned[-1]}", file=sys.stderr, flush=True)
+        warm.append(w); dns.append(dn); opts.append(o); outs.append(o_all)
+    srv = engine.Server(ctx)
+    ids = [srv.add_network(dn, o) for dn, o in zip(dns, opts)]
+    print(f"# setup {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+
+    def draw(count):
+        reqs = []
+        for _ in range(count):
+            k = int(rng.integers(len(nets)))
+            o = int(rng.choice(outs[k])) if rng.random() < 0.5 else -1
+            reqs.append((k, o, float(rng.uniform(0.8, 1.2))))
+        return reqs
+
+    def submit(k, o, sc):
+        return srv.submit(ids[k], o, sc)
+
+    def wait(t, k):
+        return srv.wait(t, ids[k], keep_state=False)
+
+    # warm-up: every network once
+    _serve_stream(submit, wait, [(k, -1, 1.0) for k in range(len(nets))], np.zeros(len(nets)))
+    sweep = []
+    rates = [100, 250, 500, 1000, 2000, 5000]
+    for rate in rates:
+        n_req = int(min(max(200, rate * args.serve_seconds), 4000))
+        reqs = draw(n_req)
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
+        srv.reset_stats()
+        res, wall = _serve_stream(submit, wait, reqs, arrivals)
+        st = srv.stats()
+        conv = sum(int(r[0]["status"] == 0) for r in res)
+        pl_n = min(n_req, args.plain_requests)
+        lat, pconv, pwall = _plain_loop(dns, opts, reqs[:pl_n], arrivals[:pl_n])
+        row = {"offered_rps": rate, "requests": n_req, "solves_per_s": round(conv / wall, 1),
+               "p50_ms": round(st["latency_p50_ms"], 3), "p99_ms": round(st["latency_p99_ms"], 3),
+               "max_ms": round(st["latency_max_ms"], 3), "mean_batch": round(st["mean_batch"], 2),
+               "launches": st["launches"], "converged": conv,
+               "plain": {"requests": pl_n, "solves_per_s": round(pconv / pwall, 1),
+                         "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                         "p99_ms": round(float(np.percentile(lat, 99)), 3)}}
+        sweep.append(row)
+        print(f"# {rate} req/s: {json.dumps(row)}", file=sys.stderr, flush=True)
+
+    # the oracle on a sample: two requests per network
+    match, max_dv, cpu = None, None, None
+    if not args.no_cpu:
+        from concurrent.futures import ThreadPoolExecutor
+        from oracle import oracle as O
+        sample = [(k, o, sc) for k in range(len(nets)) for (o, sc) in ((-1, 0.9), (int(outs[k][len(outs[k]) // 2]), 1.1))]
+        tickets = [srv.submit(ids[k], o, sc) for k, o, sc in sample]
+        got = [srv.wait(t, ids[k]) for t, (k, _, _) in zip(tickets, sample)]
+        threads = min(os.cpu_count() or 1, 16)
+
+        def one(r):
+            k, o, sc = r
+            ts = time.perf_counter()
+            rec, vm, va = O.solve(warm[k].with_load_multiplier(sc), O.options(max_iterations=10), skip=o)
+            return rec, vm, va, time.perf_counter() - ts
+
+        tc = time.perf_counter()
+        with ThreadPoolExecutor(threads) as ex:
+            ref = list(ex.map(one, sample))
+        csec = time.perf_counter() - tc
+        match, max_dv = True, 0.0
+        for (rec, vm, va, _), g in zip(ref, got):
+            match &= int(rec["status"]) == int(g[0]["status"])
+            if rec["status"] == 0:
+                max_dv = max(max_dv, float(np.abs(vm - g[1]).max()), float(np.abs(va - g[2]).max()))
+        cpu = {"value": round(sum(int(r[0]["status"] == 0) for r in ref) / csec, 1), "unit": "converged solves/s",
+               "cores": threads, "kind": "port",
+               "sample": f"{len(sample)} requests (a power flow and a contingency per network), {threads} threads, "
+                         f"C++ restatement of the reference's CPU sparse NR; single-thread per-request p50 "
+                         f"{1000 * float(np.median([r[3] for r in ref])):.1f} ms"}
+    top = sweep[-1]
+    line = {
+        "metric": "serving: converged solves/s and p50/p99 request latency, Poisson stream 100-5,000 req/s, "
+                  "8 resident networks (100-25,000 buses), 1 MI355X",
+        "value": top["solves_per_s"], "unit": "converged solves/s", "n_gpus": 1, "steps": len(rates), "warmup": 1,
+        "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "autotuned per network (f32-lu+f64 or f64)", "data": "synthetic",
+        "config": {"workload": "open-loop Poisson stream, half power flows and half single contingencies, network "
+                               "uniform over 8 resident grids with bus counts log-uniform in [100, 25,000], load "
+                               "multiplier U[0.8, 1.2], warm start from each grid's resident base case, max 10 "
+                               "iterations, 1e-8 pu; continuous batching on 4 streams",
+                   "networks": [w.n_bus for w in warm], "seed": int(seed), "grid_seeds": seeds, "autotuned": tuned,
+                   "device": info["name"] or info["arch"]},
+        "latency_ms": {"p50": top["p50_ms"], "p99": top["p99_ms"],
+                       "note": f"at the highest offered rate ({top['offered_rps']} req/s); every rate in sweep"},
+        "sweep": sweep,
+        "verdicts": {"match_cpu_oracle_on_sample": match, "max_dv_vs_oracle_on_sample": max_dv},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+    srv.close()
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -169,8 +350,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--lu", default="auto", choices=("auto", "fp32", "fp64"),
                     help="LU arithmetic of the batched sweep: auto = the autotuner's choice on the sweep itself")
-    ap.add_argument("--config", type=int, default=2, choices=(2, 3),
-                    help="2: N-1 sweep of the 2,000-bus grid (the headline); 3: 512 uneven instances, hipGraph")
+    ap.add_argument("--config", type=int, default=2, choices=(2, 3, 5),
+                    help="2: N-1 sweep of the 2,000-bus grid (the headline); 3: 512 uneven instances, hipGraph; "
+                         "5: serving, Poisson request stream over 8 resident networks")
+    ap.add_argument("--serve-seconds", type=float, default=2.0,
+                    help="config 5: length of each offered-rate step (requests = rate x seconds, 200..4000)")
+    ap.add_argument("--plain-requests", type=int, default=200,
+                    help="config 5: requests of each step replayed through the plain one-solve-per-launch loop")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device (a kernel of known traffic, for scaling the "
                          "HBM counters of a profiler run)")
@@ -192,6 +378,8 @@ def main():
 
     if args.config == 3:
         return run_config3(args)
+    if args.config == 5:
+        return run_config5(args)
 
     t0 = time.time()
     net = gridgen.config2_grid()

@@ -39,7 +39,7 @@ def build_engine(verbose: bool = True) -> str:
     out_dir = tempfile.mkdtemp(prefix="gship_", dir=os.environ.get("TMPDIR", "/tmp"))
     objs = []
     jobs = []
-    for src, kind in (("model.cpp", "host"), ("engine.cpp", "hip"), ("nr_kernels.hip", "hip")):
+    for src, kind in (("model.cpp", "host"), ("engine.cpp", "hip"), ("server.cpp", "hip"), ("nr_kernels.hip", "hip")):
         obj = os.path.join(out_dir, src + ".o")
         path = os.path.join(CSRC, src)
         if kind == "host":

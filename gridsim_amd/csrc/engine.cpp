@@ -12,97 +12,16 @@
 #include <string>
 #include <vector>
 
+#include "engine_internal.h"
 #include "fdlibm.h"
-#include "gridsim_hip.h"
-#include "model.h"
-#include "nr_kernels.h"
-
-namespace gs {
-hipError_t launch_nr_batch(int lu_fp32, int x_in_lds, const InstDesc* descs, const DevPlan& plan, int count,
-                           const NrParams& prm, size_t lds_bytes, hipStream_t stream);
-hipError_t launch_seam(const SeamDesc* descs, const DevPlan& plan, int count, size_t lds_bytes, hipStream_t stream);
-hipError_t launch_sincos(const double* x, double* s, double* c, int64_t n, hipStream_t stream);
-hipError_t launch_fp64_probe(double* out, hipStream_t stream);
-}  // namespace gs
 
 using namespace gs;
 
 static_assert(DevicePlanHost::kRsValueAhead == kRsValueAhead, "plan and kernel disagree on the refactor's look-ahead");
 static_assert(DevicePlanHost::kTracks == kTracks && DevicePlanHost::kTrackLanes == kTrackLanes, "track shape");
 
-namespace {
-
-constexpr size_t kLdsLimit = 64 * 1024;  // dynamic LDS a single-wave work-group may take for the solve workspace
-
-struct HipError {
-  gs_status code;
-  std::string what;
-};
-
-[[noreturn]] void fail(gs_status code, const std::string& what) { throw HipError{code, what}; }
-
-void check(hipError_t e, const char* what) {
-  if (e == hipSuccess) return;
-  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
-    fail(GS_OUT_OF_MEMORY, std::string(what) + ": " + hipGetErrorString(e));
-  fail(GS_DEVICE_ERROR, std::string(what) + ": " + hipGetErrorString(e));
-}
-
-// Device allocation owned by a handle.
-struct DevMem {
-  void* p = nullptr;
-  size_t bytes = 0;
-  DevMem() = default;
-  DevMem(const DevMem&) = delete;
-  DevMem& operator=(const DevMem&) = delete;
-  ~DevMem() { release(); }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    bytes = 0;
-  }
-  void ensure(size_t n) {
-    if (n <= bytes && p) return;
-    release();
-    check(hipMalloc(&p, n ? n : 16), "hipMalloc");
-    bytes = n ? n : 16;
-  }
-  template <typename T> T* as() const { return reinterpret_cast<T*>(p); }
-};
-
-template <typename T>
-void upload(DevMem& dst, const std::vector<T>& v, hipStream_t s) {
-  dst.ensure(sizeof(T) * v.size());
-  if (!v.empty()) check(hipMemcpyAsync(dst.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, s), "upload");
-}
-
-}  // namespace
-
-// ================================================================================================ context
-struct SeamPlan;
-
-struct gs_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  std::mutex mu;
-  std::string err;
-  size_t budget = 0;
-  double us_per_tick = 0.01;  // device wall clock (wall_clock64) period in microseconds
-  std::map<std::pair<std::vector<int>, std::vector<int>>, std::vector<int>> seam_orders;  // pattern -> MD order
-  std::map<std::vector<int>, std::shared_ptr<SeamPlan>> seam_plans;                       // nonzero positions -> plan
-};
-
-namespace {
-
-// A plan and its network pattern uploaded to the device.
-struct DevicePlan {
-  DevMem y_rp, y_ci, y_row, y_diag, y_g, y_b, slots, aslots, ppos, qpos, j_rp, j_ci, vm0, va0;
-  DevMem pinv, q, prow, piv_pos;
-  DevMem rs_rec, ch_a_src;
-  DevMem fs_rec, bs_rec;
-  DevPlan h{};   // passed to the kernels by value
-  DevicePlanHost dph;
-};
+// ================================================================================================ plans
+namespace gs {
 
 void upload_plan(DevicePlan& dp, const YBus* y, const JacobianPattern& jp, const BusSpec* spec, const Plan& plan,
                  double base_mva, hipStream_t s) {
@@ -229,29 +148,6 @@ gs_nr_options normalise(const gs_nr_options* o) {
   return r;
 }
 
-}  // namespace
-
-struct gs_net {
-  gs_ctx* ctx = nullptr;
-  Network net;
-  YBus y;
-  BusSpec spec;
-  bool flat_start = false;
-  JacobianPattern jp;
-  Plan plan;
-  DevicePlan dev;
-  // per-call device arenas
-  DevMem inst_buf, desc_buf, state_buf, aux_buf;
-  double last_total_ms = 0, last_kernel_ms = 0;
-  int last_launches = 0;
-};
-
-namespace {
-
-struct Layout {
-  size_t vm, va, p, q, f, dx, r, jv, tpq, vals, fval, bval, dval, xg, spec, start, stride;
-};
-
 Layout instance_layout(const DevPlan& h, bool fp32, bool x_lds) {
   auto al = [](size_t v) { return (v + 255) & ~size_t(255); };
   size_t t = fp32 ? 4 : 8;
@@ -305,11 +201,6 @@ int outage_patches(const YBus& y, const Network& net, int branch, Patch* out) {
   return np;
 }
 
-struct BatchRun {
-  bool fp32 = true, x_lds = true;
-  size_t lds_bytes = 0;
-};
-
 BatchRun plan_run(const DevPlan& h, const gs_nr_options& o, bool force_fp64) {
   BatchRun r;
   r.fp32 = !force_fp64 && o.lu_precision == GS_LU_FP32;
@@ -340,47 +231,21 @@ void fill_result(const InstState& s, gs_nr_result& r, int refreshes, double us_p
   r.phase_us[3] = s.t_update * us_per_tick;
 }
 
-// Host copies that must outlive the asynchronous uploads of a prepared batch.
-struct Staging {
-  std::vector<InstDesc> descs;
-  std::vector<double> aux;                                  // the network's own specification
-  std::vector<std::pair<int, std::vector<double>>> stage;   // per instance: [pspec | qspec | vm0 | va0]
-  std::vector<InstState> states;                            // resumed states (uploaded when not starting afresh)
-};
-
-struct PreparedRun {
-  BatchRun run;
-  Layout L;
-  NrParams prm;
-  int count = 0;
-};
-
-}  // namespace
-
-namespace {
-
-// Lays `count` instances sharing one device plan out in HBM and uploads their descriptors and own data on stream s.
 // `resume` (or null: start afresh) are paused states to continue from.
-PreparedRun prepare_instances(gs_ctx* ctx, DevicePlan& dp, DevMem& inst_buf, DevMem& desc_buf, DevMem& state_buf,
-                              DevMem& aux_buf, Staging& stg, int count, const gs_instance* inst, const YBus* y,
-                              const Network* netw, const BusSpec& base_spec, const gs_nr_options& o, bool plain,
-                              const std::vector<InstState>* resume,
-                              const std::vector<std::vector<double>>* spec_override,
-                              const std::vector<std::vector<double>>* start_override, hipStream_t s) {
-  const DevPlan& h = dp.h;
+PreparedRun prepare_into(const DevPlan& h, const RunBuffers& rb, Staging& stg, int count, const gs_instance* inst,
+                         const YBus* y, const Network* netw, const gs_nr_options& o, bool plain,
+                         const std::vector<InstState>* resume,
+                         const std::vector<std::vector<double>>* spec_override,
+                         const std::vector<std::vector<double>>* start_override, hipStream_t s) {
   PreparedRun pr;
   pr.count = count;
+  pr.desc = rb.desc;
   pr.run = plan_run(h, o, plain);
   const BatchRun& run = pr.run;
   pr.L = instance_layout(h, run.fp32, run.x_lds);
   const Layout& L = pr.L;
-  size_t need = L.stride * (size_t)count;
-  if (ctx->budget && need > ctx->budget) fail(GS_OUT_OF_MEMORY, "batch exceeds the HBM budget");
-  inst_buf.ensure(need);
-  desc_buf.ensure(sizeof(InstDesc) * (size_t)count);
-  state_buf.ensure(sizeof(InstState) * (size_t)count);
-  char* base = inst_buf.as<char>();
-  InstState* dstate = state_buf.as<InstState>();
+  char* base = rb.inst;
+  InstState* dstate = rb.state;
   std::vector<InstDesc>& descs = stg.descs;
   descs.assign(count, InstDesc{});
   stg.stage.clear();
@@ -424,11 +289,8 @@ PreparedRun prepare_instances(gs_ctx* ctx, DevicePlan& dp, DevMem& inst_buf, Dev
     d.vm_start = st ? (const double*)(b + L.start) : h.vm0;
     d.va_start = st ? d.vm_start + h.n : h.va0;
   }
-  stg.aux = base_spec.pspec;
-  stg.aux.insert(stg.aux.end(), base_spec.qspec.begin(), base_spec.qspec.end());
-  upload(aux_buf, stg.aux, s);
   for (auto& d : descs)
-    if (!d.pspec) { d.pspec = aux_buf.as<double>(); d.qspec = aux_buf.as<double>() + h.n; }
+    if (!d.pspec) { d.pspec = rb.base_spec; d.qspec = rb.base_spec + h.n; }
   for (auto& [i, buf] : stg.stage) {
     char* b = base + L.stride * i;
     if (descs[i].pspec == (const double*)(b + L.spec))
@@ -437,7 +299,7 @@ PreparedRun prepare_instances(gs_ctx* ctx, DevicePlan& dp, DevMem& inst_buf, Dev
       check(hipMemcpyAsync(b + L.start, buf.data() + 2 * h.n, 16 * (size_t)h.n, hipMemcpyHostToDevice, s),
             "stage start");
   }
-  check(hipMemcpyAsync(desc_buf.p, descs.data(), sizeof(InstDesc) * count, hipMemcpyHostToDevice, s), "descs");
+  check(hipMemcpyAsync(rb.desc, descs.data(), sizeof(InstDesc) * count, hipMemcpyHostToDevice, s), "descs");
   // a fresh start needs no state upload: the kernel's init resets every counter it reads
   if (resume) {
     stg.states = *resume;
@@ -456,12 +318,37 @@ PreparedRun prepare_instances(gs_ctx* ctx, DevicePlan& dp, DevMem& inst_buf, Dev
   return pr;
 }
 
-// The batched launch of a prepared set: the whole Newton loop of every instance in one kernel, or (schedule > 0) in
-// launches of at most `schedule` instances one after another on the stream.
-void launch_prepared(const DevicePlan& dp, const DevMem& desc_buf, const PreparedRun& pr, int schedule, hipStream_t s) {
+// Lays `count` instances sharing one device plan out in HBM (the handle's own arenas, grown as needed) and uploads
+// their descriptors and own data on stream s.
+PreparedRun prepare_instances(gs_ctx* ctx, DevicePlan& dp, DevMem& inst_buf, DevMem& desc_buf, DevMem& state_buf,
+                              DevMem& aux_buf, Staging& stg, int count, const gs_instance* inst, const YBus* y,
+                              const Network* netw, const BusSpec& base_spec, const gs_nr_options& o, bool plain,
+                              const std::vector<InstState>* resume,
+                              const std::vector<std::vector<double>>* spec_override,
+                              const std::vector<std::vector<double>>* start_override, hipStream_t s) {
+  const DevPlan& h = dp.h;
+  const BatchRun run = plan_run(h, o, plain);
+  const Layout L = instance_layout(h, run.fp32, run.x_lds);
+  size_t need = L.stride * (size_t)count;
+  if (ctx->budget && need > ctx->budget) fail(GS_OUT_OF_MEMORY, "batch exceeds the HBM budget");
+  inst_buf.ensure(need);
+  desc_buf.ensure(sizeof(InstDesc) * (size_t)count);
+  state_buf.ensure(sizeof(InstState) * (size_t)count);
+  stg.aux = base_spec.pspec;
+  stg.aux.insert(stg.aux.end(), base_spec.qspec.begin(), base_spec.qspec.end());
+  upload(aux_buf, stg.aux, s);
+  RunBuffers rb;
+  rb.inst = inst_buf.as<char>();
+  rb.desc = desc_buf.as<InstDesc>();
+  rb.state = state_buf.as<InstState>();
+  rb.base_spec = aux_buf.as<double>();
+  return prepare_into(h, rb, stg, count, inst, y, netw, o, plain, resume, spec_override, start_override, s);
+}
+
+void launch_prepared(const DevPlan& h, const PreparedRun& pr, int schedule, hipStream_t s) {
   const int slice = schedule > 0 ? schedule : pr.count;
   for (int first = 0; first < pr.count; first += slice)
-    check(launch_nr_batch(pr.run.fp32, pr.run.x_lds, desc_buf.as<InstDesc>() + first, dp.h,
+    check(launch_nr_batch(pr.run.fp32, pr.run.x_lds, pr.desc + first, h,
                           std::min(slice, pr.count - first), pr.prm, pr.run.lds_bytes, s),
           "nr kernel");
 }
@@ -487,7 +374,7 @@ double launch_instances(gs_ctx* ctx, DevicePlan& dp, DevMem& inst_buf, DevMem& d
   check(hipEventRecord(e0, s), "event");
   int nl = 0;
   if (!plain) {
-    launch_prepared(dp, desc_buf, pr, o.schedule, s);
+    launch_prepared(dp.h, pr, o.schedule, s);
     nl = o.schedule > 0 ? (count + o.schedule - 1) / o.schedule : 1;
   } else {
     // one instance per launch, one Newton step per launch, the host reading the verdict after each
@@ -621,13 +508,13 @@ void run_batch(gs_net* net, int count, const gs_instance* inst, const gs_nr_opti
   net->last_total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
-}  // namespace
-
 // ================================================================================================ seam solves
 struct SeamPlan {
   DevicePlan dev;
   std::vector<int> nonzero_pos;
 };
+
+}  // namespace gs
 
 namespace {
 
@@ -761,7 +648,7 @@ void enqueue_batch(gs_batch* b) {
   check(hipEventRecord(b->fork, s0), "fork");
   for (auto& g : b->groups) {
     check(hipStreamWaitEvent(g->stream, b->fork, 0), "fork wait");
-    launch_prepared(g->net->dev, g->desc_buf, g->pr, g->opts.schedule, g->stream);
+    launch_prepared(g->net->dev.h, g->pr, g->opts.schedule, g->stream);
     check(hipEventRecord(g->joined, g->stream), "join");
     check(hipStreamWaitEvent(s0, g->joined, 0), "join wait");
   }
@@ -810,26 +697,6 @@ constexpr double kAutotuneDv = 1e-9;
 }  // namespace
 
 // ================================================================================================ C ABI
-namespace {
-template <typename F>
-gs_status guarded(gs_ctx* ctx, F&& f) {
-  if (!ctx) return GS_INVALID_ARGUMENT;
-  std::lock_guard<std::mutex> lock(ctx->mu);
-  try {
-    return f();
-  } catch (const HipError& e) {
-    ctx->err = e.what;
-    return e.code;
-  } catch (const std::bad_alloc&) {
-    ctx->err = "host allocation failed";
-    return GS_OUT_OF_MEMORY;
-  } catch (const std::exception& e) {
-    ctx->err = e.what();
-    return GS_DEVICE_ERROR;
-  }
-}
-}  // namespace
-
 extern "C" {
 
 GS_API const char* gs_version(void) { return "gridsim-hip 0.1 (gfx950)"; }

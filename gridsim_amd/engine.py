@@ -27,7 +27,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libgridsim_hip.so")
 
 GS_OK, GS_SINGULAR, GS_NONFINITE_INPUT, GS_OUT_OF_MEMORY, GS_INVALID_ARGUMENT, GS_DEVICE_ERROR, GS_NO_DEVICE, \
-    GS_BUSY = range(8)
+    GS_BUSY, GS_PENDING = range(9)
 CONVERGED, MAX_ITERATIONS, DIVERGED, NONFINITE = 0, 1, 2, 3
 LU_FP64, LU_FP32 = 0, 1
 FLAG_X_GLOBAL = 1
@@ -79,6 +79,18 @@ class GsPlanStats(C.Structure):
                 ("stream_bytes", C.c_int64)]
 
 
+class GsServerConfig(C.Structure):
+    _fields_ = [("max_batch", C.c_int32), ("streams", C.c_int32), ("queue_capacity", C.c_int32),
+                ("reserved", C.c_int32), ("state_bytes", C.c_size_t), ("pinned_bytes", C.c_size_t)]
+
+
+class GsServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("rejected", C.c_int64),
+                ("launches", C.c_int64), ("queued", C.c_int64), ("in_flight", C.c_int64),
+                ("mean_batch", C.c_double), ("latency_p50_ms", C.c_double), ("latency_p99_ms", C.c_double),
+                ("latency_max_ms", C.c_double), ("busy_ms", C.c_double), ("state_bytes_peak", C.c_int64)]
+
+
 class GsDeviceInfo(C.Structure):
     _fields_ = [("name", C.c_char * 64), ("arch", C.c_char * 32), ("compute_units", C.c_int32),
                 ("fp64_ok", C.c_int32), ("hbm_bytes", C.c_size_t), ("fdlibm_ok", C.c_int32),
@@ -101,9 +113,15 @@ def lib():
                      "gs_network_upload", "gs_nr_solve_batch", "gs_nr_solve_plain", "gs_network_plan_sizes",
                      "gs_network_plan_get", "gs_last_timing", "gs_device_sincos", "gs_plan_build",
                      "gs_plan_sizes", "gs_plan_get", "gs_batch_create", "gs_batch_run", "gs_batch_results",
-                     "gs_nr_autotune", "gs_network_plan_stats", "gs_plan_shape"):
+                     "gs_nr_autotune", "gs_network_plan_stats", "gs_plan_shape", "gs_server_create",
+                     "gs_server_add_network", "gs_server_submit", "gs_server_wait", "gs_server_stats_get",
+                     "gs_server_stats_reset"):
             getattr(L, name).restype = C.c_int32
         L.gs_host_sincos.restype = None
+        L.gs_server_free.restype = None
+        L.gs_server_wait.argtypes = [C.c_void_p, C.c_uint64, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                     C.c_void_p]
+        L.gs_server_submit.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
 
@@ -515,3 +533,73 @@ def autotune(dn: "DeviceNetwork", outages=None, load_scale=None, options: NrOpti
     chosen = NrOptions(o.tolerance, o.max_iterations, bool(o.flat_start), o.diverge_limit, o.lu_precision,
                        o.refine_steps, o.growth_tol, o.schedule, o.flags)
     return chosen, ms.value
+
+
+class Server:
+    """Continuous-batching request loop over resident networks (gs_server_*): submit() queues a power-flow or
+    single-contingency request and returns a ticket at once; a dispatcher thread in the library batches waiting
+    requests of one network into launches on up to four streams as slots and memory free up; wait() returns the
+    result of one ticket."""
+
+    def __init__(self, ctx: Context, max_batch: int = 0, streams: int = 0, queue_capacity: int = 0,
+                 state_bytes: int = 0, pinned_bytes: int = 0):
+        self.ctx = ctx
+        cfg = GsServerConfig(max_batch, streams, queue_capacity, 0, state_bytes, pinned_bytes)
+        h = C.c_void_p()
+        ctx.check(lib().gs_server_create(ctx.h, C.byref(cfg), C.byref(h)), "gs_server_create")
+        self.h = h
+        self.nets: list[DeviceNetwork] = []
+
+    def add_network(self, dn: "DeviceNetwork", options: NrOptions | None = None) -> int:
+        nid = C.c_int32()
+        opts = (options or dn.options).c()
+        self.ctx.check(lib().gs_server_add_network(self.h, dn.h, C.byref(opts), C.byref(nid)), "gs_server_add_network")
+        self.nets.append(dn)
+        return nid.value
+
+    def submit(self, network_id: int, outage: int = -1, load_scale: float = 1.0, start=None) -> int:
+        """Queues one request; raises GsError(GS_BUSY) when the queue is full."""
+        vm0 = va0 = None
+        if start is not None:
+            vm0 = np.ascontiguousarray(start[0], np.float64)
+            va0 = np.ascontiguousarray(start[1], np.float64)
+        inst = GsInstance(int(outage), 0, float(load_scale), _p(vm0), _p(va0))
+        t = C.c_uint64()
+        st = lib().gs_server_submit(self.h, C.c_int32(network_id), C.byref(inst), C.byref(t))
+        if st != GS_OK:
+            raise GsError(st, "gs_server_submit")
+        return t.value
+
+    def wait(self, ticket: int, network_id: int, timeout_ms: int = -1, keep_state: bool = True):
+        """(result record, vm, va (radians), latency ms) of a finished request; None when timeout_ms ran out."""
+        n = self.nets[network_id].net.n_bus
+        res = np.zeros(1, RESULT_DTYPE)
+        vm = np.empty(n) if keep_state else None
+        va = np.empty(n) if keep_state else None
+        lat = C.c_double()
+        st = lib().gs_server_wait(self.h, C.c_uint64(ticket), C.c_int32(timeout_ms), _p(res), _p(vm), _p(va),
+                                  C.byref(lat))
+        if st == GS_PENDING:
+            return None
+        if st != GS_OK:
+            raise GsError(st, f"gs_server_wait (ticket {ticket})")
+        return res[0], vm, va, lat.value
+
+    def stats(self) -> dict:
+        s = GsServerStats()
+        self.ctx.check(lib().gs_server_stats_get(self.h, C.byref(s)), "gs_server_stats_get")
+        return {k: getattr(s, k) for k, _ in GsServerStats._fields_}
+
+    def reset_stats(self):
+        self.ctx.check(lib().gs_server_stats_reset(self.h), "gs_server_stats_reset")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().gs_server_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -239,6 +239,18 @@ def sized_grid(n_bus: int, seed: int, pv_frac: float = 0.15, branch_ratio: float
                           name=f"synthetic-{n_bus}")
 
 
+def config5_sizes(seed: int = BASE_SEED + 5, count: int = 8) -> list[int]:
+    """Bus counts of the serving workload's resident networks: log-uniform in [100, 25,000], sorted."""
+    rng = np.random.default_rng(seed)
+    return [int(n) for n in np.sort(np.round(np.exp(rng.uniform(np.log(100), np.log(25000), count))).astype(int))]
+
+
+def config5_grids(seed: int = BASE_SEED + 5, count: int = 8) -> list[Network]:
+    """The serving workload's resident networks: one sized grid per config5_sizes entry, grid i drawn with seed
+    seed + 1 + i."""
+    return [sized_grid(n, seed + 1 + i) for i, n in enumerate(config5_sizes(seed, count))]
+
+
 def config4_grid(seed: int = BASE_SEED + 4) -> Network:
     """70,000 buses (8,000 PV), 88,000 branches, 12 % transformers, lognormal loads."""
     return synthetic_grid(70000, 88000, 8000, seed, xfmr_frac=0.12, load_model="lognormal",

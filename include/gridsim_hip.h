@@ -37,7 +37,8 @@ enum {
   GS_INVALID_ARGUMENT = 4,  /* malformed call (bad sizes, pattern, handle) */
   GS_DEVICE_ERROR = 5,      /* any other HIP failure */
   GS_NO_DEVICE = 6,         /* no usable GPU */
-  GS_BUSY = 7               /* the server's admission queue is full */
+  GS_BUSY = 7,              /* the server's admission queue is full */
+  GS_PENDING = 8            /* gs_server_wait: the request has not finished within the timeout */
 };
 
 /* Bus types in the reference enum order (src/GridSim.Core/Model/Model.cs:4-12). */
@@ -219,6 +220,49 @@ GS_API void gs_batch_free(gs_batch* batch);
  * *best_ms: that candidate's kernel time on the sample. */
 GS_API gs_status gs_nr_autotune(gs_net* net, int32_t count, const gs_instance* sample, gs_nr_options* opts,
                                 double* best_ms);
+
+/* ---- serving: continuous batching in front of the rpc "solve" contract ----------------------------------------
+ * The reference serves gridsim-rpc/1 "solve" requests one at a time (src/GridSim.Cli/RpcCommand.cs:460-497,668-700:
+ * each request runs SolverBackend.Solver(...).Solve(model, options) to completion before the next line is read). A
+ * gs_server keeps resident networks (Y-bus, symbolic plan, base voltages) in HBM and runs a dispatcher thread that,
+ * whenever a launch slot is free, takes the oldest waiting request's network, gathers that network's waiting requests
+ * in arrival order (up to max_batch, and as many as the HBM and pinned-memory budgets admit) and launches them on the
+ * slot's own stream. Requests that arrive while launches run wait only for the next free slot: up to `streams`
+ * launches overlap on the device. Every instance is solved by its own wavefront, so a result does not depend on what
+ * else shares its launch. Networks added to a server must outlive it. */
+typedef struct gs_server gs_server;
+
+typedef struct gs_server_config {
+  int32_t max_batch;        /* instances per launch (0: 256) */
+  int32_t streams;          /* launches in flight at once, 1..4 (0: 4) */
+  int32_t queue_capacity;   /* waiting requests before gs_server_submit returns GS_BUSY (0: 65536) */
+  int32_t reserved;
+  size_t state_bytes;       /* HBM for the state of in-flight instances (0: 4 GiB) */
+  size_t pinned_bytes;      /* pinned host memory for results in flight (0: 256 MiB) */
+} gs_server_config;
+
+typedef struct gs_server_stats {
+  int64_t submitted, completed, rejected, launches, queued, in_flight;
+  double mean_batch;                                     /* instances per launch */
+  double latency_p50_ms, latency_p99_ms, latency_max_ms; /* submit -> result ready, since the last reset */
+  double busy_ms;                                        /* wall time with at least one launch in flight */
+  int64_t state_bytes_peak;
+} gs_server_stats;
+
+GS_API gs_status gs_server_create(gs_ctx* ctx, const gs_server_config* cfg, gs_server** out);
+/* Registers an uploaded network with the options its requests run under (null: the engine defaults). */
+GS_API gs_status gs_server_add_network(gs_server* srv, gs_net* net, const gs_nr_options* opts, int32_t* network_id);
+/* Queues one request (its vm0 / va0, if any, are copied). GS_BUSY when queue_capacity requests are waiting. */
+GS_API gs_status gs_server_submit(gs_server* srv, int32_t network_id, const gs_instance* inst, uint64_t* ticket);
+/* Waits up to timeout_ms (< 0: no limit) for a request's result: GS_OK and the result (vm / va, radians, n_bus each,
+ * may be null; latency_ms may be null), after which the ticket is forgotten; GS_PENDING when the time ran out; the
+ * request's own error status when its launch failed; GS_INVALID_ARGUMENT for an unknown ticket. */
+GS_API gs_status gs_server_wait(gs_server* srv, uint64_t ticket, int32_t timeout_ms, gs_nr_result* result,
+                                double* vm, double* va, double* latency_ms);
+GS_API gs_status gs_server_stats_get(gs_server* srv, gs_server_stats* out);
+GS_API gs_status gs_server_stats_reset(gs_server* srv);
+/* Finishes every queued and in-flight request, then stops the dispatcher and frees the server. */
+GS_API void gs_server_free(gs_server* srv);
 
 /* Kernel timing of the last gs_nr_solve_batch on this network (milliseconds, HIP events on its stream). */
 GS_API gs_status gs_last_timing(const gs_net* net, double* total_ms, double* kernel_ms, int32_t* launches);

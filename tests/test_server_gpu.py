@@ -1,0 +1,137 @@
+"""The serving loop (gs_server_*) on an MI355X, through the C ABI.
+
+Continuous batching must not change answers: each request's result is the same whether it runs alone (submitted and
+waited for before the next) or in a mix of requests of several networks that the dispatcher batches and overlaps on
+its streams, and the same as the batch API's for that instance. Bars: identical verdicts and iteration counts, |dV|
+<= 1e-12 between alone and mix (the arithmetic of one instance does not depend on its launch), and the oracle's
+verdicts with |dV| <= 1e-9 pu / rad for the FP64 network.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL_SAME = 1e-12
+TOL_DV = 1e-9
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from gridsim_amd import engine
+    c = engine.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def warm_nets(oracle):
+    """Three grids (118, 300 and 1,200 buses), each warm-started from its solved base case."""
+    from gridsim_amd import gridgen
+    out = []
+    for net in (gridgen.config1_grid(), gridgen.sized_grid(300, gridgen.BASE_SEED + 11),
+                gridgen.sized_grid(1200, gridgen.BASE_SEED + 12)):
+        rec, vm, va = oracle.solve(net, oracle.options(flat_start=True, max_iterations=60))
+        assert rec["status"] == 0
+        out.append(net.with_start(vm, np.degrees(va)))
+    return out
+
+
+def _requests(nets, count, seed):
+    """(network id, outage or -1, load multiplier) drawn like the serving workload: power flows and single
+    contingencies, load multiplier U[0.8, 1.2]."""
+    from gridsim_amd import gridgen
+    rng = np.random.default_rng(seed)
+    outs = [gridgen.config2_outages(n) for n in nets]
+    reqs = []
+    for _ in range(count):
+        k = int(rng.integers(len(nets)))
+        o = int(rng.choice(outs[k])) if rng.random() < 0.6 else -1
+        reqs.append((k, o, float(rng.uniform(0.8, 1.2))))
+    return reqs
+
+
+def test_server_result_is_the_same_alone_and_in_the_mix(ctx, warm_nets, oracle):
+    from gridsim_amd import engine
+    opts = [engine.NrOptions(max_iterations=10, lu_precision=engine.LU_FP64, refine_steps=0),
+            engine.NrOptions(max_iterations=10, lu_precision=engine.LU_FP32, refine_steps=1),
+            engine.NrOptions(max_iterations=10, lu_precision=engine.LU_FP32, refine_steps=2)]
+    dns = [engine.DeviceNetwork(ctx, n, o) for n, o in zip(warm_nets, opts)]
+    srv = engine.Server(ctx, max_batch=16)
+    ids = [srv.add_network(dn, o) for dn, o in zip(dns, opts)]
+    reqs = _requests(warm_nets, 60, seed=5)
+
+    print("alone: one request at a time", flush=True)
+    alone = []
+    for k, o, s in reqs:
+        alone.append(srv.wait(srv.submit(ids[k], o, s), ids[k]))
+    st = srv.stats()
+    assert st["completed"] == len(reqs) and st["mean_batch"] == 1.0
+
+    print("mix: every request submitted at once", flush=True)
+    srv.reset_stats()
+    tickets = [srv.submit(ids[k], o, s) for k, o, s in reqs]
+    mix = [srv.wait(t, ids[k]) for t, (k, _, _) in zip(tickets, reqs)]
+    st = srv.stats()
+    print("mix stats", st, flush=True)
+    assert st["completed"] == len(reqs) and st["launches"] < len(reqs), "the mix was not batched"
+    assert st["latency_p99_ms"] >= st["latency_p50_ms"] > 0
+
+    for (k, o, s), a, m in zip(reqs, alone, mix):
+        assert a[0]["status"] == m[0]["status"] and a[0]["iterations"] == m[0]["iterations"], (k, o, s)
+        assert np.abs(a[1] - m[1]).max() <= TOL_SAME and np.abs(a[2] - m[2]).max() <= TOL_SAME, (k, o, s)
+
+    # the same instances through the batch API, network by network
+    for k, dn in enumerate(dns):
+        sel = [i for i, r in enumerate(reqs) if r[0] == k]
+        if not sel:
+            continue
+        r = dn.solve(outages=[reqs[i][1] for i in sel], load_scale=[reqs[i][2] for i in sel])
+        for j, i in enumerate(sel):
+            assert r.status[j] == mix[i][0]["status"]
+            assert np.abs(r.vm[j] - mix[i][1]).max() <= TOL_SAME and np.abs(r.va[j] - mix[i][2]).max() <= TOL_SAME
+
+    # the oracle's verdicts on every request; its voltages to 1e-9 on the FP64 network
+    for (k, o, s), m in zip(reqs, mix):
+        net = warm_nets[k].with_load_multiplier(s)
+        rec, vm, va = oracle.solve(net, oracle.options(max_iterations=10), skip=o)
+        assert rec["status"] == m[0]["status"], (k, o, s)
+        if rec["status"] == 0:
+            assert m[0]["max_mismatch"] < 1e-8
+            if k == 0:
+                assert np.abs(vm - m[1]).max() <= TOL_DV and np.abs(va - m[2]).max() <= TOL_DV, (k, o, s)
+    srv.close()
+    for dn in dns:
+        dn.close()
+
+
+def test_server_queue_limit_and_argument_checks(ctx, warm_nets):
+    from gridsim_amd import engine
+    dn = engine.DeviceNetwork(ctx, warm_nets[2], engine.NrOptions(max_iterations=10))
+    srv = engine.Server(ctx, max_batch=1, streams=1, queue_capacity=4)
+    nid = srv.add_network(dn)
+    with pytest.raises(engine.GsError) as e:
+        srv.submit(nid + 1)
+    assert e.value.status == engine.GS_INVALID_ARGUMENT
+    with pytest.raises(engine.GsError) as e:
+        srv.submit(nid, outage=warm_nets[2].n_branch)
+    assert e.value.status == engine.GS_INVALID_ARGUMENT
+    with pytest.raises(engine.GsError) as e:
+        srv.wait(12345, nid)
+    assert e.value.status == engine.GS_INVALID_ARGUMENT
+    # one launch slot of one instance: a burst of submissions fills the queue of four
+    tickets, busy = [], 0
+    for i in range(40):
+        try:
+            tickets.append(srv.submit(nid, -1, 0.8 + 0.01 * i))
+        except engine.GsError as err:
+            assert err.status == engine.GS_BUSY
+            busy += 1
+    print(f"accepted {len(tickets)}, busy {busy}", flush=True)
+    assert busy > 0
+    for t in tickets:
+        r = srv.wait(t, nid)
+        assert r[0]["status"] == engine.CONVERGED
+    st = srv.stats()
+    assert st["rejected"] == busy and st["completed"] == len(tickets)
+    srv.close()
+    dn.close()
