@@ -159,6 +159,83 @@ def run_config3(args):
     ctx.close()
 
 
+def run_config4(args):
+    """One large instance: the 70,000-bus grid from flat start (max 100 iterations, 1e-8 pu), FP64 LU against FP32
+    LU with one and two FP64 refinement steps: iterations to 1e-8, device time and wall time of each, and |dV|
+    against the oracle's converged state."""
+    from gridsim_amd import engine, gridgen
+    t0 = time.time()
+    ctx = engine.Context(0)
+    info = ctx.device_info()
+    net = gridgen.config4_grid()
+    base = engine.NrOptions(flat_start=True, max_iterations=100, tolerance=1e-8)
+    dn = engine.DeviceNetwork(ctx, net, base)
+    st = dn.plan_stats()
+    print(f"# setup {time.time() - t0:.1f}s: {net.n_bus} buses, {net.n_branch} branches, m={st['m']}, "
+          f"nnz(J)={st['nnz_j']}, nnz(L+U)={st['nnz_l'] + st['nnz_u']}, levels {st['levels']}",
+          file=sys.stderr, flush=True)
+    ref = None
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        tc = time.perf_counter()
+        rec, rvm, rva = O.solve(net, O.options(flat_start=True, max_iterations=100))
+        csec = time.perf_counter() - tc
+        ref = (rec, rvm, rva)
+        cpu = {"value": round(1000 * csec, 1), "unit": "ms per solve", "cores": 1, "kind": "port",
+               "sample": f"the same solve, single thread (C++ restatement of the reference's CPU sparse NR): "
+                         f"{int(rec['iterations'])} iterations, status {int(rec['status'])}"}
+        print(f"# oracle: {csec:.2f} s, {int(rec['iterations'])} iterations", file=sys.stderr, flush=True)
+    variants = [("fp64", engine.LU_FP64, 0, 0), ("fp64", engine.LU_FP64, 0, engine.SCHEDULE_LEVELS),
+                ("fp32+1", engine.LU_FP32, 1, engine.SCHEDULE_LEVELS),
+                ("fp32+2", engine.LU_FP32, 2, engine.SCHEDULE_LEVELS)]
+    if args.c4_variants:
+        keep = set(args.c4_variants.split(","))
+        variants = [v for v in variants if f"{v[0]}-{'levels' if v[3] else 'wave'}" in keep]
+    rows = []
+    for name, lu, refine, sched in variants:
+        o = engine.NrOptions(flat_start=True, max_iterations=100, tolerance=1e-8, lu_precision=lu,
+                             refine_steps=refine, schedule=sched)
+        for _ in range(args.warmup):
+            dn.solve(count=1, options=o, keep_state=False)
+        walls, kms = [], []
+        for _ in range(args.steps):
+            ts = time.perf_counter()
+            r = dn.solve(count=1, options=o, keep_state=True)
+            walls.append(1000 * (time.perf_counter() - ts))
+            kms.append(r.kernel_ms)
+        row = {"lu": name, "schedule": "level-parallel" if sched == engine.SCHEDULE_LEVELS else "one wavefront",
+               "status": engine.STATUS_NAMES[int(r.status[0])], "graph_replays": int(r.launches),
+               "iterations": int(r.results["iterations"][0]), "max_mismatch": float(r.results["max_mismatch"][0]),
+               "wall_ms": round(float(np.median(walls)), 2), "kernel_ms": round(float(np.median(kms)), 2),
+               "phase_us": {n: round(float(v), 1) for n, v in zip(engine.PHASES, r.results["phase_us"][0])}}
+        if ref is not None and ref[0]["status"] == 0 and r.status[0] == 0:
+            row["max_dv_vs_oracle"] = float(max(np.abs(r.vm[0] - ref[1]).max(), np.abs(r.va[0] - ref[2]).max()))
+            row["iterations_oracle"] = int(ref[0]["iterations"])
+        rows.append(row)
+        print(f"# {json.dumps(row)}", file=sys.stderr, flush=True)
+    conv = [r for r in rows if r["status"] == "Converged"]
+    best = min(conv or rows, key=lambda r: r["wall_ms"])
+    line = {
+        "metric": "large single instance: wall time of one 70,000-bus NR solve from flat start to 1e-8 pu, FP64 LU vs "
+                  "FP32 LU + FP64 refinement, 1 MI355X",
+        "value": best["wall_ms"], "unit": "ms per solve", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": best["wall_ms"], "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
+        "dtype": "f64" if best["lu"] == "fp64" else "f32-lu+f64", "data": "synthetic",
+        "config": {"workload": "70,000 buses (8,000 PV, 35 regions of 2,000 buses with their own slack), 88,000 "
+                               "branches, 12 % transformers, lognormal loads, flat start, max 100 iterations, 1e-8 pu",
+                   "n_bus": net.n_bus, "n_branch": net.n_branch, "seed": int(net.seed), "m": st["m"],
+                   "nnz_lu": st["nnz_l"] + st["nnz_u"], "levels": st["levels"],
+                   "best": f"{best['lu']}, {best['schedule']}",
+                   "device": info["name"] or info["arch"]},
+        "variants": rows,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+    dn.close()
+    ctx.close()
+
+
 def _serve_stream(submit, wait, reqs, arrivals):
     """Open loop: submit request i at arrivals[i] (seconds from the start) from this thread while a collector thread
     waits for the tickets in submission order. Returns (results, wall seconds from the first arrival to the last
@@ -350,9 +427,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--lu", default="auto", choices=("auto", "fp32", "fp64"),
                     help="LU arithmetic of the batched sweep: auto = the autotuner's choice on the sweep itself")
-    ap.add_argument("--config", type=int, default=2, choices=(2, 3, 5),
+    ap.add_argument("--config", type=int, default=2, choices=(2, 3, 4, 5),
                     help="2: N-1 sweep of the 2,000-bus grid (the headline); 3: 512 uneven instances, hipGraph; "
-                         "5: serving, Poisson request stream over 8 resident networks")
+                         "4: one 70,000-bus solve, FP64 vs FP32 + refinement; 5: serving, Poisson request stream "
+                         "over 8 resident networks")
+    ap.add_argument("--c4-variants", default="",
+                    help="config 4: comma-separated subset of fp64-wave, fp64-levels, fp32+1-levels, fp32+2-levels")
     ap.add_argument("--serve-seconds", type=float, default=2.0,
                     help="config 5: length of each offered-rate step (requests = rate x seconds, 200..4000)")
     ap.add_argument("--plain-requests", type=int, default=200,
@@ -378,6 +458,8 @@ def main():
 
     if args.config == 3:
         return run_config3(args)
+    if args.config == 4:
+        return run_config4(args)
     if args.config == 5:
         return run_config5(args)
 

@@ -68,8 +68,10 @@ void upload_plan(DevicePlan& dp, const YBus* y, const JacobianPattern& jp, const
   h.wstride = d.wstride;
   h.n_a = d.n_a;
   h.n_rs = ring_up(d.n_rs);
+  // (two rings of padding: a level-parallel run of one chunk rounds its row range up to whole rings and loads one
+  // ring ahead of that)
   std::vector<int> rs_rec(d.rs_rec);
-  for (int r = d.n_rs; r < h.n_rs + kStreamPad; r++)
+  for (int r = d.n_rs; r < h.n_rs + 2 * kStreamPad; r++)
     for (int l = 0; l < 64; l++) { rs_rec.push_back(0xFFFF); rs_rec.push_back(0); rs_rec.push_back(0); rs_rec.push_back(0); }
   upload(dp.rs_rec, rs_rec, s); upload(dp.ch_a_src, d.ch_a_src, s);
   h.rs_rec = dp.rs_rec.as<int4>(); h.ch_a_src = dp.ch_a_src.as<int>();
@@ -80,13 +82,20 @@ void upload_plan(DevicePlan& dp, const YBus* y, const JacobianPattern& jp, const
   h.n_bval = d.n_bval;
   auto pad_steps = [](const std::vector<int>& rec, int steps) {
     std::vector<int> prec(rec);
-    for (int t = (int)(rec.size() / 256); t < steps + kStreamPad; t++)
+    for (int t = (int)(rec.size() / 256); t < steps + 2 * kStreamPad; t++)
       for (int l = 0; l < 64; l++) { prec.push_back(-1); prec.push_back(0); prec.push_back(-1); prec.push_back(0); }
     return prec;
   };
   std::vector<int> fs_rec = pad_steps(d.fs_rec, h.fs_steps), bs_rec = pad_steps(d.bs_rec, h.bs_steps);
   upload(dp.fs_rec, fs_rec, s); upload(dp.bs_rec, bs_rec, s);
   h.fs_rec = dp.fs_rec.as<int4>(); h.bs_rec = dp.bs_rec.as<int4>();
+  // chunk and pass boundaries for level-parallel schedules
+  upload(dp.rs_chunk_row, d.rs_chunk_row, s);
+  upload(dp.fs_pass_step, d.fs_pass_step, s);
+  upload(dp.bs_pass_step, d.bs_pass_step, s);
+  h.rs_chunk_row = dp.rs_chunk_row.as<int>();
+  h.fs_pass_step = dp.fs_pass_step.as<int>();
+  h.bs_pass_step = dp.bs_pass_step.as<int>();
   // the staging vectors above are locals: finish the copies before they go
   check(hipStreamSynchronize(s), "upload plan");
 }
@@ -107,8 +116,9 @@ gs_plan_stats plan_stats(const DevicePlanHost& d, const YBus& y, const JacobianP
   const int64_t n = y.n, nnz_y = y.rp.empty() ? 0 : y.rp[y.n], m = jp.m, rec_row = 64 * 16;
   int64_t b = 4 * (n + 1) + 4 * nnz_y * 2 + 4 * n + 16 * nnz_y + 2 * 16 * nnz_y + 2 * 4 * n;  // Y-bus, slots, pos
   b += 4 * (m + 1) + 4 * (int64_t)st.nnz_j + 2 * 8 * n + 4 * 4 * m;                            // J pattern, start, pivots
-  b += rec_row * (int64_t)(st.refactor_rows + kStreamPad) + 4 * (int64_t)d.ch_a_src.size();
-  b += rec_row * (int64_t)(st.forward_steps + kStreamPad + st.back_steps + kStreamPad);
+  b += rec_row * (int64_t)(st.refactor_rows + 2 * kStreamPad) + 4 * (int64_t)d.ch_a_src.size();
+  b += rec_row * (int64_t)(st.forward_steps + 2 * kStreamPad + st.back_steps + 2 * kStreamPad);
+  b += 4 * (int64_t)(d.rs_chunk_row.size() + d.fs_pass_step.size() + d.bs_pass_step.size());
   st.stream_bytes = b;
   return st;
 }
@@ -140,7 +150,7 @@ gs_nr_options normalise(const gs_nr_options* o) {
   r.flags = 0;
   if (o) r = *o;
   if (r.flags & ~GS_FLAG_X_GLOBAL) fail(GS_INVALID_ARGUMENT, "unknown flags");
-  if (r.schedule < 0) fail(GS_INVALID_ARGUMENT, "schedule must be >= 0");
+  if (r.schedule < GS_SCHEDULE_LEVELS) fail(GS_INVALID_ARGUMENT, "schedule must be >= 0 or GS_SCHEDULE_LEVELS");
   if (!(r.tolerance > 0)) fail(GS_INVALID_ARGUMENT, "tolerance must be positive");
   if (r.max_iterations < 0) fail(GS_INVALID_ARGUMENT, "max_iterations must be >= 0");
   if (r.refine_steps < 0 || r.refine_steps > 4) fail(GS_INVALID_ARGUMENT, "refine_steps must lie in 0..4");
@@ -207,7 +217,9 @@ BatchRun plan_run(const DevPlan& h, const gs_nr_options& o, bool force_fp64) {
   size_t t = r.fp32 ? 4 : 8;
   size_t ws = t * (size_t)kTracks * (size_t)h.wstride;
   size_t x = t * (size_t)h.m;
-  r.x_lds = !(o.flags & GS_FLAG_X_GLOBAL) && ws + x <= kLdsLimit;
+  // (the level-parallel schedule shares the solve vector between work-groups: it lives in global memory)
+  const bool levels = !force_fp64 && o.schedule == GS_SCHEDULE_LEVELS;
+  r.x_lds = !levels && !(o.flags & GS_FLAG_X_GLOBAL) && ws + x <= kLdsLimit;
   r.lds_bytes = ws + (r.x_lds ? x : 0);
   return r;
 }
@@ -346,11 +358,60 @@ PreparedRun prepare_instances(gs_ctx* ctx, DevicePlan& dp, DevMem& inst_buf, Dev
 }
 
 void launch_prepared(const DevPlan& h, const PreparedRun& pr, int schedule, hipStream_t s) {
+  if (schedule < 0) fail(GS_INVALID_ARGUMENT, "the level-parallel schedule runs one call at a time (gs_nr_solve_batch)");
   const int slice = schedule > 0 ? schedule : pr.count;
   for (int first = 0; first < pr.count; first += slice)
     check(launch_nr_batch(pr.run.fp32, pr.run.x_lds, pr.desc + first, h,
                           std::min(slice, pr.count - first), pr.prm, pr.run.lds_bytes, s),
           "nr kernel");
+}
+
+// The level-parallel schedule: the start kernel, then replays of the captured Newton step until every instance has
+// finished (or paused), polling the finished count after every second replay (a replay after the end still
+// dispatches every kernel of the step); at most max_iterations + 1 replays (the last evaluates the mismatch of the
+// final step). Returns the number of replays.
+int launch_wide(DevicePlan& dp, const PreparedRun& pr, hipStream_t s) {
+  if (!dp.wide) dp.wide = std::make_unique<WideGraph>();
+  WideGraph& wg = *dp.wide;
+  if (!wg.host) {
+    wg.ctl.ensure(sizeof(WideCtl));
+    check(hipHostMalloc((void**)&wg.host, sizeof(WideCtl), hipHostMallocDefault), "pinned control block");
+  }
+  const NrParams& p = pr.prm;
+  const bool same = wg.exec && wg.desc == pr.desc && wg.count == pr.count && wg.fp32 == (int)pr.run.fp32 &&
+                    wg.prm.tolerance == p.tolerance && wg.prm.diverge_limit == p.diverge_limit &&
+                    wg.prm.growth_tol == p.growth_tol && wg.prm.max_iterations == p.max_iterations &&
+                    wg.prm.refine_steps == p.refine_steps;
+  WideCtl* ctl = wg.ctl.as<WideCtl>();
+  if (!same) {
+    if (wg.exec) { (void)hipGraphExecDestroy(wg.exec); wg.exec = nullptr; }
+    if (wg.graph) { (void)hipGraphDestroy(wg.graph); wg.graph = nullptr; }
+    const DevicePlanHost& d = dp.dph;
+    const size_t ws_bytes = (pr.run.fp32 ? 4 : 8) * (size_t)kTracks * (size_t)dp.h.wstride;
+    check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "capture");
+    hipError_t e = launch_wide_step(pr.run.fp32, pr.desc, dp.h, pr.count, p, ctl, d.rs_level_chunk, d.fs_level_pass,
+                                    d.bs_level_pass, ws_bytes, s);
+    hipError_t e2 = hipStreamEndCapture(s, &wg.graph);
+    check(e, "level-parallel step");
+    check(e2, "end capture");
+    check(hipGraphInstantiate(&wg.exec, wg.graph, nullptr, nullptr, 0), "instantiate");
+    wg.desc = pr.desc;
+    wg.count = pr.count;
+    wg.fp32 = (int)pr.run.fp32;
+    wg.prm = p;
+  }
+  check(launch_wide_start(pr.desc, dp.h, pr.count, ctl, s), "level-parallel start");
+  const int total = p.max_iterations + 1;
+  int replays = 0;
+  while (replays < total) {
+    const int k = std::min(2, total - replays);
+    for (int i = 0; i < k; i++) check(hipGraphLaunch(wg.exec, s), "graph launch");
+    replays += k;
+    check(hipMemcpyAsync(wg.host, ctl, sizeof(int32_t), hipMemcpyDeviceToHost, s), "poll");
+    check(hipStreamSynchronize(s), "poll sync");
+    if (wg.host->finished >= pr.count) break;
+  }
+  return replays;
 }
 
 // Core launcher for instances sharing one device plan. `states` may be pre-filled (resume) when init == 0.
@@ -373,7 +434,10 @@ double launch_instances(gs_ctx* ctx, DevicePlan& dp, DevMem& inst_buf, DevMem& d
   check(hipEventCreate(&e1), "event");
   check(hipEventRecord(e0, s), "event");
   int nl = 0;
-  if (!plain) {
+  if (!plain && o.schedule == GS_SCHEDULE_LEVELS) {
+    if (!init) fail(GS_INVALID_ARGUMENT, "the level-parallel schedule starts afresh");
+    nl = launch_wide(dp, pr, s);
+  } else if (!plain) {
     launch_prepared(dp.h, pr, o.schedule, s);
     nl = o.schedule > 0 ? (count + o.schedule - 1) / o.schedule : 1;
   } else {
@@ -453,10 +517,12 @@ void finish_paused(gs_net* net, const gs_instance& in, const gs_nr_options& o, b
     start[0].insert(start[0].end(), va.begin(), va.end());
     std::vector<InstState> rs(1, st);
     rs[0].done = 0;
-    kms += launch_instances(ctx, dp, ib, db, sb, ab, 1, &one, nullptr, nullptr, spec, o, plain, rs, false, nullptr,
+    gs_nr_options ow = o;
+    ow.schedule = 0;  // a resumed run is one instance on one wavefront
+    kms += launch_instances(ctx, dp, ib, db, sb, ab, 1, &one, nullptr, nullptr, spec, ow, plain, rs, false, nullptr,
                             &start, &launches);
     st = rs[0];
-    BatchRun r2 = plan_run(dp.h, o, plain);
+    BatchRun r2 = plan_run(dp.h, ow, plain);
     Layout L2 = instance_layout(dp.h, r2.fp32, r2.x_lds);
     copy_state_out(ctx, ib, L2, 0, n, vm.data(), va.data());
     check(hipStreamSynchronize(ctx->stream), "sync resume");
@@ -941,6 +1007,14 @@ GS_API gs_status gs_plan_selfcheck(const gs_plan* p, double* rel_residual, char*
     if (why) std::snprintf(why, 128, "%s", w.c_str());
     return GS_INVALID_ARGUMENT;
   }
+  // the level-parallel decomposition of the same streams, against the padding upload_plan gives them
+  auto ring_up = [](int v) { return (v + kStreamPad - 1) / kStreamPad * kStreamPad; };
+  const DevicePlanHost& d = p->dph;
+  if (!check_level_schedule(d, kStreamPad, ring_up(d.n_rs) + 2 * kStreamPad, ring_up(d.fs_steps) + 2 * kStreamPad,
+                            ring_up(d.bs_steps) + 2 * kStreamPad, w)) {
+    if (why) std::snprintf(why, 128, "level-parallel schedule: %s", w.c_str());
+    return GS_INVALID_ARGUMENT;
+  }
   double rmax = 0, fmax = 0;
   for (int i = 0; i < m; i++) {
     double s = 0;
@@ -998,6 +1072,8 @@ GS_API gs_status gs_batch_create(gs_ctx* ctx, int32_t n_groups, gs_net* const* n
       if (counts[gi] <= 0 || !inst[gi]) fail(GS_INVALID_ARGUMENT, "group " + std::to_string(gi) + ": instances");
       gs_nr_options o = normalise(opts ? &opts[gi] : nullptr);
       if ((o.flat_start != 0) != net->flat_start) fail(GS_INVALID_ARGUMENT, "flat_start differs from the upload");
+      if (o.schedule < 0)
+        fail(GS_INVALID_ARGUMENT, "group " + std::to_string(gi) + ": batches launch a wavefront per instance");
       for (int i = 0; i < counts[gi]; i++)
         if (inst[gi][i].outage_branch < -1 || inst[gi][i].outage_branch >= net->net.nb ||
             !(inst[gi][i].load_scale > 0))

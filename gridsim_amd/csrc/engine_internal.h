@@ -20,6 +20,10 @@ hipError_t launch_nr_batch(int lu_fp32, int x_in_lds, const InstDesc* descs, con
                            const NrParams& prm, size_t lds_bytes, hipStream_t stream);
 hipError_t launch_seam(const SeamDesc* descs, const DevPlan& plan, int count, size_t lds_bytes, hipStream_t stream);
 hipError_t launch_sincos(const double* x, double* s, double* c, int64_t n, hipStream_t stream);
+hipError_t launch_wide_start(const InstDesc* descs, const DevPlan& plan, int count, WideCtl* ctl, hipStream_t stream);
+hipError_t launch_wide_step(int lu_fp32, const InstDesc* descs, const DevPlan& plan, int count, const NrParams& prm,
+                            WideCtl* ctl, const std::vector<int>& level_chunks, const std::vector<int>& fs_level_pass,
+                            const std::vector<int>& bs_level_pass, size_t ws_bytes, hipStream_t stream);
 hipError_t launch_fp64_probe(double* out, hipStream_t stream);
 
 constexpr size_t kLdsLimit = 64 * 1024;  // dynamic LDS a single-wave work-group may take for the solve workspace
@@ -66,14 +70,36 @@ void upload(DevMem& dst, const std::vector<T>& v, hipStream_t s) {
   if (!v.empty()) check(hipMemcpyAsync(dst.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, s), "upload");
 }
 
+// The captured Newton step of a level-parallel run (one per device plan; rebuilt when the launch it was captured for
+// changes), its control block and the pinned copy the host polls.
+struct WideGraph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  const InstDesc* desc = nullptr;
+  int count = 0, fp32 = 0;
+  NrParams prm{};
+  DevMem ctl;
+  WideCtl* host = nullptr;
+  WideGraph() = default;
+  WideGraph(const WideGraph&) = delete;
+  WideGraph& operator=(const WideGraph&) = delete;
+  ~WideGraph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (host) (void)hipHostFree(host);
+  }
+};
+
 // A plan and its network pattern uploaded to the device.
 struct DevicePlan {
   DevMem y_rp, y_ci, y_row, y_diag, y_g, y_b, slots, aslots, ppos, qpos, j_rp, j_ci, vm0, va0;
   DevMem pinv, q, prow, piv_pos;
   DevMem rs_rec, ch_a_src;
   DevMem fs_rec, bs_rec;
+  DevMem rs_chunk_row, fs_pass_step, bs_pass_step;
   DevPlan h{};   // passed to the kernels by value
   DevicePlanHost dph;
+  std::unique_ptr<WideGraph> wide;
 };
 
 struct SeamPlan;
@@ -163,7 +189,8 @@ PreparedRun prepare_into(const DevPlan& h, const RunBuffers& rb, Staging& stg, i
                          const std::vector<std::vector<double>>* spec_override,
                          const std::vector<std::vector<double>>* start_override, hipStream_t s);
 // The batched launch of a prepared set: every instance in one kernel, or (schedule > 0) launches of at most
-// `schedule` instances one after another on the stream.
+// `schedule` instances one after another on the stream. (The level-parallel schedule polls the device between its
+// graph replays and runs through launch_instances only.)
 void launch_prepared(const DevPlan& h, const PreparedRun& pr, int schedule, hipStream_t s);
 // A paused instance (a static pivot failed its growth check) finished by fresh host symbolic passes and resumed
 // launches; vm / va (in and out) hold its state.

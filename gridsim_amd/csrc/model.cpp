@@ -671,11 +671,15 @@ void compile_plan(const Plan& plan, DevicePlanHost& d) {
   // entry's value gets the next position of its value array, step by step, so that a step's values are contiguous.
   auto build_stream = [&](const vector<int>& order, const vector<int>& level_end, int nlv, const vector<int>& rptr,
                           const vector<int>& rcol, const vector<int>& ridx, bool back, vector<int>& rec,
-                          vector<int>& pos, int& n_val, int& steps) {
+                          vector<int>& pos, int& n_val, int& steps, vector<int>& pass_step,
+                          vector<int>& level_pass) {
     int b0 = 0;
     n_val = 0;
     steps = 0;
+    pass_step.clear();
+    level_pass.clear();
     for (int lv = 0; lv < nlv; lv++) {
+      level_pass.push_back((int)pass_step.size());
       int b1 = level_end[lv];
       vector<int> rows;
       for (int t = b0; t < b1; t++) {
@@ -687,6 +691,7 @@ void compile_plan(const Plan& plan, DevicePlanHost& d) {
       for (size_t p0 = 0; p0 < rows.size(); p0 += 64) {
         size_t p1 = std::min(rows.size(), p0 + 64);
         int len = std::max(1, rptr[rows[p0] + 1] - rptr[rows[p0]]);
+        pass_step.push_back(steps);
         for (int e = 0; e < len; e++) {
           int flags = (e == 0 ? DevicePlanHost::kStepBegin : 0) | (e + 1 == len ? DevicePlanHost::kStepEnd : 0);
           for (int lane = 0; lane < 64; lane++) {
@@ -706,13 +711,15 @@ void compile_plan(const Plan& plan, DevicePlanHost& d) {
       }
       b0 = b1;
     }
+    level_pass.push_back((int)pass_step.size());
+    pass_step.push_back(steps);
   };
   d.fpos.assign(d.nnz_l, -1);
   d.bpos.assign(d.nnz_u, -1);
   build_stream(d.fw_order, d.fw_level_end, d.fw_levels, d.lr_ptr, d.lr_col, d.lr_idx, false, d.fs_rec, d.fpos,
-               d.n_fval, d.fs_steps);
+               d.n_fval, d.fs_steps, d.fs_pass_step, d.fs_level_pass);
   build_stream(d.bw_order, d.bw_level_end, d.bw_levels, d.ur_ptr, d.ur_col, d.ur_idx, true, d.bs_rec, d.bpos,
-               d.n_bval, d.bs_steps);
+               d.n_bval, d.bs_steps, d.bs_pass_step, d.bs_level_pass);
   // The refactor's unified row stream (see DevicePlanHost): per chunk its ASSIGN rows, its UPDATE rows (no-op rows
   // first where a row would read a level that ended fewer than kRsValueAhead rows before), a PIVOT row and its WRITE
   // rows. The L values the updates read live in vals at n_a + CSC index.
@@ -734,9 +741,14 @@ void compile_plan(const Plan& plan, DevicePlanHost& d) {
       for (int l = 0; l < 64; l++) { rec[4 * l] = 0xFFFF; rec[4 * l + 1] = 0; rec[4 * l + 2] = 0; rec[4 * l + 3] = 0; }
       emit(D::kRsNop, lv, rec.data());
     };
+    d.rs_chunk_row.clear();
+    d.rs_level_chunk.assign(1, 0);
     for (int c = 0; c < d.n_chunks; c++) {
       const int t0 = d.ch_track_ptr[c], ntr = d.ch_track_ptr[c + 1] - t0;
       const int lv = level[d.tr_col[t0]];
+      if (lv + 1 < (int)d.rs_level_chunk.size()) throw std::runtime_error("plan: chunks out of level order");
+      while ((int)d.rs_level_chunk.size() <= lv) d.rs_level_chunk.push_back(c);
+      d.rs_chunk_row.push_back((int)d.rs_meta.size());
       // ASSIGN rows
       const int a0 = d.ch_a_ptr[c], a1 = d.ch_a_ptr[c + 1];
       for (int e0 = a0; e0 < std::max(a1, a0 + 1); e0 += 64) {
@@ -819,6 +831,8 @@ void compile_plan(const Plan& plan, DevicePlanHost& d) {
       }
     }
     d.n_rs = (int)d.rs_meta.size();
+    d.rs_chunk_row.push_back(d.n_rs);
+    while ((int)d.rs_level_chunk.size() <= n_levels) d.rs_level_chunk.push_back(d.n_chunks);
     // fold each row's kind and flags into its records
     for (int r = 0; r < d.n_rs; r++)
       for (int l = 0; l < 64; l++) d.rs_rec[((size_t)r * 64 + l) * 4 + 1] |= d.rs_meta[r] << D::kRsMetaShift;
@@ -839,6 +853,99 @@ void compile_plan(const Plan& plan, DevicePlanHost& d) {
 }  // namespace gs
 
 namespace gs {
+
+bool check_level_schedule(const DevicePlanHost& d, int ring, int padded_rows, int padded_fs, int padded_bs,
+                          std::string& why) {
+  using D = DevicePlanHost;
+  auto bad = [&](const std::string& w) { why = w; return false; };
+  const int nc = d.n_chunks, nl = d.n_levels;
+  if ((int)d.rs_chunk_row.size() != nc + 1 || (int)d.rs_level_chunk.size() != nl + 1) return bad("chunk tables size");
+  if (d.rs_chunk_row[0] != 0 || d.rs_chunk_row[nc] != d.n_rs) return bad("chunk rows do not cover the stream");
+  if (d.rs_level_chunk[0] != 0 || d.rs_level_chunk[nl] != nc) return bad("level chunks do not cover the chunks");
+  const int n_vals = d.n_a + d.nnz_l;
+  vector<int> val_level(n_vals, -1), bpos_chunk(std::max(1, d.n_bval), -1), dpos_chunk(d.m, -1);
+  vector<int> fpos_chunk(std::max(1, d.n_fval), -1);
+  for (int l = 0; l < nl; l++) {
+    if (d.rs_level_chunk[l + 1] <= d.rs_level_chunk[l]) return bad("a level without chunks");
+    // every read of the level first (the level's own writes must not be read by it)
+    for (int c = d.rs_level_chunk[l]; c < d.rs_level_chunk[l + 1]; c++) {
+      const int r0 = d.rs_chunk_row[c], r1 = d.rs_chunk_row[c + 1];
+      if (r1 <= r0) return bad("an empty chunk");
+      const int last_load = r0 + (r1 - r0 + ring - 1) / ring * ring - 1 + ring;
+      if (last_load >= padded_rows) return bad("a chunk's look-ahead passes the stream's padding");
+      if ((d.rs_meta[r0] & D::kRsKindMask) != D::kRsAssign || !(d.rs_meta[r0] & D::kRsClear))
+        return bad("a chunk does not start by clearing its workspace");
+      for (int r = r0; r < r1; r++) {
+        if (d.rs_level[r] != l) return bad("a chunk row of another level");
+        if (r > r0 && (d.rs_meta[r] & D::kRsClear)) return bad("a workspace clear inside a chunk");
+        if ((d.rs_meta[r] & D::kRsKindMask) != D::kRsUpdate) continue;
+        for (int lane = 0; lane < 64; lane++) {
+          const int* rr = &d.rs_rec[((size_t)r * 64 + lane) * 4];
+          if ((rr[0] & 0xFFFF) == 0xFFFF) continue;
+          const int v = rr[1] & D::kRsIndexMask;
+          if (v < d.n_a || v >= n_vals || val_level[v] < 0 || val_level[v] >= l)
+            return bad("a chunk reads an L value its level's predecessors did not write");
+        }
+      }
+    }
+    // then the level's writes: disjoint between its chunks
+    for (int c = d.rs_level_chunk[l]; c < d.rs_level_chunk[l + 1]; c++) {
+      for (int r = d.rs_chunk_row[c]; r < d.rs_chunk_row[c + 1]; r++) {
+        if ((d.rs_meta[r] & D::kRsKindMask) != D::kRsWrite) continue;
+        for (int lane = 0; lane < 64; lane++) {
+          const int* rr = &d.rs_rec[((size_t)r * 64 + lane) * 4];
+          const int wk = rr[0] >> 16;
+          if (wk == 0) continue;
+          int* owner = nullptr;
+          if (wk == D::kWriteU) owner = &bpos_chunk.at(rr[2]);
+          else if (wk == D::kWritePivot) owner = &dpos_chunk.at(rr[2]);
+          else {
+            if (val_level.at(rr[2]) >= 0) return bad("an L value written twice");
+            val_level[rr[2]] = l;
+            owner = &fpos_chunk.at(rr[3]);
+          }
+          if (*owner >= 0 && *owner != c) return bad("two chunks write one factor position");
+          *owner = c;
+        }
+      }
+    }
+  }
+  // solve passes: each level's passes cover its steps; a pass reads only rows final before its level
+  auto passes = [&](const vector<int>& rec, int steps, const vector<int>& pass_step, const vector<int>& level_pass,
+                    int levels, int padded, bool back) -> bool {
+    if ((int)level_pass.size() != levels + 1 || pass_step.empty() || pass_step.front() != 0 ||
+        pass_step.back() != steps || level_pass[levels] + 1 != (int)pass_step.size())
+      return bad("pass tables");
+    vector<int> row_level(d.m, -1);
+    if (!back)
+      for (int i = 0; i < d.m; i++) if (d.lr_ptr[i + 1] == d.lr_ptr[i]) row_level[i] = -2;  // final from the start
+    for (int l = 0; l < levels; l++) {
+      vector<int> rows;
+      for (int p = level_pass[l]; p < level_pass[l + 1]; p++) {
+        const int s0 = pass_step[p], s1 = pass_step[p + 1];
+        if (s1 <= s0) return bad("an empty pass");
+        if (s0 + (s1 - s0 + ring - 1) / ring * ring - 1 + ring >= padded) return bad("a pass's look-ahead passes the padding");
+        if (!(rec[((size_t)s0 * 64) * 4 + 3] & D::kStepBegin) || !(rec[((size_t)(s1 - 1) * 64) * 4 + 3] & D::kStepEnd))
+          return bad("a pass does not begin and end on its own steps");
+        for (int t = s0; t < s1; t++)
+          for (int lane = 0; lane < 64; lane++) {
+            const int* r4 = &rec[((size_t)t * 64 + lane) * 4];
+            if (r4[0] >= 0 && (row_level.at(r4[0]) == -1 || row_level[r4[0]] >= l))
+              return bad("a pass reads a row its level's predecessors did not finish");
+            if (t == s0 && r4[2] >= 0) rows.push_back(r4[2]);
+          }
+      }
+      for (int r : rows) {
+        if (row_level.at(r) >= 0) return bad("a row solved twice");
+        row_level[r] = l;
+      }
+    }
+    return true;
+  };
+  if (!passes(d.fs_rec, d.fs_steps, d.fs_pass_step, d.fs_level_pass, d.fw_levels, padded_fs, false)) return false;
+  if (!passes(d.bs_rec, d.bs_steps, d.bs_pass_step, d.bs_level_pass, d.bw_levels, padded_bs, true)) return false;
+  return true;
+}
 
 bool emulate_device_lu(const DevicePlanHost& d, int nnz_j, const double* jv, const double* rhs, double* out,
                        std::string& why) {

@@ -147,6 +147,9 @@ struct DevicePlanHost {
   static constexpr int kRsMetaShift = 24, kRsIndexMask = (1 << kRsMetaShift) - 1;
   int n_rs = 0, n_a = 0;
   vector<int> rs_rec, rs_meta, rs_level;            // [4 * 64 * n_rs] (meta folded into y), [n_rs], [n_rs]
+  // For level-parallel schedules: chunk c's rows are [rs_chunk_row[c], rs_chunk_row[c + 1]) of the row stream, and
+  // level l's chunks are [rs_level_chunk[l], rs_level_chunk[l + 1]).
+  vector<int> rs_chunk_row, rs_level_chunk;
   // Triangular solves as step streams: a pass is up to 64 rows of one level, one lane per row (rows by decreasing
   // length), one step per entry; record (col, value index, row, flags) per lane and step, col = -1 for no entry. The
   // factors are stored in the order the streams read them: fval holds L's off-diagonal entries in forward-stream
@@ -156,6 +159,8 @@ struct DevicePlanHost {
   static constexpr int kStepBegin = 1, kStepEnd = 2;
   int fs_steps = 0, bs_steps = 0, n_fval = 0, n_bval = 0;
   vector<int> fs_rec, bs_rec;                       // 4 ints per lane per step
+  // pass p of a stream covers steps [pass_step[p], pass_step[p + 1]); level l's passes are [level_pass[l], [l + 1])
+  vector<int> fs_pass_step, fs_level_pass, bs_pass_step, bs_level_pass;
   vector<int> fpos, bpos;                           // [nnz_l], [nnz_u]
   // Level-scheduled triangular solves, rows pulled in the reference's update order.
   int fw_levels = 0, bw_levels = 0;
@@ -169,5 +174,12 @@ void compile_plan(const Plan& plan, DevicePlanHost& out);
 // out-of-range index or a zero pivot. The check the kernels' indexing gets on the CPU.
 bool emulate_device_lu(const DevicePlanHost& d, int nnz_j, const double* jv, const double* rhs, double* out,
                        std::string& why);
+
+// The level-parallel schedule's decomposition of the same streams: every chunk (solve pass) of a level starts its own
+// workspace (row), reads only factors (rows) that earlier levels wrote, writes positions no other chunk of its level
+// writes, and its look-ahead of `ring` rows past its range rounded up to whole rings stays inside the padded stream
+// (padded_rows / padded_fs / padded_bs rows or steps). False (and why) otherwise.
+bool check_level_schedule(const DevicePlanHost& d, int ring, int padded_rows, int padded_fs, int padded_bs,
+                          std::string& why);
 
 }  // namespace gs

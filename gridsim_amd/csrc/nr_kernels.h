@@ -25,16 +25,20 @@ struct DevPlan {
   const int32_t *pinv, *q, *prow;     // row permutation, column permutation, pivot row of each step
   const int32_t* piv_pos;             // per step: candidates before the pivot in reach order
   // The refactor: one stream of rows, each one operation of the wavefront with one int4 record per lane (ASSIGN,
-  // UPDATE, PIVOT, WRITE; model.h DevicePlanHost), padded with no-op rows to a multiple of kStreamPad rows plus one
+  // UPDATE, PIVOT, WRITE; model.h DevicePlanHost), padded with no-op rows to a multiple of kStreamPad rows plus two
   // more kStreamPad, so the kernel loads ahead without bounds checks. n_a: A entries (the first part of vals).
   int32_t wstride, n_rs, n_a, pad_rs;
-  const int4* rs_rec;                 // [n_rs + kStreamPad][64]; y = value index | (kind | flags) << kRsMetaShift
+  const int4* rs_rec;                 // [n_rs + 2 kStreamPad][64]; y = value index | (kind | flags) << kRsMetaShift
   const int32_t* ch_a_src;            // per A entry: J CSR position (the seam fills vals from its matrix)
   // Triangular-solve step streams: per lane and step (col, value index, row, flags). The factors live in the order
   // the streams read them: fval (L off-diagonals, forward order), bval (U off-diagonals, back order), dval (U's
   // diagonal by pivot step).
   int32_t fs_steps, bs_steps, n_fval, n_bval;
   const int4 *fs_rec, *bs_rec;
+  // Level-parallel schedules: first row of each refactor chunk (+ the end), first step of each solve pass (+ the end).
+  // The streams carry two kStreamPad rings of padding, so one chunk's or pass's range rounded up to whole rings can
+  // load a ring ahead.
+  const int32_t *rs_chunk_row, *fs_pass_step, *bs_pass_step;
 };
 
 // refactor row kinds and flags, write kinds (model.h DevicePlanHost)
@@ -68,6 +72,16 @@ struct InstState {
   double start_mis, max_mis, min_ratio, pad2;
   // time in each phase (device wall-clock ticks): mismatch + Jacobian, refactor, solves + refinement, update
   uint64_t t_mismatch, t_refactor, t_solve, t_update;
+  // level-parallel schedules: reductions across the work-groups of one instance (bit patterns of non-negative
+  // doubles, which order as unsigned integers; a NaN orders above +inf)
+  uint64_t mis_bits, dx_bits, ratio_bits;
+  int32_t jbad, wpad;
+};
+
+// Control block of a level-parallel run: how many of its instances have finished (or paused).
+struct WideCtl {
+  int32_t finished;
+  int32_t pad[15];
 };
 
 struct InstDesc {

@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <vector>
 
 #include "fdlibm.h"
 #include "gridsim_hip.h"
@@ -112,11 +113,14 @@ __device__ __forceinline__ void put_j(const Inst<T>& I, int s, int a, double v, 
 }
 
 // ------------------------------------------------------------------------------------------------ mismatch + J
-// Returns max |f| (NaN-propagating) over the wave; sets *bad when a Jacobian value is not finite. Each Jacobian value
-// goes, in the LU precision, to its refactor record (vals) and, when the solves are refined (keep_jv), to the FP64 CSR
-// copy the refinement residual reads.
+// Each Jacobian value goes, in the LU precision, to its refactor record (vals) and, when the solves are refined
+// (keep_jv), to the FP64 CSR copy the refinement residual reads. Both passes walk blocks of 64 cells (buses): blocks
+// blk0, blk0 + stride, ... of the wave (one wave: 0, 1; a level-parallel grid: one block per work-group).
+
+// Every Y cell's two injection terms and its off-diagonal Jacobian entries, one cell per lane. Returns whether a
+// Jacobian value of this lane was not finite.
 template <typename T>
-__device__ double mismatch_jacobian(const DevPlan& P, const Inst<T>& I, int lane, bool keep_jv, int* bad) {
+__device__ int mismatch_cells(const DevPlan& P, const Inst<T>& I, int lane, bool keep_jv, int blk0, int stride) {
   GLOBAL const int* y_row = glob(P.y_row);
   GLOBAL const int* y_ci = glob(P.y_ci);
   GLOBAL const double* y_g = glob(P.y_g);
@@ -124,106 +128,127 @@ __device__ double mismatch_jacobian(const DevPlan& P, const Inst<T>& I, int lane
   GLOBAL const v4i* slots = globv<v4i>(P.slots);
   GLOBAL const v4i* aslots = globv<v4i>(P.aslots);
   const int nnz = P.nnz_y;
+  const int nblk = (nnz + kWave - 1) / kWave;
   int lbad = 0;
-  // cells: stage 0 holds the pattern of cell t + 2 (lane-strided), stage 1 the gathered state of cell t + 1
-  {
-    const int iters = (nnz + kWave - 1) / kWave;
-    int i1 = 0, j1 = 0, i2 = 0, j2 = 0;
-    double g1 = 0, b1 = 0, g2 = 0, b2 = 0;
-    v4i s1 = (v4i)(-1), a1 = s1, s2 = s1, a2 = s1;
-    double vai1 = 0, vaj1 = 0, vmi1 = 0, vmj1 = 0;
-    auto load_pattern = [&](int t, int& i, int& j, double& g, double& b, v4i& s, v4i& a) {
-      const int k = lane + t * kWave;
-      const int kk = k < nnz ? k : 0;
-      i = y_row[kk]; j = y_ci[kk]; g = y_g[kk]; b = y_b[kk]; s = slots[kk]; a = aslots[kk];
-    };
-    // (cells past the end read cell 0: every load is in bounds and unconditional, so the waits stay partial)
-    load_pattern(0, i1, j1, g1, b1, s1, a1);
-    load_pattern(1, i2, j2, g2, b2, s2, a2);
+  // stage 0 holds the pattern of block t + 2 (lane-strided), stage 1 the gathered state of block t + 1
+  const int iters = blk0 < nblk ? (nblk - blk0 + stride - 1) / stride : 0;
+  int i1 = 0, j1 = 0, i2 = 0, j2 = 0;
+  double g1 = 0, b1 = 0, g2 = 0, b2 = 0;
+  v4i s1 = (v4i)(-1), a1 = s1, s2 = s1, a2 = s1;
+  double vai1 = 0, vaj1 = 0, vmi1 = 0, vmj1 = 0;
+  auto load_pattern = [&](int t, int& i, int& j, double& g, double& b, v4i& s, v4i& a) {
+    const int k = lane + (blk0 + t * stride) * kWave;
+    const int kk = k < nnz ? k : 0;
+    i = y_row[kk]; j = y_ci[kk]; g = y_g[kk]; b = y_b[kk]; s = slots[kk]; a = aslots[kk];
+  };
+  // (cells past the end read cell 0: every load is in bounds and unconditional, so the waits stay partial)
+  load_pattern(0, i1, j1, g1, b1, s1, a1);
+  load_pattern(1, i2, j2, g2, b2, s2, a2);
+  vai1 = I.va[i1]; vaj1 = I.va[j1]; vmi1 = I.vm[i1]; vmj1 = I.vm[j1];
+  for (int t = 0; t < iters; t++) {
+    // this cell
+    const int k = lane + (blk0 + t * stride) * kWave;
+    const int i = i1, j = j1;
+    const double g0 = g1, b0 = b1, vai = vai1, vaj = vaj1, vmi = vmi1, vmj = vmj1;
+    const v4i s = s1, a = a1;
+    // advance the pipeline: gather for block t + 1, pattern for block t + 2
+    i1 = i2; j1 = j2; g1 = g2; b1 = b2; s1 = s2; a1 = a2;
     vai1 = I.va[i1]; vaj1 = I.va[j1]; vmi1 = I.vm[i1]; vmj1 = I.vm[j1];
-    for (int t = 0; t < iters; t++) {
-      // this cell
-      const int k = lane + t * kWave;
-      const int i = i1, j = j1;
-      const double g0 = g1, b0 = b1, vai = vai1, vaj = vaj1, vmi = vmi1, vmj = vmj1;
-      const v4i s = s1, a = a1;
-      // advance the pipeline: gather for cell t + 1, pattern for cell t + 2
-      i1 = i2; j1 = j2; g1 = g2; b1 = b2; s1 = s2; a1 = a2;
-      vai1 = I.va[i1]; vaj1 = I.va[j1]; vmi1 = I.vm[i1]; vmj1 = I.vm[j1];
-      load_pattern(t + 2, i2, j2, g2, b2, s2, a2);
-      if (k < nnz) {
-        double g, b;
-        y_value(P, I, k, g0, b0, &g, &b);
-        double sn, cs;
-        gs_fdlibm::sincos(vai - vaj, &sn, &cs);
-        const double tp = vmj * (g * cs + b * sn);
-        const double tq = vmj * (g * sn - b * cs);
-        I.tpq[k] = (v2d){tp, tq};
-        if (j != i) {
-          const double h = vmi * vmj * (g * sn - b * cs);
-          const double nn = vmi * (g * cs + b * sn);
-          const double mm = -vmi * vmj * (g * cs + b * sn);
-          const double l = vmi * (g * sn - b * cs);
-          put_j(I, s.x, a.x, h, keep_jv);
-          put_j(I, s.y, a.y, nn, keep_jv);
-          put_j(I, s.z, a.z, mm, keep_jv);
-          put_j(I, s.w, a.w, l, keep_jv);
-          lbad |= !(isfinite(h) && isfinite(nn) && isfinite(mm) && isfinite(l));
-        }
+    load_pattern(t + 2, i2, j2, g2, b2, s2, a2);
+    if (k < nnz) {
+      double g, b;
+      y_value(P, I, k, g0, b0, &g, &b);
+      double sn, cs;
+      gs_fdlibm::sincos(vai - vaj, &sn, &cs);
+      const double tp = vmj * (g * cs + b * sn);
+      const double tq = vmj * (g * sn - b * cs);
+      I.tpq[k] = (v2d){tp, tq};
+      if (j != i) {
+        const double h = vmi * vmj * (g * sn - b * cs);
+        const double nn = vmi * (g * cs + b * sn);
+        const double mm = -vmi * vmj * (g * cs + b * sn);
+        const double l = vmi * (g * sn - b * cs);
+        put_j(I, s.x, a.x, h, keep_jv);
+        put_j(I, s.y, a.y, nn, keep_jv);
+        put_j(I, s.z, a.z, mm, keep_jv);
+        put_j(I, s.w, a.w, l, keep_jv);
+        lbad |= !(isfinite(h) && isfinite(nn) && isfinite(mm) && isfinite(l));
       }
     }
   }
-  WG_SYNC();
-  // buses: ordered sums of the row's terms, diagonal entries, mismatch; the next bus's scalars are loaded ahead
+  return lbad;
+}
+
+// Per bus the ordered sums of its row's terms, the diagonal Jacobian entries and the mismatch; the next bus's
+// scalars are loaded ahead. Returns this lane's max |f| (NaN-propagating); *bad: a Jacobian value was not finite.
+template <typename T>
+__device__ double mismatch_buses(const DevPlan& P, const Inst<T>& I, int lane, bool keep_jv, int blk0, int stride,
+                                 int* bad) {
+  GLOBAL const double* y_g = glob(P.y_g);
+  GLOBAL const double* y_b = glob(P.y_b);
+  GLOBAL const v4i* slots = globv<v4i>(P.slots);
+  GLOBAL const v4i* aslots = globv<v4i>(P.aslots);
   GLOBAL const int* y_rp = glob(P.y_rp);
   GLOBAL const int* y_diag = glob(P.y_diag);
   GLOBAL const int* ppos = glob(P.ppos);
   GLOBAL const int* qpos = glob(P.qpos);
   const int n = P.n;
   double lmax = 0.0;
-  {
-    int i = lane;
-    int nk0 = 0, nk1 = 0, nkd = 0, npp = -1, nqq = -1;
-    double nvm = 0;
-    auto load_bus = [&](int ii) {
-      const int c = ii < n ? ii : 0;
-      nk0 = y_rp[c]; nk1 = y_rp[c + 1]; nkd = y_diag[c]; npp = ppos[c]; nqq = qpos[c]; nvm = I.vm[c];
-    };
-    load_bus(i);
-    for (; i < n; i += kWave) {
-      const int k0 = nk0, k1 = nk1, kd = nkd, pp = npp, qq = nqq;
-      const double vmi = nvm;
-      load_bus(i + kWave);
-      const double gd0 = y_g[kd], bd0 = y_b[kd];
-      const v4i s = slots[kd], a = aslots[kd];
-      const double ps = I.pspec[i], qs = I.qspec[i];
-      double pi = 0.0, qi = 0.0;
-      int k = k0;
-      for (; k + 2 <= k1; k += 2) {
-        const v2d u = I.tpq[k], v = I.tpq[k + 1];
-        pi += u.x; qi += u.y;
-        pi += v.x; qi += v.y;
-      }
-      if (k < k1) { const v2d u = I.tpq[k]; pi += u.x; qi += u.y; }
-      const double P_i = vmi * pi, Q_i = vmi * qi;
-      I.p[i] = P_i;
-      I.q[i] = Q_i;
-      double g, b;
-      y_value(P, I, kd, gd0, bd0, &g, &b);
-      const double h = -Q_i - b * vmi * vmi;
-      const double nn = P_i / vmi + g * vmi;
-      const double mm = P_i - g * vmi * vmi;
-      const double l = Q_i / vmi - b * vmi;
-      put_j(I, s.x, a.x, h, keep_jv);
-      put_j(I, s.y, a.y, nn, keep_jv);
-      put_j(I, s.z, a.z, mm, keep_jv);
-      put_j(I, s.w, a.w, l, keep_jv);
-      lbad |= !(isfinite(h) && isfinite(nn) && isfinite(mm) && isfinite(l));
-      if (pp >= 0) { const double fv = ps - P_i; I.f[pp] = fv; lmax = nan_max(lmax, fabs(fv)); }
-      if (qq >= 0) { const double fv = qs - Q_i; I.f[qq] = fv; lmax = nan_max(lmax, fabs(fv)); }
+  int lbad = 0;
+  int i = lane + blk0 * kWave;
+  const int step = stride * kWave;
+  int nk0 = 0, nk1 = 0, nkd = 0, npp = -1, nqq = -1;
+  double nvm = 0;
+  auto load_bus = [&](int ii) {
+    const int c = ii < n ? ii : 0;
+    nk0 = y_rp[c]; nk1 = y_rp[c + 1]; nkd = y_diag[c]; npp = ppos[c]; nqq = qpos[c]; nvm = I.vm[c];
+  };
+  load_bus(i);
+  for (; i < n; i += step) {
+    const int k0 = nk0, k1 = nk1, kd = nkd, pp = npp, qq = nqq;
+    const double vmi = nvm;
+    load_bus(i + step);
+    const double gd0 = y_g[kd], bd0 = y_b[kd];
+    const v4i s = slots[kd], a = aslots[kd];
+    const double ps = I.pspec[i], qs = I.qspec[i];
+    double pi = 0.0, qi = 0.0;
+    int k = k0;
+    for (; k + 2 <= k1; k += 2) {
+      const v2d u = I.tpq[k], v = I.tpq[k + 1];
+      pi += u.x; qi += u.y;
+      pi += v.x; qi += v.y;
     }
+    if (k < k1) { const v2d u = I.tpq[k]; pi += u.x; qi += u.y; }
+    const double P_i = vmi * pi, Q_i = vmi * qi;
+    I.p[i] = P_i;
+    I.q[i] = Q_i;
+    double g, b;
+    y_value(P, I, kd, gd0, bd0, &g, &b);
+    const double h = -Q_i - b * vmi * vmi;
+    const double nn = P_i / vmi + g * vmi;
+    const double mm = P_i - g * vmi * vmi;
+    const double l = Q_i / vmi - b * vmi;
+    put_j(I, s.x, a.x, h, keep_jv);
+    put_j(I, s.y, a.y, nn, keep_jv);
+    put_j(I, s.z, a.z, mm, keep_jv);
+    put_j(I, s.w, a.w, l, keep_jv);
+    lbad |= !(isfinite(h) && isfinite(nn) && isfinite(mm) && isfinite(l));
+    if (pp >= 0) { const double fv = ps - P_i; I.f[pp] = fv; lmax = nan_max(lmax, fabs(fv)); }
+    if (qq >= 0) { const double fv = qs - Q_i; I.f[qq] = fv; lmax = nan_max(lmax, fabs(fv)); }
   }
-  *bad = wave_any(lbad);
+  *bad = lbad;
+  return lmax;
+}
+
+// The whole pass for one wave. Returns max |f| (NaN-propagating) over the wave; sets *bad when a Jacobian value is
+// not finite.
+template <typename T>
+__device__ double mismatch_jacobian(const DevPlan& P, const Inst<T>& I, int lane, bool keep_jv, int* bad) {
+  const int cbad = mismatch_cells<T>(P, I, lane, keep_jv, 0, 1);
+  WG_SYNC();
+  int bbad;
+  const double lmax = mismatch_buses<T>(P, I, lane, keep_jv, 0, 1, &bbad);
+  *bad = wave_any(cbad | bbad);
   return wave_max(lmax);
 }
 
@@ -249,16 +274,18 @@ __device__ double mismatch_jacobian(const DevPlan& P, const Inst<T>& I, int lane
 constexpr int kRing = kStreamPad, kHalf = kRing / 2, kAhead = kRsValueAhead;
 static_assert(kAhead < kRing, "the value of a row is loaded after its record");
 
+// Rows [row0, row1) of the stream (the whole stream, or one chunk of a level-parallel run): the loop runs whole rings
+// from row0 and treats the rows from row1 on as no-ops.
 template <typename T, bool EXACT>
 __device__ __forceinline__ int refactor(const DevPlan& P, GLOBAL T* vals, GLOBAL T* fval, GLOBAL T* bval,
                                         GLOBAL T* dval, LDSP T* ws, double growth_tol, GLOBAL const double* rowfloor,
-                                        int lane, double* min_ratio) {
-  GLOBAL const v4i* rs_rec = globv<v4i>(P.rs_rec);
+                                        int lane, double* min_ratio, int row0, int row1) {
+  GLOBAL const v4i* rs_rec = globv<v4i>(P.rs_rec) + (size_t)row0 * kWave;
   GLOBAL const int* piv_pos = glob(P.piv_pos);
   GLOBAL const int* prow = glob(P.prow);
   const int tl = lane % kTrackLanes;
   const int wsz = kTracks * P.wstride;
-  const int nrows = P.n_rs;
+  const int nrows = row1 - row0;
   double mratio = *min_ratio;
   // The row's kind is read from the lane's own record (every lane holds the same), not from a per-row array: a value
   // the compiler knows to be uniform would be moved to a scalar register as soon as it is loaded, which waits for it.
@@ -273,11 +300,12 @@ __device__ __forceinline__ int refactor(const DevPlan& P, GLOBAL T* vals, GLOBAL
   // (a failed pivot check ends the loop at the next ring boundary, not at once: an exit from inside the unrolled body
   // would make the compiler drain every load in flight at each row; the rows run meanwhile only write factors that
   // the caller discards)
-  for (int base = 0; base < nrows && failed < 0; base += kRing) {  // nrows: a multiple of kRing
+  for (int base = 0; base < nrows && failed < 0; base += kRing) {
 #pragma unroll
     for (int s = 0; s < kRing; s++) {
       const v4i r4 = rr[s];
-      const int meta = (int)((unsigned)r4.y >> kRsMetaShift), kind = meta & kRsKindMask;
+      const int meta = base + s < nrows ? (int)((unsigned)r4.y >> kRsMetaShift) : kRsNop;
+      const int kind = meta & kRsKindMask;
       if (kind == kRsAssign) {
         if (meta & kRsClear) {
           for (int q = lane; q < wsz; q += kWave) ws[q] = T(0);
@@ -355,9 +383,13 @@ __device__ __forceinline__ int refactor(const DevPlan& P, GLOBAL T* vals, GLOBAL
 // factors are final before a solve starts; the streams are padded by kRing steps); a step's values are contiguous
 // (its active lanes are a prefix), so each value load is one coalesced segment. x is read at the step itself.
 // XP: the workspace pointer (LDS or global).
+// Steps [step0, step1) of the stream (the whole stream, or one pass of a level-parallel run): whole rings from step0,
+// the steps from step1 on doing nothing.
 template <typename T, bool BACK, bool XLDS, typename XP>
-__device__ __forceinline__ void solve_stream(GLOBAL const v4i* rec, int steps, GLOBAL const T* vals,
+__device__ __forceinline__ void solve_stream(GLOBAL const v4i* rec, int step0, int step1, GLOBAL const T* vals,
                                              GLOBAL const T* dval, XP x, int lane) {
+  rec += (size_t)step0 * kWave;
+  const int steps = step1 - step0;
   v4i rr[kRing];
   T rv[kRing], rd[kRing];
 #pragma unroll
@@ -368,11 +400,12 @@ __device__ __forceinline__ void solve_stream(GLOBAL const v4i* rec, int steps, G
     rd[s] = BACK ? dval[rr[s].z < 0 ? 0 : rr[s].z] : T(1);
   }
   T acc = T(0);
-  for (int base = 0; base < steps; base += kRing) {  // steps: a multiple of kRing (the stream is padded with no-ops)
+  for (int base = 0; base < steps; base += kRing) {
 #pragma unroll
     for (int s = 0; s < kRing; s++) {
       const int t = base + s;
-      const v4i r4 = rr[s];
+      v4i r4 = rr[s];
+      if (t >= steps) r4 = (v4i){-1, 0, -1, 0};
       const int fl = r4.w;
       if (fl & kStepBegin) acc = r4.z >= 0 ? x[r4.z] : T(0);
       if (r4.x >= 0) acc = acc - rv[s] * x[r4.x];
@@ -397,8 +430,8 @@ __device__ __forceinline__ void lu_solve(const DevPlan& P, GLOBAL const T* fval,
   GLOBAL const int* q = glob(P.q);
   for (int r = lane; r < m; r += kWave) x[pinv[r]] = (T)rhs[r];
   if (XLDS) WAVE_SYNC(); else WG_SYNC();
-  solve_stream<T, false, XLDS>(globv<v4i>(P.fs_rec), P.fs_steps, fval, dval, x, lane);
-  solve_stream<T, true, XLDS>(globv<v4i>(P.bs_rec), P.bs_steps, bval, dval, x, lane);
+  solve_stream<T, false, XLDS>(globv<v4i>(P.fs_rec), 0, P.fs_steps, fval, dval, x, lane);
+  solve_stream<T, true, XLDS>(globv<v4i>(P.bs_rec), 0, P.bs_steps, bval, dval, x, lane);
   for (int k = lane; k < m; k += kWave) out[q[k]] = (double)x[k];
   WG_SYNC();
 }
@@ -497,7 +530,7 @@ __global__ __launch_bounds__(kWave, 2) void nr_batch_kernel(const InstDesc* __re
     if (steps >= prm.steps_this_launch) { paused = 1; break; }
     if (bad || !isfinite(max_mis)) break;  // NewtonShell.SolveStep refuses a non-finite J or f
     fail_col = refactor<T, false>(P, I.vals, I.fval, I.bval, I.dval, ws, prm.growth_tol, nullptr,
-                                  opaque_lane(lane), &min_ratio);
+                                  opaque_lane(lane), &min_ratio, 0, P.n_rs);
     tick(t_fac);
     if (fail_col >= 0) { paused = 2; break; }
     solve(I.f, I.dx);
@@ -557,6 +590,260 @@ __global__ __launch_bounds__(kWave, 2) void nr_batch_kernel(const InstDesc* __re
   }
 }
 
+// ------------------------------------------------------------------------------------------------ level-parallel
+// The instances of a call spread over the whole GPU (schedule GS_SCHEDULE_LEVELS): each phase of a Newton step is its
+// own kernel over a grid of (blocks of the phase, instances), and the refactor and the two triangular solves are one
+// kernel per elimination level, a work-group per chunk of 8 columns (per pass of 64 rows), so the columns of a level
+// run on as many compute units as the level has chunks. Kernel boundaries order the levels and make each level's
+// factors visible to the next. The host captures one Newton step as a hipGraph and replays it; every kernel returns at
+// once for an instance whose run has ended (its state's done flag). Reductions across the work-groups of an
+// instance (max |f|, max |dx|, the smallest pivot ratio) go through atomics on the bit patterns of non-negative
+// doubles. The arithmetic of every value is the batched kernel's: the same device functions over the same streams.
+
+template <typename T>
+__device__ __forceinline__ Inst<T> load_inst(const InstDesc* descs, int b) {
+  const InstDesc& d = descs[b];
+  Inst<T> I;
+  I.pspec = glob(d.pspec); I.qspec = glob(d.qspec); I.vm_start = glob(d.vm_start); I.va_start = glob(d.va_start);
+  I.vm = glob(d.vm); I.va = glob(d.va); I.p = glob(d.p); I.q = glob(d.q);
+  I.f = glob(d.f); I.dx = glob(d.dx); I.r = glob(d.r); I.jv = glob(d.jv);
+  I.tpq = (GLOBAL v2d*)d.tpq;
+  I.vals = glob(reinterpret_cast<T*>(d.vals));
+  I.fval = glob(reinterpret_cast<T*>(d.fval)); I.bval = glob(reinterpret_cast<T*>(d.bval));
+  I.dval = glob(reinterpret_cast<T*>(d.dval));
+  I.xg = glob(reinterpret_cast<T*>(d.xg));
+  I.patch = glob(&descs[b].patch[0]);
+  I.n_patch = d.n_patch;
+#pragma unroll
+  for (int t = 0; t < kMaxPatch; t++) I.pidx[t] = t < I.n_patch ? d.patch[t].idx : -1;
+  return I;
+}
+
+__device__ __forceinline__ void atomic_max_bits(uint64_t* p, double v) {  // v >= 0, or a NaN with its sign clear
+  atomicMax(reinterpret_cast<unsigned long long*>(p), (unsigned long long)__double_as_longlong(v));
+}
+__device__ __forceinline__ double bits_double(uint64_t b) { return __longlong_as_double((long long)b); }
+
+// Ends an instance's run (NewtonShell.AssembleResult precedence NonFinite > Converged > Diverged > MaxIterations):
+// the non-finite scan over the buses by the calling work-group (nthreads threads, thread t), then the state, then the
+// finished count. Called by one work-group of the instance.
+__device__ void wide_finish(const DevPlan& P, InstState* st, const double* vm, const double* va, const double* p,
+                            const double* q, double max_mis, int converged, int diverged, int t, int nthreads,
+                            int* shared_flag, WideCtl* ctl) {
+  int lnf = 0;
+  for (int i = t; i < P.n; i += nthreads)
+    lnf |= !(isfinite(vm[i]) && isfinite(va[i]) && isfinite(p[i] * P.base_mva) && isfinite(q[i] * P.base_mva));
+  if (t == 0) *shared_flag = 0;
+  __syncthreads();
+  if (lnf) atomicOr(shared_flag, 1);
+  __syncthreads();
+  if (t == 0) {
+    st->max_mis = max_mis;
+    st->converged = converged;
+    st->diverged = diverged;
+    st->nonfinite = *shared_flag || !isfinite(max_mis);
+    st->min_ratio = bits_double(st->ratio_bits);
+    if (atomicCAS(&st->done, 0, 1) == 0) atomicAdd(&ctl->finished, 1);
+  }
+}
+
+// start point and state of every instance (grid: blocks of 256 buses x instances)
+__global__ __launch_bounds__(256) void wide_start(const InstDesc* __restrict__ descs, const DevPlan P, WideCtl* ctl) {
+  const InstDesc& d = descs[blockIdx.y];
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < P.n) { d.vm[i] = d.vm_start[i]; d.va[i] = d.va_start[i]; }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    InstState* st = d.state;
+    st->iter = 0; st->done = 0; st->converged = 0; st->diverged = 0; st->nonfinite = 0; st->step_failures = 0;
+    st->fail_column = -1;
+    st->start_mis = __builtin_nan(""); st->max_mis = __builtin_nan(""); st->min_ratio = 1.0;
+    st->t_mismatch = st->t_refactor = st->t_solve = st->t_update = 0;
+    st->mis_bits = 0; st->dx_bits = 0; st->ratio_bits = (uint64_t)__double_as_longlong(1.0); st->jbad = 0;
+    if (blockIdx.y == 0) ctl->finished = 0;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kWave) void wide_cells(const InstDesc* __restrict__ descs, const DevPlan P,
+                                                    const NrParams prm) {
+  InstState* st = descs[blockIdx.y].state;
+  if (st->done) return;
+  const Inst<T> I = load_inst<T>(descs, blockIdx.y);
+  const int bad = mismatch_cells<T>(P, I, threadIdx.x, prm.refine_steps > 0, blockIdx.x, gridDim.x);
+  if (wave_any(bad) && threadIdx.x == 0) atomicOr(&st->jbad, 1);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kWave) void wide_buses(const InstDesc* __restrict__ descs, const DevPlan P,
+                                                    const NrParams prm) {
+  InstState* st = descs[blockIdx.y].state;
+  if (st->done) return;
+  const Inst<T> I = load_inst<T>(descs, blockIdx.y);
+  int bad;
+  const double lmax = wave_max(mismatch_buses<T>(P, I, threadIdx.x, prm.refine_steps > 0, blockIdx.x, gridDim.x,
+                                                 &bad));
+  const int anybad = wave_any(bad);
+  if (threadIdx.x == 0) {
+    atomic_max_bits(&st->mis_bits, lmax);
+    if (anybad) atomicOr(&st->jbad, 1);
+  }
+}
+
+// the verdicts after a mismatch evaluation (RunNewton :235-238, NewtonShell.DivergedBeyondLimit :177-178); one
+// work-group per instance
+__global__ __launch_bounds__(kWave) void wide_verdict(const InstDesc* __restrict__ descs, const DevPlan P,
+                                                      const NrParams prm, WideCtl* ctl) {
+  __shared__ int flag;
+  const InstDesc& d = descs[blockIdx.y];
+  InstState* st = d.state;
+  if (st->done) return;
+  const double max_mis = bits_double(st->mis_bits);
+  const int bad = st->jbad;
+  const int iter = st->iter;
+  double start_mis = st->start_mis;
+  if (iter == 0) start_mis = max_mis;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    st->mis_bits = 0; st->jbad = 0; st->dx_bits = 0;
+    st->start_mis = start_mis;
+    st->max_mis = max_mis;
+  }
+  const int converged = max_mis < prm.tolerance;
+  const int diverged = !converged && max_mis > nan_max(prm.diverge_limit, start_mis);
+  const bool stop = converged || diverged || iter >= prm.max_iterations || bad || !isfinite(max_mis);
+  if (stop) wide_finish(P, st, d.vm, d.va, d.p, d.q, max_mis, converged, diverged, threadIdx.x, kWave, &flag, ctl);
+}
+
+// Chunks [first_chunk, end_chunk): one elimination level with a work-group per chunk, or (a single work-group) a run
+// of consecutive narrow levels walked in stream order, as the batched kernel walks them (the stream's no-op rows and
+// level-end fences order the levels inside the run).
+template <typename T>
+__global__ __launch_bounds__(kWave) void wide_refactor(const InstDesc* __restrict__ descs, const DevPlan P,
+                                                       const NrParams prm, WideCtl* ctl, int first_chunk,
+                                                       int end_chunk) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  InstState* st = descs[blockIdx.y].state;
+  if (st->done) return;
+  const Inst<T> I = load_inst<T>(descs, blockIdx.y);
+  const int per = (end_chunk - first_chunk) / (int)gridDim.x;
+  const int c = first_chunk + blockIdx.x * per;
+  const int row0 = P.rs_chunk_row[c], row1 = P.rs_chunk_row[c + per];
+  double ratio = 1.0;
+  const int fail = refactor<T, false>(P, I.vals, I.fval, I.bval, I.dval, lds(reinterpret_cast<T*>(smem)),
+                                      prm.growth_tol, nullptr, opaque_lane(threadIdx.x), &ratio, row0, row1);
+  if (threadIdx.x == 0) {
+    if (ratio < 1.0) atomicMin(reinterpret_cast<unsigned long long*>(&st->ratio_bits),
+                               (unsigned long long)__double_as_longlong(ratio));
+    if (fail >= 0) {
+      // a static pivot failed its growth check: the instance pauses for a fresh host symbolic pass
+      st->fail_column = fail;
+      st->min_ratio = bits_double(st->ratio_bits);
+      if (atomicCAS(&st->done, 0, 2) == 0) atomicAdd(&ctl->finished, 1);
+    }
+  }
+}
+
+// x (the solve vector, in global memory) = the row-permuted right-hand side (f, or the refinement residual r)
+template <typename T>
+__global__ __launch_bounds__(256) void wide_permute(const InstDesc* __restrict__ descs, const DevPlan P, int use_r) {
+  const InstDesc& d = descs[blockIdx.y];
+  if (d.state->done) return;
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if (r < P.m) reinterpret_cast<T*>(d.xg)[P.pinv[r]] = (T)(use_r ? d.r : d.f)[r];
+}
+
+// Passes [first_pass, end_pass) of a triangular solve: one level with a work-group per pass, or (a single work-group)
+// a run of consecutive narrow levels in stream order.
+template <typename T, bool BACK>
+__global__ __launch_bounds__(kWave) void wide_solve(const InstDesc* __restrict__ descs, const DevPlan P,
+                                                    int first_pass, int end_pass) {
+  const InstDesc& d = descs[blockIdx.y];
+  if (d.state->done) return;
+  const int per = (end_pass - first_pass) / (int)gridDim.x;
+  const int p = first_pass + blockIdx.x * per;
+  const int* pass_step = BACK ? P.bs_pass_step : P.fs_pass_step;
+  GLOBAL T* x = glob(reinterpret_cast<T*>(d.xg));
+  solve_stream<T, BACK, false>(globv<v4i>(BACK ? P.bs_rec : P.fs_rec), pass_step[p], pass_step[p + per],
+                               glob(reinterpret_cast<const T*>(BACK ? d.bval : d.fval)),
+                               glob(reinterpret_cast<const T*>(d.dval)), x, opaque_lane(threadIdx.x));
+}
+
+// dx (or r) = the column-permuted solve vector
+template <typename T>
+__global__ __launch_bounds__(256) void wide_unpermute(const InstDesc* __restrict__ descs, const DevPlan P,
+                                                      int to_r) {
+  const InstDesc& d = descs[blockIdx.y];
+  if (d.state->done) return;
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k < P.m) (to_r ? d.r : d.dx)[P.q[k]] = (double)reinterpret_cast<const T*>(d.xg)[k];
+}
+
+// r = f - J dx over 64 rows per work-group
+__global__ __launch_bounds__(kWave) void wide_residual(const InstDesc* __restrict__ descs, const DevPlan P) {
+  const InstDesc& d = descs[blockIdx.y];
+  if (d.state->done) return;
+  const int i = blockIdx.x * kWave + threadIdx.x;
+  if (i >= P.m) return;
+  GLOBAL const int* j_rp = glob(P.j_rp);
+  GLOBAL const int* j_ci = glob(P.j_ci);
+  GLOBAL const double* jv = glob(d.jv);
+  GLOBAL const double* dx = glob(d.dx);
+  double acc = 0.0;
+  for (int k = j_rp[i]; k < j_rp[i + 1]; k++) acc += jv[k] * dx[j_ci[k]];
+  d.r[i] = d.f[i] - acc;
+}
+
+__global__ __launch_bounds__(256) void wide_add_correction(const InstDesc* __restrict__ descs, const DevPlan P) {
+  const InstDesc& d = descs[blockIdx.y];
+  if (d.state->done) return;
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k < P.m) d.dx[k] = d.dx[k] + d.r[k];
+}
+
+// max |dx| of the instance (NaN and inf order above every finite value)
+__global__ __launch_bounds__(256) void wide_dx_max(const InstDesc* __restrict__ descs, const DevPlan P) {
+  const InstDesc& d = descs[blockIdx.y];
+  InstState* st = d.state;
+  if (st->done) return;
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  double v = 0.0;
+  if (k < P.m) {
+    const double x = d.dx[k];
+    v = isfinite(x) ? fabs(x) : __builtin_inf();
+  }
+  v = wave_max(v);
+  if ((threadIdx.x % kWave) == 0) atomic_max_bits(&st->dx_bits, v);
+}
+
+// the damped update (NewtonShell.ApplyStep :128-136), or the end of the run when a step is not finite
+__global__ __launch_bounds__(256) void wide_update(const InstDesc* __restrict__ descs, const DevPlan P,
+                                                   WideCtl* ctl) {
+  __shared__ int flag;
+  const InstDesc& d = descs[blockIdx.y];
+  InstState* st = d.state;
+  if (st->done) return;
+  const double max_abs = bits_double(st->dx_bits);
+  if (!isfinite(max_abs)) {
+    // NewtonShell.SolveStep: a non-finite step ends the run
+    if (blockIdx.x == 0) {
+      if (threadIdx.x == 0) st->step_failures++;
+      wide_finish(P, st, d.vm, d.va, d.p, d.q, st->max_mis, 0, 0, threadIdx.x, 256, &flag, ctl);
+    }
+    return;
+  }
+  const double damp = max_abs > 0.4 ? 0.4 / max_abs : 1.0;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < P.n) {
+    const int pp = P.ppos[i], qq = P.qpos[i];
+    if (pp >= 0) d.va[i] = d.va[i] + damp * d.dx[pp];
+    if (qq >= 0) {
+      double v = d.vm[i] + damp * d.dx[qq];
+      d.vm[i] = v < 0.01 ? 0.01 : v;
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->iter = st->iter + 1;
+}
+
 // ------------------------------------------------------------------------------------------------ seam solves
 // One CSR system on a plan made from that system's own factorisation: the numeric refactor replays the reference's
 // arithmetic (SparseLuFactorization.Refactor), refusing exactly where it would (a pivot that moves, or one at or below
@@ -581,7 +868,8 @@ __global__ __launch_bounds__(kWave) void seam_solve_kernel(const SeamDesc* __res
   WG_SYNC();
   double ratio = 1.0;
   const int fail =
-      refactor<double, true>(P, vals, glob(d.fval), glob(d.bval), glob(d.dval), ws, 0.0, rowfloor, lane, &ratio);
+      refactor<double, true>(P, vals, glob(d.fval), glob(d.bval), glob(d.dval), ws, 0.0, rowfloor, lane, &ratio, 0,
+                             P.n_rs);
   if (fail >= 0) {
     if (lane == 0) *d.status = 1;
     return;
@@ -621,6 +909,80 @@ hipError_t launch_nr_batch(int lu_fp32, int x_in_lds, const InstDesc* descs, con
     else
       hipLaunchKernelGGL((nr_batch_kernel<double, false>), grid, block, lds_bytes, stream, descs, plan, prm);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_wide_start(const InstDesc* descs, const DevPlan& plan, int count, WideCtl* ctl, hipStream_t stream) {
+  hipLaunchKernelGGL(wide_start, dim3((plan.n + 255) / 256, count), dim3(256), 0, stream, descs, plan, ctl);
+  return hipGetLastError();
+}
+
+// A level of at most this many chunks (solve passes) is cheaper walked in sequence by one work-group, together with
+// its narrow neighbours, than given a kernel of its own.
+constexpr int kNarrow = 4;
+
+// One Newton step of every instance, as the kernel sequence the host captures: mismatch, verdict, refactor level by
+// level, the solves level by level (with the refinement steps), the update. level_chunks / fs_level_pass /
+// bs_level_pass: first chunk (pass) of each level, and the end.
+template <typename T>
+static void enqueue_wide_step(const InstDesc* descs, const DevPlan& P, int count, const NrParams& prm, WideCtl* ctl,
+                              const std::vector<int>& level_chunks, const std::vector<int>& fs_level_pass,
+                              const std::vector<int>& bs_level_pass, size_t ws_bytes, hipStream_t s) {
+  const dim3 w(kWave), b256(256);
+  const unsigned nb64 = (unsigned)((P.nnz_y + kWave - 1) / kWave), nbus64 = (unsigned)((P.n + kWave - 1) / kWave);
+  const unsigned m256 = (unsigned)((P.m + 255) / 256), m64 = (unsigned)((P.m + kWave - 1) / kWave);
+  const unsigned n256 = (unsigned)((P.n + 255) / 256), cnt = (unsigned)count;
+  hipLaunchKernelGGL(wide_cells<T>, dim3(nb64, cnt), w, 0, s, descs, P, prm);
+  hipLaunchKernelGGL(wide_buses<T>, dim3(nbus64, cnt), w, 0, s, descs, P, prm);
+  hipLaunchKernelGGL(wide_verdict, dim3(1, cnt), w, 0, s, descs, P, prm, ctl);
+  // levels of at most kNarrow chunks (passes) run back to back on one work-group; wider levels one kernel each
+  auto segments = [](const std::vector<int>& level_first, auto&& launch) {
+    int run0 = -1, run1 = -1;
+    for (size_t l = 0; l + 1 < level_first.size(); l++) {
+      const int a = level_first[l], b = level_first[l + 1];
+      if (b <= a) continue;
+      if (b - a <= kNarrow) {
+        if (run0 < 0) run0 = a;
+        run1 = b;
+        continue;
+      }
+      if (run0 >= 0) { launch(run0, run1, 1); run0 = -1; }
+      launch(a, b, b - a);
+    }
+    if (run0 >= 0) launch(run0, run1, 1);
+  };
+  segments(level_chunks, [&](int c0, int c1, int blocks) {
+    hipLaunchKernelGGL(wide_refactor<T>, dim3(blocks, cnt), w, ws_bytes, s, descs, P, prm, ctl, c0, c1);
+  });
+  auto solve = [&](int use_r) {
+    hipLaunchKernelGGL(wide_permute<T>, dim3(m256, cnt), b256, 0, s, descs, P, use_r);
+    segments(fs_level_pass, [&](int p0, int p1, int blocks) {
+      hipLaunchKernelGGL((wide_solve<T, false>), dim3(blocks, cnt), w, 0, s, descs, P, p0, p1);
+    });
+    segments(bs_level_pass, [&](int p0, int p1, int blocks) {
+      hipLaunchKernelGGL((wide_solve<T, true>), dim3(blocks, cnt), w, 0, s, descs, P, p0, p1);
+    });
+    hipLaunchKernelGGL(wide_unpermute<T>, dim3(m256, cnt), b256, 0, s, descs, P, use_r);
+  };
+  solve(0);
+  for (int it = 0; it < prm.refine_steps; it++) {
+    hipLaunchKernelGGL(wide_residual, dim3(m64, cnt), w, 0, s, descs, P);
+    solve(1);
+    hipLaunchKernelGGL(wide_add_correction, dim3(m256, cnt), b256, 0, s, descs, P);
+  }
+  hipLaunchKernelGGL(wide_dx_max, dim3(m256, cnt), b256, 0, s, descs, P);
+  hipLaunchKernelGGL(wide_update, dim3(n256, cnt), b256, 0, s, descs, P, ctl);
+}
+
+hipError_t launch_wide_step(int lu_fp32, const InstDesc* descs, const DevPlan& plan, int count, const NrParams& prm,
+                            WideCtl* ctl, const std::vector<int>& level_chunks, const std::vector<int>& fs_level_pass,
+                            const std::vector<int>& bs_level_pass, size_t ws_bytes, hipStream_t stream) {
+  if (lu_fp32)
+    enqueue_wide_step<float>(descs, plan, count, prm, ctl, level_chunks, fs_level_pass, bs_level_pass, ws_bytes,
+                             stream);
+  else
+    enqueue_wide_step<double>(descs, plan, count, prm, ctl, level_chunks, fs_level_pass, bs_level_pass, ws_bytes,
+                              stream);
   return hipGetLastError();
 }
 

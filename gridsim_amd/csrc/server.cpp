@@ -411,6 +411,8 @@ GS_API gs_status gs_server_add_network(gs_server* srv, gs_net* net, const gs_nr_
     auto nt = std::make_unique<gs_server::Net>();
     nt->net = net;
     nt->opts = normalise(opts);
+    if (nt->opts.schedule < 0)
+      fail(GS_INVALID_ARGUMENT, "the serving loop launches a wavefront per instance (schedule >= 0)");
     nt->run = plan_run(net->dev.h, nt->opts, false);
     nt->L = instance_layout(net->dev.h, nt->run.fp32, nt->run.x_lds);
     nt->state_span = nt->L.va - nt->L.vm + 8 * (size_t)net->net.n;

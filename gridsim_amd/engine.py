@@ -31,6 +31,7 @@ GS_OK, GS_SINGULAR, GS_NONFINITE_INPUT, GS_OUT_OF_MEMORY, GS_INVALID_ARGUMENT, G
 CONVERGED, MAX_ITERATIONS, DIVERGED, NONFINITE = 0, 1, 2, 3
 LU_FP64, LU_FP32 = 0, 1
 FLAG_X_GLOBAL = 1
+SCHEDULE_LEVELS = -1   # every instance spread over the whole GPU, level by level (GS_SCHEDULE_LEVELS)
 STATUS_NAMES = {0: "Converged", 1: "MaxIterations", 2: "Diverged", 3: "NonFinite"}
 
 

@@ -154,8 +154,9 @@ def _dc_angle_range(n: int, edges: np.ndarray, x: np.ndarray, pinj: np.ndarray, 
 def synthetic_grid(n_bus: int, n_branch: int, n_pv: int, seed: int, *, xfmr_frac: float = 0.10,
                    r_range=(0.001, 0.05), load_model: str = "uniform", pf: float = 0.95,
                    pf_range=(0.9, 0.98), lognormal=(-1.0, 1.0), base_mva: float = 100.0,
-                   theta_max: float = 1.0, name: str | None = None) -> Network:
-    """One seeded synthetic grid (see the module docstring for the recipe)."""
+                   theta_max: float = 1.0, name: str | None = None, return_points: bool = False):
+    """One seeded synthetic grid (see the module docstring for the recipe); with return_points also the bus
+    coordinates in the unit square."""
     if n_pv > n_bus - 1:
         raise ValueError("too many PV buses")
     rng = np.random.default_rng(seed)
@@ -207,13 +208,14 @@ def synthetic_grid(n_bus: int, n_branch: int, n_pv: int, seed: int, *, xfmr_frac
     kappa = min(1.0, theta_max / theta) if theta > 0 else 1.0
     r = r * kappa
     x = x * kappa
-    return Network(
+    net = Network(
         base_mva=base_mva, bus_type=bus_type, pd=pd, qd=qd, gs=np.zeros(n_bus), bs=np.zeros(n_bus),
         vm=np.ones(n_bus), va_deg=np.zeros(n_bus),
         br_from=edges[:, 0], br_to=edges[:, 1], br_r=r, br_x=x, br_b=bch, br_tap=tap,
         br_shift_deg=np.zeros(nb), br_in_service=np.ones(nb, dtype=np.uint8),
         gen_bus=gen_bus, gen_pg=gen_pg, gen_qg=gen_qg, gen_vg=gen_vg,
         name=name or f"synthetic-{n_bus}b-{nb}br-s{seed}", seed=seed)
+    return (net, pts) if return_points else net
 
 
 # ---- the five workloads -------------------------------------------------------------------------------------------
@@ -251,7 +253,61 @@ def config5_grids(seed: int = BASE_SEED + 5, count: int = 8) -> list[Network]:
     return [sized_grid(n, seed + 1 + i) for i, n in enumerate(config5_sizes(seed, count))]
 
 
-def config4_grid(seed: int = BASE_SEED + 4) -> Network:
-    """70,000 buses (8,000 PV), 88,000 branches, 12 % transformers, lognormal loads."""
-    return synthetic_grid(70000, 88000, 8000, seed, xfmr_frac=0.12, load_model="lognormal",
-                          name="synthetic-70000")
+def config4_grid(seed: int = BASE_SEED + 4, cols: int = 7, rows: int = 5) -> Network:
+    """The large single instance: 70,000 buses (8,000 PV) and 88,000 branches, 12 % transformers, lognormal loads
+    (mu = -1, sigma = 1) pu at pf ~ U[0.9, 0.98], built as cols x rows regions of 2,000 buses joined by tie lines.
+
+    Each region is one synthetic grid (2,509 branches, its share of the 8,000 PV buses, its own slack bus and the
+    impedance scaling of the recipe) placed in its own cell of a cols x rows layout; every pair of side-by-side regions
+    is joined by tie lines between the bus pairs nearest across their common border (88,000 branches in all), with
+    r ~ U[0.01, 0.05] pu, x = r * U[3, 10] and charging b ~ U[0, 0.05] pu, unscaled. Drawn as one 70,000-bus recipe
+    grid instead, the impedance scaling that keeps its DC angles within 1 rad shrinks every impedance by about 1e-4
+    (median x about 1e-5 pu), and the set-point spread and transformer taps across such branches leave Newton from
+    flat start without a reachable solution; a region keeps the 2,000-bus grids' scale. Each region balances its own
+    generation at its slack (35 slack buses at angle 0)."""
+    regions = cols * rows
+    n_region, br_region = 2000, 2509
+    pv_split = np.full(regions, 8000 // regions)
+    pv_split[: 8000 - pv_split.sum()] += 1
+    rng = np.random.default_rng(seed)
+    parts = []
+    for i in range(regions):
+        net, pts = synthetic_grid(n_region, br_region, int(pv_split[i]), seed + 1 + i, xfmr_frac=0.12,
+                                  load_model="lognormal", return_points=True)
+        parts.append((net, pts + np.array([i % cols, i // cols], dtype=np.float64)))
+    # tie lines: side-by-side regions, nearest bus pairs across the border, spread evenly over the pairs
+    pairs = [(i, i + 1) for i in range(regions) if i % cols != cols - 1] + \
+            [(i, i + cols) for i in range(regions - cols)]
+    n_ties = 88000 - regions * br_region
+    per_pair = np.full(len(pairs), n_ties // len(pairs))
+    per_pair[: n_ties - per_pair.sum()] += 1
+    ties = []
+    for (a, b), k in zip(pairs, per_pair):
+        pa, pb = parts[a][1], parts[b][1]
+        tree = cKDTree(pb)
+        d, j = tree.query(pa, k=1)
+        used_b = set()
+        for ia in np.argsort(d, kind="stable"):
+            if int(j[ia]) in used_b:
+                continue
+            used_b.add(int(j[ia]))
+            ties.append((a * n_region + int(ia), b * n_region + int(j[ia])))
+            if len(used_b) == k:
+                break
+    nt = len(ties)
+    tr = rng.uniform(0.01, 0.05, nt)
+    tx = tr * rng.uniform(3.0, 10.0, nt)
+    tb = rng.uniform(0.0, 0.05, nt)
+    cat = lambda key: np.concatenate([getattr(p[0], key) for p in parts])
+    off = lambda key: np.concatenate([getattr(p[0], key) + n_region * i for i, p in enumerate(parts)])
+    tie = np.array(ties, dtype=np.int64)
+    return Network(
+        base_mva=parts[0][0].base_mva, bus_type=cat("bus_type"), pd=cat("pd"), qd=cat("qd"), gs=cat("gs"),
+        bs=cat("bs"), vm=cat("vm"), va_deg=cat("va_deg"),
+        br_from=np.concatenate([off("br_from"), tie[:, 0]]), br_to=np.concatenate([off("br_to"), tie[:, 1]]),
+        br_r=np.concatenate([cat("br_r"), tr]), br_x=np.concatenate([cat("br_x"), tx]),
+        br_b=np.concatenate([cat("br_b"), tb]), br_tap=np.concatenate([cat("br_tap"), np.zeros(nt)]),
+        br_shift_deg=np.zeros(regions * br_region + nt),
+        br_in_service=np.ones(regions * br_region + nt, dtype=np.uint8),
+        gen_bus=off("gen_bus"), gen_pg=cat("gen_pg"), gen_qg=cat("gen_qg"), gen_vg=cat("gen_vg"),
+        name="synthetic-70000", seed=seed)

@@ -75,14 +75,21 @@ typedef struct gs_nr_options {
   int32_t lu_precision;     /* GS_LU_FP64 or GS_LU_FP32 */
   int32_t refine_steps;     /* FP64 iterative-refinement steps after each FP32 solve (0..4) */
   double growth_tol;        /* a static pivot is kept while |pivot| >= growth_tol * max |candidate| */
-  int32_t schedule;         /* launch slicing: 0 = all instances of a call in one launch, k > 0 = launches of at
-                               most k instances one after another (gs_nr_autotune chooses) */
+  int32_t schedule;         /* 0 = a wavefront per instance, all instances of a call in one launch; k > 0 = the
+                               same in launches of at most k instances one after another; GS_SCHEDULE_LEVELS = every
+                               instance spread over the whole GPU, level by level (gs_nr_autotune chooses) */
   int32_t flags;            /* GS_FLAG_*: placement choices (gs_nr_autotune chooses) */
 } gs_nr_options;
 
 /* gs_nr_options.flags: keep the triangular-solve vector in global memory even where it fits in LDS (smaller LDS per
  * instance, so more instances resident per compute unit). */
 enum { GS_FLAG_X_GLOBAL = 1 };
+
+/* gs_nr_options.schedule: level-parallel. Each Newton step is a captured hipGraph of short kernels, one per phase and
+ * one per elimination level of the refactor and of each triangular solve, a work-group per chunk of columns (per pass
+ * of rows) of a level and per instance, replayed until every instance has finished. For large networks (tens of
+ * thousands of buses), where one wavefront per instance leaves the GPU idle. gs_nr_solve_batch only. */
+enum { GS_SCHEDULE_LEVELS = -1 };
 
 /* One solve of a batch: the uploaded network with one branch out of service (or none) and every load and generator
  * P scaled by load_scale. vm0 / va0 (radians), when non-null, replace the start point BuildBusSpec would give. */
@@ -192,8 +199,9 @@ GS_API void gs_plan_free(gs_plan* plan);
 /* gs_network_plan_stats of the plan gs_network_upload would upload. */
 GS_API gs_status gs_plan_shape(const gs_plan* plan, gs_plan_stats* out);
 /* Runs the device factorisation and solves of the plan in a host emulation with every index checked, on the
- * Jacobian at the start point, and returns ||J x - f||_inf / ||f||_inf. GS_INVALID_ARGUMENT (with the reason in
- * *why, 128 bytes) when an index falls outside its buffer. */
+ * Jacobian at the start point, and returns ||J x - f||_inf / ||f||_inf; also checks the level-parallel schedule's
+ * split of the same streams (GS_SCHEDULE_LEVELS). GS_INVALID_ARGUMENT (with the reason in *why, 128 bytes) when an
+ * index falls outside its buffer or the split is unsound. */
 GS_API gs_status gs_plan_selfcheck(const gs_plan* plan, double* rel_residual, char* why);
 
 /* ---- multi-network batches ------------------------------------------------------------------------------------

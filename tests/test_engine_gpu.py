@@ -241,3 +241,48 @@ def test_10k_bus_grid_solve_vector_in_global_memory(ctx, eng, oracle):
     r = dn.solve(outages=outs)
     print(f"10k N-1 sample: m={m}, {len(outs)} outages, kernel {r.kernel_ms:.2f} ms", flush=True)
     _compare(oracle, r.results, r.vm, r.va, rres, rvm, rva, "N-1 10k sample")
+
+
+@pytest.mark.parametrize("lu", ["fp64", "fp32"])
+def test_level_parallel_schedule_equals_wavefront_per_instance(ctx, eng, oracle, two_networks, lu):
+    """GS_SCHEDULE_LEVELS (one kernel per phase and per elimination level, a work-group per chunk and instance, the
+    Newton step replayed as a hipGraph) runs the same arithmetic as one wavefront per instance: identical verdicts,
+    iteration counts and voltages, bit for bit; and the oracle's verdicts and voltages to 1e-9."""
+    from gridsim_amd import gridgen
+    rng = np.random.default_rng(11)
+    prec = eng.LU_FP64 if lu == "fp64" else eng.LU_FP32
+    for net in two_networks:
+        outs = rng.choice(gridgen.config2_outages(net), 10, replace=False)
+        scales = rng.uniform(0.8, 1.2, 10)
+        wave = eng.NrOptions(max_iterations=10, lu_precision=prec, refine_steps=0 if lu == "fp64" else 2)
+        lev = eng.NrOptions(max_iterations=10, lu_precision=prec, refine_steps=wave.refine_steps,
+                            schedule=eng.SCHEDULE_LEVELS)
+        dn = eng.DeviceNetwork(ctx, net, wave)
+        a = dn.solve(outages=outs, load_scale=scales, options=wave)
+        b = dn.solve(outages=outs, load_scale=scales, options=lev)
+        print(f"{net.n_bus} buses {lu}: wavefront {a.kernel_ms:.2f} ms, levels {b.kernel_ms:.2f} ms "
+              f"({b.launches} step replays)", flush=True)
+        assert np.array_equal(a.status, b.status)
+        assert np.array_equal(a.results["iterations"], b.results["iterations"])
+        assert np.array_equal(a.vm, b.vm) and np.array_equal(a.va, b.va)
+        for i in range(len(outs)):
+            rec, vm, va = oracle.solve(net.with_load_multiplier(float(scales[i])), oracle.options(max_iterations=10),
+                                       skip=int(outs[i]))
+            assert rec["status"] == b.status[i]
+            if rec["status"] == 0:
+                assert np.abs(vm - b.vm[i]).max() <= TOL_DV and np.abs(va - b.va[i]).max() <= TOL_DV
+        dn.close()
+
+
+def test_level_parallel_flat_start(ctx, eng, oracle):
+    """A flat start on the level-parallel schedule (the 118-bus grid): the oracle's iterations and voltages."""
+    from gridsim_amd import gridgen
+    net = gridgen.config1_grid()
+    rec, vm, va = oracle.solve(net, oracle.options(flat_start=True))
+    o = eng.NrOptions(flat_start=True, lu_precision=eng.LU_FP64, refine_steps=0, schedule=eng.SCHEDULE_LEVELS)
+    dn = eng.DeviceNetwork(ctx, net, o)
+    r = dn.solve(count=1, options=o)
+    assert r.status[0] == rec["status"] == 0
+    assert r.results["iterations"][0] == rec["iterations"]
+    assert np.abs(r.vm[0] - vm).max() <= TOL_DV and np.abs(r.va[0] - va).max() <= TOL_DV
+    dn.close()
