@@ -100,6 +100,37 @@ void upload_plan(DevicePlan& dp, const YBus* y, const JacobianPattern& jp, const
   check(hipStreamSynchronize(s), "upload plan");
 }
 
+void upload_branches(DevicePlan& dp, const Network& net, std::vector<std::vector<double>>& keep, hipStream_t s) {
+  DevPlan& h = dp.h;
+  h.nb = net.nb;
+  std::vector<int> ends(2 * (size_t)std::max(1, net.nb), -1);
+  keep.emplace_back(8 * (size_t)std::max(1, net.nb), 0.0);
+  std::vector<double>& stamp = keep.back();
+  for (int k = 0; k < net.nb; k++) {
+    if (!net.br_in[k]) continue;
+    ends[2 * k] = net.br_from[k];
+    ends[2 * k + 1] = net.br_to[k];
+    const Stamp st = branch_stamp(net.br_r[k], net.br_x[k], net.br_b[k], net.br_tap[k], net.br_shift[k]);
+    const Cx v[4] = {st.yff, st.yft, st.ytf, st.ytt};
+    for (int c = 0; c < 4; c++) { stamp[8 * (size_t)k + 2 * c] = v[c].re; stamp[8 * (size_t)k + 2 * c + 1] = v[c].im; }
+  }
+  upload(dp.br_ends, ends, s);
+  // (the ends vector is a local: this copy must finish before it goes)
+  check(hipStreamSynchronize(s), "branch ends");
+  upload(dp.br_stamp, stamp, s);
+  keep.push_back(net.br_rate.empty() ? std::vector<double>(1, 0.0) : net.br_rate);
+  upload(dp.br_rate, keep.back(), s);
+  keep.push_back(net.bus_lo);
+  upload(dp.bus_lo, keep.back(), s);
+  keep.push_back(net.bus_hi);
+  upload(dp.bus_hi, keep.back(), s);
+  h.br_ends = dp.br_ends.as<int2>();
+  h.br_stamp = dp.br_stamp.as<double>();
+  h.br_rate = dp.br_rate.as<double>();
+  h.bus_lo = dp.bus_lo.as<double>();
+  h.bus_hi = dp.bus_hi.as<double>();
+}
+
 // The shape of a compiled plan as upload_plan lays it out in HBM (record streams padded the same way).
 gs_plan_stats plan_stats(const DevicePlanHost& d, const YBus& y, const JacobianPattern& jp) {
   auto ring_up = [](int v) { return (v + kStreamPad - 1) / kStreamPad * kStreamPad; };
@@ -226,8 +257,9 @@ BatchRun plan_run(const DevPlan& h, const gs_nr_options& o, bool force_fp64) {
 
 // Runs `count` instances on one plan. Fills results, optional vm/va. Handles paused instances (pivot growth) by a
 // fresh host symbolic pass on the instance's own reduced network and a resumed solve on that plan.
+struct HostFinal;
 void run_batch(gs_net* net, int count, const gs_instance* inst, const gs_nr_options& o, gs_nr_result* results,
-               double* vm_out, double* va_out, bool plain);
+               double* vm_out, double* va_out, bool plain, std::vector<HostFinal>* finals = nullptr);
 
 void fill_result(const InstState& s, gs_nr_result& r, int refreshes, double us_per_tick) {
   r.status = s.nonfinite ? GS_PF_NONFINITE : s.converged ? GS_PF_CONVERGED : s.diverged ? GS_PF_DIVERGED
@@ -273,6 +305,7 @@ PreparedRun prepare_into(const DevPlan& h, const RunBuffers& rb, Staging& stg, i
     d.xg = run.x_lds ? nullptr : b + L.xg;
     const gs_instance& in = inst[i];
     d.n_patch = (y && netw) ? outage_patches(*y, *netw, in.outage_branch, d.patch) : 0;
+    d.outage = (y && netw) ? in.outage_branch : -1;
     // specified injections: the network's, or a load-scaled copy
     const std::vector<double>* sp = spec_override ? &(*spec_override)[i] : nullptr;
     std::vector<double> scaled;
@@ -482,7 +515,8 @@ void copy_state_out(gs_ctx* ctx, const DevMem& inst_buf, const Layout& L, int i,
 // at its current state (vm, va: in and out), each followed by a resumed launch on that plan, until it finishes. The
 // resumed run keeps its iteration count and start mismatch.
 void finish_paused(gs_net* net, const gs_instance& in, const gs_nr_options& o, bool plain, InstState st,
-                   std::vector<double>& vm, std::vector<double>& va, gs_nr_result& result, double& kms, int& launches) {
+                   std::vector<double>& vm, std::vector<double>& va, gs_nr_result& result, double& kms, int& launches,
+                   std::vector<double>* p_out, std::vector<double>* q_out) {
   gs_ctx* ctx = net->ctx;
   const int n = net->net.n;
   int refreshes = 0;
@@ -525,6 +559,16 @@ void finish_paused(gs_net* net, const gs_instance& in, const gs_nr_options& o, b
     BatchRun r2 = plan_run(dp.h, ow, plain);
     Layout L2 = instance_layout(dp.h, r2.fp32, r2.x_lds);
     copy_state_out(ctx, ib, L2, 0, n, vm.data(), va.data());
+    if (p_out) {
+      p_out->resize(n);
+      check(hipMemcpyAsync(p_out->data(), ib.as<char>() + L2.p, 8 * (size_t)n, hipMemcpyDeviceToHost, ctx->stream),
+            "p out");
+    }
+    if (q_out) {
+      q_out->resize(n);
+      check(hipMemcpyAsync(q_out->data(), ib.as<char>() + L2.q, 8 * (size_t)n, hipMemcpyDeviceToHost, ctx->stream),
+            "q out");
+    }
     check(hipStreamSynchronize(ctx->stream), "sync resume");
     finished = st.done == 1;
   }
@@ -533,9 +577,16 @@ void finish_paused(gs_net* net, const gs_instance& in, const gs_nr_options& o, b
 
 // Results of a finished device run of `count` instances (states copied back): finished instances straight from the
 // device, paused ones through finish_paused.
+// Final state of an instance the host finished (a fresh symbolic pass): its index and vm, va, p, q.
+struct HostFinal {
+  int i;
+  std::vector<double> vm, va, p, q;
+};
+
 void collect_results(gs_net* net, int count, const gs_instance* inst, const gs_nr_options& o, bool plain,
                      const std::vector<InstState>& states, const DevMem& inst_buf, const Layout& L,
-                     gs_nr_result* results, double* vm_out, double* va_out, double& kms, int& launches) {
+                     gs_nr_result* results, double* vm_out, double* va_out, double& kms, int& launches,
+                     std::vector<HostFinal>* finals = nullptr) {
   gs_ctx* ctx = net->ctx;
   const int n = net->net.n;
   for (int i = 0; i < count; i++) {
@@ -548,17 +599,19 @@ void collect_results(gs_net* net, int count, const gs_instance* inst, const gs_n
   check(hipStreamSynchronize(ctx->stream), "sync out");
   for (int i = 0; i < count; i++) {
     if (states[i].done == 1) continue;
-    std::vector<double> vm(n), va(n);
+    std::vector<double> vm(n), va(n), p, q;
     copy_state_out(ctx, inst_buf, L, i, n, vm.data(), va.data());
     check(hipStreamSynchronize(ctx->stream), "sync pause");
-    finish_paused(net, inst[i], o, plain, states[i], vm, va, results[i], kms, launches);
+    finish_paused(net, inst[i], o, plain, states[i], vm, va, results[i], kms, launches, finals ? &p : nullptr,
+                  finals ? &q : nullptr);
+    if (finals) finals->push_back(HostFinal{i, vm, va, p, q});
     if (vm_out) std::copy(vm.begin(), vm.end(), vm_out + (size_t)n * i);
     if (va_out) std::copy(va.begin(), va.end(), va_out + (size_t)n * i);
   }
 }
 
 void run_batch(gs_net* net, int count, const gs_instance* inst, const gs_nr_options& o, gs_nr_result* results,
-               double* vm_out, double* va_out, bool plain) {
+               double* vm_out, double* va_out, bool plain, std::vector<HostFinal>* finals) {
   gs_ctx* ctx = net->ctx;
   std::vector<InstState> states;
   auto t0 = std::chrono::steady_clock::now();
@@ -568,7 +621,8 @@ void run_batch(gs_net* net, int count, const gs_instance* inst, const gs_nr_opti
                                 &launches);
   BatchRun run = plan_run(net->dev.h, o, plain);
   Layout L = instance_layout(net->dev.h, run.fp32, run.x_lds);
-  collect_results(net, count, inst, o, plain, states, net->inst_buf, L, results, vm_out, va_out, kms, launches);
+  collect_results(net, count, inst, o, plain, states, net->inst_buf, L, results, vm_out, va_out, kms, launches,
+                  finals);
   net->last_kernel_ms = kms;
   net->last_launches = launches;
   net->last_total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -866,6 +920,8 @@ GS_API gs_status gs_network_upload(gs_ctx* ctx, const gs_network* net, const gs_
       return GS_SINGULAR;
     }
     upload_plan(h->dev, &h->y, h->jp, &h->spec, h->plan, h->net.base, ctx->stream);
+    std::vector<std::vector<double>> keep;
+    upload_branches(h->dev, h->net, keep, ctx->stream);
     check(hipStreamSynchronize(ctx->stream), "upload sync");
     *out = h.release();
     return GS_OK;
@@ -886,6 +942,54 @@ GS_API gs_status gs_nr_solve_batch(gs_net* net, int32_t count, const gs_instance
         fail(GS_INVALID_ARGUMENT, "instance " + std::to_string(i));
     if (count == 0) return GS_OK;
     run_batch(net, count, inst, o, results, vm_out, va_out, false);
+    return GS_OK;
+  });
+}
+
+GS_API gs_status gs_nr_screen_batch(gs_net* net, int32_t count, const gs_instance* inst, const gs_nr_options* opts,
+                                   gs_nr_result* results, gs_screen_summary* summaries, double* loading,
+                                   double* flows, double* vm_out, double* va_out) {
+  if (!net) return GS_INVALID_ARGUMENT;
+  return guarded(net->ctx, [&]() -> gs_status {
+    if (count < 0 || (count > 0 && (!inst || !results || !summaries))) fail(GS_INVALID_ARGUMENT, "null argument");
+    gs_nr_options o = normalise(opts);
+    if ((o.flat_start != 0) != net->flat_start) fail(GS_INVALID_ARGUMENT, "flat_start differs from the upload");
+    for (int i = 0; i < count; i++)
+      if (inst[i].outage_branch < -1 || inst[i].outage_branch >= net->net.nb || !(inst[i].load_scale > 0))
+        fail(GS_INVALID_ARGUMENT, "instance " + std::to_string(i));
+    if (count == 0) return GS_OK;
+    std::vector<HostFinal> finals;
+    run_batch(net, count, inst, o, results, vm_out, va_out, false, &finals);
+    gs_ctx* ctx = net->ctx;
+    hipStream_t s = ctx->stream;
+    const BatchRun run = plan_run(net->dev.h, o, false);
+    const Layout L = instance_layout(net->dev.h, run.fp32, run.x_lds);
+    const int n = net->net.n, nb = net->net.nb;
+    // instances the host finished: their final state back into their blocks, where the screen reads it
+    for (const HostFinal& f : finals) {
+      char* b = net->inst_buf.as<char>() + L.stride * (size_t)f.i;
+      check(hipMemcpyAsync(b + L.vm, f.vm.data(), 8 * (size_t)n, hipMemcpyHostToDevice, s), "final vm");
+      check(hipMemcpyAsync(b + L.va, f.va.data(), 8 * (size_t)n, hipMemcpyHostToDevice, s), "final va");
+      // (a run the host stopped before any resumed launch has no new injections: the block keeps its last ones)
+      if ((int)f.p.size() == n)
+        check(hipMemcpyAsync(b + L.p, f.p.data(), 8 * (size_t)n, hipMemcpyHostToDevice, s), "final p");
+      if ((int)f.q.size() == n)
+        check(hipMemcpyAsync(b + L.q, f.q.data(), 8 * (size_t)n, hipMemcpyHostToDevice, s), "final q");
+    }
+    net->screen_buf.ensure(sizeof(gs_screen_summary) * (size_t)count);
+    if (loading) net->loading_buf.ensure(8 * (size_t)count * std::max(1, nb));
+    if (flows) net->flow_buf.ensure(32 * (size_t)count * std::max(1, nb));
+    check(launch_screen(net->desc_buf.as<InstDesc>(), net->dev.h, count, net->screen_buf.as<gs_screen_summary>(),
+                        loading ? net->loading_buf.as<double>() : nullptr, flows ? net->flow_buf.as<double>() : nullptr,
+                        s),
+          "screen kernel");
+    check(hipMemcpyAsync(summaries, net->screen_buf.p, sizeof(gs_screen_summary) * (size_t)count,
+                         hipMemcpyDeviceToHost, s), "summaries");
+    if (loading)
+      check(hipMemcpyAsync(loading, net->loading_buf.p, 8 * (size_t)count * nb, hipMemcpyDeviceToHost, s), "loading");
+    if (flows)
+      check(hipMemcpyAsync(flows, net->flow_buf.p, 32 * (size_t)count * nb, hipMemcpyDeviceToHost, s), "flows");
+    check(hipStreamSynchronize(s), "screen sync");
     return GS_OK;
   });
 }

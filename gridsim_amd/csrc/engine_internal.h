@@ -20,6 +20,8 @@ hipError_t launch_nr_batch(int lu_fp32, int x_in_lds, const InstDesc* descs, con
                            const NrParams& prm, size_t lds_bytes, hipStream_t stream);
 hipError_t launch_seam(const SeamDesc* descs, const DevPlan& plan, int count, size_t lds_bytes, hipStream_t stream);
 hipError_t launch_sincos(const double* x, double* s, double* c, int64_t n, hipStream_t stream);
+hipError_t launch_screen(const InstDesc* descs, const DevPlan& plan, int count, gs_screen_summary* out,
+                         double* loading, double* flows, hipStream_t stream);
 hipError_t launch_wide_start(const InstDesc* descs, const DevPlan& plan, int count, WideCtl* ctl, hipStream_t stream);
 hipError_t launch_wide_step(int lu_fp32, const InstDesc* descs, const DevPlan& plan, int count, const NrParams& prm,
                             WideCtl* ctl, const std::vector<int>& level_chunks, const std::vector<int>& fs_level_pass,
@@ -97,6 +99,7 @@ struct DevicePlan {
   DevMem rs_rec, ch_a_src;
   DevMem fs_rec, bs_rec;
   DevMem rs_chunk_row, fs_pass_step, bs_pass_step;
+  DevMem br_ends, br_stamp, br_rate, bus_lo, bus_hi;
   DevPlan h{};   // passed to the kernels by value
   DevicePlanHost dph;
   std::unique_ptr<WideGraph> wide;
@@ -128,6 +131,7 @@ struct gs_net {
   gs::DevicePlan dev;
   // per-call device arenas
   gs::DevMem inst_buf, desc_buf, state_buf, aux_buf;
+  gs::DevMem screen_buf, loading_buf, flow_buf;
   double last_total_ms = 0, last_kernel_ms = 0;
   int last_launches = 0;
 };
@@ -195,7 +199,11 @@ void launch_prepared(const DevPlan& h, const PreparedRun& pr, int schedule, hipS
 // A paused instance (a static pivot failed its growth check) finished by fresh host symbolic passes and resumed
 // launches; vm / va (in and out) hold its state.
 void finish_paused(gs_net* net, const gs_instance& in, const gs_nr_options& o, bool plain, InstState st,
-                   std::vector<double>& vm, std::vector<double>& va, gs_nr_result& result, double& kms, int& launches);
+                   std::vector<double>& vm, std::vector<double>& va, gs_nr_result& result, double& kms, int& launches,
+                   std::vector<double>* p_out = nullptr, std::vector<double>* q_out = nullptr);
+// The branch data screening reads (end buses, stamps, rate A) and the buses' own voltage limits, uploaded with a
+// network.
+void upload_branches(DevicePlan& dp, const Network& net, std::vector<std::vector<double>>& keep, hipStream_t s);
 
 // Runs f under the context's lock, turning exceptions into status codes and the context's last error.
 template <typename F>

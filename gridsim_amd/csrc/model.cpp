@@ -72,6 +72,10 @@ Network copy_network(const gs_network& s) {
   cp(n.br_tap, s.br_tap, n.nb); cp(n.br_shift, s.br_shift_deg, n.nb); cp(n.br_in, s.br_in_service, n.nb);
   cp(n.gen_bus, s.gen_bus, n.ng); cp(n.gen_pg, s.gen_pg, n.ng); cp(n.gen_qg, s.gen_qg, n.ng);
   cp(n.gen_vg, s.gen_vg, n.ng);
+  const double nan = std::nan("");
+  if (s.br_rate_a) cp(n.br_rate, s.br_rate_a, n.nb); else n.br_rate.assign(n.nb, 0.0);
+  if (s.bus_vmin) cp(n.bus_lo, s.bus_vmin, n.n); else n.bus_lo.assign(n.n, nan);
+  if (s.bus_vmax) cp(n.bus_hi, s.bus_vmax, n.n); else n.bus_hi.assign(n.n, nan);
   return n;
 }
 

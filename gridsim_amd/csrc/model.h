@@ -46,6 +46,7 @@ struct Network {
   vector<uint8_t> br_in;
   vector<int> gen_bus;
   vector<double> gen_pg, gen_qg, gen_vg;
+  vector<double> br_rate, bus_lo, bus_hi;   // rate A (MVA, 0 = unrated); own voltage limits (NaN = the band)
 };
 
 Network copy_network(const gs_network& src);

@@ -39,6 +39,12 @@ struct DevPlan {
   // The streams carry two kStreamPad rings of padding, so one chunk's or pass's range rounded up to whole rings can
   // load a ring ahead.
   const int32_t *rs_chunk_row, *fs_pass_step, *bs_pass_step;
+  // Screening: per branch its end buses (from = -1: out of service), pi-model stamp (Yff, Yft, Ytf, Ytt as re, im)
+  // and rate A; per bus its own voltage limits (NaN = the operational band).
+  int32_t nb, pad_nb;
+  const int2* br_ends;
+  const double* br_stamp;
+  const double *br_rate, *bus_lo, *bus_hi;
 };
 
 // refactor row kinds and flags, write kinds (model.h DevicePlanHost)
@@ -88,7 +94,7 @@ struct InstDesc {
   InstState* state;
   const double *pspec, *qspec;        // specified injections: the network's, or this instance's load-scaled copy
   const double *vm_start, *va_start;  // start point: the network's, or this instance's own
-  int32_t n_patch, pad;
+  int32_t n_patch, outage;            // patched Y cells; the outaged branch (-1: none)
   Patch patch[kMaxPatch];
   // per-instance buffers (device)
   double *vm, *va, *p, *q;

@@ -844,6 +844,106 @@ __global__ __launch_bounds__(256) void wide_update(const InstDesc* __restrict__ 
   if (blockIdx.x == 0 && threadIdx.x == 0) st->iter = st->iter + 1;
 }
 
+// ------------------------------------------------------------------------------------------------ screening
+// Line flows and the classification numbers of every instance's solved state (one work-group of 256 per instance):
+// NewtonShell.ComputeLineFlows (NewtonShell.cs:699-734) per branch, the maximum loading and the overloads
+// (ContingencyAnalysis.ScreenStateCore :1150-1194), Vmin / Vmax and the buses outside their limits (:1196-1234), and
+// AssembleResult's losses as the sums of the bus injections in bus order (:649-690).
+// Complex.Magnitude: the scaled hypotenuse |large| sqrt(1 + (small / large)^2).
+__device__ __forceinline__ double cx_magnitude(double re, double im) {
+  const double a = fabs(re), b = fabs(im);
+  const double small = a < b ? a : b, large = a < b ? b : a;
+  if (small == 0.0) return large;
+  if (isinf(large) && !isnan(small)) return __builtin_inf();
+  const double ratio = small / large;
+  return large * sqrt(1.0 + ratio * ratio);
+}
+
+__global__ __launch_bounds__(256) void screen_kernel(const InstDesc* __restrict__ descs, const DevPlan P,
+                                                     gs_screen_summary* out, double* loading, double* flows) {
+  __shared__ double s_max[256];
+  __shared__ int s_arg[256], s_over[256], s_under[256], s_overv[256];
+  __shared__ double s_vmin[256], s_vmax[256];
+  const InstDesc& d = descs[blockIdx.x];
+  const int t = threadIdx.x;
+  const double base = P.base_mva;
+  double lmax = 0.0;
+  int larg = -1, nover = 0;
+  for (int k = t; k < P.nb; k += 256) {
+    const int2 e = P.br_ends[k];
+    double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (e.x >= 0 && k != d.outage) {
+      double sf, cf, st, ct;
+      gs_fdlibm::sincos(d.va[e.x], &sf, &cf);
+      gs_fdlibm::sincos(d.va[e.y], &st, &ct);
+      const double vfr = d.vm[e.x] * cf, vfi = d.vm[e.x] * sf;  // DetMath.FromPolar
+      const double vtr = d.vm[e.y] * ct, vti = d.vm[e.y] * st;
+      const double* y = P.br_stamp + 8 * (size_t)k;  // Yff, Yft, Ytf, Ytt
+      // I_from = Yff vf + Yft vt, I_to = Ytf vf + Ytt vt (Complex products (ac - bd) + (bc + ad) i)
+      const double ifr = (y[0] * vfr - y[1] * vfi) + (y[2] * vtr - y[3] * vti);
+      const double ifi = (y[1] * vfr + y[0] * vfi) + (y[3] * vtr + y[2] * vti);
+      const double itr = (y[4] * vfr - y[5] * vfi) + (y[6] * vtr - y[7] * vti);
+      const double iti = (y[5] * vfr + y[4] * vfi) + (y[7] * vtr + y[6] * vti);
+      // S = V conj(I) * base
+      const double sfr = (vfr * ifr - vfi * -ifi) * base, sfi = (vfi * ifr + vfr * -ifi) * base;
+      const double str = (vtr * itr - vti * -iti) * base, sti = (vti * itr + vtr * -iti) * base;
+      const double mf = cx_magnitude(sfr, sfi), mt = cx_magnitude(str, sti);
+      const double mva = (isnan(mf) || isnan(mt)) ? __builtin_nan("") : (mf > mt ? mf : mt);
+      const double rate = P.br_rate[k];
+      v[0] = sfr; v[1] = sfi; v[2] = str; v[3] = sti;
+      v[4] = rate > 0 ? mva / rate * 100.0 : 0.0;
+    }
+    if (loading) loading[(size_t)blockIdx.x * P.nb + k] = v[4];
+    if (flows) {
+      double* fo = flows + ((size_t)blockIdx.x * P.nb + k) * 4;
+      fo[0] = v[0]; fo[1] = v[1]; fo[2] = v[2]; fo[3] = v[3];
+    }
+    if (v[4] > lmax) { lmax = v[4]; larg = k; }
+    nover += v[4] > 100.0;
+  }
+  double vmin = __builtin_inf(), vmax = -__builtin_inf();
+  int nunder = 0, novv = 0;
+  for (int i = t; i < P.n; i += 256) {
+    const double vm = d.vm[i];
+    vmin = vm < vmin ? vm : vmin;
+    vmax = vm > vmax ? vm : vmax;
+    const double lo = isnan(P.bus_lo[i]) ? 0.94 : P.bus_lo[i];
+    const double hi = isnan(P.bus_hi[i]) ? 1.06 : P.bus_hi[i];
+    if (vm < lo) nunder++;
+    else if (vm > hi) novv++;
+  }
+  s_max[t] = lmax; s_arg[t] = larg; s_over[t] = nover; s_under[t] = nunder; s_overv[t] = novv;
+  s_vmin[t] = vmin; s_vmax[t] = vmax;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) {
+      // the largest loading, the lowest branch index among equal ones
+      const bool take = s_max[t + w] > s_max[t] || (s_max[t + w] == s_max[t] && s_arg[t + w] >= 0 &&
+                                                    (s_arg[t] < 0 || s_arg[t + w] < s_arg[t]));
+      if (take) { s_max[t] = s_max[t + w]; s_arg[t] = s_arg[t + w]; }
+      s_over[t] += s_over[t + w]; s_under[t] += s_under[t + w]; s_overv[t] += s_overv[t + w];
+      s_vmin[t] = s_vmin[t + w] < s_vmin[t] ? s_vmin[t + w] : s_vmin[t];
+      s_vmax[t] = s_vmax[t + w] > s_vmax[t] ? s_vmax[t + w] : s_vmax[t];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    double lp = 0.0, lq = 0.0;
+    for (int i = 0; i < P.n; i++) { lp += d.p[i] * base; lq += d.q[i] * base; }
+    gs_screen_summary o;
+    o.max_loading_pct = s_max[0];
+    o.max_loading_branch = s_max[0] > 0.0 ? s_arg[0] : -1;
+    o.overloads = s_over[0];
+    o.vmin = s_vmin[0];
+    o.vmax = s_vmax[0];
+    o.undervoltage = s_under[0];
+    o.overvoltage = s_overv[0];
+    o.loss_mw = lp;
+    o.loss_mvar = lq;
+    out[blockIdx.x] = o;
+  }
+}
+
 // ------------------------------------------------------------------------------------------------ seam solves
 // One CSR system on a plan made from that system's own factorisation: the numeric refactor replays the reference's
 // arithmetic (SparseLuFactorization.Refactor), refusing exactly where it would (a pivot that moves, or one at or below
@@ -983,6 +1083,12 @@ hipError_t launch_wide_step(int lu_fp32, const InstDesc* descs, const DevPlan& p
   else
     enqueue_wide_step<double>(descs, plan, count, prm, ctl, level_chunks, fs_level_pass, bs_level_pass, ws_bytes,
                               stream);
+  return hipGetLastError();
+}
+
+hipError_t launch_screen(const InstDesc* descs, const DevPlan& plan, int count, gs_screen_summary* out,
+                         double* loading, double* flows, hipStream_t stream) {
+  hipLaunchKernelGGL(screen_kernel, dim3(count), dim3(256), 0, stream, descs, plan, out, loading, flows);
   return hipGetLastError();
 }
 

@@ -48,7 +48,8 @@ class GsNetwork(C.Structure):
                 ("br_from", C.c_void_p), ("br_to", C.c_void_p), ("br_r", C.c_void_p), ("br_x", C.c_void_p),
                 ("br_b", C.c_void_p), ("br_tap", C.c_void_p), ("br_shift_deg", C.c_void_p),
                 ("br_in_service", C.c_void_p), ("gen_bus", C.c_void_p), ("gen_pg", C.c_void_p),
-                ("gen_qg", C.c_void_p), ("gen_vg", C.c_void_p)]
+                ("gen_qg", C.c_void_p), ("gen_vg", C.c_void_p), ("br_rate_a", C.c_void_p),
+                ("bus_vmin", C.c_void_p), ("bus_vmax", C.c_void_p)]
 
 
 class GsNrOptions(C.Structure):
@@ -66,6 +67,9 @@ RESULT_DTYPE = np.dtype([("status", np.int32), ("iterations", np.int32), ("max_m
                          ("symbolic_refreshes", np.int32), ("step_failures", np.int32),
                          ("min_pivot_ratio", np.float64), ("phase_us", np.float64, (4,))])
 PHASES = ("mismatch+jacobian", "refactor", "solve+refine", "update")
+SUMMARY_DTYPE = np.dtype([("max_loading_pct", np.float64), ("max_loading_branch", np.int32), ("overloads", np.int32),
+                          ("vmin", np.float64), ("vmax", np.float64), ("undervoltage", np.int32),
+                          ("overvoltage", np.int32), ("loss_mw", np.float64), ("loss_mvar", np.float64)])
 
 
 class GsCsr(C.Structure):
@@ -116,7 +120,7 @@ def lib():
                      "gs_plan_sizes", "gs_plan_get", "gs_batch_create", "gs_batch_run", "gs_batch_results",
                      "gs_nr_autotune", "gs_network_plan_stats", "gs_plan_shape", "gs_server_create",
                      "gs_server_add_network", "gs_server_submit", "gs_server_wait", "gs_server_stats_get",
-                     "gs_server_stats_reset"):
+                     "gs_server_stats_reset", "gs_nr_screen_batch"):
             getattr(L, name).restype = C.c_int32
         L.gs_host_sincos.restype = None
         L.gs_server_free.restype = None
@@ -140,7 +144,7 @@ class NetView:
                            _p(net.qd), _p(net.gs), _p(net.bs), _p(net.vm), _p(net.va_deg), _p(net.br_from),
                            _p(net.br_to), _p(net.br_r), _p(net.br_x), _p(net.br_b), _p(net.br_tap),
                            _p(net.br_shift_deg), _p(net.br_in_service), _p(net.gen_bus), _p(net.gen_pg),
-                           _p(net.gen_qg), _p(net.gen_vg))
+                           _p(net.gen_qg), _p(net.gen_vg), _p(net.br_rate_a), _p(net.bus_vmin), _p(net.bus_vmax))
 
     @property
     def ref(self):
@@ -394,6 +398,26 @@ class DeviceNetwork:
         tot, ker, nl = C.c_double(), C.c_double(), C.c_int32()
         lib().gs_last_timing(self.h, C.byref(tot), C.byref(ker), C.byref(nl))
         return BatchResult(res, vm, va, ker.value, tot.value, nl.value)
+
+    def screen(self, outages=None, load_scale=None, options: NrOptions | None = None, loading=False, flows=False,
+               keep_state=False, starts=None, count=None):
+        """gs_nr_screen_batch: the solve and, on the device, line flows and the classification numbers of every
+        instance. Returns (BatchResult, summaries [SUMMARY_DTYPE], loading [count, nb] or None, flows [count, nb, 4]
+        or None)."""
+        opts = options or self.options
+        arr, count, keep = self._instances(outages, load_scale, count, starts)
+        res = np.zeros(count, RESULT_DTYPE)
+        summ = np.zeros(count, SUMMARY_DTYPE)
+        nb, n = self.net.n_branch, self.net.n_bus
+        ld = np.empty((count, nb)) if loading else None
+        fl = np.empty((count, nb, 4)) if flows else None
+        vm = np.empty((count, n)) if keep_state else None
+        va = np.empty((count, n)) if keep_state else None
+        self.ctx.check(lib().gs_nr_screen_batch(self.h, C.c_int32(count), arr, C.byref(opts.c()), _p(res), _p(summ),
+                                                _p(ld), _p(fl), _p(vm), _p(va)), "gs_nr_screen_batch")
+        tot, ker, nl = C.c_double(), C.c_double(), C.c_int32()
+        lib().gs_last_timing(self.h, C.byref(tot), C.byref(ker), C.byref(nl))
+        return BatchResult(res, vm, va, ker.value, tot.value, nl.value), summ, ld, fl
 
     def close(self):
         if getattr(self, "h", None):

@@ -54,10 +54,10 @@ def read_case(path: str) -> Network:
     ratio = br[:, 8]
     net = Network(
         base_mva=base, bus_type=bus_type, pd=bus[:, 2], qd=bus[:, 3], gs=bus[:, 4], bs=bus[:, 5],
-        vm=bus[:, 7], va_deg=bus[:, 8],
+        vm=bus[:, 7], va_deg=bus[:, 8], bus_vmax=bus[:, 11], bus_vmin=bus[:, 12],
         br_from=np.array([index[int(b)] for b in br[:, 0]]), br_to=np.array([index[int(b)] for b in br[:, 1]]),
         br_r=br[:, 2], br_x=br[:, 3], br_b=br[:, 4], br_tap=np.where(ratio != 0.0, ratio, 1.0),
-        br_shift_deg=br[:, 9], br_in_service=np.ones(len(br), dtype=np.uint8),
+        br_shift_deg=br[:, 9], br_in_service=np.ones(len(br), dtype=np.uint8), br_rate_a=br[:, 5],
         gen_bus=gen_bus, gen_pg=gen[:, 1], gen_qg=gen[:, 2], gen_vg=gen[:, 5], gen_qmin=gen[:, 4],
         gen_qmax=gen[:, 3], name=re.search(r"function\s+mpc\s*=\s*(\w+)", text).group(1), bus_ids=ids)
     return net

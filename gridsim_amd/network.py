@@ -38,6 +38,9 @@ class Network:
     gen_pg: np.ndarray        # MW
     gen_qg: np.ndarray        # MVAr
     gen_vg: np.ndarray        # pu
+    br_rate_a: np.ndarray = field(default=None)   # MVA, 0 = unrated (Branch.RateAMva)
+    bus_vmin: np.ndarray = field(default=None)    # pu, NaN = no limit of its own (Bus.VminPu)
+    bus_vmax: np.ndarray = field(default=None)    # pu, NaN = no limit of its own (Bus.VmaxPu)
     gen_qmin: np.ndarray = field(default=None)
     gen_qmax: np.ndarray = field(default=None)
     name: str = "network"
@@ -53,6 +56,14 @@ class Network:
         for k in f64:
             setattr(self, k, np.ascontiguousarray(getattr(self, k), dtype=np.float64))
         self.br_in_service = np.ascontiguousarray(self.br_in_service, dtype=np.uint8)
+        if self.br_rate_a is None:
+            self.br_rate_a = np.zeros(len(self.br_from))
+        self.br_rate_a = np.ascontiguousarray(self.br_rate_a, dtype=np.float64)
+        nbus = len(self.bus_type)
+        self.bus_vmin = np.ascontiguousarray(np.full(nbus, np.nan) if self.bus_vmin is None else self.bus_vmin,
+                                             dtype=np.float64)
+        self.bus_vmax = np.ascontiguousarray(np.full(nbus, np.nan) if self.bus_vmax is None else self.bus_vmax,
+                                             dtype=np.float64)
         ng = len(self.gen_bus)
         if self.gen_qmin is None:
             self.gen_qmin = np.full(ng, -np.inf)
@@ -88,7 +99,8 @@ class Network:
         keep[np.asarray(idx, dtype=np.int64)] = False
         return self.copy(br_from=self.br_from[keep], br_to=self.br_to[keep], br_r=self.br_r[keep],
                          br_x=self.br_x[keep], br_b=self.br_b[keep], br_tap=self.br_tap[keep],
-                         br_shift_deg=self.br_shift_deg[keep], br_in_service=self.br_in_service[keep])
+                         br_shift_deg=self.br_shift_deg[keep], br_in_service=self.br_in_service[keep],
+                         br_rate_a=self.br_rate_a[keep])
 
     def with_load_multiplier(self, mult: float) -> "Network":
         """Scale every load and every generator's P by ``mult`` (a load-variation instance)."""

@@ -63,6 +63,10 @@ typedef struct gs_network {
   const uint8_t* br_in_service;
   const int32_t* gen_bus;
   const double *gen_pg, *gen_qg, *gen_vg;
+  /* For screening (may be null): branch rate A in MVA (0 = unrated; Branch.RateAMva), and each bus's own voltage
+   * limits in pu (NaN = none: the operational band 0.94 / 1.06 applies on that side; Bus.VminPu / VmaxPu). */
+  const double* br_rate_a;
+  const double *bus_vmin, *bus_vmax;
 } gs_network;
 
 /* PowerFlowOptions (src/GridSim.Core/PowerFlow/PowerFlowOptions.cs:6-170) for the engine. Q limits and the
@@ -163,6 +167,25 @@ GS_API void gs_network_free(gs_net* net);
  * vm_out / va_out (radians), when non-null, receive count * n_bus values. */
 GS_API gs_status gs_nr_solve_batch(gs_net* net, int32_t count, const gs_instance* inst, const gs_nr_options* opts,
                                    gs_nr_result* results, double* vm_out, double* va_out);
+
+/* ---- screening: line flows and the contingency classification of solved states ---------------------------------
+ * gs_nr_solve_batch followed, on the device, by NewtonShell.ComputeLineFlows (NewtonShell.cs:699-734) over every
+ * instance's solved voltages and the numbers ContingencyAnalysis.ScreenStateCore classifies a converged outage by
+ * (ContingencyAnalysis.cs:1150-1236), with AssembleResult's losses (NewtonShell.cs:649-690). One summary per instance
+ * (meaningful for converged instances); loading (count * n_branch, percent of rate A) and flows (count * n_branch * 4:
+ * P_from, Q_from, P_to, Q_to in MW / MVAr) when non-null, index-aligned with the network's branches (the outaged and
+ * out-of-service branches carry 0). vm_out / va_out as gs_nr_solve_batch. */
+typedef struct gs_screen_summary {
+  double max_loading_pct;       /* largest branch loading, percent of rate A (0: no rated branch) */
+  int32_t max_loading_branch;   /* its branch index, or -1 */
+  int32_t overloads;            /* branches above 100 % */
+  double vmin, vmax;            /* pu */
+  int32_t undervoltage, overvoltage;  /* buses below / above their limits */
+  double loss_mw, loss_mvar;    /* sums of the bus injections (PowerFlowResult.TotalLossMw / TotalLossMvar) */
+} gs_screen_summary;
+GS_API gs_status gs_nr_screen_batch(gs_net* net, int32_t count, const gs_instance* inst, const gs_nr_options* opts,
+                                   gs_nr_result* results, gs_screen_summary* summaries, double* loading,
+                                   double* flows, double* vm_out, double* va_out);
 
 /* The plain path: one instance per launch, FP64 LU, the host driving each Newton iteration (the baseline the batched
  * engine is measured against). Same arguments as gs_nr_solve_batch; opts->lu_precision is ignored. */

@@ -69,6 +69,19 @@ inline Cx cx_div_real(Cx l, double r) {
   return {l.re / r, l.im / r};
 }
 
+// Complex * Complex: (ac - bd) + (bc + ad) i; Complex * double: component-wise.
+inline Cx cx_mul(Cx l, Cx r) { return {l.re * r.re - l.im * r.im, l.im * r.re + l.re * r.im}; }
+inline Cx cx_scale(Cx l, double d) { return {l.re * d, l.im * d}; }
+// Complex.Magnitude: the scaled hypotenuse |large| sqrt(1 + (small / large)^2).
+inline double cx_abs(Cx z) {
+  double a = std::fabs(z.re), b = std::fabs(z.im);
+  double small = a < b ? a : b, large = a < b ? b : a;
+  if (small == 0.0) return large;
+  if (std::isinf(large) && !std::isnan(small)) return std::numeric_limits<double>::infinity();
+  double ratio = small / large;
+  return large * std::sqrt(1.0 + ratio * ratio);
+}
+
 struct Stamp { Cx yff, ytt, yft, ytf; };
 
 // AdmittanceStamp.Branch + FromAdmittance (AdmittanceStamp.cs:41-75) at h = 1, rFactor = 1.
@@ -998,6 +1011,68 @@ int32_t orc_jacobian(const orc_network* net, int32_t skip, const double* vm, con
   if (rowptr) std::memcpy(rowptr, jac.rp.data(), sizeof(int32_t) * (m + 1));
   if (colidx) std::memcpy(colidx, jac.ci.data(), sizeof(int32_t) * jac.ci.size());
   return (int32_t)jac.ci.size();
+}
+
+// NewtonShell.ComputeLineFlows (NewtonShell.cs:699-734): per branch the from / to end complex powers in MVA and the
+// loading against rate A (percent, 0 when unrated). Index-aligned with the model's branches; an out-of-service branch
+// and the outaged one (skip) carry no flow. pf, qf, pt, qt, loading: [n_branch] each (any may be null).
+void orc_line_flows(const orc_network* net, int32_t skip, const double* rate_a, const double* vm, const double* va,
+                    double* pf, double* qf, double* pt, double* qt, double* loading) {
+  for (int k = 0; k < net->n_branch; k++) {
+    double v[5] = {0, 0, 0, 0, 0};
+    if (net->br_in_service[k] && k != skip) {
+      const int f = net->br_from[k], t = net->br_to[k];
+      double sf, cf, st, ct;
+      orc_fdlibm::sincos(va[f], &sf, &cf);
+      orc_fdlibm::sincos(va[t], &st, &ct);
+      Cx vf = {vm[f] * cf, vm[f] * sf}, vt = {vm[t] * ct, vm[t] * st};  // DetMath.FromPolar
+      Stamp s = branch_stamp(net->br_r[k], net->br_x[k], net->br_b[k], net->br_tap[k], net->br_shift_deg[k]);
+      Cx i_f = cx_add(cx_mul(s.yff, vf), cx_mul(s.yft, vt));
+      Cx i_t = cx_add(cx_mul(s.ytf, vf), cx_mul(s.ytt, vt));
+      Cx s_f = cx_scale(cx_mul(vf, cx_conj(i_f)), net->base_mva);
+      Cx s_t = cx_scale(cx_mul(vt, cx_conj(i_t)), net->base_mva);
+      const double mva = net_max(cx_abs(s_f), cx_abs(s_t));
+      const double rate = rate_a ? rate_a[k] : 0.0;
+      v[0] = s_f.re; v[1] = s_f.im; v[2] = s_t.re; v[3] = s_t.im;
+      v[4] = rate > 0 ? mva / rate * 100.0 : 0.0;
+    }
+    if (pf) pf[k] = v[0];
+    if (qf) qf[k] = v[1];
+    if (pt) pt[k] = v[2];
+    if (qt) qt[k] = v[3];
+    if (loading) loading[k] = v[4];
+  }
+}
+
+// The numbers ContingencyAnalysis.ScreenStateCore classifies a converged outage by (ContingencyAnalysis.cs:1150-1236)
+// and AssembleResult's losses (NewtonShell.cs:649-690): out[0] max branch loading (percent), out[1] branches above
+// 100 %, out[2] min Vm, out[3] max Vm, out[4] buses below their low limit, out[5] buses above their high limit,
+// out[6] total loss MW (sum of the bus P injections), out[7] total loss MVAr. p, q: the per-unit injections at the
+// solved state. bus_low / bus_high: each bus's own limits (VoltageLimits.OperationalLowFor / HighFor: NaN = none, the
+// operational band 0.94 / 1.06 pu applies on that side), or null for the band everywhere.
+void orc_screen_summary(const orc_network* net, int32_t skip, const double* rate_a, const double* bus_low,
+                        const double* bus_high, const double* vm, const double* va, const double* p, const double* q,
+                        double* out) {
+  vector<double> loading(net->n_branch);
+  orc_line_flows(net, skip, rate_a, vm, va, nullptr, nullptr, nullptr, nullptr, loading.data());
+  double max_load = 0, over = 0;
+  for (int k = 0; k < net->n_branch; k++) {
+    if (loading[k] > max_load) max_load = loading[k];
+    if (loading[k] > 100.0) over += 1;
+  }
+  double vmin = vm[0], vmax = vm[0], under_n = 0, over_n = 0, lp = 0, lq = 0;
+  for (int i = 0; i < net->n_bus; i++) {
+    vmin = vm[i] < vmin ? vm[i] : vmin;
+    vmax = vm[i] > vmax ? vm[i] : vmax;
+    const double lo = bus_low && !std::isnan(bus_low[i]) ? bus_low[i] : 0.94;
+    const double hi = bus_high && !std::isnan(bus_high[i]) ? bus_high[i] : 1.06;
+    if (vm[i] < lo) under_n += 1;
+    else if (vm[i] > hi) over_n += 1;
+    lp += p[i] * net->base_mva;
+    lq += q[i] * net->base_mva;
+  }
+  out[0] = max_load; out[1] = over; out[2] = vmin; out[3] = vmax; out[4] = under_n; out[5] = over_n;
+  out[6] = lp; out[7] = lq;
 }
 
 void orc_md_order(int32_t n, const int32_t* rowptr, const int32_t* colidx, int32_t* q) {

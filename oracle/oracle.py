@@ -242,3 +242,32 @@ def screen(net, outages, opts=None, threads=1, keep_state=True):
     lib().orc_screen(_NetView(net).ref, _p(outages), C.c_int32(count), C.byref(opts), C.c_int32(threads),
                      _p(res), _p(vm), _p(va))
     return res, vm, va
+
+
+def line_flows(net, vm, va, skip=-1):
+    """NewtonShell.ComputeLineFlows: (pf, qf, pt, qt in MW / MVAr, loading percent of rate A), per branch."""
+    nb = net.n_branch
+    out = [np.zeros(nb) for _ in range(5)]
+    rate = np.ascontiguousarray(net.br_rate_a, np.float64)
+    lib().orc_line_flows(_NetView(net).ref, int(skip), _p(rate), _p(np.ascontiguousarray(vm, np.float64)),
+                         _p(np.ascontiguousarray(va, np.float64)), *(_p(o) for o in out))
+    return tuple(out)
+
+
+SUMMARY_FIELDS = ("max_loading_pct", "overloads", "vmin", "vmax", "undervoltage", "overvoltage", "loss_mw",
+                  "loss_mvar")
+
+
+def screen_summary(net, vm, va, skip=-1):
+    """The classification numbers of a converged state (ScreenStateCore, each bus against its own limits where the
+    network gives them, as the CLI's default) and AssembleResult's losses, as a dict."""
+    vm = np.ascontiguousarray(vm, np.float64)
+    va = np.ascontiguousarray(va, np.float64)
+    p, q, _ = mismatch(net, vm, va, skip=skip)
+    out = np.zeros(8)
+    rate = np.ascontiguousarray(net.br_rate_a, np.float64)
+    lo = np.ascontiguousarray(net.bus_vmin, np.float64)
+    hi = np.ascontiguousarray(net.bus_vmax, np.float64)
+    lib().orc_screen_summary(_NetView(net).ref, int(skip), _p(rate), _p(lo), _p(hi), _p(vm), _p(va),
+                             _p(np.ascontiguousarray(p)), _p(np.ascontiguousarray(q)), _p(out))
+    return dict(zip(SUMMARY_FIELDS, out.tolist()))

@@ -286,3 +286,62 @@ def test_level_parallel_flat_start(ctx, eng, oracle):
     assert r.results["iterations"][0] == rec["iterations"]
     assert np.abs(r.vm[0] - vm).max() <= TOL_DV and np.abs(r.va[0] - va).max() <= TOL_DV
     dn.close()
+
+
+def _check_screen_against_oracle(oracle, net, outs, res, summ, ld, fl, skip_label):
+    for j, k in enumerate(outs):
+        if res.status[j] != 0:
+            continue
+        ref = oracle.screen_summary(net, res.vm[j], res.va[j], skip=int(k))
+        pf, qf, pt, qt, load = oracle.line_flows(net, res.vm[j], res.va[j], skip=int(k))
+        s = summ[j]
+        assert abs(s["max_loading_pct"] - ref["max_loading_pct"]) <= 1e-9 * max(1.0, ref["max_loading_pct"]), skip_label
+        assert s["overloads"] == ref["overloads"] and s["undervoltage"] == ref["undervoltage"], skip_label
+        assert s["overvoltage"] == ref["overvoltage"], skip_label
+        assert s["vmin"] == res.vm[j].min() and s["vmax"] == res.vm[j].max()
+        assert abs(s["loss_mw"] - ref["loss_mw"]) <= 1e-9 * max(1.0, abs(ref["loss_mw"]))
+        assert abs(s["loss_mvar"] - ref["loss_mvar"]) <= 1e-9 * max(1.0, abs(ref["loss_mvar"]))
+        if ref["max_loading_pct"] > 0:
+            assert s["max_loading_branch"] == int(np.argmax(load))
+        scale = max(1.0, np.abs(fl[j]).max())
+        np.testing.assert_allclose(fl[j], np.stack([pf, qf, pt, qt], axis=1), rtol=0, atol=1e-12 * scale)
+        np.testing.assert_allclose(ld[j], load, rtol=0, atol=1e-12 * max(1.0, load.max()))
+
+
+def test_screen_case9_matches_oracle_and_the_reference_cli_fixture(ctx, eng, oracle):
+    """gs_nr_screen_batch on every case9 branch outage (the bridges island and do not converge): the device's line
+    flows, loadings and classification numbers equal the oracle's (computed from the same voltages), and printed as
+    the reference's `gridsim nminus1` prints them they reproduce tests/golden/cli/nminus1-case9.txt."""
+    from test_oracle import _parse_nminus1
+    net = matpower_case("case9")
+    o = eng.NrOptions(max_iterations=100, lu_precision=eng.LU_FP64, refine_steps=0)
+    dn = eng.DeviceNetwork(ctx, net, o)
+    outs = np.arange(net.n_branch)
+    res, summ, ld, fl = dn.screen(outages=outs, loading=True, flows=True, keep_state=True)
+    _check_screen_against_oracle(oracle, net, outs, res, summ, ld, fl, "case9")
+    ids = list(net.bus_ids)
+    for label, status, maxload, vmin, vmax in _parse_nminus1(os.path.join(GOLDEN, "cli", "nminus1-case9.txt")):
+        a, b = (int(x) for x in label.split("-"))
+        k = [i for i in range(net.n_branch) if (ids[net.br_from[i]], ids[net.br_to[i]]) == (a, b)][0]
+        if status == "ISLANDED":
+            assert res.status[k] != eng.CONVERGED, label
+            continue
+        s = summ[k]
+        assert res.status[k] == eng.CONVERGED
+        assert (f"{s['max_loading_pct']:.0f}%", f"{s['vmin']:.3f}", f"{s['vmax']:.3f}") == (maxload, vmin, vmax), label
+    dn.close()
+
+
+def test_screen_synthetic_n1_matches_oracle(ctx, eng, oracle, n1_118):
+    """The 118-bus N-1 screen with ratings at 1.05 x each branch's base-case MVA (so outages overload some branches):
+    device summaries, loadings and flows against the oracle's on the same voltages."""
+    warm, outs, _ = n1_118
+    rec, vm, va = oracle.solve(warm)
+    pf, qf, pt, qt, _ = oracle.line_flows(warm, vm, va)
+    rated = warm.copy(br_rate_a=1.05 * np.maximum(np.hypot(pf, qf), np.hypot(pt, qt)) + 1e-3)
+    dn = eng.DeviceNetwork(ctx, rated, eng.NrOptions(max_iterations=10))
+    res, summ, ld, fl = dn.screen(outages=outs, loading=True, flows=True, keep_state=True)
+    conv = res.status == 0
+    assert conv.sum() > 0 and summ["overloads"][conv].sum() > 0
+    _check_screen_against_oracle(oracle, rated, outs, res, summ, ld, fl, "118")
+    dn.close()

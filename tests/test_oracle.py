@@ -466,3 +466,68 @@ def test_n1_screen_threads_agree_and_match_single_solves(oracle):
         rr, vmr, var = oracle.solve(warm.without_branches([outs[k]]), opts)
         np.testing.assert_array_equal(vmr, vmk)
         np.testing.assert_array_equal(var, vak)
+
+
+# ---------------------------------------------------------------- line flows and the contingency classification
+
+
+def _parse_nminus1(path):
+    """Rows of the reference's `gridsim nminus1` CLI output: (outage label, status, max load %, vmin, vmax)."""
+    rows = []
+    for line in open(path, encoding="utf-8"):
+        parts = line.split()
+        if len(parts) < 2 or "-" not in parts[0] or not parts[0][0].isdigit():
+            continue
+        if parts[1] in ("ISLANDED", "NO"):
+            rows.append((parts[0], parts[1], None, None, None))
+        else:
+            rows.append((parts[0], parts[1], parts[2], parts[4], parts[5]))
+    return rows
+
+
+def test_case9_line_flows_and_losses(oracle):
+    """ComputeLineFlows and AssembleResult on MATPOWER's case9 solution: the published losses (4.641 MW,
+    PowerFlowTests.cs:10-55) are the sum of the bus injections and equal the sum over branches of S_from + S_to (case9
+    has no bus shunts); each branch's end powers satisfy the pi-model identities."""
+    net = matpower_case("case9")
+    rec, vm, va = oracle.solve(net)
+    pf, qf, pt, qt, load = oracle.line_flows(net, vm, va)
+    summ = oracle.screen_summary(net, vm, va)
+    assert abs(summ["loss_mw"] - 4.641) < 0.005
+    assert abs((pf + pt).sum() - summ["loss_mw"]) < 1e-6
+    assert abs((qf + qt).sum() - summ["loss_mvar"]) < 1e-6
+    # the generator branches carry their unit's output: bus 1's 71.64 MW over branch 1-4, into bus 4
+    assert abs(pf[0] - 71.64) < 0.05 and abs(pt[0] + 71.64) < 0.05
+    # loading against rate A: max(|S_from|, |S_to|) / rate
+    k = 2
+    assert abs(load[k] - max(np.hypot(pf[k], qf[k]), np.hypot(pt[k], qt[k])) / net.br_rate_a[k] * 100) < 1e-9
+
+
+def test_nminus1_case9_matches_the_reference_cli_fixture(oracle):
+    """`gridsim nminus1 case9` (tests/GridSim.Tests/fixtures/cli-stdout/nminus1-case9.txt): per outage the verdict, the
+    maximum branch loading (percent of rate A, printed to 1 %) and Vmin / Vmax (printed to 3 decimals). The reference
+    solves with its study options (100 iterations, reactive limits on; no case9 generator reaches +-300 MVAr here, so
+    the unlimited solve is the same state) and judges each bus against its own VMIN / VMAX (the CLI default,
+    Program.cs:1495)."""
+    from gridsim_amd.network import bridges
+    net = matpower_case("case9")
+    ids = list(net.bus_ids)
+    rows = _parse_nminus1(os.path.join(GOLDEN, "cli", "nminus1-case9.txt"))
+    assert len(rows) == 9
+    split = bridges(net)
+    for label, status, maxload, vmin, vmax in rows:
+        a, b = (int(x) for x in label.split("-"))
+        k = [i for i in range(net.n_branch) if (ids[net.br_from[i]], ids[net.br_to[i]]) == (a, b)][0]
+        rec, vm, va = oracle.solve(net, oracle.options(max_iterations=100), skip=k)
+        if status == "ISLANDED":
+            assert split[k] and rec["status"] != oracle.CONVERGED, label
+            continue
+        assert rec["status"] == oracle.CONVERGED, label
+        s = oracle.screen_summary(net, vm, va, skip=k)
+        assert f"{s['max_loading_pct']:.0f}%" == maxload, (label, s)
+        assert f"{s['vmin']:.3f}" == vmin and f"{s['vmax']:.3f}" == vmax, (label, s)
+        # one line per overloaded branch, one for any bus under its limit, one for any bus over it
+        violations = int(s["overloads"]) + (s["undervoltage"] > 0) + (s["overvoltage"] > 0)
+        assert (status == "secure") == (violations == 0), (label, s)
+        if status.startswith("VIOLATIONx"):
+            assert violations == int(status[len("VIOLATIONx"):]), (label, s)
