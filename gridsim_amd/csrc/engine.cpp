@@ -948,7 +948,7 @@ GS_API gs_status gs_nr_solve_batch(gs_net* net, int32_t count, const gs_instance
 
 GS_API gs_status gs_nr_screen_batch(gs_net* net, int32_t count, const gs_instance* inst, const gs_nr_options* opts,
                                    gs_nr_result* results, gs_screen_summary* summaries, double* loading,
-                                   double* flows, double* vm_out, double* va_out) {
+                                   double* flows, double* vm_out, double* va_out, double* p_inj, double* q_inj) {
   if (!net) return GS_INVALID_ARGUMENT;
   return guarded(net->ctx, [&]() -> gs_status {
     if (count < 0 || (count > 0 && (!inst || !results || !summaries))) fail(GS_INVALID_ARGUMENT, "null argument");
@@ -989,7 +989,19 @@ GS_API gs_status gs_nr_screen_batch(gs_net* net, int32_t count, const gs_instanc
       check(hipMemcpyAsync(loading, net->loading_buf.p, 8 * (size_t)count * nb, hipMemcpyDeviceToHost, s), "loading");
     if (flows)
       check(hipMemcpyAsync(flows, net->flow_buf.p, 32 * (size_t)count * nb, hipMemcpyDeviceToHost, s), "flows");
+    // the injections at the final states (per unit on the device), scaled to MW / MVAr as AssembleResult does
+    for (int w = 0; w < 2; w++) {
+      double* dst = w == 0 ? p_inj : q_inj;
+      if (!dst) continue;
+      check(hipMemcpy2DAsync(dst, 8 * (size_t)n, net->inst_buf.as<char>() + (w == 0 ? L.p : L.q), L.stride,
+                             8 * (size_t)n, (size_t)count, hipMemcpyDeviceToHost, s), "injections");
+    }
     check(hipStreamSynchronize(s), "screen sync");
+    for (int w = 0; w < 2; w++) {
+      double* dst = w == 0 ? p_inj : q_inj;
+      if (dst)
+        for (size_t e = 0; e < (size_t)count * n; e++) dst[e] *= net->net.base;
+    }
     return GS_OK;
   });
 }

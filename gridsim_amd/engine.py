@@ -21,7 +21,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from .network import Network
+from .network import Network, bridges
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libgridsim_hip.so")
@@ -400,10 +400,10 @@ class DeviceNetwork:
         return BatchResult(res, vm, va, ker.value, tot.value, nl.value)
 
     def screen(self, outages=None, load_scale=None, options: NrOptions | None = None, loading=False, flows=False,
-               keep_state=False, starts=None, count=None):
+               keep_state=False, starts=None, count=None, injections=False):
         """gs_nr_screen_batch: the solve and, on the device, line flows and the classification numbers of every
         instance. Returns (BatchResult, summaries [SUMMARY_DTYPE], loading [count, nb] or None, flows [count, nb, 4]
-        or None)."""
+        or None); with injections, also (P, Q) [count, n] in MW / MVAr."""
         opts = options or self.options
         arr, count, keep = self._instances(outages, load_scale, count, starts)
         res = np.zeros(count, RESULT_DTYPE)
@@ -413,11 +413,15 @@ class DeviceNetwork:
         fl = np.empty((count, nb, 4)) if flows else None
         vm = np.empty((count, n)) if keep_state else None
         va = np.empty((count, n)) if keep_state else None
+        pi = np.empty((count, n)) if injections else None
+        qi = np.empty((count, n)) if injections else None
         self.ctx.check(lib().gs_nr_screen_batch(self.h, C.c_int32(count), arr, C.byref(opts.c()), _p(res), _p(summ),
-                                                _p(ld), _p(fl), _p(vm), _p(va)), "gs_nr_screen_batch")
+                                                _p(ld), _p(fl), _p(vm), _p(va), _p(pi), _p(qi)),
+                       "gs_nr_screen_batch")
         tot, ker, nl = C.c_double(), C.c_double(), C.c_int32()
         lib().gs_last_timing(self.h, C.byref(tot), C.byref(ker), C.byref(nl))
-        return BatchResult(res, vm, va, ker.value, tot.value, nl.value), summ, ld, fl
+        out = (BatchResult(res, vm, va, ker.value, tot.value, nl.value), summ, ld, fl)
+        return out + ((pi, qi),) if injections else out
 
     def close(self):
         if getattr(self, "h", None):
@@ -432,8 +436,24 @@ class DeviceNetwork:
 
 
 @dataclass
+class LineFlow:
+    """One branch's flows (LineFlow, NewtonShell.ComputeLineFlows :699-734): MW / MVAr at each end, the branch losses
+    and the loading against rate A (percent, 0 when unrated)."""
+    from_bus: int
+    to_bus: int
+    p_from_mw: float
+    q_from_mvar: float
+    p_to_mw: float
+    q_to_mvar: float
+    loss_mw: float
+    loss_mvar: float
+    rate_a_mva: float
+    loading_pct: float
+
+
+@dataclass
 class PowerFlowResult:
-    """The fields of the reference's PowerFlowResult (PowerFlowResult.cs:5-21) the engine reports."""
+    """The fields of the reference's PowerFlowResult (PowerFlowResult.cs:5-21, NewtonShell.AssembleResult :649-690)."""
     status: int
     converged: bool
     iterations: int
@@ -441,10 +461,16 @@ class PowerFlowResult:
     vm: np.ndarray
     va_deg: np.ndarray
     symbolic_refreshes: int
+    p_inj_mw: np.ndarray | None = None
+    q_inj_mvar: np.ndarray | None = None
+    total_loss_mw: float = 0.0
+    total_loss_mvar: float = 0.0
+    line_flows: list | None = None
 
 
 class HipNewtonPowerFlow:
-    """IPowerFlowSolver on the GPU: SparseNewtonRaphsonPowerFlow.Solve with Q limits and the DC retry off."""
+    """IPowerFlowSolver on the GPU: SparseNewtonRaphsonPowerFlow.Solve with Q limits and the DC retry off, and the
+    result assembled as NewtonShell.AssembleResult does (line flows and losses computed on the device)."""
 
     name = "newton-raphson (hip, gfx950)"
 
@@ -456,28 +482,83 @@ class HipNewtonPowerFlow:
         opts = options or self.options
         dn = DeviceNetwork(self.ctx, net, opts)
         try:
-            r = dn.solve(count=1, options=opts)
+            r, summ, ld, fl, (p_inj, q_inj) = dn.screen(count=1, options=opts, loading=True, flows=True,
+                                                        keep_state=True, injections=True)
         finally:
             dn.close()
         rec = r.results[0]
+        ids = net.bus_ids if net.bus_ids is not None else np.arange(net.n_bus)
+        flows = [LineFlow(int(ids[net.br_from[k]]), int(ids[net.br_to[k]]), *(float(v) for v in fl[0, k]),
+                          float(fl[0, k, 0] + fl[0, k, 2]), float(fl[0, k, 1] + fl[0, k, 3]),
+                          float(net.br_rate_a[k]), float(ld[0, k])) for k in range(net.n_branch)]
         return PowerFlowResult(int(rec["status"]), int(rec["status"]) == CONVERGED, int(rec["iterations"]),
                                float(rec["max_mismatch"]), r.vm[0], np.degrees(r.va[0]),
-                               int(rec["symbolic_refreshes"]))
+                               int(rec["symbolic_refreshes"]), p_inj[0], q_inj[0], float(summ[0]["loss_mw"]),
+                               float(summ[0]["loss_mvar"]), flows)
+
+
+OVERLOAD_THRESHOLD_PCT = 100.0   # ContingencyAnalysis.OverloadThresholdPercent
+
+
+@dataclass
+class ContingencyResult:
+    """ContingencyAnalysis's ContingencyResult (ContingencyAnalysis.cs:9-135) for a branch outage: the verdict, the
+    maximum branch loading, Vmin / Vmax and the violation lines ScreenStateCore writes (:1150-1236)."""
+    outage: str
+    converged: bool
+    max_branch_loading_pct: float
+    min_vm: float
+    max_vm: float
+    violations: list
+    islanded: bool = False
+
+    @property
+    def secure(self) -> bool:
+        return self.converged and not self.violations
+
+
+def branch_label(net: Network, k: int) -> str:
+    """A branch named by its end buses, "from-to" (ScreenStateCore's label for an unnamed branch)."""
+    ids = net.bus_ids if net.bus_ids is not None else np.arange(net.n_bus)
+    return f"{int(ids[net.br_from[k]])}-{int(ids[net.br_to[k]])}"
+
+
+def classify(net: Network, outage: int, converged: bool, summary, loading_row) -> ContingencyResult:
+    """One screened outage as ScreenStateCore classifies it: each branch above 100 % of rate A, then one line for
+    the buses under their limits (the lowest voltage) and one for those over them (the highest). A non-converged
+    outage reports whether it split the network (ContingencyAnalysis.NonConverged / Splits)."""
+    label = branch_label(net, outage)
+    if not converged:
+        split = bool(bridges(net)[outage]) if net.br_in_service[outage] else False
+        return ContingencyResult(label, False, 0.0, 0.0, 0.0, ["islanded" if split else "no solution"], split)
+    violations = []
+    if summary["overloads"] > 0:
+        for k in np.nonzero(loading_row > OVERLOAD_THRESHOLD_PCT)[0]:
+            violations.append(f"branch {branch_label(net, int(k))} {loading_row[k]:.0f}%")
+    if summary["undervoltage"] > 0:
+        violations.append(f"undervoltage {summary['vmin']:.3f} p.u.")
+    if summary["overvoltage"] > 0:
+        violations.append(f"overvoltage {summary['vmax']:.3f} p.u.")
+    return ContingencyResult(label, True, float(summary["max_loading_pct"]), float(summary["vmin"]),
+                             float(summary["vmax"]), violations)
 
 
 def screen_branches_batched(ctx: Context, net: Network, branch_indices=None, options: NrOptions | None = None,
-                            keep_state=False) -> BatchResult:
-    """N-1 screen of every in-service branch (or the given ones) in one batched launch
-    (ContingencyAnalysis.ScreenBranchesBatched, ContingencyAnalysis.Batched.cs:55-136)."""
+                            keep_state=False):
+    """N-1 screen of every in-service branch (or the given ones) in one batched launch with the classification on the
+    device (ContingencyAnalysis.ScreenBranchesBatched, ContingencyAnalysis.Batched.cs:55-136). Returns
+    (BatchResult, [ContingencyResult])."""
     idx = np.nonzero(net.br_in_service)[0] if branch_indices is None else np.asarray(branch_indices)
     for i in idx:
         if not net.br_in_service[int(i)]:
             raise ValueError(f"branch {int(i)} is out of service: removing it is not an outage.")
     dn = DeviceNetwork(ctx, net, options)
     try:
-        return dn.solve(outages=idx, options=options, keep_state=keep_state)
+        res, summ, ld, _ = dn.screen(outages=idx, options=options, loading=True, keep_state=keep_state)
     finally:
         dn.close()
+    out = [classify(net, int(k), bool(res.status[j] == CONVERGED), summ[j], ld[j]) for j, k in enumerate(idx)]
+    return res, out
 
 
 # ---------------------------------------------------------------- multi-network batches and the autotuner

@@ -174,7 +174,8 @@ GS_API gs_status gs_nr_solve_batch(gs_net* net, int32_t count, const gs_instance
  * (ContingencyAnalysis.cs:1150-1236), with AssembleResult's losses (NewtonShell.cs:649-690). One summary per instance
  * (meaningful for converged instances); loading (count * n_branch, percent of rate A) and flows (count * n_branch * 4:
  * P_from, Q_from, P_to, Q_to in MW / MVAr) when non-null, index-aligned with the network's branches (the outaged and
- * out-of-service branches carry 0). vm_out / va_out as gs_nr_solve_batch. */
+ * out-of-service branches carry 0). vm_out / va_out as gs_nr_solve_batch; p_inj / q_inj (count * n_bus, MW / MVAr,
+ * PowerFlowResult.PInjMw / QInjMvar) when non-null. */
 typedef struct gs_screen_summary {
   double max_loading_pct;       /* largest branch loading, percent of rate A (0: no rated branch) */
   int32_t max_loading_branch;   /* its branch index, or -1 */
@@ -185,7 +186,7 @@ typedef struct gs_screen_summary {
 } gs_screen_summary;
 GS_API gs_status gs_nr_screen_batch(gs_net* net, int32_t count, const gs_instance* inst, const gs_nr_options* opts,
                                    gs_nr_result* results, gs_screen_summary* summaries, double* loading,
-                                   double* flows, double* vm_out, double* va_out);
+                                   double* flows, double* vm_out, double* va_out, double* p_inj, double* q_inj);
 
 /* The plain path: one instance per launch, FP64 LU, the host driving each Newton iteration (the baseline the batched
  * engine is measured against). Same arguments as gs_nr_solve_batch; opts->lu_precision is ignored. */

@@ -345,3 +345,29 @@ def test_screen_synthetic_n1_matches_oracle(ctx, eng, oracle, n1_118):
     assert conv.sum() > 0 and summ["overloads"][conv].sum() > 0
     _check_screen_against_oracle(oracle, rated, outs, res, summ, ld, fl, "118")
     dn.close()
+
+
+def test_assembled_result_and_contingency_records_case9(ctx, eng, oracle):
+    """HipNewtonPowerFlow assembles the reference's PowerFlowResult (injections, losses, line flows) from the device,
+    and screen_branches_batched's ContingencyResult records give the reference CLI's case9 verdicts."""
+    from test_oracle import _parse_nminus1
+    net = matpower_case("case9")
+    o = eng.NrOptions(lu_precision=eng.LU_FP64, refine_steps=0)
+    r = eng.HipNewtonPowerFlow(ctx, o).solve(net)
+    assert r.converged and abs(r.p_inj_mw[0] - 71.64) < 0.05 and abs(r.total_loss_mw - 4.641) < 0.005
+    assert abs(r.total_loss_mw - r.p_inj_mw.sum()) < 1e-9
+    pf, qf, pt, qt, load = oracle.line_flows(net, r.vm, np.radians(r.va_deg))
+    got = np.array([[f.p_from_mw, f.q_from_mvar, f.p_to_mw, f.q_to_mvar, f.loading_pct] for f in r.line_flows])
+    np.testing.assert_allclose(got, np.stack([pf, qf, pt, qt, load], axis=1), rtol=0, atol=1e-9)
+    res, recs = eng.screen_branches_batched(ctx, net, options=eng.NrOptions(max_iterations=100,
+                                                                            lu_precision=eng.LU_FP64, refine_steps=0))
+    by_label = {c.outage: c for c in recs}
+    for label, status, maxload, vmin, vmax in _parse_nminus1(os.path.join(GOLDEN, "cli", "nminus1-case9.txt")):
+        c = by_label[label]
+        if status == "ISLANDED":
+            assert not c.converged and c.islanded, label
+        elif status == "secure":
+            assert c.secure, (label, c)
+        else:
+            assert len(c.violations) == int(status[len("VIOLATIONx"):]), (label, c)
+            assert f"{c.max_branch_loading_pct:.0f}%" == maxload
