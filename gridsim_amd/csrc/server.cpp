@@ -94,7 +94,8 @@ struct Request {
 struct Finished {
   gs_status status = GS_OK;
   gs_nr_result result{};
-  std::vector<double> vm, va;
+  std::vector<double> vm, va, p, q;   // p, q: per unit
+  double base_mva = 100.0;
   double latency_ms = 0;
 };
 
@@ -110,7 +111,7 @@ struct gs_server {
     BatchRun run;
     Layout L{};
     DevMem base_spec;   // pspec | qspec of the network as uploaded
-    size_t state_span = 0;  // bytes of one instance's vm..va region copied back
+    size_t state_span = 0;  // bytes of one instance's vm, va, p, q region copied back
   };
   std::vector<std::unique_ptr<Net>> nets;
 
@@ -224,13 +225,17 @@ std::vector<std::pair<uint64_t, Finished>> reap_slot(gs_server* srv, gs_server::
   const char* hbase = srv->pinned + s.host_off;
   const InstState* st = reinterpret_cast<const InstState*>(hbase);
   const char* volts = hbase + align_up(sizeof(InstState) * (size_t)count);
-  const size_t va_off = nt.L.va - nt.L.vm;
+  auto span = [&](int i, size_t off) {
+    return reinterpret_cast<const double*>(volts + nt.state_span * (size_t)i + (off - nt.L.vm));
+  };
   for (int i = 0; i < count; i++) {
     Finished f;
-    const double* vm = reinterpret_cast<const double*>(volts + nt.state_span * (size_t)i);
-    const double* va = reinterpret_cast<const double*>(volts + nt.state_span * (size_t)i + va_off);
+    f.base_mva = nt.net->net.base;
+    const double *vm = span(i, nt.L.vm), *va = span(i, nt.L.va), *p = span(i, nt.L.p), *q = span(i, nt.L.q);
     f.vm.assign(vm, vm + n);
     f.va.assign(va, va + n);
+    f.p.assign(p, p + n);
+    f.q.assign(q, q + n);
     if (st[i].done == 1) {
       fill_result(st[i], f.result, 0, srv->ctx->us_per_tick);
     } else {
@@ -239,7 +244,10 @@ std::vector<std::pair<uint64_t, Finished>> reap_slot(gs_server* srv, gs_server::
         int nl = 0;
         gs_instance in = s.reqs[i].inst;
         in.vm0 = in.va0 = nullptr;
-        finish_paused(nt.net, in, nt.opts, false, st[i], f.vm, f.va, f.result, kms, nl);
+        std::vector<double> pp, qq;
+        finish_paused(nt.net, in, nt.opts, false, st[i], f.vm, f.va, f.result, kms, nl, &pp, &qq);
+        if ((int)pp.size() == n) f.p = pp;
+        if ((int)qq.size() == n) f.q = qq;
       } catch (const HipError& e) {
         f.status = e.code;
       }
@@ -415,7 +423,10 @@ GS_API gs_status gs_server_add_network(gs_server* srv, gs_net* net, const gs_nr_
       fail(GS_INVALID_ARGUMENT, "the serving loop launches a wavefront per instance (schedule >= 0)");
     nt->run = plan_run(net->dev.h, nt->opts, false);
     nt->L = instance_layout(net->dev.h, nt->run.fp32, nt->run.x_lds);
-    nt->state_span = nt->L.va - nt->L.vm + 8 * (size_t)net->net.n;
+    // vm, va, p, q lie in that order in an instance's block: one strided copy brings all four back
+    static_assert(sizeof(double) == 8, "double layout");
+    if (!(nt->L.vm < nt->L.va && nt->L.va < nt->L.p && nt->L.p < nt->L.q)) fail(GS_DEVICE_ERROR, "instance layout");
+    nt->state_span = nt->L.q - nt->L.vm + 8 * (size_t)net->net.n;
     if (device_block(*nt, 1) > srv->dev_alloc.size() || host_block(*nt, 1) > srv->host_alloc.size())
       fail(GS_OUT_OF_MEMORY, "one instance of this network exceeds the server's state or pinned budget");
     std::vector<double> spec(net->spec.pspec);
@@ -462,7 +473,7 @@ GS_API gs_status gs_server_submit(gs_server* srv, int32_t network_id, const gs_i
 }
 
 GS_API gs_status gs_server_wait(gs_server* srv, uint64_t ticket, int32_t timeout_ms, gs_nr_result* result,
-                                double* vm, double* va, double* latency_ms) {
+                                double* vm, double* va, double* p_inj, double* q_inj, double* latency_ms) {
   if (!srv) return GS_INVALID_ARGUMENT;
   std::unique_lock<std::mutex> lock(srv->mu);
   auto known = [&] {
@@ -482,6 +493,9 @@ GS_API gs_status gs_server_wait(gs_server* srv, uint64_t ticket, int32_t timeout
   if (result) *result = f.result;
   if (vm) std::copy(f.vm.begin(), f.vm.end(), vm);
   if (va) std::copy(f.va.begin(), f.va.end(), va);
+  // injections in MW / MVAr, as PowerFlowResult.PInjMw / QInjMvar
+  if (p_inj) for (size_t i = 0; i < f.p.size(); i++) p_inj[i] = f.p[i] * f.base_mva;
+  if (q_inj) for (size_t i = 0; i < f.q.size(); i++) q_inj[i] = f.q[i] * f.base_mva;
   if (latency_ms) *latency_ms = f.latency_ms;
   return GS_OK;
 }

@@ -125,7 +125,7 @@ def lib():
         L.gs_host_sincos.restype = None
         L.gs_server_free.restype = None
         L.gs_server_wait.argtypes = [C.c_void_p, C.c_uint64, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
-                                     C.c_void_p]
+                                     C.c_void_p, C.c_void_p, C.c_void_p]
         L.gs_server_submit.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
@@ -676,20 +676,25 @@ class Server:
             raise GsError(st, "gs_server_submit")
         return t.value
 
-    def wait(self, ticket: int, network_id: int, timeout_ms: int = -1, keep_state: bool = True):
-        """(result record, vm, va (radians), latency ms) of a finished request; None when timeout_ms ran out."""
+    def wait(self, ticket: int, network_id: int, timeout_ms: int = -1, keep_state: bool = True,
+             injections: bool = False):
+        """(result record, vm, va (radians), latency ms) of a finished request, with (P, Q) in MW / MVAr appended when
+        injections; None when timeout_ms ran out."""
         n = self.nets[network_id].net.n_bus
         res = np.zeros(1, RESULT_DTYPE)
         vm = np.empty(n) if keep_state else None
         va = np.empty(n) if keep_state else None
+        pi = np.empty(n) if injections else None
+        qi = np.empty(n) if injections else None
         lat = C.c_double()
         st = lib().gs_server_wait(self.h, C.c_uint64(ticket), C.c_int32(timeout_ms), _p(res), _p(vm), _p(va),
-                                  C.byref(lat))
+                                  _p(pi), _p(qi), C.byref(lat))
         if st == GS_PENDING:
             return None
         if st != GS_OK:
             raise GsError(st, f"gs_server_wait (ticket {ticket})")
-        return res[0], vm, va, lat.value
+        out = (res[0], vm, va, lat.value)
+        return out + (pi, qi) if injections else out
 
     def stats(self) -> dict:
         s = GsServerStats()

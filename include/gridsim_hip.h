@@ -286,11 +286,12 @@ GS_API gs_status gs_server_create(gs_ctx* ctx, const gs_server_config* cfg, gs_s
 GS_API gs_status gs_server_add_network(gs_server* srv, gs_net* net, const gs_nr_options* opts, int32_t* network_id);
 /* Queues one request (its vm0 / va0, if any, are copied). GS_BUSY when queue_capacity requests are waiting. */
 GS_API gs_status gs_server_submit(gs_server* srv, int32_t network_id, const gs_instance* inst, uint64_t* ticket);
-/* Waits up to timeout_ms (< 0: no limit) for a request's result: GS_OK and the result (vm / va, radians, n_bus each,
- * may be null; latency_ms may be null), after which the ticket is forgotten; GS_PENDING when the time ran out; the
- * request's own error status when its launch failed; GS_INVALID_ARGUMENT for an unknown ticket. */
+/* Waits up to timeout_ms (< 0: no limit) for a request's result: GS_OK and the result (vm / va in pu and radians,
+ * p_inj / q_inj in MW / MVAr, n_bus each, any may be null; latency_ms may be null), after which the ticket is
+ * forgotten; GS_PENDING when the time ran out; the request's own error status when its launch failed;
+ * GS_INVALID_ARGUMENT for an unknown ticket. */
 GS_API gs_status gs_server_wait(gs_server* srv, uint64_t ticket, int32_t timeout_ms, gs_nr_result* result,
-                                double* vm, double* va, double* latency_ms);
+                                double* vm, double* va, double* p_inj, double* q_inj, double* latency_ms);
 GS_API gs_status gs_server_stats_get(gs_server* srv, gs_server_stats* out);
 GS_API gs_status gs_server_stats_reset(gs_server* srv);
 /* Finishes every queued and in-flight request, then stops the dispatcher and frees the server. */

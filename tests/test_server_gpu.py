@@ -135,3 +135,49 @@ def test_server_queue_limit_and_argument_checks(ctx, warm_nets):
     assert st["rejected"] == busy and st["completed"] == len(tickets)
     srv.close()
     dn.close()
+
+
+def test_rpc_session_over_the_gpu_server(ctx, oracle):
+    """A gridsim-rpc/1 session (gridsim_amd/rpc.py) over the GPU server: two cases loaded, a burst of pipelined solves
+    (power flows and single contingencies) answered in request order, each equal to the oracle's solve of the same
+    request (verdict, |dV| <= 1e-9, losses), with the solves batched on the device."""
+    import io
+    import json
+    from conftest import GOLDEN
+    from gridsim_amd import gridgen, rpc
+    backend = rpc.GpuBackend(ctx, max_batch=64)
+    host = rpc.RpcHost(backend, base_dir=GOLDEN)
+    syn = gridgen.sized_grid(300, gridgen.BASE_SEED + 21)
+    outs9 = [1, 2, 4, 5, 7, 8]                       # case9's non-bridge branches
+    outs300 = list(gridgen.config2_outages(syn)[:10])
+    lines = [{"op": "load", "id": "l1", "case": "matpower/case9.m"},
+             {"op": "load", "id": "l2", "case": f"synthetic:300:{gridgen.BASE_SEED + 21}"}]
+    reqs = []
+    for i in range(40):
+        if i % 2 == 0:
+            reqs.append(("case-1", outs9[i % len(outs9)] if i % 4 == 0 else -1, 1.0 + 0.01 * (i % 5)))
+        else:
+            reqs.append(("case-2", int(outs300[i % len(outs300)]) if i % 3 else -1, 0.9 + 0.02 * (i % 7)))
+    for i, (h, o, s) in enumerate(reqs):
+        lines.append({"op": "solve", "id": i, "handle": h, "outage": o, "load_scale": s})
+    lines += [{"op": "stats", "id": "s"}, {"op": "shutdown", "id": "x"}]
+    out = io.StringIO()
+    host.serve(io.StringIO("\n".join(json.dumps(l) for l in lines) + "\n"), out)
+    resp = [json.loads(x) for x in out.getvalue().splitlines()]
+    assert [r["id"] for r in resp] == ["l1", "l2"] + list(range(40)) + ["s", "x"]
+    from conftest import matpower_case
+    nets = {"case-1": matpower_case("case9"), "case-2": syn}
+    for (h, o, s), r in zip(reqs, resp[2:42]):
+        net = nets[h].with_load_multiplier(s) if s != 1.0 else nets[h]
+        rec, vm, va = oracle.solve(net, oracle.options(max_iterations=100), skip=o)
+        res = r["result"]
+        assert res["converged"] == (rec["status"] == 0), (h, o, s)
+        if rec["status"] == 0:
+            assert np.abs(np.array(res["vm_pu"]) - vm).max() <= TOL_DV
+            assert np.abs(np.radians(res["va_deg"]) - va).max() <= 1e-9
+            p, q, _ = oracle.mismatch(net, vm, va, skip=o)
+            assert abs(res["loss_mw"] - (p * net.base_mva).sum()) <= 1e-6
+    st = resp[42]["result"]
+    print("rpc stats", st, flush=True)
+    assert st["solves"] == 40 and st["server"]["completed"] >= 40
+    backend.close()
