@@ -371,3 +371,19 @@ def test_assembled_result_and_contingency_records_case9(ctx, eng, oracle):
         else:
             assert len(c.violations) == int(status[len("VIOLATIONx"):]), (label, c)
             assert f"{c.max_branch_loading_pct:.0f}%" == maxload
+
+
+def test_solve_export_gpu_against_cpu_the_reference_comparer_way(ctx, eng, oracle):
+    """The reference's CPU-vs-GPU check (tools/run-gpu.sh:27-50) on case118 and case300: the device's solve export
+    against the oracle's, same bus set, max |dVm| <= 1e-6 pu and max |dVa| <= 1e-4 deg (they agree far closer)."""
+    from gridsim_amd import export
+    from test_export import _oracle_result
+    for name in ("case118", "case300"):
+        net = matpower_case(name)
+        gpu = eng.HipNewtonPowerFlow(ctx, eng.NrOptions(lu_precision=eng.LU_FP64, refine_steps=0)).solve(net)
+        a = export.solve_export(net, _oracle_result(oracle, net), case=name, backend="cpu (oracle)")
+        b = export.solve_export(net, gpu, case=name, backend="gpu (gfx950)")
+        ok, dvm, dva = export.compare_exports(a, b)
+        print(f"COMPARE {name}: max|dVm| = {dvm:.3g} pu, max|dVa| = {dva:.3g} deg", flush=True)
+        assert ok and dvm <= 1e-9 and dva <= 1e-7
+        assert abs(a["result"]["totalLossMw"] - b["result"]["totalLossMw"]) <= 1e-6
