@@ -243,6 +243,7 @@ void csr_multiply(const Csr& a, const double* x, double* y) {
 struct BusSpec {
   vector<int> type0;
   vector<double> vm, va, pspec, qspec;
+  vector<double> qd, vg, qmin, qmax;   // MVAr / pu: the reactive-limit loop's data (sums over a bus's generators)
 };
 
 // NewtonShell.BuildBusSpec (NewtonShell.cs:433-486), constant-power buses.
@@ -253,13 +254,19 @@ BusSpec build_bus_spec(const orc_network& net, bool flat_start) {
   s.vm.assign(n, 0); s.va.assign(n, 0); s.pspec.assign(n, 0); s.qspec.assign(n, 0);
   vector<double> pg(n, 0), qg(n, 0), vg(n, 0);
   vector<char> has_gen(n, 0);
+  s.qmin.assign(n, 0);
+  s.qmax.assign(n, 0);
   for (int g = 0; g < net.n_gen; g++) {
     int i = net.gen_bus[g];
     pg[i] += net.gen_pg[g];
     qg[i] += net.gen_qg[g];
+    s.qmin[i] += net.gen_qmin ? net.gen_qmin[g] : -INFINITY;
+    s.qmax[i] += net.gen_qmax ? net.gen_qmax[g] : INFINITY;
     vg[i] = net.gen_vg[g];
     has_gen[i] = 1;
   }
+  s.vg = vg;
+  s.qd.assign(net.qd, net.qd + n);
   double base = net.base_mva;
   for (int i = 0; i < n; i++) {
     int bt = net.bus_type[i];
@@ -918,6 +925,48 @@ NewtonOut run_newton(const Ycsr& y, const vector<int>& type, vector<double>& vm,
   return out;
 }
 
+// NewtonShell.ApplyQLimits (NewtonShell.cs:519-558) with RevertQLimits off, then RevertBrokenPins (:583-595): a bus
+// that started PV and is still PV switches to PQ at the limit its generators' reactive output crossed (by more than
+// QLimitTolMva); once nothing more switches, a pinned bus whose voltage lies on the wrong side of its set point (by
+// more than RevertVoltTolPu) goes back to PV at its set point, at most once per solve. Returns whether any bus changed.
+enum { kPinNone = 0, kPinMax = 1, kPinMin = 2, kPinFixed = 3 };
+constexpr double kQLimitTolMva = 1e-3, kRevertVoltTolPu = 1e-4;
+
+bool apply_q_limits(const BusSpec& s, vector<int>& eff, const vector<double>& q, vector<double>& qspec,
+                    vector<double>& vm, vector<int>& pinned, vector<int>& reverts, double base) {
+  const int n = (int)eff.size();
+  bool changed = false;
+  for (int i = 0; i < n; i++) {
+    if (s.type0[i] != ORC_PV || eff[i] != ORC_PV) continue;
+    const double qgen = q[i] * base + s.qd[i];
+    if (qgen > s.qmax[i] + kQLimitTolMva) {
+      eff[i] = ORC_PQ;
+      qspec[i] = (s.qmax[i] - s.qd[i]) / base;
+      pinned[i] = s.qmin[i] == s.qmax[i] ? kPinFixed : kPinMax;
+      changed = true;
+    } else if (qgen < s.qmin[i] - kQLimitTolMva) {
+      eff[i] = ORC_PQ;
+      qspec[i] = (s.qmin[i] - s.qd[i]) / base;
+      pinned[i] = s.qmin[i] == s.qmax[i] ? kPinFixed : kPinMin;
+      changed = true;
+    }
+  }
+  if (changed) return true;
+  for (int i = 0; i < n; i++) {
+    if (s.type0[i] != ORC_PV || eff[i] != ORC_PQ) continue;
+    if (pinned[i] == kPinNone || pinned[i] == kPinFixed || reverts[i] >= 1) continue;
+    const bool rmax = pinned[i] == kPinMax && vm[i] > s.vg[i] + kRevertVoltTolPu;
+    const bool rmin = pinned[i] == kPinMin && vm[i] < s.vg[i] - kRevertVoltTolPu;
+    if (!(rmax || rmin)) continue;
+    eff[i] = ORC_PV;
+    vm[i] = s.vg[i];
+    pinned[i] = kPinNone;
+    reverts[i]++;
+    changed = true;
+  }
+  return changed;
+}
+
 void solve_one(const orc_network& net, int skip, const orc_options& opt, double* vm_out, double* va_out,
                orc_result* res) {
   auto t0 = std::chrono::steady_clock::now();
@@ -925,7 +974,28 @@ void solve_one(const orc_network& net, int skip, const orc_options& opt, double*
   BusSpec spec = build_bus_spec(net, opt.flat_start != 0);
   int n = net.n_bus;
   vector<double> p(n), q(n);
-  NewtonOut o = run_newton(y, spec.type0, spec.vm, spec.va, spec.pspec, spec.qspec, p, q, opt);
+  // SparseNewtonRaphsonPowerFlow.SolveAcAttempt's reactive-limit outer loop (:118-160)
+  vector<int> eff(spec.type0), pinned(n, kPinNone), reverts(n, 0);
+  const bool enforce = opt.enforce_q_limits != 0;
+  const int passes = enforce ? std::max(1, opt.max_q_passes) : 1;
+  int total = 0;
+  bool pending = false;
+  NewtonOut o;
+  for (int qp = 0; qp < passes; qp++) {
+    pending = false;
+    o = run_newton(y, eff, spec.vm, spec.va, spec.pspec, spec.qspec, p, q, opt);
+    total += o.iters;
+    if (!enforce || !o.converged) break;
+    pending = apply_q_limits(spec, eff, q, spec.qspec, spec.vm, pinned, reverts, net.base_mva);
+    if (!pending) break;
+  }
+  if (pending) {
+    // the cap came right after a switch: solve the switched types once more and report the state not converged
+    o = run_newton(y, eff, spec.vm, spec.va, spec.pspec, spec.qspec, p, q, opt);
+    total += o.iters;
+    o.converged = false;
+  }
+  o.iters = total;
   compute_injections(y, spec.vm.data(), spec.va.data(), p.data(), q.data());
   bool finite = std::isfinite(o.max_mis);
   for (int i = 0; i < n && finite; i++)

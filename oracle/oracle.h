@@ -30,6 +30,7 @@ typedef struct orc_network {
   const uint8_t* br_in_service;
   const int32_t* gen_bus;
   const double *gen_pg, *gen_qg, *gen_vg;
+  const double *gen_qmin, *gen_qmax;   /* MVAr; may be null (no limits) */
 } orc_network;
 
 typedef struct orc_options {
@@ -37,6 +38,8 @@ typedef struct orc_options {
   int32_t max_iterations;
   double diverge_limit;  /* PowerFlowOptions.DivergeLimitPu (10) */
   int32_t flat_start;
+  int32_t enforce_q_limits;  /* PowerFlowOptions.EnforceQLimits: the reactive-limit outer loop */
+  int32_t max_q_passes;      /* PowerFlowOptions.MaxQLimitPasses (100) */
 } orc_options;
 
 typedef struct orc_result {

@@ -24,12 +24,12 @@ class OrcNetwork(C.Structure):
                 ("br_from", C.c_void_p), ("br_to", C.c_void_p), ("br_r", C.c_void_p), ("br_x", C.c_void_p),
                 ("br_b", C.c_void_p), ("br_tap", C.c_void_p), ("br_shift_deg", C.c_void_p),
                 ("br_in_service", C.c_void_p), ("gen_bus", C.c_void_p), ("gen_pg", C.c_void_p),
-                ("gen_qg", C.c_void_p), ("gen_vg", C.c_void_p)]
+                ("gen_qg", C.c_void_p), ("gen_vg", C.c_void_p), ("gen_qmin", C.c_void_p), ("gen_qmax", C.c_void_p)]
 
 
 class OrcOptions(C.Structure):
     _fields_ = [("tolerance", C.c_double), ("max_iterations", C.c_int32), ("diverge_limit", C.c_double),
-                ("flat_start", C.c_int32)]
+                ("flat_start", C.c_int32), ("enforce_q_limits", C.c_int32), ("max_q_passes", C.c_int32)]
 
 
 class OrcResult(C.Structure):
@@ -77,15 +77,18 @@ class _NetView:
                             _p(net.bus_type), _p(net.pd), _p(net.qd), _p(net.gs), _p(net.bs), _p(net.vm),
                             _p(net.va_deg), _p(net.br_from), _p(net.br_to), _p(net.br_r), _p(net.br_x),
                             _p(net.br_b), _p(net.br_tap), _p(net.br_shift_deg), _p(net.br_in_service),
-                            _p(net.gen_bus), _p(net.gen_pg), _p(net.gen_qg), _p(net.gen_vg))
+                            _p(net.gen_bus), _p(net.gen_pg), _p(net.gen_qg), _p(net.gen_vg), _p(net.gen_qmin),
+                            _p(net.gen_qmax))
 
     @property
     def ref(self):
         return C.byref(self.s)
 
 
-def options(tolerance=1e-8, max_iterations=30, diverge_limit=10.0, flat_start=False):
-    return OrcOptions(tolerance, max_iterations, diverge_limit, 1 if flat_start else 0)
+def options(tolerance=1e-8, max_iterations=30, diverge_limit=10.0, flat_start=False, q_limits=False,
+            max_q_passes=100):
+    return OrcOptions(tolerance, max_iterations, diverge_limit, 1 if flat_start else 0, 1 if q_limits else 0,
+                      max_q_passes)
 
 
 def sincos(x):

@@ -472,16 +472,24 @@ def test_n1_screen_threads_agree_and_match_single_solves(oracle):
 
 
 def _parse_nminus1(path):
-    """Rows of the reference's `gridsim nminus1` CLI output: (outage label, status, max load %, vmin, vmax)."""
-    rows = []
+    """Rows of the reference's `gridsim nminus1` CLI table (RpcCommand-independent; Program.cs prints each row as
+    outage (30 wide), status (12 wide), then MaxLoad%, WorstBnd%, Vmin, Vmax, or the first violation of a non-converged
+    outage): (outage label without the " (tx)" mark, status, max load, vmin, vmax)."""
+    rows, inside = [], 0
     for line in open(path, encoding="utf-8"):
-        parts = line.split()
-        if len(parts) < 2 or "-" not in parts[0] or not parts[0][0].isdigit():
+        line = line.rstrip("\n")
+        if line.startswith("-----"):
+            inside += 1
             continue
-        if parts[1] in ("ISLANDED", "NO"):
-            rows.append((parts[0], parts[1], None, None, None))
+        if inside != 1 or not line.strip():
+            continue
+        label = line[:30].strip().replace(" (tx)", "")
+        status = line[32:44].strip()
+        rest = line[46:].split()
+        if status in ("ISLANDED", "NO SOLUTION"):
+            rows.append((label, status, None, None, None))
         else:
-            rows.append((parts[0], parts[1], parts[2], parts[4], parts[5]))
+            rows.append((label, status, rest[0], rest[2], rest[3]))
     return rows
 
 
@@ -531,3 +539,40 @@ def test_nminus1_case9_matches_the_reference_cli_fixture(oracle):
         assert (status == "secure") == (violations == 0), (label, s)
         if status.startswith("VIOLATIONx"):
             assert violations == int(status[len("VIOLATIONx"):]), (label, s)
+
+
+def test_nminus1_case14_with_reactive_limits_matches_the_reference_cli_fixture(oracle):
+    """`gridsim nminus1 case14` (nminus1-case14.txt) solves with reactive limits enforced (StudyOptions); case14's
+    generators hit them, so Vmin / Vmax (3 decimals) and the verdicts pin the reactive-limit outer loop
+    (ApplyQLimits, RevertBrokenPins, the pass cap), and 1-2's NO SOLUTION the non-converged verdict."""
+    net = matpower_case("case14")
+    ids = list(net.bus_ids)
+    rows = _parse_nminus1(os.path.join(GOLDEN, "cli", "nminus1-case14.txt"))
+    assert len(rows) == 20
+    opt = oracle.options(max_iterations=100, q_limits=True)
+    for label, status, maxload, vmin, vmax in rows:
+        a, b = (int(x) for x in label.split("-"))
+        k = [i for i in range(net.n_branch) if (ids[net.br_from[i]], ids[net.br_to[i]]) == (a, b)][0]
+        rec, vm, va = oracle.solve(net, opt, skip=k)
+        if status == "NO SOLUTION":
+            assert rec["status"] != oracle.CONVERGED, label
+            continue
+        assert rec["status"] == oracle.CONVERGED, label
+        s = oracle.screen_summary(net, vm, va, skip=k)
+        assert (f"{s['max_loading_pct']:.0f}%", f"{s['vmin']:.3f}", f"{s['vmax']:.3f}") == (maxload, vmin, vmax), \
+            (label, s)
+
+
+def test_reactive_limits_decide_case14_outages(oracle):
+    """The fixture above needs the outer loop: without it, outage 1-2 converges (the reference reports NO SOLUTION)
+    and outage 2-3 keeps Vmin at 1.010 (the reference, with generators 2 and 3 at their limits, 0.954)."""
+    net = matpower_case("case14")
+    ids = list(net.bus_ids)
+    k12 = [i for i in range(net.n_branch) if (ids[net.br_from[i]], ids[net.br_to[i]]) == (1, 2)][0]
+    k23 = [i for i in range(net.n_branch) if (ids[net.br_from[i]], ids[net.br_to[i]]) == (2, 3)][0]
+    plain = oracle.options(max_iterations=100)
+    limited = oracle.options(max_iterations=100, q_limits=True)
+    assert oracle.solve(net, plain, skip=k12)[0]["status"] == oracle.CONVERGED
+    assert oracle.solve(net, limited, skip=k12)[0]["status"] != oracle.CONVERGED
+    assert f"{oracle.solve(net, plain, skip=k23)[1].min():.3f}" == "1.010"
+    assert f"{oracle.solve(net, limited, skip=k23)[1].min():.3f}" == "0.954"
