@@ -180,7 +180,7 @@ gs_nr_options normalise(const gs_nr_options* o) {
   r.schedule = 0;
   r.flags = 0;
   if (o) r = *o;
-  if (r.flags & ~GS_FLAG_X_GLOBAL) fail(GS_INVALID_ARGUMENT, "unknown flags");
+  if (r.flags & ~(GS_FLAG_X_GLOBAL | GS_FLAG_Q_LIMITS)) fail(GS_INVALID_ARGUMENT, "unknown flags");
   if (r.schedule < GS_SCHEDULE_LEVELS) fail(GS_INVALID_ARGUMENT, "schedule must be >= 0 or GS_SCHEDULE_LEVELS");
   if (!(r.tolerance > 0)) fail(GS_INVALID_ARGUMENT, "tolerance must be positive");
   if (r.max_iterations < 0) fail(GS_INVALID_ARGUMENT, "max_iterations must be >= 0");
@@ -514,9 +514,99 @@ void copy_state_out(gs_ctx* ctx, const DevMem& inst_buf, const Layout& L, int i,
 // A paused instance (a static pivot failed its growth check): fresh CPU symbolic passes on the instance's own network
 // at its current state (vm, va: in and out), each followed by a resumed launch on that plan, until it finishes. The
 // resumed run keeps its iteration count and start mismatch.
+// The network pattern alone on the device (no LU plan): what the iterative step reads.
+void upload_pattern(DevicePlan& dp, const YBus& y, const JacobianPattern& jp, double base, hipStream_t s) {
+  DevPlan& h = dp.h;
+  h = DevPlan{};
+  h.n = y.n;
+  h.m = jp.m;
+  h.nnz_y = y.rp[y.n];
+  h.nnz_j = jp.rp[jp.m];
+  h.base_mva = base;
+  std::vector<int> y_row(h.nnz_y), aslots(jp.slots.size(), -1), jdiag(std::max(1, jp.m), -1);
+  for (int i = 0; i < y.n; i++)
+    for (int k = y.rp[i]; k < y.rp[i + 1]; k++) y_row[k] = i;
+  for (int r = 0; r < jp.m; r++)
+    for (int k = jp.rp[r]; k < jp.rp[r + 1]; k++)
+      if (jp.ci[k] == r) jdiag[r] = k;
+  upload(dp.y_rp, y.rp, s); upload(dp.y_ci, y.ci, s); upload(dp.y_row, y_row, s); upload(dp.y_diag, y.diag, s);
+  upload(dp.y_g, y.g, s); upload(dp.y_b, y.b, s); upload(dp.slots, jp.slots, s); upload(dp.aslots, aslots, s);
+  upload(dp.ppos, jp.ppos, s); upload(dp.qpos, jp.qpos, s); upload(dp.j_rp, jp.rp, s); upload(dp.j_ci, jp.ci, s);
+  upload(dp.j_diag, jdiag, s);
+  h.y_rp = dp.y_rp.as<int>(); h.y_ci = dp.y_ci.as<int>(); h.y_row = dp.y_row.as<int>(); h.y_diag = dp.y_diag.as<int>();
+  h.y_g = dp.y_g.as<double>(); h.y_b = dp.y_b.as<double>();
+  h.slots = dp.slots.as<int4>(); h.aslots = dp.aslots.as<int4>();
+  h.ppos = dp.ppos.as<int>(); h.qpos = dp.qpos.as<int>(); h.j_rp = dp.j_rp.as<int>(); h.j_ci = dp.j_ci.as<int>();
+  h.j_diag = dp.j_diag.as<int>();
+  check(hipStreamSynchronize(s), "pattern upload");   // the staging vectors are locals
+}
+
+// One Newton iteration of one instance with SolveStep's ILU(0) + BiCGSTAB fallback on the device (the LU refused the
+// Jacobian): state st and vm / va in and out; p / q (when non-null) the injections at the final state.
+void iterative_step(gs_ctx* ctx, const YBus& y, const JacobianPattern& jp, const BusSpec& spec, double base,
+                    const gs_nr_options& o, InstState& st, std::vector<double>& vm, std::vector<double>& va,
+                    std::vector<double>* p_out, std::vector<double>* q_out) {
+  hipStream_t s = ctx->stream;
+  DevicePlan dp;
+  upload_pattern(dp, y, jp, base, s);
+  const size_t n = (size_t)y.n, m = (size_t)jp.m, nj = (size_t)jp.rp[jp.m], ny = (size_t)y.rp[y.n];
+  auto al = [](size_t v) { return (v + 255) & ~size_t(255); };
+  size_t off = 0;
+  const size_t o_vm = off; off += al(8 * n);
+  const size_t o_va = off; off += al(8 * n);
+  const size_t o_p = off; off += al(8 * n);
+  const size_t o_q = off; off += al(8 * n);
+  const size_t o_f = off; off += al(8 * m);
+  const size_t o_dx = off; off += al(8 * m);
+  const size_t o_jv = off; off += al(8 * nj);
+  const size_t o_tpq = off; off += al(16 * ny);
+  const size_t o_spec = off; off += al(16 * n);
+  const size_t o_start = off; off += al(16 * n);
+  const size_t o_work = off; off += al(8 * (nj + 8 * m));
+  const size_t o_iw = off; off += al(4 * std::max<size_t>(1, m));
+  const size_t o_state = off; off += al(sizeof(InstState));
+  const size_t o_desc = off; off += al(sizeof(InstDesc));
+  DevMem blk;
+  blk.ensure(off);
+  char* b = blk.as<char>();
+  std::vector<double> specv(spec.pspec), start(vm);
+  specv.insert(specv.end(), spec.qspec.begin(), spec.qspec.end());
+  start.insert(start.end(), va.begin(), va.end());
+  InstDesc d{};
+  d.state = reinterpret_cast<InstState*>(b + o_state);
+  d.pspec = reinterpret_cast<const double*>(b + o_spec);
+  d.qspec = d.pspec + n;
+  d.vm_start = reinterpret_cast<const double*>(b + o_start);
+  d.va_start = d.vm_start + n;
+  d.n_patch = 0;
+  d.outage = -1;
+  d.vm = (double*)(b + o_vm); d.va = (double*)(b + o_va); d.p = (double*)(b + o_p); d.q = (double*)(b + o_q);
+  d.f = (double*)(b + o_f); d.dx = (double*)(b + o_dx); d.r = nullptr; d.jv = (double*)(b + o_jv);
+  d.tpq = (double*)(b + o_tpq);
+  check(hipMemcpyAsync(b + o_spec, specv.data(), 16 * n, hipMemcpyHostToDevice, s), "iterative spec");
+  check(hipMemcpyAsync(b + o_start, start.data(), 16 * n, hipMemcpyHostToDevice, s), "iterative start");
+  check(hipMemcpyAsync(b + o_state, &st, sizeof(InstState), hipMemcpyHostToDevice, s), "iterative state");
+  check(hipMemcpyAsync(b + o_desc, &d, sizeof(InstDesc), hipMemcpyHostToDevice, s), "iterative desc");
+  NrParams prm{};
+  prm.tolerance = o.tolerance;
+  prm.diverge_limit = o.diverge_limit;
+  prm.max_iterations = o.max_iterations;
+  prm.load_start = 1;
+  check(launch_iterative_step(reinterpret_cast<const InstDesc*>(b + o_desc), dp.h, prm,
+                              reinterpret_cast<double*>(b + o_work), reinterpret_cast<int*>(b + o_iw), s),
+        "iterative step");
+  check(hipMemcpyAsync(&st, b + o_state, sizeof(InstState), hipMemcpyDeviceToHost, s), "iterative state back");
+  check(hipMemcpyAsync(vm.data(), b + o_vm, 8 * n, hipMemcpyDeviceToHost, s), "iterative vm");
+  check(hipMemcpyAsync(va.data(), b + o_va, 8 * n, hipMemcpyDeviceToHost, s), "iterative va");
+  if (p_out) { p_out->resize(n); check(hipMemcpyAsync(p_out->data(), b + o_p, 8 * n, hipMemcpyDeviceToHost, s), "p"); }
+  if (q_out) { q_out->resize(n); check(hipMemcpyAsync(q_out->data(), b + o_q, 8 * n, hipMemcpyDeviceToHost, s), "q"); }
+  check(hipStreamSynchronize(s), "iterative step");
+}
+
 void finish_paused(gs_net* net, const gs_instance& in, const gs_nr_options& o, bool plain, InstState st,
                    std::vector<double>& vm, std::vector<double>& va, gs_nr_result& result, double& kms, int& launches,
-                   std::vector<double>* p_out, std::vector<double>* q_out) {
+                   std::vector<double>* p_out, std::vector<double>* q_out, const std::vector<int>* types,
+                   const std::vector<double>* qspec) {
   gs_ctx* ctx = net->ctx;
   const int n = net->net.n;
   int refreshes = 0;
@@ -528,16 +618,26 @@ void finish_paused(gs_net* net, const gs_instance& in, const gs_nr_options& o, b
   }
   YBus y = build_ybus(own, -1);
   BusSpec spec = build_bus_spec(own, o.flat_start != 0, 1.0);
+  if (types) spec.type = *types;
+  if (qspec) spec.qspec = *qspec;
   JacobianPattern jp = build_jacobian_pattern(y, spec.type);
   DevicePlan dp;
   DevMem ib, db, sb, ab;
   bool finished = false;
+  int iterative = 0;
   while (!finished) {
-    refreshes++;
     Plan plan;
     std::string why;
-    if (!symbolic_pass(y, jp, vm.data(), va.data(), plan, why) || refreshes > 8) {
-      // singular at this state: the step cannot be solved and the run stops (NewtonShell.SolveStep false)
+    if (!symbolic_pass(y, jp, vm.data(), va.data(), plan, why)) {
+      // singular at this state: the LU refuses the step, so SolveStep falls back to ILU(0) + BiCGSTAB for it (on the
+      // device); the LU is tried again at the next step
+      if (++iterative > o.max_iterations + 1) { st.done = 1; st.step_failures++; break; }
+      iterative_step(ctx, y, jp, spec, own.base, o, st, vm, va, p_out, q_out);
+      finished = st.done == 1;
+      continue;
+    }
+    if (++refreshes > 8) {
+      // the fresh plans keep failing their growth checks: the run stops
       st.done = 1;
       st.step_failures++;
       break;
@@ -626,6 +726,195 @@ void run_batch(gs_net* net, int count, const gs_instance* inst, const gs_nr_opti
   net->last_kernel_ms = kms;
   net->last_launches = launches;
   net->last_total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// ================================================================================================ reactive limits
+// The outer loop of SparseNewtonRaphsonPowerFlow.SolveAcAttempt (:118-160) over a batch: after a converged run each
+// instance's PV buses are checked against their generators' limits (NewtonShell.ApplyQLimits :519-558, RevertQLimits
+// off, then RevertBrokenPins :583-595); the instances that switched a bus are grouped by their bus types, each group
+// gets the symbolic plan of its Jacobian pattern, and the group runs again on the device from its current state with
+// its pinned Q specification. At most kMaxQLimitPasses runs; a switch after the last one gets one more run and the
+// instance is reported not converged (as the reference does at its pass cap).
+namespace {
+
+enum { kPinNone = 0, kPinMax = 1, kPinMin = 2, kPinFixed = 3 };
+constexpr double kQLimitTolMva = 1e-3, kRevertVoltTolPu = 1e-4;
+constexpr int kMaxQLimitPasses = 100;  // PowerFlowOptions.MaxQLimitPasses
+
+struct QState {
+  BusSpec spec;                          // the instance's own (load-scaled) specification
+  std::vector<int> eff, pinned, reverts;
+  std::vector<double> vm, va, p, q;      // current state (pu, rad; injections pu)
+};
+
+bool apply_q_limits(QState& st, double base) {
+  const BusSpec& s = st.spec;
+  const int n = (int)st.eff.size();
+  bool changed = false;
+  for (int i = 0; i < n; i++) {
+    if (s.type[i] != GS_BUS_PV || st.eff[i] != GS_BUS_PV) continue;
+    const double qgen = st.q[i] * base + s.qd[i];
+    if (qgen > s.qmax[i] + kQLimitTolMva) {
+      st.eff[i] = GS_BUS_PQ;
+      st.spec.qspec[i] = (s.qmax[i] - s.qd[i]) / base;
+      st.pinned[i] = s.qmin[i] == s.qmax[i] ? kPinFixed : kPinMax;
+      changed = true;
+    } else if (qgen < s.qmin[i] - kQLimitTolMva) {
+      st.eff[i] = GS_BUS_PQ;
+      st.spec.qspec[i] = (s.qmin[i] - s.qd[i]) / base;
+      st.pinned[i] = s.qmin[i] == s.qmax[i] ? kPinFixed : kPinMin;
+      changed = true;
+    }
+  }
+  if (changed) return true;
+  for (int i = 0; i < n; i++) {
+    if (s.type[i] != GS_BUS_PV || st.eff[i] != GS_BUS_PQ) continue;
+    if (st.pinned[i] == kPinNone || st.pinned[i] == kPinFixed || st.reverts[i] >= 1) continue;
+    const bool rmax = st.pinned[i] == kPinMax && st.vm[i] > s.vg[i] + kRevertVoltTolPu;
+    const bool rmin = st.pinned[i] == kPinMin && st.vm[i] < s.vg[i] - kRevertVoltTolPu;
+    if (!(rmax || rmin)) continue;
+    st.eff[i] = GS_BUS_PV;
+    st.vm[i] = s.vg[i];
+    st.pinned[i] = kPinNone;
+    st.reverts[i]++;
+    changed = true;
+  }
+  return changed;
+}
+
+// A run's result added to the instance's earlier runs (iterations, refreshes, step failures and device time add up).
+void add_run(gs_nr_result& total, const gs_nr_result& run) {
+  const gs_nr_result before = total;
+  total = run;
+  total.iterations += before.iterations;
+  total.symbolic_refreshes += before.symbolic_refreshes;
+  total.step_failures += before.step_failures;
+  total.min_pivot_ratio = std::min(before.min_pivot_ratio, run.min_pivot_ratio);
+  for (int k = 0; k < 4; k++) total.phase_us[k] += before.phase_us[k];
+}
+
+}  // namespace
+
+// After run_batch (the first run of every instance, its blocks still in net->inst_buf with layout L and paused
+// instances' finals in `paused`): the reactive-limit passes. Returns the final state of every instance that ran
+// again.
+std::vector<HostFinal> q_limit_passes(gs_net* net, int count, const gs_instance* inst, const gs_nr_options& o,
+                                      gs_nr_result* results, const Layout& L, const std::vector<HostFinal>& paused) {
+  gs_ctx* ctx = net->ctx;
+  hipStream_t s = ctx->stream;
+  const int n = net->net.n;
+  const double base = net->net.base;
+  // every instance's state after its first run
+  std::vector<double> vm((size_t)count * n), va((size_t)count * n), pp((size_t)count * n), qq((size_t)count * n);
+  const size_t offs[4] = {L.vm, L.va, L.p, L.q};
+  double* dsts[4] = {vm.data(), va.data(), pp.data(), qq.data()};
+  for (int w = 0; w < 4; w++)
+    check(hipMemcpy2DAsync(dsts[w], 8 * (size_t)n, net->inst_buf.as<char>() + offs[w], L.stride, 8 * (size_t)n,
+                           (size_t)count, hipMemcpyDeviceToHost, s), "state for the limits");
+  check(hipStreamSynchronize(s), "state for the limits");
+  std::map<int, const HostFinal*> host_done;
+  for (const HostFinal& f : paused) host_done[f.i] = &f;
+  std::map<int, QState> st;   // the instances still switching
+  for (int i = 0; i < count; i++) {
+    if (results[i].status != GS_PF_CONVERGED) continue;
+    QState q;
+    q.spec = build_bus_spec(net->net, net->flat_start, inst[i].load_scale);
+    q.eff = q.spec.type;
+    q.pinned.assign(n, kPinNone);
+    q.reverts.assign(n, 0);
+    auto hf = host_done.find(i);
+    if (hf != host_done.end() && (int)hf->second->q.size() == n) {
+      q.vm = hf->second->vm; q.va = hf->second->va; q.p = hf->second->p; q.q = hf->second->q;
+    } else {
+      const size_t b = (size_t)i * n;
+      q.vm.assign(vm.begin() + b, vm.begin() + b + n); q.va.assign(va.begin() + b, va.begin() + b + n);
+      q.p.assign(pp.begin() + b, pp.begin() + b + n); q.q.assign(qq.begin() + b, qq.begin() + b + n);
+    }
+    if (apply_q_limits(q, base)) st.emplace(i, std::move(q));
+  }
+  std::map<int, HostFinal> out;
+  gs_nr_options og = o;
+  og.flags &= ~GS_FLAG_Q_LIMITS;
+  for (int run = 1; run <= kMaxQLimitPasses && !st.empty(); run++) {
+    const bool last = run == kMaxQLimitPasses;   // the run after a switch at the pass cap
+    std::map<std::vector<int>, std::vector<int>> groups;
+    for (auto& kv : st) groups[kv.second.eff].push_back(kv.first);
+    std::vector<int> still;
+    for (auto& g : groups) {
+      const std::vector<int>& types = g.first;
+      const std::vector<int>& ids = g.second;
+      const int cnt = (int)ids.size();
+      JacobianPattern jp = build_jacobian_pattern(net->y, types);
+      Plan plan;
+      std::string why;
+      if (!symbolic_pass(net->y, jp, st[ids[0]].vm.data(), st[ids[0]].va.data(), plan, why)) {
+        // singular at this state: the run cannot take a step (NewtonShell.SolveStep false) and stops there
+        for (int i : ids) {
+          results[i].status = GS_PF_MAX_ITERATIONS;
+          results[i].step_failures++;
+          QState& q = st[i];
+          out[i] = HostFinal{i, q.vm, q.va, q.p, q.q};
+        }
+        continue;
+      }
+      BusSpec gspec = net->spec;
+      gspec.type = types;
+      DevicePlan dp;
+      upload_plan(dp, &net->y, jp, &gspec, plan, base, s);
+      std::vector<gs_instance> gi(cnt);
+      std::vector<std::vector<double>> spec_ov(cnt), start_ov(cnt);
+      for (int k = 0; k < cnt; k++) {
+        const QState& q = st[ids[k]];
+        gi[k] = inst[ids[k]];
+        gi[k].vm0 = gi[k].va0 = nullptr;
+        spec_ov[k] = q.spec.pspec;
+        spec_ov[k].insert(spec_ov[k].end(), q.spec.qspec.begin(), q.spec.qspec.end());
+        start_ov[k] = q.vm;
+        start_ov[k].insert(start_ov[k].end(), q.va.begin(), q.va.end());
+      }
+      DevMem ib, db, sb, ab;
+      std::vector<InstState> states;
+      int launches = 0;
+      double kms = launch_instances(ctx, dp, ib, db, sb, ab, cnt, gi.data(), &net->y, &net->net, gspec, og, false,
+                                    states, true, &spec_ov, &start_ov, &launches);
+      const BatchRun rg = plan_run(dp.h, og, false);
+      const Layout Lg = instance_layout(dp.h, rg.fp32, rg.x_lds);
+      for (int k = 0; k < cnt; k++) {
+        const int i = ids[k];
+        QState& q = st[i];
+        gs_nr_result r{};
+        const char* b = ib.as<char>() + Lg.stride * (size_t)k;
+        double* dst[4] = {q.vm.data(), q.va.data(), q.p.data(), q.q.data()};
+        const size_t src[4] = {Lg.vm, Lg.va, Lg.p, Lg.q};
+        for (int w = 0; w < 4; w++)
+          check(hipMemcpyAsync(dst[w], b + src[w], 8 * (size_t)n, hipMemcpyDeviceToHost, s), "state back");
+        check(hipStreamSynchronize(s), "state back");
+        if (states[k].done == 1) {
+          fill_result(states[k], r, 0, ctx->us_per_tick);
+        } else {
+          std::vector<double> p2, q2;
+          finish_paused(net, gi[k], og, false, states[k], q.vm, q.va, r, kms, launches, &p2, &q2, &types,
+                        &q.spec.qspec);
+          if ((int)p2.size() == n) q.p = p2;
+          if ((int)q2.size() == n) q.q = q2;
+        }
+        add_run(results[i], r);
+        out[i] = HostFinal{i, q.vm, q.va, q.p, q.q};
+        if (last) {
+          // the cap came right after a switch: this run's state is reported, not converged
+          if (results[i].status == GS_PF_CONVERGED) results[i].status = GS_PF_MAX_ITERATIONS;
+        } else if (r.status == GS_PF_CONVERGED && apply_q_limits(q, base)) {
+          still.push_back(i);
+        }
+      }
+    }
+    std::map<int, QState> next;
+    for (int i : still) next.emplace(i, std::move(st[i]));
+    st.swap(next);
+  }
+  std::vector<HostFinal> finals;
+  for (auto& kv : out) finals.push_back(std::move(kv.second));
+  return finals;
 }
 
 // ================================================================================================ seam solves
@@ -941,7 +1230,19 @@ GS_API gs_status gs_nr_solve_batch(gs_net* net, int32_t count, const gs_instance
       if (inst[i].outage_branch < -1 || inst[i].outage_branch >= net->net.nb || !(inst[i].load_scale > 0))
         fail(GS_INVALID_ARGUMENT, "instance " + std::to_string(i));
     if (count == 0) return GS_OK;
-    run_batch(net, count, inst, o, results, vm_out, va_out, false);
+    if (!(o.flags & GS_FLAG_Q_LIMITS)) {
+      run_batch(net, count, inst, o, results, vm_out, va_out, false);
+      return GS_OK;
+    }
+    std::vector<HostFinal> paused;
+    run_batch(net, count, inst, o, results, vm_out, va_out, false, &paused);
+    const BatchRun run = plan_run(net->dev.h, o, false);
+    const Layout L = instance_layout(net->dev.h, run.fp32, run.x_lds);
+    const int n = net->net.n;
+    for (const HostFinal& f : q_limit_passes(net, count, inst, o, results, L, paused)) {
+      if (vm_out) std::copy(f.vm.begin(), f.vm.end(), vm_out + (size_t)n * f.i);
+      if (va_out) std::copy(f.va.begin(), f.va.end(), va_out + (size_t)n * f.i);
+    }
     return GS_OK;
   });
 }
@@ -965,6 +1266,14 @@ GS_API gs_status gs_nr_screen_batch(gs_net* net, int32_t count, const gs_instanc
     const BatchRun run = plan_run(net->dev.h, o, false);
     const Layout L = instance_layout(net->dev.h, run.fp32, run.x_lds);
     const int n = net->net.n, nb = net->net.nb;
+    if (o.flags & GS_FLAG_Q_LIMITS) {
+      // the reactive-limit passes; the instances that ran again have their final state uploaded below
+      for (HostFinal& f : q_limit_passes(net, count, inst, o, results, L, finals)) {
+        if (vm_out) std::copy(f.vm.begin(), f.vm.end(), vm_out + (size_t)n * f.i);
+        if (va_out) std::copy(f.va.begin(), f.va.end(), va_out + (size_t)n * f.i);
+        finals.push_back(std::move(f));
+      }
+    }
     // instances the host finished: their final state back into their blocks, where the screen reads it
     for (const HostFinal& f : finals) {
       char* b = net->inst_buf.as<char>() + L.stride * (size_t)f.i;
@@ -1014,6 +1323,7 @@ GS_API gs_status gs_nr_solve_plain(gs_net* net, int32_t count, const gs_instance
     gs_nr_options o = normalise(opts);
     o.lu_precision = GS_LU_FP64;
     if ((o.flat_start != 0) != net->flat_start) fail(GS_INVALID_ARGUMENT, "flat_start differs from the upload");
+    if (o.flags & GS_FLAG_Q_LIMITS) fail(GS_INVALID_ARGUMENT, "the plain path does not run the reactive-limit passes");
     if (count == 0) return GS_OK;
     run_batch(net, count, inst, o, results, vm_out, va_out, true);
     return GS_OK;
@@ -1190,6 +1500,8 @@ GS_API gs_status gs_batch_create(gs_ctx* ctx, int32_t n_groups, gs_net* const* n
       if ((o.flat_start != 0) != net->flat_start) fail(GS_INVALID_ARGUMENT, "flat_start differs from the upload");
       if (o.schedule < 0)
         fail(GS_INVALID_ARGUMENT, "group " + std::to_string(gi) + ": batches launch a wavefront per instance");
+      if (o.flags & GS_FLAG_Q_LIMITS)
+        fail(GS_INVALID_ARGUMENT, "group " + std::to_string(gi) + ": reactive limits run through gs_nr_solve_batch");
       for (int i = 0; i < counts[gi]; i++)
         if (inst[gi][i].outage_branch < -1 || inst[gi][i].outage_branch >= net->net.nb ||
             !(inst[gi][i].load_scale > 0))
@@ -1245,6 +1557,7 @@ GS_API gs_status gs_nr_autotune(gs_net* net, int32_t count, const gs_instance* s
     if (count <= 0 || !sample || !opts) fail(GS_INVALID_ARGUMENT, "null argument");
     gs_nr_options base = normalise(opts);
     if ((base.flat_start != 0) != net->flat_start) fail(GS_INVALID_ARGUMENT, "flat_start differs from the upload");
+    if (base.flags & GS_FLAG_Q_LIMITS) fail(GS_INVALID_ARGUMENT, "the autotuner times single runs (no limit passes)");
     // verdicts of the FP64 LU: the reference every candidate must reproduce
     const size_t nv = (size_t)count * net->net.n;
     std::vector<gs_nr_result> ref(count), res(count);

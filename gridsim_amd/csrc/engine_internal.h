@@ -20,6 +20,8 @@ hipError_t launch_nr_batch(int lu_fp32, int x_in_lds, const InstDesc* descs, con
                            const NrParams& prm, size_t lds_bytes, hipStream_t stream);
 hipError_t launch_seam(const SeamDesc* descs, const DevPlan& plan, int count, size_t lds_bytes, hipStream_t stream);
 hipError_t launch_sincos(const double* x, double* s, double* c, int64_t n, hipStream_t stream);
+hipError_t launch_iterative_step(const InstDesc* descs, const DevPlan& plan, const NrParams& prm, double* work,
+                                 int* iw, hipStream_t stream);
 hipError_t launch_screen(const InstDesc* descs, const DevPlan& plan, int count, gs_screen_summary* out,
                          double* loading, double* flows, hipStream_t stream);
 hipError_t launch_wide_start(const InstDesc* descs, const DevPlan& plan, int count, WideCtl* ctl, hipStream_t stream);
@@ -99,7 +101,7 @@ struct DevicePlan {
   DevMem rs_rec, ch_a_src;
   DevMem fs_rec, bs_rec;
   DevMem rs_chunk_row, fs_pass_step, bs_pass_step;
-  DevMem br_ends, br_stamp, br_rate, bus_lo, bus_hi;
+  DevMem br_ends, br_stamp, br_rate, bus_lo, bus_hi, j_diag;
   DevPlan h{};   // passed to the kernels by value
   DevicePlanHost dph;
   std::unique_ptr<WideGraph> wide;
@@ -198,9 +200,11 @@ PreparedRun prepare_into(const DevPlan& h, const RunBuffers& rb, Staging& stg, i
 void launch_prepared(const DevPlan& h, const PreparedRun& pr, int schedule, hipStream_t s);
 // A paused instance (a static pivot failed its growth check) finished by fresh host symbolic passes and resumed
 // launches; vm / va (in and out) hold its state.
+// types / qspec (or null): bus types and Q specification other than the network's (a reactive-limit pass).
 void finish_paused(gs_net* net, const gs_instance& in, const gs_nr_options& o, bool plain, InstState st,
                    std::vector<double>& vm, std::vector<double>& va, gs_nr_result& result, double& kms, int& launches,
-                   std::vector<double>* p_out = nullptr, std::vector<double>* q_out = nullptr);
+                   std::vector<double>* p_out = nullptr, std::vector<double>* q_out = nullptr,
+                   const std::vector<int>* types = nullptr, const std::vector<double>* qspec = nullptr);
 // The branch data screening reads (end buses, stamps, rate A) and the buses' own voltage limits, uploaded with a
 // network.
 void upload_branches(DevicePlan& dp, const Network& net, std::vector<std::vector<double>>& keep, hipStream_t s);

@@ -76,6 +76,8 @@ Network copy_network(const gs_network& s) {
   if (s.br_rate_a) cp(n.br_rate, s.br_rate_a, n.nb); else n.br_rate.assign(n.nb, 0.0);
   if (s.bus_vmin) cp(n.bus_lo, s.bus_vmin, n.n); else n.bus_lo.assign(n.n, nan);
   if (s.bus_vmax) cp(n.bus_hi, s.bus_vmax, n.n); else n.bus_hi.assign(n.n, nan);
+  if (s.gen_qmin) cp(n.gen_qmin, s.gen_qmin, n.ng); else n.gen_qmin.assign(n.ng, -INFINITY);
+  if (s.gen_qmax) cp(n.gen_qmax, s.gen_qmax, n.ng); else n.gen_qmax.assign(n.ng, INFINITY);
   return n;
 }
 
@@ -152,13 +154,20 @@ BusSpec build_bus_spec(const Network& net, bool flat_start, double load_scale) {
   s.vm.assign(n, 0.0); s.va.assign(n, 0.0); s.pspec.assign(n, 0.0); s.qspec.assign(n, 0.0);
   vector<double> pg(n, 0.0), qg(n, 0.0), vg(n, 0.0);
   vector<char> has(n, 0);
+  s.qmin.assign(n, 0.0);
+  s.qmax.assign(n, 0.0);
   for (int g = 0; g < net.ng; g++) {
     int i = net.gen_bus[g];
     pg[i] += net.gen_pg[g] * load_scale;
     qg[i] += net.gen_qg[g];
+    s.qmin[i] += g < (int)net.gen_qmin.size() ? net.gen_qmin[g] : -INFINITY;
+    s.qmax[i] += g < (int)net.gen_qmax.size() ? net.gen_qmax[g] : INFINITY;
     vg[i] = net.gen_vg[g];
     has[i] = 1;
   }
+  s.vg = vg;
+  s.qd.resize(n);
+  for (int i = 0; i < n; i++) s.qd[i] = net.qd[i] * load_scale;
   for (int i = 0; i < n; i++) {
     int bt = net.bus_type[i];
     s.type[i] = (bt == GS_BUS_PV && !has[i]) ? GS_BUS_PQ : bt;

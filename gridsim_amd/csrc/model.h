@@ -47,6 +47,7 @@ struct Network {
   vector<int> gen_bus;
   vector<double> gen_pg, gen_qg, gen_vg;
   vector<double> br_rate, bus_lo, bus_hi;   // rate A (MVA, 0 = unrated); own voltage limits (NaN = the band)
+  vector<double> gen_qmin, gen_qmax;        // MVAr (-inf / +inf: none)
 };
 
 Network copy_network(const gs_network& src);
@@ -57,6 +58,7 @@ YBus build_ybus(const Network& net, int skip);
 struct BusSpec {
   vector<int> type;                       // effective types (a PV bus without a generator reads PQ)
   vector<double> vm, va, pspec, qspec;    // start magnitudes, start angles (rad), specified injections (pu)
+  vector<double> qd, vg, qmin, qmax;      // load MVAr, voltage set point, summed generator limits (MVAr)
 };
 BusSpec build_bus_spec(const Network& net, bool flat_start, double load_scale);
 

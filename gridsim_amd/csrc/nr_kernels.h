@@ -42,6 +42,7 @@ struct DevPlan {
   // Screening: per branch its end buses (from = -1: out of service), pi-model stamp (Yff, Yft, Ytf, Ytt as re, im)
   // and rate A; per bus its own voltage limits (NaN = the operational band).
   int32_t nb, pad_nb;
+  const int32_t* j_diag;              // per Jacobian row: CSR position of its diagonal (the iterative fallback)
   const int2* br_ends;
   const double* br_stamp;
   const double *br_rate, *bus_lo, *bus_hi;

@@ -844,6 +844,176 @@ __global__ __launch_bounds__(256) void wide_update(const InstDesc* __restrict__ 
   if (blockIdx.x == 0 && threadIdx.x == 0) st->iter = st->iter + 1;
 }
 
+// ------------------------------------------------------------------------------------------------ iterative step
+// NewtonShell.SolveStep's last resort (NewtonShell.cs:97-105): when the sparse LU refuses the Jacobian (the host's
+// fresh symbolic pass found it singular), the step is solved by BiCGSTAB with an ILU(0) preconditioner on the FP64
+// Jacobian (Sparse.cs:230-405: Ilu0 with the row-relative pivot guard, BiCGSTAB to min(1e-10, tol 1e-3) within
+// 4 m + 200 iterations, its breakdown rules and the true-residual check). One Newton iteration of one instance: the
+// mismatch and Jacobian (CSR, FP64) by the wave, the verdicts, then the factorisation, the Krylov loop and the damped
+// update by one lane in the reference's sequential order. A rare path (an island with nothing to balance, such as a
+// synchronous condenser cut off by its only branch), so one lane is enough. work: lu [nnz_j], then r, rhat, p, v, s,
+// t, ph, sh [m each]; iw: [m].
+__device__ void ilu0_apply_seq(const DevPlan& P, const double* lu, const double* r, double* z) {
+  for (int i = 0; i < P.m; i++) {
+    double s = r[i];
+    for (int k = P.j_rp[i]; k < P.j_diag[i]; k++) s -= lu[k] * z[P.j_ci[k]];
+    z[i] = s;
+  }
+  for (int i = P.m - 1; i >= 0; i--) {
+    double s = z[i];
+    for (int k = P.j_diag[i] + 1; k < P.j_rp[i + 1]; k++) s -= lu[k] * z[P.j_ci[k]];
+    z[i] = s / lu[P.j_diag[i]];
+  }
+}
+
+__device__ void csr_mul_seq(const DevPlan& P, const double* a, const double* x, double* y) {
+  for (int i = 0; i < P.m; i++) {
+    double s = 0;
+    for (int k = P.j_rp[i]; k < P.j_rp[i + 1]; k++) s += a[k] * x[P.j_ci[k]];
+    y[i] = s;
+  }
+}
+
+__device__ double dot_seq(int n, const double* a, const double* b) {
+  double s = 0;
+  for (int i = 0; i < n; i++) s += a[i] * b[i];
+  return s;
+}
+
+// Returns whether BiCGSTAB produced x (A x = b to the tolerance).
+__device__ bool ilu0_bicgstab(const DevPlan& P, const double* a, const double* b, double* x, double tol, double* work,
+                              int* iw) {
+  const int n = P.m;
+  double* lu = work;
+  double *r = lu + P.nnz_j, *rhat = r + n, *p = rhat + n, *v = p + n, *s = v + n, *t = s + n, *ph = t + n, *sh = ph + n;
+  // ILU(0) (Sparse.cs:230-300)
+  for (int k = 0; k < P.nnz_j; k++) lu[k] = a[k];
+  for (int i = 0; i < n; i++) iw[i] = -1;
+  for (int i = 0; i < n; i++) {
+    for (int k = P.j_rp[i]; k < P.j_rp[i + 1]; k++) iw[P.j_ci[k]] = k;
+    for (int k = P.j_rp[i]; k < P.j_diag[i]; k++) {
+      const int j = P.j_ci[k];
+      const double piv = lu[P.j_diag[j]];
+      const double lij = (piv != 0.0) ? lu[k] / piv : 0.0;
+      lu[k] = lij;
+      if (lij == 0.0) continue;
+      for (int kk = P.j_diag[j] + 1; kk < P.j_rp[j + 1]; kk++) {
+        const int pos = iw[P.j_ci[kk]];
+        if (pos != -1) lu[pos] -= lij * lu[kk];
+      }
+    }
+    double row_scale = 0.0;
+    for (int k = P.j_rp[i]; k < P.j_rp[i + 1]; k++) { const double av = fabs(a[k]); if (av > row_scale) row_scale = av; }
+    const double guard = 1e-12 * (row_scale > 0.0 ? row_scale : 1.0);
+    const double d = lu[P.j_diag[i]];
+    if (fabs(d) < guard) lu[P.j_diag[i]] = d < 0 ? -guard : guard;
+    for (int k = P.j_rp[i]; k < P.j_rp[i + 1]; k++) iw[P.j_ci[k]] = -1;
+  }
+  // BiCGSTAB (Sparse.cs:302-405)
+  for (int i = 0; i < n; i++) { x[i] = 0.0; r[i] = b[i]; rhat[i] = b[i]; p[i] = 0; v[i] = 0; }
+  const double bnorm = sqrt(dot_seq(n, b, b));
+  if (bnorm == 0) {
+    for (int i = 0; i < n; i++) if (b[i] != 0.0) return false;
+    return true;
+  }
+  auto true_residual = [&]() {
+    csr_mul_seq(P, a, x, t);
+    double sum = 0.0;
+    for (int i = 0; i < n; i++) { const double d = b[i] - t[i]; sum += d * d; }
+    return sqrt(sum) / bnorm < tol;
+  };
+  double rho = 1, alpha = 1, omega = 1;
+  const int max_iter = 4 * n + 200;
+  for (int it = 0; it < max_iter; it++) {
+    const double rho_new = dot_seq(n, rhat, r);
+    if (fabs(rho_new) < 1e-300 || isnan(rho_new)) return false;
+    const double beta = (rho_new / rho) * (alpha / omega);
+    for (int i = 0; i < n; i++) p[i] = r[i] + beta * (p[i] - omega * v[i]);
+    ilu0_apply_seq(P, lu, p, ph);
+    csr_mul_seq(P, a, ph, v);
+    const double rhat_v = dot_seq(n, rhat, v);
+    if (rhat_v == 0.0 || isnan(rhat_v)) return false;
+    alpha = rho_new / rhat_v;
+    for (int i = 0; i < n; i++) s[i] = r[i] - alpha * v[i];
+    if (sqrt(dot_seq(n, s, s)) / bnorm < tol) {
+      for (int i = 0; i < n; i++) x[i] += alpha * ph[i];
+      return true_residual();
+    }
+    ilu0_apply_seq(P, lu, s, sh);
+    csr_mul_seq(P, a, sh, t);
+    const double tt = dot_seq(n, t, t);
+    omega = tt > 0 ? dot_seq(n, t, s) / tt : 0.0;
+    for (int i = 0; i < n; i++) { x[i] += alpha * ph[i] + omega * sh[i]; r[i] = s[i] - omega * t[i]; }
+    rho = rho_new;
+    if (sqrt(dot_seq(n, r, r)) / bnorm < tol) return true_residual();
+    if (omega == 0) return false;
+  }
+  return false;
+}
+
+__global__ __launch_bounds__(kWave) void iterative_step_kernel(const InstDesc* __restrict__ descs, const DevPlan P,
+                                                               const NrParams prm, double* work, int* iw) {
+  const int lane = threadIdx.x;
+  const Inst<double> I = load_inst<double>(descs, 0);
+  InstState* st = descs[0].state;
+  if (prm.load_start) {
+    for (int i = lane; i < P.n; i += kWave) { I.vm[i] = I.vm_start[i]; I.va[i] = I.va_start[i]; }
+    WG_SYNC();
+  }
+  int bad;
+  const double max_mis = mismatch_jacobian<double>(P, I, opaque_lane(lane), true, &bad);
+  WG_SYNC();
+  if (lane != 0) return;
+  const int iter = st->iter;
+  if (iter == 0) st->start_mis = max_mis;
+  st->max_mis = max_mis;
+  int finish = 0, converged = 0, diverged = 0;
+  if (max_mis < prm.tolerance) { finish = 1; converged = 1; }
+  else if (max_mis > nan_max(prm.diverge_limit, st->start_mis)) { finish = 1; diverged = 1; }
+  else if (iter >= prm.max_iterations) finish = 1;
+  else if (bad || !isfinite(max_mis)) finish = 1;
+  if (!finish) {
+    double* dx = descs[0].dx;
+    st->step_failures++;   // the LU refused this step
+    const double tol = fmin(1e-10, prm.tolerance * 1e-3);
+    if (!ilu0_bicgstab(P, descs[0].jv, descs[0].f, dx, tol, work, iw)) {
+      finish = 1;          // NewtonShell.SolveStep false: the run stops
+    } else {
+      double max_abs = 0;
+      bool finite = true;
+      for (int k = 0; k < P.m; k++) {
+        if (!isfinite(dx[k])) { finite = false; break; }
+        const double av = fabs(dx[k]);
+        if (av > max_abs) max_abs = av;
+      }
+      if (!finite) {
+        finish = 1;
+      } else {
+        const double damp = max_abs > 0.4 ? 0.4 / max_abs : 1.0;
+        for (int i = 0; i < P.n; i++) {
+          if (P.ppos[i] >= 0) I.va[i] = I.va[i] + damp * dx[P.ppos[i]];
+          if (P.qpos[i] >= 0) {
+            const double v = I.vm[i] + damp * dx[P.qpos[i]];
+            I.vm[i] = v < 0.01 ? 0.01 : v;
+          }
+        }
+        st->iter = iter + 1;
+      }
+    }
+  }
+  if (finish) {
+    int nf = !isfinite(max_mis);
+    for (int i = 0; i < P.n; i++)
+      nf |= !(isfinite(I.vm[i]) && isfinite(I.va[i]) && isfinite(I.p[i] * P.base_mva) && isfinite(I.q[i] * P.base_mva));
+    st->converged = converged;
+    st->diverged = diverged;
+    st->nonfinite = nf;
+    st->done = 1;
+  } else {
+    st->done = 0;
+  }
+}
+
 // ------------------------------------------------------------------------------------------------ screening
 // Line flows and the classification numbers of every instance's solved state (one work-group of 256 per instance):
 // NewtonShell.ComputeLineFlows (NewtonShell.cs:699-734) per branch, the maximum loading and the overloads
@@ -1083,6 +1253,12 @@ hipError_t launch_wide_step(int lu_fp32, const InstDesc* descs, const DevPlan& p
   else
     enqueue_wide_step<double>(descs, plan, count, prm, ctl, level_chunks, fs_level_pass, bs_level_pass, ws_bytes,
                               stream);
+  return hipGetLastError();
+}
+
+hipError_t launch_iterative_step(const InstDesc* descs, const DevPlan& plan, const NrParams& prm, double* work,
+                                 int* iw, hipStream_t stream) {
+  hipLaunchKernelGGL(iterative_step_kernel, dim3(1), dim3(kWave), 0, stream, descs, plan, prm, work, iw);
   return hipGetLastError();
 }
 

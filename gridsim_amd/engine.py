@@ -31,6 +31,7 @@ GS_OK, GS_SINGULAR, GS_NONFINITE_INPUT, GS_OUT_OF_MEMORY, GS_INVALID_ARGUMENT, G
 CONVERGED, MAX_ITERATIONS, DIVERGED, NONFINITE = 0, 1, 2, 3
 LU_FP64, LU_FP32 = 0, 1
 FLAG_X_GLOBAL = 1
+FLAG_Q_LIMITS = 2      # enforce generator reactive limits (the outer loop of PV -> PQ switching)
 SCHEDULE_LEVELS = -1   # every instance spread over the whole GPU, level by level (GS_SCHEDULE_LEVELS)
 STATUS_NAMES = {0: "Converged", 1: "MaxIterations", 2: "Diverged", 3: "NonFinite"}
 
@@ -49,7 +50,8 @@ class GsNetwork(C.Structure):
                 ("br_b", C.c_void_p), ("br_tap", C.c_void_p), ("br_shift_deg", C.c_void_p),
                 ("br_in_service", C.c_void_p), ("gen_bus", C.c_void_p), ("gen_pg", C.c_void_p),
                 ("gen_qg", C.c_void_p), ("gen_vg", C.c_void_p), ("br_rate_a", C.c_void_p),
-                ("bus_vmin", C.c_void_p), ("bus_vmax", C.c_void_p)]
+                ("bus_vmin", C.c_void_p), ("bus_vmax", C.c_void_p), ("gen_qmin", C.c_void_p),
+                ("gen_qmax", C.c_void_p)]
 
 
 class GsNrOptions(C.Structure):
@@ -144,7 +146,8 @@ class NetView:
                            _p(net.qd), _p(net.gs), _p(net.bs), _p(net.vm), _p(net.va_deg), _p(net.br_from),
                            _p(net.br_to), _p(net.br_r), _p(net.br_x), _p(net.br_b), _p(net.br_tap),
                            _p(net.br_shift_deg), _p(net.br_in_service), _p(net.gen_bus), _p(net.gen_pg),
-                           _p(net.gen_qg), _p(net.gen_vg), _p(net.br_rate_a), _p(net.bus_vmin), _p(net.bus_vmax))
+                           _p(net.gen_qg), _p(net.gen_vg), _p(net.br_rate_a), _p(net.bus_vmin), _p(net.bus_vmax),
+                           _p(net.gen_qmin), _p(net.gen_qmax))
 
     @property
     def ref(self):

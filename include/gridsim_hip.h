@@ -67,6 +67,8 @@ typedef struct gs_network {
    * limits in pu (NaN = none: the operational band 0.94 / 1.06 applies on that side; Bus.VminPu / VmaxPu). */
   const double* br_rate_a;
   const double *bus_vmin, *bus_vmax;
+  /* Generator reactive limits in MVAr (may be null: none), for GS_FLAG_Q_LIMITS (Generator.Qmin / Qmax). */
+  const double *gen_qmin, *gen_qmax;
 } gs_network;
 
 /* PowerFlowOptions (src/GridSim.Core/PowerFlow/PowerFlowOptions.cs:6-170) for the engine. Q limits and the
@@ -88,6 +90,14 @@ typedef struct gs_nr_options {
 /* gs_nr_options.flags: keep the triangular-solve vector in global memory even where it fits in LDS (smaller LDS per
  * instance, so more instances resident per compute unit). */
 enum { GS_FLAG_X_GLOBAL = 1 };
+
+/* gs_nr_options.flags: enforce generator reactive limits (PowerFlowOptions.EnforceQLimits, the outer loop of
+ * SparseNewtonRaphsonPowerFlow.SolveAcAttempt :118-160 with NewtonShell.ApplyQLimits / RevertBrokenPins :519-595,
+ * RevertQLimits off, at most 100 passes as MaxQLimitPasses). After a converged pass the host switches each PV bus
+ * whose generators' reactive output crossed a limit to PQ at that limit (or back, once, when its voltage shows the
+ * limit no longer binds), groups the instances by their bus types and solves them again on the device from their
+ * current state; iterations add up over the passes. gs_nr_solve_batch and gs_nr_screen_batch only. */
+enum { GS_FLAG_Q_LIMITS = 2 };
 
 /* gs_nr_options.schedule: level-parallel. Each Newton step is a captured hipGraph of short kernels, one per phase and
  * one per elimination level of the refactor and of each triangular solve, a work-group per chunk of columns (per pass

@@ -387,3 +387,82 @@ def test_solve_export_gpu_against_cpu_the_reference_comparer_way(ctx, eng, oracl
         print(f"COMPARE {name}: max|dVm| = {dvm:.3g} pu, max|dVa| = {dva:.3g} deg", flush=True)
         assert ok and dvm <= 1e-9 and dva <= 1e-7
         assert abs(a["result"]["totalLossMw"] - b["result"]["totalLossMw"]) <= 1e-6
+
+
+def test_reactive_limits_case14_n1_matches_oracle_and_the_reference_cli_fixture(ctx, eng, oracle):
+    """GS_FLAG_Q_LIMITS on case14's N-1 screen (where generators hit their limits): every outage's verdict, total
+    iterations and voltages equal the oracle's reactive-limit loop (|dV| <= 1e-9), and the screen prints the
+    reference's nminus1-case14 rows (1-2 NO SOLUTION; Vmin / Vmax to 3 decimals)."""
+    from test_oracle import _parse_nminus1
+    net = matpower_case("case14")
+    o = eng.NrOptions(max_iterations=100, lu_precision=eng.LU_FP64, refine_steps=0, flags=eng.FLAG_Q_LIMITS)
+    dn = eng.DeviceNetwork(ctx, net, o)
+    outs = np.arange(net.n_branch)
+    res, summ, _, _ = dn.screen(outages=outs, keep_state=True)
+    for k in outs:
+        rec, vm, va = oracle.solve(net, oracle.options(max_iterations=100, q_limits=True), skip=int(k))
+        assert rec["status"] == res.status[k], k
+        assert rec["iterations"] == res.results["iterations"][k], k
+        if rec["status"] == 0:
+            assert np.abs(vm - res.vm[k]).max() <= TOL_DV and np.abs(va - res.va[k]).max() <= TOL_DV, k
+            s = oracle.screen_summary(net, vm, va, skip=int(k))
+            assert abs(summ[k]["vmin"] - s["vmin"]) <= TOL_DV and abs(summ[k]["loss_mw"] - s["loss_mw"]) <= 1e-6
+    ids = list(net.bus_ids)
+    for label, status, maxload, vmin, vmax in _parse_nminus1(os.path.join(GOLDEN, "cli", "nminus1-case14.txt")):
+        a, b = (int(x) for x in label.split("-"))
+        k = [i for i in range(net.n_branch) if (ids[net.br_from[i]], ids[net.br_to[i]]) == (a, b)][0]
+        if status == "NO SOLUTION":
+            assert res.status[k] != eng.CONVERGED, label
+            continue
+        assert (f"{summ[k]['vmin']:.3f}", f"{summ[k]['vmax']:.3f}") == (vmin, vmax), label
+    dn.close()
+
+
+def test_iterative_fallback_when_the_lu_refuses_the_step(ctx, eng, oracle):
+    """case14's outage of 7-8 cuts the synchronous condenser at bus 8 off: the Jacobian is singular, the LU refuses
+    the step and NewtonShell.SolveStep's ILU(0) + BiCGSTAB fallback takes it (on the device). Every outage's verdict,
+    iterations, step failures and voltages (|dV| <= 1e-9) equal the oracle's, and 7-8 does take the fallback."""
+    net = matpower_case("case14")
+    o = eng.NrOptions(max_iterations=100, lu_precision=eng.LU_FP64, refine_steps=0)
+    dn = eng.DeviceNetwork(ctx, net, o)
+    outs = np.arange(net.n_branch)
+    r = dn.solve(outages=outs)
+    fallback = 0
+    for k in outs:
+        rec, vm, va = oracle.solve(net, oracle.options(max_iterations=100), skip=int(k))
+        print(f"outage {k}: oracle {rec['status']} / {rec['iterations']} / {rec['step_failures']}, gpu "
+              f"{r.status[k]} / {r.results['iterations'][k]} / {r.results['step_failures'][k]}", flush=True)
+        assert rec["status"] == r.status[k], k
+        assert rec["iterations"] == r.results["iterations"][k], k
+        assert rec["step_failures"] == r.results["step_failures"][k], k
+        if rec["status"] == 0:
+            assert np.abs(vm - r.vm[k]).max() <= TOL_DV and np.abs(va - r.va[k]).max() <= TOL_DV, k
+        fallback += int(rec["step_failures"] > 0)
+    assert fallback > 0
+    dn.close()
+
+
+def test_reactive_limits_synthetic_batch_matches_oracle(ctx, eng, oracle, n1_118):
+    """The 118-bus N-1 batch with every generator's reactive range cut to +-60 % of its base-case output (so outages
+    switch buses to PQ, in different patterns per outage): verdicts, iterations and voltages equal the oracle's."""
+    warm, outs, _ = n1_118
+    rec, vm, va = oracle.solve(warm)
+    p, q, _ = oracle.mismatch(warm, vm, va)
+    qgen = q[warm.gen_bus] * warm.base_mva + warm.qd[warm.gen_bus]
+    lim = 0.6 * np.abs(qgen) + 5.0
+    tight = warm.copy(gen_qmin=-lim, gen_qmax=lim)
+    o = eng.NrOptions(max_iterations=30, lu_precision=eng.LU_FP64, refine_steps=0, flags=eng.FLAG_Q_LIMITS)
+    dn = eng.DeviceNetwork(ctx, tight, o)
+    r = dn.solve(outages=outs, options=o)
+    switched = 0
+    for j, k in enumerate(outs):
+        rec, vm, va = oracle.solve(tight, oracle.options(max_iterations=30, q_limits=True), skip=int(k))
+        assert rec["status"] == r.status[j], k
+        assert rec["iterations"] == r.results["iterations"][j], k
+        if rec["status"] == 0:
+            assert np.abs(vm - r.vm[j]).max() <= TOL_DV and np.abs(va - r.va[j]).max() <= TOL_DV, k
+        plain, _, _ = oracle.solve(tight, oracle.options(max_iterations=30), skip=int(k))
+        switched += int(plain["iterations"] != rec["iterations"])
+    print(f"{switched} of {len(outs)} outages ran limit passes", flush=True)
+    assert switched > 0
+    dn.close()
