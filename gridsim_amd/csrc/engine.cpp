@@ -547,6 +547,15 @@ void iterative_step(gs_ctx* ctx, const YBus& y, const JacobianPattern& jp, const
                     const gs_nr_options& o, InstState& st, std::vector<double>& vm, std::vector<double>& va,
                     std::vector<double>* p_out, std::vector<double>* q_out) {
   hipStream_t s = ctx->stream;
+  for (int r = 0; r < jp.m; r++) {
+    bool diag = false;
+    for (int k = jp.rp[r]; k < jp.rp[r + 1]; k++) diag |= jp.ci[k] == r;
+    if (!diag) {  // ILU(0) needs every diagonal entry: the step fails and the run stops, as SolveStep's false
+      st.done = 1;
+      st.step_failures++;
+      return;
+    }
+  }
   DevicePlan dp;
   upload_pattern(dp, y, jp, base, s);
   const size_t n = (size_t)y.n, m = (size_t)jp.m, nj = (size_t)jp.rp[jp.m], ny = (size_t)y.rp[y.n];

@@ -71,8 +71,8 @@ typedef struct gs_network {
   const double *gen_qmin, *gen_qmax;
 } gs_network;
 
-/* PowerFlowOptions (src/GridSim.Core/PowerFlow/PowerFlowOptions.cs:6-170) for the engine. Q limits and the
- * DC-start retry are off in every mode this engine runs (EnforceQLimits = false, RetryFromDcStart = false). */
+/* PowerFlowOptions (src/GridSim.Core/PowerFlow/PowerFlowOptions.cs:6-170) for the engine. Reactive limits are
+ * GS_FLAG_Q_LIMITS (EnforceQLimits; off by default); the DC-start retry is off (RetryFromDcStart = false). */
 typedef struct gs_nr_options {
   double tolerance;         /* PowerFlowOptions.Tolerance, pu on ||mismatch||_inf (1e-8) */
   int32_t max_iterations;   /* PowerFlowOptions.MaxIterations (30) */
@@ -84,7 +84,7 @@ typedef struct gs_nr_options {
   int32_t schedule;         /* 0 = a wavefront per instance, all instances of a call in one launch; k > 0 = the
                                same in launches of at most k instances one after another; GS_SCHEDULE_LEVELS = every
                                instance spread over the whole GPU, level by level (gs_nr_autotune chooses) */
-  int32_t flags;            /* GS_FLAG_*: placement choices (gs_nr_autotune chooses) */
+  int32_t flags;            /* GS_FLAG_*: placement choices (gs_nr_autotune chooses) and reactive limits */
 } gs_nr_options;
 
 /* gs_nr_options.flags: keep the triangular-solve vector in global memory even where it fits in LDS (smaller LDS per

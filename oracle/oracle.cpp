@@ -1194,6 +1194,38 @@ int32_t orc_sparse_solve(int32_t n, const int32_t* rowptr, const int32_t* colidx
   return s.solve(a, b, x);
 }
 
+static Csr csr_with_diag(int32_t n, const int32_t* rowptr, const int32_t* colidx, const double* val) {
+  Csr a;
+  a.n = n;
+  a.rp.assign(rowptr, rowptr + n + 1);
+  a.ci.assign(colidx, colidx + rowptr[n]);
+  a.val.assign(val, val + rowptr[n]);
+  a.diag.assign(n, -1);
+  for (int i = 0; i < n; i++)
+    for (int k = a.rp[i]; k < a.rp[i + 1]; k++)
+      if (a.ci[k] == i) a.diag[i] = k;
+  return a;
+}
+
+int32_t orc_ilu0(int32_t n, const int32_t* rowptr, const int32_t* colidx, const double* val, double* lu_out) {
+  Csr a = csr_with_diag(n, rowptr, colidx, val);
+  Ilu0 pc;
+  if (!ilu0_factor(a, pc)) return 0;
+  std::copy(pc.lu.begin(), pc.lu.end(), lu_out);
+  return 1;
+}
+
+int32_t orc_bicgstab(int32_t n, const int32_t* rowptr, const int32_t* colidx, const double* val, const double* b,
+                     double tol, int32_t max_iter, double* x) {
+  Csr a = csr_with_diag(n, rowptr, colidx, val);
+  Ilu0 pc;
+  if (!ilu0_factor(a, pc)) return 0;
+  vector<double> bv(b, b + n), xv(n, 0.0);
+  const bool ok = bicgstab(a, bv, xv, pc, tol, max_iter);
+  std::copy(xv.begin(), xv.end(), x);
+  return ok ? 1 : 0;
+}
+
 void orc_solve(const orc_network* net, int32_t skip, const orc_options* opt, double* vm_out, double* va_out,
                orc_result* res) {
   solve_one(*net, skip, *opt, vm_out, va_out, res);

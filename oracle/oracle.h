@@ -98,6 +98,12 @@ void orc_lu_free(orc_lu* h);
 /* CpuSparseLuSolver.Solve: returns 0 on success, 1 singular (null), 2 non-finite right-hand side. */
 int32_t orc_sparse_solve(int32_t n, const int32_t* rowptr, const int32_t* colidx, const double* val,
                          const double* b, double* x);
+/* ILU(0) of a CSR matrix with sorted columns (Sparse.cs:230-300): lu_out gets the factors in the pattern's positions
+   (L below the diagonal, unit diagonal implied; U from the diagonal on). 0 when a row has no diagonal entry. */
+int32_t orc_ilu0(int32_t n, const int32_t* rowptr, const int32_t* colidx, const double* val, double* lu_out);
+/* BiCGSTAB preconditioned by that ILU(0) (Sparse.cs:302-405): 1 when x solves A x = b to tol (true residual). */
+int32_t orc_bicgstab(int32_t n, const int32_t* rowptr, const int32_t* colidx, const double* val, const double* b,
+                     double tol, int32_t max_iter, double* x);
 
 /* One power-flow solve (SparseNewtonRaphsonPowerFlow.Solve with Q limits and the DC retry off).
  * vm_out / va_out (rad) receive the final state, may be null. */

@@ -221,6 +221,29 @@ def sparse_solve(rowptr, colidx, val, b):
     return rc, x
 
 
+def ilu0(rowptr, colidx, val):
+    """ILU(0) factors in the pattern's positions, or None when a row has no diagonal entry."""
+    rowptr = np.ascontiguousarray(rowptr, np.int32)
+    colidx = np.ascontiguousarray(colidx, np.int32)
+    val = np.ascontiguousarray(val, np.float64)
+    out = np.zeros_like(val)
+    ok = lib().orc_ilu0(C.c_int32(len(rowptr) - 1), _p(rowptr), _p(colidx), _p(val), _p(out))
+    return out if ok else None
+
+
+def bicgstab(rowptr, colidx, val, b, tol=1e-10, max_iter=None):
+    """(solved, x) of ILU(0)-preconditioned BiCGSTAB; max_iter defaults to SolveStep's 4 n + 200."""
+    rowptr = np.ascontiguousarray(rowptr, np.int32)
+    colidx = np.ascontiguousarray(colidx, np.int32)
+    n = len(rowptr) - 1
+    val = np.ascontiguousarray(val, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    x = np.zeros(n)
+    ok = lib().orc_bicgstab(C.c_int32(n), _p(rowptr), _p(colidx), _p(val), _p(b), C.c_double(tol),
+                            C.c_int32(4 * n + 200 if max_iter is None else max_iter), _p(x))
+    return bool(ok), x
+
+
 def solve(net, opts=None, skip=-1):
     """One solve: (result record, vm, va in radians)."""
     opts = opts or options()

@@ -381,6 +381,88 @@ def test_row_relative_pivot_floor(oracle):
     assert rc == 2
 
 
+def _unpack_ilu(A, lu):
+    """Dense unit-lower L and upper U from ILU(0) factors stored in A's pattern."""
+    n = A.shape[0]
+    L, U = np.eye(n), np.zeros((n, n))
+    for i in range(n):
+        for k in range(A.indptr[i], A.indptr[i + 1]):
+            j = A.indices[k]
+            if j < i:
+                L[i, j] = lu[k]
+            else:
+                U[i, j] = lu[k]
+    return L, U
+
+
+def test_ilu0_reproduces_the_matrix_on_its_pattern(oracle):
+    """ILU(0) (Sparse.cs:230-300): L U equals A at every position of A's pattern (the defining property of ILU(0));
+    on a banded matrix, whose exact LU has no fill, it is the exact LU, so L U = A everywhere."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(77)
+    for n in (30, 120):
+        A = _random_system(rng, n, density=0.04)
+        A = (A + 4 * sp.eye(n) * np.abs(A).max()).tocsr()
+        A.sort_indices()
+        lu = oracle.ilu0(A.indptr, A.indices, A.data)
+        L, U = _unpack_ilu(A, lu)
+        LU = L @ U
+        D = A.toarray()
+        mask = A.toarray() != 0
+        assert np.abs(LU - D)[mask].max() <= 1e-12 * np.abs(D).max()
+    n = 60
+    B = sp.diags([rng.uniform(-1, 1, n - 2), rng.uniform(-1, 1, n - 1), rng.uniform(5, 6, n),
+                  rng.uniform(-1, 1, n - 1), rng.uniform(-1, 1, n - 2)], [-2, -1, 0, 1, 2], format="csr")
+    B.sort_indices()
+    L, U = _unpack_ilu(B, oracle.ilu0(B.indptr, B.indices, B.data))
+    assert np.abs(L @ U - B.toarray()).max() <= 1e-13
+    # a row without a diagonal entry: ILU(0) is refused (SolveStep then fails the step)
+    rp = np.array([0, 1, 2], np.int32)
+    assert oracle.ilu0(rp, np.array([1, 0], np.int32), np.array([1.0, 1.0])) is None
+
+
+def test_ilu0_pivot_guard(oracle):
+    """A zero (or tiny) pivot is replaced by +-1e-12 times the row's largest |a|, keeping its sign (1.0 for zero)."""
+    rp = np.array([0, 2, 4], np.int32)
+    ci = np.array([0, 1, 0, 1], np.int32)
+    lu = oracle.ilu0(rp, ci, np.array([2.0, 4.0, 1.0, 2.0]))          # U[1,1] = 2 - 0.5 * 4 = 0
+    assert lu[3] == 1e-12 * 2.0 and lu[2] == 0.5
+    lu = oracle.ilu0(rp, ci, np.array([2.0, 4.0, 1.0, 2.0 - 1e-15]))  # -1e-15 < guard: negative guard
+    assert lu[3] == -1e-12 * (2.0 - 1e-15)                              # the row's largest |a| sets the guard
+    lu = oracle.ilu0(rp, ci, np.array([0.0, 3.0, 5.0, 1.0]))          # first pivot 0: guard 1e-12 * 3, L = 5 / guard
+    assert lu[0] == 3e-12 and lu[2] == 5.0 / 3e-12
+
+
+def test_bicgstab_solves_to_its_tolerance(oracle):
+    """BiCGSTAB (Sparse.cs:302-405) with the ILU(0) preconditioner: on random non-symmetric systems x solves A x = b
+    to the relative true residual it was given, and agrees with a direct solve; with an exact preconditioner (a
+    banded matrix) it converges at once; b = 0 gives x = 0; a singular system is refused."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(91)
+    for n in (40, 200, 600):
+        A = _random_system(rng, n, density=0.02)
+        A = (A + 3 * sp.eye(n)).tocsr()
+        A.sort_indices()
+        b = rng.normal(size=n)
+        ok, x = oracle.bicgstab(A.indptr, A.indices, A.data, b, tol=1e-10)
+        assert ok, n
+        assert np.linalg.norm(A @ x - b) / np.linalg.norm(b) < 1e-10
+        np.testing.assert_allclose(x, np.linalg.solve(A.toarray(), b), rtol=1e-7, atol=1e-9)
+    n = 50
+    B = sp.diags([rng.uniform(-1, 1, n - 1), rng.uniform(4, 5, n), rng.uniform(-1, 1, n - 1)], [-1, 0, 1],
+                 format="csr")
+    B.sort_indices()
+    b = rng.normal(size=n)
+    ok, x = oracle.bicgstab(B.indptr, B.indices, B.data, b, tol=1e-12, max_iter=1)
+    assert ok and np.abs(x - np.linalg.solve(B.toarray(), b)).max() < 1e-12
+    ok, x = oracle.bicgstab(B.indptr, B.indices, B.data, np.zeros(n))
+    assert ok and not x.any()
+    rp = np.array([0, 2, 4], np.int32)
+    ci = np.array([0, 1, 0, 1], np.int32)
+    ok, _ = oracle.bicgstab(rp, ci, np.array([1.0, 2.0, 2.0, 4.0]), np.array([1.0, 0.0]))
+    assert not ok
+
+
 # ---------------------------------------------------------------- the Newton loop
 
 
