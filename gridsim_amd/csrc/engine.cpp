@@ -151,6 +151,15 @@ gs_plan_stats plan_stats(const DevicePlanHost& d, const YBus& y, const JacobianP
   b += rec_row * (int64_t)(st.forward_steps + 2 * kStreamPad + st.back_steps + 2 * kStreamPad);
   b += 4 * (int64_t)(d.rs_chunk_row.size() + d.fs_pass_step.size() + d.bs_pass_step.size());
   st.stream_bytes = b;
+  for (int r = 0; r < d.n_rs; r++) {
+    const int kind = d.rs_meta[r] & DevicePlanHost::kRsKindMask;
+    if (kind < 5) st.rows_by_kind[kind]++;
+    if (kind != DevicePlanHost::kRsUpdate) continue;
+    for (int l = 0; l < 64; l++)
+      st.update_lanes += (d.rs_rec[((size_t)r * 64 + l) * 4] & 0xFFFF) != d.spare(l);
+  }
+  for (size_t k = 0; k < d.fs_rec.size(); k += 4) st.forward_lanes += d.fs_rec[k] >= 0;
+  for (size_t k = 0; k < d.bs_rec.size(); k += 4) st.back_lanes += d.bs_rec[k] >= 0;
   return st;
 }
 
@@ -180,7 +189,7 @@ gs_nr_options normalise(const gs_nr_options* o) {
   r.schedule = 0;
   r.flags = 0;
   if (o) r = *o;
-  if (r.flags & ~(GS_FLAG_X_GLOBAL | GS_FLAG_Q_LIMITS)) fail(GS_INVALID_ARGUMENT, "unknown flags");
+  if (r.flags & ~(GS_FLAG_X_GLOBAL | GS_FLAG_Q_LIMITS | GS_FLAG_SINGLE_WAVE)) fail(GS_INVALID_ARGUMENT, "unknown flags");
   if (r.schedule < GS_SCHEDULE_LEVELS) fail(GS_INVALID_ARGUMENT, "schedule must be >= 0 or GS_SCHEDULE_LEVELS");
   if (!(r.tolerance > 0)) fail(GS_INVALID_ARGUMENT, "tolerance must be positive");
   if (r.max_iterations < 0) fail(GS_INVALID_ARGUMENT, "max_iterations must be >= 0");
@@ -246,12 +255,23 @@ BatchRun plan_run(const DevPlan& h, const gs_nr_options& o, bool force_fp64) {
   BatchRun r;
   r.fp32 = !force_fp64 && o.lu_precision == GS_LU_FP32;
   size_t t = r.fp32 ? 4 : 8;
-  size_t ws = t * (size_t)kTracks * (size_t)h.wstride;
+  size_t ws = t * (size_t)ws_slots(h);
   size_t x = t * (size_t)h.m;
   // (the level-parallel schedule shares the solve vector between work-groups: it lives in global memory)
   const bool levels = !force_fp64 && o.schedule == GS_SCHEDULE_LEVELS;
   r.x_lds = !levels && !(o.flags & GS_FLAG_X_GLOBAL) && ws + x <= kLdsLimit;
   r.lds_bytes = ws + (r.x_lds ? x : 0);
+  // Two wavefronts per work-group sharing the solve vector, where that keeps more instances resident per compute
+  // unit (at most kWavesPerCu, the register limit).
+  if (r.x_lds && !force_fp64 && !(o.flags & GS_FLAG_SINGLE_WAVE)) {
+    const size_t pair_bytes = 2 * ws + x + 16;
+    const size_t single_res = std::min<size_t>(kWavesPerCu, kLdsPerCu / r.lds_bytes);
+    const size_t pair_res = std::min<size_t>(kWavesPerCu, 2 * (kLdsPerCu / pair_bytes));
+    if (pair_bytes <= kLdsLimit && pair_res > single_res) {
+      r.pair = true;
+      r.lds_bytes = pair_bytes;
+    }
+  }
   return r;
 }
 
@@ -394,7 +414,7 @@ void launch_prepared(const DevPlan& h, const PreparedRun& pr, int schedule, hipS
   if (schedule < 0) fail(GS_INVALID_ARGUMENT, "the level-parallel schedule runs one call at a time (gs_nr_solve_batch)");
   const int slice = schedule > 0 ? schedule : pr.count;
   for (int first = 0; first < pr.count; first += slice)
-    check(launch_nr_batch(pr.run.fp32, pr.run.x_lds, pr.desc + first, h,
+    check(launch_nr_batch(pr.run.fp32, pr.run.x_lds, pr.run.pair, pr.desc + first, h,
                           std::min(slice, pr.count - first), pr.prm, pr.run.lds_bytes, s),
           "nr kernel");
 }
@@ -420,7 +440,7 @@ int launch_wide(DevicePlan& dp, const PreparedRun& pr, hipStream_t s) {
     if (wg.exec) { (void)hipGraphExecDestroy(wg.exec); wg.exec = nullptr; }
     if (wg.graph) { (void)hipGraphDestroy(wg.graph); wg.graph = nullptr; }
     const DevicePlanHost& d = dp.dph;
-    const size_t ws_bytes = (pr.run.fp32 ? 4 : 8) * (size_t)kTracks * (size_t)dp.h.wstride;
+    const size_t ws_bytes = (pr.run.fp32 ? 4 : 8) * (size_t)ws_slots(dp.h);
     check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "capture");
     hipError_t e = launch_wide_step(pr.run.fp32, pr.desc, dp.h, pr.count, p, ctl, d.rs_level_chunk, d.fs_level_pass,
                                     d.bs_level_pass, ws_bytes, s);
@@ -483,7 +503,7 @@ double launch_instances(gs_ctx* ctx, DevicePlan& dp, DevMem& inst_buf, DevMem& d
         prm.init = (first && init) ? 1 : 0;
         prm.load_start = first ? load_start : 0;
         first = false;
-        check(launch_nr_batch(0, run.x_lds, desc_buf.as<InstDesc>() + i, h, 1, prm, run.lds_bytes, s),
+        check(launch_nr_batch(0, run.x_lds, run.pair, desc_buf.as<InstDesc>() + i, h, 1, prm, run.lds_bytes, s),
               "plain kernel");
         nl++;
         InstState hs;
@@ -980,7 +1000,7 @@ gs_status seam_solve_one(gs_ctx* ctx, int n, const int* rowptr, const int* colid
                bv.as<double>(), dv.as<double>(), xs.as<double>(), rf.as<double>(), stt.as<int>()};
     desc.ensure(sizeof(SeamDesc));
     check(hipMemcpyAsync(desc.p, &d, sizeof(SeamDesc), hipMemcpyHostToDevice, s), "seam desc");
-    size_t lds = 8 * (size_t)kTracks * (size_t)sp.dev.h.wstride;
+    size_t lds = 8 * (size_t)ws_slots(sp.dev.h);
     check(launch_seam(desc.as<SeamDesc>(), sp.dev.h, 1, lds, s), "seam kernel");
     int status = 1;
     check(hipMemcpyAsync(&status, stt.p, 4, hipMemcpyDeviceToHost, s), "seam status");
@@ -1577,11 +1597,19 @@ GS_API gs_status gs_nr_autotune(gs_net* net, int32_t count, const gs_instance* s
     o.schedule = 0;
     run_batch(net, count, sample, o, ref.data(), ref_vm.data(), ref_va.data(), false);
     std::vector<Candidate> cands;
-    for (int32_t flags : {0, (int32_t)GS_FLAG_X_GLOBAL})
+    for (int32_t flags : {0, (int32_t)GS_FLAG_SINGLE_WAVE, (int32_t)GS_FLAG_X_GLOBAL})
       for (int32_t sched : {0, 512, 256})
         for (Candidate c : {Candidate{GS_LU_FP32, 1, 0, 0}, Candidate{GS_LU_FP32, 0, 0, 0},
-                            Candidate{GS_LU_FP32, 2, 0, 0}, Candidate{GS_LU_FP64, 0, 0, 0}})
-          if (sched == 0 || sched < count) cands.push_back({c.lu, c.refine, sched, flags});
+                            Candidate{GS_LU_FP32, 2, 0, 0}, Candidate{GS_LU_FP64, 0, 0, 0}}) {
+          if (sched != 0 && sched >= count) continue;
+          if (flags == GS_FLAG_SINGLE_WAVE) {  // only where the default would pair the instances
+            gs_nr_options t = base;
+            t.lu_precision = c.lu;
+            t.flags = 0;
+            if (!plan_run(net->dev.h, t, false).pair) continue;
+          }
+          cands.push_back({c.lu, c.refine, sched, flags});
+        }
     double best = 1e300;
     gs_nr_options chosen = base;
     for (const Candidate& c : cands) {

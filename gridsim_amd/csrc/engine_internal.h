@@ -16,8 +16,8 @@
 
 namespace gs {
 
-hipError_t launch_nr_batch(int lu_fp32, int x_in_lds, const InstDesc* descs, const DevPlan& plan, int count,
-                           const NrParams& prm, size_t lds_bytes, hipStream_t stream);
+hipError_t launch_nr_batch(int lu_fp32, int x_in_lds, int pair, const InstDesc* descs, const DevPlan& plan,
+                           int count, const NrParams& prm, size_t lds_bytes, hipStream_t stream);
 hipError_t launch_seam(const SeamDesc* descs, const DevPlan& plan, int count, size_t lds_bytes, hipStream_t stream);
 hipError_t launch_sincos(const double* x, double* s, double* c, int64_t n, hipStream_t stream);
 hipError_t launch_iterative_step(const InstDesc* descs, const DevPlan& plan, const NrParams& prm, double* work,
@@ -30,7 +30,9 @@ hipError_t launch_wide_step(int lu_fp32, const InstDesc* descs, const DevPlan& p
                             const std::vector<int>& bs_level_pass, size_t ws_bytes, hipStream_t stream);
 hipError_t launch_fp64_probe(double* out, hipStream_t stream);
 
-constexpr size_t kLdsLimit = 64 * 1024;  // dynamic LDS a single-wave work-group may take for the solve workspace
+constexpr size_t kLdsLimit = 64 * 1024;  // dynamic LDS a work-group may take for the solve workspace
+constexpr size_t kLdsPerCu = 160 * 1024;  // LDS of one compute unit (gfx950)
+constexpr size_t kWavesPerCu = 8;         // the batched kernel's register use allows two waves per SIMD
 
 struct HipError {
   gs_status code;
@@ -148,6 +150,7 @@ struct Layout {
 // Arithmetic and LDS placement of a launch.
 struct BatchRun {
   bool fp32 = true, x_lds = true;
+  bool pair = false;   // two instances per work-group sharing the LDS solve vector
   size_t lds_bytes = 0;
 };
 

@@ -547,8 +547,11 @@ void compile_plan(const Plan& plan, DevicePlanHost& d) {
   d.n_levels = n_levels;
   // Level chunks: within each level, columns by decreasing chain length, kTracks at a time.
   const int G = DevicePlanHost::kTracks, TL = DevicePlanHost::kTrackLanes;
-  d.wstride = (d.max_col + 7) & ~7;
-  if (d.wstride * G > 0xFFFF) throw std::runtime_error("plan: column workspace too large for 16-bit slot indices");
+  // An odd stride puts the same slot of the 8 tracks in 8 different LDS banks (the tracks' updates tend to touch the
+  // same slot positions at once).
+  d.wstride = d.max_col | 1;
+  if (d.wstride * G + DevicePlanHost::kSpareSlots > 0xFFFF)
+    throw std::runtime_error("plan: column workspace too large for 16-bit slot indices");
   d.ch_track_ptr.assign(1, 0);
   d.ch_row_ptr.assign(1, 0);
   d.ch_a_ptr.assign(1, 0);
@@ -768,7 +771,7 @@ void compile_plan(const Plan& plan, DevicePlanHost& d) {
         for (int l = 0; l < 64; l++) {
           const int e = e0 + l;
           const bool in = e < a1;
-          rec[4 * l] = in ? d.ch_a_ws[e] : 0xFFFF;
+          rec[4 * l] = in ? d.ch_a_ws[e] : d.spare(l);
           rec[4 * l + 1] = in ? e : 0;
           rec[4 * l + 2] = 0;
           rec[4 * l + 3] = 0;
@@ -791,7 +794,7 @@ void compile_plan(const Plan& plan, DevicePlanHost& d) {
         for (int l = 0; l < 64; l++) {
           const int rr = d.row_rec[(size_t)r * 64 + l];
           const bool in = (rr & 0xFFFF) != 0xFFFF;
-          rec[4 * l] = in ? rr : 0xFFFF;
+          rec[4 * l] = in ? rr : d.spare(l) | (d.spare(l) << 16);
           rec[4 * l + 1] = in ? d.n_a + d.row_lsrc[(size_t)r * 64 + l] : 0;
           rec[4 * l + 2] = 0;
           rec[4 * l + 3] = 0;
@@ -894,7 +897,7 @@ bool check_level_schedule(const DevicePlanHost& d, int ring, int padded_rows, in
         if ((d.rs_meta[r] & D::kRsKindMask) != D::kRsUpdate) continue;
         for (int lane = 0; lane < 64; lane++) {
           const int* rr = &d.rs_rec[((size_t)r * 64 + lane) * 4];
-          if ((rr[0] & 0xFFFF) == 0xFFFF) continue;
+          if ((rr[0] & 0xFFFF) == d.spare(lane)) continue;
           const int v = rr[1] & D::kRsIndexMask;
           if (v < d.n_a || v >= n_vals || val_level[v] < 0 || val_level[v] >= l)
             return bad("a chunk reads an L value its level's predecessors did not write");
@@ -993,13 +996,13 @@ bool emulate_device_lu(const DevicePlanHost& d, int nnz_j, const double* jv, con
       if (rc[1] < 0 || rc[1] >= n_vals) return bad("row value index");
       const int load_row = r - D::kRsValueAhead;  // the value is loaded at the end of this row
       if (kind == D::kRsAssign) {
-        if (rc[0] == 0xFFFF) continue;
+        if (rc[0] == d.spare(lane)) continue;
         if (rc[0] < 0 || rc[0] >= wsz) return bad("A workspace index");  // any lane may assign any slot
         if (rc[1] >= n_a || std::isnan(vals[rc[1]])) return bad("A value not written by the Jacobian fill");
         ws[rc[0]] = vals[rc[1]];
       } else if (kind == D::kRsUpdate) {
         const int dst = rc[0] & 0xFFFF, xj = rc[0] >> 16;
-        if (dst == 0xFFFF) continue;
+        if (dst == d.spare(lane) && xj == d.spare(lane)) continue;
         if (dst >= wsz || xj >= wsz || dst / d.wstride != lane / TL || xj / d.wstride != lane / TL)
           return bad("row record outside its track's workspace");
         if (rc[1] < n_a || written_row[rc[1]] < 0) return bad("row reads an L value not yet written");

@@ -121,6 +121,8 @@ struct DevicePlanHost {
   // serves track l / kTrackLanes), so every lane knows its record's address without a search.
   static constexpr int kTracks = 8, kTrackLanes = 8;
   int wstride = 0, n_chunks = 0, n_rows = 0;
+  static constexpr int kSpareSlots = 64;   // one spare workspace slot per lane, after the kTracks workspaces
+  int spare(int lane) const { return kTracks * wstride + lane; }
   vector<int> ch_track_ptr, ch_row_ptr, ch_a_ptr, ch_fence;  // per chunk; ch_fence: last chunk of its level
   vector<int> tr_col;                                        // column of each track, chunk order
   vector<int> ch_a_ws, ch_a_src;                             // A entries: workspace index, J CSR position
@@ -137,7 +139,9 @@ struct DevicePlanHost {
   // (4 ints) per lane, so the kernel loads every row's record and value a fixed number of rows ahead. The record's y
   // holds the value index in its low 24 bits and the row's kind and flags (the same in every lane) in its high 8:
   //   ASSIGN  ws[x] = vals[y]                     (a chunk's A entries; kRsClear on its first: zero the workspace)
-  //   UPDATE  ws[x & 0xFFFF] -= vals[y] * ws[x >> 16]   (x = 0xFFFF: no update in this lane)
+  //   UPDATE  ws[x & 0xFFFF] -= vals[y] * ws[x >> 16]
+  //   (a lane with nothing to assign or update is pointed at its own spare slot spare(lane) past the tracks'
+  //   workspaces, so the device runs these rows with every lane active and no branch)
   //   PIVOT   x = pivot slot, z = candidate slots [z & 0xFFFF, z >> 16), w = pivot step (or -1: idle track)
   //   WRITE   slot x & 0xFFFF, kind x >> 16: U (bval[z] = ws), PIVOT (dval[z] = ws), L (vals[z] = fval[w] = ws / pivot)
   // vals = [A values (n_a) | L in column order (nnz_l)], so y and the L destinations index one array. kRsLevelEnd marks

@@ -60,6 +60,10 @@ constexpr int kStreamPad = 16;
 constexpr int kRsValueAhead = 8;
 
 constexpr int kTracks = 8, kTrackLanes = 8;
+// the refactor workspace: kTracks columns of wstride slots, then one spare slot per lane (model.h DevicePlanHost)
+constexpr int kWsSpare = 64;
+// slots of the refactor workspace (the triangular-solve vector follows it in LDS)
+__host__ __device__ inline int ws_slots(const DevPlan& p) { return kTracks * p.wstride + kWsSpare; }
 
 constexpr int kMaxPatch = 8;
 
@@ -121,7 +125,14 @@ struct NrParams {
   int32_t steps_this_launch;          // Newton steps one launch may take (the plain path takes 1)
   int32_t init;                       // 1: reset the instance's counters before stepping
   int32_t load_start;                 // 1: copy the start point into the iterate before stepping
-  int32_t pad;
+  int32_t n_inst;                     // instances of the launch (set by the launcher)
 };
+
+// Instances per work-group of the batched kernel: 1, or 2 wavefronts sharing one LDS solve vector (each wave takes
+// it in turn for its triangular solves), which halves the solve vector's LDS per instance.
+constexpr int kPairWaves = 2;
+// Spins of a wave waiting for the shared solve vector before it gives up (the instance then ends as failed); the
+// holder releases it within one pair of triangular solves, far below this bound.
+constexpr int kPairSpinLimit = 1 << 24;
 
 }  // namespace gs

@@ -83,6 +83,11 @@ __device__ __forceinline__ int opaque_lane(int lane) {
 // Global-memory writes of this wavefront made visible to its own later reads (work-group scope: same CU, shared L1).
 #define WG_SYNC() __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup")
 
+// Waits for this wave's LDS (and scalar-memory) operations only. Placed at the end of a rarely taken branch whose LDS
+// reads may otherwise still be pending where the branches join: the compiler would then wait there on every path,
+// including the frequent one, which never issued them. (s_waitcnt lgkmcnt(0), vmcnt and expcnt left at their maxima.)
+__device__ __forceinline__ void lds_drained() { __builtin_amdgcn_s_waitcnt(0xC07F); }
+
 // One instance's buffers, as global-memory pointers.
 template <typename T>
 struct Inst {
@@ -120,7 +125,7 @@ __device__ __forceinline__ void put_j(const Inst<T>& I, int s, int a, double v, 
 // Every Y cell's two injection terms and its off-diagonal Jacobian entries, one cell per lane. Returns whether a
 // Jacobian value of this lane was not finite.
 template <typename T>
-__device__ int mismatch_cells(const DevPlan& P, const Inst<T>& I, int lane, bool keep_jv, int blk0, int stride) {
+__device__ __forceinline__ int mismatch_cells(const DevPlan& P, const Inst<T>& I, int lane, bool keep_jv, int blk0, int stride) {
   GLOBAL const int* y_row = glob(P.y_row);
   GLOBAL const int* y_ci = glob(P.y_ci);
   GLOBAL const double* y_g = glob(P.y_g);
@@ -182,7 +187,7 @@ __device__ int mismatch_cells(const DevPlan& P, const Inst<T>& I, int lane, bool
 // Per bus the ordered sums of its row's terms, the diagonal Jacobian entries and the mismatch; the next bus's
 // scalars are loaded ahead. Returns this lane's max |f| (NaN-propagating); *bad: a Jacobian value was not finite.
 template <typename T>
-__device__ double mismatch_buses(const DevPlan& P, const Inst<T>& I, int lane, bool keep_jv, int blk0, int stride,
+__device__ __forceinline__ double mismatch_buses(const DevPlan& P, const Inst<T>& I, int lane, bool keep_jv, int blk0, int stride,
                                  int* bad) {
   GLOBAL const double* y_g = glob(P.y_g);
   GLOBAL const double* y_b = glob(P.y_b);
@@ -243,7 +248,7 @@ __device__ double mismatch_buses(const DevPlan& P, const Inst<T>& I, int lane, b
 // The whole pass for one wave. Returns max |f| (NaN-propagating) over the wave; sets *bad when a Jacobian value is
 // not finite.
 template <typename T>
-__device__ double mismatch_jacobian(const DevPlan& P, const Inst<T>& I, int lane, bool keep_jv, int* bad) {
+__device__ __forceinline__ double mismatch_jacobian(const DevPlan& P, const Inst<T>& I, int lane, bool keep_jv, int* bad) {
   const int cbad = mismatch_cells<T>(P, I, lane, keep_jv, 0, 1);
   WG_SYNC();
   int bbad;
@@ -291,10 +296,19 @@ __device__ __forceinline__ int refactor(const DevPlan& P, GLOBAL T* vals, GLOBAL
   // the compiler knows to be uniform would be moved to a scalar register as soon as it is loaded, which waits for it.
   v4i rr[kRing];
   T rv[kRing];
+  // The first loads in the order the loop's last kAhead rows issue them (records of rows 0..kRing-kAhead-1, then per
+  // row the record of row kRing-kAhead+h and the value of row h), each pinned in place: the wait counts the compiler
+  // derives at the loop head merge this path with the loop's back edge, so any other order here would make every pass
+  // of the loop wait for nearly all loads in flight.
 #pragma unroll
-  for (int s = 0; s < kRing; s++) rr[s] = rs_rec[(size_t)s * kWave + lane];
-#pragma unroll
-  for (int s = 0; s < kAhead; s++) rv[s] = vals[rr[s].y & kRsIndexMask];
+  for (int s = 0; s < kRing; s++) {
+    rr[s] = rs_rec[(size_t)s * kWave + lane];
+    if (s >= kRing - kAhead) {
+      const int h = s - (kRing - kAhead);
+      rv[h] = vals[rr[h].y & kRsIndexMask];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
   T piv = T(1);
   int failed = -1;
   // (a failed pivot check ends the loop at the next ring boundary, not at once: an exit from inside the unrolled body
@@ -304,18 +318,22 @@ __device__ __forceinline__ int refactor(const DevPlan& P, GLOBAL T* vals, GLOBAL
 #pragma unroll
     for (int s = 0; s < kRing; s++) {
       const v4i r4 = rr[s];
-      const int meta = base + s < nrows ? (int)((unsigned)r4.y >> kRsMetaShift) : kRsNop;
+      // Every lane holds the row's kind and flags: taken as a scalar here, at the row itself (the record arrived
+      // kRing rows ago), so the dispatch below is a scalar branch and the row's lanes are never masked.
+      int meta = (int)((unsigned)__builtin_amdgcn_readfirstlane(r4.y) >> kRsMetaShift);
+      if (base + s >= nrows) meta = kRsNop;
       const int kind = meta & kRsKindMask;
-      if (kind == kRsAssign) {
+      if (kind == kRsUpdate) {
+        // a lane without an update works on its own spare slot
+        const int dst = r4.x & 0xFFFF, src = (int)((unsigned)r4.x >> 16);
+        ws[dst] = ws[dst] - rv[s] * ws[src];
+        WAVE_SYNC();
+      } else if (kind == kRsAssign) {
         if (meta & kRsClear) {
           for (int q = lane; q < wsz; q += kWave) ws[q] = T(0);
           WAVE_SYNC();
         }
-        if (r4.x != 0xFFFF) ws[r4.x] = rv[s];
-        WAVE_SYNC();
-      } else if (kind == kRsUpdate) {
-        const int dst = r4.x & 0xFFFF;
-        if (dst != 0xFFFF) ws[dst] = ws[dst] - rv[s] * ws[r4.x >> 16];
+        ws[r4.x] = rv[s];   // (a lane without an A entry: its spare slot)
         WAVE_SYNC();
       } else if (kind == kRsPivot) {
         // pivot and growth check, one track per 8 lanes (all 8 lanes of a track are active or idle together)
@@ -346,20 +364,19 @@ __device__ __forceinline__ int refactor(const DevPlan& P, GLOBAL T* vals, GLOBAL
         // any failing track fails the factorisation (the first failure's lowest lane's column reported)
         const unsigned long long fmask = __ballot(fail >= 0);
         if (fmask && failed < 0) failed = __shfl(fail, __ffsll((long long)fmask) - 1, kWave);
+        lds_drained();
       } else if (kind == kRsWrite) {
-        const int wk = r4.x >> 16;
-        if (wk != 0) {
-          const T v = ws[r4.x & 0xFFFF];
-          if (wk == kWriteU) {
-            bval[r4.z] = v;
-          } else if (wk == kWritePivot) {
-            dval[r4.z] = v;
-          } else {
-            const T lv = v / piv;
-            vals[r4.z] = lv;
-            fval[r4.w] = lv;
-          }
+        // every lane reads its slot (an idle lane slot 0) and forms the L quotient; the stores are masked by kind
+        const int wk = (int)((unsigned)r4.x >> 16);
+        const T v = ws[r4.x & 0xFFFF];
+        const T lv = v / piv;
+        if (wk == kWriteU) bval[r4.z] = v;
+        if (wk == kWritePivot) dval[r4.z] = v;
+        if (wk == kWriteL) {
+          vals[r4.z] = lv;
+          fval[r4.w] = lv;
         }
+        lds_drained();
       }
       // the level's L columns are written: publish them before any later row loads them
       if (meta & kRsLevelEnd) WG_SYNC();
@@ -392,12 +409,17 @@ __device__ __forceinline__ void solve_stream(GLOBAL const v4i* rec, int step0, i
   const int steps = step1 - step0;
   v4i rr[kRing];
   T rv[kRing], rd[kRing];
-#pragma unroll
-  for (int s = 0; s < kRing; s++) rr[s] = rec[(size_t)s * kWave + lane];
+  // first loads in the loop's own order (see refactor): records of steps 0..kHalf-1, then per step the record of step
+  // kHalf+h and the values of step h; the loop's first kHalf steps load the values of steps kHalf..kRing-1
 #pragma unroll
   for (int s = 0; s < kRing; s++) {
-    rv[s] = vals[rr[s].y];
-    rd[s] = BACK ? dval[rr[s].z < 0 ? 0 : rr[s].z] : T(1);
+    rr[s] = rec[(size_t)s * kWave + lane];
+    if (s >= kRing - kHalf) {
+      const int h = s - (kRing - kHalf);
+      rv[h] = vals[rr[h].y];
+      rd[h] = BACK ? dval[rr[h].z < 0 ? 0 : rr[h].z] : T(1);
+    }
+    __builtin_amdgcn_sched_barrier(0);
   }
   T acc = T(0);
   for (int base = 0; base < steps; base += kRing) {
@@ -469,13 +491,47 @@ __device__ __forceinline__ void residual(const DevPlan& P, GLOBAL const double* 
 }
 
 // ------------------------------------------------------------------------------------------------ the kernel
-template <typename T, bool XLDS>
-// At least two waves per SIMD (registers capped at 256): 8 instances resident per CU.
-__global__ __launch_bounds__(kWave, 2) void nr_batch_kernel(const InstDesc* __restrict__ descs, const DevPlan P,
-                                                         const NrParams prm) {
+// The pair's solve vector: one wave at a time. Lane 0 takes the LDS lock word with a compare-and-swap and the wave
+// spins (sleeping between tries) until it has it, at most kPairSpinLimit tries; the holder never waits on the other
+// wave, so the lock always comes free. Taking it orders the holder's LDS accesses after the previous holder's.
+__device__ __forceinline__ bool take_lock(LDSP int* lock, int lane) {
+  for (int tries = 0; tries < kPairSpinLimit; tries++) {
+    int got = 0;
+    if (lane == 0) {
+      int expected = 0;
+      got = __hip_atomic_compare_exchange_strong(lock, &expected, 1, __ATOMIC_ACQUIRE, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    if (__shfl(got, 0, kWave)) return true;
+    __builtin_amdgcn_s_sleep(4);
+  }
+  return false;
+}
+
+__device__ __forceinline__ void give_lock(LDSP int* lock, int lane) {
+  WAVE_SYNC();
+  if (lane == 0) __hip_atomic_store(lock, 0, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// At least two waves per SIMD (registers capped at 256): 8 instances resident per CU when LDS allows.
+template <typename T, bool XLDS, bool PAIR>
+__global__ __launch_bounds__(kWave * (PAIR ? kPairWaves : 1), 2) void nr_batch_kernel(
+    const InstDesc* __restrict__ descs, const DevPlan P, const NrParams prm) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int lane = threadIdx.x;
-  const InstDesc& d = descs[blockIdx.x];
+  const int lane = threadIdx.x % kWave;
+  const int wave = PAIR ? (int)(threadIdx.x / kWave) : 0;
+  const int inst = PAIR ? (int)blockIdx.x * kPairWaves + wave : (int)blockIdx.x;
+  // LDS: each wave's refactor workspace, then (when it fits) the triangular-solve vector, shared by the pair, then
+  // the pair's lock word
+  LDSP T* ws = lds(reinterpret_cast<T*>(smem)) + wave * ws_slots(P);
+  LDSP T* xl = lds(reinterpret_cast<T*>(smem)) + (PAIR ? kPairWaves : 1) * ws_slots(P);
+  LDSP int* x_lock = reinterpret_cast<LDSP int*>(xl + P.m);
+  if constexpr (PAIR) {
+    if (threadIdx.x == 0) *x_lock = 0;
+    __syncthreads();   // the only work-group barrier: from here on the two waves run independently
+    if (inst >= prm.n_inst) return;
+  }
+  const InstDesc& d = descs[inst];
   Inst<T> I;
   I.pspec = glob(d.pspec); I.qspec = glob(d.qspec); I.vm_start = glob(d.vm_start); I.va_start = glob(d.va_start);
   I.vm = glob(d.vm); I.va = glob(d.va); I.p = glob(d.p); I.q = glob(d.q);
@@ -485,15 +541,12 @@ __global__ __launch_bounds__(kWave, 2) void nr_batch_kernel(const InstDesc* __re
   I.fval = glob(reinterpret_cast<T*>(d.fval)); I.bval = glob(reinterpret_cast<T*>(d.bval));
   I.dval = glob(reinterpret_cast<T*>(d.dval));
   I.xg = glob(reinterpret_cast<T*>(d.xg));
-  I.patch = glob(&descs[blockIdx.x].patch[0]);
+  I.patch = glob(&descs[inst].patch[0]);
   I.n_patch = d.n_patch;
 #pragma unroll
   for (int t = 0; t < kMaxPatch; t++) I.pidx[t] = t < I.n_patch ? d.patch[t].idx : -1;
   InstState* st = d.state;
   const int n = P.n, m = P.m;
-  // LDS: the refactor's workspace, then (when it fits) the triangular-solve vector
-  LDSP T* ws = lds(reinterpret_cast<T*>(smem));
-  LDSP T* xl = ws + kTracks * P.wstride;
 
   if (prm.init || prm.load_start) {
     for (int i = lane; i < n; i += kWave) { I.vm[i] = I.vm_start[i]; I.va[i] = I.va_start[i]; }
@@ -514,9 +567,17 @@ __global__ __launch_bounds__(kWave, 2) void nr_batch_kernel(const InstDesc* __re
     acc += now - clk;
     clk = now;
   };
+  int lock_timeout = 0;
   auto solve = [&](GLOBAL const double* rhs, GLOBAL double* out) {
-    if constexpr (XLDS) lu_solve<T, true>(P, I.fval, I.bval, I.dval, rhs, out, xl, opaque_lane(lane));
-    else lu_solve<T, false>(P, I.fval, I.bval, I.dval, rhs, out, I.xg, opaque_lane(lane));
+    if constexpr (PAIR) {
+      if (!take_lock(x_lock, lane)) { lock_timeout = 1; return; }
+      lu_solve<T, true>(P, I.fval, I.bval, I.dval, rhs, out, xl, opaque_lane(lane));
+      give_lock(x_lock, lane);
+    } else if constexpr (XLDS) {
+      lu_solve<T, true>(P, I.fval, I.bval, I.dval, rhs, out, xl, opaque_lane(lane));
+    } else {
+      lu_solve<T, false>(P, I.fval, I.bval, I.dval, rhs, out, I.xg, opaque_lane(lane));
+    }
   };
   for (;;) {
     int bad;
@@ -541,6 +602,7 @@ __global__ __launch_bounds__(kWave, 2) void nr_batch_kernel(const InstDesc* __re
       WG_SYNC();
     }
     tick(t_sol);
+    if (PAIR && lock_timeout) { step_failures++; break; }   // the solve vector never came free: the run ends
     double lmax = 0.0;
     int lbad = 0;
     for (int k = lane; k < m; k += kWave) {
@@ -1165,19 +1227,26 @@ __global__ void fp64_probe_kernel(double* out, double a) {
 }
 
 // host-side launchers (engine.cpp)
-hipError_t launch_nr_batch(int lu_fp32, int x_in_lds, const InstDesc* descs, const DevPlan& plan, int count,
-                           const NrParams& prm, size_t lds_bytes, hipStream_t stream) {
-  dim3 grid(count), block(kWave);
+hipError_t launch_nr_batch(int lu_fp32, int x_in_lds, int pair, const InstDesc* descs, const DevPlan& plan,
+                           int count, const NrParams& prm0, size_t lds_bytes, hipStream_t stream) {
+  NrParams prm = prm0;
+  prm.n_inst = count;
+  if (pair && !x_in_lds) return hipErrorInvalidValue;
+  const dim3 grid(pair ? (count + kPairWaves - 1) / kPairWaves : count), block(pair ? kWave * kPairWaves : kWave);
   if (lu_fp32) {
-    if (x_in_lds)
-      hipLaunchKernelGGL((nr_batch_kernel<float, true>), grid, block, lds_bytes, stream, descs, plan, prm);
+    if (pair)
+      hipLaunchKernelGGL((nr_batch_kernel<float, true, true>), grid, block, lds_bytes, stream, descs, plan, prm);
+    else if (x_in_lds)
+      hipLaunchKernelGGL((nr_batch_kernel<float, true, false>), grid, block, lds_bytes, stream, descs, plan, prm);
     else
-      hipLaunchKernelGGL((nr_batch_kernel<float, false>), grid, block, lds_bytes, stream, descs, plan, prm);
+      hipLaunchKernelGGL((nr_batch_kernel<float, false, false>), grid, block, lds_bytes, stream, descs, plan, prm);
   } else {
-    if (x_in_lds)
-      hipLaunchKernelGGL((nr_batch_kernel<double, true>), grid, block, lds_bytes, stream, descs, plan, prm);
+    if (pair)
+      hipLaunchKernelGGL((nr_batch_kernel<double, true, true>), grid, block, lds_bytes, stream, descs, plan, prm);
+    else if (x_in_lds)
+      hipLaunchKernelGGL((nr_batch_kernel<double, true, false>), grid, block, lds_bytes, stream, descs, plan, prm);
     else
-      hipLaunchKernelGGL((nr_batch_kernel<double, false>), grid, block, lds_bytes, stream, descs, plan, prm);
+      hipLaunchKernelGGL((nr_batch_kernel<double, false, false>), grid, block, lds_bytes, stream, descs, plan, prm);
   }
   return hipGetLastError();
 }

@@ -31,6 +31,7 @@ GS_OK, GS_SINGULAR, GS_NONFINITE_INPUT, GS_OUT_OF_MEMORY, GS_INVALID_ARGUMENT, G
 CONVERGED, MAX_ITERATIONS, DIVERGED, NONFINITE = 0, 1, 2, 3
 LU_FP64, LU_FP32 = 0, 1
 FLAG_X_GLOBAL = 1
+FLAG_SINGLE_WAVE = 4
 FLAG_Q_LIMITS = 2      # enforce generator reactive limits (the outer loop of PV -> PQ switching)
 SCHEDULE_LEVELS = -1   # every instance spread over the whole GPU, level by level (GS_SCHEDULE_LEVELS)
 STATUS_NAMES = {0: "Converged", 1: "MaxIterations", 2: "Diverged", 3: "NonFinite"}
@@ -83,7 +84,18 @@ class GsPlanStats(C.Structure):
     _fields_ = [("m", C.c_int32), ("nnz_j", C.c_int32), ("nnz_l", C.c_int32), ("nnz_u", C.c_int32),
                 ("levels", C.c_int32), ("refactor_rows", C.c_int32), ("forward_steps", C.c_int32),
                 ("back_steps", C.c_int32), ("workspace_stride", C.c_int32), ("reserved", C.c_int32),
-                ("stream_bytes", C.c_int64)]
+                ("stream_bytes", C.c_int64), ("rows_by_kind", C.c_int32 * 5), ("update_lanes", C.c_int32),
+                ("forward_lanes", C.c_int32), ("back_lanes", C.c_int32)]
+
+
+def _plan_stats_dict(st: "GsPlanStats") -> dict:
+    out = {}
+    for k, _ in GsPlanStats._fields_:
+        if k == "reserved":
+            continue
+        v = getattr(st, k)
+        out[k] = list(v) if k == "rows_by_kind" else v
+    return out
 
 
 class GsServerConfig(C.Structure):
@@ -205,7 +217,7 @@ class HostPlan:
         st = GsPlanStats()
         if lib().gs_plan_shape(self.h, C.byref(st)) != GS_OK:
             raise GsError(GS_INVALID_ARGUMENT, "gs_plan_shape")
-        return {k: getattr(st, k) for k, _ in GsPlanStats._fields_ if k != "reserved"}
+        return _plan_stats_dict(st)
 
     def __del__(self):
         try:
@@ -370,7 +382,7 @@ class DeviceNetwork:
         """Shape of the compiled device plan (record streams, levels, shared bytes)."""
         st = GsPlanStats()
         self.ctx.check(lib().gs_network_plan_stats(self.h, C.byref(st)), "gs_network_plan_stats")
-        return {k: getattr(st, k) for k, _ in GsPlanStats._fields_ if k != "reserved"}
+        return _plan_stats_dict(st)
 
     def _instances(self, outages=None, load_scale=None, count=None, starts=None):
         count = count if count is not None else (len(outages) if outages is not None else len(load_scale))

@@ -91,6 +91,11 @@ typedef struct gs_nr_options {
  * instance, so more instances resident per compute unit). */
 enum { GS_FLAG_X_GLOBAL = 1 };
 
+/* gs_nr_options.flags: one wavefront per work-group. By default, where the solve vector sits in LDS and two instances
+ * per work-group keep more of them resident per compute unit, the two wavefronts of a work-group share one LDS solve
+ * vector and take it in turn for their triangular solves (same arithmetic, same results). */
+enum { GS_FLAG_SINGLE_WAVE = 4 };
+
 /* gs_nr_options.flags: enforce generator reactive limits (PowerFlowOptions.EnforceQLimits, the outer loop of
  * SparseNewtonRaphsonPowerFlow.SolveAcAttempt :118-160 with NewtonShell.ApplyQLimits / RevertBrokenPins :519-595,
  * RevertQLimits off, at most 100 passes as MaxQLimitPasses). After a converged pass the host switches each PV bus
@@ -218,6 +223,12 @@ typedef struct gs_plan_stats {
   int32_t workspace_stride;     /* refactor workspace slots per track */
   int32_t reserved;
   int64_t stream_bytes;
+  /* refactor rows by kind (no-op, assign, update, pivot, write; padding not counted) and the lanes of the update
+   * rows that carry an update (of 64 per row) */
+  int32_t rows_by_kind[5];
+  int32_t update_lanes;
+  /* solve steps' lanes that carry an entry (forward, back; of 64 per step) */
+  int32_t forward_lanes, back_lanes;
 } gs_plan_stats;
 GS_API gs_status gs_network_plan_stats(const gs_net* net, gs_plan_stats* out);
 
