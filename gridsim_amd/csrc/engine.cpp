@@ -158,8 +158,10 @@ gs_plan_stats plan_stats(const DevicePlanHost& d, const YBus& y, const JacobianP
     for (int l = 0; l < 64; l++)
       st.update_lanes += (d.rs_rec[((size_t)r * 64 + l) * 4] & 0xFFFF) != d.spare(l);
   }
-  for (size_t k = 0; k < d.fs_rec.size(); k += 4) st.forward_lanes += d.fs_rec[k] >= 0;
-  for (size_t k = 0; k < d.bs_rec.size(); k += 4) st.back_lanes += d.bs_rec[k] >= 0;
+  for (size_t k = 0; k < d.fs_rec.size(); k += 4)
+    st.forward_lanes += (d.fs_rec[k + 3] & DevicePlanHost::kStepKindMask) == DevicePlanHost::kStepProduct && d.fs_rec[k] >= 0;
+  for (size_t k = 0; k < d.bs_rec.size(); k += 4)
+    st.back_lanes += (d.bs_rec[k + 3] & DevicePlanHost::kStepKindMask) == DevicePlanHost::kStepProduct && d.bs_rec[k] >= 0;
   return st;
 }
 

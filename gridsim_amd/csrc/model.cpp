@@ -563,18 +563,35 @@ void compile_plan(const Plan& plan, DevicePlanHost& d) {
     });
     for (size_t c0 = 0; c0 < cols.size(); c0 += G) {
       size_t c1 = std::min(cols.size(), c0 + G);
-      // rows of each track: one per (U entry t, group of TL updates)
-      vector<vector<std::pair<int, int>>> trows(c1 - c0);  // per track: list of (t, first update r) sub-steps
+      // Rows of each track: its column's updates list-scheduled, up to TL per row. An update x[dst] -= L * x[src]
+      // (src: the slot of U entry t, dst: an L row of that entry's column) goes in the first row with a free lane
+      // after every update of its source slot (x[src] is then final) and after the previous update of its
+      // destination slot (each slot receives its updates in the reference's order, one row at a time). The
+      // reference's order reaches a source slot's updates before the slot is read, so one pass suffices.
+      struct Upd { int rec, lsrc; };
+      vector<vector<vector<Upd>>> trows(c1 - c0);  // per track: rows, each up to TL updates
       size_t nrows = 0;
       for (size_t g = 0; g < c1 - c0; g++) {
         int k = cols[c0 + g];
         d.tr_col.push_back(k);
         int nu = plan.up[k + 1] - plan.up[k];
+        const int base = (int)g * d.wstride;
+        vector<int> final_row(d.wstride, 0), next_row(d.wstride, 0);  // first row a slot may be read / next written
+        auto& rows_g = trows[g];
         for (int t = 0; t < nu - 1; t++) {
           int ue = plan.up[k] + t, r0 = d.upd_ptr[ue], r1 = d.upd_ptr[ue + 1];
-          for (int r = r0; r < r1; r += TL) trows[g].push_back({t, r});
+          int j = plan.ui[ue];
+          for (int r = r0; r < r1; r++) {
+            const int dst = d.upd_slot[r];
+            size_t row = (size_t)std::max(final_row[t], next_row[dst]);
+            while (row < rows_g.size() && (int)rows_g[row].size() >= TL) row++;
+            if (row == rows_g.size()) rows_g.emplace_back();
+            rows_g[row].push_back({(base + dst) | ((base + t) << 16), plan.lp[j] + 1 + (r - d.upd_ptr[ue])});
+            next_row[dst] = (int)row + 1;
+            final_row[dst] = std::max(final_row[dst], (int)row + 1);
+          }
         }
-        nrows = std::max(nrows, trows[g].size());
+        nrows = std::max(nrows, rows_g.size());
         int col = plan.q[k];
         for (int e = d.col_a_ptr[k]; e < d.col_a_ptr[k + 1]; e++) {
           d.ch_a_ws.push_back((int)g * d.wstride + d.a_slot[e]);
@@ -605,17 +622,9 @@ void compile_plan(const Plan& plan, DevicePlanHost& d) {
           size_t g = lane / TL;
           int idx = lane % TL;
           int rec = 0xFFFF, lsrc = 0;
-          if (g < c1 - c0 && rho < trows[g].size()) {
-            int k = d.tr_col[d.tr_col.size() - (c1 - c0) + g];
-            auto [t, rstart] = trows[g][rho];
-            int ue = plan.up[k] + t;
-            int r = rstart + idx;
-            if (r < d.upd_ptr[ue + 1]) {
-              int j = plan.ui[ue];
-              int base = (int)g * d.wstride;
-              rec = (base + d.upd_slot[r]) | ((base + t) << 16);
-              lsrc = plan.lp[j] + 1 + (r - d.upd_ptr[ue]);
-            }
+          if (g < c1 - c0 && rho < trows[g].size() && idx < (int)trows[g][rho].size()) {
+            rec = trows[g][rho][idx].rec;
+            lsrc = trows[g][rho][idx].lsrc;
           }
           d.row_rec.push_back(rec);
           d.row_lsrc.push_back(lsrc);
@@ -683,12 +692,16 @@ void compile_plan(const Plan& plan, DevicePlanHost& d) {
     for (int i = 0; i < m; i++) d.bw_order[cnt[lvl[i]]++] = i;
     d.bw_levels = nlv;
   }
-  // Solve step streams: per level, rows by decreasing entry count, 64 rows per pass, one step per entry index. Each
-  // entry's value gets the next position of its value array, step by step, so that a step's values are contiguous.
+  // Solve step streams: per level, rows by decreasing entry count, 64 rows per pass; per pass (segment) its PRODUCT
+  // steps, the entries lane by lane (each lane's entries contiguous in the scratch area, in the reference's order),
+  // then one CHAIN step. Each entry's value gets the next position of its value array, step by step, so that a
+  // step's values are contiguous.
   auto build_stream = [&](const vector<int>& order, const vector<int>& level_end, int nlv, const vector<int>& rptr,
                           const vector<int>& rcol, const vector<int>& ridx, bool back, vector<int>& rec,
                           vector<int>& pos, int& n_val, int& steps, vector<int>& pass_step,
                           vector<int>& level_pass) {
+    using D = DevicePlanHost;
+    const int cap = d.solve_scratch();
     int b0 = 0;
     n_val = 0;
     steps = 0;
@@ -705,24 +718,60 @@ void compile_plan(const Plan& plan, DevicePlanHost& d) {
       std::stable_sort(rows.begin(), rows.end(),
                        [&](int a, int b) { return rptr[a + 1] - rptr[a] > rptr[b + 1] - rptr[b]; });
       for (size_t p0 = 0; p0 < rows.size(); p0 += 64) {
-        size_t p1 = std::min(rows.size(), p0 + 64);
-        int len = std::max(1, rptr[rows[p0] + 1] - rptr[rows[p0]]);
+        const int k = (int)std::min<size_t>(64, rows.size() - p0);
         pass_step.push_back(steps);
-        for (int e = 0; e < len; e++) {
-          int flags = (e == 0 ? DevicePlanHost::kStepBegin : 0) | (e + 1 == len ? DevicePlanHost::kStepEnd : 0);
-          for (int lane = 0; lane < 64; lane++) {
-            int row = -1, col = -1, idx = 0;
-            if (p0 + lane < p1) {
-              row = rows[p0 + lane];
-              if (rptr[row] + e < rptr[row + 1]) {
-                col = rcol[rptr[row] + e];
-                idx = n_val++;
-                pos[ridx[rptr[row] + e]] = idx;
-              }
+        vector<int> done(k, 0), len(k);
+        for (int l = 0; l < k; l++) len[l] = rptr[rows[p0 + l] + 1] - rptr[rows[p0 + l]];
+        vector<char> begun(k, 0), ended(k, 0);
+        for (;;) {
+          // one segment: up to cap entries, lane by lane
+          vector<int> st(k, 0), take(k, 0);
+          int fill = 0;
+          for (int l = 0; l < k; l++) {
+            st[l] = fill;
+            take[l] = std::min(len[l] - done[l], cap - fill);
+            fill += take[l];
+          }
+          // PRODUCT steps
+          vector<int> ent_col, ent_idx;
+          for (int l = 0; l < k; l++)
+            for (int e = done[l]; e < done[l] + take[l]; e++) {
+              ent_col.push_back(rcol[rptr[rows[p0 + l]] + e]);
+              ent_idx.push_back(ridx[rptr[rows[p0 + l]] + e]);
             }
-            rec.push_back(col); rec.push_back(idx); rec.push_back(row); rec.push_back(flags);
+          for (int e0 = 0; e0 < fill; e0 += 64) {
+            for (int lane = 0; lane < 64; lane++) {
+              const int e = e0 + lane;
+              int col = -1, idx = 0;
+              if (e < fill) {
+                col = ent_col[e];
+                idx = n_val++;
+                pos[ent_idx[e]] = idx;
+              }
+              rec.push_back(col); rec.push_back(idx); rec.push_back(-1); rec.push_back(D::kStepProduct);
+            }
+            steps++;
+          }
+          // the CHAIN step
+          int maxlen = 0;
+          for (int l = 0; l < k; l++) maxlen = std::max(maxlen, take[l]);
+          for (int lane = 0; lane < 64; lane++) {
+            int x0 = 0, row = -1, fl = D::kStepChain | (maxlen << D::kChainLenShift);
+            if (lane < k && !ended[lane] && (take[lane] > 0 || done[lane] == len[lane])) {
+              row = rows[p0 + lane];
+              x0 = st[lane] | (take[lane] << 16);
+              if (!begun[lane]) { fl |= D::kChainBegin; begun[lane] = 1; }
+              if (done[lane] + take[lane] == len[lane]) { fl |= D::kChainEnd; ended[lane] = 1; }
+            }
+            rec.push_back(x0); rec.push_back(0); rec.push_back(row); rec.push_back(fl);
           }
           steps++;
+          bool all = true;
+          for (int l = 0; l < k; l++) {
+            done[l] += take[l];
+            all = all && ended[l];
+          }
+          if (all) break;
         }
       }
       b0 = b1;
@@ -941,14 +990,15 @@ bool check_level_schedule(const DevicePlanHost& d, int ring, int padded_rows, in
         const int s0 = pass_step[p], s1 = pass_step[p + 1];
         if (s1 <= s0) return bad("an empty pass");
         if (s0 + (s1 - s0 + ring - 1) / ring * ring - 1 + ring >= padded) return bad("a pass's look-ahead passes the padding");
-        if (!(rec[((size_t)s0 * 64) * 4 + 3] & D::kStepBegin) || !(rec[((size_t)(s1 - 1) * 64) * 4 + 3] & D::kStepEnd))
-          return bad("a pass does not begin and end on its own steps");
+        if ((rec[((size_t)(s1 - 1) * 64) * 4 + 3] & D::kStepKindMask) != D::kStepChain)
+          return bad("a pass does not end on its own CHAIN step");
         for (int t = s0; t < s1; t++)
           for (int lane = 0; lane < 64; lane++) {
             const int* r4 = &rec[((size_t)t * 64 + lane) * 4];
-            if (r4[0] >= 0 && (row_level.at(r4[0]) == -1 || row_level[r4[0]] >= l))
+            const int kind = r4[3] & D::kStepKindMask;
+            if (kind == D::kStepProduct && r4[0] >= 0 && (row_level.at(r4[0]) == -1 || row_level[r4[0]] >= l))
               return bad("a pass reads a row its level's predecessors did not finish");
-            if (t == s0 && r4[2] >= 0) rows.push_back(r4[2]);
+            if (kind == D::kStepChain && r4[2] >= 0 && (r4[3] & D::kChainEnd)) rows.push_back(r4[2]);
           }
       }
       for (int r : rows) {
@@ -985,9 +1035,43 @@ bool emulate_device_lu(const DevicePlanHost& d, int nnz_j, const double* jv, con
   vector<int> fence_row(n_vals, -1), written_row(n_vals, -1);
   vector<int> pending;  // values written in the current level, published at its level end
   vector<double> piv(64, 0.0);
+  // the reference's update order: per chunk, each workspace slot's updates come in increasing U entry t, and a slot
+  // is read (as an update's source) only after all of its own updates
+  vector<int> n_pending(wsz, 0), last_t(wsz, -1);
   for (int r = 0; r < d.n_rs; r++) {
     const int meta = d.rs_meta[r], kind = meta & D::kRsKindMask;
-    if (kind == D::kRsAssign && (meta & D::kRsClear)) std::fill(ws.begin(), ws.end(), 0.0);
+    if (kind == D::kRsAssign && (meta & D::kRsClear)) {
+      std::fill(ws.begin(), ws.end(), 0.0);
+      std::fill(n_pending.begin(), n_pending.end(), 0);
+      std::fill(last_t.begin(), last_t.end(), -1);
+      for (int r2 = r; r2 < d.n_rs; r2++) {
+        if (r2 > r && (d.rs_meta[r2] & D::kRsKindMask) == D::kRsAssign && (d.rs_meta[r2] & D::kRsClear)) break;
+        if ((d.rs_meta[r2] & D::kRsKindMask) != D::kRsUpdate) continue;
+        for (int lane = 0; lane < 64; lane++) {
+          const int x = d.rs_rec[((size_t)r2 * 64 + lane) * 4], dst = x & 0xFFFF;
+          if (dst < wsz) n_pending[dst]++;
+        }
+      }
+    }
+    if (kind == D::kRsUpdate) {
+      vector<char> seen(wsz, 0);
+      for (int lane = 0; lane < 64; lane++) {
+        const int x = d.rs_rec[((size_t)r * 64 + lane) * 4], dst = x & 0xFFFF, src = (int)((unsigned)x >> 16);
+        if (dst == d.spare(lane)) continue;
+        if (dst >= wsz || src >= wsz) return bad("row record outside the workspace");
+        const int t = src % d.wstride;
+        if (seen[dst]) return bad("two lanes of a row update one slot");
+        seen[dst] = 1;
+        if (n_pending[src] != 0) return bad("an update reads a slot before all of the slot's updates");
+        if (t <= last_t[dst]) return bad("a slot's updates out of the reference's order");
+      }
+      for (int lane = 0; lane < 64; lane++) {
+        const int x = d.rs_rec[((size_t)r * 64 + lane) * 4], dst = x & 0xFFFF, src = (int)((unsigned)x >> 16);
+        if (dst == d.spare(lane)) continue;
+        n_pending[dst]--;
+        last_t[dst] = src % d.wstride;
+      }
+    }
     bool fail = false;
     for (int lane = 0; lane < 64; lane++) {
       const int* rr = &d.rs_rec[((size_t)r * 64 + lane) * 4];
@@ -1050,30 +1134,57 @@ bool emulate_device_lu(const DevicePlanHost& d, int nnz_j, const double* jv, con
   // the step streams, with the rows a step may read checked against the rows already final
   auto run_stream = [&](const vector<int>& rec, int steps, const vector<double>& vals, bool back,
                         vector<char>& final_rows) -> bool {
-    vector<double> acc(64, 0.0);
+    using D = DevicePlanHost;
+    const int cap = d.solve_scratch();
+    vector<double> acc(64, 0.0), scratch(cap, std::nan(""));
     vector<int> row(64, -1);
+    int sbase = 0;
     if ((int)rec.size() != steps * 64 * 4) { why = "stream size"; return false; }
     for (int t = 0; t < steps; t++) {
-      const int fl = rec[((size_t)t * 64) * 4 + 3];
+      const int kind = rec[((size_t)t * 64) * 4 + 3] & D::kStepKindMask;
+      const int maxlen = (unsigned)rec[((size_t)t * 64) * 4 + 3] >> D::kChainLenShift;
+      vector<int> ended;
       for (int lane = 0; lane < 64; lane++) {
         const int* r4 = &rec[((size_t)t * 64 + lane) * 4];
-        if (r4[3] != fl) { why = "stream flags differ across lanes"; return false; }
-        if (fl & DevicePlanHost::kStepBegin) {
-          row[lane] = r4[2];
-          acc[lane] = row[lane] >= 0 ? x[row[lane]] : 0.0;
-        }
-        if (r4[0] >= 0) {
+        if ((r4[3] & D::kStepKindMask) != kind || ((unsigned)r4[3] >> D::kChainLenShift) != (unsigned)maxlen)
+          { why = "stream step kind differs across lanes"; return false; }
+        if (kind == D::kStepProduct) {
+          if (sbase + 64 > cap) { why = "products pass the scratch area"; return false; }
+          if (r4[0] < 0) continue;
           if (r4[0] >= m || !final_rows[r4[0]]) { why = "stream reads a row not yet final"; return false; }
           if (r4[1] < 0 || r4[1] >= (int)vals.size()) { why = "stream value index"; return false; }
-          acc[lane] = acc[lane] - vals[r4[1]] * x[r4[0]];
-        }
-        if ((fl & DevicePlanHost::kStepEnd) && row[lane] >= 0) {
-          if (r4[2] != row[lane]) { why = "stream row changes inside a pass"; return false; }
-          x[row[lane]] = back ? acc[lane] / dval[row[lane]] : acc[lane];
+          scratch[sbase + lane] = vals[r4[1]] * x[r4[0]];
+        } else if (kind == D::kStepChain) {
+          if (r4[2] < 0) continue;
+          const int st = r4[0] & 0xFFFF, len = (unsigned)r4[0] >> 16;
+          if (len > maxlen || st + len > cap) { why = "chain outside the scratch area"; return false; }
+          if (r4[3] & D::kChainBegin) {
+            row[lane] = r4[2];
+            acc[lane] = x[row[lane]];
+          } else if (row[lane] != r4[2]) {
+            why = "a chain continues another lane's row";
+            return false;
+          }
+          for (int e = st; e < st + len; e++) {
+            if (std::isnan(scratch[e])) { why = "chain reads a product not formed"; return false; }
+            acc[lane] = acc[lane] - scratch[e];
+          }
+          if (r4[3] & D::kChainEnd) {
+            x[row[lane]] = back ? acc[lane] / dval[row[lane]] : acc[lane];
+            ended.push_back(row[lane]);
+            row[lane] = -1;
+          }
+        } else if (kind != 0) {
+          why = "stream step kind";
+          return false;
         }
       }
-      if (fl & DevicePlanHost::kStepEnd)
-        for (int lane = 0; lane < 64; lane++) if (row[lane] >= 0) final_rows[row[lane]] = 1;
+      if (kind == D::kStepProduct) sbase += 64;
+      if (kind == D::kStepChain) {
+        sbase = 0;
+        std::fill(scratch.begin(), scratch.end(), std::nan(""));
+        for (int r : ended) final_rows[r] = 1;
+      }
     }
     return true;
   };

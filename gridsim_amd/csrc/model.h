@@ -157,13 +157,26 @@ struct DevicePlanHost {
   // For level-parallel schedules: chunk c's rows are [rs_chunk_row[c], rs_chunk_row[c + 1]) of the row stream, and
   // level l's chunks are [rs_level_chunk[l], rs_level_chunk[l + 1]).
   vector<int> rs_chunk_row, rs_level_chunk;
-  // Triangular solves as step streams: a pass is up to 64 rows of one level, one lane per row (rows by decreasing
-  // length), one step per entry; record (col, value index, row, flags) per lane and step, col = -1 for no entry. The
-  // factors are stored in the order the streams read them: fval holds L's off-diagonal entries in forward-stream
-  // order, bval U's off-diagonal entries in back-stream order (step by step, the active lanes of a step being a prefix,
-  // so a step's values are contiguous), dval U's diagonal by pivot step. fpos / bpos map an L / U entry (CSC index) to
-  // its position, -1 for a diagonal. The refactor keeps L in column order too (lx, CSC), for its own updates.
-  static constexpr int kStepBegin = 1, kStepEnd = 2;
+  // Triangular solves as step streams. A pass is up to 64 rows of one level, one lane per row (rows by decreasing
+  // length). Its entries' products are formed first, 64 at a time, into a scratch area of the wavefront's LDS (the
+  // refactor workspace, free during the solves); then each lane subtracts its row's products from the row's value
+  // one by one, in the order the reference's column sweeps apply them (forward: columns ascending; back: columns
+  // descending, then the division by the pivot). A product rounds the same whoever forms it, so every entry is the
+  // reference's value, and a long row no longer costs one step per entry. A pass whose products exceed the scratch
+  // area runs in segments, each lane's running value kept across them. Records (4 ints per lane and step):
+  //   PRODUCT  (col, value index, -1, kStepProduct): scratch[s + lane] = vals[value index] * x[col] (col = -1: idle),
+  //            s = 64 * (PRODUCT steps since the last CHAIN)
+  //   CHAIN    (start | len << 16, 0, row, kStepChain | kChainBegin? | kChainEnd? | maxlen << kChainLenShift):
+  //            kChainBegin: acc = x[row]; then acc -= scratch[start + t] for t < len; kChainEnd: x[row] = acc (back:
+  //            acc / pivot). row = -1: idle lane. maxlen (the largest len of the step) is the same in every lane.
+  // The factors are stored in the order the PRODUCT steps read them: fval holds L's off-diagonal entries in
+  // forward-stream order, bval U's off-diagonal entries in back-stream order (the active lanes of a step being a
+  // prefix, a step's values are contiguous), dval U's diagonal by pivot step. fpos / bpos map an L / U entry (CSC
+  // index) to its position, -1 for a diagonal. The refactor keeps L in column order too (lx, CSC), for its own updates.
+  static constexpr int kStepProduct = 1, kStepChain = 2, kStepKindMask = 15;
+  static constexpr int kChainBegin = 16, kChainEnd = 32, kChainLenShift = 8;
+  // scratch slots the solves may use: the refactor workspace rounded down to whole steps
+  int solve_scratch() const { return (kTracks * wstride + kSpareSlots) / 64 * 64; }
   int fs_steps = 0, bs_steps = 0, n_fval = 0, n_bval = 0;
   vector<int> fs_rec, bs_rec;                       // 4 ints per lane per step
   // pass p of a stream covers steps [pass_step[p], pass_step[p + 1]); level l's passes are [level_pass[l], [l + 1])

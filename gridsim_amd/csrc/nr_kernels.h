@@ -53,7 +53,9 @@ constexpr int kRsNop = 0, kRsAssign = 1, kRsUpdate = 2, kRsPivot = 3, kRsWrite =
 constexpr int kRsKindMask = 7, kRsClear = 8, kRsLevelEnd = 16;
 constexpr int kRsMetaShift = 24, kRsIndexMask = (1 << kRsMetaShift) - 1;
 constexpr int kWriteU = 1, kWritePivot = 2, kWriteL = 3;
-constexpr int kStepBegin = 1, kStepEnd = 2;
+// solve step kinds and CHAIN flags (model.h DevicePlanHost)
+constexpr int kStepProduct = 1, kStepChain = 2, kStepKindMask = 15;
+constexpr int kChainBegin = 16, kChainEnd = 32, kChainLenShift = 8;
 // load-ahead distance of the row and step streams, and the padding each stream carries; rows of the refactor load
 // their value kRsValueAhead rows ahead (the plan places its rows for that distance)
 constexpr int kStreamPad = 16;

@@ -394,17 +394,21 @@ __device__ __forceinline__ int refactor(const DevPlan& P, GLOBAL T* vals, GLOBAL
 
 // ------------------------------------------------------------------------------------------------ solves
 // out = A^{-1} rhs with the factors. Rows of one level are independent: a pass takes up to 64 rows of one level, one
-// per lane, and each lane pulls its row's updates, one per step, in the order the reference's column sweeps apply them
-// (forward: columns ascending; back: columns descending, then the division by the pivot), so every entry is the
-// reference's value. The step records are loaded kRing steps ahead and the factor values kHalf steps ahead (the
-// factors are final before a solve starts; the streams are padded by kRing steps); a step's values are contiguous
-// (its active lanes are a prefix), so each value load is one coalesced segment. x is read at the step itself.
-// XP: the workspace pointer (LDS or global).
+// per lane. Its PRODUCT steps form the products L_ij x_j (U_ij x_j) of the pass's entries, 64 per step, into the scratch
+// area (each lane's row's entries contiguous, in the order the reference's column sweeps apply them: forward columns
+// ascending, back columns descending); its CHAIN step then has each lane subtract its row's products from the row's
+// value one by one in that order (back: then divide by the pivot). A product rounds the same whoever forms it, so
+// every entry is the reference's value, and a row costs one CHAIN iteration per entry (LDS reads and one subtraction)
+// instead of one step. Passes larger than the scratch area run in segments; a lane's running value stays in its
+// register across them. The step records are loaded kRing steps ahead and the factor values kHalf steps ahead (the
+// factors are final before a solve starts; the streams are padded by kRing steps); a PRODUCT step's values are
+// contiguous (its active lanes are a prefix), so each value load is one coalesced segment. x is read at the step
+// itself. XP: the solve vector (LDS or global); scr: the scratch area (LDS, the wave's refactor workspace).
 // Steps [step0, step1) of the stream (the whole stream, or one pass of a level-parallel run): whole rings from step0,
 // the steps from step1 on doing nothing.
 template <typename T, bool BACK, bool XLDS, typename XP>
 __device__ __forceinline__ void solve_stream(GLOBAL const v4i* rec, int step0, int step1, GLOBAL const T* vals,
-                                             GLOBAL const T* dval, XP x, int lane) {
+                                             GLOBAL const T* dval, XP x, LDSP T* scr, int lane) {
   rec += (size_t)step0 * kWave;
   const int steps = step1 - step0;
   v4i rr[kRing];
@@ -422,18 +426,35 @@ __device__ __forceinline__ void solve_stream(GLOBAL const v4i* rec, int step0, i
     __builtin_amdgcn_sched_barrier(0);
   }
   T acc = T(0);
+  int sbase = 0;   // scratch slot of this PRODUCT step's lane 0
   for (int base = 0; base < steps; base += kRing) {
 #pragma unroll
     for (int s = 0; s < kRing; s++) {
       const int t = base + s;
-      v4i r4 = rr[s];
-      if (t >= steps) r4 = (v4i){-1, 0, -1, 0};
-      const int fl = r4.w;
-      if (fl & kStepBegin) acc = r4.z >= 0 ? x[r4.z] : T(0);
-      if (r4.x >= 0) acc = acc - rv[s] * x[r4.x];
-      if (fl & kStepEnd) {
-        if (r4.z >= 0) x[r4.z] = BACK ? acc / rd[s] : acc;
+      const v4i r4 = rr[s];
+      // the step's kind (and a CHAIN's longest row) is the same in every lane: a scalar, taken at the step itself
+      int meta = __builtin_amdgcn_readfirstlane(r4.w);
+      if (t >= steps) meta = 0;
+      const int kind = meta & kStepKindMask;
+      if (kind == kStepProduct) {
+        const int c = r4.x;
+        const T xv = x[c >= 0 ? c : 0];
+        scr[sbase + lane] = rv[s] * xv;   // (an idle lane's slot is never read)
+        sbase += kWave;
+        WAVE_SYNC();
+      } else if (kind == kStepChain) {
+        const int row = r4.z, st = r4.x & 0xFFFF, len = (int)((unsigned)r4.x >> 16);
+        const int maxlen = (int)((unsigned)meta >> kChainLenShift);
+        const T x0 = x[row >= 0 ? row : 0];
+        if (r4.w & kChainBegin) acc = x0;
+#pragma unroll 4
+        for (int e = 0; e < maxlen; e++) {
+          const T pv = scr[st + (e < len ? e : 0)];
+          acc = e < len ? acc - pv : acc;
+        }
+        if ((r4.w & kChainEnd) && row >= 0) x[row] = BACK ? acc / rd[s] : acc;
         if (XLDS) WAVE_SYNC(); else WG_SYNC();
+        sbase = 0;
       }
       rr[s] = rec[(size_t)(t + kRing) * kWave + lane];
       const int hs = (s + kHalf) % kRing;
@@ -446,14 +467,14 @@ __device__ __forceinline__ void solve_stream(GLOBAL const v4i* rec, int step0, i
 template <typename T, bool XLDS, typename XP>
 __device__ __forceinline__ void lu_solve(const DevPlan& P, GLOBAL const T* fval, GLOBAL const T* bval,
                                          GLOBAL const T* dval, GLOBAL const double* rhs, GLOBAL double* out, XP x,
-                                         int lane) {
+                                         LDSP T* scr, int lane) {
   const int m = P.m;
   GLOBAL const int* pinv = glob(P.pinv);
   GLOBAL const int* q = glob(P.q);
   for (int r = lane; r < m; r += kWave) x[pinv[r]] = (T)rhs[r];
   if (XLDS) WAVE_SYNC(); else WG_SYNC();
-  solve_stream<T, false, XLDS>(globv<v4i>(P.fs_rec), 0, P.fs_steps, fval, dval, x, lane);
-  solve_stream<T, true, XLDS>(globv<v4i>(P.bs_rec), 0, P.bs_steps, bval, dval, x, lane);
+  solve_stream<T, false, XLDS>(globv<v4i>(P.fs_rec), 0, P.fs_steps, fval, dval, x, scr, lane);
+  solve_stream<T, true, XLDS>(globv<v4i>(P.bs_rec), 0, P.bs_steps, bval, dval, x, scr, lane);
   for (int k = lane; k < m; k += kWave) out[q[k]] = (double)x[k];
   WG_SYNC();
 }
@@ -571,12 +592,12 @@ __global__ __launch_bounds__(kWave * (PAIR ? kPairWaves : 1), 2) void nr_batch_k
   auto solve = [&](GLOBAL const double* rhs, GLOBAL double* out) {
     if constexpr (PAIR) {
       if (!take_lock(x_lock, lane)) { lock_timeout = 1; return; }
-      lu_solve<T, true>(P, I.fval, I.bval, I.dval, rhs, out, xl, opaque_lane(lane));
+      lu_solve<T, true>(P, I.fval, I.bval, I.dval, rhs, out, xl, ws, opaque_lane(lane));
       give_lock(x_lock, lane);
     } else if constexpr (XLDS) {
-      lu_solve<T, true>(P, I.fval, I.bval, I.dval, rhs, out, xl, opaque_lane(lane));
+      lu_solve<T, true>(P, I.fval, I.bval, I.dval, rhs, out, xl, ws, opaque_lane(lane));
     } else {
-      lu_solve<T, false>(P, I.fval, I.bval, I.dval, rhs, out, I.xg, opaque_lane(lane));
+      lu_solve<T, false>(P, I.fval, I.bval, I.dval, rhs, out, I.xg, ws, opaque_lane(lane));
     }
   };
   for (;;) {
@@ -825,9 +846,11 @@ __global__ __launch_bounds__(kWave) void wide_solve(const InstDesc* __restrict__
   const int p = first_pass + blockIdx.x * per;
   const int* pass_step = BACK ? P.bs_pass_step : P.fs_pass_step;
   GLOBAL T* x = glob(reinterpret_cast<T*>(d.xg));
+  extern __shared__ __align__(16) unsigned char smem[];
   solve_stream<T, BACK, false>(globv<v4i>(BACK ? P.bs_rec : P.fs_rec), pass_step[p], pass_step[p + per],
                                glob(reinterpret_cast<const T*>(BACK ? d.bval : d.fval)),
-                               glob(reinterpret_cast<const T*>(d.dval)), x, opaque_lane(threadIdx.x));
+                               glob(reinterpret_cast<const T*>(d.dval)), x, lds(reinterpret_cast<T*>(smem)),
+                               opaque_lane(threadIdx.x));
 }
 
 // dx (or r) = the column-permuted solve vector
@@ -1206,7 +1229,7 @@ __global__ __launch_bounds__(kWave) void seam_solve_kernel(const SeamDesc* __res
     if (lane == 0) *d.status = 1;
     return;
   }
-  lu_solve<double, false>(P, glob(d.fval), glob(d.bval), glob(d.dval), glob(d.b), glob(d.x), glob(d.xs), lane);
+  lu_solve<double, false>(P, glob(d.fval), glob(d.bval), glob(d.dval), glob(d.b), glob(d.x), glob(d.xs), ws, lane);
   if (lane == 0) *d.status = 0;
 }
 
@@ -1296,10 +1319,10 @@ static void enqueue_wide_step(const InstDesc* descs, const DevPlan& P, int count
   auto solve = [&](int use_r) {
     hipLaunchKernelGGL(wide_permute<T>, dim3(m256, cnt), b256, 0, s, descs, P, use_r);
     segments(fs_level_pass, [&](int p0, int p1, int blocks) {
-      hipLaunchKernelGGL((wide_solve<T, false>), dim3(blocks, cnt), w, 0, s, descs, P, p0, p1);
+      hipLaunchKernelGGL((wide_solve<T, false>), dim3(blocks, cnt), w, ws_bytes, s, descs, P, p0, p1);
     });
     segments(bs_level_pass, [&](int p0, int p1, int blocks) {
-      hipLaunchKernelGGL((wide_solve<T, true>), dim3(blocks, cnt), w, 0, s, descs, P, p0, p1);
+      hipLaunchKernelGGL((wide_solve<T, true>), dim3(blocks, cnt), w, ws_bytes, s, descs, P, p0, p1);
     });
     hipLaunchKernelGGL(wide_unpermute<T>, dim3(m256, cnt), b256, 0, s, descs, P, use_r);
   };
