@@ -72,9 +72,9 @@ void upload_plan(DevicePlan& dp, const YBus* y, const JacobianPattern& jp, const
   // ring ahead of that)
   std::vector<int> rs_rec(d.rs_rec);
   for (int r = d.n_rs; r < h.n_rs + 2 * kStreamPad; r++)
-    for (int l = 0; l < 64; l++) { rs_rec.push_back(0xFFFF); rs_rec.push_back(0); rs_rec.push_back(0); rs_rec.push_back(0); }
+    for (int l = 0; l < 64; l++) { rs_rec.push_back(0); rs_rec.push_back(0); }
   upload(dp.rs_rec, rs_rec, s); upload(dp.ch_a_src, d.ch_a_src, s);
-  h.rs_rec = dp.rs_rec.as<int4>(); h.ch_a_src = dp.ch_a_src.as<int>();
+  h.rs_rec = dp.rs_rec.as<int2>(); h.ch_a_src = dp.ch_a_src.as<int>();
   // the solves' step streams (padded with steps that touch nothing) and the factors' stream-order sizes
   h.fs_steps = ring_up(d.fs_steps);
   h.bs_steps = ring_up(d.bs_steps);
@@ -82,13 +82,13 @@ void upload_plan(DevicePlan& dp, const YBus* y, const JacobianPattern& jp, const
   h.n_bval = d.n_bval;
   auto pad_steps = [](const std::vector<int>& rec, int steps) {
     std::vector<int> prec(rec);
-    for (int t = (int)(rec.size() / 256); t < steps + 2 * kStreamPad; t++)
-      for (int l = 0; l < 64; l++) { prec.push_back(-1); prec.push_back(0); prec.push_back(-1); prec.push_back(0); }
+    for (int t = (int)(rec.size() / 128); t < steps + 2 * kStreamPad; t++)
+      for (int l = 0; l < 64; l++) { prec.push_back(-1); prec.push_back(0); }
     return prec;
   };
   std::vector<int> fs_rec = pad_steps(d.fs_rec, h.fs_steps), bs_rec = pad_steps(d.bs_rec, h.bs_steps);
   upload(dp.fs_rec, fs_rec, s); upload(dp.bs_rec, bs_rec, s);
-  h.fs_rec = dp.fs_rec.as<int4>(); h.bs_rec = dp.bs_rec.as<int4>();
+  h.fs_rec = dp.fs_rec.as<int2>(); h.bs_rec = dp.bs_rec.as<int2>();
   // chunk and pass boundaries for level-parallel schedules
   upload(dp.rs_chunk_row, d.rs_chunk_row, s);
   upload(dp.fs_pass_step, d.fs_pass_step, s);
@@ -144,7 +144,7 @@ gs_plan_stats plan_stats(const DevicePlanHost& d, const YBus& y, const JacobianP
   st.forward_steps = ring_up(d.fs_steps);
   st.back_steps = ring_up(d.bs_steps);
   st.workspace_stride = d.wstride;
-  const int64_t n = y.n, nnz_y = y.rp.empty() ? 0 : y.rp[y.n], m = jp.m, rec_row = 64 * 16;
+  const int64_t n = y.n, nnz_y = y.rp.empty() ? 0 : y.rp[y.n], m = jp.m, rec_row = 64 * 8;
   int64_t b = 4 * (n + 1) + 4 * nnz_y * 2 + 4 * n + 16 * nnz_y + 2 * 16 * nnz_y + 2 * 4 * n;  // Y-bus, slots, pos
   b += 4 * (m + 1) + 4 * (int64_t)st.nnz_j + 2 * 8 * n + 4 * 4 * m;                            // J pattern, start, pivots
   b += rec_row * (int64_t)(st.refactor_rows + 2 * kStreamPad) + 4 * (int64_t)d.ch_a_src.size();
@@ -156,12 +156,13 @@ gs_plan_stats plan_stats(const DevicePlanHost& d, const YBus& y, const JacobianP
     if (kind < 5) st.rows_by_kind[kind]++;
     if (kind != DevicePlanHost::kRsUpdate) continue;
     for (int l = 0; l < 64; l++)
-      st.update_lanes += (d.rs_rec[((size_t)r * 64 + l) * 4] & 0xFFFF) != d.spare(l);
+      st.update_lanes += (d.rs_rec[((size_t)r * 64 + l) * 2] & 0xFFFF) != d.spare(l);
   }
-  for (size_t k = 0; k < d.fs_rec.size(); k += 4)
-    st.forward_lanes += (d.fs_rec[k + 3] & DevicePlanHost::kStepKindMask) == DevicePlanHost::kStepProduct && d.fs_rec[k] >= 0;
-  for (size_t k = 0; k < d.bs_rec.size(); k += 4)
-    st.back_lanes += (d.bs_rec[k + 3] & DevicePlanHost::kStepKindMask) == DevicePlanHost::kStepProduct && d.bs_rec[k] >= 0;
+  auto product = [](int y) {
+    return (((unsigned)y >> DevicePlanHost::kRsMetaShift) & DevicePlanHost::kStepKindMask) == DevicePlanHost::kStepProduct;
+  };
+  for (size_t k = 0; k < d.fs_rec.size(); k += 2) st.forward_lanes += product(d.fs_rec[k + 1]) && d.fs_rec[k] >= 0;
+  for (size_t k = 0; k < d.bs_rec.size(); k += 2) st.back_lanes += product(d.bs_rec[k + 1]) && d.bs_rec[k] >= 0;
   return st;
 }
 

@@ -748,22 +748,20 @@ void compile_plan(const Plan& plan, DevicePlanHost& d) {
                 idx = n_val++;
                 pos[ent_idx[e]] = idx;
               }
-              rec.push_back(col); rec.push_back(idx); rec.push_back(-1); rec.push_back(D::kStepProduct);
+              rec.push_back(col); rec.push_back(idx | (D::kStepProduct << D::kRsMetaShift));
             }
             steps++;
           }
           // the CHAIN step
-          int maxlen = 0;
-          for (int l = 0; l < k; l++) maxlen = std::max(maxlen, take[l]);
           for (int lane = 0; lane < 64; lane++) {
-            int x0 = 0, row = -1, fl = D::kStepChain | (maxlen << D::kChainLenShift);
+            int x0 = 0, row = D::kRsIndexMask, fl = D::kStepChain;
             if (lane < k && !ended[lane] && (take[lane] > 0 || done[lane] == len[lane])) {
               row = rows[p0 + lane];
               x0 = st[lane] | (take[lane] << 16);
               if (!begun[lane]) { fl |= D::kChainBegin; begun[lane] = 1; }
               if (done[lane] + take[lane] == len[lane]) { fl |= D::kChainEnd; ended[lane] = 1; }
             }
-            rec.push_back(x0); rec.push_back(0); rec.push_back(row); rec.push_back(fl);
+            rec.push_back(x0); rec.push_back(row | (fl << D::kRsMetaShift));
           }
           steps++;
           bool all = true;
@@ -795,15 +793,17 @@ void compile_plan(const Plan& plan, DevicePlanHost& d) {
     for (int j = 0; j < m; j++)
       for (int p = plan.lp[j]; p < plan.lp[j + 1]; p++) col_of_l[p] = j;
     vector<int> level_end_row(n_levels, -1);
-    auto emit = [&](int kind, int lv, const int* rec4) {
+    auto emit = [&](int kind, int lv, const int* rec2) {
       d.rs_meta.push_back(kind);
       d.rs_level.push_back(lv);
-      d.rs_rec.insert(d.rs_rec.end(), rec4, rec4 + 4 * 64);
+      d.rs_rec.insert(d.rs_rec.end(), rec2, rec2 + 2 * 64);
     };
     if ((int64_t)d.n_a + d.nnz_l > D::kRsIndexMask) throw std::runtime_error("plan: too many values for the row stream");
-    vector<int> rec(4 * 64);
+    if (d.n_fval > D::kRsIndexMask || d.n_bval > D::kRsIndexMask || m > D::kRsIndexMask)
+      throw std::runtime_error("plan: factor positions too large for the row stream");
+    vector<int> rec(2 * 64);
     auto nop = [&](int lv) {
-      for (int l = 0; l < 64; l++) { rec[4 * l] = 0xFFFF; rec[4 * l + 1] = 0; rec[4 * l + 2] = 0; rec[4 * l + 3] = 0; }
+      std::fill(rec.begin(), rec.end(), 0);
       emit(D::kRsNop, lv, rec.data());
     };
     d.rs_chunk_row.clear();
@@ -814,20 +814,18 @@ void compile_plan(const Plan& plan, DevicePlanHost& d) {
       if (lv + 1 < (int)d.rs_level_chunk.size()) throw std::runtime_error("plan: chunks out of level order");
       while ((int)d.rs_level_chunk.size() <= lv) d.rs_level_chunk.push_back(c);
       d.rs_chunk_row.push_back((int)d.rs_meta.size());
-      // ASSIGN rows
+      // ASSIGN rows: (slot, A value index)
       const int a0 = d.ch_a_ptr[c], a1 = d.ch_a_ptr[c + 1];
       for (int e0 = a0; e0 < std::max(a1, a0 + 1); e0 += 64) {
         for (int l = 0; l < 64; l++) {
           const int e = e0 + l;
           const bool in = e < a1;
-          rec[4 * l] = in ? d.ch_a_ws[e] : d.spare(l);
-          rec[4 * l + 1] = in ? e : 0;
-          rec[4 * l + 2] = 0;
-          rec[4 * l + 3] = 0;
+          rec[2 * l] = in ? d.ch_a_ws[e] : d.spare(l);
+          rec[2 * l + 1] = in ? e : 0;
         }
         emit(D::kRsAssign | (e0 == a0 ? D::kRsClear : 0), lv, rec.data());
       }
-      // UPDATE rows
+      // UPDATE rows: (dst | src << 16, L value index)
       for (int r = d.ch_row_ptr[c]; r < d.ch_row_ptr[c + 1]; r++) {
         int need = -1;
         bool any = false;
@@ -843,49 +841,47 @@ void compile_plan(const Plan& plan, DevicePlanHost& d) {
         for (int l = 0; l < 64; l++) {
           const int rr = d.row_rec[(size_t)r * 64 + l];
           const bool in = (rr & 0xFFFF) != 0xFFFF;
-          rec[4 * l] = in ? rr : d.spare(l) | (d.spare(l) << 16);
-          rec[4 * l + 1] = in ? d.n_a + d.row_lsrc[(size_t)r * 64 + l] : 0;
-          rec[4 * l + 2] = 0;
-          rec[4 * l + 3] = 0;
+          rec[2 * l] = in ? rr : d.spare(l) | (d.spare(l) << 16);
+          rec[2 * l + 1] = in ? d.n_a + d.row_lsrc[(size_t)r * 64 + l] : 0;
         }
         emit(D::kRsUpdate, lv, rec.data());
       }
-      // PIVOT row
+      // PIVOT row: (pivot slot | candidates << 16, pivot step), -1 for an idle track
       int wrows = 0;
       for (int l = 0; l < 64; l++) {
         const int g = l / TL;
-        rec[4 * l] = 0; rec[4 * l + 1] = 0; rec[4 * l + 2] = 0; rec[4 * l + 3] = -1;
+        rec[2 * l] = -1;
+        rec[2 * l + 1] = 0;
         if (g < ntr) {
           const int k = d.tr_col[t0 + g], nu = plan.up[k + 1] - plan.up[k], nl = plan.lp[k + 1] - plan.lp[k];
           const int ps = g * d.wstride + nu - 1;
-          rec[4 * l] = ps;
-          rec[4 * l + 2] = (ps + 1) | ((ps + nl) << 16);
-          rec[4 * l + 3] = k;
+          rec[2 * l] = ps | ((nl - 1) << 16);   // candidates: slots ps + 1 .. ps + nl - 1
+          rec[2 * l + 1] = k;
           wrows = std::max(wrows, (nu + nl - 1 + TL - 1) / TL);
         }
       }
       emit(D::kRsPivot, lv, rec.data());
-      // WRITE rows
+      // WRITE rows: (fval position | kind << 24, bval / dval / vals position); write row w of a chunk writes slot
+      // t = w * kTrackLanes + lane % kTrackLanes of the lane's track
       for (int w = 0; w < wrows; w++) {
         for (int l = 0; l < 64; l++) {
           const int g = l / TL, t = w * TL + l % TL;
-          rec[4 * l] = 0; rec[4 * l + 1] = 0; rec[4 * l + 2] = 0; rec[4 * l + 3] = 0;
+          rec[2 * l] = 0;
+          rec[2 * l + 1] = 0;
           if (g >= ntr) continue;
           const int k = d.tr_col[t0 + g], u0 = plan.up[k], nu = plan.up[k + 1] - u0, l0 = plan.lp[k];
           const int nl = plan.lp[k + 1] - l0;
           if (t >= nu + nl - 1) continue;
-          const int slot = g * d.wstride + t;
           if (t < nu - 1) {
-            rec[4 * l] = slot | (D::kWriteU << 16);
-            rec[4 * l + 2] = d.bpos[u0 + t];
+            rec[2 * l] = D::kWriteU << 24;
+            rec[2 * l + 1] = d.bpos[u0 + t];
           } else if (t == nu - 1) {
-            rec[4 * l] = slot | (D::kWritePivot << 16);
-            rec[4 * l + 2] = k;
+            rec[2 * l] = D::kWritePivot << 24;
+            rec[2 * l + 1] = k;
           } else {
             const int p = l0 + (t - (nu - 1));
-            rec[4 * l] = slot | (D::kWriteL << 16);
-            rec[4 * l + 2] = d.n_a + p;
-            rec[4 * l + 3] = d.fpos[p];
+            rec[2 * l] = d.fpos[p] | (D::kWriteL << 24);
+            rec[2 * l + 1] = d.n_a + p;
           }
         }
         emit(D::kRsWrite, lv, rec.data());
@@ -900,7 +896,7 @@ void compile_plan(const Plan& plan, DevicePlanHost& d) {
     while ((int)d.rs_level_chunk.size() <= n_levels) d.rs_level_chunk.push_back(d.n_chunks);
     // fold each row's kind and flags into its records
     for (int r = 0; r < d.n_rs; r++)
-      for (int l = 0; l < 64; l++) d.rs_rec[((size_t)r * 64 + l) * 4 + 1] |= d.rs_meta[r] << D::kRsMetaShift;
+      for (int l = 0; l < 64; l++) d.rs_rec[((size_t)r * 64 + l) * 2 + 1] |= d.rs_meta[r] << D::kRsMetaShift;
   }
   // For the exact pivot check: how many candidates (rows not yet pivotal at step k) precede the pivot in reach order.
   d.piv_pos.assign(m, 0);
@@ -945,7 +941,7 @@ bool check_level_schedule(const DevicePlanHost& d, int ring, int padded_rows, in
         if (r > r0 && (d.rs_meta[r] & D::kRsClear)) return bad("a workspace clear inside a chunk");
         if ((d.rs_meta[r] & D::kRsKindMask) != D::kRsUpdate) continue;
         for (int lane = 0; lane < 64; lane++) {
-          const int* rr = &d.rs_rec[((size_t)r * 64 + lane) * 4];
+          const int* rr = &d.rs_rec[((size_t)r * 64 + lane) * 2];
           if ((rr[0] & 0xFFFF) == d.spare(lane)) continue;
           const int v = rr[1] & D::kRsIndexMask;
           if (v < d.n_a || v >= n_vals || val_level[v] < 0 || val_level[v] >= l)
@@ -958,16 +954,16 @@ bool check_level_schedule(const DevicePlanHost& d, int ring, int padded_rows, in
       for (int r = d.rs_chunk_row[c]; r < d.rs_chunk_row[c + 1]; r++) {
         if ((d.rs_meta[r] & D::kRsKindMask) != D::kRsWrite) continue;
         for (int lane = 0; lane < 64; lane++) {
-          const int* rr = &d.rs_rec[((size_t)r * 64 + lane) * 4];
-          const int wk = rr[0] >> 16;
+          const int* rr = &d.rs_rec[((size_t)r * 64 + lane) * 2];
+          const int wk = (int)((unsigned)rr[0] >> 24), pos = rr[1] & D::kRsIndexMask;
           if (wk == 0) continue;
           int* owner = nullptr;
-          if (wk == D::kWriteU) owner = &bpos_chunk.at(rr[2]);
-          else if (wk == D::kWritePivot) owner = &dpos_chunk.at(rr[2]);
+          if (wk == D::kWriteU) owner = &bpos_chunk.at(pos);
+          else if (wk == D::kWritePivot) owner = &dpos_chunk.at(pos);
           else {
-            if (val_level.at(rr[2]) >= 0) return bad("an L value written twice");
-            val_level[rr[2]] = l;
-            owner = &fpos_chunk.at(rr[3]);
+            if (val_level.at(pos) >= 0) return bad("an L value written twice");
+            val_level[pos] = l;
+            owner = &fpos_chunk.at(rr[0] & D::kRsIndexMask);
           }
           if (*owner >= 0 && *owner != c) return bad("two chunks write one factor position");
           *owner = c;
@@ -990,15 +986,16 @@ bool check_level_schedule(const DevicePlanHost& d, int ring, int padded_rows, in
         const int s0 = pass_step[p], s1 = pass_step[p + 1];
         if (s1 <= s0) return bad("an empty pass");
         if (s0 + (s1 - s0 + ring - 1) / ring * ring - 1 + ring >= padded) return bad("a pass's look-ahead passes the padding");
-        if ((rec[((size_t)(s1 - 1) * 64) * 4 + 3] & D::kStepKindMask) != D::kStepChain)
+        if ((((unsigned)rec[((size_t)(s1 - 1) * 64) * 2 + 1] >> D::kRsMetaShift) & D::kStepKindMask) != D::kStepChain)
           return bad("a pass does not end on its own CHAIN step");
         for (int t = s0; t < s1; t++)
           for (int lane = 0; lane < 64; lane++) {
-            const int* r4 = &rec[((size_t)t * 64 + lane) * 4];
-            const int kind = r4[3] & D::kStepKindMask;
-            if (kind == D::kStepProduct && r4[0] >= 0 && (row_level.at(r4[0]) == -1 || row_level[r4[0]] >= l))
+            const int* r2 = &rec[((size_t)t * 64 + lane) * 2];
+            const int meta = (int)((unsigned)r2[1] >> D::kRsMetaShift), kind = meta & D::kStepKindMask;
+            const int field = r2[1] & D::kRsIndexMask;
+            if (kind == D::kStepProduct && r2[0] >= 0 && (row_level.at(r2[0]) == -1 || row_level[r2[0]] >= l))
               return bad("a pass reads a row its level's predecessors did not finish");
-            if (kind == D::kStepChain && r4[2] >= 0 && (r4[3] & D::kChainEnd)) rows.push_back(r4[2]);
+            if (kind == D::kStepChain && field != D::kRsIndexMask && (meta & D::kChainEnd)) rows.push_back(field);
           }
       }
       for (int r : rows) {
@@ -1029,12 +1026,13 @@ bool emulate_device_lu(const DevicePlanHost& d, int nnz_j, const double* jv, con
     if (e < -1 || e >= n_a) return bad("j2a index");
     if (e >= 0) vals[e] = jv[s];
   }
-  if ((int)d.rs_meta.size() != d.n_rs || (int)d.rs_rec.size() != 4 * 64 * d.n_rs) return bad("row stream size");
+  if ((int)d.rs_meta.size() != d.n_rs || (int)d.rs_rec.size() != 2 * 64 * d.n_rs) return bad("row stream size");
   // The device loads row r's value at the end of row r - kRsValueAhead (or before the first row). fence_row[v] is the
   // row whose level-end fence publishes vals[v] (-1: final before the refactor starts).
   vector<int> fence_row(n_vals, -1), written_row(n_vals, -1);
   vector<int> pending;  // values written in the current level, published at its level end
   vector<double> piv(64, 0.0);
+  int write_row = 0;   // WRITE rows since the chunk's PIVOT row
   // the reference's update order: per chunk, each workspace slot's updates come in increasing U entry t, and a slot
   // is read (as an update's source) only after all of its own updates
   vector<int> n_pending(wsz, 0), last_t(wsz, -1);
@@ -1048,7 +1046,7 @@ bool emulate_device_lu(const DevicePlanHost& d, int nnz_j, const double* jv, con
         if (r2 > r && (d.rs_meta[r2] & D::kRsKindMask) == D::kRsAssign && (d.rs_meta[r2] & D::kRsClear)) break;
         if ((d.rs_meta[r2] & D::kRsKindMask) != D::kRsUpdate) continue;
         for (int lane = 0; lane < 64; lane++) {
-          const int x = d.rs_rec[((size_t)r2 * 64 + lane) * 4], dst = x & 0xFFFF;
+          const int x = d.rs_rec[((size_t)r2 * 64 + lane) * 2], dst = x & 0xFFFF;
           if (dst < wsz) n_pending[dst]++;
         }
       }
@@ -1056,7 +1054,7 @@ bool emulate_device_lu(const DevicePlanHost& d, int nnz_j, const double* jv, con
     if (kind == D::kRsUpdate) {
       vector<char> seen(wsz, 0);
       for (int lane = 0; lane < 64; lane++) {
-        const int x = d.rs_rec[((size_t)r * 64 + lane) * 4], dst = x & 0xFFFF, src = (int)((unsigned)x >> 16);
+        const int x = d.rs_rec[((size_t)r * 64 + lane) * 2], dst = x & 0xFFFF, src = (int)((unsigned)x >> 16);
         if (dst == d.spare(lane)) continue;
         if (dst >= wsz || src >= wsz) return bad("row record outside the workspace");
         const int t = src % d.wstride;
@@ -1066,18 +1064,19 @@ bool emulate_device_lu(const DevicePlanHost& d, int nnz_j, const double* jv, con
         if (t <= last_t[dst]) return bad("a slot's updates out of the reference's order");
       }
       for (int lane = 0; lane < 64; lane++) {
-        const int x = d.rs_rec[((size_t)r * 64 + lane) * 4], dst = x & 0xFFFF, src = (int)((unsigned)x >> 16);
+        const int x = d.rs_rec[((size_t)r * 64 + lane) * 2], dst = x & 0xFFFF, src = (int)((unsigned)x >> 16);
         if (dst == d.spare(lane)) continue;
         n_pending[dst]--;
         last_t[dst] = src % d.wstride;
       }
     }
     bool fail = false;
+    if (kind == D::kRsPivot) write_row = 0;
     for (int lane = 0; lane < 64; lane++) {
-      const int* rr = &d.rs_rec[((size_t)r * 64 + lane) * 4];
+      const int* rr = &d.rs_rec[((size_t)r * 64 + lane) * 2];
       if ((int)((unsigned)rr[1] >> D::kRsMetaShift) != meta) return bad("row kind differs across lanes");
-      const int rc[4] = {rr[0], rr[1] & D::kRsIndexMask, rr[2], rr[3]};
-      if (rc[1] < 0 || rc[1] >= n_vals) return bad("row value index");
+      const int rc[2] = {rr[0], rr[1] & D::kRsIndexMask};
+      if ((kind == D::kRsAssign || kind == D::kRsUpdate) && rc[1] >= n_vals) return bad("row value index");
       const int load_row = r - D::kRsValueAhead;  // the value is loaded at the end of this row
       if (kind == D::kRsAssign) {
         if (rc[0] == d.spare(lane)) continue;
@@ -1093,28 +1092,30 @@ bool emulate_device_lu(const DevicePlanHost& d, int nnz_j, const double* jv, con
         if (fence_row[rc[1]] < 0 || fence_row[rc[1]] > load_row) return bad("row value loaded before its fence");
         ws[dst] = ws[dst] - vals[rc[1]] * ws[xj];
       } else if (kind == D::kRsPivot) {
-        if (rc[3] < 0) continue;
-        const int ps = rc[0], lo = rc[2] & 0xFFFF, hi = rc[2] >> 16;
+        if (rc[0] == -1) continue;
+        const int ps = rc[0] & 0xFFFF, lo = ps + 1, hi = ps + 1 + (int)((unsigned)rc[0] >> 16);
         if (ps < 0 || ps >= wsz || ps / d.wstride != lane / TL || lo != ps + 1 || hi < lo || hi > (lane / TL + 1) * d.wstride)
           return bad("pivot record");
         piv[lane] = ws[ps];
         fail |= piv[lane] == 0.0;
       } else if (kind == D::kRsWrite) {
-        const int wk = rc[0] >> 16, slot = rc[0] & 0xFFFF;
+        const int wk = (int)((unsigned)rc[0] >> 24);
+        const int slot = (lane / TL) * d.wstride + write_row * TL + lane % TL;
         if (wk == 0) continue;
         if (slot >= wsz || slot / d.wstride != lane / TL) return bad("write slot");
         if (wk == D::kWriteU) {
-          if (rc[2] < 0 || rc[2] >= d.n_bval) return bad("U write position");
-          bval[rc[2]] = ws[slot];
+          if (rc[1] >= d.n_bval) return bad("U write position");
+          bval[rc[1]] = ws[slot];
         } else if (wk == D::kWritePivot) {
-          if (rc[2] < 0 || rc[2] >= m) return bad("pivot write position");
-          dval[rc[2]] = ws[slot];
+          if (rc[1] >= m) return bad("pivot write position");
+          dval[rc[1]] = ws[slot];
         } else if (wk == D::kWriteL) {
-          if (rc[2] < n_a || rc[2] >= n_vals || rc[3] < 0 || rc[3] >= d.n_fval) return bad("L write position");
+          const int fp = rc[0] & D::kRsIndexMask;
+          if (rc[1] < n_a || rc[1] >= n_vals || fp >= d.n_fval) return bad("L write position");
           if (piv[lane] == 0.0) return bad("L write without a pivot");
-          vals[rc[2]] = fval[rc[3]] = ws[slot] / piv[lane];
-          written_row[rc[2]] = r;
-          pending.push_back(rc[2]);
+          vals[rc[1]] = fval[fp] = ws[slot] / piv[lane];
+          written_row[rc[1]] = r;
+          pending.push_back(rc[1]);
         } else {
           return bad("write kind");
         }
@@ -1123,6 +1124,7 @@ bool emulate_device_lu(const DevicePlanHost& d, int nnz_j, const double* jv, con
       }
     }
     if (fail) return bad("zero pivot");
+    if (kind == D::kRsWrite) write_row++;
     if (meta & D::kRsLevelEnd) {
       for (int v : pending) fence_row[v] = r;
       pending.clear();
@@ -1139,29 +1141,29 @@ bool emulate_device_lu(const DevicePlanHost& d, int nnz_j, const double* jv, con
     vector<double> acc(64, 0.0), scratch(cap, std::nan(""));
     vector<int> row(64, -1);
     int sbase = 0;
-    if ((int)rec.size() != steps * 64 * 4) { why = "stream size"; return false; }
+    if ((int)rec.size() != steps * 64 * 2) { why = "stream size"; return false; }
     for (int t = 0; t < steps; t++) {
-      const int kind = rec[((size_t)t * 64) * 4 + 3] & D::kStepKindMask;
-      const int maxlen = (unsigned)rec[((size_t)t * 64) * 4 + 3] >> D::kChainLenShift;
+      const int kind = (int)((unsigned)rec[((size_t)t * 64) * 2 + 1] >> D::kRsMetaShift) & D::kStepKindMask;
       vector<int> ended;
       for (int lane = 0; lane < 64; lane++) {
-        const int* r4 = &rec[((size_t)t * 64 + lane) * 4];
-        if ((r4[3] & D::kStepKindMask) != kind || ((unsigned)r4[3] >> D::kChainLenShift) != (unsigned)maxlen)
-          { why = "stream step kind differs across lanes"; return false; }
+        const int* r2 = &rec[((size_t)t * 64 + lane) * 2];
+        const int meta = (int)((unsigned)r2[1] >> D::kRsMetaShift), field = r2[1] & D::kRsIndexMask;
+        if ((meta & D::kStepKindMask) != kind) { why = "stream step kind differs across lanes"; return false; }
         if (kind == D::kStepProduct) {
           if (sbase + 64 > cap) { why = "products pass the scratch area"; return false; }
-          if (r4[0] < 0) continue;
-          if (r4[0] >= m || !final_rows[r4[0]]) { why = "stream reads a row not yet final"; return false; }
-          if (r4[1] < 0 || r4[1] >= (int)vals.size()) { why = "stream value index"; return false; }
-          scratch[sbase + lane] = vals[r4[1]] * x[r4[0]];
+          if (r2[0] < 0) continue;
+          if (r2[0] >= m || !final_rows[r2[0]]) { why = "stream reads a row not yet final"; return false; }
+          if (field >= (int)vals.size()) { why = "stream value index"; return false; }
+          scratch[sbase + lane] = vals[field] * x[r2[0]];
         } else if (kind == D::kStepChain) {
-          if (r4[2] < 0) continue;
-          const int st = r4[0] & 0xFFFF, len = (unsigned)r4[0] >> 16;
-          if (len > maxlen || st + len > cap) { why = "chain outside the scratch area"; return false; }
-          if (r4[3] & D::kChainBegin) {
-            row[lane] = r4[2];
+          if (field == D::kRsIndexMask) continue;
+          if (field >= m) { why = "chain row"; return false; }
+          const int st = r2[0] & 0xFFFF, len = (int)((unsigned)r2[0] >> 16);
+          if (st + len > cap) { why = "chain outside the scratch area"; return false; }
+          if (meta & D::kChainBegin) {
+            row[lane] = field;
             acc[lane] = x[row[lane]];
-          } else if (row[lane] != r4[2]) {
+          } else if (row[lane] != field) {
             why = "a chain continues another lane's row";
             return false;
           }
@@ -1169,7 +1171,7 @@ bool emulate_device_lu(const DevicePlanHost& d, int nnz_j, const double* jv, con
             if (std::isnan(scratch[e])) { why = "chain reads a product not formed"; return false; }
             acc[lane] = acc[lane] - scratch[e];
           }
-          if (r4[3] & D::kChainEnd) {
+          if (meta & D::kChainEnd) {
             x[row[lane]] = back ? acc[lane] / dval[row[lane]] : acc[lane];
             ended.push_back(row[lane]);
             row[lane] = -1;

@@ -136,14 +136,16 @@ struct DevicePlanHost {
   vector<int> tr_u0, tr_nu, tr_l0, tr_nl;           // per track (chunk order)
   vector<int> j2a;                                  // [nnz_j]: A record of each Jacobian CSR entry, or -1
   // The refactor as the device runs it: one stream of rows, each row one operation of the wavefront with one record
-  // (4 ints) per lane, so the kernel loads every row's record and value a fixed number of rows ahead. The record's y
-  // holds the value index in its low 24 bits and the row's kind and flags (the same in every lane) in its high 8:
+  // (x, y: 2 ints) per lane, so the kernel loads every row's record and value a fixed number of rows ahead. The
+  // record's y holds a 24-bit field (y & kRsIndexMask) and the row's kind and flags (the same in every lane) above it:
   //   ASSIGN  ws[x] = vals[y]                     (a chunk's A entries; kRsClear on its first: zero the workspace)
   //   UPDATE  ws[x & 0xFFFF] -= vals[y] * ws[x >> 16]
   //   (a lane with nothing to assign or update is pointed at its own spare slot spare(lane) past the tracks'
   //   workspaces, so the device runs these rows with every lane active and no branch)
-  //   PIVOT   x = pivot slot, z = candidate slots [z & 0xFFFF, z >> 16), w = pivot step (or -1: idle track)
-  //   WRITE   slot x & 0xFFFF, kind x >> 16: U (bval[z] = ws), PIVOT (dval[z] = ws), L (vals[z] = fval[w] = ws / pivot)
+  //   PIVOT   x = pivot slot ps | (candidates nc << 16) (the candidates are slots ps + 1 .. ps + nc), y = pivot step;
+  //           x = -1: idle track
+  //   WRITE   slot: track base + w * kTrackLanes + lane % kTrackLanes for the chunk's w-th WRITE row; kind x >> 24:
+  //           U (bval[y] = ws), PIVOT (dval[y] = ws), L (vals[y] = fval[x & kRsIndexMask] = ws / pivot), 0: idle
   // vals = [A values (n_a) | L in column order (nnz_l)], so y and the L destinations index one array. kRsLevelEnd marks
   // the last row of a dependency level (the device fences there). No-op rows are inserted so that an UPDATE row stands
   // at least kRsValueAhead rows after the end of every level it reads: its value, loaded that far ahead, is then final.
@@ -153,7 +155,7 @@ struct DevicePlanHost {
   static constexpr int kRsValueAhead = 8;
   static constexpr int kRsMetaShift = 24, kRsIndexMask = (1 << kRsMetaShift) - 1;
   int n_rs = 0, n_a = 0;
-  vector<int> rs_rec, rs_meta, rs_level;            // [4 * 64 * n_rs] (meta folded into y), [n_rs], [n_rs]
+  vector<int> rs_rec, rs_meta, rs_level;            // [2 * 64 * n_rs] (meta folded into y), [n_rs], [n_rs]
   // For level-parallel schedules: chunk c's rows are [rs_chunk_row[c], rs_chunk_row[c + 1]) of the row stream, and
   // level l's chunks are [rs_level_chunk[l], rs_level_chunk[l + 1]).
   vector<int> rs_chunk_row, rs_level_chunk;
@@ -164,21 +166,22 @@ struct DevicePlanHost {
   // descending, then the division by the pivot). A product rounds the same whoever forms it, so every entry is the
   // reference's value, and a long row no longer costs one step per entry. A pass whose products exceed the scratch
   // area runs in segments, each lane's running value kept across them. Records (4 ints per lane and step):
-  //   PRODUCT  (col, value index, -1, kStepProduct): scratch[s + lane] = vals[value index] * x[col] (col = -1: idle),
-  //            s = 64 * (PRODUCT steps since the last CHAIN)
-  //   CHAIN    (start | len << 16, 0, row, kStepChain | kChainBegin? | kChainEnd? | maxlen << kChainLenShift):
-  //            kChainBegin: acc = x[row]; then acc -= scratch[start + t] for t < len; kChainEnd: x[row] = acc (back:
-  //            acc / pivot). row = -1: idle lane. maxlen (the largest len of the step) is the same in every lane.
+  // Records are 2 ints per lane and step, the second holding a 24-bit field and the step's kind and flags above it:
+  //   PRODUCT  (col, value index | kStepProduct << 24): scratch[s + lane] = vals[value index] * x[col] (col = -1:
+  //            idle), s = 64 * (PRODUCT steps since the last CHAIN)
+  //   CHAIN    (start | len << 16, row | (kStepChain | kChainBegin? | kChainEnd?) << 24): kChainBegin: acc = x[row];
+  //            then acc -= scratch[start + t] for t < len; kChainEnd: x[row] = acc (back: acc / pivot). row =
+  //            kRsIndexMask: idle lane.
   // The factors are stored in the order the PRODUCT steps read them: fval holds L's off-diagonal entries in
   // forward-stream order, bval U's off-diagonal entries in back-stream order (the active lanes of a step being a
   // prefix, a step's values are contiguous), dval U's diagonal by pivot step. fpos / bpos map an L / U entry (CSC
   // index) to its position, -1 for a diagonal. The refactor keeps L in column order too (lx, CSC), for its own updates.
   static constexpr int kStepProduct = 1, kStepChain = 2, kStepKindMask = 15;
-  static constexpr int kChainBegin = 16, kChainEnd = 32, kChainLenShift = 8;
+  static constexpr int kChainBegin = 16, kChainEnd = 32;
   // scratch slots the solves may use: the refactor workspace rounded down to whole steps
   int solve_scratch() const { return (kTracks * wstride + kSpareSlots) / 64 * 64; }
   int fs_steps = 0, bs_steps = 0, n_fval = 0, n_bval = 0;
-  vector<int> fs_rec, bs_rec;                       // 4 ints per lane per step
+  vector<int> fs_rec, bs_rec;                       // 2 ints per lane per step
   // pass p of a stream covers steps [pass_step[p], pass_step[p + 1]); level l's passes are [level_pass[l], [l + 1])
   vector<int> fs_pass_step, fs_level_pass, bs_pass_step, bs_level_pass;
   vector<int> fpos, bpos;                           // [nnz_l], [nnz_u]

@@ -28,13 +28,14 @@ struct DevPlan {
   // UPDATE, PIVOT, WRITE; model.h DevicePlanHost), padded with no-op rows to a multiple of kStreamPad rows plus two
   // more kStreamPad, so the kernel loads ahead without bounds checks. n_a: A entries (the first part of vals).
   int32_t wstride, n_rs, n_a, pad_rs;
-  const int4* rs_rec;                 // [n_rs + 2 kStreamPad][64]; y = value index | (kind | flags) << kRsMetaShift
+  const int2* rs_rec;                 // [n_rs + 2 kStreamPad][64] (x, y); y = 24-bit field | (kind | flags) << 24
   const int32_t* ch_a_src;            // per A entry: J CSR position (the seam fills vals from its matrix)
-  // Triangular-solve step streams: per lane and step (col, value index, row, flags). The factors live in the order
-  // the streams read them: fval (L off-diagonals, forward order), bval (U off-diagonals, back order), dval (U's
+  // Triangular-solve step streams: per lane and step (x, y): PRODUCT (col, value index | kind << 24) and CHAIN
+  // (start | len << 16, row | (kind | flags) << 24) steps (model.h DevicePlanHost). The factors live in the order the
+  // PRODUCT steps read them (fval: L off-diagonal, forward order; bval: U off-diagonal, back order; dval: U's
   // diagonal by pivot step).
   int32_t fs_steps, bs_steps, n_fval, n_bval;
-  const int4 *fs_rec, *bs_rec;
+  const int2 *fs_rec, *bs_rec;
   // Level-parallel schedules: first row of each refactor chunk (+ the end), first step of each solve pass (+ the end).
   // The streams carry two kStreamPad rings of padding, so one chunk's or pass's range rounded up to whole rings can
   // load a ring ahead.
@@ -55,7 +56,7 @@ constexpr int kRsMetaShift = 24, kRsIndexMask = (1 << kRsMetaShift) - 1;
 constexpr int kWriteU = 1, kWritePivot = 2, kWriteL = 3;
 // solve step kinds and CHAIN flags (model.h DevicePlanHost)
 constexpr int kStepProduct = 1, kStepChain = 2, kStepKindMask = 15;
-constexpr int kChainBegin = 16, kChainEnd = 32, kChainLenShift = 8;
+constexpr int kChainBegin = 16, kChainEnd = 32;
 // load-ahead distance of the row and step streams, and the padding each stream carries; rows of the refactor load
 // their value kRsValueAhead rows ahead (the plan places its rows for that distance)
 constexpr int kStreamPad = 16;

@@ -285,7 +285,7 @@ template <typename T, bool EXACT>
 __device__ __forceinline__ int refactor(const DevPlan& P, GLOBAL T* vals, GLOBAL T* fval, GLOBAL T* bval,
                                         GLOBAL T* dval, LDSP T* ws, double growth_tol, GLOBAL const double* rowfloor,
                                         int lane, double* min_ratio, int row0, int row1) {
-  GLOBAL const v4i* rs_rec = globv<v4i>(P.rs_rec) + (size_t)row0 * kWave;
+  GLOBAL const v2i* rs_rec = globv<v2i>(P.rs_rec) + (size_t)row0 * kWave;
   GLOBAL const int* piv_pos = glob(P.piv_pos);
   GLOBAL const int* prow = glob(P.prow);
   const int tl = lane % kTrackLanes;
@@ -294,7 +294,7 @@ __device__ __forceinline__ int refactor(const DevPlan& P, GLOBAL T* vals, GLOBAL
   double mratio = *min_ratio;
   // The row's kind is read from the lane's own record (every lane holds the same), not from a per-row array: a value
   // the compiler knows to be uniform would be moved to a scalar register as soon as it is loaded, which waits for it.
-  v4i rr[kRing];
+  v2i rr[kRing];
   T rv[kRing];
   // The first loads in the order the loop's last kAhead rows issue them (records of rows 0..kRing-kAhead-1, then per
   // row the record of row kRing-kAhead+h and the value of row h), each pinned in place: the wait counts the compiler
@@ -305,19 +305,21 @@ __device__ __forceinline__ int refactor(const DevPlan& P, GLOBAL T* vals, GLOBAL
     rr[s] = rs_rec[(size_t)s * kWave + lane];
     if (s >= kRing - kAhead) {
       const int h = s - (kRing - kAhead);
-      rv[h] = vals[rr[h].y & kRsIndexMask];
+      const int hk = (int)((unsigned)rr[h].y >> kRsMetaShift) & kRsKindMask;
+      rv[h] = vals[hk == kRsAssign || hk == kRsUpdate ? (rr[h].y & kRsIndexMask) : 0];
     }
     __builtin_amdgcn_sched_barrier(0);
   }
   T piv = T(1);
   int failed = -1;
+  int write_row = 0;   // WRITE rows since the chunk's PIVOT row (a WRITE row's slots follow from it)
   // (a failed pivot check ends the loop at the next ring boundary, not at once: an exit from inside the unrolled body
   // would make the compiler drain every load in flight at each row; the rows run meanwhile only write factors that
   // the caller discards)
   for (int base = 0; base < nrows && failed < 0; base += kRing) {
 #pragma unroll
     for (int s = 0; s < kRing; s++) {
-      const v4i r4 = rr[s];
+      const v2i r4 = rr[s];
       // Every lane holds the row's kind and flags: taken as a scalar here, at the row itself (the record arrived
       // kRing rows ago), so the dispatch below is a scalar branch and the row's lanes are never masked.
       int meta = (int)((unsigned)__builtin_amdgcn_readfirstlane(r4.y) >> kRsMetaShift);
@@ -338,9 +340,11 @@ __device__ __forceinline__ int refactor(const DevPlan& P, GLOBAL T* vals, GLOBAL
       } else if (kind == kRsPivot) {
         // pivot and growth check, one track per 8 lanes (all 8 lanes of a track are active or idle together)
         int fail = -1;
-        if (r4.w >= 0) {
-          const int k = r4.w, lo = r4.z & 0xFFFF, hi = r4.z >> 16;
-          piv = ws[r4.x];
+        write_row = 0;
+        if (r4.x != -1) {
+          const int k = r4.y & kRsIndexMask, ps = r4.x & 0xFFFF;
+          const int lo = ps + 1, hi = lo + (int)((unsigned)r4.x >> 16);
+          piv = ws[ps];
           const double pabs = fabs((double)piv);
           double cand = pabs;
           int moved = 0;
@@ -366,24 +370,28 @@ __device__ __forceinline__ int refactor(const DevPlan& P, GLOBAL T* vals, GLOBAL
         if (fmask && failed < 0) failed = __shfl(fail, __ffsll((long long)fmask) - 1, kWave);
         lds_drained();
       } else if (kind == kRsWrite) {
-        // every lane reads its slot (an idle lane slot 0) and forms the L quotient; the stores are masked by kind
-        const int wk = (int)((unsigned)r4.x >> 16);
-        const T v = ws[r4.x & 0xFFFF];
+        // every lane reads its slot (an idle lane a slot inside the workspace) and forms the L quotient; the stores
+        // are masked by kind
+        const int wk = (int)((unsigned)r4.x >> 24), pos = r4.y & kRsIndexMask;
+        const T v = ws[(lane / kTrackLanes) * P.wstride + write_row * kTrackLanes + tl];
         const T lv = v / piv;
-        if (wk == kWriteU) bval[r4.z] = v;
-        if (wk == kWritePivot) dval[r4.z] = v;
+        if (wk == kWriteU) bval[pos] = v;
+        if (wk == kWritePivot) dval[pos] = v;
         if (wk == kWriteL) {
-          vals[r4.z] = lv;
-          fval[r4.w] = lv;
+          vals[pos] = lv;
+          fval[r4.x & kRsIndexMask] = lv;
         }
+        write_row++;
         lds_drained();
       }
       // the level's L columns are written: publish them before any later row loads them
       if (meta & kRsLevelEnd) WG_SYNC();
       // refill this slot with row base + s + kRing; load the value of row base + s + kHalf
       rr[s] = rs_rec[(size_t)(base + s + kRing) * kWave + lane];
+      // the value of row base + s + kAhead (the rows without one load an entry that is always there)
       const int hs = (s + kAhead) % kRing;
-      rv[hs] = vals[rr[hs].y & kRsIndexMask];
+      const int hk = (int)((unsigned)rr[hs].y >> kRsMetaShift) & kRsKindMask;
+      rv[hs] = vals[hk == kRsAssign || hk == kRsUpdate ? (rr[hs].y & kRsIndexMask) : 0];
     }
   }
   WG_SYNC();
@@ -406,13 +414,32 @@ __device__ __forceinline__ int refactor(const DevPlan& P, GLOBAL T* vals, GLOBAL
 // itself. XP: the solve vector (LDS or global); scr: the scratch area (LDS, the wave's refactor workspace).
 // Steps [step0, step1) of the stream (the whole stream, or one pass of a level-parallel run): whole rings from step0,
 // the steps from step1 on doing nothing.
+// Maximum of a non-negative int over the wave, with DPP moves (no LDS traffic): shifts within rows of 16 lanes, then
+// the row broadcasts carry each row's maximum up; lane 63 ends with the maximum. Lanes a move has no source for read 0.
+__device__ __forceinline__ int wave_max_nonneg(int v) {
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x111, 0xF, 0xF, true));   // row_shr:1
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x112, 0xF, 0xF, true));   // row_shr:2
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x114, 0xF, 0xF, true));   // row_shr:4
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x118, 0xF, 0xF, true));   // row_shr:8
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x142, 0xA, 0xF, true));   // row_bcast:15
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x143, 0xC, 0xF, true));   // row_bcast:31
+  return __builtin_amdgcn_readlane(v, 63);
+}
+
 template <typename T, bool BACK, bool XLDS, typename XP>
-__device__ __forceinline__ void solve_stream(GLOBAL const v4i* rec, int step0, int step1, GLOBAL const T* vals,
+__device__ __forceinline__ void solve_stream(GLOBAL const v2i* rec, int step0, int step1, GLOBAL const T* vals,
                                              GLOBAL const T* dval, XP x, LDSP T* scr, int lane) {
   rec += (size_t)step0 * kWave;
   const int steps = step1 - step0;
-  v4i rr[kRing];
+  v2i rr[kRing];
   T rv[kRing], rd[kRing];
+  // the kind of a step (every lane holds it), and the value index / pivot row a lane's record names for it
+  auto kind_of = [](const v2i& r) { return (int)((unsigned)r.y >> kRsMetaShift) & kStepKindMask; };
+  auto value_index = [&](const v2i& r) { return kind_of(r) == kStepProduct ? (r.y & kRsIndexMask) : 0; };
+  auto pivot_index = [&](const v2i& r) {
+    const int row = r.y & kRsIndexMask;
+    return kind_of(r) == kStepChain && row != kRsIndexMask ? row : 0;
+  };
   // first loads in the loop's own order (see refactor): records of steps 0..kHalf-1, then per step the record of step
   // kHalf+h and the values of step h; the loop's first kHalf steps load the values of steps kHalf..kRing-1
 #pragma unroll
@@ -420,8 +447,8 @@ __device__ __forceinline__ void solve_stream(GLOBAL const v4i* rec, int step0, i
     rr[s] = rec[(size_t)s * kWave + lane];
     if (s >= kRing - kHalf) {
       const int h = s - (kRing - kHalf);
-      rv[h] = vals[rr[h].y];
-      rd[h] = BACK ? dval[rr[h].z < 0 ? 0 : rr[h].z] : T(1);
+      rv[h] = vals[value_index(rr[h])];
+      rd[h] = BACK ? dval[pivot_index(rr[h])] : T(1);
     }
     __builtin_amdgcn_sched_barrier(0);
   }
@@ -431,35 +458,38 @@ __device__ __forceinline__ void solve_stream(GLOBAL const v4i* rec, int step0, i
 #pragma unroll
     for (int s = 0; s < kRing; s++) {
       const int t = base + s;
-      const v4i r4 = rr[s];
-      // the step's kind (and a CHAIN's longest row) is the same in every lane: a scalar, taken at the step itself
-      int meta = __builtin_amdgcn_readfirstlane(r4.w);
-      if (t >= steps) meta = 0;
-      const int kind = meta & kStepKindMask;
+      const v2i r2 = rr[s];
+      // the step's kind is the same in every lane: a scalar, taken at the step itself
+      int kind = (int)((unsigned)__builtin_amdgcn_readfirstlane(r2.y) >> kRsMetaShift) & kStepKindMask;
+      if (t >= steps) kind = 0;
       if (kind == kStepProduct) {
-        const int c = r4.x;
+        const int c = r2.x;
         const T xv = x[c >= 0 ? c : 0];
         scr[sbase + lane] = rv[s] * xv;   // (an idle lane's slot is never read)
         sbase += kWave;
         WAVE_SYNC();
       } else if (kind == kStepChain) {
-        const int row = r4.z, st = r4.x & 0xFFFF, len = (int)((unsigned)r4.x >> 16);
-        const int maxlen = (int)((unsigned)meta >> kChainLenShift);
-        const T x0 = x[row >= 0 ? row : 0];
-        if (r4.w & kChainBegin) acc = x0;
+        const int field = r2.y & kRsIndexMask, fl = (int)((unsigned)r2.y >> kRsMetaShift);
+        const bool active = field != kRsIndexMask;
+        const int row = active ? field : 0, st = r2.x & 0xFFFF, len = (int)((unsigned)r2.x >> 16);
+        const int maxlen = wave_max_nonneg(len);
+        const T x0 = x[row];
+        if (fl & kChainBegin) acc = x0;
 #pragma unroll 4
         for (int e = 0; e < maxlen; e++) {
           const T pv = scr[st + (e < len ? e : 0)];
           acc = e < len ? acc - pv : acc;
         }
-        if ((r4.w & kChainEnd) && row >= 0) x[row] = BACK ? acc / rd[s] : acc;
+        if ((fl & kChainEnd) && active) x[row] = BACK ? acc / rd[s] : acc;
         if (XLDS) WAVE_SYNC(); else WG_SYNC();
         sbase = 0;
       }
       rr[s] = rec[(size_t)(t + kRing) * kWave + lane];
+      // the factor values of step t + kHalf: a PRODUCT step's entries, a back CHAIN step's pivots (the other steps
+      // load an entry that is always there)
       const int hs = (s + kHalf) % kRing;
-      rv[hs] = vals[rr[hs].y];
-      if (BACK) rd[hs] = dval[rr[hs].z < 0 ? 0 : rr[hs].z];
+      rv[hs] = vals[value_index(rr[hs])];
+      if (BACK) rd[hs] = dval[pivot_index(rr[hs])];
     }
   }
 }
@@ -473,8 +503,8 @@ __device__ __forceinline__ void lu_solve(const DevPlan& P, GLOBAL const T* fval,
   GLOBAL const int* q = glob(P.q);
   for (int r = lane; r < m; r += kWave) x[pinv[r]] = (T)rhs[r];
   if (XLDS) WAVE_SYNC(); else WG_SYNC();
-  solve_stream<T, false, XLDS>(globv<v4i>(P.fs_rec), 0, P.fs_steps, fval, dval, x, scr, lane);
-  solve_stream<T, true, XLDS>(globv<v4i>(P.bs_rec), 0, P.bs_steps, bval, dval, x, scr, lane);
+  solve_stream<T, false, XLDS>(globv<v2i>(P.fs_rec), 0, P.fs_steps, fval, dval, x, scr, lane);
+  solve_stream<T, true, XLDS>(globv<v2i>(P.bs_rec), 0, P.bs_steps, bval, dval, x, scr, lane);
   for (int k = lane; k < m; k += kWave) out[q[k]] = (double)x[k];
   WG_SYNC();
 }
@@ -847,7 +877,7 @@ __global__ __launch_bounds__(kWave) void wide_solve(const InstDesc* __restrict__
   const int* pass_step = BACK ? P.bs_pass_step : P.fs_pass_step;
   GLOBAL T* x = glob(reinterpret_cast<T*>(d.xg));
   extern __shared__ __align__(16) unsigned char smem[];
-  solve_stream<T, BACK, false>(globv<v4i>(BACK ? P.bs_rec : P.fs_rec), pass_step[p], pass_step[p + per],
+  solve_stream<T, BACK, false>(globv<v2i>(BACK ? P.bs_rec : P.fs_rec), pass_step[p], pass_step[p + per],
                                glob(reinterpret_cast<const T*>(BACK ? d.bval : d.fval)),
                                glob(reinterpret_cast<const T*>(d.dval)), x, lds(reinterpret_cast<T*>(smem)),
                                opaque_lane(threadIdx.x));
