@@ -519,6 +519,7 @@ def main():
     kavg = float(np.mean(kernel_ms))
     achieved = alg / (kavg / 1000) / 1e9
     stats = dn.plan_stats()
+    shape = dn.launch_shape(opts)
     traffic, traffic_src = measured_traffic(stats, "fp32" if opts.lu_precision == engine.LU_FP32 else "fp64",
                                             opts.flags, alg)
 
@@ -589,8 +590,11 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "traffic_source": traffic_src,
                      "plan_stream_bytes": stats["stream_bytes"],
-                     "kernel": "nr_batch_kernel<%s,true>" % ("float" if opts.lu_precision == engine.LU_FP32
-                                                             else "double"), "kernel_ms": round(kavg, 3),
+                     "kernel": "nr_batch_kernel<%s,%s,%s>" % ("float" if shape["lu_fp32"] else "double",
+                                                               str(shape["x_in_lds"]).lower(),
+                                                               str(shape["waves_per_group"] == 2).lower()),
+                     "kernel_ms": round(kavg, 3),
+                     "launch": shape,
                      "algorithmic_bytes": int(alg),
                      "phase_us_per_instance": {n: round(float(v), 1) for n, v in zip(engine.PHASES, phase)}},
         "cpu_baseline": cpu,

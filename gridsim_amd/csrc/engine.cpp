@@ -1485,6 +1485,20 @@ GS_API gs_status gs_plan_selfcheck(const gs_plan* p, double* rel_residual, char*
   return GS_OK;
 }
 
+GS_API gs_status gs_network_launch_shape(const gs_net* net, const gs_nr_options* opts, int32_t* lu_fp32,
+                                         int32_t* x_in_lds, int32_t* waves_per_group, int64_t* lds_bytes) {
+  if (!net) return GS_INVALID_ARGUMENT;
+  return guarded(net->ctx, [&]() -> gs_status {
+    const gs_nr_options o = normalise(opts);
+    const BatchRun r = plan_run(net->dev.h, o, false);
+    if (lu_fp32) *lu_fp32 = r.fp32;
+    if (x_in_lds) *x_in_lds = r.x_lds;
+    if (waves_per_group) *waves_per_group = r.pair ? 2 : 1;
+    if (lds_bytes) *lds_bytes = (int64_t)r.lds_bytes;
+    return GS_OK;
+  });
+}
+
 GS_API gs_status gs_last_timing(const gs_net* net, double* total_ms, double* kernel_ms, int32_t* launches) {
   if (!net) return GS_INVALID_ARGUMENT;
   if (total_ms) *total_ms = net->last_total_ms;

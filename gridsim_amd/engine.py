@@ -134,7 +134,7 @@ def lib():
                      "gs_plan_sizes", "gs_plan_get", "gs_batch_create", "gs_batch_run", "gs_batch_results",
                      "gs_nr_autotune", "gs_network_plan_stats", "gs_plan_shape", "gs_server_create",
                      "gs_server_add_network", "gs_server_submit", "gs_server_wait", "gs_server_stats_get",
-                     "gs_server_stats_reset", "gs_nr_screen_batch"):
+                     "gs_server_stats_reset", "gs_nr_screen_batch", "gs_network_launch_shape"):
             getattr(L, name).restype = C.c_int32
         L.gs_host_sincos.restype = None
         L.gs_server_free.restype = None
@@ -377,6 +377,15 @@ class DeviceNetwork:
         lib().gs_network_plan_get(self.h, *(_p(out[k]) for k in ("q", "pivot_row", "lp", "li", "up", "ui",
                                                                  "j_rowptr", "j_colidx")))
         return out
+
+    def launch_shape(self, options: NrOptions | None = None) -> dict:
+        """How the batched kernel would be launched for these options (gs_network_launch_shape)."""
+        fp32, xl, waves, lds = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int64()
+        o = (options or self.options).c()
+        self.ctx.check(lib().gs_network_launch_shape(self.h, C.byref(o), C.byref(fp32), C.byref(xl), C.byref(waves),
+                                                     C.byref(lds)), "gs_network_launch_shape")
+        return {"lu_fp32": bool(fp32.value), "x_in_lds": bool(xl.value), "waves_per_group": waves.value,
+                "lds_bytes": lds.value}
 
     def plan_stats(self) -> dict:
         """Shape of the compiled device plan (record streams, levels, shared bytes)."""

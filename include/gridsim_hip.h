@@ -321,6 +321,12 @@ GS_API void gs_server_free(gs_server* srv);
 /* Kernel timing of the last gs_nr_solve_batch on this network (milliseconds, HIP events on its stream). */
 GS_API gs_status gs_last_timing(const gs_net* net, double* total_ms, double* kernel_ms, int32_t* launches);
 
+/* How gs_nr_solve_batch would launch the batched kernel for these options (no counterpart in the reference): LU
+ * arithmetic, whether the triangular-solve vector sits in LDS, wavefronts (instances) per work-group and the LDS
+ * bytes per work-group. Any pointer may be null. */
+GS_API gs_status gs_network_launch_shape(const gs_net* net, const gs_nr_options* opts, int32_t* lu_fp32,
+                                         int32_t* x_in_lds, int32_t* waves_per_group, int64_t* lds_bytes);
+
 /* Device FDLIBM sincos over n host values (for the bit-parity test against the StrictMath vectors). */
 GS_API gs_status gs_device_sincos(gs_ctx* ctx, int64_t n, const double* x, double* s, double* c);
 
