@@ -68,13 +68,17 @@ void upload_plan(DevicePlan& dp, const YBus* y, const JacobianPattern& jp, const
   h.wstride = d.wstride;
   h.n_a = d.n_a;
   h.n_rs = ring_up(d.n_rs);
-  // (two rings of padding: a level-parallel run of one chunk rounds its row range up to whole rings and loads one
-  // ring ahead of that)
-  std::vector<int> rs_rec(d.rs_rec);
-  for (int r = d.n_rs; r < h.n_rs + 2 * kStreamPad; r++)
-    for (int l = 0; l < 64; l++) { rs_rec.push_back(0); rs_rec.push_back(0); }
+  // (three rings of padding: a level-parallel run of one chunk starts at the even row at or before the chunk's
+  // first, rounds its row range up to whole rings and loads one ring ahead of that). On the device the rows go in
+  // pairs: pair q holds rows 2q and 2q + 1 of each lane side by side, so one 16-byte load brings a lane two rows.
+  const int rs_rows = h.n_rs + kRsPadRings * kStreamPad;
+  std::vector<int> rs_rec((size_t)rs_rows * 64 * 2, 0);
+  for (int r = 0; r < d.n_rs; r++)
+    for (int l = 0; l < 64; l++)
+      for (int c = 0; c < 2; c++)
+        rs_rec[(((size_t)(r / 2) * 64 + l) * 2 + (r % 2)) * 2 + c] = d.rs_rec[((size_t)r * 64 + l) * 2 + c];
   upload(dp.rs_rec, rs_rec, s); upload(dp.ch_a_src, d.ch_a_src, s);
-  h.rs_rec = dp.rs_rec.as<int2>(); h.ch_a_src = dp.ch_a_src.as<int>();
+  h.rs_rec = dp.rs_rec.as<int4>(); h.ch_a_src = dp.ch_a_src.as<int>();
   // the solves' step streams (padded with steps that touch nothing) and the factors' stream-order sizes
   h.fs_steps = ring_up(d.fs_steps);
   h.bs_steps = ring_up(d.bs_steps);
@@ -1468,7 +1472,7 @@ GS_API gs_status gs_plan_selfcheck(const gs_plan* p, double* rel_residual, char*
   // the level-parallel decomposition of the same streams, against the padding upload_plan gives them
   auto ring_up = [](int v) { return (v + kStreamPad - 1) / kStreamPad * kStreamPad; };
   const DevicePlanHost& d = p->dph;
-  if (!check_level_schedule(d, kStreamPad, ring_up(d.n_rs) + 2 * kStreamPad, ring_up(d.fs_steps) + 2 * kStreamPad,
+  if (!check_level_schedule(d, kStreamPad, ring_up(d.n_rs) + kRsPadRings * kStreamPad, ring_up(d.fs_steps) + 2 * kStreamPad,
                             ring_up(d.bs_steps) + 2 * kStreamPad, w)) {
     if (why) std::snprintf(why, 128, "level-parallel schedule: %s", w.c_str());
     return GS_INVALID_ARGUMENT;

@@ -932,7 +932,8 @@ bool check_level_schedule(const DevicePlanHost& d, int ring, int padded_rows, in
     for (int c = d.rs_level_chunk[l]; c < d.rs_level_chunk[l + 1]; c++) {
       const int r0 = d.rs_chunk_row[c], r1 = d.rs_chunk_row[c + 1];
       if (r1 <= r0) return bad("an empty chunk");
-      const int last_load = r0 + (r1 - r0 + ring - 1) / ring * ring - 1 + ring;
+      const int r0e = r0 & ~1;   // the device starts a chunk at the even row of its first row's pair
+      const int last_load = r0e + (r1 - r0e + ring - 1) / ring * ring - 1 + ring;
       if (last_load >= padded_rows) return bad("a chunk's look-ahead passes the stream's padding");
       if ((d.rs_meta[r0] & D::kRsKindMask) != D::kRsAssign || !(d.rs_meta[r0] & D::kRsClear))
         return bad("a chunk does not start by clearing its workspace");

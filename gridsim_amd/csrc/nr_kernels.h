@@ -28,7 +28,9 @@ struct DevPlan {
   // UPDATE, PIVOT, WRITE; model.h DevicePlanHost), padded with no-op rows to a multiple of kStreamPad rows plus two
   // more kStreamPad, so the kernel loads ahead without bounds checks. n_a: A entries (the first part of vals).
   int32_t wstride, n_rs, n_a, pad_rs;
-  const int2* rs_rec;                 // [n_rs + 2 kStreamPad][64] (x, y); y = 24-bit field | (kind | flags) << 24
+  // [(n_rs + kRsPadRings kStreamPad) / 2][64] pairs of rows: lane l of pair q holds (x, y) of rows 2q and 2q + 1;
+  // y = 24-bit field | (kind | flags) << 24
+  const int4* rs_rec;
   const int32_t* ch_a_src;            // per A entry: J CSR position (the seam fills vals from its matrix)
   // Triangular-solve step streams: per lane and step (x, y): PRODUCT (col, value index | kind << 24) and CHAIN
   // (start | len << 16, row | (kind | flags) << 24) steps (model.h DevicePlanHost). The factors live in the order the
@@ -60,6 +62,7 @@ constexpr int kChainBegin = 16, kChainEnd = 32;
 // load-ahead distance of the row and step streams, and the padding each stream carries; rows of the refactor load
 // their value kRsValueAhead rows ahead (the plan places its rows for that distance)
 constexpr int kStreamPad = 16;
+constexpr int kRsPadRings = 3;   // rings of no-op rows after the refactor stream (rounded to whole rings)
 constexpr int kRsValueAhead = 8;
 
 constexpr int kTracks = 8, kTrackLanes = 8;

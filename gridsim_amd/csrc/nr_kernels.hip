@@ -285,28 +285,38 @@ template <typename T, bool EXACT>
 __device__ __forceinline__ int refactor(const DevPlan& P, GLOBAL T* vals, GLOBAL T* fval, GLOBAL T* bval,
                                         GLOBAL T* dval, LDSP T* ws, double growth_tol, GLOBAL const double* rowfloor,
                                         int lane, double* min_ratio, int row0, int row1) {
-  GLOBAL const v2i* rs_rec = globv<v2i>(P.rs_rec) + (size_t)row0 * kWave;
+  // rows go in pairs (one 16-byte record per lane for two rows): start at the even row of row0's pair, the rows
+  // before row0 doing nothing
+  const int r0e = row0 & ~1;
+  GLOBAL const v4i* rs_pair = globv<v4i>(P.rs_rec) + (size_t)(r0e / 2) * kWave;
   GLOBAL const int* piv_pos = glob(P.piv_pos);
   GLOBAL const int* prow = glob(P.prow);
   const int tl = lane % kTrackLanes;
   const int wsz = kTracks * P.wstride;
-  const int nrows = row1 - row0;
+  const int first = row0 - r0e, nrows = row1 - r0e;
   double mratio = *min_ratio;
   // The row's kind is read from the lane's own record (every lane holds the same), not from a per-row array: a value
   // the compiler knows to be uniform would be moved to a scalar register as soon as it is loaded, which waits for it.
-  v2i rr[kRing];
+  v4i rp[kRing / 2];
   T rv[kRing];
-  // The first loads in the order the loop's last kAhead rows issue them (records of rows 0..kRing-kAhead-1, then per
-  // row the record of row kRing-kAhead+h and the value of row h), each pinned in place: the wait counts the compiler
-  // derives at the loop head merge this path with the loop's back edge, so any other order here would make every pass
-  // of the loop wait for nearly all loads in flight.
+  auto rec_of = [&](int s) -> v2i {
+    const v4i q = rp[s / 2];
+    return (s % 2) ? (v2i){q.z, q.w} : (v2i){q.x, q.y};
+  };
+  auto value_index = [](const v2i& r) {
+    const int k = (int)((unsigned)r.y >> kRsMetaShift) & kRsKindMask;
+    return k == kRsAssign || k == kRsUpdate ? (r.y & kRsIndexMask) : 0;
+  };
+  // The first loads in the order the loop's rows issue them (each odd row reloads its pair; from row kRing - kAhead on
+  // each row also loads the value of the row kAhead back), each pinned in place: the wait counts the compiler derives
+  // at the loop head merge this path with the loop's back edge, so any other order here would make every pass of the
+  // loop wait for nearly all loads in flight.
 #pragma unroll
   for (int s = 0; s < kRing; s++) {
-    rr[s] = rs_rec[(size_t)s * kWave + lane];
+    if (s % 2 == 1) rp[s / 2] = rs_pair[(size_t)(s / 2) * kWave + lane];
     if (s >= kRing - kAhead) {
       const int h = s - (kRing - kAhead);
-      const int hk = (int)((unsigned)rr[h].y >> kRsMetaShift) & kRsKindMask;
-      rv[h] = vals[hk == kRsAssign || hk == kRsUpdate ? (rr[h].y & kRsIndexMask) : 0];
+      rv[h] = vals[value_index(rec_of(h))];
     }
     __builtin_amdgcn_sched_barrier(0);
   }
@@ -319,11 +329,11 @@ __device__ __forceinline__ int refactor(const DevPlan& P, GLOBAL T* vals, GLOBAL
   for (int base = 0; base < nrows && failed < 0; base += kRing) {
 #pragma unroll
     for (int s = 0; s < kRing; s++) {
-      const v2i r4 = rr[s];
+      const v2i r4 = rec_of(s);
       // Every lane holds the row's kind and flags: taken as a scalar here, at the row itself (the record arrived
       // kRing rows ago), so the dispatch below is a scalar branch and the row's lanes are never masked.
       int meta = (int)((unsigned)__builtin_amdgcn_readfirstlane(r4.y) >> kRsMetaShift);
-      if (base + s >= nrows) meta = kRsNop;
+      if (base + s >= nrows || base + s < first) meta = kRsNop;
       const int kind = meta & kRsKindMask;
       if (kind == kRsUpdate) {
         // a lane without an update works on its own spare slot
@@ -387,11 +397,11 @@ __device__ __forceinline__ int refactor(const DevPlan& P, GLOBAL T* vals, GLOBAL
       // the level's L columns are written: publish them before any later row loads them
       if (meta & kRsLevelEnd) WG_SYNC();
       // refill this slot with row base + s + kRing; load the value of row base + s + kHalf
-      rr[s] = rs_rec[(size_t)(base + s + kRing) * kWave + lane];
+      // after the second row of a pair: the pair kRing rows on
+      if (s % 2 == 1) rp[s / 2] = rs_pair[(size_t)((base + s) / 2 + kRing / 2) * kWave + lane];
       // the value of row base + s + kAhead (the rows without one load an entry that is always there)
       const int hs = (s + kAhead) % kRing;
-      const int hk = (int)((unsigned)rr[hs].y >> kRsMetaShift) & kRsKindMask;
-      rv[hs] = vals[hk == kRsAssign || hk == kRsUpdate ? (rr[hs].y & kRsIndexMask) : 0];
+      rv[hs] = vals[value_index(rec_of(hs))];
     }
   }
   WG_SYNC();
