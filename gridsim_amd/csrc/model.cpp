@@ -546,7 +546,7 @@ void compile_plan(const Plan& plan, DevicePlanHost& d) {
   for (int k = 0; k < m; k++) d.col_order[count[level[k]]++] = k;
   d.n_levels = n_levels;
   // Level chunks: within each level, columns by decreasing chain length, kTracks at a time.
-  const int G = DevicePlanHost::kTracks, TL = DevicePlanHost::kTrackLanes;
+  const int G = DevicePlanHost::kTracks;
   // An odd stride puts the same slot of the 8 tracks in 8 different LDS banks (the tracks' updates tend to touch the
   // same slot positions at once).
   d.wstride = d.max_col | 1;
@@ -555,14 +555,20 @@ void compile_plan(const Plan& plan, DevicePlanHost& d) {
   d.ch_track_ptr.assign(1, 0);
   d.ch_row_ptr.assign(1, 0);
   d.ch_a_ptr.assign(1, 0);
+  d.ch_tl.clear();
   for (int lv = 0; lv < n_levels; lv++) {
     int b0 = lv == 0 ? 0 : d.level_end[lv - 1], b1 = d.level_end[lv];
     vector<int> cols(d.col_order.begin() + b0, d.col_order.begin() + b1);
     std::stable_sort(cols.begin(), cols.end(), [&](int a, int b) {
       return (plan.up[a + 1] - plan.up[a]) > (plan.up[b + 1] - plan.up[b]);
     });
-    for (size_t c0 = 0; c0 < cols.size(); c0 += G) {
-      size_t c1 = std::min(cols.size(), c0 + G);
+    // a level of fewer than kTracks columns gives each column more lanes: 64 for one, 32 for two, 16 for four
+    int gl = G;
+    while (gl > 1 && gl / 2 >= (int)cols.size()) gl /= 2;
+    const int TL = 64 / gl;
+    for (size_t c0 = 0; c0 < cols.size(); c0 += gl) {
+      size_t c1 = std::min(cols.size(), c0 + gl);
+      d.ch_tl.push_back(TL);
       // Rows of each track: its column's updates list-scheduled, up to TL per row. An update x[dst] -= L * x[src]
       // (src: the slot of U entry t, dst: an L row of that entry's column) goes in the first row with a free lane
       // after every update of its source slot (x[src] is then final) and after the previous update of its
@@ -811,6 +817,8 @@ void compile_plan(const Plan& plan, DevicePlanHost& d) {
     for (int c = 0; c < d.n_chunks; c++) {
       const int t0 = d.ch_track_ptr[c], ntr = d.ch_track_ptr[c + 1] - t0;
       const int lv = level[d.tr_col[t0]];
+      const int TL = d.ch_tl[c];
+      const int width = D::width_code(TL);   // folded into every row of the chunk
       if (lv + 1 < (int)d.rs_level_chunk.size()) throw std::runtime_error("plan: chunks out of level order");
       while ((int)d.rs_level_chunk.size() <= lv) d.rs_level_chunk.push_back(c);
       d.rs_chunk_row.push_back((int)d.rs_meta.size());
@@ -823,7 +831,7 @@ void compile_plan(const Plan& plan, DevicePlanHost& d) {
           rec[2 * l] = in ? d.ch_a_ws[e] : d.spare(l);
           rec[2 * l + 1] = in ? e : 0;
         }
-        emit(D::kRsAssign | (e0 == a0 ? D::kRsClear : 0), lv, rec.data());
+        emit(D::kRsAssign | (e0 == a0 ? D::kRsClear : 0) | width, lv, rec.data());
       }
       // UPDATE rows: (dst | src << 16, L value index)
       for (int r = d.ch_row_ptr[c]; r < d.ch_row_ptr[c + 1]; r++) {
@@ -844,7 +852,7 @@ void compile_plan(const Plan& plan, DevicePlanHost& d) {
           rec[2 * l] = in ? rr : d.spare(l) | (d.spare(l) << 16);
           rec[2 * l + 1] = in ? d.n_a + d.row_lsrc[(size_t)r * 64 + l] : 0;
         }
-        emit(D::kRsUpdate, lv, rec.data());
+        emit(D::kRsUpdate | width, lv, rec.data());
       }
       // PIVOT row: (pivot slot | candidates << 16, pivot step), -1 for an idle track
       int wrows = 0;
@@ -860,9 +868,9 @@ void compile_plan(const Plan& plan, DevicePlanHost& d) {
           wrows = std::max(wrows, (nu + nl - 1 + TL - 1) / TL);
         }
       }
-      emit(D::kRsPivot, lv, rec.data());
+      emit(D::kRsPivot | width, lv, rec.data());
       // WRITE rows: (fval position | kind << 24, bval / dval / vals position); write row w of a chunk writes slot
-      // t = w * kTrackLanes + lane % kTrackLanes of the lane's track
+      // t = w * TL + lane % TL of the lane's track (TL lanes per track)
       for (int w = 0; w < wrows; w++) {
         for (int l = 0; l < 64; l++) {
           const int g = l / TL, t = w * TL + l % TL;
@@ -884,7 +892,7 @@ void compile_plan(const Plan& plan, DevicePlanHost& d) {
             rec[2 * l + 1] = d.n_a + p;
           }
         }
-        emit(D::kRsWrite, lv, rec.data());
+        emit(D::kRsWrite | width, lv, rec.data());
       }
       if (d.ch_fence[c]) {
         d.rs_meta.back() |= D::kRsLevelEnd;
@@ -1013,7 +1021,7 @@ bool check_level_schedule(const DevicePlanHost& d, int ring, int padded_rows, in
 
 bool emulate_device_lu(const DevicePlanHost& d, int nnz_j, const double* jv, const double* rhs, double* out,
                        std::string& why) {
-  const int m = d.m, G = DevicePlanHost::kTracks, TL = DevicePlanHost::kTrackLanes;
+  const int m = d.m, G = DevicePlanHost::kTracks;
   const int wsz = G * d.wstride;
   vector<double> ws(wsz), fval(std::max(1, d.n_fval), std::nan("")), bval(std::max(1, d.n_bval), std::nan(""));
   vector<double> dval(m, std::nan(""));
@@ -1072,6 +1080,7 @@ bool emulate_device_lu(const DevicePlanHost& d, int nnz_j, const double* jv, con
       }
     }
     bool fail = false;
+    const int TL = D::track_lanes(meta);   // lanes per track of the row's chunk
     if (kind == D::kRsPivot) write_row = 0;
     for (int lane = 0; lane < 64; lane++) {
       const int* rr = &d.rs_rec[((size_t)r * 64 + lane) * 2];

@@ -121,6 +121,12 @@ struct DevicePlanHost {
   // serves track l / kTrackLanes), so every lane knows its record's address without a search.
   static constexpr int kTracks = 8, kTrackLanes = 8;
   int wstride = 0, n_chunks = 0, n_rows = 0;
+  // Lanes per track of each chunk: kTrackLanes, or more for a level of fewer than kTracks columns (16, 32, 64; the
+  // chunk's tracks then number 64 / TL). Every row of a chunk carries its width code in its flags.
+  vector<int> ch_tl;
+  static constexpr int kRsWidthShift = 5;
+  static int width_code(int tl) { return (tl == 8 ? 0 : tl == 16 ? 1 : tl == 32 ? 2 : 3) << kRsWidthShift; }
+  static int track_lanes(int meta) { return 8 << ((meta >> kRsWidthShift) & 3); }
   static constexpr int kSpareSlots = 64;   // one spare workspace slot per lane, after the kTracks workspaces
   int spare(int lane) const { return kTracks * wstride + lane; }
   vector<int> ch_track_ptr, ch_row_ptr, ch_a_ptr, ch_fence;  // per chunk; ch_fence: last chunk of its level

@@ -54,6 +54,7 @@ struct DevPlan {
 // refactor row kinds and flags, write kinds (model.h DevicePlanHost)
 constexpr int kRsNop = 0, kRsAssign = 1, kRsUpdate = 2, kRsPivot = 3, kRsWrite = 4;
 constexpr int kRsKindMask = 7, kRsClear = 8, kRsLevelEnd = 16;
+constexpr int kRsWidthShift = 5;   // a chunk's lanes per track: kTrackLanes << ((meta >> kRsWidthShift) & 3)
 constexpr int kRsMetaShift = 24, kRsIndexMask = (1 << kRsMetaShift) - 1;
 constexpr int kWriteU = 1, kWritePivot = 2, kWriteL = 3;
 // solve step kinds and CHAIN flags (model.h DevicePlanHost)

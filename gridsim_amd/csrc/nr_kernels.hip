@@ -57,14 +57,14 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 
-// maximum over the kTrackLanes lanes of one track (aligned groups of 8)
-__device__ __forceinline__ double track_max(double v) {
-  for (int off = kTrackLanes / 2; off > 0; off >>= 1) v = nan_max(v, __shfl_xor(v, off, kWave));
+// maximum over the tl lanes of one track (aligned groups of tl, a power of two the same in every lane)
+__device__ __forceinline__ double track_max(double v, int tl) {
+  for (int off = tl / 2; off > 0; off >>= 1) v = nan_max(v, __shfl_xor(v, off, kWave));
   return v;
 }
 
-__device__ __forceinline__ int track_any(int v) {
-  for (int off = kTrackLanes / 2; off > 0; off >>= 1) v |= __shfl_xor(v, off, kWave);
+__device__ __forceinline__ int track_any(int v, int tl) {
+  for (int off = tl / 2; off > 0; off >>= 1) v |= __shfl_xor(v, off, kWave);
   return v;
 }
 
@@ -291,7 +291,6 @@ __device__ __forceinline__ int refactor(const DevPlan& P, GLOBAL T* vals, GLOBAL
   GLOBAL const v4i* rs_pair = globv<v4i>(P.rs_rec) + (size_t)(r0e / 2) * kWave;
   GLOBAL const int* piv_pos = glob(P.piv_pos);
   GLOBAL const int* prow = glob(P.prow);
-  const int tl = lane % kTrackLanes;
   const int wsz = kTracks * P.wstride;
   const int first = row0 - r0e, nrows = row1 - r0e;
   double mratio = *min_ratio;
@@ -335,6 +334,7 @@ __device__ __forceinline__ int refactor(const DevPlan& P, GLOBAL T* vals, GLOBAL
       int meta = (int)((unsigned)__builtin_amdgcn_readfirstlane(r4.y) >> kRsMetaShift);
       if (base + s >= nrows || base + s < first) meta = kRsNop;
       const int kind = meta & kRsKindMask;
+      const int TLr = kTrackLanes << ((meta >> kRsWidthShift) & 3);   // lanes per track of the row's chunk
       if (kind == kRsUpdate) {
         // a lane without an update works on its own spare slot
         const int dst = r4.x & 0xFFFF, src = (int)((unsigned)r4.x >> 16);
@@ -348,7 +348,7 @@ __device__ __forceinline__ int refactor(const DevPlan& P, GLOBAL T* vals, GLOBAL
         ws[r4.x] = rv[s];   // (a lane without an A entry: its spare slot)
         WAVE_SYNC();
       } else if (kind == kRsPivot) {
-        // pivot and growth check, one track per 8 lanes (all 8 lanes of a track are active or idle together)
+        // pivot and growth check, one track per TLr lanes (all lanes of a track are active or idle together)
         int fail = -1;
         write_row = 0;
         if (r4.x != -1) {
@@ -359,15 +359,15 @@ __device__ __forceinline__ int refactor(const DevPlan& P, GLOBAL T* vals, GLOBAL
           double cand = pabs;
           int moved = 0;
           const int before = EXACT ? piv_pos[k] : 0;
-          for (int c = lo + tl; c < hi; c += kTrackLanes) {
+          for (int c = lo + (lane & (TLr - 1)); c < hi; c += TLr) {
             const double cabs = fabs((double)ws[c]);
             cand = nan_max(cand, cabs);
             if (EXACT) moved |= (c - lo < before) ? !(cabs < pabs) : (cabs > pabs);
           }
-          const double best = track_max(cand);
+          const double best = track_max(cand, TLr);
           bool bad;
           if (EXACT) {
-            bad = track_any(moved) || !(pabs > rowfloor[prow[k]]);
+            bad = track_any(moved, TLr) || !(pabs > rowfloor[prow[k]]);
           } else {
             const double ratio = pabs / best;
             bad = !(ratio >= growth_tol) || !(pabs > 0.0) || !isfinite(pabs);
@@ -383,7 +383,7 @@ __device__ __forceinline__ int refactor(const DevPlan& P, GLOBAL T* vals, GLOBAL
         // every lane reads its slot (an idle lane a slot inside the workspace) and forms the L quotient; the stores
         // are masked by kind
         const int wk = (int)((unsigned)r4.x >> 24), pos = r4.y & kRsIndexMask;
-        const T v = ws[(lane / kTrackLanes) * P.wstride + write_row * kTrackLanes + tl];
+        const T v = ws[(lane / TLr) * P.wstride + write_row * TLr + (lane & (TLr - 1))];
         const T lv = v / piv;
         if (wk == kWriteU) bval[pos] = v;
         if (wk == kWritePivot) dval[pos] = v;
