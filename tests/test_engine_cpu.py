@@ -116,6 +116,25 @@ def test_device_lu_indexing_emulated_on_host(eng, case):
     assert res.value < 1e-12
 
 
+@pytest.mark.parametrize("case", ["case118", "synthetic-2000"])
+def test_device_plan_covers_every_update_and_solve_entry_once(eng, case):
+    """Closed forms of the compiled streams' work: the refactor's UPDATE lanes are exactly the Gilbert-Peierls updates
+    (for each column k and each off-diagonal U(j, k), one per off-diagonal L(:, j) entry), the forward and back
+    PRODUCT lanes exactly the off-diagonal entries of L and U, and the row kinds add up to the stream."""
+    from gridsim_amd import gridgen
+    net = gridgen.config2_grid() if case == "synthetic-2000" else matpower_case(case)
+    plan = eng.HostPlan(net, eng.NrOptions(flat_start=True))
+    st = plan.stats()
+    lp, up, ui, m = plan.lp, plan.up, plan.ui, plan.m
+    updates = sum(int(lp[j + 1] - lp[j] - 1) for k in range(m) for j in ui[up[k]:up[k + 1] - 1])
+    assert st["update_lanes"] == updates
+    assert st["forward_lanes"] == plan.nnz_l - m and st["back_lanes"] == plan.nnz_u - m
+    kinds = st["rows_by_kind"]
+    assert sum(kinds) <= st["refactor_rows"] < sum(kinds) + 16   # padding: at most one ring of no-op rows
+    assert kinds[3] >= plan.n_levels                               # at least one PIVOT row (chunk) per level
+    print(case, st, flush=True)
+
+
 def test_plan_of_warm_started_outage_grid(eng, oracle):
     """The N-1 grid's plan is taken at the base solution (warm start), as the screening run uses it."""
     from gridsim_amd import gridgen

@@ -132,6 +132,37 @@ def test_n1_batch_2000_subset_matches_oracle(ctx, eng, oracle):
     _compare(oracle, r.results, r.vm, r.va, rres, rvm, rva, "N-1 2000 subset")
 
 
+def test_two_instances_per_work_group_equal_one(ctx, eng, oracle):
+    """Where the solve vector sits in LDS and two instances per work-group keep more of them resident, the batched
+    kernel pairs them on one shared LDS solve vector (gs_network_launch_shape says so). The pairing changes no bit:
+    an odd batch (the last work-group has one idle wave) gives the same statuses, iterations and voltages as one
+    instance per work-group (GS_FLAG_SINGLE_WAVE), and both match the oracle."""
+    from gridsim_amd import gridgen
+    net = gridgen.config2_grid()
+    rec, vm, va = oracle.solve(net, oracle.options(flat_start=True, max_iterations=60))
+    warm = net.with_start(vm, np.degrees(va))
+    outs = gridgen.config2_outages(warm)[::25][:41]
+    pair = eng.NrOptions(max_iterations=10, lu_precision=eng.LU_FP64)
+    single = eng.NrOptions(max_iterations=10, lu_precision=eng.LU_FP64, flags=eng.FLAG_SINGLE_WAVE)
+    dn = eng.DeviceNetwork(ctx, warm, pair)
+    sp, ss = dn.launch_shape(pair), dn.launch_shape(single)
+    print("launch shapes", sp, ss, flush=True)
+    assert sp["x_in_lds"] and sp["waves_per_group"] == 2 and ss["waves_per_group"] == 1
+    assert sp["lds_bytes"] < 2 * ss["lds_bytes"]
+    fp32 = dn.launch_shape(eng.NrOptions(lu_precision=eng.LU_FP32, refine_steps=1))
+    glob = dn.launch_shape(eng.NrOptions(flags=eng.FLAG_X_GLOBAL))
+    assert fp32["lu_fp32"] and not glob["x_in_lds"] and glob["waves_per_group"] == 1
+    a = dn.solve(outages=outs, options=pair)
+    b = dn.solve(outages=outs, options=single)
+    assert len(outs) % 2 == 1
+    assert np.array_equal(a.status, b.status)
+    assert np.array_equal(a.results["iterations"], b.results["iterations"])
+    assert np.array_equal(a.vm, b.vm) and np.array_equal(a.va, b.va)
+    rres, rvm, rva = oracle.screen(warm, outs, oracle.options(max_iterations=10), threads=8)
+    _compare(oracle, a.results, a.vm, a.va, rres, rvm, rva, "paired N-1 2000")
+    dn.close()
+
+
 def test_load_scaled_instances_match_oracle(ctx, eng, oracle, n1_118):
     warm, outs, _ = n1_118
     scales = np.array([0.8, 0.95, 1.1, 1.2])
