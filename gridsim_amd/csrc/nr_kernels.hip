@@ -279,9 +279,10 @@ __device__ __forceinline__ double mismatch_jacobian(const DevPlan& P, const Inst
 constexpr int kRing = kStreamPad, kHalf = kRing / 2, kAhead = kRsValueAhead;
 static_assert(kAhead < kRing, "the value of a row is loaded after its record");
 
-// Rows [row0, row1) of the stream (the whole stream, or one chunk of a level-parallel run): the loop runs whole rings
-// from row0 and treats the rows from row1 on as no-ops.
-template <typename T, bool EXACT>
+// Rows [row0, row1) of the stream (the whole stream, or with CHUNK one chunk of a level-parallel run): the loop runs
+// whole rings from row0's pair and, with CHUNK, treats the rows outside [row0, row1) as no-ops (the whole stream's
+// rounding rows are no-ops already).
+template <typename T, bool EXACT, bool CHUNK = false>
 __device__ __forceinline__ int refactor(const DevPlan& P, GLOBAL T* vals, GLOBAL T* fval, GLOBAL T* bval,
                                         GLOBAL T* dval, LDSP T* ws, double growth_tol, GLOBAL const double* rowfloor,
                                         int lane, double* min_ratio, int row0, int row1) {
@@ -332,9 +333,9 @@ __device__ __forceinline__ int refactor(const DevPlan& P, GLOBAL T* vals, GLOBAL
       // Every lane holds the row's kind and flags: taken as a scalar here, at the row itself (the record arrived
       // kRing rows ago), so the dispatch below is a scalar branch and the row's lanes are never masked.
       int meta = (int)((unsigned)__builtin_amdgcn_readfirstlane(r4.y) >> kRsMetaShift);
-      if (base + s >= nrows || base + s < first) meta = kRsNop;
+      if (CHUNK && (base + s >= nrows || base + s < first)) meta = kRsNop;
       const int kind = meta & kRsKindMask;
-      const int TLr = kTrackLanes << ((meta >> kRsWidthShift) & 3);   // lanes per track of the row's chunk
+      const int wsh = 3 + ((meta >> kRsWidthShift) & 3), TLr = 1 << wsh;   // lanes per track of the row's chunk
       if (kind == kRsUpdate) {
         // a lane without an update works on its own spare slot
         const int dst = r4.x & 0xFFFF, src = (int)((unsigned)r4.x >> 16);
@@ -383,7 +384,7 @@ __device__ __forceinline__ int refactor(const DevPlan& P, GLOBAL T* vals, GLOBAL
         // every lane reads its slot (an idle lane a slot inside the workspace) and forms the L quotient; the stores
         // are masked by kind
         const int wk = (int)((unsigned)r4.x >> 24), pos = r4.y & kRsIndexMask;
-        const T v = ws[(lane / TLr) * P.wstride + write_row * TLr + (lane & (TLr - 1))];
+        const T v = ws[(lane >> wsh) * P.wstride + write_row * TLr + (lane & (TLr - 1))];
         const T lv = v / piv;
         if (wk == kWriteU) bval[pos] = v;
         if (wk == kWritePivot) dval[pos] = v;
@@ -852,7 +853,7 @@ __global__ __launch_bounds__(kWave) void wide_refactor(const InstDesc* __restric
   const int c = first_chunk + blockIdx.x * per;
   const int row0 = P.rs_chunk_row[c], row1 = P.rs_chunk_row[c + per];
   double ratio = 1.0;
-  const int fail = refactor<T, false>(P, I.vals, I.fval, I.bval, I.dval, lds(reinterpret_cast<T*>(smem)),
+  const int fail = refactor<T, false, true>(P, I.vals, I.fval, I.bval, I.dval, lds(reinterpret_cast<T*>(smem)),
                                       prm.growth_tol, nullptr, opaque_lane(threadIdx.x), &ratio, row0, row1);
   if (threadIdx.x == 0) {
     if (ratio < 1.0) atomicMin(reinterpret_cast<unsigned long long*>(&st->ratio_bits),
