@@ -219,10 +219,10 @@ Layout instance_layout(const DevPlan& h, bool fp32, bool x_lds) {
   L.r = o; o += al(8 * (size_t)h.m);
   L.jv = o; o += al(8 * (size_t)h.nnz_j);
   L.tpq = o; o += al(16 * (size_t)h.nnz_y);
-  L.vals = o; o += al(t * ((size_t)h.n_a + h.nnz_l));
-  L.fval = o; o += al(t * (size_t)std::max(1, h.n_fval));
-  L.bval = o; o += al(t * (size_t)std::max(1, h.n_bval));
-  L.dval = o; o += al(t * (size_t)h.m);
+  L.vals = o; o += al(t * (size_t)vals_slots(h));
+  L.fval = o; o += al(t * (size_t)fval_slots(h));
+  L.bval = o; o += al(t * (size_t)bval_slots(h));
+  L.dval = o; o += al(t * (size_t)dval_slots(h));
   L.xg = o; o += x_lds ? 0 : al(t * (size_t)h.m);
   L.spec = o; o += al(16 * (size_t)h.n);
   L.start = o; o += al(16 * (size_t)h.n);
@@ -998,9 +998,9 @@ gs_status seam_solve_one(gs_ctx* ctx, int n, const int* rowptr, const int* colid
   auto run_device = [&](SeamPlan& sp) -> int {
     DevMem a, bb, xx, fv, bv, dv, xs, rf, av, stt, desc;
     a.ensure(8 * (size_t)nnz); bb.ensure(8 * (size_t)n); xx.ensure(8 * (size_t)n);
-    fv.ensure(8 * (size_t)std::max(1, sp.dev.h.n_fval)); bv.ensure(8 * (size_t)std::max(1, sp.dev.h.n_bval));
-    dv.ensure(8 * (size_t)n); xs.ensure(8 * (size_t)n);
-    rf.ensure(8 * (size_t)n); av.ensure(8 * ((size_t)sp.dev.h.n_a + sp.dev.h.nnz_l)); stt.ensure(4);
+    fv.ensure(8 * (size_t)fval_slots(sp.dev.h)); bv.ensure(8 * (size_t)bval_slots(sp.dev.h));
+    dv.ensure(8 * (size_t)dval_slots(sp.dev.h)); xs.ensure(8 * (size_t)n);
+    rf.ensure(8 * (size_t)n); av.ensure(8 * (size_t)vals_slots(sp.dev.h)); stt.ensure(4);
     check(hipMemcpyAsync(a.p, val, 8 * (size_t)nnz, hipMemcpyHostToDevice, s), "seam a");
     check(hipMemcpyAsync(bb.p, b, 8 * (size_t)n, hipMemcpyHostToDevice, s), "seam b");
     SeamDesc d{a.as<double>(),  bb.as<double>(), xx.as<double>(), av.as<double>(), fv.as<double>(),

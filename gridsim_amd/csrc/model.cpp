@@ -760,8 +760,9 @@ void compile_plan(const Plan& plan, DevicePlanHost& d) {
           }
           // the CHAIN step
           for (int lane = 0; lane < 64; lane++) {
-            int x0 = 0, row = D::kRsIndexMask, fl = D::kStepChain;
+            int x0 = 0, row = 0, fl = D::kStepChain | D::kChainIdle;
             if (lane < k && !ended[lane] && (take[lane] > 0 || done[lane] == len[lane])) {
+              fl = D::kStepChain;
               row = rows[p0 + lane];
               x0 = st[lane] | (take[lane] << 16);
               if (!begun[lane]) { fl |= D::kChainBegin; begun[lane] = 1; }
@@ -1004,7 +1005,7 @@ bool check_level_schedule(const DevicePlanHost& d, int ring, int padded_rows, in
             const int field = r2[1] & D::kRsIndexMask;
             if (kind == D::kStepProduct && r2[0] >= 0 && (row_level.at(r2[0]) == -1 || row_level[r2[0]] >= l))
               return bad("a pass reads a row its level's predecessors did not finish");
-            if (kind == D::kStepChain && field != D::kRsIndexMask && (meta & D::kChainEnd)) rows.push_back(field);
+            if (kind == D::kStepChain && !(meta & D::kChainIdle) && (meta & D::kChainEnd)) rows.push_back(field);
           }
       }
       for (int r : rows) {
@@ -1087,6 +1088,8 @@ bool emulate_device_lu(const DevicePlanHost& d, int nnz_j, const double* jv, con
       if ((int)((unsigned)rr[1] >> D::kRsMetaShift) != meta) return bad("row kind differs across lanes");
       const int rc[2] = {rr[0], rr[1] & D::kRsIndexMask};
       if ((kind == D::kRsAssign || kind == D::kRsUpdate) && rc[1] >= n_vals) return bad("row value index");
+      // the device loads vals[field] for every row (rows without a value too): vals holds max(n_vals, m, n_bval)
+      if (rc[1] >= std::max({n_vals, m, d.n_bval})) return bad("row field outside the value array");
       const int load_row = r - D::kRsValueAhead;  // the value is loaded at the end of this row
       if (kind == D::kRsAssign) {
         if (rc[0] == d.spare(lane)) continue;
@@ -1159,6 +1162,10 @@ bool emulate_device_lu(const DevicePlanHost& d, int nnz_j, const double* jv, con
         const int* r2 = &rec[((size_t)t * 64 + lane) * 2];
         const int meta = (int)((unsigned)r2[1] >> D::kRsMetaShift), field = r2[1] & D::kRsIndexMask;
         if ((meta & D::kStepKindMask) != kind) { why = "stream step kind differs across lanes"; return false; }
+        // the device loads the value array and (back) the pivot array at every step's field: they hold
+        // max(values, m) and max(n_bval, m) entries
+        if (field >= std::max({(int)vals.size(), m, 1}) || (back && field >= std::max({d.n_bval, m, 1})))
+          { why = "step field outside the value or pivot array"; return false; }
         if (kind == D::kStepProduct) {
           if (sbase + 64 > cap) { why = "products pass the scratch area"; return false; }
           if (r2[0] < 0) continue;
@@ -1166,8 +1173,11 @@ bool emulate_device_lu(const DevicePlanHost& d, int nnz_j, const double* jv, con
           if (field >= (int)vals.size()) { why = "stream value index"; return false; }
           scratch[sbase + lane] = vals[field] * x[r2[0]];
         } else if (kind == D::kStepChain) {
-          if (field == D::kRsIndexMask) continue;
           if (field >= m) { why = "chain row"; return false; }
+          if (meta & D::kChainIdle) {
+            if (meta & (D::kChainBegin | D::kChainEnd)) { why = "an idle chain lane begins or ends a row"; return false; }
+            continue;
+          }
           const int st = r2[0] & 0xFFFF, len = (int)((unsigned)r2[0] >> 16);
           if (st + len > cap) { why = "chain outside the scratch area"; return false; }
           if (meta & D::kChainBegin) {

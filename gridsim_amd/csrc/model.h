@@ -175,15 +175,15 @@ struct DevicePlanHost {
   // Records are 2 ints per lane and step, the second holding a 24-bit field and the step's kind and flags above it:
   //   PRODUCT  (col, value index | kStepProduct << 24): scratch[s + lane] = vals[value index] * x[col] (col = -1:
   //            idle), s = 64 * (PRODUCT steps since the last CHAIN)
-  //   CHAIN    (start | len << 16, row | (kStepChain | kChainBegin? | kChainEnd?) << 24): kChainBegin: acc = x[row];
-  //            then acc -= scratch[start + t] for t < len; kChainEnd: x[row] = acc (back: acc / pivot). row =
-  //            kRsIndexMask: idle lane.
+  //   CHAIN    (start | len << 16, row | (kStepChain | kChainBegin? | kChainEnd? | kChainIdle?) << 24): kChainBegin:
+  //            acc = x[row]; then acc -= scratch[start + t] for t < len; kChainEnd: x[row] = acc (back: acc / pivot).
+  //            kChainIdle: a lane with no row (its row field 0).
   // The factors are stored in the order the PRODUCT steps read them: fval holds L's off-diagonal entries in
   // forward-stream order, bval U's off-diagonal entries in back-stream order (the active lanes of a step being a
   // prefix, a step's values are contiguous), dval U's diagonal by pivot step. fpos / bpos map an L / U entry (CSC
   // index) to its position, -1 for a diagonal. The refactor keeps L in column order too (lx, CSC), for its own updates.
   static constexpr int kStepProduct = 1, kStepChain = 2, kStepKindMask = 15;
-  static constexpr int kChainBegin = 16, kChainEnd = 32;
+  static constexpr int kChainBegin = 16, kChainEnd = 32, kChainIdle = 64;
   // scratch slots the solves may use: the refactor workspace rounded down to whole steps
   int solve_scratch() const { return (kTracks * wstride + kSpareSlots) / 64 * 64; }
   int fs_steps = 0, bs_steps = 0, n_fval = 0, n_bval = 0;

@@ -59,7 +59,7 @@ constexpr int kRsMetaShift = 24, kRsIndexMask = (1 << kRsMetaShift) - 1;
 constexpr int kWriteU = 1, kWritePivot = 2, kWriteL = 3;
 // solve step kinds and CHAIN flags (model.h DevicePlanHost)
 constexpr int kStepProduct = 1, kStepChain = 2, kStepKindMask = 15;
-constexpr int kChainBegin = 16, kChainEnd = 32;
+constexpr int kChainBegin = 16, kChainEnd = 32, kChainIdle = 64;
 // load-ahead distance of the row and step streams, and the padding each stream carries; rows of the refactor load
 // their value kRsValueAhead rows ahead (the plan places its rows for that distance)
 constexpr int kStreamPad = 16;
@@ -71,6 +71,15 @@ constexpr int kTracks = 8, kTrackLanes = 8;
 constexpr int kWsSpare = 64;
 // slots of the refactor workspace (the triangular-solve vector follows it in LDS)
 __host__ __device__ inline int ws_slots(const DevPlan& p) { return kTracks * p.wstride + kWsSpare; }
+// Entries of each factor array per instance. Every stream record's 24-bit field indexes the array its value load
+// reads (vals for the refactor rows, fval / bval and dval for the solve steps), also in the rows and steps that take no
+// value (pivot steps, factor positions, solve rows), so the arrays are sized to hold every such index and the loads
+// need no select.
+__host__ __device__ inline int imax3(int a, int b, int c) { return a > b ? (a > c ? a : c) : (b > c ? b : c); }
+__host__ __device__ inline int vals_slots(const DevPlan& p) { return imax3(p.n_a + p.nnz_l, p.m, p.n_bval); }
+__host__ __device__ inline int fval_slots(const DevPlan& p) { return imax3(p.n_fval, p.m, 1); }
+__host__ __device__ inline int bval_slots(const DevPlan& p) { return imax3(p.n_bval, p.m, 1); }
+__host__ __device__ inline int dval_slots(const DevPlan& p) { return imax3(p.n_bval, p.m, 1); }
 
 constexpr int kMaxPatch = 8;
 

@@ -303,7 +303,12 @@ __device__ __forceinline__ int refactor(const DevPlan& P, GLOBAL T* vals, GLOBAL
     const v4i q = rp[s / 2];
     return (s % 2) ? (v2i){q.z, q.w} : (v2i){q.x, q.y};
   };
+  // The rows without a value (no-op, pivot, write) load entry 0 in a batch run: the same line for every lane, where
+  // their own field (a pivot step, a factor position) would touch 64 scattered lines of the value array, which costs
+  // more than the select when many waves share a CU. A level-parallel chunk (one wave on its CU, latency-bound) loads
+  // at the field unselected: every field indexes inside vals (vals_slots).
   auto value_index = [](const v2i& r) {
+    if constexpr (CHUNK) return r.y & kRsIndexMask;
     const int k = (int)((unsigned)r.y >> kRsMetaShift) & kRsKindMask;
     return k == kRsAssign || k == kRsUpdate ? (r.y & kRsIndexMask) : 0;
   };
@@ -437,7 +442,7 @@ __device__ __forceinline__ int wave_max_nonneg(int v) {
   return __builtin_amdgcn_readlane(v, 63);
 }
 
-template <typename T, bool BACK, bool XLDS, typename XP>
+template <typename T, bool BACK, bool XLDS, typename XP, bool WIDE = false>
 __device__ __forceinline__ void solve_stream(GLOBAL const v2i* rec, int step0, int step1, GLOBAL const T* vals,
                                              GLOBAL const T* dval, XP x, LDSP T* scr, int lane) {
   rec += (size_t)step0 * kWave;
@@ -445,11 +450,17 @@ __device__ __forceinline__ void solve_stream(GLOBAL const v2i* rec, int step0, i
   v2i rr[kRing];
   T rv[kRing], rd[kRing];
   // the kind of a step (every lane holds it), and the value index / pivot row a lane's record names for it
+  // a PRODUCT step's value index, a CHAIN step's row (its pivot); in a batch run the other steps load entry 0, one
+  // line for every lane, rather than 64 scattered ones; a level-parallel pass loads at the field unselected (every
+  // field indexes inside the value and pivot arrays: fval_slots, bval_slots, dval_slots)
   auto kind_of = [](const v2i& r) { return (int)((unsigned)r.y >> kRsMetaShift) & kStepKindMask; };
-  auto value_index = [&](const v2i& r) { return kind_of(r) == kStepProduct ? (r.y & kRsIndexMask) : 0; };
+  auto value_index = [&](const v2i& r) {
+    if constexpr (WIDE) return r.y & kRsIndexMask;
+    return kind_of(r) == kStepProduct ? (r.y & kRsIndexMask) : 0;
+  };
   auto pivot_index = [&](const v2i& r) {
-    const int row = r.y & kRsIndexMask;
-    return kind_of(r) == kStepChain && row != kRsIndexMask ? row : 0;
+    if constexpr (WIDE) return r.y & kRsIndexMask;
+    return kind_of(r) == kStepChain ? (r.y & kRsIndexMask) : 0;
   };
   // first loads in the loop's own order (see refactor): records of steps 0..kHalf-1, then per step the record of step
   // kHalf+h and the values of step h; the loop's first kHalf steps load the values of steps kHalf..kRing-1
@@ -480,9 +491,9 @@ __device__ __forceinline__ void solve_stream(GLOBAL const v2i* rec, int step0, i
         sbase += kWave;
         WAVE_SYNC();
       } else if (kind == kStepChain) {
-        const int field = r2.y & kRsIndexMask, fl = (int)((unsigned)r2.y >> kRsMetaShift);
-        const bool active = field != kRsIndexMask;
-        const int row = active ? field : 0, st = r2.x & 0xFFFF, len = (int)((unsigned)r2.x >> 16);
+        const int row = r2.y & kRsIndexMask, fl = (int)((unsigned)r2.y >> kRsMetaShift);
+        const bool active = !(fl & kChainIdle);
+        const int st = r2.x & 0xFFFF, len = (int)((unsigned)r2.x >> 16);
         const int maxlen = wave_max_nonneg(len);
         const T x0 = x[row];
         if (fl & kChainBegin) acc = x0;
@@ -888,7 +899,7 @@ __global__ __launch_bounds__(kWave) void wide_solve(const InstDesc* __restrict__
   const int* pass_step = BACK ? P.bs_pass_step : P.fs_pass_step;
   GLOBAL T* x = glob(reinterpret_cast<T*>(d.xg));
   extern __shared__ __align__(16) unsigned char smem[];
-  solve_stream<T, BACK, false>(globv<v2i>(BACK ? P.bs_rec : P.fs_rec), pass_step[p], pass_step[p + per],
+  solve_stream<T, BACK, false, GLOBAL T*, true>(globv<v2i>(BACK ? P.bs_rec : P.fs_rec), pass_step[p], pass_step[p + per],
                                glob(reinterpret_cast<const T*>(BACK ? d.bval : d.fval)),
                                glob(reinterpret_cast<const T*>(d.dval)), x, lds(reinterpret_cast<T*>(smem)),
                                opaque_lane(threadIdx.x));
