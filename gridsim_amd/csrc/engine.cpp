@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -23,9 +24,16 @@ static_assert(DevicePlanHost::kTracks == kTracks && DevicePlanHost::kTrackLanes 
 // ================================================================================================ plans
 namespace gs {
 
+// GRIDSIM_SOLVE_SCRATCH_CAP (slots, for tests): compile plans whose triangular-solve passes run in segments of at most
+// that many products, to exercise the segment carry on small networks. Unset or 0: the whole refactor workspace.
+int solve_scratch_cap() {
+  const char* v = std::getenv("GRIDSIM_SOLVE_SCRATCH_CAP");
+  return v ? std::max(0, std::atoi(v)) : 0;
+}
+
 void upload_plan(DevicePlan& dp, const YBus* y, const JacobianPattern& jp, const BusSpec* spec, const Plan& plan,
                  double base_mva, hipStream_t s) {
-  compile_plan(plan, dp.dph);
+  compile_plan(plan, dp.dph, solve_scratch_cap());
   const DevicePlanHost& d = dp.dph;
   DevPlan& h = dp.h;
   h = DevPlan{};
@@ -1422,7 +1430,7 @@ GS_API gs_status gs_plan_build(const gs_network* net, const gs_nr_options* opts,
     p->jp = build_jacobian_pattern(p->y, p->spec.type);
     std::string why;
     if (!symbolic_pass(p->y, p->jp, p->spec.vm.data(), p->spec.va.data(), p->plan, why)) return GS_SINGULAR;
-    compile_plan(p->plan, p->dph);
+    compile_plan(p->plan, p->dph, solve_scratch_cap());
     *out = p.release();
     return GS_OK;
   } catch (const HipError& e) {

@@ -476,9 +476,10 @@ bool gp_plan(int m, const vector<int>& rowptr, const vector<int>& colidx, const 
   return true;
 }
 
-void compile_plan(const Plan& plan, DevicePlanHost& d) {
+void compile_plan(const Plan& plan, DevicePlanHost& d, int scratch_cap) {
   int m = plan.m;
   d = DevicePlanHost();
+  d.scratch_cap = scratch_cap;
   d.m = m;
   d.lp = plan.lp;
   d.up = plan.up;

@@ -184,8 +184,13 @@ struct DevicePlanHost {
   // index) to its position, -1 for a diagonal. The refactor keeps L in column order too (lx, CSC), for its own updates.
   static constexpr int kStepProduct = 1, kStepChain = 2, kStepKindMask = 15;
   static constexpr int kChainBegin = 16, kChainEnd = 32, kChainIdle = 64;
-  // scratch slots the solves may use: the refactor workspace rounded down to whole steps
-  int solve_scratch() const { return (kTracks * wstride + kSpareSlots) / 64 * 64; }
+  // scratch slots the solves may use: the refactor workspace rounded down to whole steps (or fewer, when the plan was
+  // compiled with a smaller cap: scratch_cap, 0 = none; tests use it to make passes run in several segments)
+  int scratch_cap = 0;
+  int solve_scratch() const {
+    const int full = (kTracks * wstride + kSpareSlots) / 64 * 64;
+    return scratch_cap > 0 && scratch_cap < full ? scratch_cap / 64 * 64 : full;
+  }
   int fs_steps = 0, bs_steps = 0, n_fval = 0, n_bval = 0;
   vector<int> fs_rec, bs_rec;                       // 2 ints per lane per step
   // pass p of a stream covers steps [pass_step[p], pass_step[p + 1]); level l's passes are [level_pass[l], [l + 1])
@@ -196,7 +201,8 @@ struct DevicePlanHost {
   vector<int> fw_order, fw_level_end, lr_ptr, lr_col, lr_idx;  // L rows: columns ascending
   vector<int> bw_order, bw_level_end, ur_ptr, ur_col, ur_idx;  // U rows: columns descending, diagonal excluded
 };
-void compile_plan(const Plan& plan, DevicePlanHost& out);
+// scratch_cap > 0 caps the solves' scratch area (slots), so that passes run in several segments (for tests)
+void compile_plan(const Plan& plan, DevicePlanHost& out, int scratch_cap = 0);
 
 // Host emulation of the device refactor (level chunks, rows of 64 records) and level-scheduled solves, with every
 // index checked against its buffer: out = A^{-1} rhs for the CSR values jv. Returns false (and why) on an

@@ -135,6 +135,25 @@ def test_device_plan_covers_every_update_and_solve_entry_once(eng, case):
     print(case, st, flush=True)
 
 
+@pytest.mark.parametrize("case", ["case300", "synthetic-2000"])
+def test_solve_passes_in_segments_emulated_on_host(eng, case, monkeypatch):
+    """With the solves' scratch area capped at one step of products (GRIDSIM_SOLVE_SCRATCH_CAP=64), passes with more
+    entries run in several segments, each lane's running value carried from CHAIN to CHAIN: the host emulation still
+    checks every index and solves J dx = f to FP64 accuracy, and the streams hold more steps than uncapped."""
+    from gridsim_amd import gridgen
+    net = gridgen.config2_grid() if case == "synthetic-2000" else matpower_case(case)
+    full = eng.HostPlan(net, eng.NrOptions(flat_start=True)).stats()
+    monkeypatch.setenv("GRIDSIM_SOLVE_SCRATCH_CAP", "64")
+    plan = eng.HostPlan(net, eng.NrOptions(flat_start=True))
+    st = plan.stats()
+    assert st["forward_lanes"] == full["forward_lanes"] and st["back_lanes"] == full["back_lanes"]
+    assert st["forward_steps"] + st["back_steps"] > full["forward_steps"] + full["back_steps"]
+    res = C.c_double()
+    why = C.create_string_buffer(128)
+    assert eng.lib().gs_plan_selfcheck(plan.h, C.byref(res), why) == eng.GS_OK, why.value.decode()
+    assert res.value < 1e-12
+
+
 def test_plan_of_warm_started_outage_grid(eng, oracle):
     """The N-1 grid's plan is taken at the base solution (warm start), as the screening run uses it."""
     from gridsim_amd import gridgen

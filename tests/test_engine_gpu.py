@@ -163,6 +163,35 @@ def test_two_instances_per_work_group_equal_one(ctx, eng, oracle):
     dn.close()
 
 
+def test_solve_passes_in_segments_change_no_bit(ctx, eng, oracle, monkeypatch):
+    """Plans whose triangular-solve passes run in segments of one step of products (GRIDSIM_SOLVE_SCRATCH_CAP=64)
+    give bit-identical batched results to the uncapped plan (the products and each row's subtraction order are the
+    same; only where the running values wait between CHAIN steps differs), on the wavefront and level-parallel
+    schedules of the 2,000-bus grid."""
+    from gridsim_amd import gridgen
+    net = gridgen.config2_grid()
+    rec, vm, va = oracle.solve(net, oracle.options(flat_start=True, max_iterations=60))
+    warm = net.with_start(vm, np.degrees(va))
+    outs = gridgen.config2_outages(warm)
+    for sched, sel in ((0, outs[::80]), (eng.SCHEDULE_LEVELS, outs[:3])):
+        opts = eng.NrOptions(max_iterations=10, lu_precision=eng.LU_FP64, schedule=sched)
+        monkeypatch.delenv("GRIDSIM_SOLVE_SCRATCH_CAP", raising=False)
+        full = eng.DeviceNetwork(ctx, warm, opts)
+        monkeypatch.setenv("GRIDSIM_SOLVE_SCRATCH_CAP", "64")
+        seg = eng.DeviceNetwork(ctx, warm, opts)
+        monkeypatch.delenv("GRIDSIM_SOLVE_SCRATCH_CAP")
+        fs, ff = seg.plan_stats(), full.plan_stats()
+        assert fs["forward_steps"] + fs["back_steps"] > ff["forward_steps"] + ff["back_steps"]
+        a = full.solve(outages=sel, options=opts)
+        b = seg.solve(outages=sel, options=opts)
+        print(f"schedule {sched}: statuses {np.bincount(a.status)}", flush=True)
+        assert np.array_equal(a.status, b.status)
+        assert np.array_equal(a.results["iterations"], b.results["iterations"])
+        assert np.array_equal(a.vm, b.vm) and np.array_equal(a.va, b.va)
+        full.close()
+        seg.close()
+
+
 def test_load_scaled_instances_match_oracle(ctx, eng, oracle, n1_118):
     warm, outs, _ = n1_118
     scales = np.array([0.8, 0.95, 1.1, 1.2])
