@@ -1333,7 +1333,7 @@ hipError_t launch_wide_start(const InstDesc* descs, const DevPlan& plan, int cou
 
 // A level of at most this many chunks (solve passes) is cheaper walked in sequence by one work-group, together with
 // its narrow neighbours, than given a kernel of its own.
-constexpr int kNarrow = 4;
+constexpr int kNarrow = 2, kNarrowSolve = 4;  // measured on the 70,000-bus grid: 2 and 4 fastest
 
 // One Newton step of every instance, as the kernel sequence the host captures: mismatch, verdict, refactor level by
 // level, the solves level by level (with the refinement steps), the update. level_chunks / fs_level_pass /
@@ -1350,12 +1350,12 @@ static void enqueue_wide_step(const InstDesc* descs, const DevPlan& P, int count
   hipLaunchKernelGGL(wide_buses<T>, dim3(nbus64, cnt), w, 0, s, descs, P, prm);
   hipLaunchKernelGGL(wide_verdict, dim3(1, cnt), w, 0, s, descs, P, prm, ctl);
   // levels of at most kNarrow chunks (passes) run back to back on one work-group; wider levels one kernel each
-  auto segments = [](const std::vector<int>& level_first, auto&& launch) {
+  auto segments = [](const std::vector<int>& level_first, int narrow, auto&& launch) {
     int run0 = -1, run1 = -1;
     for (size_t l = 0; l + 1 < level_first.size(); l++) {
       const int a = level_first[l], b = level_first[l + 1];
       if (b <= a) continue;
-      if (b - a <= kNarrow) {
+      if (b - a <= narrow) {
         if (run0 < 0) run0 = a;
         run1 = b;
         continue;
@@ -1365,15 +1365,15 @@ static void enqueue_wide_step(const InstDesc* descs, const DevPlan& P, int count
     }
     if (run0 >= 0) launch(run0, run1, 1);
   };
-  segments(level_chunks, [&](int c0, int c1, int blocks) {
+  segments(level_chunks, kNarrow, [&](int c0, int c1, int blocks) {
     hipLaunchKernelGGL(wide_refactor<T>, dim3(blocks, cnt), w, ws_bytes, s, descs, P, prm, ctl, c0, c1);
   });
   auto solve = [&](int use_r) {
     hipLaunchKernelGGL(wide_permute<T>, dim3(m256, cnt), b256, 0, s, descs, P, use_r);
-    segments(fs_level_pass, [&](int p0, int p1, int blocks) {
+    segments(fs_level_pass, kNarrowSolve, [&](int p0, int p1, int blocks) {
       hipLaunchKernelGGL((wide_solve<T, false>), dim3(blocks, cnt), w, ws_bytes, s, descs, P, p0, p1);
     });
-    segments(bs_level_pass, [&](int p0, int p1, int blocks) {
+    segments(bs_level_pass, kNarrowSolve, [&](int p0, int p1, int blocks) {
       hipLaunchKernelGGL((wide_solve<T, true>), dim3(blocks, cnt), w, ws_bytes, s, descs, P, p0, p1);
     });
     hipLaunchKernelGGL(wide_unpermute<T>, dim3(m256, cnt), b256, 0, s, descs, P, use_r);
