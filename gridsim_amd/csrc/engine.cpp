@@ -31,6 +31,29 @@ int solve_scratch_cap() {
   return v ? std::max(0, std::atoi(v)) : 0;
 }
 
+WideSplit wide_split_of(const DevicePlanHost& d) {
+  WideSplit sp;
+  const int nl = d.n_levels;
+  // the trunk: the final run of levels the schedule walks on one work-group (at most kNarrow chunks each)
+  int t = nl;
+  while (t > 0 && d.rs_level_chunk[t] - d.rs_level_chunk[t - 1] <= kNarrow) t--;
+  sp.trunk_level = t;
+  if (t == 0 || t == nl) { sp.trunk_level = nl; return sp; }
+  std::vector<int> col_level(d.m, 0);
+  for (int l = 0, c = 0; l < nl; l++)
+    for (; c < d.level_end[l]; c++) col_level[d.col_order[c]] = l;
+  // the first forward level with a row that reads a factor column of the trunk
+  int f = d.fw_levels;
+  for (int l = 0, b0 = 0; l < d.fw_levels && f == d.fw_levels; b0 = d.fw_level_end[l], l++)
+    for (int k = b0; k < d.fw_level_end[l] && f == d.fw_levels; k++) {
+      const int i = d.fw_order[k];
+      for (int e = d.lr_ptr[i]; e < d.lr_ptr[i + 1]; e++)
+        if (col_level[d.lr_col[e]] >= t) { f = l; break; }
+    }
+  sp.fs_level = f;
+  return sp;
+}
+
 void upload_plan(DevicePlan& dp, const YBus* y, const JacobianPattern& jp, const BusSpec* spec, const Plan& plan,
                  double base_mva, hipStream_t s) {
   compile_plan(plan, dp.dph, solve_scratch_cap());
@@ -456,9 +479,18 @@ int launch_wide(DevicePlan& dp, const PreparedRun& pr, hipStream_t s) {
     if (wg.graph) { (void)hipGraphDestroy(wg.graph); wg.graph = nullptr; }
     const DevicePlanHost& d = dp.dph;
     const size_t ws_bytes = (pr.run.fp32 ? 4 : 8) * (size_t)ws_slots(dp.h);
+    WideSplit split = wide_split_of(d);
+    if (split.trunk_level < d.n_levels && split.fs_level > 0) {
+      if (!wg.side) check(hipStreamCreateWithFlags(&wg.side, hipStreamNonBlocking), "side stream");
+      if (!wg.fork) check(hipEventCreateWithFlags(&wg.fork, hipEventDisableTiming), "fork event");
+      if (!wg.join) check(hipEventCreateWithFlags(&wg.join, hipEventDisableTiming), "join event");
+      split.side = wg.side;
+      split.fork = wg.fork;
+      split.join = wg.join;
+    }
     check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "capture");
     hipError_t e = launch_wide_step(pr.run.fp32, pr.desc, dp.h, pr.count, p, ctl, d.rs_level_chunk, d.fs_level_pass,
-                                    d.bs_level_pass, ws_bytes, s);
+                                    d.bs_level_pass, ws_bytes, split, s);
     hipError_t e2 = hipStreamEndCapture(s, &wg.graph);
     check(e, "level-parallel step");
     check(e2, "end capture");

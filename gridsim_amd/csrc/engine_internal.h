@@ -27,7 +27,10 @@ hipError_t launch_screen(const InstDesc* descs, const DevPlan& plan, int count, 
 hipError_t launch_wide_start(const InstDesc* descs, const DevPlan& plan, int count, WideCtl* ctl, hipStream_t stream);
 hipError_t launch_wide_step(int lu_fp32, const InstDesc* descs, const DevPlan& plan, int count, const NrParams& prm,
                             WideCtl* ctl, const std::vector<int>& level_chunks, const std::vector<int>& fs_level_pass,
-                            const std::vector<int>& bs_level_pass, size_t ws_bytes, hipStream_t stream);
+                            const std::vector<int>& bs_level_pass, size_t ws_bytes, const WideSplit& split,
+                            hipStream_t stream);
+// the split point of a plan (side / events left empty): WideSplit::trunk_level = n_levels when there is no trunk
+WideSplit wide_split_of(const DevicePlanHost& d);
 hipError_t launch_fp64_probe(double* out, hipStream_t stream);
 
 constexpr size_t kLdsLimit = 64 * 1024;  // dynamic LDS a work-group may take for the solve workspace
@@ -81,6 +84,8 @@ void upload(DevMem& dst, const std::vector<T>& v, hipStream_t s) {
 struct WideGraph {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  hipStream_t side = nullptr;            // the fork's second branch (captured into the graph; default priority)
+  hipEvent_t fork = nullptr, join = nullptr;
   const InstDesc* desc = nullptr;
   int count = 0, fp32 = 0;
   NrParams prm{};
@@ -92,6 +97,9 @@ struct WideGraph {
   ~WideGraph() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
+    if (fork) (void)hipEventDestroy(fork);
+    if (join) (void)hipEventDestroy(join);
+    if (side) (void)hipStreamDestroy(side);
     if (host) (void)hipHostFree(host);
   }
 };

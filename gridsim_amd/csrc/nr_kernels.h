@@ -81,6 +81,19 @@ __host__ __device__ inline int fval_slots(const DevPlan& p) { return imax3(p.n_f
 __host__ __device__ inline int bval_slots(const DevPlan& p) { return imax3(p.n_bval, p.m, 1); }
 __host__ __device__ inline int dval_slots(const DevPlan& p) { return imax3(p.n_bval, p.m, 1); }
 
+// Level-parallel schedule: a level of at most this many chunks (solve passes) is cheaper walked in sequence by one
+// work-group, together with its narrow neighbours, than given a kernel of its own (measured on the 70,000-bus grid).
+constexpr int kNarrow = 2, kNarrowSolve = 4;
+
+// Where a level-parallel step may fork: the refactor's levels from trunk_level on (the final run of narrow levels)
+// go to the side stream while the forward solve's levels before fs_level (which read no factor of those columns)
+// proceed; the two branches join before forward level fs_level. side == nullptr: no fork.
+struct WideSplit {
+  int trunk_level = 0, fs_level = 0;
+  hipStream_t side = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+};
+
 constexpr int kMaxPatch = 8;
 
 struct Patch {

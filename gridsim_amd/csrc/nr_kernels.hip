@@ -1331,9 +1331,6 @@ hipError_t launch_wide_start(const InstDesc* descs, const DevPlan& plan, int cou
   return hipGetLastError();
 }
 
-// A level of at most this many chunks (solve passes) is cheaper walked in sequence by one work-group, together with
-// its narrow neighbours, than given a kernel of its own.
-constexpr int kNarrow = 2, kNarrowSolve = 4;  // measured on the 70,000-bus grid: 2 and 4 fastest
 
 // One Newton step of every instance, as the kernel sequence the host captures: mismatch, verdict, refactor level by
 // level, the solves level by level (with the refinement steps), the update. level_chunks / fs_level_pass /
@@ -1341,7 +1338,8 @@ constexpr int kNarrow = 2, kNarrowSolve = 4;  // measured on the 70,000-bus grid
 template <typename T>
 static void enqueue_wide_step(const InstDesc* descs, const DevPlan& P, int count, const NrParams& prm, WideCtl* ctl,
                               const std::vector<int>& level_chunks, const std::vector<int>& fs_level_pass,
-                              const std::vector<int>& bs_level_pass, size_t ws_bytes, hipStream_t s) {
+                              const std::vector<int>& bs_level_pass, size_t ws_bytes, const WideSplit& split,
+                              hipStream_t s) {
   const dim3 w(kWave), b256(256);
   const unsigned nb64 = (unsigned)((P.nnz_y + kWave - 1) / kWave), nbus64 = (unsigned)((P.n + kWave - 1) / kWave);
   const unsigned m256 = (unsigned)((P.m + 255) / 256), m64 = (unsigned)((P.m + kWave - 1) / kWave);
@@ -1365,23 +1363,51 @@ static void enqueue_wide_step(const InstDesc* descs, const DevPlan& P, int count
     }
     if (run0 >= 0) launch(run0, run1, 1);
   };
-  segments(level_chunks, kNarrow, [&](int c0, int c1, int blocks) {
-    hipLaunchKernelGGL(wide_refactor<T>, dim3(blocks, cnt), w, ws_bytes, s, descs, P, prm, ctl, c0, c1);
-  });
-  auto solve = [&](int use_r) {
-    hipLaunchKernelGGL(wide_permute<T>, dim3(m256, cnt), b256, 0, s, descs, P, use_r);
-    segments(fs_level_pass, kNarrowSolve, [&](int p0, int p1, int blocks) {
+  auto slice = [](const std::vector<int>& v, int a, int b) {   // the table of levels [a, b)
+    return std::vector<int>(v.begin() + a, v.begin() + b + 1);
+  };
+  auto refactor_levels = [&](const std::vector<int>& tab, hipStream_t st) {
+    segments(tab, kNarrow, [&](int c0, int c1, int blocks) {
+      hipLaunchKernelGGL(wide_refactor<T>, dim3(blocks, cnt), w, ws_bytes, st, descs, P, prm, ctl, c0, c1);
+    });
+  };
+  auto forward_levels = [&](const std::vector<int>& tab) {
+    segments(tab, kNarrowSolve, [&](int p0, int p1, int blocks) {
       hipLaunchKernelGGL((wide_solve<T, false>), dim3(blocks, cnt), w, ws_bytes, s, descs, P, p0, p1);
     });
+  };
+  const int nl = (int)level_chunks.size() - 1, nf = (int)fs_level_pass.size() - 1;
+  const bool fork = split.side != nullptr && split.trunk_level > 0 && split.trunk_level < nl && split.fs_level > 0 &&
+                    split.fs_level <= nf;
+  if (fork) {
+    // the trunk of the refactor on the side branch; the forward levels that read none of its columns meanwhile
+    refactor_levels(slice(level_chunks, 0, split.trunk_level), s);
+    (void)hipEventRecord(split.fork, s);
+    (void)hipStreamWaitEvent(split.side, split.fork, 0);
+    refactor_levels(slice(level_chunks, split.trunk_level, nl), split.side);
+    (void)hipEventRecord(split.join, split.side);
+    hipLaunchKernelGGL(wide_permute<T>, dim3(m256, cnt), b256, 0, s, descs, P, 0);
+    forward_levels(slice(fs_level_pass, 0, split.fs_level));
+    (void)hipStreamWaitEvent(s, split.join, 0);
+  } else {
+    refactor_levels(level_chunks, s);
+  }
+  auto solve = [&](int use_r, bool forward_from_split) {
+    if (forward_from_split) {
+      forward_levels(slice(fs_level_pass, split.fs_level, nf));
+    } else {
+      hipLaunchKernelGGL(wide_permute<T>, dim3(m256, cnt), b256, 0, s, descs, P, use_r);
+      forward_levels(fs_level_pass);
+    }
     segments(bs_level_pass, kNarrowSolve, [&](int p0, int p1, int blocks) {
       hipLaunchKernelGGL((wide_solve<T, true>), dim3(blocks, cnt), w, ws_bytes, s, descs, P, p0, p1);
     });
     hipLaunchKernelGGL(wide_unpermute<T>, dim3(m256, cnt), b256, 0, s, descs, P, use_r);
   };
-  solve(0);
+  solve(0, fork);
   for (int it = 0; it < prm.refine_steps; it++) {
     hipLaunchKernelGGL(wide_residual, dim3(m64, cnt), w, 0, s, descs, P);
-    solve(1);
+    solve(1, false);
     hipLaunchKernelGGL(wide_add_correction, dim3(m256, cnt), b256, 0, s, descs, P);
   }
   hipLaunchKernelGGL(wide_dx_max, dim3(m256, cnt), b256, 0, s, descs, P);
@@ -1390,13 +1416,14 @@ static void enqueue_wide_step(const InstDesc* descs, const DevPlan& P, int count
 
 hipError_t launch_wide_step(int lu_fp32, const InstDesc* descs, const DevPlan& plan, int count, const NrParams& prm,
                             WideCtl* ctl, const std::vector<int>& level_chunks, const std::vector<int>& fs_level_pass,
-                            const std::vector<int>& bs_level_pass, size_t ws_bytes, hipStream_t stream) {
+                            const std::vector<int>& bs_level_pass, size_t ws_bytes, const WideSplit& split,
+                            hipStream_t stream) {
   if (lu_fp32)
     enqueue_wide_step<float>(descs, plan, count, prm, ctl, level_chunks, fs_level_pass, bs_level_pass, ws_bytes,
-                             stream);
+                             split, stream);
   else
     enqueue_wide_step<double>(descs, plan, count, prm, ctl, level_chunks, fs_level_pass, bs_level_pass, ws_bytes,
-                              stream);
+                              split, stream);
   return hipGetLastError();
 }
 
