@@ -171,8 +171,8 @@ struct DevicePlanHost {
   // one by one, in the order the reference's column sweeps apply them (forward: columns ascending; back: columns
   // descending, then the division by the pivot). A product rounds the same whoever forms it, so every entry is the
   // reference's value, and a long row no longer costs one step per entry. A pass whose products exceed the scratch
-  // area runs in segments, each lane's running value kept across them. Records (4 ints per lane and step):
-  // Records are 2 ints per lane and step, the second holding a 24-bit field and the step's kind and flags above it:
+  // area runs in segments, each lane's running value kept across them. Records are 2 ints per lane and step, the
+  // second holding a 24-bit field and the step's kind and flags above it:
   //   PRODUCT  (col, value index | kStepProduct << 24): scratch[s + lane] = vals[value index] * x[col] (col = -1:
   //            idle), s = 64 * (PRODUCT steps since the last CHAIN)
   //   CHAIN    (start | len << 16, row | (kStepChain | kChainBegin? | kChainEnd? | kChainIdle?) << 24): kChainBegin:
