@@ -31,6 +31,18 @@ int solve_scratch_cap() {
   return v ? std::max(0, std::atoi(v)) : 0;
 }
 
+// The Y cells as the Jacobian fill reads them, one 8-byte (row, column) and one 16-byte (g, b) record per cell.
+static void pack_cells(const YBus& y, std::vector<int>& ij, std::vector<double>& gb) {
+  const int nnz = y.rp[y.n];
+  ij.assign(2 * (size_t)nnz, 0);
+  gb.assign(2 * (size_t)nnz, 0.0);
+  for (int i = 0; i < y.n; i++)
+    for (int k = y.rp[i]; k < y.rp[i + 1]; k++) {
+      ij[2 * (size_t)k] = i; ij[2 * (size_t)k + 1] = y.ci[k];
+      gb[2 * (size_t)k] = y.g[k]; gb[2 * (size_t)k + 1] = y.b[k];
+    }
+}
+
 WideSplit wide_split_of(const DevicePlanHost& d) {
   WideSplit sp;
   const int nl = d.n_levels;
@@ -73,16 +85,15 @@ void upload_plan(DevicePlan& dp, const YBus* y, const JacobianPattern& jp, const
   if (y) {
     h.n = y->n;
     h.nnz_y = y->rp[y->n];
-    std::vector<int> y_row(h.nnz_y), aslots(jp.slots.size());
-    for (int i = 0; i < y->n; i++)
-      for (int k = y->rp[i]; k < y->rp[i + 1]; k++) y_row[k] = i;
+    std::vector<int> y_ij, aslots(jp.slots.size());
+    std::vector<double> y_gb;
+    pack_cells(*y, y_ij, y_gb);
     for (size_t e = 0; e < jp.slots.size(); e++) aslots[e] = jp.slots[e] >= 0 ? j2a[jp.slots[e]] : -1;
-    upload(dp.y_rp, y->rp, s); upload(dp.y_ci, y->ci, s); upload(dp.y_row, y_row, s);
-    upload(dp.y_diag, y->diag, s); upload(dp.y_g, y->g, s); upload(dp.y_b, y->b, s);
+    upload(dp.y_rp, y->rp, s); upload(dp.y_ij, y_ij, s); upload(dp.y_diag, y->diag, s); upload(dp.y_gb, y_gb, s);
     upload(dp.slots, jp.slots, s); upload(dp.aslots, aslots, s);
     upload(dp.ppos, jp.ppos, s); upload(dp.qpos, jp.qpos, s);
-    h.y_rp = dp.y_rp.as<int>(); h.y_ci = dp.y_ci.as<int>(); h.y_row = dp.y_row.as<int>();
-    h.y_diag = dp.y_diag.as<int>(); h.y_g = dp.y_g.as<double>(); h.y_b = dp.y_b.as<double>();
+    h.y_rp = dp.y_rp.as<int>(); h.y_ij = dp.y_ij.as<int2>(); h.y_diag = dp.y_diag.as<int>();
+    h.y_gb = dp.y_gb.as<double2>();
     h.slots = dp.slots.as<int4>(); h.aslots = dp.aslots.as<int4>();
     h.ppos = dp.ppos.as<int>(); h.qpos = dp.qpos.as<int>();
   }
@@ -590,18 +601,18 @@ void upload_pattern(DevicePlan& dp, const YBus& y, const JacobianPattern& jp, do
   h.nnz_y = y.rp[y.n];
   h.nnz_j = jp.rp[jp.m];
   h.base_mva = base;
-  std::vector<int> y_row(h.nnz_y), aslots(jp.slots.size(), -1), jdiag(std::max(1, jp.m), -1);
-  for (int i = 0; i < y.n; i++)
-    for (int k = y.rp[i]; k < y.rp[i + 1]; k++) y_row[k] = i;
+  std::vector<int> y_ij, aslots(jp.slots.size(), -1), jdiag(std::max(1, jp.m), -1);
+  std::vector<double> y_gb;
+  pack_cells(y, y_ij, y_gb);
   for (int r = 0; r < jp.m; r++)
     for (int k = jp.rp[r]; k < jp.rp[r + 1]; k++)
       if (jp.ci[k] == r) jdiag[r] = k;
-  upload(dp.y_rp, y.rp, s); upload(dp.y_ci, y.ci, s); upload(dp.y_row, y_row, s); upload(dp.y_diag, y.diag, s);
-  upload(dp.y_g, y.g, s); upload(dp.y_b, y.b, s); upload(dp.slots, jp.slots, s); upload(dp.aslots, aslots, s);
+  upload(dp.y_rp, y.rp, s); upload(dp.y_ij, y_ij, s); upload(dp.y_diag, y.diag, s); upload(dp.y_gb, y_gb, s);
+  upload(dp.slots, jp.slots, s); upload(dp.aslots, aslots, s);
   upload(dp.ppos, jp.ppos, s); upload(dp.qpos, jp.qpos, s); upload(dp.j_rp, jp.rp, s); upload(dp.j_ci, jp.ci, s);
   upload(dp.j_diag, jdiag, s);
-  h.y_rp = dp.y_rp.as<int>(); h.y_ci = dp.y_ci.as<int>(); h.y_row = dp.y_row.as<int>(); h.y_diag = dp.y_diag.as<int>();
-  h.y_g = dp.y_g.as<double>(); h.y_b = dp.y_b.as<double>();
+  h.y_rp = dp.y_rp.as<int>(); h.y_ij = dp.y_ij.as<int2>(); h.y_diag = dp.y_diag.as<int>();
+  h.y_gb = dp.y_gb.as<double2>();
   h.slots = dp.slots.as<int4>(); h.aslots = dp.aslots.as<int4>();
   h.ppos = dp.ppos.as<int>(); h.qpos = dp.qpos.as<int>(); h.j_rp = dp.j_rp.as<int>(); h.j_ci = dp.j_ci.as<int>();
   h.j_diag = dp.j_diag.as<int>();

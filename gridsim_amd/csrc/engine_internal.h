@@ -106,7 +106,7 @@ struct WideGraph {
 
 // A plan and its network pattern uploaded to the device.
 struct DevicePlan {
-  DevMem y_rp, y_ci, y_row, y_diag, y_g, y_b, slots, aslots, ppos, qpos, j_rp, j_ci, vm0, va0;
+  DevMem y_rp, y_ij, y_diag, y_gb, slots, aslots, ppos, qpos, j_rp, j_ci, vm0, va0;
   DevMem pinv, q, prow, piv_pos;
   DevMem rs_rec, ch_a_src;
   DevMem fs_rec, bs_rec;

@@ -12,8 +12,9 @@ namespace gs {
 struct DevPlan {
   int32_t n, m, nnz_y, nnz_j, nnz_l, nnz_u, max_col, n_levels;
   // network pattern
-  const int32_t *y_rp, *y_ci, *y_row, *y_diag;
-  const double *y_g, *y_b;            // base Y values (instances patch at most kMaxPatch cells)
+  const int32_t *y_rp, *y_diag;
+  const int2* y_ij;                   // per Y cell: (row, column)
+  const double2* y_gb;                // per Y cell: base value (g, b) (instances patch at most kMaxPatch cells)
   const int4* slots;                  // per Y cell: J CSR position of its H, N, M, L entries, or -1
   const int4* aslots;                 // per Y cell: refactor record (aval index) of the same four entries, or -1
   const int32_t *ppos, *qpos;         // equation numbering

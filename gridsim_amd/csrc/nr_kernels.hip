@@ -111,25 +111,24 @@ __device__ __forceinline__ void y_value(const DevPlan& P, const Inst<T>& I, int 
     if (I.pidx[t] == k) { *g = I.patch[t].g; *b = I.patch[t].b; }
 }
 
-template <typename T>
-__device__ __forceinline__ void put_j(const Inst<T>& I, int s, int a, double v, bool keep_jv) {
-  if (keep_jv && s >= 0) I.jv[s] = v;
+template <typename T, bool KEEP>
+__device__ __forceinline__ void put_j(const Inst<T>& I, int s, int a, double v) {
+  if (KEEP && s >= 0) I.jv[s] = v;
   if (a >= 0) I.vals[a] = (T)v;
 }
 
 // ------------------------------------------------------------------------------------------------ mismatch + J
 // Each Jacobian value goes, in the LU precision, to its refactor record (vals) and, when the solves are refined
-// (keep_jv), to the FP64 CSR copy the refinement residual reads. Both passes walk blocks of 64 cells (buses): blocks
+// (KEEP), to the FP64 CSR copy the refinement residual reads (without it, the CSR positions are not even loaded).
+// Every Y cell's pattern and base value are two loads: (row, column) and (g, b). Both passes walk blocks of 64 cells (buses): blocks
 // blk0, blk0 + stride, ... of the wave (one wave: 0, 1; a level-parallel grid: one block per work-group).
 
 // Every Y cell's two injection terms and its off-diagonal Jacobian entries, one cell per lane. Returns whether a
 // Jacobian value of this lane was not finite.
-template <typename T>
-__device__ __forceinline__ int mismatch_cells(const DevPlan& P, const Inst<T>& I, int lane, bool keep_jv, int blk0, int stride) {
-  GLOBAL const int* y_row = glob(P.y_row);
-  GLOBAL const int* y_ci = glob(P.y_ci);
-  GLOBAL const double* y_g = glob(P.y_g);
-  GLOBAL const double* y_b = glob(P.y_b);
+template <typename T, bool KEEP>
+__device__ __forceinline__ int mismatch_cells(const DevPlan& P, const Inst<T>& I, int lane, int blk0, int stride) {
+  GLOBAL const v2i* y_ij = globv<v2i>(P.y_ij);
+  GLOBAL const v2d* y_gb = globv<v2d>(P.y_gb);
   GLOBAL const v4i* slots = globv<v4i>(P.slots);
   GLOBAL const v4i* aslots = globv<v4i>(P.aslots);
   const int nnz = P.nnz_y;
@@ -144,7 +143,10 @@ __device__ __forceinline__ int mismatch_cells(const DevPlan& P, const Inst<T>& I
   auto load_pattern = [&](int t, int& i, int& j, double& g, double& b, v4i& s, v4i& a) {
     const int k = lane + (blk0 + t * stride) * kWave;
     const int kk = k < nnz ? k : 0;
-    i = y_row[kk]; j = y_ci[kk]; g = y_g[kk]; b = y_b[kk]; s = slots[kk]; a = aslots[kk];
+    const v2i ij = y_ij[kk];
+    const v2d gb = y_gb[kk];
+    i = ij.x; j = ij.y; g = gb.x; b = gb.y; a = aslots[kk];
+    if (KEEP) s = slots[kk];
   };
   // (cells past the end read cell 0: every load is in bounds and unconditional, so the waits stay partial)
   load_pattern(0, i1, j1, g1, b1, s1, a1);
@@ -173,10 +175,10 @@ __device__ __forceinline__ int mismatch_cells(const DevPlan& P, const Inst<T>& I
         const double nn = vmi * (g * cs + b * sn);
         const double mm = -vmi * vmj * (g * cs + b * sn);
         const double l = vmi * (g * sn - b * cs);
-        put_j(I, s.x, a.x, h, keep_jv);
-        put_j(I, s.y, a.y, nn, keep_jv);
-        put_j(I, s.z, a.z, mm, keep_jv);
-        put_j(I, s.w, a.w, l, keep_jv);
+        put_j<T, KEEP>(I, s.x, a.x, h);
+        put_j<T, KEEP>(I, s.y, a.y, nn);
+        put_j<T, KEEP>(I, s.z, a.z, mm);
+        put_j<T, KEEP>(I, s.w, a.w, l);
         lbad |= !(isfinite(h) && isfinite(nn) && isfinite(mm) && isfinite(l));
       }
     }
@@ -186,11 +188,10 @@ __device__ __forceinline__ int mismatch_cells(const DevPlan& P, const Inst<T>& I
 
 // Per bus the ordered sums of its row's terms, the diagonal Jacobian entries and the mismatch; the next bus's
 // scalars are loaded ahead. Returns this lane's max |f| (NaN-propagating); *bad: a Jacobian value was not finite.
-template <typename T>
-__device__ __forceinline__ double mismatch_buses(const DevPlan& P, const Inst<T>& I, int lane, bool keep_jv, int blk0, int stride,
-                                 int* bad) {
-  GLOBAL const double* y_g = glob(P.y_g);
-  GLOBAL const double* y_b = glob(P.y_b);
+template <typename T, bool KEEP>
+__device__ __forceinline__ double mismatch_buses(const DevPlan& P, const Inst<T>& I, int lane, int blk0, int stride,
+                                                 int* bad) {
+  GLOBAL const v2d* y_gb = globv<v2d>(P.y_gb);
   GLOBAL const v4i* slots = globv<v4i>(P.slots);
   GLOBAL const v4i* aslots = globv<v4i>(P.aslots);
   GLOBAL const int* y_rp = glob(P.y_rp);
@@ -213,8 +214,9 @@ __device__ __forceinline__ double mismatch_buses(const DevPlan& P, const Inst<T>
     const int k0 = nk0, k1 = nk1, kd = nkd, pp = npp, qq = nqq;
     const double vmi = nvm;
     load_bus(i + step);
-    const double gd0 = y_g[kd], bd0 = y_b[kd];
-    const v4i s = slots[kd], a = aslots[kd];
+    const v2d gbd = y_gb[kd];
+    const double gd0 = gbd.x, bd0 = gbd.y;
+    const v4i s = KEEP ? slots[kd] : (v4i)(-1), a = aslots[kd];
     const double ps = I.pspec[i], qs = I.qspec[i];
     double pi = 0.0, qi = 0.0;
     int k = k0;
@@ -233,10 +235,10 @@ __device__ __forceinline__ double mismatch_buses(const DevPlan& P, const Inst<T>
     const double nn = P_i / vmi + g * vmi;
     const double mm = P_i - g * vmi * vmi;
     const double l = Q_i / vmi - b * vmi;
-    put_j(I, s.x, a.x, h, keep_jv);
-    put_j(I, s.y, a.y, nn, keep_jv);
-    put_j(I, s.z, a.z, mm, keep_jv);
-    put_j(I, s.w, a.w, l, keep_jv);
+    put_j<T, KEEP>(I, s.x, a.x, h);
+    put_j<T, KEEP>(I, s.y, a.y, nn);
+    put_j<T, KEEP>(I, s.z, a.z, mm);
+    put_j<T, KEEP>(I, s.w, a.w, l);
     lbad |= !(isfinite(h) && isfinite(nn) && isfinite(mm) && isfinite(l));
     if (pp >= 0) { const double fv = ps - P_i; I.f[pp] = fv; lmax = nan_max(lmax, fabs(fv)); }
     if (qq >= 0) { const double fv = qs - Q_i; I.f[qq] = fv; lmax = nan_max(lmax, fabs(fv)); }
@@ -247,12 +249,12 @@ __device__ __forceinline__ double mismatch_buses(const DevPlan& P, const Inst<T>
 
 // The whole pass for one wave. Returns max |f| (NaN-propagating) over the wave; sets *bad when a Jacobian value is
 // not finite.
-template <typename T>
-__device__ __forceinline__ double mismatch_jacobian(const DevPlan& P, const Inst<T>& I, int lane, bool keep_jv, int* bad) {
-  const int cbad = mismatch_cells<T>(P, I, lane, keep_jv, 0, 1);
+template <typename T, bool KEEP>
+__device__ __forceinline__ double mismatch_jacobian(const DevPlan& P, const Inst<T>& I, int lane, int* bad) {
+  const int cbad = mismatch_cells<T, KEEP>(P, I, lane, 0, 1);
   WG_SYNC();
   int bbad;
-  const double lmax = mismatch_buses<T>(P, I, lane, keep_jv, 0, 1, &bbad);
+  const double lmax = mismatch_buses<T, KEEP>(P, I, lane, 0, 1, &bbad);
   *bad = wave_any(cbad | bbad);
   return wave_max(lmax);
 }
@@ -654,7 +656,8 @@ __global__ __launch_bounds__(kWave * (PAIR ? kPairWaves : 1), 2) void nr_batch_k
   };
   for (;;) {
     int bad;
-    max_mis = mismatch_jacobian<T>(P, I, opaque_lane(lane), prm.refine_steps > 0, &bad);
+    max_mis = prm.refine_steps > 0 ? mismatch_jacobian<T, true>(P, I, opaque_lane(lane), &bad)
+                                   : mismatch_jacobian<T, false>(P, I, opaque_lane(lane), &bad);
     WG_SYNC();
     tick(t_mis);
     if (iter == 0) start_mis = max_mis;
@@ -804,7 +807,8 @@ __global__ __launch_bounds__(kWave) void wide_cells(const InstDesc* __restrict__
   InstState* st = descs[blockIdx.y].state;
   if (st->done) return;
   const Inst<T> I = load_inst<T>(descs, blockIdx.y);
-  const int bad = mismatch_cells<T>(P, I, threadIdx.x, prm.refine_steps > 0, blockIdx.x, gridDim.x);
+  const int bad = prm.refine_steps > 0 ? mismatch_cells<T, true>(P, I, threadIdx.x, blockIdx.x, gridDim.x)
+                                       : mismatch_cells<T, false>(P, I, threadIdx.x, blockIdx.x, gridDim.x);
   if (wave_any(bad) && threadIdx.x == 0) atomicOr(&st->jbad, 1);
 }
 
@@ -815,8 +819,9 @@ __global__ __launch_bounds__(kWave) void wide_buses(const InstDesc* __restrict__
   if (st->done) return;
   const Inst<T> I = load_inst<T>(descs, blockIdx.y);
   int bad;
-  const double lmax = wave_max(mismatch_buses<T>(P, I, threadIdx.x, prm.refine_steps > 0, blockIdx.x, gridDim.x,
-                                                 &bad));
+  const double lmax = wave_max(prm.refine_steps > 0
+                                   ? mismatch_buses<T, true>(P, I, threadIdx.x, blockIdx.x, gridDim.x, &bad)
+                                   : mismatch_buses<T, false>(P, I, threadIdx.x, blockIdx.x, gridDim.x, &bad));
   const int anybad = wave_any(bad);
   if (threadIdx.x == 0) {
     atomic_max_bits(&st->mis_bits, lmax);
@@ -1098,7 +1103,7 @@ __global__ __launch_bounds__(kWave) void iterative_step_kernel(const InstDesc* _
     WG_SYNC();
   }
   int bad;
-  const double max_mis = mismatch_jacobian<double>(P, I, opaque_lane(lane), true, &bad);
+  const double max_mis = mismatch_jacobian<double, true>(P, I, opaque_lane(lane), &bad);
   WG_SYNC();
   if (lane != 0) return;
   const int iter = st->iter;
