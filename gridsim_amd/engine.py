@@ -66,6 +66,8 @@ class GsInstance(C.Structure):
                 ("vm0", C.c_void_p), ("va0", C.c_void_p)]
 
 
+INSTANCE_DTYPE = np.dtype([("outage_branch", np.int32), ("reserved", np.int32), ("load_scale", np.float64),
+                           ("vm0", np.uint64), ("va0", np.uint64)])   # the layout of GsInstance
 RESULT_DTYPE = np.dtype([("status", np.int32), ("iterations", np.int32), ("max_mismatch", np.float64),
                          ("symbolic_refreshes", np.int32), ("step_failures", np.int32),
                          ("min_pivot_ratio", np.float64), ("phase_us", np.float64, (4,))])
@@ -396,16 +398,16 @@ class DeviceNetwork:
     def _instances(self, outages=None, load_scale=None, count=None, starts=None):
         count = count if count is not None else (len(outages) if outages is not None else len(load_scale))
         arr = (GsInstance * max(count, 1))()
+        view = np.frombuffer(arr, INSTANCE_DTYPE)[:count]    # filled column by column, not record by record
+        view["outage_branch"] = -1 if outages is None else np.asarray(outages, np.int64)[:count]
+        view["load_scale"] = 1.0 if load_scale is None else np.asarray(load_scale, np.float64)[:count]
         keep = []
-        for i in range(count):
-            o = int(outages[i]) if outages is not None else -1
-            s = float(load_scale[i]) if load_scale is not None else 1.0
-            vm0 = va0 = None
-            if starts is not None:
-                vm0 = np.ascontiguousarray(starts[0][i], np.float64)
-                va0 = np.ascontiguousarray(starts[1][i], np.float64)
-                keep += [vm0, va0]
-            arr[i] = GsInstance(o, 0, s, _p(vm0), _p(va0))
+        if starts is not None and count > 0:
+            vm0 = np.ascontiguousarray(np.asarray(starts[0], np.float64)[:count].reshape(count, -1))
+            va0 = np.ascontiguousarray(np.asarray(starts[1], np.float64)[:count].reshape(count, -1))
+            keep += [vm0, va0]
+            view["vm0"] = vm0.ctypes.data + vm0.strides[0] * np.arange(count, dtype=np.uint64)
+            view["va0"] = va0.ctypes.data + va0.strides[0] * np.arange(count, dtype=np.uint64)
         return arr, count, keep
 
     def solve(self, outages=None, load_scale=None, options: NrOptions | None = None, keep_state=True,

@@ -164,3 +164,30 @@ def test_plan_of_warm_started_outage_grid(eng, oracle):
     rp, ci, lu = _oracle_plan(oracle, warm, False)
     np.testing.assert_array_equal(plan.pivot_row, lu.pivot_row)
     np.testing.assert_array_equal(plan.li, lu.li)
+
+
+def test_instance_records_filled_as_columns(eng):
+    """The gs_instance array the wrapper hands to the library: outage, load scale and start pointers per record, with
+    the numpy view's field layout equal to the C struct's."""
+    assert eng.INSTANCE_DTYPE.itemsize == C.sizeof(eng.GsInstance)
+    for name, _ in eng.GsInstance._fields_:
+        assert eng.INSTANCE_DTYPE.fields[name][1] == getattr(eng.GsInstance, name).offset, name
+
+    class Unbound(eng.DeviceNetwork):
+        def __init__(self):
+            pass
+
+    dn = Unbound()
+    arr, count, keep = dn._instances(outages=np.arange(1995))
+    assert count == 1995 and arr[1994].outage_branch == 1994 and arr[7].load_scale == 1.0 and arr[7].vm0 is None
+    rng = np.random.default_rng(5)
+    vm, va, ls = rng.random((7, 11)), rng.random((7, 11)), np.linspace(0.8, 1.2, 7)
+    arr, count, keep = dn._instances(load_scale=ls, starts=(vm, va))
+    for i in range(count):
+        assert arr[i].outage_branch == -1 and arr[i].load_scale == ls[i]
+        assert np.array_equal(np.ctypeslib.as_array((C.c_double * 11).from_address(arr[i].vm0)), vm[i])
+        assert np.array_equal(np.ctypeslib.as_array((C.c_double * 11).from_address(arr[i].va0)), va[i])
+    arr, count, keep = dn._instances(outages=[3, 4], starts=([vm[0], vm[1]], [va[0], va[1]]))
+    assert arr[1].outage_branch == 4
+    assert np.array_equal(np.ctypeslib.as_array((C.c_double * 11).from_address(arr[1].va0)), va[1])
+    assert dn._instances(outages=[], starts=([], []))[1] == 0
