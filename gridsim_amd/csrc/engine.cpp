@@ -468,6 +468,20 @@ void launch_prepared(const DevPlan& h, const PreparedRun& pr, int schedule, hipS
           "nr kernel");
 }
 
+// The plan's split point, with the fork's side stream and events (created on first use) when the plan has a trunk.
+WideSplit forked_split(const DevicePlanHost& d, WideGraph& wg) {
+  WideSplit split = wide_split_of(d);
+  if (split.trunk_level < d.n_levels && split.fs_level > 0) {
+    if (!wg.side) check(hipStreamCreateWithFlags(&wg.side, hipStreamNonBlocking), "side stream");
+    if (!wg.fork) check(hipEventCreateWithFlags(&wg.fork, hipEventDisableTiming), "fork event");
+    if (!wg.join) check(hipEventCreateWithFlags(&wg.join, hipEventDisableTiming), "join event");
+    split.side = wg.side;
+    split.fork = wg.fork;
+    split.join = wg.join;
+  }
+  return split;
+}
+
 // The level-parallel schedule: the start kernel, then replays of the captured Newton step until every instance has
 // finished (or paused), polling the finished count after every second replay (a replay after the end still
 // dispatches every kernel of the step); at most max_iterations + 1 replays (the last evaluates the mismatch of the
@@ -490,15 +504,7 @@ int launch_wide(DevicePlan& dp, const PreparedRun& pr, hipStream_t s) {
     if (wg.graph) { (void)hipGraphDestroy(wg.graph); wg.graph = nullptr; }
     const DevicePlanHost& d = dp.dph;
     const size_t ws_bytes = (pr.run.fp32 ? 4 : 8) * (size_t)ws_slots(dp.h);
-    WideSplit split = wide_split_of(d);
-    if (split.trunk_level < d.n_levels && split.fs_level > 0) {
-      if (!wg.side) check(hipStreamCreateWithFlags(&wg.side, hipStreamNonBlocking), "side stream");
-      if (!wg.fork) check(hipEventCreateWithFlags(&wg.fork, hipEventDisableTiming), "fork event");
-      if (!wg.join) check(hipEventCreateWithFlags(&wg.join, hipEventDisableTiming), "join event");
-      split.side = wg.side;
-      split.fork = wg.fork;
-      split.join = wg.join;
-    }
+    const WideSplit split = forked_split(d, wg);
     check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "capture");
     hipError_t e = launch_wide_step(pr.run.fp32, pr.desc, dp.h, pr.count, p, ctl, d.rs_level_chunk, d.fs_level_pass,
                                     d.bs_level_pass, ws_bytes, split, s);
@@ -776,13 +782,16 @@ void collect_results(gs_net* net, int count, const gs_instance* inst, const gs_n
                      std::vector<HostFinal>* finals = nullptr) {
   gs_ctx* ctx = net->ctx;
   const int n = net->net.n;
-  for (int i = 0; i < count; i++) {
-    if (states[i].done == 1) {
-      fill_result(states[i], results[i], 0, ctx->us_per_tick);
-      copy_state_out(ctx, inst_buf, L, i, n, vm_out ? vm_out + (size_t)n * i : nullptr,
-                     va_out ? va_out + (size_t)n * i : nullptr);
-    }
-  }
+  for (int i = 0; i < count; i++)
+    if (states[i].done == 1) fill_result(states[i], results[i], 0, ctx->us_per_tick);
+  // every instance's voltages in one strided copy per array (the rows of paused instances are replaced below)
+  const char* b = inst_buf.as<char>();
+  if (vm_out && count > 0)
+    check(hipMemcpy2DAsync(vm_out, 8 * (size_t)n, b + L.vm, L.stride, 8 * (size_t)n, (size_t)count,
+                           hipMemcpyDeviceToHost, ctx->stream), "vm out");
+  if (va_out && count > 0)
+    check(hipMemcpy2DAsync(va_out, 8 * (size_t)n, b + L.va, L.stride, 8 * (size_t)n, (size_t)count,
+                           hipMemcpyDeviceToHost, ctx->stream), "va out");
   check(hipStreamSynchronize(ctx->stream), "sync out");
   for (int i = 0; i < count; i++) {
     if (states[i].done == 1) continue;
@@ -1114,9 +1123,10 @@ struct gs_batch {
     PreparedRun pr;
     hipStream_t stream = nullptr;
     hipEvent_t joined = nullptr;
+    std::unique_ptr<WideGraph> wide;     // the level-parallel schedule's control block, side stream and events
     std::vector<InstState> states;
     std::vector<gs_nr_result> results;
-    std::vector<double> vm, va;
+    std::vector<HostFinal> finals;      // instances the host finished (fresh symbolic passes): their final state
   };
   std::vector<std::unique_ptr<Group>> groups;
   bool use_graph = false;
@@ -1138,13 +1148,35 @@ struct gs_batch {
 
 namespace {
 
+// A group's launches on its stream. The level-parallel schedule cannot poll the host from inside a graph, so its
+// Newton steps are unrolled: the start kernel, then max_iterations + 1 steps (as many as launch_wide replays at most);
+// every kernel of a step returns at once for an instance that has finished or paused, so the steps after the last
+// instance ends cost only their dispatches.
+void enqueue_group(gs_batch::Group& g) {
+  const DevicePlan& dp = g.net->dev;
+  if (g.opts.schedule != GS_SCHEDULE_LEVELS) {
+    launch_prepared(dp.h, g.pr, g.opts.schedule, g.stream);
+    return;
+  }
+  const DevicePlanHost& d = dp.dph;
+  WideGraph& wg = *g.wide;
+  WideCtl* ctl = wg.ctl.as<WideCtl>();
+  const WideSplit split = forked_split(d, wg);
+  const size_t ws_bytes = (g.pr.run.fp32 ? 4 : 8) * (size_t)ws_slots(dp.h);
+  check(launch_wide_start(g.pr.desc, dp.h, g.count, ctl, g.stream), "level-parallel start");
+  for (int k = 0; k <= g.pr.prm.max_iterations; k++)
+    check(launch_wide_step(g.pr.run.fp32, g.pr.desc, dp.h, g.count, g.pr.prm, ctl, d.rs_level_chunk, d.fs_level_pass,
+                           d.bs_level_pass, ws_bytes, split, g.stream),
+          "level-parallel step");
+}
+
 // Every group's launches on its own stream, forked from and joined back into the context's stream.
 void enqueue_batch(gs_batch* b) {
   hipStream_t s0 = b->ctx->stream;
   check(hipEventRecord(b->fork, s0), "fork");
   for (auto& g : b->groups) {
     check(hipStreamWaitEvent(g->stream, b->fork, 0), "fork wait");
-    launch_prepared(g->net->dev.h, g->pr, g->opts.schedule, g->stream);
+    enqueue_group(*g);
     check(hipEventRecord(g->joined, g->stream), "join");
     check(hipStreamWaitEvent(s0, g->joined, 0), "join wait");
   }
@@ -1174,12 +1206,12 @@ gs_status batch_run(gs_batch* b, double* device_ms) {
   for (auto& g : b->groups) {
     const int n = g->net->net.n;
     g->results.assign(g->count, gs_nr_result{});
-    g->vm.assign((size_t)n * g->count, 0.0);
-    g->va.assign((size_t)n * g->count, 0.0);
+    g->finals.clear();
     double kms = 0;
     int launches = 0;
+    // the voltages stay in HBM until gs_batch_results asks for them
     collect_results(g->net, g->count, g->inst.data(), g->opts, false, g->states, g->inst_buf, g->pr.L,
-                    g->results.data(), g->vm.data(), g->va.data(), kms, launches);
+                    g->results.data(), nullptr, nullptr, kms, launches, &g->finals);
   }
   b->ran = true;
   return GS_OK;
@@ -1599,8 +1631,6 @@ GS_API gs_status gs_batch_create(gs_ctx* ctx, int32_t n_groups, gs_net* const* n
       if (counts[gi] <= 0 || !inst[gi]) fail(GS_INVALID_ARGUMENT, "group " + std::to_string(gi) + ": instances");
       gs_nr_options o = normalise(opts ? &opts[gi] : nullptr);
       if ((o.flat_start != 0) != net->flat_start) fail(GS_INVALID_ARGUMENT, "flat_start differs from the upload");
-      if (o.schedule < 0)
-        fail(GS_INVALID_ARGUMENT, "group " + std::to_string(gi) + ": batches launch a wavefront per instance");
       if (o.flags & GS_FLAG_Q_LIMITS)
         fail(GS_INVALID_ARGUMENT, "group " + std::to_string(gi) + ": reactive limits run through gs_nr_solve_batch");
       for (int i = 0; i < counts[gi]; i++)
@@ -1614,6 +1644,10 @@ GS_API gs_status gs_batch_create(gs_ctx* ctx, int32_t n_groups, gs_net* const* n
       g->opts = o;
       check(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking), "stream");
       check(hipEventCreateWithFlags(&g->joined, hipEventDisableTiming), "event");
+      if (o.schedule == GS_SCHEDULE_LEVELS) {
+        g->wide = std::make_unique<WideGraph>();
+        g->wide->ctl.ensure(sizeof(WideCtl));
+      }
       g->pr = prepare_instances(ctx, net->dev, g->inst_buf, g->desc_buf, g->state_buf, g->aux_buf, g->stg, g->count,
                                 g->inst.data(), &net->y, &net->net, net->spec, o, false, nullptr, nullptr, nullptr,
                                 ctx->stream);
@@ -1639,8 +1673,21 @@ GS_API gs_status gs_batch_results(gs_batch* batch, int32_t group, gs_nr_result* 
     if (!batch->ran) fail(GS_INVALID_ARGUMENT, "the batch has not run");
     const auto& g = *batch->groups[group];
     std::copy(g.results.begin(), g.results.end(), results);
-    if (vm_out) std::copy(g.vm.begin(), g.vm.end(), vm_out);
-    if (va_out) std::copy(g.va.begin(), g.va.end(), va_out);
+    // voltages of the last run: the device's rows (the instance blocks are untouched until the next run), then the
+    // rows of the instances the host finished
+    const int n = g.net->net.n;
+    const size_t row = 8 * (size_t)n;
+    const char* b = g.inst_buf.as<char>();
+    hipStream_t s = batch->ctx->stream;
+    if (vm_out) check(hipMemcpy2DAsync(vm_out, row, b + g.pr.L.vm, g.pr.L.stride, row, (size_t)g.count,
+                                       hipMemcpyDeviceToHost, s), "batch vm");
+    if (va_out) check(hipMemcpy2DAsync(va_out, row, b + g.pr.L.va, g.pr.L.stride, row, (size_t)g.count,
+                                       hipMemcpyDeviceToHost, s), "batch va");
+    check(hipStreamSynchronize(s), "batch results");
+    for (const HostFinal& f : g.finals) {
+      if (vm_out) std::copy(f.vm.begin(), f.vm.end(), vm_out + (size_t)n * f.i);
+      if (va_out) std::copy(f.va.begin(), f.va.end(), va_out + (size_t)n * f.i);
+    }
     return GS_OK;
   });
 }
@@ -1682,6 +1729,10 @@ GS_API gs_status gs_nr_autotune(gs_net* net, int32_t count, const gs_instance* s
           }
           cands.push_back({c.lu, c.refine, sched, flags});
         }
+    if (base.schedule == GS_SCHEDULE_LEVELS) {   // asked for: the level-parallel schedule too
+      cands.push_back({GS_LU_FP64, 0, GS_SCHEDULE_LEVELS, 0});
+      cands.push_back({GS_LU_FP32, 1, GS_SCHEDULE_LEVELS, 0});
+    }
     double best = 1e300;
     gs_nr_options chosen = base;
     for (const Candidate& c : cands) {

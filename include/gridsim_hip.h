@@ -255,7 +255,10 @@ GS_API gs_status gs_plan_selfcheck(const gs_plan* plan, double* rel_residual, ch
  * gs_batch_create lays every group's instances out in HBM once (outages, load-scaled specifications, start points);
  * each gs_batch_run launches every group's kernel on its own stream, forked from and joined into the context's
  * stream, or with use_graph replays one captured hipGraph of those launches. opts: one per group (null: defaults).
- * Results of the last run per group: results[count], vm_out / va_out (radians, count * n_bus, may be null). */
+ * A group with schedule GS_SCHEDULE_LEVELS runs the level-parallel schedule with its Newton steps unrolled (the
+ * start kernel and max_iterations + 1 steps; steps after its last instance ends only dispatch their kernels).
+ * Results of the last run per group: results[count], vm_out / va_out (radians, count * n_bus, may be null; the
+ * voltages stay in HBM after a run and are copied out by this call). */
 typedef struct gs_batch gs_batch;
 GS_API gs_status gs_batch_create(gs_ctx* ctx, int32_t n_groups, gs_net* const* nets, const int32_t* counts,
                                  const gs_instance* const* inst, const gs_nr_options* opts, int32_t use_graph,
@@ -270,6 +273,8 @@ GS_API void gs_batch_free(gs_batch* batch);
  * launch or slices of 512 or 256; the solve vector in LDS or in global memory - and writes into *opts the fastest
  * whose verdicts equal the FP64 LU's, whose
  * converged instances meet the tolerance and whose converged voltages lie within 1e-9 (pu, rad) of the FP64 LU's.
+ * With opts->schedule = GS_SCHEDULE_LEVELS on entry the level-parallel schedule (FP64 LU, or FP32 LU with one
+ * refinement step) is a candidate too; otherwise it is not offered, as the serving loop does not run it.
  * *best_ms: that candidate's kernel time on the sample. */
 GS_API gs_status gs_nr_autotune(gs_net* net, int32_t count, const gs_instance* sample, gs_nr_options* opts,
                                 double* best_ms);

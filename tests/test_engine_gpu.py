@@ -275,6 +275,50 @@ def test_batch_of_two_networks_equals_separate_solves(ctx, eng, two_networks, us
     b.close()
 
 
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_batch_with_a_level_parallel_group(ctx, eng, two_networks, use_graph):
+    """A batch whose second group runs the level-parallel schedule (its Newton steps unrolled into the batch's launches
+    or graph) beside a wavefront group: every instance, on two runs, equals the wavefront solve of the same instances
+    bit for bit (verdicts, iteration counts, voltages)."""
+    from gridsim_amd import gridgen
+    rng = np.random.default_rng(13)
+    groups, singles = [], []
+    for k, net in enumerate(two_networks):
+        dn = eng.DeviceNetwork(ctx, net, eng.NrOptions(max_iterations=10))
+        outs = rng.choice(gridgen.config2_outages(net), 12, replace=False)
+        scales = rng.uniform(0.8, 1.2, 12)
+        opts = eng.NrOptions(max_iterations=10, schedule=eng.SCHEDULE_LEVELS if k == 1 else 0)
+        groups.append(eng.Group(dn, outages=outs, load_scale=scales, options=opts))
+        singles.append(dn.solve(outages=outs, load_scale=scales, options=eng.NrOptions(max_iterations=10)))
+    b = eng.Batch(ctx, groups, use_graph=use_graph)
+    for rep in range(2):
+        ms = b.run()
+        print(f"batch with a level-parallel group (graph={use_graph}) run {rep}: {ms:.3f} ms", flush=True)
+        for g, single in enumerate(singles):
+            r = b.results(g, keep_state=True)
+            assert np.array_equal(r.status, single.status)
+            assert np.array_equal(r.results["iterations"], single.results["iterations"])
+            assert np.array_equal(r.vm, single.vm) and np.array_equal(r.va, single.va)
+    b.close()
+
+
+def test_autotune_with_the_level_parallel_schedule_offered(ctx, eng, oracle, two_networks):
+    """Options asking for GS_SCHEDULE_LEVELS let the autotuner weigh the level-parallel schedule too; its choice keeps
+    the oracle's verdicts and voltages."""
+    from gridsim_amd import gridgen
+    net = two_networks[1]
+    outs = gridgen.config2_outages(net)[::40][:8]
+    rres, rvm, rva = oracle.screen(net, outs, oracle.options(max_iterations=10), threads=8)
+    dn = eng.DeviceNetwork(ctx, net, eng.NrOptions(max_iterations=10))
+    chosen, ms = eng.autotune(dn, outages=outs, options=eng.NrOptions(max_iterations=10, schedule=eng.SCHEDULE_LEVELS))
+    print(f"autotune with levels offered: lu {chosen.lu_precision} refine {chosen.refine_steps} "
+          f"schedule {chosen.schedule}, {ms:.3f} ms", flush=True)
+    assert chosen.schedule in (0, 256, 512, eng.SCHEDULE_LEVELS)
+    r = dn.solve(outages=outs, options=chosen)
+    _compare(oracle, r.results, r.vm, r.va, rres, rvm, rva, "300-bus N-1 sample, autotuned with levels offered")
+    dn.close()
+
+
 def test_autotune_choice_keeps_the_oracle_verdicts(ctx, eng, oracle, n1_118):
     warm, outs, (rres, rvm, rva) = n1_118
     dn = eng.DeviceNetwork(ctx, warm, eng.NrOptions(max_iterations=10))
