@@ -302,6 +302,31 @@ def test_batch_with_a_level_parallel_group(ctx, eng, two_networks, use_graph):
     b.close()
 
 
+def test_batch_voltages_of_instances_the_host_finished(ctx, eng, two_networks):
+    """With a growth threshold of 0.9 static pivots fail their check, so instances pause and the host finishes them
+    with fresh symbolic passes; a batch's voltages (read from HBM when asked for, the host-finished rows replaced)
+    equal gs_nr_solve_batch's for the same instances bit for bit."""
+    from gridsim_amd import gridgen
+    net = two_networks[1]
+    rng = np.random.default_rng(17)
+    outs = rng.choice(gridgen.config2_outages(net), 12, replace=False)
+    opts = eng.NrOptions(max_iterations=10, lu_precision=eng.LU_FP64, refine_steps=0, growth_tol=0.9)
+    dn = eng.DeviceNetwork(ctx, net, opts)
+    single = dn.solve(outages=outs, options=opts)
+    refreshed = int((single.results["symbolic_refreshes"] > 0).sum())
+    print(f"growth 0.9: {refreshed} of {len(outs)} instances finished by the host", flush=True)
+    assert refreshed > 0
+    b = eng.Batch(ctx, [eng.Group(dn, outages=outs, options=opts)], use_graph=True)
+    for rep in range(2):
+        b.run()
+        r = b.results(0, keep_state=True)
+        assert np.array_equal(r.status, single.status)
+        assert np.array_equal(r.results["symbolic_refreshes"], single.results["symbolic_refreshes"])
+        assert np.array_equal(r.vm, single.vm) and np.array_equal(r.va, single.va)
+    b.close()
+    dn.close()
+
+
 def test_autotune_with_the_level_parallel_schedule_offered(ctx, eng, oracle, two_networks):
     """Options asking for GS_SCHEDULE_LEVELS let the autotuner weigh the level-parallel schedule too; its choice keeps
     the oracle's verdicts and voltages."""
