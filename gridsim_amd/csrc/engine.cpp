@@ -66,12 +66,14 @@ WideSplit wide_split_of(const DevicePlanHost& d) {
   return sp;
 }
 
-void upload_plan(DevicePlan& dp, const YBus* y, const JacobianPattern& jp, const BusSpec* spec, const Plan& plan,
-                 double base_mva, hipStream_t s) {
-  compile_plan(plan, dp.dph, solve_scratch_cap());
-  const DevicePlanHost& d = dp.dph;
-  DevPlan& h = dp.h;
+// The host image of a plan: the network pattern (when y is given), the start point (when spec is given), the Jacobian
+// pattern and the compiled plan's streams, padded as the kernels expect; h's scalar fields are set, its pointer fields
+// are recorded in the image for bind().
+void build_plan_image(PlanImage& img, DevPlan& h, DevicePlanHost& d, const YBus* y, const JacobianPattern& jp,
+                      const BusSpec* spec, const Plan& plan, double base_mva) {
+  compile_plan(plan, d, solve_scratch_cap());
   h = DevPlan{};
+  img = PlanImage{};
   h.m = jp.m;
   h.nnz_j = jp.rp[jp.m];
   h.nnz_l = d.nnz_l;
@@ -89,22 +91,14 @@ void upload_plan(DevicePlan& dp, const YBus* y, const JacobianPattern& jp, const
     std::vector<double> y_gb;
     pack_cells(*y, y_ij, y_gb);
     for (size_t e = 0; e < jp.slots.size(); e++) aslots[e] = jp.slots[e] >= 0 ? j2a[jp.slots[e]] : -1;
-    upload(dp.y_rp, y->rp, s); upload(dp.y_ij, y_ij, s); upload(dp.y_diag, y->diag, s); upload(dp.y_gb, y_gb, s);
-    upload(dp.slots, jp.slots, s); upload(dp.aslots, aslots, s);
-    upload(dp.ppos, jp.ppos, s); upload(dp.qpos, jp.qpos, s);
-    h.y_rp = dp.y_rp.as<int>(); h.y_ij = dp.y_ij.as<int2>(); h.y_diag = dp.y_diag.as<int>();
-    h.y_gb = dp.y_gb.as<double2>();
-    h.slots = dp.slots.as<int4>(); h.aslots = dp.aslots.as<int4>();
-    h.ppos = dp.ppos.as<int>(); h.qpos = dp.qpos.as<int>();
+    img.put(&DevPlan::y_rp, y->rp); img.put(&DevPlan::y_ij, y_ij); img.put(&DevPlan::y_diag, y->diag);
+    img.put(&DevPlan::y_gb, y_gb); img.put(&DevPlan::slots, jp.slots); img.put(&DevPlan::aslots, aslots);
+    img.put(&DevPlan::ppos, jp.ppos); img.put(&DevPlan::qpos, jp.qpos);
   }
-  if (spec) {
-    upload(dp.vm0, spec->vm, s); upload(dp.va0, spec->va, s);
-    h.vm0 = dp.vm0.as<double>(); h.va0 = dp.va0.as<double>();
-  }
-  upload(dp.j_rp, jp.rp, s); upload(dp.j_ci, jp.ci, s);
-  h.j_rp = dp.j_rp.as<int>(); h.j_ci = dp.j_ci.as<int>();
-  upload(dp.pinv, d.pinv, s); upload(dp.q, d.q, s); upload(dp.prow, d.prow, s); upload(dp.piv_pos, d.piv_pos, s);
-  h.pinv = dp.pinv.as<int>(); h.q = dp.q.as<int>(); h.prow = dp.prow.as<int>(); h.piv_pos = dp.piv_pos.as<int>();
+  if (spec) { img.put(&DevPlan::vm0, spec->vm); img.put(&DevPlan::va0, spec->va); }
+  img.put(&DevPlan::j_rp, jp.rp); img.put(&DevPlan::j_ci, jp.ci);
+  img.put(&DevPlan::pinv, d.pinv); img.put(&DevPlan::q, d.q); img.put(&DevPlan::prow, d.prow);
+  img.put(&DevPlan::piv_pos, d.piv_pos);
   // the refactor's row stream, rounded up to whole rings of no-op rows plus one ring of look-ahead padding
   auto ring_up = [](int v) { return (v + kStreamPad - 1) / kStreamPad * kStreamPad; };
   h.wstride = d.wstride;
@@ -119,8 +113,7 @@ void upload_plan(DevicePlan& dp, const YBus* y, const JacobianPattern& jp, const
     for (int l = 0; l < 64; l++)
       for (int c = 0; c < 2; c++)
         rs_rec[(((size_t)(r / 2) * 64 + l) * 2 + (r % 2)) * 2 + c] = d.rs_rec[((size_t)r * 64 + l) * 2 + c];
-  upload(dp.rs_rec, rs_rec, s); upload(dp.ch_a_src, d.ch_a_src, s);
-  h.rs_rec = dp.rs_rec.as<int4>(); h.ch_a_src = dp.ch_a_src.as<int>();
+  img.put(&DevPlan::rs_rec, rs_rec); img.put(&DevPlan::ch_a_src, d.ch_a_src);
   // the solves' step streams (padded with steps that touch nothing) and the factors' stream-order sizes
   h.fs_steps = ring_up(d.fs_steps);
   h.bs_steps = ring_up(d.bs_steps);
@@ -132,26 +125,35 @@ void upload_plan(DevicePlan& dp, const YBus* y, const JacobianPattern& jp, const
       for (int l = 0; l < 64; l++) { prec.push_back(-1); prec.push_back(0); }
     return prec;
   };
-  std::vector<int> fs_rec = pad_steps(d.fs_rec, h.fs_steps), bs_rec = pad_steps(d.bs_rec, h.bs_steps);
-  upload(dp.fs_rec, fs_rec, s); upload(dp.bs_rec, bs_rec, s);
-  h.fs_rec = dp.fs_rec.as<int2>(); h.bs_rec = dp.bs_rec.as<int2>();
+  img.put(&DevPlan::fs_rec, pad_steps(d.fs_rec, h.fs_steps));
+  img.put(&DevPlan::bs_rec, pad_steps(d.bs_rec, h.bs_steps));
   // chunk and pass boundaries for level-parallel schedules
-  upload(dp.rs_chunk_row, d.rs_chunk_row, s);
-  upload(dp.fs_pass_step, d.fs_pass_step, s);
-  upload(dp.bs_pass_step, d.bs_pass_step, s);
-  h.rs_chunk_row = dp.rs_chunk_row.as<int>();
-  h.fs_pass_step = dp.fs_pass_step.as<int>();
-  h.bs_pass_step = dp.bs_pass_step.as<int>();
-  // the staging vectors above are locals: finish the copies before they go
-  check(hipStreamSynchronize(s), "upload plan");
+  img.put(&DevPlan::rs_chunk_row, d.rs_chunk_row);
+  img.put(&DevPlan::fs_pass_step, d.fs_pass_step);
+  img.put(&DevPlan::bs_pass_step, d.bs_pass_step);
 }
 
-void upload_branches(DevicePlan& dp, const Network& net, std::vector<std::vector<double>>& keep, hipStream_t s) {
+void upload_plan(DevicePlan& dp, const YBus* y, const JacobianPattern& jp, const BusSpec* spec, const Plan& plan,
+                 double base_mva, hipStream_t s) {
+  PlanImage img;
+  DevPlan h;
+  build_plan_image(img, h, dp.dph, y, jp, spec, plan, base_mva);
+  dp.block.ensure(img.bytes.size());
+  check(hipMemcpyAsync(dp.block.p, img.bytes.data(), img.bytes.size(), hipMemcpyHostToDevice, s), "upload plan");
+  // the image is a local: finish the copy before it goes
+  check(hipStreamSynchronize(s), "upload plan");
+  // (the branch fields, set by upload_branches, survive a plan upload)
+  h.nb = dp.h.nb; h.br_ends = dp.h.br_ends; h.br_stamp = dp.h.br_stamp; h.br_rate = dp.h.br_rate;
+  h.bus_lo = dp.h.bus_lo; h.bus_hi = dp.h.bus_hi;
+  img.bind(h, dp.block.as<char>());
+  dp.h = h;
+}
+
+void upload_branches(DevicePlan& dp, const Network& net, hipStream_t s) {
   DevPlan& h = dp.h;
   h.nb = net.nb;
   std::vector<int> ends(2 * (size_t)std::max(1, net.nb), -1);
-  keep.emplace_back(8 * (size_t)std::max(1, net.nb), 0.0);
-  std::vector<double>& stamp = keep.back();
+  std::vector<double> stamp(8 * (size_t)std::max(1, net.nb), 0.0);
   for (int k = 0; k < net.nb; k++) {
     if (!net.br_in[k]) continue;
     ends[2 * k] = net.br_from[k];
@@ -160,21 +162,16 @@ void upload_branches(DevicePlan& dp, const Network& net, std::vector<std::vector
     const Cx v[4] = {st.yff, st.yft, st.ytf, st.ytt};
     for (int c = 0; c < 4; c++) { stamp[8 * (size_t)k + 2 * c] = v[c].re; stamp[8 * (size_t)k + 2 * c + 1] = v[c].im; }
   }
-  upload(dp.br_ends, ends, s);
-  // (the ends vector is a local: this copy must finish before it goes)
-  check(hipStreamSynchronize(s), "branch ends");
-  upload(dp.br_stamp, stamp, s);
-  keep.push_back(net.br_rate.empty() ? std::vector<double>(1, 0.0) : net.br_rate);
-  upload(dp.br_rate, keep.back(), s);
-  keep.push_back(net.bus_lo);
-  upload(dp.bus_lo, keep.back(), s);
-  keep.push_back(net.bus_hi);
-  upload(dp.bus_hi, keep.back(), s);
-  h.br_ends = dp.br_ends.as<int2>();
-  h.br_stamp = dp.br_stamp.as<double>();
-  h.br_rate = dp.br_rate.as<double>();
-  h.bus_lo = dp.bus_lo.as<double>();
-  h.bus_hi = dp.bus_hi.as<double>();
+  PlanImage img;
+  img.put(&DevPlan::br_ends, ends);
+  img.put(&DevPlan::br_stamp, stamp);
+  img.put(&DevPlan::br_rate, net.br_rate.empty() ? std::vector<double>(1, 0.0) : net.br_rate);
+  img.put(&DevPlan::bus_lo, net.bus_lo);
+  img.put(&DevPlan::bus_hi, net.bus_hi);
+  dp.branch_block.ensure(img.bytes.size());
+  check(hipMemcpyAsync(dp.branch_block.p, img.bytes.data(), img.bytes.size(), hipMemcpyHostToDevice, s), "branches");
+  check(hipStreamSynchronize(s), "branches");   // the image is a local
+  img.bind(h, dp.branch_block.as<char>());
 }
 
 // The shape of a compiled plan as upload_plan lays it out in HBM (record streams padded the same way).
@@ -272,32 +269,78 @@ Layout instance_layout(const DevPlan& h, bool fp32, bool x_lds) {
   return L;
 }
 
-// Re-sums the cells an outage touches, skipping the outaged branch: bit-identical to the rebuilt Y-bus.
-int outage_patches(const YBus& y, const Network& net, int branch, Patch* out) {
-  if (branch < 0) return 0;
-  int f = net.br_from[branch], t = net.br_to[branch];
-  int cells[4] = {y.find(f, f), y.find(t, t), y.find(f, t), y.find(t, f)};
-  int np = 0;
-  for (int c = 0; c < 4; c++) {
-    int k = cells[c];
-    if (k < 0) continue;
-    bool dup = false;
-    for (int e = 0; e < np; e++) dup |= out[e].idx == k;
-    if (dup) continue;
-    double g = 0.0, b = 0.0;
-    bool any = false;
-    for (int e = y.contrib_ptr[k]; e < y.contrib_ptr[k + 1]; e++) {
-      if (y.contrib_branch[e] == branch) continue;
-      if (!any) { g = 0.0 + y.contrib_g[e]; b = 0.0 + y.contrib_b[e]; any = true; }
-      else { g += y.contrib_g[e]; b += y.contrib_b[e]; }
-    }
-    out[np].idx = k;
-    out[np].pad = 0;
-    out[np].g = any ? g : 0.0;
-    out[np].b = any ? b : 0.0;
-    np++;
+// The outaged branches of an instance (outage_branch, then the list), checked: in range, each once, at most
+// kMaxOutages.
+std::vector<int> outage_set(const gs_instance& in, int n_branch) {
+  std::vector<int> out;
+  if (in.n_outages < 0 || in.n_outages > kMaxOutages || (in.n_outages > 0 && !in.outages))
+    fail(GS_INVALID_ARGUMENT, "n_outages must lie in 0..GS_MAX_OUTAGES, with the list given");
+  if (in.outage_branch < -1 || in.outage_branch >= n_branch) fail(GS_INVALID_ARGUMENT, "outage_branch out of range");
+  if (in.outage_branch >= 0) out.push_back(in.outage_branch);
+  for (int k = 0; k < in.n_outages; k++) {
+    const int b = in.outages[k];
+    if (b < 0 || b >= n_branch) fail(GS_INVALID_ARGUMENT, "outaged branch out of range");
+    if (std::find(out.begin(), out.end(), b) != out.end()) fail(GS_INVALID_ARGUMENT, "a branch is outaged twice");
+    out.push_back(b);
   }
-  return np;
+  if ((int)out.size() > kMaxOutages) fail(GS_INVALID_ARGUMENT, "more than GS_MAX_OUTAGES outaged branches");
+  return out;
+}
+
+// The instances of a call, checked (outage sets, load scale, start point): GS_INVALID_ARGUMENT naming the first bad one.
+void check_instances(int n_branch, int count, const gs_instance* inst, const std::string& where) {
+  for (int i = 0; i < count; i++) {
+    try {
+      (void)outage_set(inst[i], n_branch);
+    } catch (const HipError& e) {
+      fail(GS_INVALID_ARGUMENT, where + "instance " + std::to_string(i) + ": " + e.what);
+    }
+    if (!(inst[i].load_scale > 0) || !std::isfinite(inst[i].load_scale))
+      fail(GS_INVALID_ARGUMENT, where + "instance " + std::to_string(i) + ": load_scale must be positive");
+    if ((inst[i].vm0 == nullptr) != (inst[i].va0 == nullptr))
+      fail(GS_INVALID_ARGUMENT, where + "instance " + std::to_string(i) + ": give both vm0 and va0, or neither");
+  }
+}
+
+// Every Y cell the outaged branches touch, re-summed in insertion order without them: bit-identical to the Y-bus
+// rebuilt without those branches (WithoutMany keeps model order and the builder sums from +0.0).
+std::vector<Patch> outage_patches(const YBus& y, const Network& net, const std::vector<int>& outaged) {
+  std::vector<Patch> out;
+  for (int br : outaged) {
+    const int f = net.br_from[br], t = net.br_to[br];
+    const int cells[4] = {y.find(f, f), y.find(t, t), y.find(f, t), y.find(t, f)};
+    for (int k : cells) {
+      if (k < 0) continue;
+      bool dup = false;
+      for (const Patch& p : out) dup |= p.idx == k;
+      if (dup) continue;
+      double g = 0.0, b = 0.0;
+      bool any = false;
+      for (int e = y.contrib_ptr[k]; e < y.contrib_ptr[k + 1]; e++) {
+        if (std::find(outaged.begin(), outaged.end(), y.contrib_branch[e]) != outaged.end()) continue;
+        if (!any) { g = 0.0 + y.contrib_g[e]; b = 0.0 + y.contrib_b[e]; any = true; }
+        else { g += y.contrib_g[e]; b += y.contrib_b[e]; }
+      }
+      Patch p{};
+      p.idx = k;
+      p.g = any ? g : 0.0;
+      p.b = any ? b : 0.0;
+      out.push_back(p);
+    }
+  }
+  return out;
+}
+
+// Instances of a batch whose outage touches more Y cells than the descriptor patches in place: they get Y value arrays
+// of their own (16 bytes per cell each).
+int own_y_count(const YBus* y, const Network* netw, int count, const gs_instance* inst) {
+  if (!y || !netw) return 0;
+  int c = 0;
+  for (int i = 0; i < count; i++) {
+    const std::vector<int> out = outage_set(inst[i], netw->nb);
+    if (out.size() > 2 && (int)outage_patches(*y, *netw, out).size() > kMaxPatch) c++;
+  }
+  return c;
 }
 
 BatchRun plan_run(const DevPlan& h, const gs_nr_options& o, bool force_fp64) {
@@ -342,6 +385,8 @@ void fill_result(const InstState& s, gs_nr_result& r, int refreshes, double us_p
   r.phase_us[1] = s.t_refactor * us_per_tick;
   r.phase_us[2] = s.t_solve * us_per_tick;
   r.phase_us[3] = s.t_update * us_per_tick;
+  r.engine_stop = s.engine_stop;
+  r.dc_retry = GS_RETRY_NOT_TRIED;
 }
 
 // `resume` (or null: start afresh) are paused states to continue from.
@@ -362,6 +407,8 @@ PreparedRun prepare_into(const DevPlan& h, const RunBuffers& rb, Staging& stg, i
   std::vector<InstDesc>& descs = stg.descs;
   descs.assign(count, InstDesc{});
   stg.stage.clear();
+  stg.own_y.clear();
+  int own_used = 0;
   for (int i = 0; i < count; i++) {
     InstDesc& d = descs[i];
     std::memset(&d, 0, sizeof(d));
@@ -373,8 +420,30 @@ PreparedRun prepare_into(const DevPlan& h, const RunBuffers& rb, Staging& stg, i
     d.vals = b + L.vals; d.fval = b + L.fval; d.bval = b + L.bval; d.dval = b + L.dval;
     d.xg = run.x_lds ? nullptr : b + L.xg;
     const gs_instance& in = inst[i];
-    d.n_patch = (y && netw) ? outage_patches(*y, *netw, in.outage_branch, d.patch) : 0;
-    d.outage = (y && netw) ? in.outage_branch : -1;
+    d.n_patch = 0;
+    d.n_out = 0;
+    d.y_gb_own = nullptr;
+    for (int t = 0; t < kMaxOutages; t++) d.out[t] = -1;
+    if (y && netw) {
+      const std::vector<int> outaged = outage_set(in, netw->nb);
+      d.n_out = (int)outaged.size();
+      std::copy(outaged.begin(), outaged.end(), d.out);
+      const std::vector<Patch> patches = outage_patches(*y, *netw, outaged);
+      if ((int)patches.size() <= kMaxPatch) {
+        d.n_patch = (int)patches.size();
+        std::copy(patches.begin(), patches.end(), d.patch);
+      } else {
+        // more cells than the descriptor holds: the instance's own Y values, the network's with the patches applied
+        if (own_used >= rb.own_y_count) fail(GS_DEVICE_ERROR, "own Y-value arrays under-counted");
+        std::vector<double> gb(2 * (size_t)y->rp[y->n]);
+        for (int k = 0; k < y->rp[y->n]; k++) { gb[2 * (size_t)k] = y->g[k]; gb[2 * (size_t)k + 1] = y->b[k]; }
+        for (const Patch& p : patches) { gb[2 * (size_t)p.idx] = p.g; gb[2 * (size_t)p.idx + 1] = p.b; }
+        double* dst = rb.own_y + (size_t)own_used * 2 * (size_t)y->rp[y->n];
+        d.y_gb_own = dst;
+        stg.own_y.emplace_back(dst, std::move(gb));
+        own_used++;
+      }
+    }
     // specified injections: the network's, or a load-scaled copy
     const std::vector<double>* sp = spec_override ? &(*spec_override)[i] : nullptr;
     std::vector<double> scaled;
@@ -413,6 +482,8 @@ PreparedRun prepare_into(const DevPlan& h, const RunBuffers& rb, Staging& stg, i
       check(hipMemcpyAsync(b + L.start, buf.data() + 2 * h.n, 16 * (size_t)h.n, hipMemcpyHostToDevice, s),
             "stage start");
   }
+  for (auto& [dst, gb] : stg.own_y)
+    check(hipMemcpyAsync(dst, gb.data(), sizeof(double) * gb.size(), hipMemcpyHostToDevice, s), "own Y values");
   check(hipMemcpyAsync(rb.desc, descs.data(), sizeof(InstDesc) * count, hipMemcpyHostToDevice, s), "descs");
   // a fresh start needs no state upload: the kernel's init resets every counter it reads
   if (resume) {
@@ -448,14 +519,20 @@ PreparedRun prepare_instances(gs_ctx* ctx, DevicePlan& dp, DevMem& inst_buf, Dev
   inst_buf.ensure(need);
   desc_buf.ensure(sizeof(InstDesc) * (size_t)count);
   state_buf.ensure(sizeof(InstState) * (size_t)count);
+  // the base specification, then the own Y-value arrays of the instances that need them
+  const int own = own_y_count(y, netw, count, inst);
+  const size_t spec_bytes = (16 * (size_t)h.n + 255) & ~size_t(255);
+  aux_buf.ensure(spec_bytes + 16 * (size_t)h.nnz_y * own);
   stg.aux = base_spec.pspec;
   stg.aux.insert(stg.aux.end(), base_spec.qspec.begin(), base_spec.qspec.end());
-  upload(aux_buf, stg.aux, s);
+  check(hipMemcpyAsync(aux_buf.p, stg.aux.data(), sizeof(double) * stg.aux.size(), hipMemcpyHostToDevice, s), "spec");
   RunBuffers rb;
   rb.inst = inst_buf.as<char>();
   rb.desc = desc_buf.as<InstDesc>();
   rb.state = state_buf.as<InstState>();
   rb.base_spec = aux_buf.as<double>();
+  rb.own_y = reinterpret_cast<double*>(aux_buf.as<char>() + spec_bytes);
+  rb.own_y_count = own;
   return prepare_into(h, rb, stg, count, inst, y, netw, o, plain, resume, spec_override, start_override, s);
 }
 
@@ -600,8 +677,7 @@ void copy_state_out(gs_ctx* ctx, const DevMem& inst_buf, const Layout& L, int i,
 // resumed run keeps its iteration count and start mismatch.
 // The network pattern alone on the device (no LU plan): what the iterative step reads.
 void upload_pattern(DevicePlan& dp, const YBus& y, const JacobianPattern& jp, double base, hipStream_t s) {
-  DevPlan& h = dp.h;
-  h = DevPlan{};
+  DevPlan h{};
   h.n = y.n;
   h.m = jp.m;
   h.nnz_y = y.rp[y.n];
@@ -613,16 +689,16 @@ void upload_pattern(DevicePlan& dp, const YBus& y, const JacobianPattern& jp, do
   for (int r = 0; r < jp.m; r++)
     for (int k = jp.rp[r]; k < jp.rp[r + 1]; k++)
       if (jp.ci[k] == r) jdiag[r] = k;
-  upload(dp.y_rp, y.rp, s); upload(dp.y_ij, y_ij, s); upload(dp.y_diag, y.diag, s); upload(dp.y_gb, y_gb, s);
-  upload(dp.slots, jp.slots, s); upload(dp.aslots, aslots, s);
-  upload(dp.ppos, jp.ppos, s); upload(dp.qpos, jp.qpos, s); upload(dp.j_rp, jp.rp, s); upload(dp.j_ci, jp.ci, s);
-  upload(dp.j_diag, jdiag, s);
-  h.y_rp = dp.y_rp.as<int>(); h.y_ij = dp.y_ij.as<int2>(); h.y_diag = dp.y_diag.as<int>();
-  h.y_gb = dp.y_gb.as<double2>();
-  h.slots = dp.slots.as<int4>(); h.aslots = dp.aslots.as<int4>();
-  h.ppos = dp.ppos.as<int>(); h.qpos = dp.qpos.as<int>(); h.j_rp = dp.j_rp.as<int>(); h.j_ci = dp.j_ci.as<int>();
-  h.j_diag = dp.j_diag.as<int>();
-  check(hipStreamSynchronize(s), "pattern upload");   // the staging vectors are locals
+  PlanImage img;
+  img.put(&DevPlan::y_rp, y.rp); img.put(&DevPlan::y_ij, y_ij); img.put(&DevPlan::y_diag, y.diag);
+  img.put(&DevPlan::y_gb, y_gb); img.put(&DevPlan::slots, jp.slots); img.put(&DevPlan::aslots, aslots);
+  img.put(&DevPlan::ppos, jp.ppos); img.put(&DevPlan::qpos, jp.qpos); img.put(&DevPlan::j_rp, jp.rp);
+  img.put(&DevPlan::j_ci, jp.ci); img.put(&DevPlan::j_diag, jdiag);
+  dp.block.ensure(img.bytes.size());
+  check(hipMemcpyAsync(dp.block.p, img.bytes.data(), img.bytes.size(), hipMemcpyHostToDevice, s), "pattern upload");
+  check(hipStreamSynchronize(s), "pattern upload");   // the image is a local
+  img.bind(h, dp.block.as<char>());
+  dp.h = h;
 }
 
 // One Newton iteration of one instance with SolveStep's ILU(0) + BiCGSTAB fallback on the device (the LU refused the
@@ -672,7 +748,8 @@ void iterative_step(gs_ctx* ctx, const YBus& y, const JacobianPattern& jp, const
   d.vm_start = reinterpret_cast<const double*>(b + o_start);
   d.va_start = d.vm_start + n;
   d.n_patch = 0;
-  d.outage = -1;
+  d.n_out = 0;
+  d.y_gb_own = nullptr;
   d.vm = (double*)(b + o_vm); d.va = (double*)(b + o_va); d.p = (double*)(b + o_p); d.q = (double*)(b + o_q);
   d.f = (double*)(b + o_f); d.dx = (double*)(b + o_dx); d.r = nullptr; d.jv = (double*)(b + o_jv);
   d.tpq = (double*)(b + o_tpq);
@@ -704,7 +781,7 @@ void finish_paused(gs_net* net, const gs_instance& in, const gs_nr_options& o, b
   const int n = net->net.n;
   int refreshes = 0;
   Network own = net->net;
-  if (in.outage_branch >= 0) own.br_in[in.outage_branch] = 0;
+  for (int br : outage_set(in, own.nb)) own.br_in[br] = 0;
   if (in.load_scale != 1.0) {
     for (int b = 0; b < own.n; b++) { own.pd[b] *= in.load_scale; own.qd[b] *= in.load_scale; }
     for (int g = 0; g < own.ng; g++) own.gen_pg[g] *= in.load_scale;
@@ -724,7 +801,12 @@ void finish_paused(gs_net* net, const gs_instance& in, const gs_nr_options& o, b
     if (!symbolic_pass(y, jp, vm.data(), va.data(), plan, why)) {
       // singular at this state: the LU refuses the step, so SolveStep falls back to ILU(0) + BiCGSTAB for it (on the
       // device); the LU is tried again at the next step
-      if (++iterative > o.max_iterations + 1) { st.done = 1; st.step_failures++; break; }
+      if (++iterative > o.max_iterations + 1) {
+        st.done = 1;
+        st.step_failures++;
+        st.engine_stop = GS_STOP_FALLBACK_CAP;
+        break;
+      }
       iterative_step(ctx, y, jp, spec, own.base, o, st, vm, va, p_out, q_out);
       finished = st.done == 1;
       continue;
@@ -733,11 +815,14 @@ void finish_paused(gs_net* net, const gs_instance& in, const gs_nr_options& o, b
       // the fresh plans keep failing their growth checks: the run stops
       st.done = 1;
       st.step_failures++;
+      st.engine_stop = GS_STOP_REFRESH_CAP;
       break;
     }
     upload_plan(dp, &y, jp, &spec, plan, own.base, ctx->stream);
     gs_instance one = in;
     one.outage_branch = -1;
+    one.n_outages = 0;
+    one.outages = nullptr;
     one.load_scale = 1.0;
     std::vector<std::vector<double>> start(1);
     start[0] = vm;
@@ -1016,15 +1101,65 @@ std::vector<HostFinal> q_limit_passes(gs_net* net, int count, const gs_instance*
 // ================================================================================================ seam solves
 struct SeamPlan {
   DevicePlan dev;
-  std::vector<int> nonzero_pos;
 };
 
-}  // namespace gs
+// Byte offsets of the seam's per-call device buffers for a system of h.m rows and nnz stored entries on plan h: the
+// matrix values and right-hand side as given, the solution, the factor arrays, the global solve vector, the row
+// floors, the status word and the kernel's descriptor.
+struct SeamLayout {
+  size_t a, b, x, vals, fval, bval, dval, xs, rf, status, desc, total;
+};
 
-namespace {
+SeamLayout seam_layout(const DevPlan& h, int nnz) {
+  auto al = [](size_t v) { return (v + 255) & ~size_t(255); };
+  const size_t m = (size_t)h.m;
+  SeamLayout L{};
+  size_t o = 0;
+  L.a = o; o += al(8 * (size_t)std::max(nnz, 1));
+  L.b = o; o += al(8 * m);
+  L.x = o; o += al(8 * m);
+  L.vals = o; o += al(8 * (size_t)vals_slots(h));
+  L.fval = o; o += al(8 * (size_t)fval_slots(h));
+  L.bval = o; o += al(8 * (size_t)bval_slots(h));
+  L.dval = o; o += al(8 * (size_t)dval_slots(h));
+  L.xs = o; o += al(8 * m);
+  L.rf = o; o += al(8 * m);
+  L.status = o; o += al(sizeof(int32_t));
+  L.desc = o; o += al(sizeof(SeamDesc));
+  L.total = o;
+  return L;
+}
 
-gs_status seam_solve_one(gs_ctx* ctx, int n, const int* rowptr, const int* colidx, const double* val,
-                         const double* b, double* x) {
+// One seam system on plan h with its per-call buffers at `base` (device memory laid out by seam_layout): the values
+// and right-hand side go up, the kernel replays the factorisation exactly, x comes back. Returns the kernel's status
+// (0 solved, 1 refused).
+int seam_run(const DevPlan& h, char* base, const SeamLayout& L, int nnz, const double* val, const double* b,
+             std::vector<double>& xt, hipStream_t s) {
+  const size_t m = (size_t)h.m;
+  SeamDesc d{reinterpret_cast<double*>(base + L.a),    reinterpret_cast<double*>(base + L.b),
+             reinterpret_cast<double*>(base + L.x),    reinterpret_cast<double*>(base + L.vals),
+             reinterpret_cast<double*>(base + L.fval), reinterpret_cast<double*>(base + L.bval),
+             reinterpret_cast<double*>(base + L.dval), reinterpret_cast<double*>(base + L.xs),
+             reinterpret_cast<double*>(base + L.rf),   reinterpret_cast<int32_t*>(base + L.status)};
+  const int32_t refused = 1;
+  check(hipMemcpyAsync(base + L.a, val, 8 * (size_t)nnz, hipMemcpyHostToDevice, s), "seam a");
+  check(hipMemcpyAsync(base + L.b, b, 8 * m, hipMemcpyHostToDevice, s), "seam b");
+  check(hipMemcpyAsync(base + L.status, &refused, sizeof(refused), hipMemcpyHostToDevice, s), "seam status");
+  check(hipMemcpyAsync(base + L.desc, &d, sizeof(SeamDesc), hipMemcpyHostToDevice, s), "seam desc");
+  check(launch_seam(reinterpret_cast<const SeamDesc*>(base + L.desc), h, 1, 8 * (size_t)ws_slots(h), s), "seam kernel");
+  int32_t status = 1;
+  xt.resize(m);
+  check(hipMemcpyAsync(&status, base + L.status, sizeof(status), hipMemcpyDeviceToHost, s), "seam status");
+  check(hipMemcpyAsync(xt.data(), base + L.x, 8 * m, hipMemcpyDeviceToHost, s), "seam x");
+  // the host arguments and the descriptor are read by the stream: finish before they go
+  check(hipStreamSynchronize(s), "seam sync");
+  return status;
+}
+
+// The seam's checks of its arguments (CpuSparseLuSolver.Solve and SparseLuFactorization.Factor): GS_OK, or the
+// status to return with ctx->err set; a malformed call throws GS_INVALID_ARGUMENT.
+gs_status seam_check(gs_ctx* ctx, int n, const int* rowptr, const int* colidx, const double* val, const double* b,
+                     const double* x) {
   if (n < 0 || !rowptr || (n > 0 && (!colidx || !val || !b || !x))) fail(GS_INVALID_ARGUMENT, "null argument");
   if (n == 0) return GS_OK;
   if (rowptr[0] != 0) fail(GS_INVALID_ARGUMENT, "rowptr[0] must be 0");
@@ -1035,75 +1170,87 @@ gs_status seam_solve_one(gs_ctx* ctx, int n, const int* rowptr, const int* colid
       if (k > rowptr[i] && colidx[k] <= colidx[k - 1]) fail(GS_INVALID_ARGUMENT, "columns must be sorted per row");
     }
   }
-  int nnz = rowptr[n];
-  for (int k = 0; k < nnz; k++)
+  for (int k = 0; k < rowptr[n]; k++)
     if (!std::isfinite(val[k])) {
       ctx->err = "Sparse LU: the matrix contains a non-finite entry, so it cannot be factorised.";
       return GS_NONFINITE_INPUT;
     }
-  std::vector<int> rp(rowptr, rowptr + n + 1), ci(colidx, colidx + nnz);
-  auto key = std::make_pair(rp, ci);
-  auto it = ctx->seam_orders.find(key);
-  if (it == ctx->seam_orders.end()) it = ctx->seam_orders.emplace(key, minimum_degree(n, rowptr, colidx)).first;
-  const std::vector<int>& order = it->second;
-  std::vector<int> nz;
-  nz.reserve(nnz);
-  for (int k = 0; k < nnz; k++) if (val[k] != 0.0) nz.push_back(k);
-  hipStream_t s = ctx->stream;
-  JacobianPattern jp;
-  jp.m = n;
-  jp.rp = rp;
-  jp.ci = ci;
-  std::vector<double> xt(n);
-  auto run_device = [&](SeamPlan& sp) -> int {
-    DevMem a, bb, xx, fv, bv, dv, xs, rf, av, stt, desc;
-    a.ensure(8 * (size_t)nnz); bb.ensure(8 * (size_t)n); xx.ensure(8 * (size_t)n);
-    fv.ensure(8 * (size_t)fval_slots(sp.dev.h)); bv.ensure(8 * (size_t)bval_slots(sp.dev.h));
-    dv.ensure(8 * (size_t)dval_slots(sp.dev.h)); xs.ensure(8 * (size_t)n);
-    rf.ensure(8 * (size_t)n); av.ensure(8 * (size_t)vals_slots(sp.dev.h)); stt.ensure(4);
-    check(hipMemcpyAsync(a.p, val, 8 * (size_t)nnz, hipMemcpyHostToDevice, s), "seam a");
-    check(hipMemcpyAsync(bb.p, b, 8 * (size_t)n, hipMemcpyHostToDevice, s), "seam b");
-    SeamDesc d{a.as<double>(),  bb.as<double>(), xx.as<double>(), av.as<double>(), fv.as<double>(),
-               bv.as<double>(), dv.as<double>(), xs.as<double>(), rf.as<double>(), stt.as<int>()};
-    desc.ensure(sizeof(SeamDesc));
-    check(hipMemcpyAsync(desc.p, &d, sizeof(SeamDesc), hipMemcpyHostToDevice, s), "seam desc");
-    size_t lds = 8 * (size_t)ws_slots(sp.dev.h);
-    check(launch_seam(desc.as<SeamDesc>(), sp.dev.h, 1, lds, s), "seam kernel");
-    int status = 1;
-    check(hipMemcpyAsync(&status, stt.p, 4, hipMemcpyDeviceToHost, s), "seam status");
-    check(hipMemcpyAsync(xt.data(), xx.p, 8 * (size_t)n, hipMemcpyDeviceToHost, s), "seam x");
-    check(hipStreamSynchronize(s), "seam sync");
-    return status;
-  };
-  auto pit = ctx->seam_plans.find(nz);
-  if (pit != ctx->seam_plans.end()) {
-    if (run_device(*pit->second) == 0) {
-      for (int k = 0; k < n; k++)
-        if (!std::isfinite(b[k])) {
-          ctx->err = "Sparse LU: the right-hand side has a non-finite entry.";
-          return GS_NONFINITE_INPUT;
-        }
-      std::copy(xt.begin(), xt.end(), x);
-      return GS_OK;
-    }
-  }
-  Plan plan;
-  std::string why;
-  if (!gp_plan(n, rp, ci, val, order, 0.0, false, plan, why)) {
-    ctx->err = why;
-    return GS_SINGULAR;
-  }
+  return GS_OK;
+}
+
+gs_status seam_rhs_check(gs_ctx* ctx, int n, const double* b) {
   for (int k = 0; k < n; k++)
     if (!std::isfinite(b[k])) {
       ctx->err = "Sparse LU: the right-hand side has a non-finite entry.";
       return GS_NONFINITE_INPUT;
     }
+  return GS_OK;
+}
+
+// The minimum-degree order of a pattern, cached per pattern as CpuSparseLuSolver caches it (CpuSparseLuSolver.cs:34-39).
+const std::vector<int>& seam_order(gs_ctx* ctx, int n, const int* rowptr, const int* colidx) {
+  std::vector<int> key;
+  key.reserve(1 + (size_t)n + 1 + (size_t)rowptr[n]);
+  key.push_back(n);
+  key.insert(key.end(), rowptr, rowptr + n + 1);
+  key.insert(key.end(), colidx, colidx + rowptr[n]);
+  if (std::vector<int>* hit = ctx->seam_orders.find(key)) return *hit;
+  return ctx->seam_orders.put(key, minimum_degree(n, rowptr, colidx));
+}
+
+}  // namespace gs
+
+namespace {
+
+gs_status seam_solve_one(gs_ctx* ctx, int n, const int* rowptr, const int* colidx, const double* val,
+                         const double* b, double* x) {
+  if (gs_status st = seam_check(ctx, n, rowptr, colidx, val, b, x)) return st;
+  if (n == 0) return GS_OK;
+  const int nnz = rowptr[n];
+  const std::vector<int>& order = seam_order(ctx, n, rowptr, colidx);
+  // a device plan is keyed by the whole structure it was made for: the pattern and the positions of the nonzero
+  // values (the factorisation drops exact zeros, as SparseLu.ToCsc does)
+  std::vector<int> key;
+  key.reserve(2 + (size_t)n + 2 * (size_t)nnz);
+  key.push_back(n);
+  key.insert(key.end(), rowptr, rowptr + n + 1);
+  key.insert(key.end(), colidx, colidx + nnz);
+  key.push_back(-1);
+  for (int k = 0; k < nnz; k++) if (val[k] != 0.0) key.push_back(k);
+  hipStream_t s = ctx->stream;
+  std::vector<double> xt;
+  if (std::shared_ptr<SeamPlan>* hit = ctx->seam_plans.find(key)) {
+    std::shared_ptr<SeamPlan> sp = *hit;
+    const SeamLayout L = seam_layout(sp->dev.h, nnz);
+    ctx->seam_buf.ensure(L.total);
+    if (seam_run(sp->dev.h, ctx->seam_buf.as<char>(), L, nnz, val, b, xt, s) == 0) {
+      if (gs_status st = seam_rhs_check(ctx, n, b)) return st;
+      ctx->seam_plan_hits++;
+      std::copy(xt.begin(), xt.end(), x);
+      return GS_OK;
+    }
+  }
+  // no plan for this structure, or a pivot of the cached one would move: the factorisation afresh on the host
+  Plan plan;
+  std::string why;
+  std::vector<int> rp(rowptr, rowptr + n + 1), ci(colidx, colidx + nnz);
+  if (!gp_plan(n, rp, ci, val, order, 0.0, false, plan, why)) {
+    ctx->err = why;
+    return GS_SINGULAR;
+  }
+  if (gs_status st = seam_rhs_check(ctx, n, b)) return st;
+  JacobianPattern jp;
+  jp.m = n;
+  jp.rp = std::move(rp);
+  jp.ci = std::move(ci);
   auto sp = std::make_shared<SeamPlan>();
   upload_plan(sp->dev, nullptr, jp, nullptr, plan, 100.0, s);
-  sp->nonzero_pos = nz;
-  int st = run_device(*sp);
-  if (st != 0) fail(GS_DEVICE_ERROR, "the device refused the replay of its own factorisation");
-  ctx->seam_plans[nz] = sp;
+  const SeamLayout L = seam_layout(sp->dev.h, nnz);
+  ctx->seam_buf.ensure(L.total);
+  if (seam_run(sp->dev.h, ctx->seam_buf.as<char>(), L, nnz, val, b, xt, s) != 0)
+    fail(GS_DEVICE_ERROR, "the device refused the replay of its own factorisation");
+  ctx->seam_plans.put(key, sp);
+  ctx->seam_plan_builds++;
   std::copy(xt.begin(), xt.end(), x);
   return GS_OK;
 }
@@ -1117,6 +1264,7 @@ struct gs_batch {
     gs_net* net = nullptr;
     int count = 0;
     std::vector<gs_instance> inst;
+    std::vector<std::vector<int32_t>> outage_lists;   // the instances' outage lists (inst[i].outages points here)
     gs_nr_options opts{};
     DevMem inst_buf, desc_buf, state_buf, aux_buf;
     Staging stg;
@@ -1204,7 +1352,6 @@ gs_status batch_run(gs_batch* b, double* device_ms) {
   check(hipEventElapsedTime(&ms, b->e0, b->e1), "elapsed");
   if (device_ms) *device_ms = ms;
   for (auto& g : b->groups) {
-    const int n = g->net->net.n;
     g->results.assign(g->count, gs_nr_result{});
     g->finals.clear();
     double kms = 0;
@@ -1328,8 +1475,7 @@ GS_API gs_status gs_network_upload(gs_ctx* ctx, const gs_network* net, const gs_
       return GS_SINGULAR;
     }
     upload_plan(h->dev, &h->y, h->jp, &h->spec, h->plan, h->net.base, ctx->stream);
-    std::vector<std::vector<double>> keep;
-    upload_branches(h->dev, h->net, keep, ctx->stream);
+    upload_branches(h->dev, h->net, ctx->stream);
     check(hipStreamSynchronize(ctx->stream), "upload sync");
     *out = h.release();
     return GS_OK;
@@ -1345,9 +1491,7 @@ GS_API gs_status gs_nr_solve_batch(gs_net* net, int32_t count, const gs_instance
     if (count < 0 || (count > 0 && (!inst || !results))) fail(GS_INVALID_ARGUMENT, "null argument");
     gs_nr_options o = normalise(opts);
     if ((o.flat_start != 0) != net->flat_start) fail(GS_INVALID_ARGUMENT, "flat_start differs from the upload");
-    for (int i = 0; i < count; i++)
-      if (inst[i].outage_branch < -1 || inst[i].outage_branch >= net->net.nb || !(inst[i].load_scale > 0))
-        fail(GS_INVALID_ARGUMENT, "instance " + std::to_string(i));
+    check_instances(net->net.nb, count, inst, "");
     if (count == 0) return GS_OK;
     if (!(o.flags & GS_FLAG_Q_LIMITS)) {
       run_batch(net, count, inst, o, results, vm_out, va_out, false);
@@ -1374,9 +1518,7 @@ GS_API gs_status gs_nr_screen_batch(gs_net* net, int32_t count, const gs_instanc
     if (count < 0 || (count > 0 && (!inst || !results || !summaries))) fail(GS_INVALID_ARGUMENT, "null argument");
     gs_nr_options o = normalise(opts);
     if ((o.flat_start != 0) != net->flat_start) fail(GS_INVALID_ARGUMENT, "flat_start differs from the upload");
-    for (int i = 0; i < count; i++)
-      if (inst[i].outage_branch < -1 || inst[i].outage_branch >= net->net.nb || !(inst[i].load_scale > 0))
-        fail(GS_INVALID_ARGUMENT, "instance " + std::to_string(i));
+    check_instances(net->net.nb, count, inst, "");
     if (count == 0) return GS_OK;
     std::vector<HostFinal> finals;
     run_batch(net, count, inst, o, results, vm_out, va_out, false, &finals);
@@ -1633,14 +1775,18 @@ GS_API gs_status gs_batch_create(gs_ctx* ctx, int32_t n_groups, gs_net* const* n
       if ((o.flat_start != 0) != net->flat_start) fail(GS_INVALID_ARGUMENT, "flat_start differs from the upload");
       if (o.flags & GS_FLAG_Q_LIMITS)
         fail(GS_INVALID_ARGUMENT, "group " + std::to_string(gi) + ": reactive limits run through gs_nr_solve_batch");
-      for (int i = 0; i < counts[gi]; i++)
-        if (inst[gi][i].outage_branch < -1 || inst[gi][i].outage_branch >= net->net.nb ||
-            !(inst[gi][i].load_scale > 0))
-          fail(GS_INVALID_ARGUMENT, "group " + std::to_string(gi) + ", instance " + std::to_string(i));
+      check_instances(net->net.nb, counts[gi], inst[gi], "group " + std::to_string(gi) + ", ");
       auto g = std::make_unique<gs_batch::Group>();
       g->net = net;
       g->count = counts[gi];
       g->inst.assign(inst[gi], inst[gi] + counts[gi]);
+      // the outage lists are the caller's: the group keeps its own copies (the host reads them after the call)
+      g->outage_lists.resize(g->count);
+      for (int i = 0; i < g->count; i++) {
+        gs_instance& in = g->inst[i];
+        if (in.n_outages > 0) g->outage_lists[i].assign(in.outages, in.outages + in.n_outages);
+        in.outages = in.n_outages > 0 ? g->outage_lists[i].data() : nullptr;
+      }
       g->opts = o;
       check(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking), "stream");
       check(hipEventCreateWithFlags(&g->joined, hipEventDisableTiming), "event");

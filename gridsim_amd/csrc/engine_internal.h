@@ -4,6 +4,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+#include <list>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -104,20 +106,81 @@ struct WideGraph {
   }
 };
 
-// A plan and its network pattern uploaded to the device.
+// Host image of the device arrays of one plan (or network pattern): every array at a 256-byte aligned offset of one
+// byte block, so the whole plan goes to the device in one allocation and one copy, into a block of its own or into
+// memory the caller provides. `bind` points a DevPlan's fields into the block once it lies at `base`.
+struct PlanImage {
+  std::vector<char> bytes;
+  std::vector<std::pair<size_t, size_t>> fields;   // (offset of the DevPlan pointer field, offset in the block)
+  size_t add(const void* data, size_t n) {
+    const size_t off = (bytes.size() + 255) & ~size_t(255);
+    bytes.resize(off + (n ? n : 16), 0);   // an empty array still gets an address of its own
+    if (n) std::memcpy(bytes.data() + off, data, n);
+    return off;
+  }
+  template <typename P, typename T>
+  void put(P DevPlan::*field, const std::vector<T>& v) {
+    const size_t off = add(v.data(), sizeof(T) * v.size());
+    DevPlan probe{};
+    fields.emplace_back((size_t)(reinterpret_cast<char*>(&(probe.*field)) - reinterpret_cast<char*>(&probe)), off);
+  }
+  void bind(DevPlan& h, char* base) const {
+    for (const auto& [field, off] : fields) {
+      char* p = base + off;
+      std::memcpy(reinterpret_cast<char*>(&h) + field, &p, sizeof(p));
+    }
+  }
+};
+
+// A plan and its network pattern uploaded to the device: one block for the plan image, one for the branch data the
+// screening reads.
 struct DevicePlan {
-  DevMem y_rp, y_ij, y_diag, y_gb, slots, aslots, ppos, qpos, j_rp, j_ci, vm0, va0;
-  DevMem pinv, q, prow, piv_pos;
-  DevMem rs_rec, ch_a_src;
-  DevMem fs_rec, bs_rec;
-  DevMem rs_chunk_row, fs_pass_step, bs_pass_step;
-  DevMem br_ends, br_stamp, br_rate, bus_lo, bus_hi, j_diag;
+  DevMem block, branch_block;
   DevPlan h{};   // passed to the kernels by value
   DevicePlanHost dph;
   std::unique_ptr<WideGraph> wide;
 };
 
 struct SeamPlan;
+
+// A map holding at most `cap` entries: inserting past the cap drops the least recently used one.
+template <typename K, typename V>
+class LruCache {
+ public:
+  explicit LruCache(size_t cap) : cap_(cap) {}
+  V* find(const K& k) {
+    auto it = map_.find(k);
+    if (it == map_.end()) return nullptr;
+    order_.splice(order_.begin(), order_, it->second.second);
+    return &it->second.first;
+  }
+  V& put(const K& k, V v) {
+    auto it = map_.find(k);
+    if (it != map_.end()) {
+      it->second.first = std::move(v);
+      order_.splice(order_.begin(), order_, it->second.second);
+      return it->second.first;
+    }
+    while (!map_.empty() && map_.size() >= cap_) {
+      map_.erase(order_.back());
+      order_.pop_back();
+    }
+    order_.push_front(k);
+    auto ins = map_.emplace(k, std::make_pair(std::move(v), order_.begin())).first;
+    return ins->second.first;
+  }
+  void clear() { map_.clear(); order_.clear(); }
+  size_t size() const { return map_.size(); }
+
+ private:
+  size_t cap_;
+  std::list<K> order_;   // most recently used first
+  std::map<K, std::pair<V, typename std::list<K>::iterator>> map_;
+};
+
+// The sparse-solver seam keeps the minimum-degree order of its last kSeamOrderCap patterns and the device plans of its
+// last kSeamPlanCap structures (pattern and nonzero positions).
+constexpr size_t kSeamOrderCap = 256, kSeamPlanCap = 16;
 
 }  // namespace gs
 
@@ -128,8 +191,12 @@ struct gs_ctx {
   std::string err;
   size_t budget = 0;
   double us_per_tick = 0.01;  // device wall clock (wall_clock64) period in microseconds
-  std::map<std::pair<std::vector<int>, std::vector<int>>, std::vector<int>> seam_orders;  // pattern -> MD order
-  std::map<std::vector<int>, std::shared_ptr<gs::SeamPlan>> seam_plans;                   // nonzero positions -> plan
+  // key: n, rowptr, colidx -> minimum-degree order
+  gs::LruCache<std::vector<int>, std::vector<int>> seam_orders{gs::kSeamOrderCap};
+  // key: n, rowptr, colidx, -1, positions of the nonzero values -> the device plan of the last factorisation
+  gs::LruCache<std::vector<int>, std::shared_ptr<gs::SeamPlan>> seam_plans{gs::kSeamPlanCap};
+  gs::DevMem seam_buf;        // the seam's per-call device buffers (grown to the largest system, reused)
+  int64_t seam_plan_hits = 0, seam_plan_builds = 0;
 };
 
 struct gs_net {
@@ -168,6 +235,7 @@ struct Staging {
   std::vector<double> aux;                                  // the network's own specification
   std::vector<std::pair<int, std::vector<double>>> stage;   // per instance: [pspec | qspec | vm0 | va0]
   std::vector<InstState> states;                            // resumed states (uploaded when not starting afresh)
+  std::vector<std::pair<double*, std::vector<double>>> own_y;  // instances' own Y values and their device places
 };
 
 // Device buffers of one prepared run: `count` instance blocks of Layout::stride bytes each, their descriptors and
@@ -177,6 +245,8 @@ struct RunBuffers {
   InstDesc* desc = nullptr;
   InstState* state = nullptr;
   const double* base_spec = nullptr;
+  double* own_y = nullptr;    // room for own_y_count instances' own Y values (2 nnz_y doubles each)
+  int own_y_count = 0;
 };
 
 struct PreparedRun {
@@ -195,7 +265,10 @@ bool symbolic_pass(const YBus& y, const JacobianPattern& jp, const double* vm, c
 gs_nr_options normalise(const gs_nr_options* o);
 Layout instance_layout(const DevPlan& h, bool fp32, bool x_lds);
 BatchRun plan_run(const DevPlan& h, const gs_nr_options& o, bool force_fp64);
-int outage_patches(const YBus& y, const Network& net, int branch, Patch* out);
+std::vector<int> outage_set(const gs_instance& in, int n_branch);
+void check_instances(int n_branch, int count, const gs_instance* inst, const std::string& where);
+std::vector<Patch> outage_patches(const YBus& y, const Network& net, const std::vector<int>& outaged);
+int own_y_count(const YBus* y, const Network* netw, int count, const gs_instance* inst);
 void fill_result(const InstState& s, gs_nr_result& r, int refreshes, double us_per_tick);
 
 // Fills descriptors for `count` instances in the buffers rb and uploads them and the instances' own data on stream s
@@ -218,7 +291,10 @@ void finish_paused(gs_net* net, const gs_instance& in, const gs_nr_options& o, b
                    const std::vector<int>* types = nullptr, const std::vector<double>* qspec = nullptr);
 // The branch data screening reads (end buses, stamps, rate A) and the buses' own voltage limits, uploaded with a
 // network.
-void upload_branches(DevicePlan& dp, const Network& net, std::vector<std::vector<double>>& keep, hipStream_t s);
+void upload_branches(DevicePlan& dp, const Network& net, hipStream_t s);
+// The host image upload_plan copies to the device (scalar fields of h set, pointer fields recorded for img.bind).
+void build_plan_image(PlanImage& img, DevPlan& h, DevicePlanHost& d, const YBus* y, const JacobianPattern& jp,
+                      const BusSpec* spec, const Plan& plan, double base_mva);
 
 // Runs f under the context's lock, turning exceptions into status codes and the context's last error.
 template <typename F>

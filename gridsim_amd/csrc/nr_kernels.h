@@ -95,7 +95,11 @@ struct WideSplit {
   hipEvent_t fork = nullptr, join = nullptr;
 };
 
+// Y cells an instance patches in place (every cell an N-1 or N-2 outage touches fits); an outage touching more cells
+// gives its instance a Y value array of its own (InstDesc::y_gb_own).
 constexpr int kMaxPatch = 8;
+// outaged branches of one instance (the ABI's GS_MAX_OUTAGES)
+constexpr int kMaxOutages = 8;
 
 struct Patch {
   int32_t idx;      // Y cell
@@ -109,8 +113,10 @@ struct InstState {
   int32_t done;         // 1: finished (status valid); 2: paused for a fresh symbolic pass
   int32_t converged, diverged;
   int32_t nonfinite, step_failures;
-  int32_t fail_column, pad;
-  double start_mis, max_mis, min_ratio, pad2;
+  int32_t fail_column;
+  int32_t engine_stop;  // GS_STOP_* set on the device (the shared solve vector never came free)
+  double start_mis, max_mis, min_ratio;
+  double last_from;     // the mismatch before the last step taken (NaN: none), for the DC retry's stall test
   // time in each phase (device wall-clock ticks): mismatch + Jacobian, refactor, solves + refinement, update
   uint64_t t_mismatch, t_refactor, t_solve, t_update;
   // level-parallel schedules: reductions across the work-groups of one instance (bit patterns of non-negative
@@ -129,8 +135,10 @@ struct InstDesc {
   InstState* state;
   const double *pspec, *qspec;        // specified injections: the network's, or this instance's load-scaled copy
   const double *vm_start, *va_start;  // start point: the network's, or this instance's own
-  int32_t n_patch, outage;            // patched Y cells; the outaged branch (-1: none)
+  int32_t n_patch, n_out;             // patched Y cells; outaged branches
   Patch patch[kMaxPatch];
+  int32_t out[kMaxOutages];           // the outaged branches (their flows are not reported)
+  const double* y_gb_own;             // this instance's own Y values (g, b per cell), or null: the network's + patches
   // per-instance buffers (device)
   double *vm, *va, *p, *q;
   double *f, *dx, *r, *jv;            // f, dx, residual, FP64 Jacobian values (CSR order)

@@ -96,6 +96,7 @@ struct Inst {
   GLOBAL v2d* tpq;
   GLOBAL T *vals, *fval, *bval, *dval, *xg;
   GLOBAL const Patch* patch;
+  GLOBAL const v2d* ygb;   // the Y values (g, b) per cell: the network's, or the instance's own
   int n_patch;
   int pidx[kMaxPatch];   // patched Y cells (-1: unused)
 };
@@ -128,7 +129,7 @@ __device__ __forceinline__ void put_j(const Inst<T>& I, int s, int a, double v) 
 template <typename T, bool KEEP>
 __device__ __forceinline__ int mismatch_cells(const DevPlan& P, const Inst<T>& I, int lane, int blk0, int stride) {
   GLOBAL const v2i* y_ij = globv<v2i>(P.y_ij);
-  GLOBAL const v2d* y_gb = globv<v2d>(P.y_gb);
+  GLOBAL const v2d* y_gb = I.ygb;
   GLOBAL const v4i* slots = globv<v4i>(P.slots);
   GLOBAL const v4i* aslots = globv<v4i>(P.aslots);
   const int nnz = P.nnz_y;
@@ -191,7 +192,7 @@ __device__ __forceinline__ int mismatch_cells(const DevPlan& P, const Inst<T>& I
 template <typename T, bool KEEP>
 __device__ __forceinline__ double mismatch_buses(const DevPlan& P, const Inst<T>& I, int lane, int blk0, int stride,
                                                  int* bad) {
-  GLOBAL const v2d* y_gb = globv<v2d>(P.y_gb);
+  GLOBAL const v2d* y_gb = I.ygb;
   GLOBAL const v4i* slots = globv<v4i>(P.slots);
   GLOBAL const v4i* aslots = globv<v4i>(P.aslots);
   GLOBAL const int* y_rp = glob(P.y_rp);
@@ -617,6 +618,7 @@ __global__ __launch_bounds__(kWave * (PAIR ? kPairWaves : 1), 2) void nr_batch_k
   I.dval = glob(reinterpret_cast<T*>(d.dval));
   I.xg = glob(reinterpret_cast<T*>(d.xg));
   I.patch = glob(&descs[inst].patch[0]);
+  I.ygb = d.y_gb_own ? globv<v2d>(d.y_gb_own) : globv<v2d>(P.y_gb);
   I.n_patch = d.n_patch;
 #pragma unroll
   for (int t = 0; t < kMaxPatch; t++) I.pidx[t] = t < I.n_patch ? d.patch[t].idx : -1;
@@ -631,6 +633,7 @@ __global__ __launch_bounds__(kWave * (PAIR ? kPairWaves : 1), 2) void nr_batch_k
   int iter = prm.init ? 0 : st->iter;
   double start_mis = prm.init ? __builtin_nan("") : st->start_mis;
   double min_ratio = prm.init ? 1.0 : st->min_ratio;
+  double last_from = prm.init ? __builtin_nan("") : st->last_from;
   int step_failures = prm.init ? 0 : st->step_failures;
   int converged = 0, diverged = 0, paused = 0, fail_col = -1, steps = 0;
   double max_mis;
@@ -701,6 +704,7 @@ __global__ __launch_bounds__(kWave * (PAIR ? kPairWaves : 1), 2) void nr_batch_k
     }
     WG_SYNC();
     tick(t_upd);
+    last_from = max_mis;
     iter++;
     steps++;
   }
@@ -717,6 +721,8 @@ __global__ __launch_bounds__(kWave * (PAIR ? kPairWaves : 1), 2) void nr_batch_k
     st->min_ratio = min_ratio;
     st->step_failures = step_failures;
     st->fail_column = fail_col;
+    st->last_from = last_from;
+    st->engine_stop = lock_timeout ? GS_STOP_LOCK_TIMEOUT : 0;
     st->converged = converged;
     st->diverged = diverged;
     st->nonfinite = nonfinite;
@@ -739,7 +745,7 @@ __global__ __launch_bounds__(kWave * (PAIR ? kPairWaves : 1), 2) void nr_batch_k
 // doubles. The arithmetic of every value is the batched kernel's: the same device functions over the same streams.
 
 template <typename T>
-__device__ __forceinline__ Inst<T> load_inst(const InstDesc* descs, int b) {
+__device__ __forceinline__ Inst<T> load_inst(const InstDesc* descs, const DevPlan& P, int b) {
   const InstDesc& d = descs[b];
   Inst<T> I;
   I.pspec = glob(d.pspec); I.qspec = glob(d.qspec); I.vm_start = glob(d.vm_start); I.va_start = glob(d.va_start);
@@ -751,6 +757,7 @@ __device__ __forceinline__ Inst<T> load_inst(const InstDesc* descs, int b) {
   I.dval = glob(reinterpret_cast<T*>(d.dval));
   I.xg = glob(reinterpret_cast<T*>(d.xg));
   I.patch = glob(&descs[b].patch[0]);
+  I.ygb = d.y_gb_own ? globv<v2d>(d.y_gb_own) : globv<v2d>(P.y_gb);
   I.n_patch = d.n_patch;
 #pragma unroll
   for (int t = 0; t < kMaxPatch; t++) I.pidx[t] = t < I.n_patch ? d.patch[t].idx : -1;
@@ -793,8 +800,9 @@ __global__ __launch_bounds__(256) void wide_start(const InstDesc* __restrict__ d
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     InstState* st = d.state;
     st->iter = 0; st->done = 0; st->converged = 0; st->diverged = 0; st->nonfinite = 0; st->step_failures = 0;
-    st->fail_column = -1;
+    st->fail_column = -1; st->engine_stop = 0;
     st->start_mis = __builtin_nan(""); st->max_mis = __builtin_nan(""); st->min_ratio = 1.0;
+    st->last_from = __builtin_nan("");
     st->t_mismatch = st->t_refactor = st->t_solve = st->t_update = 0;
     st->mis_bits = 0; st->dx_bits = 0; st->ratio_bits = (uint64_t)__double_as_longlong(1.0); st->jbad = 0;
     if (blockIdx.y == 0) ctl->finished = 0;
@@ -806,7 +814,7 @@ __global__ __launch_bounds__(kWave) void wide_cells(const InstDesc* __restrict__
                                                     const NrParams prm) {
   InstState* st = descs[blockIdx.y].state;
   if (st->done) return;
-  const Inst<T> I = load_inst<T>(descs, blockIdx.y);
+  const Inst<T> I = load_inst<T>(descs, P, blockIdx.y);
   const int bad = prm.refine_steps > 0 ? mismatch_cells<T, true>(P, I, threadIdx.x, blockIdx.x, gridDim.x)
                                        : mismatch_cells<T, false>(P, I, threadIdx.x, blockIdx.x, gridDim.x);
   if (wave_any(bad) && threadIdx.x == 0) atomicOr(&st->jbad, 1);
@@ -817,7 +825,7 @@ __global__ __launch_bounds__(kWave) void wide_buses(const InstDesc* __restrict__
                                                     const NrParams prm) {
   InstState* st = descs[blockIdx.y].state;
   if (st->done) return;
-  const Inst<T> I = load_inst<T>(descs, blockIdx.y);
+  const Inst<T> I = load_inst<T>(descs, P, blockIdx.y);
   int bad;
   const double lmax = wave_max(prm.refine_steps > 0
                                    ? mismatch_buses<T, true>(P, I, threadIdx.x, blockIdx.x, gridDim.x, &bad)
@@ -864,7 +872,7 @@ __global__ __launch_bounds__(kWave) void wide_refactor(const InstDesc* __restric
   extern __shared__ __align__(16) unsigned char smem[];
   InstState* st = descs[blockIdx.y].state;
   if (st->done) return;
-  const Inst<T> I = load_inst<T>(descs, blockIdx.y);
+  const Inst<T> I = load_inst<T>(descs, P, blockIdx.y);
   const int per = (end_chunk - first_chunk) / (int)gridDim.x;
   const int c = first_chunk + blockIdx.x * per;
   const int row0 = P.rs_chunk_row[c], row1 = P.rs_chunk_row[c + per];
@@ -983,7 +991,10 @@ __global__ __launch_bounds__(256) void wide_update(const InstDesc* __restrict__ 
       d.vm[i] = v < 0.01 ? 0.01 : v;
     }
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) st->iter = st->iter + 1;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->last_from = st->max_mis;   // the mismatch this step started from
+    st->iter = st->iter + 1;
+  }
 }
 
 // ------------------------------------------------------------------------------------------------ iterative step
@@ -1096,7 +1107,7 @@ __device__ bool ilu0_bicgstab(const DevPlan& P, const double* a, const double* b
 __global__ __launch_bounds__(kWave) void iterative_step_kernel(const InstDesc* __restrict__ descs, const DevPlan P,
                                                                const NrParams prm, double* work, int* iw) {
   const int lane = threadIdx.x;
-  const Inst<double> I = load_inst<double>(descs, 0);
+  const Inst<double> I = load_inst<double>(descs, P, 0);
   InstState* st = descs[0].state;
   if (prm.load_start) {
     for (int i = lane; i < P.n; i += kWave) { I.vm[i] = I.vm_start[i]; I.va[i] = I.va_start[i]; }
@@ -1139,6 +1150,7 @@ __global__ __launch_bounds__(kWave) void iterative_step_kernel(const InstDesc* _
             I.vm[i] = v < 0.01 ? 0.01 : v;
           }
         }
+        st->last_from = max_mis;
         st->iter = iter + 1;
       }
     }
@@ -1184,7 +1196,9 @@ __global__ __launch_bounds__(256) void screen_kernel(const InstDesc* __restrict_
   for (int k = t; k < P.nb; k += 256) {
     const int2 e = P.br_ends[k];
     double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    if (e.x >= 0 && k != d.outage) {
+    bool out = false;
+    for (int t = 0; t < d.n_out; t++) out |= d.out[t] == k;
+    if (e.x >= 0 && !out) {
       double sf, cf, st, ct;
       gs_fdlibm::sincos(d.va[e.x], &sf, &cf);
       gs_fdlibm::sincos(d.va[e.y], &st, &ct);

@@ -88,6 +88,7 @@ struct Request {
   int net = 0;
   gs_instance inst{};
   std::vector<double> start;  // vm0 | va0 when the request brought its own start point
+  std::vector<int32_t> outages;  // the request's outage list (gs_instance::outages points here when launched)
   Clock::time_point submitted;
 };
 
@@ -188,6 +189,7 @@ void launch_slot(gs_server* srv, gs_server::Slot& s) {
   std::vector<gs_instance> inst(count);
   for (int i = 0; i < count; i++) {
     inst[i] = s.reqs[i].inst;
+    inst[i].outages = s.reqs[i].outages.empty() ? nullptr : s.reqs[i].outages.data();
     if (!s.reqs[i].start.empty()) {
       inst[i].vm0 = s.reqs[i].start.data();
       inst[i].va0 = s.reqs[i].start.data() + h.n;
@@ -244,7 +246,11 @@ std::vector<std::pair<uint64_t, Finished>> reap_slot(gs_server* srv, gs_server::
         int nl = 0;
         gs_instance in = s.reqs[i].inst;
         in.vm0 = in.va0 = nullptr;
+        in.outages = s.reqs[i].outages.empty() ? nullptr : s.reqs[i].outages.data();
         std::vector<double> pp, qq;
+        // the fresh symbolic passes and resumed launches run on the context's stream: under the context's lock, as
+        // every other call on the context, so they never land inside a capture a client has begun on that stream
+        std::lock_guard<std::mutex> ctx_lock(srv->ctx->mu);
         finish_paused(nt.net, in, nt.opts, false, st[i], f.vm, f.va, f.result, kms, nl, &pp, &qq);
         if ((int)pp.size() == n) f.p = pp;
         if ((int)qq.size() == n) f.q = qq;
@@ -449,9 +455,13 @@ GS_API gs_status gs_server_submit(gs_server* srv, int32_t network_id, const gs_i
     std::lock_guard<std::mutex> lock(srv->mu);
     if (network_id < 0 || network_id >= (int32_t)srv->nets.size()) return GS_INVALID_ARGUMENT;
     const gs_net* net = srv->nets[network_id]->net;
-    if (inst->outage_branch < -1 || inst->outage_branch >= net->net.nb) return GS_INVALID_ARGUMENT;
-    if (!(inst->load_scale > 0) || !std::isfinite(inst->load_scale)) return GS_INVALID_ARGUMENT;
-    if ((inst->vm0 == nullptr) != (inst->va0 == nullptr)) return GS_INVALID_ARGUMENT;
+    try {
+      check_instances(net->net.nb, 1, inst, "");
+      // an outage touching more Y cells than a descriptor patches needs an array of its own: not in the serving loop
+      if (own_y_count(&net->y, &net->net, 1, inst) > 0) return GS_INVALID_ARGUMENT;
+    } catch (const HipError&) {
+      return GS_INVALID_ARGUMENT;
+    }
     if ((int64_t)srv->queue.size() >= srv->cfg.queue_capacity) {
       srv->rejected++;
       return GS_BUSY;
@@ -464,6 +474,9 @@ GS_API gs_status gs_server_submit(gs_server* srv, int32_t network_id, const gs_i
       r.start.insert(r.start.end(), inst->va0, inst->va0 + n);
     }
     r.inst.vm0 = r.inst.va0 = nullptr;
+    // the outage list is the caller's: keep a copy
+    r.outages.assign(inst->outages, inst->outages + inst->n_outages);
+    r.inst.outages = nullptr;
     r.ticket = srv->next_ticket++;
     r.submitted = Clock::now();
     *ticket = r.ticket;

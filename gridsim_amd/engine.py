@@ -62,15 +62,20 @@ class GsNrOptions(C.Structure):
 
 
 class GsInstance(C.Structure):
-    _fields_ = [("outage_branch", C.c_int32), ("reserved", C.c_int32), ("load_scale", C.c_double),
-                ("vm0", C.c_void_p), ("va0", C.c_void_p)]
+    _fields_ = [("outage_branch", C.c_int32), ("n_outages", C.c_int32), ("load_scale", C.c_double),
+                ("vm0", C.c_void_p), ("va0", C.c_void_p), ("outages", C.c_void_p)]
 
 
-INSTANCE_DTYPE = np.dtype([("outage_branch", np.int32), ("reserved", np.int32), ("load_scale", np.float64),
-                           ("vm0", np.uint64), ("va0", np.uint64)])   # the layout of GsInstance
+MAX_OUTAGES = 8   # GS_MAX_OUTAGES
+INSTANCE_DTYPE = np.dtype([("outage_branch", np.int32), ("n_outages", np.int32), ("load_scale", np.float64),
+                           ("vm0", np.uint64), ("va0", np.uint64), ("outages", np.uint64)])   # the layout of GsInstance
 RESULT_DTYPE = np.dtype([("status", np.int32), ("iterations", np.int32), ("max_mismatch", np.float64),
                          ("symbolic_refreshes", np.int32), ("step_failures", np.int32),
-                         ("min_pivot_ratio", np.float64), ("phase_us", np.float64, (4,))])
+                         ("min_pivot_ratio", np.float64), ("phase_us", np.float64, (4,)),
+                         ("engine_stop", np.int32), ("dc_retry", np.int32)])
+# gs_nr_result.engine_stop and .dc_retry (GS_STOP_*, GS_RETRY_* = PowerFlowResult.NewtonRetry)
+STOP_NONE, STOP_LOCK_TIMEOUT, STOP_REFRESH_CAP, STOP_FALLBACK_CAP = 0, 1, 2, 3
+RETRY_NOT_TRIED, RETRY_CONVERGED, RETRY_DID_NOT_CONVERGE, RETRY_REJECTED, RETRY_UNAVAILABLE = 0, 1, 2, 3, 4
 PHASES = ("mismatch+jacobian", "refactor", "solve+refine", "update")
 SUMMARY_DTYPE = np.dtype([("max_loading_pct", np.float64), ("max_loading_branch", np.int32), ("overloads", np.int32),
                           ("vmin", np.float64), ("vmax", np.float64), ("undervoltage", np.int32),
@@ -396,12 +401,22 @@ class DeviceNetwork:
         return _plan_stats_dict(st)
 
     def _instances(self, outages=None, load_scale=None, count=None, starts=None):
+        """GsInstance records. outages: one branch index per instance (-1: none), or per instance a sequence of branch
+        indices (N-2 / N-k: WithoutMany's index list)."""
         count = count if count is not None else (len(outages) if outages is not None else len(load_scale))
         arr = (GsInstance * max(count, 1))()
         view = np.frombuffer(arr, INSTANCE_DTYPE)[:count]    # filled column by column, not record by record
-        view["outage_branch"] = -1 if outages is None else np.asarray(outages, np.int64)[:count]
-        view["load_scale"] = 1.0 if load_scale is None else np.asarray(load_scale, np.float64)[:count]
         keep = []
+        if outages is not None and any(np.ndim(o) > 0 for o in list(outages)[:count]):
+            lists = [np.atleast_1d(np.asarray(o, np.int32)) for o in list(outages)[:count]]
+            view["outage_branch"] = [int(o[0]) if len(o) else -1 for o in lists]
+            rest = [np.ascontiguousarray(o[1:]) for o in lists]
+            keep += rest
+            view["n_outages"] = [len(r) for r in rest]
+            view["outages"] = [r.ctypes.data if len(r) else 0 for r in rest]
+        else:
+            view["outage_branch"] = -1 if outages is None else np.asarray(outages, np.int64)[:count]
+        view["load_scale"] = 1.0 if load_scale is None else np.asarray(load_scale, np.float64)[:count]
         if starts is not None and count > 0:
             vm0 = np.ascontiguousarray(np.asarray(starts[0], np.float64)[:count].reshape(count, -1))
             va0 = np.ascontiguousarray(np.asarray(starts[1], np.float64)[:count].reshape(count, -1))

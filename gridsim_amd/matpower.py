@@ -34,7 +34,12 @@ def _matrix(text: str, name: str) -> np.ndarray:
 
 
 def read_case(path: str) -> Network:
-    text = open(path, encoding="utf-8").read()
+    with open(path, encoding="utf-8") as fh:
+        return read_text(fh.read())
+
+
+def read_text(text: str) -> Network:
+    """A case from the text of a MATPOWER file."""
     base = float(re.search(r"mpc\.baseMVA\s*=\s*([0-9.eE+-]+)", text).group(1))
     bus = _matrix(text, "bus")
     gen = _matrix(text, "gen")

@@ -112,6 +112,35 @@ class Network:
         return self.copy(vm=np.asarray(vm, dtype=np.float64), va_deg=np.asarray(va_deg, dtype=np.float64))
 
 
+def connected_parts(net: Network, removed=()) -> int:
+    """Connected parts of the network with the `removed` branches taken out: buses joined by in-service branches
+    (union-find; a bus with no branch is a part of its own), as ContingencyAnalysis.ConnectedParts counts them
+    (ContingencyAnalysis.cs:1672-1687)."""
+    parent = np.arange(net.n_bus)
+
+    def find(x):
+        while parent[x] != x:
+            parent[x] = parent[parent[x]]
+            x = parent[x]
+        return x
+
+    gone = set(int(b) for b in np.atleast_1d(np.asarray(removed, np.int64)) if b >= 0)
+    parts = net.n_bus
+    for k in range(net.n_branch):
+        if not net.br_in_service[k] or k in gone:
+            continue
+        a, c = find(int(net.br_from[k])), find(int(net.br_to[k]))
+        if a != c:
+            parent[a] = c
+            parts -= 1
+    return parts
+
+
+def splits(net: Network, removed) -> bool:
+    """Whether taking the branches out splits the network (ContingencyAnalysis.Splits, :1665)."""
+    return connected_parts(net, removed) > connected_parts(net)
+
+
 def bridges(net: Network) -> np.ndarray:
     """Boolean mask of in-service branches whose removal splits the network (graph bridges).
 

@@ -72,7 +72,8 @@ typedef struct gs_network {
 } gs_network;
 
 /* PowerFlowOptions (src/GridSim.Core/PowerFlow/PowerFlowOptions.cs:6-170) for the engine. Reactive limits are
- * GS_FLAG_Q_LIMITS (EnforceQLimits; off by default); the DC-start retry is off (RetryFromDcStart = false). */
+ * GS_FLAG_Q_LIMITS (EnforceQLimits; off by default), the DC-start retry GS_FLAG_DC_RETRY (RetryFromDcStart; off unless
+ * the flag is set). */
 typedef struct gs_nr_options {
   double tolerance;         /* PowerFlowOptions.Tolerance, pu on ||mismatch||_inf (1e-8) */
   int32_t max_iterations;   /* PowerFlowOptions.MaxIterations (30) */
@@ -104,21 +105,51 @@ enum { GS_FLAG_SINGLE_WAVE = 4 };
  * current state; iterations add up over the passes. gs_nr_solve_batch and gs_nr_screen_batch only. */
 enum { GS_FLAG_Q_LIMITS = 2 };
 
+/* gs_nr_options.flags: the automatic retry from a DC power-flow angle start (PowerFlowOptions.RetryFromDcStart,
+ * NewtonShell.SolveWithRetry / Stalled / TrySeedDcAngles, src/GridSim.Core/PowerFlow/NewtonShell.cs:142-299). An
+ * instance whose Newton run did not converge and stalled or diverged (not stopped by the iteration cap while still
+ * converging; never when max_iterations <= 1) is solved again on the device from its ordinary start magnitudes and
+ * the DC angles of its own network (reactance-only view, one island referenced at the single slack), and the retry's
+ * state is reported when it converges with no in-service branch angle above 90 degrees; gs_nr_result.dc_retry says
+ * what happened. gs_nr_solve_batch and gs_nr_screen_batch only (with or without GS_FLAG_Q_LIMITS). */
+enum { GS_FLAG_DC_RETRY = 8 };
+
 /* gs_nr_options.schedule: level-parallel. Each Newton step is a captured hipGraph of short kernels, one per phase and
  * one per elimination level of the refactor and of each triangular solve, a work-group per chunk of columns (per pass
  * of rows) of a level and per instance, replayed until every instance has finished. For large networks (tens of
  * thousands of buses), where one wavefront per instance leaves the GPU idle. gs_nr_solve_batch only. */
 enum { GS_SCHEDULE_LEVELS = -1 };
 
-/* One solve of a batch: the uploaded network with one branch out of service (or none) and every load and generator
- * P scaled by load_scale. vm0 / va0 (radians), when non-null, replace the start point BuildBusSpec would give. */
+/* One solve of a batch: the uploaded network with some branches out of service and every load and generator P scaled
+ * by load_scale, as ContingencyAnalysis.ScreenOutageCore solves WithoutMany(model, indices)
+ * (src/GridSim.Core/Analysis/ContingencyAnalysis.cs:1087-1097,1839-1845). The outaged set is outage_branch (when
+ * >= 0) together with outages[0 .. n_outages): N-1 sets outage_branch alone, N-2 / N-k add the list (at most
+ * GS_MAX_OUTAGES branches in all, each once). vm0 / va0 (radians), when non-null, replace the start point
+ * BuildBusSpec would give. */
+#define GS_MAX_OUTAGES 8
 typedef struct gs_instance {
   int32_t outage_branch;    /* index into the network's branches, or -1 */
-  int32_t reserved;
+  int32_t n_outages;        /* further outaged branches in `outages` (0: none) */
   double load_scale;        /* 1.0 = as uploaded */
   const double* vm0;
   const double* va0;
+  const int32_t* outages;   /* n_outages branch indices (may be null when n_outages is 0) */
 } gs_instance;
+
+/* Why the engine, rather than the Newton iteration, ended a run (gs_nr_result.engine_stop). The reference has no such
+ * limits; a run that ends on one is reported with status GS_PF_MAX_ITERATIONS and this reason, so callers and parity
+ * checks can tell it from a real verdict. */
+enum {
+  GS_STOP_NONE = 0,
+  GS_STOP_LOCK_TIMEOUT = 1,     /* a wavefront never obtained its work-group's shared solve vector (bounded spin) */
+  GS_STOP_REFRESH_CAP = 2,      /* the fresh symbolic passes of one instance kept failing the pivot-growth check */
+  GS_STOP_FALLBACK_CAP = 3      /* more ILU(0) + BiCGSTAB steps than max_iterations + 1 */
+};
+
+/* NewtonRetryOutcome (src/GridSim.Core/PowerFlow/PowerFlowResult.cs:24-41): what the retry from the DC power-flow
+ * angle start did (GS_FLAG_DC_RETRY). */
+enum { GS_RETRY_NOT_TRIED = 0, GS_RETRY_CONVERGED = 1, GS_RETRY_DID_NOT_CONVERGE = 2, GS_RETRY_REJECTED = 3,
+       GS_RETRY_UNAVAILABLE = 4 };
 
 typedef struct gs_nr_result {
   int32_t status;               /* GS_PF_* */
@@ -128,6 +159,8 @@ typedef struct gs_nr_result {
   int32_t step_failures;        /* steps whose factorisation was refused (singular) */
   double min_pivot_ratio;       /* smallest |pivot| / max |candidate| seen (1 = the pivot was the largest) */
   double phase_us[4];           /* device time of this instance: mismatch + Jacobian, refactor, solves, update */
+  int32_t engine_stop;          /* GS_STOP_*: an engine limit ended the run (status then GS_PF_MAX_ITERATIONS) */
+  int32_t dc_retry;             /* GS_RETRY_*: PowerFlowResult.NewtonRetry */
 } gs_nr_result;
 
 typedef struct gs_device_info {

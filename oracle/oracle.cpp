@@ -19,6 +19,10 @@
 //   src/GridSim.Core/Numerics/NumericTolerances.cs  (RelativePivotFloor :36, row floors :99-115, MaxAbs :165,
 //       Ilu0PivotGuard :226-234, UnderflowGuard :242, Krylov cap :256-260)
 //   src/GridSim.Core/Analysis/ContingencyAnalysis.cs  (ScreenBranchesParallel :323-347, WithoutMany :1839-1845)
+//   src/GridSim.Core/PowerFlow/NewtonShell.cs  (Stalled :162,200-206, SolveWithRetry :227-268,
+//       TrySeedDcAngles :282-299, MaxBranchAngleDeg :311-320), SparseNewtonRaphsonPowerFlow.cs (SolveAcAttempt :71-165)
+//   src/GridSim.Core/Analysis/DcPowerFlow.cs :78-86, DcDemand.cs (LegacyView :38-50),
+//       DistributionFactors.cs (Prepare :143-248, IslandsCore :261-360, DcSolve / InjectionRhs :454-500)
 #include "oracle.h"
 
 #include <algorithm>
@@ -862,6 +866,8 @@ struct NewtonOut {
   int iters = 0;
   double max_mis = INFINITY;
   int step_failures = 0;
+  double start_mis = NAN, last_from = NAN;   // NewtonShell.NewtonRunEnd's StartMismatch, BeforeLastStepMismatch
+  bool cap_reached = false;
 };
 
 NewtonOut run_newton(const Ycsr& y, const vector<int>& type, vector<double>& vm, vector<double>& va,
@@ -914,6 +920,7 @@ NewtonOut run_newton(const Ycsr& y, const vector<int>& type, vector<double>& vm,
       if (av > max_abs) max_abs = av;
     }
     if (!finite) break;
+    out.last_from = out.max_mis;   // the retry's stall test only (NewtonShell.Stalled)
     // NewtonShell.ApplyStep
     double damp = max_abs > kMaxUndampedStep ? kMaxUndampedStep / max_abs : 1.0;
     for (int i = 0; i < n; i++) {
@@ -922,6 +929,8 @@ NewtonOut run_newton(const Ycsr& y, const vector<int>& type, vector<double>& vm,
     }
   }
   out.iters = iter;
+  out.start_mis = start_mis;
+  out.cap_reached = !out.converged && !out.diverged && iter >= opt.max_iterations;
   return out;
 }
 
@@ -967,24 +976,121 @@ bool apply_q_limits(const BusSpec& s, vector<int>& eff, const vector<double>& q,
   return changed;
 }
 
-void solve_one(const orc_network& net, int skip, const orc_options& opt, double* vm_out, double* va_out,
-               orc_result* res) {
-  auto t0 = std::chrono::steady_clock::now();
+// ---------------------------------------------------------------- the DC-angle start retry (NewtonShell.cs:142-299)
+constexpr double kRetryStallFactor = 1e-3, kRetryLastStepFactor = 0.5, kMaxRetryBranchAngleDeg = 90.0;
+
+// NewtonShell.Stalled(NewtonRunEnd, double) (:200-206) over Stalled(double, double) (:162).
+bool stalled(const NewtonOut& run, double final_mis) {
+  if (run.cap_reached && run.iters == 0) return false;
+  if (final_mis <= kRetryStallFactor * run.start_mis) return false;
+  if (run.cap_reached && run.iters >= 2 && final_mis <= kRetryLastStepFactor * run.last_from) return false;
+  return true;
+}
+
+// NewtonShell.TrySeedDcAngles (:282-299) with DcPowerFlow.Solve(DcDemand.LegacyView(model)) (DcPowerFlow.cs:78-86,
+// DcDemand.cs:38-50) on DistributionFactors.Prepare / IslandsCore / DcSolve (DistributionFactors.cs:143-360,454-520):
+// the reactance-only B' (every tap 1, every shift 0, every Gs 0: b = 1 / x per in-service branch, stamped in branch
+// order with the reference bus's row and column left out), factored by SparseLu.Factor (minimum degree, explicit zeros
+// dropped, partial pivoting); the right-hand side is each bus's net injection (0 - Pd, then + Pg per generator in
+// generator order) / base; the angles are reduced to [-180, 180] degrees (IEEE remainder) and offset by the slack's
+// start angle. False, spec unchanged, unless the case has exactly one slack bus, its DC model forms a single island
+// (a connected graph over the in-service branches, none of zero reactance) and B' factors.
+bool try_seed_dc_angles(const orc_network& net, BusSpec& spec) {
+  const int n = net.n_bus;
+  int slack = -1, slacks = 0;
+  for (int i = 0; i < n; i++)
+    if (spec.type0[i] == ORC_SLACK) { slack = i; slacks++; }
+  if (slacks != 1) return false;
+  // islands: union-find over the branches the DC model stamps; an in-service zero-reactance branch is refused
+  vector<int> parent(n);
+  for (int i = 0; i < n; i++) parent[i] = i;
+  auto find = [&](int x) { while (parent[x] != x) { parent[x] = parent[parent[x]]; x = parent[x]; } return x; };
+  for (int l = 0; l < net.n_branch; l++) {
+    if (!net.br_in_service[l]) continue;
+    if (net.br_x[l] == 0.0) return false;
+    const int rf = find(net.br_from[l]), rt = find(net.br_to[l]);
+    if (rf != rt) parent[rf] = rt;
+  }
+  for (int i = 0; i < n; i++)
+    if (find(i) != find(slack)) return false;   // more than one island
+  const int nr = n - 1;
+  if (nr == 0) return false;                     // SparseLu.Factor of an empty B' is null
+  vector<int> red(n);
+  for (int i = 0, r = 0; i < n; i++) red[i] = i == slack ? -1 : r++;
+  CsrBuilder bld(nr);
+  for (int l = 0; l < net.n_branch; l++) {
+    if (!net.br_in_service[l]) continue;
+    const int f = net.br_from[l], t = net.br_to[l];
+    const double bl = 1.0 / net.br_x[l];
+    const bool ff = red[f] >= 0, tf = red[t] >= 0;
+    if (ff) bld.add(red[f], red[f], bl);
+    if (tf) bld.add(red[t], red[t], bl);
+    if (ff && tf) { bld.add(red[f], red[t], -bl); bld.add(red[t], red[f], -bl); }
+  }
+  Csr bp = bld.build();
+  vector<int> q = md_order(nr, bp.rp.data(), bp.ci.data());
+  Csc c = to_csc(nr, bp.rp.data(), bp.ci.data(), bp.val.data(), false);
+  bool nonfinite = false;
+  orc_lu* f = gp_factor(c, q.data(), 0.0, &nonfinite);
+  if (!f) return false;
+  // DcPowerFlow.NetInjections: per bus 0 - Pd, then + Pg per generator in generator order (MW)
+  vector<double> inj(n);
+  for (int i = 0; i < n; i++) inj[i] = 0.0 - net.pd[i];
+  for (int g = 0; g < net.n_gen; g++) inj[net.gen_bus[g]] += net.gen_pg[g];
+  vector<double> rhs(nr, 0.0), x(nr);
+  for (int i = 0; i < n; i++)
+    if (red[i] >= 0) rhs[red[i]] += inj[i] / net.base_mva;
+  lu_solve(*f, rhs.data(), x.data());
+  delete f;
+  const double va0 = spec.va[slack];
+  for (int i = 0; i < n; i++) {
+    if (i == slack) continue;
+    const double va_deg = x[red[i]] * 180.0 / kPi;
+    spec.va[i] = va0 + std::remainder(va_deg, 360.0) * kPi / 180.0;
+  }
+  return true;
+}
+
+// NewtonShell.MaxBranchAngleDeg (:311-320): the largest |theta_from - shift - theta_to| (degrees, wrapped) over the
+// in-service branches.
+double max_branch_angle_deg(const orc_network& net, const vector<double>& va) {
+  double worst = 0;
+  for (int l = 0; l < net.n_branch; l++) {
+    if (!net.br_in_service[l]) continue;
+    const double df = va[net.br_from[l]] * 180.0 / kPi, dt = va[net.br_to[l]] * 180.0 / kPi;
+    const double d = std::fabs(std::remainder(df - net.br_shift_deg[l] - dt, 360.0));
+    if (d > worst) worst = d;
+  }
+  return worst;
+}
+
+struct Attempt {
+  orc_result res{};
+  vector<double> vm, va;
+  bool newton_failed = false;
+  NewtonOut end;   // how the last Newton run ended
+};
+
+// SparseNewtonRaphsonPowerFlow.SolveAcAttempt (:71-165): one complete solve (reactive-limit passes included), from the
+// ordinary start or, with dc_start, from the DC angles; false when that start cannot be formed.
+bool solve_attempt(const orc_network& net, int skip, const orc_options& opt, bool dc_start, Attempt& a) {
   Ycsr y = build_y(net, skip);
   BusSpec spec = build_bus_spec(net, opt.flat_start != 0);
+  if (dc_start && !try_seed_dc_angles(net, spec)) return false;
   int n = net.n_bus;
   vector<double> p(n), q(n);
-  // SparseNewtonRaphsonPowerFlow.SolveAcAttempt's reactive-limit outer loop (:118-160)
   vector<int> eff(spec.type0), pinned(n, kPinNone), reverts(n, 0);
   const bool enforce = opt.enforce_q_limits != 0;
   const int passes = enforce ? std::max(1, opt.max_q_passes) : 1;
-  int total = 0;
+  int total = 0, failures = 0;
   bool pending = false;
   NewtonOut o;
   for (int qp = 0; qp < passes; qp++) {
     pending = false;
     o = run_newton(y, eff, spec.vm, spec.va, spec.pspec, spec.qspec, p, q, opt);
     total += o.iters;
+    failures += o.step_failures;
+    a.newton_failed = !o.converged;
     if (!enforce || !o.converged) break;
     pending = apply_q_limits(spec, eff, q, spec.qspec, spec.vm, pinned, reverts, net.base_mva);
     if (!pending) break;
@@ -993,23 +1099,70 @@ void solve_one(const orc_network& net, int skip, const orc_options& opt, double*
     // the cap came right after a switch: solve the switched types once more and report the state not converged
     o = run_newton(y, eff, spec.vm, spec.va, spec.pspec, spec.qspec, p, q, opt);
     total += o.iters;
+    failures += o.step_failures;
+    a.newton_failed = !o.converged;
     o.converged = false;
   }
-  o.iters = total;
+  a.end = o;
   compute_injections(y, spec.vm.data(), spec.va.data(), p.data(), q.data());
   bool finite = std::isfinite(o.max_mis);
   for (int i = 0; i < n && finite; i++)
     finite = std::isfinite(spec.vm[i]) && std::isfinite(spec.va[i]) && std::isfinite(p[i] * net.base_mva) &&
              std::isfinite(q[i] * net.base_mva);
-  res->status = !finite ? ORC_NONFINITE : o.converged ? ORC_CONVERGED : o.diverged ? ORC_DIVERGED
-                                                                                   : ORC_MAX_ITERATIONS;
-  res->iterations = o.iters;
-  res->max_mismatch = o.max_mis;
-  res->step_failures = o.step_failures;
-  res->pad = 0;
-  if (vm_out) std::memcpy(vm_out, spec.vm.data(), sizeof(double) * n);
-  if (va_out) std::memcpy(va_out, spec.va.data(), sizeof(double) * n);
+  // AssembleResult: diverged only for a run the divergence stop ended
+  a.res.status = !finite ? ORC_NONFINITE : o.converged ? ORC_CONVERGED : o.diverged ? ORC_DIVERGED
+                                                                                     : ORC_MAX_ITERATIONS;
+  a.res.iterations = total;
+  a.res.max_mismatch = o.max_mis;
+  a.res.step_failures = failures;
+  a.res.dc_retry = ORC_RETRY_NOT_TRIED;
+  a.vm = std::move(spec.vm);
+  a.va = std::move(spec.va);
+  return true;
+}
+
+// NewtonShell.SolveWithRetry (:227-268): the ordinary solve, then (only when its last Newton run did not converge,
+// stalled or diverged, and the options allow it) the solve from the DC-angle start, whose state is reported when it
+// converged with every branch angle within 90 degrees.
+void solve_one(const orc_network& net, int skip, const orc_options& opt, double* vm_out, double* va_out,
+               orc_result* res) {
+  auto t0 = std::chrono::steady_clock::now();
+  Attempt first;
+  solve_attempt(net, skip, opt, false, first);
+  const Attempt* out = &first;
+  Attempt retry;
+  if (first.res.status != ORC_CONVERGED && first.newton_failed && opt.retry_dc_start && opt.max_iterations > 1 &&
+      stalled(first.end, first.res.max_mismatch)) {
+    if (!solve_attempt(net, skip, opt, true, retry)) {
+      first.res.dc_retry = ORC_RETRY_UNAVAILABLE;
+    } else if (retry.res.status == ORC_CONVERGED) {
+      if (max_branch_angle_deg(net, retry.va) <= kMaxRetryBranchAngleDeg) {
+        retry.res.dc_retry = ORC_RETRY_CONVERGED;
+        out = &retry;
+      } else {
+        first.res.dc_retry = ORC_RETRY_REJECTED;
+      }
+    } else {
+      first.res.dc_retry = ORC_RETRY_DID_NOT_CONVERGE;
+    }
+  }
+  *res = out->res;
+  const int n = net.n_bus;
+  if (vm_out) std::memcpy(vm_out, out->vm.data(), sizeof(double) * n);
+  if (va_out) std::memcpy(va_out, out->va.data(), sizeof(double) * n);
   res->solve_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// A copy of the network with the given branches out of service: the oracle's WithoutMany (ContingencyAnalysis.cs:
+// 1839-1845; the Y-bus builder, the line flows and the DC model all skip an out-of-service branch, so removing a branch
+// and switching it out give the same numbers). `mask` holds the copy's in-service flags.
+orc_network without(const orc_network& net, const int32_t* skips, int n_skip, vector<uint8_t>& mask) {
+  mask.assign(net.br_in_service, net.br_in_service + net.n_branch);
+  for (int k = 0; k < n_skip; k++)
+    if (skips[k] >= 0 && skips[k] < net.n_branch) mask[skips[k]] = 0;
+  orc_network c = net;
+  c.br_in_service = mask.data();
+  return c;
 }
 
 }  // namespace
@@ -1229,6 +1382,32 @@ int32_t orc_bicgstab(int32_t n, const int32_t* rowptr, const int32_t* colidx, co
 void orc_solve(const orc_network* net, int32_t skip, const orc_options* opt, double* vm_out, double* va_out,
                orc_result* res) {
   solve_one(*net, skip, *opt, vm_out, va_out, res);
+}
+
+void orc_screen_sets(const orc_network* net, const int32_t* offsets, const int32_t* branches, int32_t count,
+                     const orc_options* opt, int32_t threads, orc_result* res, double* vm_out, double* va_out) {
+  if (threads < 1) threads = 1;
+  std::atomic<int> next{0};
+  auto worker = [&]() {
+    vector<uint8_t> mask;
+    for (;;) {
+      int k = next.fetch_add(1);
+      if (k >= count) break;
+      size_t off = (size_t)k * net->n_bus;
+      orc_network c = without(*net, branches + offsets[k], offsets[k + 1] - offsets[k], mask);
+      solve_one(c, -1, *opt, vm_out ? vm_out + off : nullptr, va_out ? va_out + off : nullptr, &res[k]);
+    }
+  };
+  vector<std::thread> pool;
+  for (int t = 0; t < threads; t++) pool.emplace_back(worker);
+  for (auto& t : pool) t.join();
+}
+
+int32_t orc_dc_seed(const orc_network* net, int32_t flat_start, double* va) {
+  BusSpec spec = build_bus_spec(*net, flat_start != 0);
+  if (!try_seed_dc_angles(*net, spec)) return 0;
+  std::memcpy(va, spec.va.data(), sizeof(double) * net->n_bus);
+  return 1;
 }
 
 void orc_screen(const orc_network* net, const int32_t* outages, int32_t count, const orc_options* opt,

@@ -40,7 +40,12 @@ typedef struct orc_options {
   int32_t flat_start;
   int32_t enforce_q_limits;  /* PowerFlowOptions.EnforceQLimits: the reactive-limit outer loop */
   int32_t max_q_passes;      /* PowerFlowOptions.MaxQLimitPasses (100) */
+  int32_t retry_dc_start;    /* PowerFlowOptions.RetryFromDcStart: the retry from a DC-angle start */
 } orc_options;
+
+/* NewtonRetryOutcome (PowerFlowResult.cs:24-41). */
+enum { ORC_RETRY_NOT_TRIED = 0, ORC_RETRY_CONVERGED = 1, ORC_RETRY_DID_NOT_CONVERGE = 2, ORC_RETRY_REJECTED = 3,
+       ORC_RETRY_UNAVAILABLE = 4 };
 
 typedef struct orc_result {
   int32_t status;          /* ORC_CONVERGED ... */
@@ -48,7 +53,7 @@ typedef struct orc_result {
   double max_mismatch;
   double solve_seconds;    /* wall time of this solve on its worker thread */
   int32_t step_failures;   /* steps the LU could not solve (ILU0/BiCGSTAB fallback taken) */
-  int32_t pad;
+  int32_t dc_retry;        /* ORC_RETRY_*: PowerFlowResult.NewtonRetry */
 } orc_result;
 
 /* FDLIBM sincos over an array. */
@@ -105,7 +110,7 @@ int32_t orc_ilu0(int32_t n, const int32_t* rowptr, const int32_t* colidx, const 
 int32_t orc_bicgstab(int32_t n, const int32_t* rowptr, const int32_t* colidx, const double* val, const double* b,
                      double tol, int32_t max_iter, double* x);
 
-/* One power-flow solve (SparseNewtonRaphsonPowerFlow.Solve with Q limits and the DC retry off).
+/* One power-flow solve (SparseNewtonRaphsonPowerFlow.Solve: reactive limits and the DC-start retry as the options say).
  * vm_out / va_out (rad) receive the final state, may be null. */
 void orc_solve(const orc_network* net, int32_t skip, const orc_options* opt, double* vm_out, double* va_out,
                orc_result* res);
@@ -114,6 +119,13 @@ void orc_solve(const orc_network* net, int32_t skip, const orc_options* opt, dou
  * per worker, one outage per task). vm_out / va_out may be null, else count * n_bus each. */
 void orc_screen(const orc_network* net, const int32_t* outages, int32_t count, const orc_options* opt,
                 int32_t threads, orc_result* res, double* vm_out, double* va_out);
+
+/* N-k screen: outage set k is branches[offsets[k] .. offsets[k + 1]) (WithoutMany's index list); as orc_screen. */
+void orc_screen_sets(const orc_network* net, const int32_t* offsets, const int32_t* branches, int32_t count,
+                     const orc_options* opt, int32_t threads, orc_result* res, double* vm_out, double* va_out);
+
+/* NewtonShell.TrySeedDcAngles on the ordinary start: 1 and va (rad, n_bus) the seeded angles, or 0 (unavailable). */
+int32_t orc_dc_seed(const orc_network* net, int32_t flat_start, double* va);
 
 #ifdef __cplusplus
 }

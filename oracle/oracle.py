@@ -29,16 +29,19 @@ class OrcNetwork(C.Structure):
 
 class OrcOptions(C.Structure):
     _fields_ = [("tolerance", C.c_double), ("max_iterations", C.c_int32), ("diverge_limit", C.c_double),
-                ("flat_start", C.c_int32), ("enforce_q_limits", C.c_int32), ("max_q_passes", C.c_int32)]
+                ("flat_start", C.c_int32), ("enforce_q_limits", C.c_int32), ("max_q_passes", C.c_int32),
+                ("retry_dc_start", C.c_int32)]
 
 
 class OrcResult(C.Structure):
     _fields_ = [("status", C.c_int32), ("iterations", C.c_int32), ("max_mismatch", C.c_double),
-                ("solve_seconds", C.c_double), ("step_failures", C.c_int32), ("pad", C.c_int32)]
+                ("solve_seconds", C.c_double), ("step_failures", C.c_int32), ("dc_retry", C.c_int32)]
 
 
 RESULT_DTYPE = np.dtype([("status", np.int32), ("iterations", np.int32), ("max_mismatch", np.float64),
-                         ("solve_seconds", np.float64), ("step_failures", np.int32), ("pad", np.int32)])
+                         ("solve_seconds", np.float64), ("step_failures", np.int32), ("dc_retry", np.int32)])
+# NewtonRetryOutcome
+RETRY_NOT_TRIED, RETRY_CONVERGED, RETRY_DID_NOT_CONVERGE, RETRY_REJECTED, RETRY_UNAVAILABLE = 0, 1, 2, 3, 4
 
 _lib = None
 
@@ -86,9 +89,19 @@ class _NetView:
 
 
 def options(tolerance=1e-8, max_iterations=30, diverge_limit=10.0, flat_start=False, q_limits=False,
-            max_q_passes=100):
+            max_q_passes=100, retry_dc_start=False):
     return OrcOptions(tolerance, max_iterations, diverge_limit, 1 if flat_start else 0, 1 if q_limits else 0,
-                      max_q_passes)
+                      max_q_passes, 1 if retry_dc_start else 0)
+
+
+def without(net, branches):
+    """The network with the given branches out of service (WithoutMany: every routine here skips an out-of-service
+    branch exactly as it skips a removed one)."""
+    mask = np.array(net.br_in_service, dtype=np.uint8, copy=True)
+    for b in np.atleast_1d(np.asarray(branches, np.int64)):
+        if b >= 0:
+            mask[int(b)] = 0
+    return net.copy(br_in_service=mask)
 
 
 def sincos(x):
@@ -245,8 +258,10 @@ def bicgstab(rowptr, colidx, val, b, tol=1e-10, max_iter=None):
 
 
 def solve(net, opts=None, skip=-1):
-    """One solve: (result record, vm, va in radians)."""
+    """One solve: (result record, vm, va in radians). skip: one outaged branch, or a sequence of them (N-k)."""
     opts = opts or options()
+    if np.ndim(skip) > 0:
+        net, skip = without(net, skip), -1
     vm = np.empty(net.n_bus)
     va = np.empty(net.n_bus)
     res = OrcResult()
@@ -258,8 +273,21 @@ def solve(net, opts=None, skip=-1):
 
 
 def screen(net, outages, opts=None, threads=1, keep_state=True):
-    """N-1 screen: (results [count] structured array, vm [count, n] or None, va [count, n] or None)."""
+    """N-1 screen: (results [count] structured array, vm [count, n] or None, va [count, n] or None). outages: one
+    branch per solve, or per solve a sequence of branches (N-k, WithoutMany's index list)."""
     opts = opts or options()
+    if any(np.ndim(o) > 0 for o in list(outages)):
+        sets = [np.atleast_1d(np.asarray(o, np.int32)) for o in outages]
+        offsets = np.zeros(len(sets) + 1, np.int32)
+        offsets[1:] = np.cumsum([len(x) for x in sets])
+        flat = np.ascontiguousarray(np.concatenate(sets) if sets else np.zeros(0, np.int32), np.int32)
+        count = len(sets)
+        res = np.zeros(count, RESULT_DTYPE)
+        vm = np.empty((count, net.n_bus)) if keep_state else None
+        va = np.empty((count, net.n_bus)) if keep_state else None
+        lib().orc_screen_sets(_NetView(net).ref, _p(offsets), _p(flat), C.c_int32(count), C.byref(opts),
+                              C.c_int32(threads), _p(res), _p(vm), _p(va))
+        return res, vm, va
     outages = np.ascontiguousarray(outages, np.int32)
     count = len(outages)
     res = np.zeros(count, RESULT_DTYPE)
@@ -270,8 +298,17 @@ def screen(net, outages, opts=None, threads=1, keep_state=True):
     return res, vm, va
 
 
+def dc_seed(net, flat_start=False):
+    """NewtonShell.TrySeedDcAngles on the ordinary start: the seeded angles (rad), or None when unavailable."""
+    va = np.empty(net.n_bus)
+    ok = lib().orc_dc_seed(_NetView(net).ref, C.c_int32(1 if flat_start else 0), _p(va))
+    return va if ok else None
+
+
 def line_flows(net, vm, va, skip=-1):
     """NewtonShell.ComputeLineFlows: (pf, qf, pt, qt in MW / MVAr, loading percent of rate A), per branch."""
+    if np.ndim(skip) > 0:
+        net, skip = without(net, skip), -1
     nb = net.n_branch
     out = [np.zeros(nb) for _ in range(5)]
     rate = np.ascontiguousarray(net.br_rate_a, np.float64)
@@ -287,6 +324,8 @@ SUMMARY_FIELDS = ("max_loading_pct", "overloads", "vmin", "vmax", "undervoltage"
 def screen_summary(net, vm, va, skip=-1):
     """The classification numbers of a converged state (ScreenStateCore, each bus against its own limits where the
     network gives them, as the CLI's default) and AssembleResult's losses, as a dict."""
+    if np.ndim(skip) > 0:
+        net, skip = without(net, skip), -1
     vm = np.ascontiguousarray(vm, np.float64)
     va = np.ascontiguousarray(va, np.float64)
     p, q, _ = mismatch(net, vm, va, skip=skip)

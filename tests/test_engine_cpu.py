@@ -35,8 +35,8 @@ def test_library_exports_every_declared_symbol(eng):
 def test_struct_sizes_match_the_header(eng):
     """The ctypes mirrors of the C structs have the C layout (the P/Invoke shape test of RocmInteropShapeTests)."""
     assert C.sizeof(eng.GsNrOptions) == 48
-    assert C.sizeof(eng.GsInstance) == 32
-    assert eng.RESULT_DTYPE.itemsize == 64
+    assert C.sizeof(eng.GsInstance) == 40
+    assert eng.RESULT_DTYPE.itemsize == 72
     assert C.sizeof(eng.GsCsr) == 32
     assert C.sizeof(eng.GsNetwork) == 4 * 3 + 4 + 8 + 24 * 8
     assert eng.SUMMARY_DTYPE.itemsize == 56
