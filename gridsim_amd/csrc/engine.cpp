@@ -214,13 +214,15 @@ gs_plan_stats plan_stats(const DevicePlanHost& d, const YBus& y, const JacobianP
 // (FixedPatternSparseLuSolver.cs:52,75-99).
 constexpr double kDiagonalPreference = 0.1;
 
+// growth_tol: the growth check the plan's pivots will face; the diagonal is preferred only where it also passes that
+// check, so a plan always passes it at the state it was made at.
 bool symbolic_pass(const YBus& y, const JacobianPattern& jp, const double* vm, const double* va, Plan& plan,
-                   std::string& why) {
+                   std::string& why, double growth_tol) {
   std::vector<double> p(y.n), q(y.n), jv(jp.rp[jp.m]);
   host_injections(y, vm, va, p.data(), q.data());
   host_jacobian(y, jp, vm, va, p.data(), q.data(), jv.data());
   std::vector<int> order = minimum_degree(jp.m, jp.rp.data(), jp.ci.data());
-  return gp_plan(jp.m, jp.rp, jp.ci, jv.data(), order, kDiagonalPreference, true, plan, why);
+  return gp_plan(jp.m, jp.rp, jp.ci, jv.data(), order, std::max(kDiagonalPreference, growth_tol), true, plan, why);
 }
 
 gs_nr_options normalise(const gs_nr_options* o) {
@@ -235,7 +237,8 @@ gs_nr_options normalise(const gs_nr_options* o) {
   r.schedule = 0;
   r.flags = 0;
   if (o) r = *o;
-  if (r.flags & ~(GS_FLAG_X_GLOBAL | GS_FLAG_Q_LIMITS | GS_FLAG_SINGLE_WAVE)) fail(GS_INVALID_ARGUMENT, "unknown flags");
+  if (r.flags & ~(GS_FLAG_X_GLOBAL | GS_FLAG_Q_LIMITS | GS_FLAG_SINGLE_WAVE | GS_FLAG_DC_RETRY))
+    fail(GS_INVALID_ARGUMENT, "unknown flags");
   if (r.schedule < GS_SCHEDULE_LEVELS) fail(GS_INVALID_ARGUMENT, "schedule must be >= 0 or GS_SCHEDULE_LEVELS");
   if (!(r.tolerance > 0)) fail(GS_INVALID_ARGUMENT, "tolerance must be positive");
   if (r.max_iterations < 0) fail(GS_INVALID_ARGUMENT, "max_iterations must be >= 0");
@@ -370,8 +373,10 @@ BatchRun plan_run(const DevPlan& h, const gs_nr_options& o, bool force_fp64) {
 // Runs `count` instances on one plan. Fills results, optional vm/va. Handles paused instances (pivot growth) by a
 // fresh host symbolic pass on the instance's own reduced network and a resumed solve on that plan.
 struct HostFinal;
+// (arena: the device buffers the instances are laid out in, or null: the network's own)
 void run_batch(gs_net* net, int count, const gs_instance* inst, const gs_nr_options& o, gs_nr_result* results,
-               double* vm_out, double* va_out, bool plain, std::vector<HostFinal>* finals = nullptr);
+               double* vm_out, double* va_out, bool plain, std::vector<HostFinal>* finals = nullptr,
+               std::vector<InstState>* final_states = nullptr, RunArena* arena = nullptr);
 
 void fill_result(const InstState& s, gs_nr_result& r, int refreshes, double us_per_tick) {
   r.status = s.nonfinite ? GS_PF_NONFINITE : s.converged ? GS_PF_CONVERGED : s.diverged ? GS_PF_DIVERGED
@@ -609,17 +614,21 @@ int launch_wide(DevicePlan& dp, const PreparedRun& pr, hipStream_t s) {
 }
 
 // Core launcher for instances sharing one device plan. `states` may be pre-filled (resume) when init == 0.
+// trust_first: the plan was made at the instances' start state (a fresh symbolic pass), so their first step skips the
+// growth and floor checks.
 double launch_instances(gs_ctx* ctx, DevicePlan& dp, DevMem& inst_buf, DevMem& desc_buf, DevMem& state_buf,
                         DevMem& aux_buf, int count, const gs_instance* inst, const YBus* y, const Network* netw,
                         const BusSpec& base_spec, const gs_nr_options& o, bool plain,
                         std::vector<InstState>& states, bool init,
                         const std::vector<std::vector<double>>* spec_override,
-                        const std::vector<std::vector<double>>* start_override, int* launches) {
+                        const std::vector<std::vector<double>>* start_override, int* launches,
+                        bool trust_first = false) {
   hipStream_t s = ctx->stream;
   const DevPlan& h = dp.h;
   Staging stg;
   PreparedRun pr = prepare_instances(ctx, dp, inst_buf, desc_buf, state_buf, aux_buf, stg, count, inst, y, netw,
                                      base_spec, o, plain, init ? nullptr : &states, spec_override, start_override, s);
+  pr.prm.trust_first = trust_first ? 1 : 0;
   const BatchRun& run = pr.run;
   NrParams prm = pr.prm;
   InstState* dstate = state_buf.as<InstState>();
@@ -643,6 +652,7 @@ double launch_instances(gs_ctx* ctx, DevicePlan& dp, DevMem& inst_buf, DevMem& d
       for (;;) {
         prm.init = (first && init) ? 1 : 0;
         prm.load_start = first ? load_start : 0;
+        prm.trust_first = first ? pr.prm.trust_first : 0;
         first = false;
         check(launch_nr_batch(0, run.x_lds, run.pair, desc_buf.as<InstDesc>() + i, h, 1, prm, run.lds_bytes, s),
               "plain kernel");
@@ -670,6 +680,18 @@ void copy_state_out(gs_ctx* ctx, const DevMem& inst_buf, const Layout& L, int i,
   const char* b = inst_buf.as<char>() + L.stride * i;
   if (vm) check(hipMemcpyAsync(vm, b + L.vm, 8 * (size_t)n, hipMemcpyDeviceToHost, ctx->stream), "vm out");
   if (va) check(hipMemcpyAsync(va, b + L.va, 8 * (size_t)n, hipMemcpyDeviceToHost, ctx->stream), "va out");
+}
+
+// The network an instance solves: the uploaded one with its outaged branches out of service and every load and
+// generator P scaled by its load multiplier.
+Network instance_network(const Network& base, const gs_instance& in) {
+  Network own = base;
+  for (int br : outage_set(in, own.nb)) own.br_in[br] = 0;
+  if (in.load_scale != 1.0) {
+    for (int b = 0; b < own.n; b++) { own.pd[b] *= in.load_scale; own.qd[b] *= in.load_scale; }
+    for (int g = 0; g < own.ng; g++) own.gen_pg[g] *= in.load_scale;
+  }
+  return own;
 }
 
 // A paused instance (a static pivot failed its growth check): fresh CPU symbolic passes on the instance's own network
@@ -776,16 +798,11 @@ void iterative_step(gs_ctx* ctx, const YBus& y, const JacobianPattern& jp, const
 void finish_paused(gs_net* net, const gs_instance& in, const gs_nr_options& o, bool plain, InstState st,
                    std::vector<double>& vm, std::vector<double>& va, gs_nr_result& result, double& kms, int& launches,
                    std::vector<double>* p_out, std::vector<double>* q_out, const std::vector<int>* types,
-                   const std::vector<double>* qspec) {
+                   const std::vector<double>* qspec, InstState* st_out) {
   gs_ctx* ctx = net->ctx;
   const int n = net->net.n;
   int refreshes = 0;
-  Network own = net->net;
-  for (int br : outage_set(in, own.nb)) own.br_in[br] = 0;
-  if (in.load_scale != 1.0) {
-    for (int b = 0; b < own.n; b++) { own.pd[b] *= in.load_scale; own.qd[b] *= in.load_scale; }
-    for (int g = 0; g < own.ng; g++) own.gen_pg[g] *= in.load_scale;
-  }
+  Network own = instance_network(net->net, in);
   YBus y = build_ybus(own, -1);
   BusSpec spec = build_bus_spec(own, o.flat_start != 0, 1.0);
   if (types) spec.type = *types;
@@ -798,7 +815,7 @@ void finish_paused(gs_net* net, const gs_instance& in, const gs_nr_options& o, b
   while (!finished) {
     Plan plan;
     std::string why;
-    if (!symbolic_pass(y, jp, vm.data(), va.data(), plan, why)) {
+    if (!symbolic_pass(y, jp, vm.data(), va.data(), plan, why, o.growth_tol)) {
       // singular at this state: the LU refuses the step, so SolveStep falls back to ILU(0) + BiCGSTAB for it (on the
       // device); the LU is tried again at the next step
       if (++iterative > o.max_iterations + 1) {
@@ -811,7 +828,8 @@ void finish_paused(gs_net* net, const gs_instance& in, const gs_nr_options& o, b
       finished = st.done == 1;
       continue;
     }
-    if (++refreshes > 8) {
+    // (a fresh plan takes at least the step at the state it was made at, so this bound is a safety net only)
+    if (++refreshes > o.max_iterations + 2) {
       // the fresh plans keep failing their growth checks: the run stops
       st.done = 1;
       st.step_failures++;
@@ -832,7 +850,7 @@ void finish_paused(gs_net* net, const gs_instance& in, const gs_nr_options& o, b
     gs_nr_options ow = o;
     ow.schedule = 0;  // a resumed run is one instance on one wavefront
     kms += launch_instances(ctx, dp, ib, db, sb, ab, 1, &one, nullptr, nullptr, spec, ow, plain, rs, false, nullptr,
-                            &start, &launches);
+                            &start, &launches, true);
     st = rs[0];
     BatchRun r2 = plan_run(dp.h, ow, plain);
     Layout L2 = instance_layout(dp.h, r2.fp32, r2.x_lds);
@@ -851,6 +869,7 @@ void finish_paused(gs_net* net, const gs_instance& in, const gs_nr_options& o, b
     finished = st.done == 1;
   }
   fill_result(st, result, refreshes, ctx->us_per_tick);
+  if (st_out) *st_out = st;
 }
 
 // Results of a finished device run of `count` instances (states copied back): finished instances straight from the
@@ -864,7 +883,8 @@ struct HostFinal {
 void collect_results(gs_net* net, int count, const gs_instance* inst, const gs_nr_options& o, bool plain,
                      const std::vector<InstState>& states, const DevMem& inst_buf, const Layout& L,
                      gs_nr_result* results, double* vm_out, double* va_out, double& kms, int& launches,
-                     std::vector<HostFinal>* finals = nullptr) {
+                     std::vector<HostFinal>* finals = nullptr, std::vector<InstState>* final_states = nullptr) {
+  if (final_states) *final_states = states;
   gs_ctx* ctx = net->ctx;
   const int n = net->net.n;
   for (int i = 0; i < count; i++)
@@ -884,7 +904,7 @@ void collect_results(gs_net* net, int count, const gs_instance* inst, const gs_n
     copy_state_out(ctx, inst_buf, L, i, n, vm.data(), va.data());
     check(hipStreamSynchronize(ctx->stream), "sync pause");
     finish_paused(net, inst[i], o, plain, states[i], vm, va, results[i], kms, launches, finals ? &p : nullptr,
-                  finals ? &q : nullptr);
+                  finals ? &q : nullptr, nullptr, nullptr, final_states ? &(*final_states)[i] : nullptr);
     if (finals) finals->push_back(HostFinal{i, vm, va, p, q});
     if (vm_out) std::copy(vm.begin(), vm.end(), vm_out + (size_t)n * i);
     if (va_out) std::copy(va.begin(), va.end(), va_out + (size_t)n * i);
@@ -892,18 +912,19 @@ void collect_results(gs_net* net, int count, const gs_instance* inst, const gs_n
 }
 
 void run_batch(gs_net* net, int count, const gs_instance* inst, const gs_nr_options& o, gs_nr_result* results,
-               double* vm_out, double* va_out, bool plain, std::vector<HostFinal>* finals) {
+               double* vm_out, double* va_out, bool plain, std::vector<HostFinal>* finals,
+               std::vector<InstState>* final_states, RunArena* arena) {
   gs_ctx* ctx = net->ctx;
+  RunArena& a = arena ? *arena : net->arena;
   std::vector<InstState> states;
   auto t0 = std::chrono::steady_clock::now();
   int launches = 0;
-  double kms = launch_instances(ctx, net->dev, net->inst_buf, net->desc_buf, net->state_buf, net->aux_buf, count,
-                                inst, &net->y, &net->net, net->spec, o, plain, states, true, nullptr, nullptr,
-                                &launches);
+  double kms = launch_instances(ctx, net->dev, a.inst_buf, a.desc_buf, a.state_buf, a.aux_buf, count, inst, &net->y,
+                                &net->net, net->spec, o, plain, states, true, nullptr, nullptr, &launches);
   BatchRun run = plan_run(net->dev.h, o, plain);
   Layout L = instance_layout(net->dev.h, run.fp32, run.x_lds);
-  collect_results(net, count, inst, o, plain, states, net->inst_buf, L, results, vm_out, va_out, kms, launches,
-                  finals);
+  collect_results(net, count, inst, o, plain, states, a.inst_buf, L, results, vm_out, va_out, kms, launches, finals,
+                  final_states);
   net->last_kernel_ms = kms;
   net->last_launches = launches;
   net->last_total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -976,11 +997,12 @@ void add_run(gs_nr_result& total, const gs_nr_result& run) {
 
 }  // namespace
 
-// After run_batch (the first run of every instance, its blocks still in net->inst_buf with layout L and paused
+// After run_batch (the first run of every instance, its blocks still in inst_buf with layout L and paused
 // instances' finals in `paused`): the reactive-limit passes. Returns the final state of every instance that ran
 // again.
 std::vector<HostFinal> q_limit_passes(gs_net* net, int count, const gs_instance* inst, const gs_nr_options& o,
-                                      gs_nr_result* results, const Layout& L, const std::vector<HostFinal>& paused) {
+                                      gs_nr_result* results, const Layout& L, const std::vector<HostFinal>& paused,
+                                      const DevMem& inst_buf, std::vector<InstState>* final_states) {
   gs_ctx* ctx = net->ctx;
   hipStream_t s = ctx->stream;
   const int n = net->net.n;
@@ -990,7 +1012,7 @@ std::vector<HostFinal> q_limit_passes(gs_net* net, int count, const gs_instance*
   const size_t offs[4] = {L.vm, L.va, L.p, L.q};
   double* dsts[4] = {vm.data(), va.data(), pp.data(), qq.data()};
   for (int w = 0; w < 4; w++)
-    check(hipMemcpy2DAsync(dsts[w], 8 * (size_t)n, net->inst_buf.as<char>() + offs[w], L.stride, 8 * (size_t)n,
+    check(hipMemcpy2DAsync(dsts[w], 8 * (size_t)n, inst_buf.as<char>() + offs[w], L.stride, 8 * (size_t)n,
                            (size_t)count, hipMemcpyDeviceToHost, s), "state for the limits");
   check(hipStreamSynchronize(s), "state for the limits");
   std::map<int, const HostFinal*> host_done;
@@ -1028,13 +1050,21 @@ std::vector<HostFinal> q_limit_passes(gs_net* net, int count, const gs_instance*
       JacobianPattern jp = build_jacobian_pattern(net->y, types);
       Plan plan;
       std::string why;
-      if (!symbolic_pass(net->y, jp, st[ids[0]].vm.data(), st[ids[0]].va.data(), plan, why)) {
+      if (!symbolic_pass(net->y, jp, st[ids[0]].vm.data(), st[ids[0]].va.data(), plan, why, og.growth_tol)) {
         // singular at this state: the run cannot take a step (NewtonShell.SolveStep false) and stops there
         for (int i : ids) {
           results[i].status = GS_PF_MAX_ITERATIONS;
           results[i].step_failures++;
           QState& q = st[i];
           out[i] = HostFinal{i, q.vm, q.va, q.p, q.q};
+          if (final_states) {
+            // the pass could not take its first step (NewtonShell.SolveStep false): a run of no steps that ended at
+            // the mismatch it started from
+            InstState& e = (*final_states)[i];
+            e.iter = 0; e.converged = 0; e.diverged = 0;
+            e.start_mis = results[i].max_mismatch;
+            e.last_from = __builtin_nan("");
+          }
         }
         continue;
       }
@@ -1056,6 +1086,8 @@ std::vector<HostFinal> q_limit_passes(gs_net* net, int count, const gs_instance*
       DevMem ib, db, sb, ab;
       std::vector<InstState> states;
       int launches = 0;
+      // (the plan was made at the first instance's state: only that instance's first step is the plan's own, so
+      // the group's steps are checked as usual)
       double kms = launch_instances(ctx, dp, ib, db, sb, ab, cnt, gi.data(), &net->y, &net->net, gspec, og, false,
                                     states, true, &spec_ov, &start_ov, &launches);
       const BatchRun rg = plan_run(dp.h, og, false);
@@ -1070,17 +1102,19 @@ std::vector<HostFinal> q_limit_passes(gs_net* net, int count, const gs_instance*
         for (int w = 0; w < 4; w++)
           check(hipMemcpyAsync(dst[w], b + src[w], 8 * (size_t)n, hipMemcpyDeviceToHost, s), "state back");
         check(hipStreamSynchronize(s), "state back");
+        InstState run_end = states[k];
         if (states[k].done == 1) {
           fill_result(states[k], r, 0, ctx->us_per_tick);
         } else {
           std::vector<double> p2, q2;
           finish_paused(net, gi[k], og, false, states[k], q.vm, q.va, r, kms, launches, &p2, &q2, &types,
-                        &q.spec.qspec);
+                        &q.spec.qspec, &run_end);
           if ((int)p2.size() == n) q.p = p2;
           if ((int)q2.size() == n) q.q = q2;
         }
         add_run(results[i], r);
         out[i] = HostFinal{i, q.vm, q.va, q.p, q.q};
+        if (final_states) (*final_states)[i] = run_end;
         if (last) {
           // the cap came right after a switch: this run's state is reported, not converged
           if (results[i].status == GS_PF_CONVERGED) results[i].status = GS_PF_MAX_ITERATIONS;
@@ -1096,6 +1130,144 @@ std::vector<HostFinal> q_limit_passes(gs_net* net, int count, const gs_instance*
   std::vector<HostFinal> finals;
   for (auto& kv : out) finals.push_back(std::move(kv.second));
   return finals;
+}
+
+// ================================================================================================ DC-start retry
+// NewtonShell.SolveWithRetry (NewtonShell.cs:227-268): an instance whose last Newton run did not converge, and stalled
+// or diverged rather than being stopped by the iteration cap while still converging (Stalled :162,200-206), is solved
+// again from its ordinary start magnitudes and the DC angles of its own network (TrySeedDcAngles :282-299), with the
+// same options; the retry's state is reported when it converges with every branch angle within MaxRetryBranchAngleDeg
+// (:142), otherwise the first run's, with the outcome in gs_nr_result.dc_retry.
+constexpr double kRetryStallFactor = 1e-3, kRetryLastStepFactor = 0.5, kMaxRetryBranchAngleDeg = 90.0;
+
+// The retry's stall test on how the last Newton run ended (its final state e) and its final mismatch.
+bool stalled(const InstState& e, double final_mis, int max_iterations) {
+  const bool cap = !e.converged && !e.diverged && e.iter >= max_iterations;
+  if (cap && e.iter == 0) return false;
+  if (final_mis <= kRetryStallFactor * e.start_mis) return false;
+  if (cap && e.iter >= 2 && final_mis <= kRetryLastStepFactor * e.last_from) return false;
+  return true;
+}
+
+std::vector<HostFinal> solve_all(gs_net* net, int count, const gs_instance* inst, const gs_nr_options& o,
+                                 gs_nr_result* results, double* vm_out, double* va_out, RunArena* arena);
+
+// The retry of every instance that qualifies (ends: the final state of each instance's last Newton run). Returns the
+// final states of the instances whose retry is reported.
+std::vector<HostFinal> dc_retry_pass(gs_net* net, int count, const gs_instance* inst, const gs_nr_options& o,
+                                     gs_nr_result* results, const std::vector<InstState>& ends) {
+  const int n = net->net.n;
+  std::vector<int> ids;
+  std::vector<Network> nets;
+  std::vector<std::vector<double>> starts;   // vm | va of each retried instance
+  for (int i = 0; i < count; i++) {
+    if (results[i].status == GS_PF_CONVERGED || ends[i].converged || o.max_iterations <= 1) continue;
+    if (!stalled(ends[i], results[i].max_mismatch, o.max_iterations)) continue;
+    Network own = instance_network(net->net, inst[i]);
+    const BusSpec spec = build_bus_spec(own, net->flat_start, 1.0);
+    std::vector<double> vm = inst[i].vm0 ? std::vector<double>(inst[i].vm0, inst[i].vm0 + n) : spec.vm;
+    std::vector<double> va = inst[i].va0 ? std::vector<double>(inst[i].va0, inst[i].va0 + n) : spec.va;
+    if (!dc_seed_angles(own, spec.type, va)) {
+      results[i].dc_retry = GS_RETRY_UNAVAILABLE;
+      continue;
+    }
+    vm.insert(vm.end(), va.begin(), va.end());
+    ids.push_back(i);
+    nets.push_back(std::move(own));
+    starts.push_back(std::move(vm));
+  }
+  if (ids.empty()) return {};
+  const int k = (int)ids.size();
+  std::vector<gs_instance> ri(k);
+  for (int j = 0; j < k; j++) {
+    ri[j] = inst[ids[j]];
+    ri[j].vm0 = starts[j].data();
+    ri[j].va0 = starts[j].data() + n;
+  }
+  gs_nr_options ro = o;
+  ro.flags &= ~GS_FLAG_DC_RETRY;
+  if (ro.schedule == GS_SCHEDULE_LEVELS) ro.schedule = 0;   // a handful of instances: a wavefront each
+  std::vector<gs_nr_result> rr(k);
+  std::vector<double> rvm((size_t)k * n), rva((size_t)k * n);
+  RunArena arena;
+  const double kms = net->last_kernel_ms;
+  const int nl = net->last_launches;
+  std::map<int, HostFinal> fin;
+  for (HostFinal& f : solve_all(net, k, ri.data(), ro, rr.data(), rvm.data(), rva.data(), &arena))
+    fin[f.i] = std::move(f);
+  net->last_kernel_ms += kms;
+  net->last_launches += nl;
+  // the injections of the retried instances the device finished, from their blocks
+  const BatchRun run = plan_run(net->dev.h, ro, false);
+  const Layout L = instance_layout(net->dev.h, run.fp32, run.x_lds);
+  std::vector<double> pp((size_t)k * n), qq((size_t)k * n);
+  hipStream_t s = net->ctx->stream;
+  check(hipMemcpy2DAsync(pp.data(), 8 * (size_t)n, arena.inst_buf.as<char>() + L.p, L.stride, 8 * (size_t)n,
+                         (size_t)k, hipMemcpyDeviceToHost, s), "retry p");
+  check(hipMemcpy2DAsync(qq.data(), 8 * (size_t)n, arena.inst_buf.as<char>() + L.q, L.stride, 8 * (size_t)n,
+                         (size_t)k, hipMemcpyDeviceToHost, s), "retry q");
+  check(hipStreamSynchronize(s), "retry injections");
+  std::vector<HostFinal> out;
+  for (int j = 0; j < k; j++) {
+    const int i = ids[j];
+    if (rr[j].status != GS_PF_CONVERGED) {
+      results[i].dc_retry = GS_RETRY_DID_NOT_CONVERGE;
+      continue;
+    }
+    const double* va = rva.data() + (size_t)j * n;
+    if (max_branch_angle_deg(nets[j], va) > kMaxRetryBranchAngleDeg) {
+      results[i].dc_retry = GS_RETRY_REJECTED;
+      continue;
+    }
+    results[i] = rr[j];
+    results[i].dc_retry = GS_RETRY_CONVERGED;
+    HostFinal f{i, std::vector<double>(rvm.begin() + (size_t)j * n, rvm.begin() + (size_t)(j + 1) * n),
+                std::vector<double>(va, va + n), {}, {}};
+    auto it = fin.find(j);
+    if (it != fin.end() && (int)it->second.p.size() == n) {
+      f.p = it->second.p;
+      f.q = it->second.q;
+    } else {
+      f.p.assign(pp.begin() + (size_t)j * n, pp.begin() + (size_t)(j + 1) * n);
+      f.q.assign(qq.begin() + (size_t)j * n, qq.begin() + (size_t)(j + 1) * n);
+    }
+    out.push_back(std::move(f));
+  }
+  return out;
+}
+
+// The whole solve of a call: the first run of every instance (laid out in `arena`, or the network's own when null),
+// then, as the flags ask, the reactive-limit passes and the DC-start retry. vm_out / va_out (when non-null) get every
+// instance's final state; the returned list holds the final states (vm, va, and p, q when known) of the instances the
+// host finished or ran again, whose device blocks still hold their first run.
+std::vector<HostFinal> solve_all(gs_net* net, int count, const gs_instance* inst, const gs_nr_options& o,
+                                 gs_nr_result* results, double* vm_out, double* va_out, RunArena* arena) {
+  const bool qlim = o.flags & GS_FLAG_Q_LIMITS, retry = o.flags & GS_FLAG_DC_RETRY;
+  RunArena& a = arena ? *arena : net->arena;
+  std::vector<HostFinal> first;
+  std::vector<InstState> ends;
+  run_batch(net, count, inst, o, results, vm_out, va_out, false, &first, retry ? &ends : nullptr, &a);
+  std::map<int, HostFinal> fin;
+  for (HostFinal& f : first) fin[f.i] = std::move(f);
+  if (qlim) {
+    const BatchRun run = plan_run(net->dev.h, o, false);
+    const Layout L = instance_layout(net->dev.h, run.fp32, run.x_lds);
+    std::vector<HostFinal> paused;
+    for (auto& kv : fin) paused.push_back(kv.second);
+    for (HostFinal& f : q_limit_passes(net, count, inst, o, results, L, paused, a.inst_buf, retry ? &ends : nullptr))
+      fin[f.i] = std::move(f);
+  }
+  if (retry)
+    for (HostFinal& f : dc_retry_pass(net, count, inst, o, results, ends)) fin[f.i] = std::move(f);
+  const int n = net->net.n;
+  std::vector<HostFinal> out;
+  for (auto& kv : fin) {
+    const HostFinal& f = kv.second;
+    if (vm_out) std::copy(f.vm.begin(), f.vm.end(), vm_out + (size_t)n * f.i);
+    if (va_out) std::copy(f.va.begin(), f.va.end(), va_out + (size_t)n * f.i);
+    out.push_back(std::move(kv.second));
+  }
+  return out;
 }
 
 // ================================================================================================ seam solves
@@ -1470,7 +1642,7 @@ GS_API gs_status gs_network_upload(gs_ctx* ctx, const gs_network* net, const gs_
     h->spec = build_bus_spec(h->net, h->flat_start, 1.0);
     h->jp = build_jacobian_pattern(h->y, h->spec.type);
     std::string why;
-    if (!symbolic_pass(h->y, h->jp, h->spec.vm.data(), h->spec.va.data(), h->plan, why)) {
+    if (!symbolic_pass(h->y, h->jp, h->spec.vm.data(), h->spec.va.data(), h->plan, why, o.growth_tol)) {
       ctx->err = why;
       return GS_SINGULAR;
     }
@@ -1493,19 +1665,7 @@ GS_API gs_status gs_nr_solve_batch(gs_net* net, int32_t count, const gs_instance
     if ((o.flat_start != 0) != net->flat_start) fail(GS_INVALID_ARGUMENT, "flat_start differs from the upload");
     check_instances(net->net.nb, count, inst, "");
     if (count == 0) return GS_OK;
-    if (!(o.flags & GS_FLAG_Q_LIMITS)) {
-      run_batch(net, count, inst, o, results, vm_out, va_out, false);
-      return GS_OK;
-    }
-    std::vector<HostFinal> paused;
-    run_batch(net, count, inst, o, results, vm_out, va_out, false, &paused);
-    const BatchRun run = plan_run(net->dev.h, o, false);
-    const Layout L = instance_layout(net->dev.h, run.fp32, run.x_lds);
-    const int n = net->net.n;
-    for (const HostFinal& f : q_limit_passes(net, count, inst, o, results, L, paused)) {
-      if (vm_out) std::copy(f.vm.begin(), f.vm.end(), vm_out + (size_t)n * f.i);
-      if (va_out) std::copy(f.va.begin(), f.va.end(), va_out + (size_t)n * f.i);
-    }
+    solve_all(net, count, inst, o, results, vm_out, va_out, nullptr);
     return GS_OK;
   });
 }
@@ -1520,24 +1680,15 @@ GS_API gs_status gs_nr_screen_batch(gs_net* net, int32_t count, const gs_instanc
     if ((o.flat_start != 0) != net->flat_start) fail(GS_INVALID_ARGUMENT, "flat_start differs from the upload");
     check_instances(net->net.nb, count, inst, "");
     if (count == 0) return GS_OK;
-    std::vector<HostFinal> finals;
-    run_batch(net, count, inst, o, results, vm_out, va_out, false, &finals);
+    const std::vector<HostFinal> finals = solve_all(net, count, inst, o, results, vm_out, va_out, nullptr);
     gs_ctx* ctx = net->ctx;
     hipStream_t s = ctx->stream;
     const BatchRun run = plan_run(net->dev.h, o, false);
     const Layout L = instance_layout(net->dev.h, run.fp32, run.x_lds);
     const int n = net->net.n, nb = net->net.nb;
-    if (o.flags & GS_FLAG_Q_LIMITS) {
-      // the reactive-limit passes; the instances that ran again have their final state uploaded below
-      for (HostFinal& f : q_limit_passes(net, count, inst, o, results, L, finals)) {
-        if (vm_out) std::copy(f.vm.begin(), f.vm.end(), vm_out + (size_t)n * f.i);
-        if (va_out) std::copy(f.va.begin(), f.va.end(), va_out + (size_t)n * f.i);
-        finals.push_back(std::move(f));
-      }
-    }
-    // instances the host finished: their final state back into their blocks, where the screen reads it
+    // instances the host finished or ran again: their final state back into their blocks, where the screen reads it
     for (const HostFinal& f : finals) {
-      char* b = net->inst_buf.as<char>() + L.stride * (size_t)f.i;
+      char* b = net->arena.inst_buf.as<char>() + L.stride * (size_t)f.i;
       check(hipMemcpyAsync(b + L.vm, f.vm.data(), 8 * (size_t)n, hipMemcpyHostToDevice, s), "final vm");
       check(hipMemcpyAsync(b + L.va, f.va.data(), 8 * (size_t)n, hipMemcpyHostToDevice, s), "final va");
       // (a run the host stopped before any resumed launch has no new injections: the block keeps its last ones)
@@ -1549,7 +1700,7 @@ GS_API gs_status gs_nr_screen_batch(gs_net* net, int32_t count, const gs_instanc
     net->screen_buf.ensure(sizeof(gs_screen_summary) * (size_t)count);
     if (loading) net->loading_buf.ensure(8 * (size_t)count * std::max(1, nb));
     if (flows) net->flow_buf.ensure(32 * (size_t)count * std::max(1, nb));
-    check(launch_screen(net->desc_buf.as<InstDesc>(), net->dev.h, count, net->screen_buf.as<gs_screen_summary>(),
+    check(launch_screen(net->arena.desc_buf.as<InstDesc>(), net->dev.h, count, net->screen_buf.as<gs_screen_summary>(),
                         loading ? net->loading_buf.as<double>() : nullptr, flows ? net->flow_buf.as<double>() : nullptr,
                         s),
           "screen kernel");
@@ -1563,7 +1714,7 @@ GS_API gs_status gs_nr_screen_batch(gs_net* net, int32_t count, const gs_instanc
     for (int w = 0; w < 2; w++) {
       double* dst = w == 0 ? p_inj : q_inj;
       if (!dst) continue;
-      check(hipMemcpy2DAsync(dst, 8 * (size_t)n, net->inst_buf.as<char>() + (w == 0 ? L.p : L.q), L.stride,
+      check(hipMemcpy2DAsync(dst, 8 * (size_t)n, net->arena.inst_buf.as<char>() + (w == 0 ? L.p : L.q), L.stride,
                              8 * (size_t)n, (size_t)count, hipMemcpyDeviceToHost, s), "injections");
     }
     check(hipStreamSynchronize(s), "screen sync");
@@ -1584,7 +1735,8 @@ GS_API gs_status gs_nr_solve_plain(gs_net* net, int32_t count, const gs_instance
     gs_nr_options o = normalise(opts);
     o.lu_precision = GS_LU_FP64;
     if ((o.flat_start != 0) != net->flat_start) fail(GS_INVALID_ARGUMENT, "flat_start differs from the upload");
-    if (o.flags & GS_FLAG_Q_LIMITS) fail(GS_INVALID_ARGUMENT, "the plain path does not run the reactive-limit passes");
+    if (o.flags & (GS_FLAG_Q_LIMITS | GS_FLAG_DC_RETRY))
+      fail(GS_INVALID_ARGUMENT, "the plain path runs neither the reactive-limit passes nor the DC-start retry");
     if (count == 0) return GS_OK;
     run_batch(net, count, inst, o, results, vm_out, va_out, true);
     return GS_OK;
@@ -1646,7 +1798,7 @@ GS_API gs_status gs_plan_build(const gs_network* net, const gs_nr_options* opts,
     p->spec = build_bus_spec(p->net, o.flat_start != 0, 1.0);
     p->jp = build_jacobian_pattern(p->y, p->spec.type);
     std::string why;
-    if (!symbolic_pass(p->y, p->jp, p->spec.vm.data(), p->spec.va.data(), p->plan, why)) return GS_SINGULAR;
+    if (!symbolic_pass(p->y, p->jp, p->spec.vm.data(), p->spec.va.data(), p->plan, why, o.growth_tol)) return GS_SINGULAR;
     compile_plan(p->plan, p->dph, solve_scratch_cap());
     *out = p.release();
     return GS_OK;
@@ -1773,8 +1925,9 @@ GS_API gs_status gs_batch_create(gs_ctx* ctx, int32_t n_groups, gs_net* const* n
       if (counts[gi] <= 0 || !inst[gi]) fail(GS_INVALID_ARGUMENT, "group " + std::to_string(gi) + ": instances");
       gs_nr_options o = normalise(opts ? &opts[gi] : nullptr);
       if ((o.flat_start != 0) != net->flat_start) fail(GS_INVALID_ARGUMENT, "flat_start differs from the upload");
-      if (o.flags & GS_FLAG_Q_LIMITS)
-        fail(GS_INVALID_ARGUMENT, "group " + std::to_string(gi) + ": reactive limits run through gs_nr_solve_batch");
+      if (o.flags & (GS_FLAG_Q_LIMITS | GS_FLAG_DC_RETRY))
+        fail(GS_INVALID_ARGUMENT, "group " + std::to_string(gi) +
+                                      ": reactive limits and the DC-start retry run through gs_nr_solve_batch");
       check_instances(net->net.nb, counts[gi], inst[gi], "group " + std::to_string(gi) + ", ");
       auto g = std::make_unique<gs_batch::Group>();
       g->net = net;
@@ -1864,8 +2017,9 @@ GS_API gs_status gs_nr_autotune(gs_net* net, int32_t count, const gs_instance* s
     std::vector<Candidate> cands;
     for (int32_t flags : {0, (int32_t)GS_FLAG_SINGLE_WAVE, (int32_t)GS_FLAG_X_GLOBAL})
       for (int32_t sched : {0, 512, 256})
-        for (Candidate c : {Candidate{GS_LU_FP32, 1, 0, 0}, Candidate{GS_LU_FP32, 0, 0, 0},
-                            Candidate{GS_LU_FP32, 2, 0, 0}, Candidate{GS_LU_FP64, 0, 0, 0}}) {
+        // FP32 LU always with one or two FP64 refinement steps: the refined FP64 residual is what sets the answer
+        for (Candidate c : {Candidate{GS_LU_FP32, 1, 0, 0}, Candidate{GS_LU_FP32, 2, 0, 0},
+                            Candidate{GS_LU_FP64, 0, 0, 0}}) {
           if (sched != 0 && sched >= count) continue;
           if (flags == GS_FLAG_SINGLE_WAVE) {  // only where the default would pair the instances
             gs_nr_options t = base;
@@ -1905,6 +2059,7 @@ GS_API gs_status gs_nr_autotune(gs_net* net, int32_t count, const gs_instance* s
         chosen = o;
       }
     }
+    chosen.flags |= base.flags & GS_FLAG_DC_RETRY;   // the caller's retry choice is not the autotuner's to make
     *opts = chosen;
     if (best_ms) *best_ms = best;
     return GS_OK;

@@ -199,6 +199,14 @@ struct gs_ctx {
   int64_t seam_plan_hits = 0, seam_plan_builds = 0;
 };
 
+namespace gs {
+// The device buffers one batched run lays its instances out in: instance blocks, descriptors, states, and the base
+// specification followed by any instances' own Y values.
+struct RunArena {
+  DevMem inst_buf, desc_buf, state_buf, aux_buf;
+};
+}  // namespace gs
+
 struct gs_net {
   gs_ctx* ctx = nullptr;
   gs::Network net;
@@ -209,7 +217,7 @@ struct gs_net {
   gs::Plan plan;
   gs::DevicePlan dev;
   // per-call device arenas
-  gs::DevMem inst_buf, desc_buf, state_buf, aux_buf;
+  gs::RunArena arena;
   gs::DevMem screen_buf, loading_buf, flow_buf;
   double last_total_ms = 0, last_kernel_ms = 0;
   int last_launches = 0;
@@ -261,7 +269,7 @@ void upload_plan(DevicePlan& dp, const YBus* y, const JacobianPattern& jp, const
                  double base_mva, hipStream_t s);
 gs_plan_stats plan_stats(const DevicePlanHost& d, const YBus& y, const JacobianPattern& jp);
 bool symbolic_pass(const YBus& y, const JacobianPattern& jp, const double* vm, const double* va, Plan& plan,
-                   std::string& why);
+                   std::string& why, double growth_tol);
 gs_nr_options normalise(const gs_nr_options* o);
 Layout instance_layout(const DevPlan& h, bool fp32, bool x_lds);
 BatchRun plan_run(const DevPlan& h, const gs_nr_options& o, bool force_fp64);
@@ -288,7 +296,8 @@ void launch_prepared(const DevPlan& h, const PreparedRun& pr, int schedule, hipS
 void finish_paused(gs_net* net, const gs_instance& in, const gs_nr_options& o, bool plain, InstState st,
                    std::vector<double>& vm, std::vector<double>& va, gs_nr_result& result, double& kms, int& launches,
                    std::vector<double>* p_out = nullptr, std::vector<double>* q_out = nullptr,
-                   const std::vector<int>* types = nullptr, const std::vector<double>* qspec = nullptr);
+                   const std::vector<int>* types = nullptr, const std::vector<double>* qspec = nullptr,
+                   InstState* st_out = nullptr);
 // The branch data screening reads (end buses, stamps, rate A) and the buses' own voltage limits, uploaded with a
 // network.
 void upload_branches(DevicePlan& dp, const Network& net, hipStream_t s);

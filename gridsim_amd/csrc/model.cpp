@@ -443,6 +443,7 @@ bool gp_plan(int m, const vector<int>& rowptr, const vector<int>& colidx, const 
         if (t > best) { best = t; ipiv = i; }
       } else {
         plan.ui.push_back(plan.pinv[i]);
+        plan.ux.push_back(x[i]);
       }
     }
     if (ipiv < 0 || best <= floor_row[ipiv]) {
@@ -460,6 +461,7 @@ bool gp_plan(int m, const vector<int>& rowptr, const vector<int>& colidx, const 
     double pivot = x[ipiv];
     plan.pivot_row[k] = ipiv;
     plan.ui.push_back(k);
+    plan.ux.push_back(pivot);
     plan.pinv[ipiv] = k;
     li.push_back(ipiv);
     lx.push_back(1.0);
@@ -473,7 +475,115 @@ bool gp_plan(int m, const vector<int>& rowptr, const vector<int>& colidx, const 
   plan.up[m] = (int)plan.ui.size();
   plan.li.resize(li.size());
   for (size_t p = 0; p < li.size(); p++) plan.li[p] = plan.pinv[li[p]];
+  plan.lx = std::move(lx);
   return true;
+}
+
+// SparseLuFactorization.Solve (SparseLuFactorization.cs:338-371) on a plan's own factors: x[pinv[k]] = b[k], the
+// forward column sweep over unit-diagonal L, the back column sweep over U (each column's pivot last), out[q[k]] = x[k].
+void plan_solve(const Plan& f, const double* b, double* out) {
+  const int n = f.m;
+  vector<double> x(n);
+  for (int k = 0; k < n; k++) x[f.pinv[k]] = b[k];
+  for (int j = 0; j < n; j++) {
+    const double xj = x[j];
+    for (int p = f.lp[j] + 1; p < f.lp[j + 1]; p++) x[f.li[p]] -= f.lx[p] * xj;
+  }
+  for (int j = n - 1; j >= 0; j--) {
+    const double xj = x[j] / f.ux[f.up[j + 1] - 1];
+    x[j] = xj;
+    for (int p = f.up[j]; p < f.up[j + 1] - 1; p++) x[f.ui[p]] -= f.ux[p] * xj;
+  }
+  for (int k = 0; k < n; k++) out[f.q[k]] = x[k];
+}
+
+// NewtonShell.TrySeedDcAngles (NewtonShell.cs:282-299) over DcPowerFlow.Solve(DcDemand.LegacyView(model))
+// (DcPowerFlow.cs:78-86, DcDemand.cs:38-50, DistributionFactors.cs:143-360,454-520). The reactance-only view: every
+// tap 1, every shift 0, every Gs 0, so an in-service branch stamps b = 1 / x into the reduced nodal susceptance B'
+// (branch order; the reference bus's row and column left out), the right-hand side is each bus's net injection
+// (0 - Pd, then + Pg per generator in generator order) / base, B' is factored as SparseLu.Factor does (minimum degree,
+// explicit zeros dropped, partial pivoting) and solved; each angle is reduced to [-180, 180] degrees (the IEEE
+// remainder Math.IEEERemainder computes) and offset by the slack's start angle. False, va untouched, unless exactly one
+// bus is a slack, the in-service branches (none of zero reactance) connect every bus, and B' factors.
+bool dc_seed_angles(const Network& net, const vector<int>& type, vector<double>& va) {
+  const int n = net.n;
+  int slack = -1, slacks = 0;
+  for (int i = 0; i < n; i++)
+    if (type[i] == GS_BUS_SLACK) { slack = i; slacks++; }
+  if (slacks != 1) return false;
+  vector<int> parent(n);
+  for (int i = 0; i < n; i++) parent[i] = i;
+  auto find = [&](int v) { while (parent[v] != v) { parent[v] = parent[parent[v]]; v = parent[v]; } return v; };
+  for (int l = 0; l < net.nb; l++) {
+    if (!net.br_in[l]) continue;
+    if (net.br_x[l] == 0.0) return false;   // the reactance-only model cannot take a zero-reactance branch
+    const int a = find(net.br_from[l]), c = find(net.br_to[l]);
+    if (a != c) parent[a] = c;
+  }
+  for (int i = 0; i < n; i++)
+    if (find(i) != find(slack)) return false;   // more than one DC island
+  const int nr = n - 1;
+  if (nr == 0) return false;
+  vector<int> red(n);
+  for (int i = 0, r = 0; i < n; i++) red[i] = i == slack ? -1 : r++;
+  // B' by stamps in branch order, duplicates summed in insertion order from +0.0, an explicit diagonal in every row
+  // (CsrMatrix.Builder), then the columns sorted per row with a stable sort
+  vector<vector<std::pair<int, double>>> rows(nr);
+  auto add = [&](int r, int c, double v) { rows[r].push_back({c, v}); };
+  for (int l = 0; l < net.nb; l++) {
+    if (!net.br_in[l]) continue;
+    const int f = red[net.br_from[l]], t = red[net.br_to[l]];
+    const double bl = 1.0 / net.br_x[l];
+    if (f >= 0) add(f, f, bl);
+    if (t >= 0) add(t, t, bl);
+    if (f >= 0 && t >= 0) { add(f, t, -bl); add(t, f, -bl); }
+  }
+  vector<int> rp(nr + 1, 0), ci;
+  vector<double> val;
+  for (int r = 0; r < nr; r++) {
+    auto& row = rows[r];
+    std::stable_sort(row.begin(), row.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+    bool diag = false;
+    for (size_t e = 0; e < row.size(); e++) {
+      if (!ci.empty() && (int)ci.size() > rp[r] && ci.back() == row[e].first) { val.back() += row[e].second; continue; }
+      if (!diag && row[e].first > r) { ci.push_back(r); val.push_back(0.0); diag = true; }
+      ci.push_back(row[e].first);
+      val.push_back(row[e].second);
+      diag |= row[e].first == r;
+    }
+    if (!diag) { ci.push_back(r); val.push_back(0.0); }
+    rp[r + 1] = (int)ci.size();
+  }
+  Plan f;
+  std::string why;
+  const vector<int> order = minimum_degree(nr, rp.data(), ci.data());
+  if (!gp_plan(nr, rp, ci, val.data(), order, 0.0, false, f, why)) return false;
+  vector<double> inj(n), rhs(nr, 0.0), x(nr);
+  for (int i = 0; i < n; i++) inj[i] = 0.0 - net.pd[i];
+  for (int g = 0; g < net.ng; g++) inj[net.gen_bus[g]] += net.gen_pg[g];
+  for (int i = 0; i < n; i++)
+    if (red[i] >= 0) rhs[red[i]] += inj[i] / net.base;
+  plan_solve(f, rhs.data(), x.data());
+  const double va0 = va[slack];
+  for (int i = 0; i < n; i++) {
+    if (i == slack) continue;
+    const double deg = x[red[i]] * 180.0 / kPi;
+    va[i] = va0 + std::remainder(deg, 360.0) * kPi / 180.0;
+  }
+  return true;
+}
+
+// NewtonShell.MaxBranchAngleDeg (NewtonShell.cs:311-320): the largest angle across the series element of any in-service
+// branch, net of its phase shift, wrapped to [-180, 180] degrees.
+double max_branch_angle_deg(const Network& net, const double* va) {
+  double worst = 0;
+  for (int l = 0; l < net.nb; l++) {
+    if (!net.br_in[l]) continue;
+    const double df = va[net.br_from[l]] * 180.0 / kPi, dt = va[net.br_to[l]] * 180.0 / kPi;
+    const double d = std::fabs(std::remainder(df - net.br_shift[l] - dt, 360.0));
+    if (d > worst) worst = d;
+  }
+  return worst;
 }
 
 void compile_plan(const Plan& plan, DevicePlanHost& d, int scratch_cap) {

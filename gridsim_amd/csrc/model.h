@@ -92,6 +92,7 @@ struct Plan {
   vector<int> reach_ptr, reach;
   vector<int> lp, li, up, ui;    // li, ui: pivot steps
   vector<int> li_orig;           // L rows as original rows
+  vector<double> lx, ux;         // the factors of the values the plan was made from (L unit diagonal first)
 };
 
 // Factor A (CSR values `val`) with ordering q and partial pivoting under the given diagonal preference, on the CSC
@@ -99,6 +100,16 @@ struct Plan {
 // solver's). Returns false when a pivot is at or below its row's floor (singular) or an entry is not finite.
 bool gp_plan(int m, const vector<int>& rowptr, const vector<int>& colidx, const double* val, const vector<int>& q,
              double diagonal_preference, bool keep_zeros, Plan& plan, std::string& why);
+
+// SparseLuFactorization.Solve on the factors a plan was made from: out = A^{-1} b.
+void plan_solve(const Plan& f, const double* b, double* out);
+
+// The DC power-flow angle start of the retry (NewtonShell.TrySeedDcAngles): va (start angles, radians, in and out) gets
+// the reactance-only DC angles of the network offset by the slack's start angle; false (va untouched) when the
+// network has no single slack, is not one DC island, or its susceptance does not factor.
+bool dc_seed_angles(const Network& net, const vector<int>& type, vector<double>& va);
+// NewtonShell.MaxBranchAngleDeg: the largest in-service branch angle (degrees) of the state va (radians).
+double max_branch_angle_deg(const Network& net, const double* va);
 
 // The plan compiled into the flat arrays the refactor and solve kernels walk (see nr_kernels.hip).
 struct DevicePlanHost {

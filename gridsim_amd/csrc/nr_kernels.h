@@ -123,6 +123,8 @@ struct InstState {
   // doubles, which order as unsigned integers; a NaN orders above +inf)
   uint64_t mis_bits, dx_bits, ratio_bits;
   int32_t jbad, wpad;
+  uint64_t jn_bits;     // level-parallel: max |J entry| of the step being formed (bits), and of the current step
+  double jnorm;
 };
 
 // Control block of a level-parallel run: how many of its instances have finished (or paused).
@@ -164,7 +166,21 @@ struct NrParams {
   int32_t init;                       // 1: reset the instance's counters before stepping
   int32_t load_start;                 // 1: copy the start point into the iterate before stepping
   int32_t n_inst;                     // instances of the launch (set by the launcher)
+  int32_t trust_first;                // 1: the launch's first refactor runs on a plan the host made at this very
+                                      // state (its pivots are the factorisation's own): no growth or floor check
+  int32_t pad;
 };
+
+// The pivot floor of the Newton refactor: a static pivot is kept only above kPivotFloorRel x the largest |entry| of
+// the Jacobian (at or above every row's own floor, NumericTolerances.RelativePivotFloor x max |A(r, :)|, so the device
+// never accepts a pivot the reference's factorisation would refuse; a pivot below it goes to the host).
+constexpr double kPivotFloorRel = 1e-12;
+// In FP32 LU a pivot that rounding alone made non-zero (a column the FP64 factorisation would find singular) stands
+// about eps32 x its column's scale, far above that floor: the FP32 refactor keeps a pivot only above 2^-20 x max |J|
+// and hands anything smaller to the host's FP64 symbolic pass (real Jacobians' smallest pivots lie near 1e-4 x max |J|).
+constexpr double kPivotFloorRelF32 = 9.5367431640625e-07;
+template <typename T>
+__host__ __device__ constexpr double pivot_floor_rel() { return sizeof(T) == 4 ? kPivotFloorRelF32 : kPivotFloorRel; }
 
 // Instances per work-group of the batched kernel: 1, or 2 wavefronts sharing one LDS solve vector (each wave takes
 // it in turn for its triangular solves), which halves the solve vector's LDS per instance.

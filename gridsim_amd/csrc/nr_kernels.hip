@@ -127,7 +127,8 @@ __device__ __forceinline__ void put_j(const Inst<T>& I, int s, int a, double v) 
 // Every Y cell's two injection terms and its off-diagonal Jacobian entries, one cell per lane. Returns whether a
 // Jacobian value of this lane was not finite.
 template <typename T, bool KEEP>
-__device__ __forceinline__ int mismatch_cells(const DevPlan& P, const Inst<T>& I, int lane, int blk0, int stride) {
+__device__ __forceinline__ int mismatch_cells(const DevPlan& P, const Inst<T>& I, int lane, int blk0, int stride,
+                                              double* jmax) {
   GLOBAL const v2i* y_ij = globv<v2i>(P.y_ij);
   GLOBAL const v2d* y_gb = I.ygb;
   GLOBAL const v4i* slots = globv<v4i>(P.slots);
@@ -135,6 +136,7 @@ __device__ __forceinline__ int mismatch_cells(const DevPlan& P, const Inst<T>& I
   const int nnz = P.nnz_y;
   const int nblk = (nnz + kWave - 1) / kWave;
   int lbad = 0;
+  double jm = 0.0;
   // stage 0 holds the pattern of block t + 2 (lane-strided), stage 1 the gathered state of block t + 1
   const int iters = blk0 < nblk ? (nblk - blk0 + stride - 1) / stride : 0;
   int i1 = 0, j1 = 0, i2 = 0, j2 = 0;
@@ -181,9 +183,14 @@ __device__ __forceinline__ int mismatch_cells(const DevPlan& P, const Inst<T>& I
         put_j<T, KEEP>(I, s.z, a.z, mm);
         put_j<T, KEEP>(I, s.w, a.w, l);
         lbad |= !(isfinite(h) && isfinite(nn) && isfinite(mm) && isfinite(l));
+        jm = a.x >= 0 && fabs(h) > jm ? fabs(h) : jm;
+        jm = a.y >= 0 && fabs(nn) > jm ? fabs(nn) : jm;
+        jm = a.z >= 0 && fabs(mm) > jm ? fabs(mm) : jm;
+        jm = a.w >= 0 && fabs(l) > jm ? fabs(l) : jm;
       }
     }
   }
+  *jmax = jm;
   return lbad;
 }
 
@@ -191,7 +198,7 @@ __device__ __forceinline__ int mismatch_cells(const DevPlan& P, const Inst<T>& I
 // scalars are loaded ahead. Returns this lane's max |f| (NaN-propagating); *bad: a Jacobian value was not finite.
 template <typename T, bool KEEP>
 __device__ __forceinline__ double mismatch_buses(const DevPlan& P, const Inst<T>& I, int lane, int blk0, int stride,
-                                                 int* bad) {
+                                                 int* bad, double* jmax) {
   GLOBAL const v2d* y_gb = I.ygb;
   GLOBAL const v4i* slots = globv<v4i>(P.slots);
   GLOBAL const v4i* aslots = globv<v4i>(P.aslots);
@@ -200,7 +207,7 @@ __device__ __forceinline__ double mismatch_buses(const DevPlan& P, const Inst<T>
   GLOBAL const int* ppos = glob(P.ppos);
   GLOBAL const int* qpos = glob(P.qpos);
   const int n = P.n;
-  double lmax = 0.0;
+  double lmax = 0.0, jm = 0.0;
   int lbad = 0;
   int i = lane + blk0 * kWave;
   const int step = stride * kWave;
@@ -241,22 +248,31 @@ __device__ __forceinline__ double mismatch_buses(const DevPlan& P, const Inst<T>
     put_j<T, KEEP>(I, s.z, a.z, mm);
     put_j<T, KEEP>(I, s.w, a.w, l);
     lbad |= !(isfinite(h) && isfinite(nn) && isfinite(mm) && isfinite(l));
+    jm = a.x >= 0 && fabs(h) > jm ? fabs(h) : jm;
+    jm = a.y >= 0 && fabs(nn) > jm ? fabs(nn) : jm;
+    jm = a.z >= 0 && fabs(mm) > jm ? fabs(mm) : jm;
+    jm = a.w >= 0 && fabs(l) > jm ? fabs(l) : jm;
     if (pp >= 0) { const double fv = ps - P_i; I.f[pp] = fv; lmax = nan_max(lmax, fabs(fv)); }
     if (qq >= 0) { const double fv = qs - Q_i; I.f[qq] = fv; lmax = nan_max(lmax, fabs(fv)); }
   }
   *bad = lbad;
+  *jmax = jm;
   return lmax;
 }
 
 // The whole pass for one wave. Returns max |f| (NaN-propagating) over the wave; sets *bad when a Jacobian value is
 // not finite.
+// *jnorm: the largest |entry| of the Jacobian (entries that are finite).
 template <typename T, bool KEEP>
-__device__ __forceinline__ double mismatch_jacobian(const DevPlan& P, const Inst<T>& I, int lane, int* bad) {
-  const int cbad = mismatch_cells<T, KEEP>(P, I, lane, 0, 1);
+__device__ __forceinline__ double mismatch_jacobian(const DevPlan& P, const Inst<T>& I, int lane, int* bad,
+                                                   double* jnorm) {
+  double jc, jb;
+  const int cbad = mismatch_cells<T, KEEP>(P, I, lane, 0, 1, &jc);
   WG_SYNC();
   int bbad;
-  const double lmax = mismatch_buses<T, KEEP>(P, I, lane, 0, 1, &bbad);
+  const double lmax = mismatch_buses<T, KEEP>(P, I, lane, 0, 1, &bbad, &jb);
   *bad = wave_any(cbad | bbad);
+  *jnorm = wave_max(jc > jb ? jc : jb);
   return wave_max(lmax);
 }
 
@@ -264,7 +280,8 @@ __device__ __forceinline__ double mismatch_jacobian(const DevPlan& P, const Inst
 // Level-chunked left-looking Gilbert-Peierls on the plan. Track g of a chunk (lanes 8g..8g+7) owns one column and the
 // workspace ws[g * wstride, ...): slots [0, nu - 1) its off-diagonal U rows in topological order, slot nu - 1 the
 // pivot, slots [nu, nu + nl - 1) its L rows. Every workspace slot receives its updates in the reference's order.
-//   EXACT = false (Newton): a static pivot is kept while |pivot| >= growth_tol * max |candidate|.
+//   EXACT = false (Newton): a static pivot is kept while |pivot| >= growth_tol * max |candidate| and |pivot| >
+//   abs_floor (kPivotFloorRel x max |J|, above every row's own floor).
 //   EXACT = true (seam): the reference's own pivot search must pick the plan's row (first strict maximum over the
 //   candidates in reach order), above that row's floor; anything else refuses, as Refactor returns null.
 // Returns -1 on success, else a failing pivot step.
@@ -287,8 +304,9 @@ static_assert(kAhead < kRing, "the value of a row is loaded after its record");
 // rounding rows are no-ops already).
 template <typename T, bool EXACT, bool CHUNK = false>
 __device__ __forceinline__ int refactor(const DevPlan& P, GLOBAL T* vals, GLOBAL T* fval, GLOBAL T* bval,
-                                        GLOBAL T* dval, LDSP T* ws, double growth_tol, GLOBAL const double* rowfloor,
-                                        int lane, double* min_ratio, int row0, int row1) {
+                                        GLOBAL T* dval, LDSP T* ws, double growth_tol, double abs_floor,
+                                        GLOBAL const double* rowfloor, int lane, double* min_ratio, int row0,
+                                        int row1) {
   // rows go in pairs (one 16-byte record per lane for two rows): start at the even row of row0's pair, the rows
   // before row0 doing nothing
   const int r0e = row0 & ~1;
@@ -379,7 +397,7 @@ __device__ __forceinline__ int refactor(const DevPlan& P, GLOBAL T* vals, GLOBAL
             bad = track_any(moved, TLr) || !(pabs > rowfloor[prow[k]]);
           } else {
             const double ratio = pabs / best;
-            bad = !(ratio >= growth_tol) || !(pabs > 0.0) || !isfinite(pabs);
+            bad = !(ratio >= growth_tol) || !(pabs > abs_floor) || !(pabs > 0.0) || !isfinite(pabs);
             mratio = ratio < mratio ? ratio : mratio;
           }
           if (bad) fail = k;
@@ -659,8 +677,9 @@ __global__ __launch_bounds__(kWave * (PAIR ? kPairWaves : 1), 2) void nr_batch_k
   };
   for (;;) {
     int bad;
-    max_mis = prm.refine_steps > 0 ? mismatch_jacobian<T, true>(P, I, opaque_lane(lane), &bad)
-                                   : mismatch_jacobian<T, false>(P, I, opaque_lane(lane), &bad);
+    double jnorm;
+    max_mis = prm.refine_steps > 0 ? mismatch_jacobian<T, true>(P, I, opaque_lane(lane), &bad, &jnorm)
+                                   : mismatch_jacobian<T, false>(P, I, opaque_lane(lane), &bad, &jnorm);
     WG_SYNC();
     tick(t_mis);
     if (iter == 0) start_mis = max_mis;
@@ -669,8 +688,11 @@ __global__ __launch_bounds__(kWave * (PAIR ? kPairWaves : 1), 2) void nr_batch_k
     if (iter >= prm.max_iterations) break;
     if (steps >= prm.steps_this_launch) { paused = 1; break; }
     if (bad || !isfinite(max_mis)) break;  // NewtonShell.SolveStep refuses a non-finite J or f
-    fail_col = refactor<T, false>(P, I.vals, I.fval, I.bval, I.dval, ws, prm.growth_tol, nullptr,
-                                  opaque_lane(lane), &min_ratio, 0, P.n_rs);
+    // (a plan the host made at this very state is the factorisation's own: its first step is not checked)
+    const bool trust = prm.trust_first && steps == 0;
+    fail_col = refactor<T, false>(P, I.vals, I.fval, I.bval, I.dval, ws, trust ? 0.0 : prm.growth_tol,
+                                  trust ? 0.0 : pivot_floor_rel<T>() * jnorm, nullptr, opaque_lane(lane), &min_ratio, 0,
+                                  P.n_rs);
     tick(t_fac);
     if (fail_col >= 0) { paused = 2; break; }
     solve(I.f, I.dx);
@@ -805,6 +827,7 @@ __global__ __launch_bounds__(256) void wide_start(const InstDesc* __restrict__ d
     st->last_from = __builtin_nan("");
     st->t_mismatch = st->t_refactor = st->t_solve = st->t_update = 0;
     st->mis_bits = 0; st->dx_bits = 0; st->ratio_bits = (uint64_t)__double_as_longlong(1.0); st->jbad = 0;
+    st->jn_bits = 0; st->jnorm = 0.0;
     if (blockIdx.y == 0) ctl->finished = 0;
   }
 }
@@ -815,9 +838,15 @@ __global__ __launch_bounds__(kWave) void wide_cells(const InstDesc* __restrict__
   InstState* st = descs[blockIdx.y].state;
   if (st->done) return;
   const Inst<T> I = load_inst<T>(descs, P, blockIdx.y);
-  const int bad = prm.refine_steps > 0 ? mismatch_cells<T, true>(P, I, threadIdx.x, blockIdx.x, gridDim.x)
-                                       : mismatch_cells<T, false>(P, I, threadIdx.x, blockIdx.x, gridDim.x);
-  if (wave_any(bad) && threadIdx.x == 0) atomicOr(&st->jbad, 1);
+  double jm;
+  const int bad = prm.refine_steps > 0 ? mismatch_cells<T, true>(P, I, threadIdx.x, blockIdx.x, gridDim.x, &jm)
+                                       : mismatch_cells<T, false>(P, I, threadIdx.x, blockIdx.x, gridDim.x, &jm);
+  const int anybad = wave_any(bad);
+  jm = wave_max(jm);
+  if (threadIdx.x == 0) {
+    if (anybad) atomicOr(&st->jbad, 1);
+    atomic_max_bits(&st->jn_bits, jm);
+  }
 }
 
 template <typename T>
@@ -827,12 +856,15 @@ __global__ __launch_bounds__(kWave) void wide_buses(const InstDesc* __restrict__
   if (st->done) return;
   const Inst<T> I = load_inst<T>(descs, P, blockIdx.y);
   int bad;
+  double jm;
   const double lmax = wave_max(prm.refine_steps > 0
-                                   ? mismatch_buses<T, true>(P, I, threadIdx.x, blockIdx.x, gridDim.x, &bad)
-                                   : mismatch_buses<T, false>(P, I, threadIdx.x, blockIdx.x, gridDim.x, &bad));
+                                   ? mismatch_buses<T, true>(P, I, threadIdx.x, blockIdx.x, gridDim.x, &bad, &jm)
+                                   : mismatch_buses<T, false>(P, I, threadIdx.x, blockIdx.x, gridDim.x, &bad, &jm));
   const int anybad = wave_any(bad);
+  jm = wave_max(jm);
   if (threadIdx.x == 0) {
     atomic_max_bits(&st->mis_bits, lmax);
+    atomic_max_bits(&st->jn_bits, jm);
     if (anybad) atomicOr(&st->jbad, 1);
   }
 }
@@ -850,9 +882,11 @@ __global__ __launch_bounds__(kWave) void wide_verdict(const InstDesc* __restrict
   const int iter = st->iter;
   double start_mis = st->start_mis;
   if (iter == 0) start_mis = max_mis;
+  const double jnorm = bits_double(st->jn_bits);
   __syncthreads();
   if (threadIdx.x == 0) {
-    st->mis_bits = 0; st->jbad = 0; st->dx_bits = 0;
+    st->mis_bits = 0; st->jbad = 0; st->dx_bits = 0; st->jn_bits = 0;
+    st->jnorm = jnorm;
     st->start_mis = start_mis;
     st->max_mis = max_mis;
   }
@@ -878,7 +912,8 @@ __global__ __launch_bounds__(kWave) void wide_refactor(const InstDesc* __restric
   const int row0 = P.rs_chunk_row[c], row1 = P.rs_chunk_row[c + per];
   double ratio = 1.0;
   const int fail = refactor<T, false, true>(P, I.vals, I.fval, I.bval, I.dval, lds(reinterpret_cast<T*>(smem)),
-                                      prm.growth_tol, nullptr, opaque_lane(threadIdx.x), &ratio, row0, row1);
+                                            prm.growth_tol, pivot_floor_rel<T>() * st->jnorm, nullptr,
+                                            opaque_lane(threadIdx.x), &ratio, row0, row1);
   if (threadIdx.x == 0) {
     if (ratio < 1.0) atomicMin(reinterpret_cast<unsigned long long*>(&st->ratio_bits),
                                (unsigned long long)__double_as_longlong(ratio));
@@ -1114,7 +1149,8 @@ __global__ __launch_bounds__(kWave) void iterative_step_kernel(const InstDesc* _
     WG_SYNC();
   }
   int bad;
-  const double max_mis = mismatch_jacobian<double, true>(P, I, opaque_lane(lane), &bad);
+  double jnorm;
+  const double max_mis = mismatch_jacobian<double, true>(P, I, opaque_lane(lane), &bad, &jnorm);
   WG_SYNC();
   if (lane != 0) return;
   const int iter = st->iter;
@@ -1294,7 +1330,7 @@ __global__ __launch_bounds__(kWave) void seam_solve_kernel(const SeamDesc* __res
   WG_SYNC();
   double ratio = 1.0;
   const int fail =
-      refactor<double, true>(P, vals, glob(d.fval), glob(d.bval), glob(d.dval), ws, 0.0, rowfloor, lane, &ratio, 0,
+      refactor<double, true>(P, vals, glob(d.fval), glob(d.bval), glob(d.dval), ws, 0.0, 0.0, rowfloor, lane, &ratio, 0,
                              P.n_rs);
   if (fail >= 0) {
     if (lane == 0) *d.status = 1;

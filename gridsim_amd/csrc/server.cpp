@@ -427,8 +427,8 @@ GS_API gs_status gs_server_add_network(gs_server* srv, gs_net* net, const gs_nr_
     nt->opts = normalise(opts);
     if (nt->opts.schedule < 0)
       fail(GS_INVALID_ARGUMENT, "the serving loop launches a wavefront per instance (schedule >= 0)");
-    if (nt->opts.flags & GS_FLAG_Q_LIMITS)
-      fail(GS_INVALID_ARGUMENT, "the serving loop does not run the reactive-limit passes");
+    if (nt->opts.flags & (GS_FLAG_Q_LIMITS | GS_FLAG_DC_RETRY))
+      fail(GS_INVALID_ARGUMENT, "the serving loop runs neither the reactive-limit passes nor the DC-start retry");
     nt->run = plan_run(net->dev.h, nt->opts, false);
     nt->L = instance_layout(net->dev.h, nt->run.fp32, nt->run.x_lds);
     // vm, va, p, q lie in that order in an instance's block: one strided copy brings all four back

@@ -21,7 +21,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from .network import Network, bridges
+from .network import SLACK, Network, bridges, splits
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libgridsim_hip.so")
@@ -33,6 +33,7 @@ LU_FP64, LU_FP32 = 0, 1
 FLAG_X_GLOBAL = 1
 FLAG_SINGLE_WAVE = 4
 FLAG_Q_LIMITS = 2      # enforce generator reactive limits (the outer loop of PV -> PQ switching)
+FLAG_DC_RETRY = 8      # retry a stalled or diverged solve from the DC power-flow angle start (RetryFromDcStart)
 SCHEDULE_LEVELS = -1   # every instance spread over the whole GPU, level by level (GS_SCHEDULE_LEVELS)
 STATUS_NAMES = {0: "Converged", 1: "MaxIterations", 2: "Diverged", 3: "NonFinite"}
 
@@ -564,13 +565,22 @@ def branch_label(net: Network, k: int) -> str:
     return f"{int(ids[net.br_from[k]])}-{int(ids[net.br_to[k]])}"
 
 
-def classify(net: Network, outage: int, converged: bool, summary, loading_row) -> ContingencyResult:
-    """One screened outage as ScreenStateCore classifies it: each branch above 100 % of rate A, then one line for
-    the buses under their limits (the lowest voltage) and one for those over them (the highest). A non-converged
-    outage reports whether it split the network (ContingencyAnalysis.NonConverged / Splits)."""
-    label = branch_label(net, outage)
+def set_label(net: Network, outages) -> str:
+    """An outage set named as ContingencyAnalysis.LabelMany names it: the branch labels joined by " + "."""
+    return " + ".join(branch_label(net, int(k)) for k in np.atleast_1d(outages))
+
+
+def classify(net: Network, outage, converged: bool, summary, loading_row) -> ContingencyResult:
+    """One screened outage (a branch, or a set of branches) as ScreenStateCore classifies it: each branch above 100 %
+    of rate A, then one line for the buses under their limits (the lowest voltage) and one for those over them (the
+    highest). A non-converged outage reports whether it split the network (ContingencyAnalysis.NonConverged /
+    Splits)."""
+    label = set_label(net, outage)
     if not converged:
-        split = bool(bridges(net)[outage]) if net.br_in_service[outage] else False
+        if np.ndim(outage) == 0:
+            split = bool(bridges(net)[outage]) if net.br_in_service[outage] else False
+        else:
+            split = splits(net, outage)
         return ContingencyResult(label, False, 0.0, 0.0, 0.0, ["islanded" if split else "no solution"], split)
     violations = []
     if summary["overloads"] > 0:
@@ -600,6 +610,167 @@ def screen_branches_batched(ctx: Context, net: Network, branch_indices=None, opt
         dn.close()
     out = [classify(net, int(k), bool(res.status[j] == CONVERGED), summ[j], ld[j]) for j, k in enumerate(idx)]
     return res, out
+
+
+# ---------------------------------------------------------------- N-k screening
+
+STRESS_VOLTAGE_WEIGHT_PER_PU = 1000.0     # ContingencyAnalysis.StressVoltageWeightPerPu
+DEFAULT_CANDIDATE_COUNT = 20              # ContingencyAnalysis.DefaultCandidateCount
+MAX_COMBINATIONS = 20_000_000             # ContingencyAnalysis.MaxCombinations
+OPERATIONAL_LOW_PU, OPERATIONAL_HIGH_PU = 0.94, 1.06   # VoltageLimits.OperationalLowPu / HighPu
+UNSOLVABLE_MARGIN_PU = 1e-6               # ContingencyAnalysis.UnsolvableMarginPu
+
+
+def stress_score(c: ContingencyResult) -> float:
+    """ContingencyAnalysis.StressScore (ContingencyAnalysis.cs:1073-1080): +inf for a non-converged outage, else the
+    largest branch loading plus 1000 x the voltage excursion beyond the 0.94 - 1.06 pu band (no boundaries here)."""
+    if not c.converged:
+        return float("inf")
+    s = c.max_branch_loading_pct
+    s += max(0.0, OPERATIONAL_LOW_PU - c.min_vm) * STRESS_VOLTAGE_WEIGHT_PER_PU
+    s += max(0.0, c.max_vm - OPERATIONAL_HIGH_PU) * STRESS_VOLTAGE_WEIGHT_PER_PU
+    return s
+
+
+def rank_candidates(single, candidate_count: int, minimum: int):
+    """ContingencyAnalysis.RankCandidates (:474-486): the single outages that solve, most stressed first (a stable
+    sort, ties in branch order), the top max(candidate_count, minimum) of them; the ones that do not solve are
+    returned apart, in branch order. Returns (positions in `single`, excluded results)."""
+    solvable = [i for i, c in enumerate(single) if c.converged]
+    excluded = [c for c in single if not c.converged]
+    take = min(max(candidate_count, minimum), len(solvable))
+    ranked = sorted(solvable, key=lambda i: -stress_score(single[i]))   # sorted() is stable
+    return ranked[:take], excluded
+
+
+def combinations(items, k: int):
+    """All k-subsets of items in the order ContingencyAnalysis.Combinations (:1851-1866) appends them: by the
+    positions of their members, lexicographically."""
+    out, pick = [], [0] * k
+
+    def rec(start, depth):
+        if depth == k:
+            out.append(list(pick))
+            return
+        for i in range(start, len(items) - (k - depth) + 1):
+            pick[depth] = int(items[i])
+            rec(i + 1, depth + 1)
+
+    if k <= len(items):
+        rec(0, 0)
+    return out
+
+
+def provably_unsolvable(net: Network, outages, tolerance: float) -> bool:
+    """ContingencyAnalysis.ProvablyUnsolvablePart (:1750-1829) for constant-power loads: after the outage, a connected
+    part with no slack bus and no negative loss (no branch R < 0, no Gs < 0) whose specified net injection is below
+    -(buses x tolerance + 1e-6) pu (demand beyond generation: losses cannot be negative), or, when the part is lossless,
+    whose |net injection| reaches that margin. Such an outage is reported not converged without a solve."""
+    if not (tolerance > 0 and np.isfinite(tolerance)) or not (net.base_mva > 0 and np.isfinite(net.base_mva)):
+        return False
+    n = net.n_bus
+    parent = np.arange(n)
+
+    def find(x):
+        while parent[x] != x:
+            parent[x] = parent[parent[x]]
+            x = parent[x]
+        return x
+
+    gone = set(int(b) for b in np.atleast_1d(np.asarray(outages, np.int64)) if b >= 0)
+    live = [k for k in range(net.n_branch) if net.br_in_service[k] and k not in gone]
+    for k in live:
+        a, c = find(int(net.br_from[k])), find(int(net.br_to[k]))
+        if a != c:
+            parent[a] = c
+    pg = np.zeros(n)
+    np.add.at(pg, net.gen_bus, net.gen_pg)
+    root = np.array([find(i) for i in range(n)])
+    has_slack = np.zeros(n, bool); neg_loss = np.zeros(n, bool); loss = np.zeros(n, bool)
+    count = np.zeros(n, np.int64); inj = np.zeros(n)
+    for i in range(n):
+        r = root[i]
+        count[r] += 1
+        has_slack[r] |= net.bus_type[i] == SLACK
+        neg_loss[r] |= net.gs[i] < 0.0
+        loss[r] |= net.gs[i] != 0.0
+        inj[r] += (pg[i] - net.pd[i]) / net.base_mva
+    for k in live:
+        r = root[int(net.br_from[k])]
+        neg_loss[r] |= net.br_r[k] < 0.0
+        loss[r] |= net.br_r[k] != 0.0
+    for r in dict.fromkeys(root.tolist()):   # parts in the order of their first bus
+        if has_slack[r] or neg_loss[r]:
+            continue
+        margin = count[r] * tolerance + UNSOLVABLE_MARGIN_PU
+        if inj[r] <= -margin or (not loss[r] and abs(inj[r]) >= margin):
+            return True
+    return False
+
+
+@dataclass
+class NkScreenReport:
+    """ContingencyAnalysis.NkScreenReport: the screened combinations worst first, and the single outages the pruned
+    screen left out of the pairing because they already fail at N-1."""
+    results: list
+    excluded: list
+    batch: object = None
+
+
+def screen_outage_sets(ctx: Context, net: Network, sets, options: NrOptions | None = None, chunk: int = 4096):
+    """ScreenOutageCore over outage sets (each a branch list, WithoutMany): the provably unsolvable ones reported not
+    converged at once, the others solved and classified on the device in batches of `chunk`. Returns a
+    ContingencyResult per set, in the order given."""
+    options = options or NrOptions()
+    out = [None] * len(sets)
+    todo = []
+    for j, st in enumerate(sets):
+        if provably_unsolvable(net, st, options.tolerance):
+            out[j] = classify(net, list(st), False, None, None)
+        else:
+            todo.append(j)
+    if todo:
+        dn = DeviceNetwork(ctx, net, options)
+        try:
+            for a in range(0, len(todo), chunk):
+                idx = todo[a:a + chunk]
+                res, summ, ld, _ = dn.screen(outages=[list(sets[j]) for j in idx], options=options, loading=True)
+                for t, j in enumerate(idx):
+                    out[j] = classify(net, list(sets[j]), bool(res.status[t] == CONVERGED), summ[t], ld[t])
+        finally:
+            dn.close()
+    return out
+
+
+def screen_n_minus_k(ctx: Context, net: Network, k: int, options: NrOptions | None = None,
+                     candidate_count: int = DEFAULT_CANDIDATE_COUNT, exhaustive: bool = False) -> NkScreenReport:
+    """ContingencyAnalysis.ScreenNMinusKReport (ContingencyAnalysis.cs:378-441): unless exhaustive, screen N-1 first,
+    rank the single outages that solve by stress and keep the top candidate_count, then screen every k-subset of that
+    set (the outage lists of one batched device solve); results worst first (a stable sort on StressScore)."""
+    if k < 1:
+        raise ValueError("contingency depth k must be >= 1")
+    live = np.nonzero(net.br_in_service)[0]
+    if k > len(live):
+        return NkScreenReport([], [])
+    if k == 1:
+        single = screen_outage_sets(ctx, net, [[int(b)] for b in live], options)
+        return NkScreenReport(sorted(single, key=lambda c: -stress_score(c)), [])
+    excluded = []
+    if exhaustive:
+        candidates = [int(b) for b in live]
+    else:
+        single = screen_outage_sets(ctx, net, [[int(b)] for b in live], options)
+        pos, excluded = rank_candidates(single, candidate_count, k)
+        candidates = [int(live[i]) for i in pos]
+    count = 1.0
+    for c in range(k):
+        count = count * (len(candidates) - c) / (c + 1)
+    if count > MAX_COMBINATIONS:
+        raise ValueError(f"N-{k}: {len(candidates)} candidate branches give about {count:.4g} outage combinations, "
+                         f"more than the {MAX_COMBINATIONS} one screen takes")
+    combos = combinations(candidates, k)
+    results = screen_outage_sets(ctx, net, combos, options)
+    return NkScreenReport(sorted(results, key=lambda c: -stress_score(c)), excluded)
 
 
 # ---------------------------------------------------------------- multi-network batches and the autotuner
@@ -704,13 +875,19 @@ class Server:
         self.nets.append(dn)
         return nid.value
 
-    def submit(self, network_id: int, outage: int = -1, load_scale: float = 1.0, start=None) -> int:
-        """Queues one request; raises GsError(GS_BUSY) when the queue is full."""
+    def submit(self, network_id: int, outage=-1, load_scale: float = 1.0, start=None) -> int:
+        """Queues one request (outage: a branch index, -1, or a list of branches for an N-2 request); raises
+        GsError(GS_BUSY) when the queue is full. The library copies the start point and the outage list."""
         vm0 = va0 = None
         if start is not None:
             vm0 = np.ascontiguousarray(start[0], np.float64)
             va0 = np.ascontiguousarray(start[1], np.float64)
-        inst = GsInstance(int(outage), 0, float(load_scale), _p(vm0), _p(va0))
+        more = None
+        if np.ndim(outage) > 0:
+            lst = np.atleast_1d(np.asarray(outage, np.int32))
+            outage, more = (int(lst[0]), np.ascontiguousarray(lst[1:])) if len(lst) else (-1, None)
+        inst = GsInstance(int(outage), 0 if more is None else len(more), float(load_scale), _p(vm0), _p(va0),
+                          _p(more) if more is not None and len(more) else None)
         t = C.c_uint64()
         st = lib().gs_server_submit(self.h, C.c_int32(network_id), C.byref(inst), C.byref(t))
         if st != GS_OK:

@@ -302,7 +302,7 @@ GS_API gs_status gs_batch_results(gs_batch* batch, int32_t group, gs_nr_result* 
 GS_API void gs_batch_free(gs_batch* batch);
 
 /* Autotuner (no counterpart in the reference, whose CPU path has nothing to tune): solves the `count` sample
- * instances under each candidate - FP32 LU with 0, 1 or 2 FP64 refinement steps, or FP64 LU; all instances in one
+ * instances under each candidate - FP32 LU with 1 or 2 FP64 refinement steps, or FP64 LU; all instances in one
  * launch or slices of 512 or 256; the solve vector in LDS or in global memory - and writes into *opts the fastest
  * whose verdicts equal the FP64 LU's, whose
  * converged instances meet the tolerance and whose converged voltages lie within 1e-9 (pu, rad) of the FP64 LU's.

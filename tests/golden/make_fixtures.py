@@ -6,6 +6,8 @@ where the reference is mounted; the fixtures are committed and the GPU box never
 * matpower/case*.m: MATPOWER case files the reference bundles (data/matpower/lib), the real-grid inputs.
 * worked_example_case9.json: the numbers printed in the reference's sparse-LU worked example
   (docs/technical/0004 Sparse and GPU numerics.md:778-838) that the oracle must reproduce.
+* cli/*.txt: the reference's recorded CLI output of its contingency screens
+  (tests/GridSim.Tests/fixtures/cli-stdout: nminus1-case9, nminus1-case14, nminus2-case9).
 """
 import json
 import os
@@ -23,6 +25,9 @@ def main():
     with open(os.path.join(HERE, "strictmath_sincos.txt"), "w", encoding="utf-8") as f:
         f.write("# sin/cos lines of the reference's Java StrictMath vector file (FDLIBM 5.3 results).\n")
         f.writelines(lines)
+    for name in ("nminus1-case9", "nminus1-case14", "nminus2-case9"):
+        shutil.copy(os.path.join(REF, "tests/GridSim.Tests/fixtures/cli-stdout", name + ".txt"),
+                    os.path.join(HERE, "cli", name + ".txt"))
     for case in ("case9", "case14", "case118", "case300"):
         shutil.copy(os.path.join(REF, "data/matpower/lib", case + ".m"), os.path.join(HERE, "matpower", case + ".m"))
     doc = open(os.path.join(REF, "docs/technical/0004 Sparse and GPU numerics.md"), encoding="utf-8").read()
