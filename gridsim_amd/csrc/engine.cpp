@@ -21,6 +21,35 @@ using namespace gs;
 static_assert(DevicePlanHost::kRsValueAhead == kRsValueAhead, "plan and kernel disagree on the refactor's look-ahead");
 static_assert(DevicePlanHost::kTracks == kTracks && DevicePlanHost::kTrackLanes == kTrackLanes, "track shape");
 
+// Device memory of gs_sparse_solve_noalloc: the plan image and the per-call buffers of one system, placed in this
+// file-scope array instead of allocated. Each GPU the library runs on holds its own copy (kNoAllocScratchBytes) until
+// the process ends. In host code `scratch` only names the symbol: its device address on each device comes from
+// hipGetSymbolAddress (scratch_address) and is the only form host code passes to a kernel or a copy.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) char scratch[kNoAllocScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, char*> g_scratch_addr;   // device -> device address of `scratch` there
+}  // namespace
+
+// The device address of `scratch` on `device`, looked up once per device with that device current.
+static gs_status scratch_address(int device, char** out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  auto it = g_scratch_addr.find(device);
+  if (it != g_scratch_addr.end()) { *out = it->second; return GS_OK; }
+  int prev = -1;
+  if (hipGetDevice(&prev) != hipSuccess) return GS_DEVICE_ERROR;
+  if (prev != device && hipSetDevice(device) != hipSuccess) return GS_DEVICE_ERROR;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (prev != device) (void)hipSetDevice(prev);
+  if (e != hipSuccess || !ptr) return GS_DEVICE_ERROR;
+  g_scratch_addr[device] = static_cast<char*>(ptr);
+  *out = static_cast<char*>(ptr);
+  return GS_OK;
+}
+
 // ================================================================================================ plans
 namespace gs {
 
@@ -1427,6 +1456,50 @@ gs_status seam_solve_one(gs_ctx* ctx, int n, const int* rowptr, const int* colid
   return GS_OK;
 }
 
+// gs_sparse_solve with no device allocation: the factorisation's plan is made on the host as gs_sparse_solve makes a
+// new one, and its image and the call's buffers are written into the context's device's scratch array (address
+// ctx->scratch, from hipGetSymbolAddress at gs_init), then the same kernel replays the factorisation and solves. The
+// same arithmetic as gs_sparse_solve, so the same bits.
+gs_status seam_solve_noalloc(gs_ctx* ctx, int n, const int* rowptr, const int* colidx, const double* val,
+                             const double* b, double* x) {
+  if (gs_status st = seam_check(ctx, n, rowptr, colidx, val, b, x)) return st;
+  if (n == 0) return GS_OK;
+  if (!ctx->scratch) fail(GS_DEVICE_ERROR, "the scratch array has no device address on this device");
+  const int nnz = rowptr[n];
+  const std::vector<int>& order = seam_order(ctx, n, rowptr, colidx);
+  Plan plan;
+  std::string why;
+  std::vector<int> rp(rowptr, rowptr + n + 1), ci(colidx, colidx + nnz);
+  if (!gp_plan(n, rp, ci, val, order, 0.0, false, plan, why)) {
+    ctx->err = why;
+    return GS_SINGULAR;
+  }
+  if (gs_status st = seam_rhs_check(ctx, n, b)) return st;
+  JacobianPattern jp;
+  jp.m = n;
+  jp.rp = std::move(rp);
+  jp.ci = std::move(ci);
+  PlanImage img;
+  DevPlan h;
+  DevicePlanHost dph;
+  build_plan_image(img, h, dph, nullptr, jp, nullptr, plan, 100.0);
+  const size_t plan_bytes = (img.bytes.size() + 255) & ~size_t(255);
+  const SeamLayout L = seam_layout(h, nnz);
+  if (plan_bytes + L.total > kNoAllocScratchBytes)
+    fail(GS_INVALID_ARGUMENT, "the system needs " + std::to_string(plan_bytes + L.total) +
+                                  " bytes of device scratch; gs_sparse_solve_noalloc holds " +
+                                  std::to_string(kNoAllocScratchBytes));
+  hipStream_t s = ctx->stream;
+  check(hipMemcpyAsync(ctx->scratch, img.bytes.data(), img.bytes.size(), hipMemcpyHostToDevice, s), "scratch plan");
+  img.bind(h, ctx->scratch);
+  std::vector<double> xt;
+  // (seam_run finishes the stream before it returns, so the image above is consumed before it goes)
+  if (seam_run(h, ctx->scratch + plan_bytes, L, nnz, val, b, xt, s) != 0)
+    fail(GS_DEVICE_ERROR, "the device refused the replay of its own factorisation");
+  std::copy(xt.begin(), xt.end(), x);
+  return GS_OK;
+}
+
 }  // namespace
 
 // ================================================================================================ multi-network batches
@@ -1565,6 +1638,12 @@ GS_API gs_status gs_init(int32_t device, size_t hbm_budget_bytes, gs_ctx** out) 
     delete ctx;
     return GS_DEVICE_ERROR;
   }
+  // set-up of the no-allocation entry point: this device's address of its scratch array
+  if (scratch_address(device, &ctx->scratch) != GS_OK) {
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return GS_DEVICE_ERROR;
+  }
   *out = ctx;
   return GS_OK;
 }
@@ -1605,6 +1684,11 @@ GS_API gs_status gs_device_info_get(gs_ctx* ctx, gs_device_info* out) {
 GS_API gs_status gs_sparse_solve(gs_ctx* ctx, int32_t n, const int32_t* rowptr, const int32_t* colidx,
                                  const double* val, const double* b, double* x) {
   return guarded(ctx, [&]() { return seam_solve_one(ctx, n, rowptr, colidx, val, b, x); });
+}
+
+GS_API gs_status gs_sparse_solve_noalloc(gs_ctx* ctx, int32_t n, const int32_t* rowptr, const int32_t* colidx,
+                                         const double* val, const double* b, double* x) {
+  return guarded(ctx, [&]() { return seam_solve_noalloc(ctx, n, rowptr, colidx, val, b, x); });
 }
 
 GS_API gs_status gs_sparse_solve_batch(gs_ctx* ctx, int32_t count, const gs_csr* systems, const double* const* b,

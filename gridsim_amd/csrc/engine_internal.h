@@ -196,6 +196,7 @@ struct gs_ctx {
   // key: n, rowptr, colidx, -1, positions of the nonzero values -> the device plan of the last factorisation
   gs::LruCache<std::vector<int>, std::shared_ptr<gs::SeamPlan>> seam_plans{gs::kSeamPlanCap};
   gs::DevMem seam_buf;        // the seam's per-call device buffers (grown to the largest system, reused)
+  char* scratch = nullptr;    // device address of the no-allocation entry point's scratch array on this device
   int64_t seam_plan_hits = 0, seam_plan_builds = 0;
 };
 

@@ -191,6 +191,15 @@ GS_API const char* gs_version(void);
 GS_API gs_status gs_sparse_solve(gs_ctx* ctx, int32_t n, const int32_t* rowptr, const int32_t* colidx,
                                  const double* val, const double* b, double* x);
 
+/* gs_sparse_solve that allocates no device memory (no counterpart in the reference: for callers that cannot allocate,
+ * such as code under HIP graph capture or a serving loop that must not call hipMalloc). The same contract, arithmetic
+ * and results as gs_sparse_solve; the plan and the call's device buffers go into a fixed scratch array of 64 MiB that
+ * each GPU holds until the process ends (no plan is cached between calls). A system needing more than that returns
+ * GS_INVALID_ARGUMENT (gs_last_error says how much it needs). Calls on contexts of the same GPU share the array: they
+ * must not overlap (calls on one context never do, the context serialises them). */
+GS_API gs_status gs_sparse_solve_noalloc(gs_ctx* ctx, int32_t n, const int32_t* rowptr, const int32_t* colidx,
+                                         const double* val, const double* b, double* x);
+
 typedef struct gs_csr {
   int32_t n;
   int32_t reserved;
