@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdlib>
 #include <chrono>
 #include <cmath>
@@ -11,6 +12,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "engine_internal.h"
@@ -246,12 +248,13 @@ constexpr double kDiagonalPreference = 0.1;
 // growth_tol: the growth check the plan's pivots will face; the diagonal is preferred only where it also passes that
 // check, so a plan always passes it at the state it was made at.
 bool symbolic_pass(const YBus& y, const JacobianPattern& jp, const double* vm, const double* va, Plan& plan,
-                   std::string& why, double growth_tol) {
+                   std::string& why, double growth_tol, const std::vector<int>* order) {
   std::vector<double> p(y.n), q(y.n), jv(jp.rp[jp.m]);
   host_injections(y, vm, va, p.data(), q.data());
   host_jacobian(y, jp, vm, va, p.data(), q.data(), jv.data());
-  std::vector<int> order = minimum_degree(jp.m, jp.rp.data(), jp.ci.data());
-  return gp_plan(jp.m, jp.rp, jp.ci, jv.data(), order, std::max(kDiagonalPreference, growth_tol), true, plan, why);
+  const std::vector<int> own = order ? std::vector<int>() : minimum_degree(jp.m, jp.rp.data(), jp.ci.data());
+  return gp_plan(jp.m, jp.rp, jp.ci, jv.data(), order ? *order : own, std::max(kDiagonalPreference, growth_tol), true,
+                 plan, why);
 }
 
 gs_nr_options normalise(const gs_nr_options* o) {
@@ -711,6 +714,80 @@ void copy_state_out(gs_ctx* ctx, const DevMem& inst_buf, const Layout& L, int i,
   if (va) check(hipMemcpyAsync(va, b + L.va, 8 * (size_t)n, hipMemcpyDeviceToHost, ctx->stream), "va out");
 }
 
+std::vector<int> OrderCache::order(const std::vector<int>& rp, const std::vector<int>& ci, int m) {
+  // FNV-1a over the four bytes of every integer, least significant first
+  uint64_t h = 14695981039346656037ull;
+  auto mix = [&h](int v) {
+    const uint32_t u = (uint32_t)v;
+    for (int b = 0; b < 4; b++) { h ^= (u >> (8 * b)) & 0xFF; h *= 1099511628211ull; }
+  };
+  mix(m);
+  for (int v : rp) mix(v);
+  for (int v : ci) mix(v);
+  {
+    std::lock_guard<std::mutex> lock(mu_);
+    if (std::shared_ptr<Entry>* e = map_.find(h))
+      if ((*e)->rp == rp && (*e)->ci == ci) {
+        hits_++;
+        return (*e)->order;
+      }
+    misses_++;
+  }
+  auto e = std::make_shared<Entry>();
+  e->rp = rp;
+  e->ci = ci;
+  e->order = minimum_degree(m, rp.data(), ci.data());
+  std::vector<int> order = e->order;
+  std::lock_guard<std::mutex> lock(mu_);
+  map_.put(h, e);
+  return order;
+}
+
+void fresh_symbolic(const Network& base, bool flat_start, const gs_nr_options& o, int count, const gs_instance* inst,
+                    const std::vector<const double*>& vm, const std::vector<const double*>& va,
+                    const std::vector<int>* types, const std::vector<double>* qspec, int threads, OrderCache& cache,
+                    std::vector<FreshSymbolic>& out) {
+  out.assign(count, FreshSymbolic{});
+  auto one = [&](int i) {
+    FreshSymbolic& f = out[i];
+    f.own = instance_network(base, inst[i]);
+    f.y = build_ybus(f.own, -1);
+    f.spec = build_bus_spec(f.own, flat_start, 1.0);
+    if (types) f.spec.type = *types;
+    if (qspec) f.spec.qspec = *qspec;
+    f.jp = build_jacobian_pattern(f.y, f.spec.type);
+    const std::vector<int> order = cache.order(f.jp.rp, f.jp.ci, f.jp.m);
+    f.ok = symbolic_pass(f.y, f.jp, vm[i], va[i], f.plan, f.why, o.growth_tol, &order);
+  };
+  const int workers = std::max(1, std::min(threads, count));
+  if (workers == 1) {
+    for (int i = 0; i < count; i++) one(i);
+    return;
+  }
+  std::atomic<int> next{0};
+  std::vector<std::thread> pool;
+  std::vector<std::string> errors(workers);
+  for (int w = 0; w < workers; w++)
+    pool.emplace_back([&, w]() {
+      try {
+        for (int i; (i = next.fetch_add(1)) < count;) one(i);
+      } catch (const std::exception& e) {
+        errors[w] = e.what();
+      } catch (const HipError& e) {
+        errors[w] = e.what;
+      }
+    });
+  for (auto& t : pool) t.join();
+  for (const std::string& e : errors)
+    if (!e.empty()) fail(GS_INVALID_ARGUMENT, "symbolic pass: " + e);
+}
+
+// Host threads for the fresh symbolic passes of one call: the cores this process may use, at most 16.
+int symbolic_threads() {
+  const unsigned hc = std::thread::hardware_concurrency();
+  return (int)std::max(1u, std::min(hc ? hc : 1u, 16u));
+}
+
 // The network an instance solves: the uploaded one with its outaged branches out of service and every load and
 // generator P scaled by its load multiplier.
 Network instance_network(const Network& base, const gs_instance& in) {
@@ -827,24 +904,41 @@ void iterative_step(gs_ctx* ctx, const YBus& y, const JacobianPattern& jp, const
 void finish_paused(gs_net* net, const gs_instance& in, const gs_nr_options& o, bool plain, InstState st,
                    std::vector<double>& vm, std::vector<double>& va, gs_nr_result& result, double& kms, int& launches,
                    std::vector<double>* p_out, std::vector<double>* q_out, const std::vector<int>* types,
-                   const std::vector<double>* qspec, InstState* st_out) {
+                   const std::vector<double>* qspec, InstState* st_out, FreshSymbolic* first) {
   gs_ctx* ctx = net->ctx;
   const int n = net->net.n;
   int refreshes = 0;
-  Network own = instance_network(net->net, in);
-  YBus y = build_ybus(own, -1);
-  BusSpec spec = build_bus_spec(own, o.flat_start != 0, 1.0);
-  if (types) spec.type = *types;
-  if (qspec) spec.qspec = *qspec;
-  JacobianPattern jp = build_jacobian_pattern(y, spec.type);
+  FreshSymbolic fs;
+  if (first) {
+    fs = std::move(*first);
+  } else {
+    std::vector<FreshSymbolic> one;
+    fresh_symbolic(net->net, o.flat_start != 0, o, 1, &in, {vm.data()}, {va.data()}, types, qspec, 1, net->orders,
+                   one);
+    fs = std::move(one[0]);
+  }
+  const Network& own = fs.own;
+  const YBus& y = fs.y;
+  const BusSpec& spec = fs.spec;
+  const JacobianPattern& jp = fs.jp;
   DevicePlan dp;
   DevMem ib, db, sb, ab;
   bool finished = false;
   int iterative = 0;
+  bool have_first = true;   // the first pass is done (fs.plan, fs.ok)
   while (!finished) {
     Plan plan;
     std::string why;
-    if (!symbolic_pass(y, jp, vm.data(), va.data(), plan, why, o.growth_tol)) {
+    bool ok;
+    if (have_first) {
+      plan = std::move(fs.plan);
+      ok = fs.ok;
+      have_first = false;
+    } else {
+      const std::vector<int> order = net->orders.order(jp.rp, jp.ci, jp.m);
+      ok = symbolic_pass(y, jp, vm.data(), va.data(), plan, why, o.growth_tol, &order);
+    }
+    if (!ok) {
       // singular at this state: the LU refuses the step, so SolveStep falls back to ILU(0) + BiCGSTAB for it (on the
       // device); the LU is tried again at the next step
       if (++iterative > o.max_iterations + 1) {
@@ -927,13 +1021,29 @@ void collect_results(gs_net* net, int count, const gs_instance* inst, const gs_n
     check(hipMemcpy2DAsync(va_out, 8 * (size_t)n, b + L.va, L.stride, 8 * (size_t)n, (size_t)count,
                            hipMemcpyDeviceToHost, ctx->stream), "va out");
   check(hipStreamSynchronize(ctx->stream), "sync out");
-  for (int i = 0; i < count; i++) {
-    if (states[i].done == 1) continue;
-    std::vector<double> vm(n), va(n), p, q;
-    copy_state_out(ctx, inst_buf, L, i, n, vm.data(), va.data());
-    check(hipStreamSynchronize(ctx->stream), "sync pause");
+  // paused instances: their states, their first fresh symbolic passes on host threads (independent of each other),
+  // then each one's resumed launches in turn
+  std::vector<int> paused;
+  for (int i = 0; i < count; i++)
+    if (states[i].done != 1) paused.push_back(i);
+  if (paused.empty()) return;
+  const int k = (int)paused.size();
+  std::vector<std::vector<double>> vms(k, std::vector<double>(n)), vas(k, std::vector<double>(n));
+  for (int j = 0; j < k; j++) copy_state_out(ctx, inst_buf, L, paused[j], n, vms[j].data(), vas[j].data());
+  check(hipStreamSynchronize(ctx->stream), "sync pause");
+  std::vector<gs_instance> pin(k);
+  std::vector<const double*> pvm(k), pva(k);
+  for (int j = 0; j < k; j++) { pin[j] = inst[paused[j]]; pvm[j] = vms[j].data(); pva[j] = vas[j].data(); }
+  std::vector<FreshSymbolic> firsts;
+  fresh_symbolic(net->net, o.flat_start != 0, o, k, pin.data(), pvm, pva, nullptr, nullptr, symbolic_threads(),
+                 net->orders, firsts);
+  for (int j = 0; j < k; j++) {
+    const int i = paused[j];
+    std::vector<double>& vm = vms[j];
+    std::vector<double>& va = vas[j];
+    std::vector<double> p, q;
     finish_paused(net, inst[i], o, plain, states[i], vm, va, results[i], kms, launches, finals ? &p : nullptr,
-                  finals ? &q : nullptr, nullptr, nullptr, final_states ? &(*final_states)[i] : nullptr);
+                  finals ? &q : nullptr, nullptr, nullptr, final_states ? &(*final_states)[i] : nullptr, &firsts[j]);
     if (finals) finals->push_back(HostFinal{i, vm, va, p, q});
     if (vm_out) std::copy(vm.begin(), vm.end(), vm_out + (size_t)n * i);
     if (va_out) std::copy(va.begin(), va.end(), va_out + (size_t)n * i);
@@ -1886,6 +1996,56 @@ GS_API gs_status gs_plan_build(const gs_network* net, const gs_nr_options* opts,
     compile_plan(p->plan, p->dph, solve_scratch_cap());
     *out = p.release();
     return GS_OK;
+  } catch (const HipError& e) {
+    return e.code;
+  } catch (...) {
+    return GS_INVALID_ARGUMENT;
+  }
+}
+
+GS_API gs_status gs_plan_fresh_batch(const gs_network* net, const gs_nr_options* opts, int32_t count,
+                                     const gs_instance* inst, const double* vm, const double* va, int32_t threads,
+                                     gs_plan** plans, int64_t* order_cache_hits) {
+  if (!net || net->n_bus <= 0 || count < 0 || (count > 0 && (!inst || !plans))) return GS_INVALID_ARGUMENT;
+  try {
+    const gs_nr_options o = normalise(opts);
+    const Network base = copy_network(*net);
+    check_instances(base.nb, count, inst, "");
+    for (int i = 0; i < count; i++) plans[i] = nullptr;
+    const int n = base.n;
+    // each instance's state: the given one, or its own start point
+    std::vector<std::vector<double>> own_vm(count), own_va(count);
+    std::vector<const double*> pvm(count), pva(count);
+    for (int i = 0; i < count; i++) {
+      if (vm && va) {
+        pvm[i] = vm + (size_t)n * i;
+        pva[i] = va + (size_t)n * i;
+      } else {
+        const BusSpec sp = build_bus_spec(instance_network(base, inst[i]), o.flat_start != 0, 1.0);
+        own_vm[i] = inst[i].vm0 ? std::vector<double>(inst[i].vm0, inst[i].vm0 + n) : sp.vm;
+        own_va[i] = inst[i].va0 ? std::vector<double>(inst[i].va0, inst[i].va0 + n) : sp.va;
+        pvm[i] = own_vm[i].data();
+        pva[i] = own_va[i].data();
+      }
+    }
+    OrderCache cache;
+    std::vector<FreshSymbolic> fs;
+    fresh_symbolic(base, o.flat_start != 0, o, count, inst, pvm, pva, nullptr, nullptr, std::max(1, threads), cache,
+                   fs);
+    if (order_cache_hits) *order_cache_hits = cache.hits();
+    gs_status worst = GS_OK;
+    for (int i = 0; i < count; i++) {
+      if (!fs[i].ok) { worst = GS_SINGULAR; continue; }
+      auto p = std::make_unique<gs_plan>();
+      p->net = std::move(fs[i].own);
+      p->y = std::move(fs[i].y);
+      p->spec = std::move(fs[i].spec);
+      p->jp = std::move(fs[i].jp);
+      p->plan = std::move(fs[i].plan);
+      compile_plan(p->plan, p->dph, solve_scratch_cap());
+      plans[i] = p.release();
+    }
+    return worst;
   } catch (const HipError& e) {
     return e.code;
   } catch (...) {

@@ -178,6 +178,23 @@ class LruCache {
   std::map<K, std::pair<V, typename std::list<K>::iterator>> map_;
 };
 
+// Minimum-degree orders by Jacobian pattern, shared by the fresh symbolic passes of paused instances (the order
+// depends on the pattern alone, as CpuSparseLuSolver caches it per pattern, CpuSparseLuSolver.cs:30-39). Keyed by a
+// 64-bit FNV-1a hash of (m, rowptr, colidx) with the pattern kept to confirm a hit; at most kOrderCacheCap patterns.
+// Thread-safe.
+class OrderCache {
+ public:
+  std::vector<int> order(const std::vector<int>& rp, const std::vector<int>& ci, int m);
+  int64_t hits() const { return hits_; }
+  int64_t misses() const { return misses_; }
+
+ private:
+  struct Entry { std::vector<int> rp, ci, order; };
+  std::mutex mu_;
+  LruCache<uint64_t, std::shared_ptr<Entry>> map_{256};
+  int64_t hits_ = 0, misses_ = 0;
+};
+
 // The sparse-solver seam keeps the minimum-degree order of its last kSeamOrderCap patterns and the device plans of its
 // last kSeamPlanCap structures (pattern and nonzero positions).
 constexpr size_t kSeamOrderCap = 256, kSeamPlanCap = 16;
@@ -220,6 +237,7 @@ struct gs_net {
   // per-call device arenas
   gs::RunArena arena;
   gs::DevMem screen_buf, loading_buf, flow_buf;
+  gs::OrderCache orders;      // minimum-degree orders of the patterns its paused instances' symbolic passes met
   double last_total_ms = 0, last_kernel_ms = 0;
   int last_launches = 0;
 };
@@ -270,7 +288,9 @@ void upload_plan(DevicePlan& dp, const YBus* y, const JacobianPattern& jp, const
                  double base_mva, hipStream_t s);
 gs_plan_stats plan_stats(const DevicePlanHost& d, const YBus& y, const JacobianPattern& jp);
 bool symbolic_pass(const YBus& y, const JacobianPattern& jp, const double* vm, const double* va, Plan& plan,
-                   std::string& why, double growth_tol);
+                   std::string& why, double growth_tol, const std::vector<int>* order = nullptr);
+Network instance_network(const Network& base, const gs_instance& in);
+int symbolic_threads();
 gs_nr_options normalise(const gs_nr_options* o);
 Layout instance_layout(const DevPlan& h, bool fp32, bool x_lds);
 BatchRun plan_run(const DevPlan& h, const gs_nr_options& o, bool force_fp64);
@@ -294,11 +314,28 @@ void launch_prepared(const DevPlan& h, const PreparedRun& pr, int schedule, hipS
 // A paused instance (a static pivot failed its growth check) finished by fresh host symbolic passes and resumed
 // launches; vm / va (in and out) hold its state.
 // types / qspec (or null): bus types and Q specification other than the network's (a reactive-limit pass).
+// The host half of one paused instance's first fresh symbolic pass: its own network (outages out, loads scaled), Y-bus,
+// specification, Jacobian pattern and the plan at its paused state (ok false: the LU refuses that state).
+struct FreshSymbolic {
+  Network own;
+  YBus y;
+  BusSpec spec;
+  JacobianPattern jp;
+  Plan plan;
+  bool ok = false;
+  std::string why;
+};
+// FreshSymbolic of `count` instances at their states (vm / va per instance) on up to `threads` host threads, the
+// orders from `cache`. types / qspec (or null): bus types and Q specification other than the network's.
+void fresh_symbolic(const Network& base, bool flat_start, const gs_nr_options& o, int count, const gs_instance* inst,
+                    const std::vector<const double*>& vm, const std::vector<const double*>& va,
+                    const std::vector<int>* types, const std::vector<double>* qspec, int threads, OrderCache& cache,
+                    std::vector<FreshSymbolic>& out);
 void finish_paused(gs_net* net, const gs_instance& in, const gs_nr_options& o, bool plain, InstState st,
                    std::vector<double>& vm, std::vector<double>& va, gs_nr_result& result, double& kms, int& launches,
                    std::vector<double>* p_out = nullptr, std::vector<double>* q_out = nullptr,
                    const std::vector<int>* types = nullptr, const std::vector<double>* qspec = nullptr,
-                   InstState* st_out = nullptr);
+                   InstState* st_out = nullptr, FreshSymbolic* first = nullptr);
 // The branch data screening reads (end buses, stamps, rate A) and the buses' own voltage limits, uploaded with a
 // network.
 void upload_branches(DevicePlan& dp, const Network& net, hipStream_t s);

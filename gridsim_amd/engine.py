@@ -203,12 +203,15 @@ def host_sincos(x):
 class HostPlan:
     """The symbolic plan the engine would upload for a network (gs_plan_build), computed on the host."""
 
-    def __init__(self, net: Network, options: NrOptions | None = None):
-        opts = (options or NrOptions()).c()
-        h = C.c_void_p()
-        st = lib().gs_plan_build(NetView(net).ref, C.byref(opts), C.byref(h))
-        if st != GS_OK:
-            raise GsError(st, "gs_plan_build")
+    def __init__(self, net: Network, options: NrOptions | None = None, handle=None):
+        if handle is None:
+            opts = (options or NrOptions()).c()
+            h = C.c_void_p()
+            st = lib().gs_plan_build(NetView(net).ref, C.byref(opts), C.byref(h))
+            if st != GS_OK:
+                raise GsError(st, "gs_plan_build")
+        else:
+            h = handle
         self.h = h
         m, nj, nl, nu, lv = (C.c_int32() for _ in range(5))
         lib().gs_plan_sizes(h, C.byref(m), C.byref(nj), C.byref(nl), C.byref(nu), C.byref(lv))
@@ -232,6 +235,29 @@ class HostPlan:
             lib().gs_plan_free(self.h)
         except Exception:
             pass
+
+
+def fresh_plans(net: Network, outages, load_scale=None, options: NrOptions | None = None, states=None,
+                threads: int = 8):
+    """The engine's fresh symbolic passes for instances of `net` (gs_plan_fresh_batch) on `threads` host threads:
+    ([HostPlan or None per instance], minimum-degree orders the pattern cache supplied). states: (vm, va) per instance
+    ([count, n] each), or None for each instance's start point."""
+    probe = DeviceNetwork.__new__(DeviceNetwork)
+    probe.net = net
+    arr, count, keep = probe._instances(outages, load_scale)
+    handles = (C.c_void_p * max(count, 1))()
+    hits = C.c_int64()
+    opts = (options or NrOptions()).c()
+    vm = va = None
+    if states is not None:
+        vm = np.ascontiguousarray(states[0], np.float64)
+        va = np.ascontiguousarray(states[1], np.float64)
+    st = lib().gs_plan_fresh_batch(NetView(net).ref, C.byref(opts), C.c_int32(count), arr, _p(vm), _p(va),
+                                   C.c_int32(threads), handles, C.byref(hits))
+    if st not in (GS_OK, GS_SINGULAR):
+        raise GsError(st, "gs_plan_fresh_batch")
+    plans = [HostPlan(net, handle=C.c_void_p(handles[i])) if handles[i] else None for i in range(count)]
+    return plans, hits.value
 
 
 class Context:

@@ -283,6 +283,16 @@ GS_API gs_status gs_plan_sizes(const gs_plan* plan, int32_t* m, int32_t* nnz_j, 
 GS_API gs_status gs_plan_get(const gs_plan* plan, int32_t* q, int32_t* pivot_row, int32_t* lp, int32_t* li,
                              int32_t* up, int32_t* ui, int32_t* j_rowptr, int32_t* j_colidx);
 GS_API void gs_plan_free(gs_plan* plan);
+/* The fresh symbolic passes the engine runs for paused instances (a static pivot failed its check), on the host alone:
+ * for each instance its own network (outages out, loads scaled), Jacobian pattern, minimum-degree order and
+ * Gilbert-Peierls plan at its state (vm / va: count * n_bus values, or null for each instance's start point), on up to
+ * `threads` host threads, with the orders shared through a cache keyed by the Jacobian pattern (the order depends on
+ * the pattern alone, as CpuSparseLuSolver.cs:30-39 caches it). plans[count] receive handles (null where the LU refuses
+ * that state; the call then returns GS_SINGULAR); *order_cache_hits (may be null) the orders the cache supplied. No
+ * counterpart in the reference, which orders every outage's pattern in its own solver. */
+GS_API gs_status gs_plan_fresh_batch(const gs_network* net, const gs_nr_options* opts, int32_t count,
+                                     const gs_instance* inst, const double* vm, const double* va, int32_t threads,
+                                     gs_plan** plans, int64_t* order_cache_hits);
 /* gs_network_plan_stats of the plan gs_network_upload would upload. */
 GS_API gs_status gs_plan_shape(const gs_plan* plan, gs_plan_stats* out);
 /* Runs the device factorisation and solves of the plan in a host emulation with every index checked, on the

@@ -191,3 +191,44 @@ def test_instance_records_filled_as_columns(eng):
     assert arr[1].outage_branch == 4
     assert np.array_equal(np.ctypeslib.as_array((C.c_double * 11).from_address(arr[1].va0)), va[1])
     assert dn._instances(outages=[], starts=([], []))[1] == 0
+
+
+def test_fresh_symbolic_passes_are_thread_independent_and_share_orders(eng, oracle):
+    """The host's fresh symbolic passes of paused instances (gs_plan_fresh_batch) give the same plans on 1 and on 8
+    threads, take each instance's order from the pattern cache when its outage set repeats (the order depends on the
+    pattern alone), and equal the oracle's factorisation of the same reduced network at the same state (ToCsc keeping
+    zeros, diagonal preference 0.1): same order and pivot rows."""
+    import ctypes as C
+    from gridsim_amd import gridgen
+    net = gridgen.config1_grid()
+    rec, vm, va = oracle.solve(net, oracle.options(flat_start=True))
+    warm = net.with_start(vm, np.degrees(va))
+    outs = list(gridgen.config2_outages(warm)[:12])
+    sets = outs + outs[:6]                 # six outage sets repeat: their patterns are met twice
+    scales = [1.0] * 12 + [0.9] * 6
+    one, hits1 = eng.fresh_plans(warm, sets, scales, threads=1)
+    many, hits8 = eng.fresh_plans(warm, sets, scales, threads=8)
+    assert hits1 == 6 and hits8 >= 0
+    for a, b in zip(one, many):
+        for k in ("q", "pivot_row", "lp", "li", "up", "ui", "j_rowptr", "j_colidx"):
+            assert np.array_equal(getattr(a, k), getattr(b, k)), k
+    # against the oracle at the instance's own start state
+    for j in (0, 5, 14):
+        o, sc = sets[j], scales[j]
+        red = warm.with_load_multiplier(sc)
+        rp, ci, val = oracle.jacobian(red, warm.vm, np.radians(warm.va_deg), skip=int(o))
+        n = len(rp) - 1
+        q = np.empty(n, np.int32)
+        oracle.lib().orc_md_order(C.c_int32(n), oracle._p(rp), oracle._p(ci), oracle._p(q))
+        h = oracle.lib().orc_lu_factor(C.c_int32(n), oracle._p(rp), oracle._p(ci), oracle._p(val), oracle._p(q),
+                                       C.c_int32(1), C.c_double(0.1))
+        assert h
+        lnz, unz, rn = C.c_int32(), C.c_int32(), C.c_int32()
+        oracle.lib().orc_lu_sizes(C.c_void_p(h), C.byref(lnz), C.byref(unz), C.byref(rn))
+        arrs = [np.empty(k, t) for k, t in ((n + 1, np.int32), (lnz.value, np.int32), (lnz.value, np.float64),
+                                            (n + 1, np.int32), (unz.value, np.int32), (unz.value, np.float64),
+                                            (n, np.int32), (n, np.int32), (n, np.int32), (n + 1, np.int32),
+                                            (rn.value, np.int32))]
+        oracle.lib().orc_lu_get(C.c_void_p(h), *(oracle._p(a) for a in arrs))
+        oracle.lib().orc_lu_free(C.c_void_p(h))
+        assert np.array_equal(one[j].q, arrs[7]) and np.array_equal(one[j].pivot_row, arrs[8]), j
