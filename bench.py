@@ -12,6 +12,7 @@ CPU oracle (a C++ restatement of the reference's CPU sparse NR) timed on this ho
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -59,12 +60,22 @@ def measured_traffic(stats: dict, lu: str, flags: int, alg: int):
     return int(rec["read_bytes"] + rec["write_bytes"]), rec["source"]
 
 
+def _schedule_name(opts):
+    from gridsim_amd import engine
+    if opts.schedule == engine.SCHEDULE_LEVELS:
+        return "levels"
+    if opts.schedule == engine.SCHEDULE_GROUP:
+        w = (opts.flags >> 8) & 15
+        return f"work-group of {w or 8} waves per instance"
+    return "one wavefront per instance" + (f", launches of {opts.schedule}" if opts.schedule > 0 else "")
+
+
 def run_config3(args):
     """512 instances, a third each on 300-, 2,000- and 10,000-bus grids (branches 1.35 n, 15 % PV), each with a load
     multiplier U[0.8, 1.2] and one random non-bridge branch outage, warm-started from its grid's base case; options
-    autotuned per grid (the level-parallel schedule offered to the largest); all 512 solved by one replay of a
-    captured hipGraph (each grid's launches on its own branch, concurrent: one wavefront kernel, or the level-parallel
-    schedule's unrolled Newton steps)."""
+    autotuned per grid (a wavefront or a work-group of waves per instance; the level-parallel schedule offered to the
+    largest too); all 512 solved by one replay of a captured hipGraph (each grid's launches on its own branch,
+    concurrent: one kernel per grid, or the level-parallel schedule's unrolled Newton steps)."""
     from gridsim_amd import engine, gridgen
     t0 = time.time()
     ctx = engine.Context(0)
@@ -85,14 +96,15 @@ def run_config3(args):
         outs = rng.choice(gridgen.config2_outages(warm), count, replace=False)
         scales = rng.uniform(0.8, 1.2, count)
         dn = engine.DeviceNetwork(ctx, warm, engine.NrOptions(max_iterations=10))
-        # the level-parallel schedule is offered to the largest grid only: a graph runs two groups' level-parallel
-        # step chains one after the other (measured), while a wavefront group's single kernel overlaps the chain
+        # the level-parallel schedule and a work-group of waves per instance are offered to the largest grid only: a
+        # graph runs two groups' level-parallel step chains one after the other (measured), and a wavefront group's
+        # single kernel overlaps the chain, where work-groups of many waves would hold the compute units it needs
         offer = engine.SCHEDULE_LEVELS if n == max(sizes) else 0
         opts, ms = engine.autotune(dn, outages=outs, load_scale=scales,
                                    options=engine.NrOptions(max_iterations=10, schedule=offer))
         tuned.append({"n_bus": n, "lu": "fp32" if opts.lu_precision == engine.LU_FP32 else "fp64",
                       "refine_steps": opts.refine_steps,
-                      "schedule": "levels" if opts.schedule == engine.SCHEDULE_LEVELS else opts.schedule,
+                      "schedule": _schedule_name(opts),
                       "solve_vector": "global" if opts.flags & engine.FLAG_X_GLOBAL else "lds when it fits",
                       "sample_kernel_ms": round(ms, 3)})
         print(f"# {n} buses: m={len(dn.plan()['q'])}, autotuned {tuned[-1]}", file=sys.stderr, flush=True)
@@ -323,9 +335,11 @@ def run_config5(args):
                                                            refine_steps=1))
         o_all = gridgen.config2_outages(w)
         so = np.where(rng.random(16) < 0.5, rng.choice(o_all, 16), -1)
-        o, ms = engine.autotune(dn, outages=so, load_scale=rng.uniform(0.8, 1.2, 16), options=dn.options)
+        # (a work-group of waves per instance offered too: the serving loop runs either)
+        o, ms = engine.autotune(dn, outages=so, load_scale=rng.uniform(0.8, 1.2, 16),
+                                options=dataclasses.replace(dn.options, schedule=engine.SCHEDULE_GROUP))
         tuned.append({"n_bus": w.n_bus, "lu": "fp32" if o.lu_precision == engine.LU_FP32 else "fp64",
-                      "refine_steps": o.refine_steps, "schedule": o.schedule})
+                      "refine_steps": o.refine_steps, "schedule": _schedule_name(o)})
         print(f"# {w.n_bus} buses: {tuned[-1]}", file=sys.stderr, flush=True)
         warm.append(w); dns.append(dn); opts.append(o); outs.append(o_all)
     srv = engine.Server(ctx)
@@ -498,7 +512,7 @@ def main():
         opts = fp32 if args.lu == "fp32" else fp64
     lu_name = _lu_name(opts)
     x_place = "global" if opts.flags & engine.FLAG_X_GLOBAL else "lds when it fits"
-    print(f"# arithmetic: {lu_name}, schedule {opts.schedule}, solve vector {x_place}" +
+    print(f"# arithmetic: {lu_name}, schedule {_schedule_name(opts)}, solve vector {x_place}" +
           (" (autotuned)" if tuned_ms else ""), file=sys.stderr, flush=True)
     print(f"# setup {time.time() - t0:.1f}s: {net.n_bus} buses, {net.n_branch} branches, {len(outages)} outages, "
           f"m={len(plan['q'])}, nnz(J)={len(plan['j_colidx'])}, nnz(L+U)={len(plan['li']) + len(plan['ui'])}",
@@ -610,7 +624,7 @@ def main():
         "config": {"workload": "batched N-1 screen, 2000-bus synthetic grid (1 slack / 300 PV), every non-bridge "
                                "single-branch outage from the base-case voltages, max 10 iterations, 1e-8 pu",
                    "n_bus": net.n_bus, "n_branch": net.n_branch, "outages": int(len(outages)),
-                   "seed": int(net.seed), "lu": lu_name, "schedule": opts.schedule, "solve_vector": x_place,
+                   "seed": int(net.seed), "lu": lu_name, "schedule": _schedule_name(opts), "solve_vector": x_place,
                    "autotuned": tuned_ms is not None,
                    "device": info["name"] or info["arch"]},
         "latency_ms": {"p50": serving["p50_ms"], "p99": serving["p99_ms"],

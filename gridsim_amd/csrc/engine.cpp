@@ -162,6 +162,30 @@ void build_plan_image(PlanImage& img, DevPlan& h, DevicePlanHost& d, const YBus*
   img.put(&DevPlan::rs_chunk_row, d.rs_chunk_row);
   img.put(&DevPlan::fs_pass_step, d.fs_pass_step);
   img.put(&DevPlan::bs_pass_step, d.bs_pass_step);
+  // the group schedule's segments: levels of more than `narrow` chunks (passes) split between the waves, runs of the
+  // narrower levels walked by one wave
+  std::vector<int> gseg;
+  auto segs = [&gseg](const std::vector<int>& level_first, int narrow) {
+    int run0 = -1, run1 = -1, n = 0;
+    auto put = [&](int a, int b, int split) { gseg.insert(gseg.end(), {a, b, split, 0}); n++; };
+    for (size_t l = 0; l + 1 < level_first.size(); l++) {
+      const int a = level_first[l], b = level_first[l + 1];
+      if (b <= a) continue;
+      if (b - a <= narrow) {
+        if (run0 < 0) run0 = a;
+        run1 = b;
+        continue;
+      }
+      if (run0 >= 0) { put(run0, run1, 0); run0 = -1; }
+      put(a, b, 1);
+    }
+    if (run0 >= 0) put(run0, run1, 0);
+    return n;
+  };
+  h.n_gseg_rs = segs(d.rs_level_chunk, kGroupNarrow);
+  h.n_gseg_fs = segs(d.fs_level_pass, kGroupNarrowSolve);
+  h.n_gseg_bs = segs(d.bs_level_pass, kGroupNarrowSolve);
+  img.put(&DevPlan::gseg, gseg);
 }
 
 void upload_plan(DevicePlan& dp, const YBus* y, const JacobianPattern& jp, const BusSpec* spec, const Plan& plan,
@@ -269,9 +293,12 @@ gs_nr_options normalise(const gs_nr_options* o) {
   r.schedule = 0;
   r.flags = 0;
   if (o) r = *o;
-  if (r.flags & ~(GS_FLAG_X_GLOBAL | GS_FLAG_Q_LIMITS | GS_FLAG_SINGLE_WAVE | GS_FLAG_DC_RETRY))
+  if (r.flags & ~(GS_FLAG_X_GLOBAL | GS_FLAG_Q_LIMITS | GS_FLAG_SINGLE_WAVE | GS_FLAG_DC_RETRY | GS_GROUP_WAVES(15)))
     fail(GS_INVALID_ARGUMENT, "unknown flags");
-  if (r.schedule < GS_SCHEDULE_LEVELS) fail(GS_INVALID_ARGUMENT, "schedule must be >= 0 or GS_SCHEDULE_LEVELS");
+  if (GS_GROUP_WAVES_OF(r.flags) > kGroupWavesMax)
+    fail(GS_INVALID_ARGUMENT, "a work-group per instance holds at most " + std::to_string(kGroupWavesMax) + " waves");
+  if (r.schedule < GS_SCHEDULE_GROUP)
+    fail(GS_INVALID_ARGUMENT, "schedule must be >= 0, GS_SCHEDULE_LEVELS or GS_SCHEDULE_GROUP");
   if (!(r.tolerance > 0)) fail(GS_INVALID_ARGUMENT, "tolerance must be positive");
   if (r.max_iterations < 0) fail(GS_INVALID_ARGUMENT, "max_iterations must be >= 0");
   if (r.refine_steps < 0 || r.refine_steps > 4) fail(GS_INVALID_ARGUMENT, "refine_steps must lie in 0..4");
@@ -386,6 +413,16 @@ BatchRun plan_run(const DevPlan& h, const gs_nr_options& o, bool force_fp64) {
   size_t x = t * (size_t)h.m;
   // (the level-parallel schedule shares the solve vector between work-groups: it lives in global memory)
   const bool levels = !force_fp64 && o.schedule == GS_SCHEDULE_LEVELS;
+  if (!force_fp64 && o.schedule == GS_SCHEDULE_GROUP) {
+    // a workspace per wave, as many waves as fit (at least one), then the solve vector when it fits beside them
+    int w = GS_GROUP_WAVES_OF(o.flags) ? GS_GROUP_WAVES_OF(o.flags) : kGroupWavesDefault;
+    while (w > 1 && w * ws > kLdsLimit) w--;
+    if (ws > kLdsLimit) fail(GS_INVALID_ARGUMENT, "the refactor workspace exceeds the LDS a work-group may take");
+    r.group_waves = w;
+    r.x_lds = !(o.flags & GS_FLAG_X_GLOBAL) && w * ws + x <= kLdsLimit;
+    r.lds_bytes = w * ws + (r.x_lds ? x : 0);
+    return r;
+  }
   r.x_lds = !levels && !(o.flags & GS_FLAG_X_GLOBAL) && ws + x <= kLdsLimit;
   r.lds_bytes = ws + (r.x_lds ? x : 0);
   // Two wavefronts per work-group sharing the solve vector, where that keeps more instances resident per compute
@@ -574,6 +611,12 @@ PreparedRun prepare_instances(gs_ctx* ctx, DevicePlan& dp, DevMem& inst_buf, Dev
 }
 
 void launch_prepared(const DevPlan& h, const PreparedRun& pr, int schedule, hipStream_t s) {
+  if (pr.run.group_waves > 0) {
+    check(launch_group(pr.run.fp32, pr.run.x_lds, pr.run.group_waves, pr.desc, h, pr.count, pr.prm, pr.run.lds_bytes,
+                       s),
+          "nr kernel (work-group per instance)");
+    return;
+  }
   if (schedule < 0) fail(GS_INVALID_ARGUMENT, "the level-parallel schedule runs one call at a time (gs_nr_solve_batch)");
   const int slice = schedule > 0 ? schedule : pr.count;
   for (int first = 0; first < pr.count; first += slice)
@@ -2118,7 +2161,7 @@ GS_API gs_status gs_network_launch_shape(const gs_net* net, const gs_nr_options*
     const BatchRun r = plan_run(net->dev.h, o, false);
     if (lu_fp32) *lu_fp32 = r.fp32;
     if (x_in_lds) *x_in_lds = r.x_lds;
-    if (waves_per_group) *waves_per_group = r.pair ? 2 : 1;
+    if (waves_per_group) *waves_per_group = r.group_waves > 0 ? r.group_waves : r.pair ? 2 : 1;
     if (lds_bytes) *lds_bytes = (int64_t)r.lds_bytes;
     return GS_OK;
   });
@@ -2273,6 +2316,12 @@ GS_API gs_status gs_nr_autotune(gs_net* net, int32_t count, const gs_instance* s
           }
           cands.push_back({c.lu, c.refine, sched, flags});
         }
+    // asked for: a work-group of 2, 4 or 8 waves per instance
+    if (base.schedule == GS_SCHEDULE_GROUP || base.schedule == GS_SCHEDULE_LEVELS)
+      for (int32_t w : {2, 4, 8}) {
+        cands.push_back({GS_LU_FP64, 0, GS_SCHEDULE_GROUP, GS_GROUP_WAVES(w)});
+        cands.push_back({GS_LU_FP32, 1, GS_SCHEDULE_GROUP, GS_GROUP_WAVES(w)});
+      }
     if (base.schedule == GS_SCHEDULE_LEVELS) {   // asked for: the level-parallel schedule too
       cands.push_back({GS_LU_FP64, 0, GS_SCHEDULE_LEVELS, 0});
       cands.push_back({GS_LU_FP32, 1, GS_SCHEDULE_LEVELS, 0});

@@ -26,6 +26,8 @@ hipError_t launch_iterative_step(const InstDesc* descs, const DevPlan& plan, con
                                  int* iw, hipStream_t stream);
 hipError_t launch_screen(const InstDesc* descs, const DevPlan& plan, int count, gs_screen_summary* out,
                          double* loading, double* flows, hipStream_t stream);
+hipError_t launch_group(int lu_fp32, int x_in_lds, int waves, const InstDesc* descs, const DevPlan& plan, int count,
+                        const NrParams& prm, size_t lds_bytes, hipStream_t stream);
 hipError_t launch_wide_start(const InstDesc* descs, const DevPlan& plan, int count, WideCtl* ctl, hipStream_t stream);
 hipError_t launch_wide_step(int lu_fp32, const InstDesc* descs, const DevPlan& plan, int count, const NrParams& prm,
                             WideCtl* ctl, const std::vector<int>& level_chunks, const std::vector<int>& fs_level_pass,
@@ -38,6 +40,8 @@ hipError_t launch_fp64_probe(double* out, hipStream_t stream);
 constexpr size_t kLdsLimit = 64 * 1024;  // dynamic LDS a work-group may take for the solve workspace
 constexpr size_t kLdsPerCu = 160 * 1024;  // LDS of one compute unit (gfx950)
 constexpr size_t kWavesPerCu = 8;         // the batched kernel's register use allows two waves per SIMD
+constexpr int kGroupWavesMax = 8;         // waves of a work-group-per-instance launch (two per SIMD)
+constexpr int kGroupWavesDefault = 8;
 
 struct HipError {
   gs_status code;
@@ -253,6 +257,7 @@ struct Layout {
 struct BatchRun {
   bool fp32 = true, x_lds = true;
   bool pair = false;   // two instances per work-group sharing the LDS solve vector
+  int group_waves = 0; // > 0: a work-group of that many waves per instance (GS_SCHEDULE_GROUP)
   size_t lds_bytes = 0;
 };
 

@@ -46,6 +46,11 @@ struct DevPlan {
   // Screening: per branch its end buses (from = -1: out of service), pi-model stamp (Yff, Yft, Ytf, Ytt as re, im)
   // and rate A; per bus its own voltage limits (NaN = the operational band).
   int32_t nb, pad_nb;
+  // The group schedule's segments (x, y, z) = (first, end, split) of the refactor's chunks, then the forward and back
+  // solves' passes: a level of two or more chunks (passes) split between the waves (z = 1), or a run of one-chunk levels
+  // walked by one wave (z = 0).
+  int32_t n_gseg_rs, n_gseg_fs, n_gseg_bs, pad_gseg;
+  const int4* gseg;
   const int32_t* j_diag;              // per Jacobian row: CSR position of its diagonal (the iterative fallback)
   const int2* br_ends;
   const double* br_stamp;
@@ -85,6 +90,9 @@ __host__ __device__ inline int dval_slots(const DevPlan& p) { return imax3(p.n_b
 // Level-parallel schedule: a level of at most this many chunks (solve passes) is cheaper walked in sequence by one
 // work-group, together with its narrow neighbours, than given a kernel of its own (measured on the 70,000-bus grid).
 constexpr int kNarrow = 2, kNarrowSolve = 4;
+// The work-group-per-instance schedule splits levels of more chunks (passes) than these between its waves; the
+// narrower ones are walked by one wave without barriers.
+constexpr int kGroupNarrow = 1, kGroupNarrowSolve = 1;
 
 // Where a level-parallel step may fork: the refactor's levels from trunk_level on (the final run of narrow levels)
 // go to the side stream while the forward solve's levels before fs_level (which read no factor of those columns)

@@ -553,12 +553,13 @@ __device__ __forceinline__ void lu_solve(const DevPlan& P, GLOBAL const T* fval,
 }
 
 // r = f - J dx (FP64, CSR rows; each row's products summed in column order).
+// Rows t, t + nt, ... (a wave: its lane and 64; a work-group: its thread and size).
 __device__ __forceinline__ void residual(const DevPlan& P, GLOBAL const double* jv, GLOBAL const double* f,
-                                         GLOBAL const double* dx, GLOBAL double* r, int lane) {
+                                         GLOBAL const double* dx, GLOBAL double* r, int t, int nt = kWave) {
   GLOBAL const int* j_rp = glob(P.j_rp);
   GLOBAL const int* j_ci = glob(P.j_ci);
   const int m = P.m;
-  int i = lane;
+  int i = t;
   int nk0 = 0, nk1 = 0;
   double nf = 0;
   auto load_row = [&](int ii) {
@@ -566,10 +567,10 @@ __device__ __forceinline__ void residual(const DevPlan& P, GLOBAL const double* 
     nk0 = j_rp[c]; nk1 = j_rp[c + 1]; nf = f[c];
   };
   load_row(i);
-  for (; i < m; i += kWave) {
+  for (; i < m; i += nt) {
     const int k0 = nk0, k1 = nk1;
     const double fi = nf;
-    load_row(i + kWave);
+    load_row(i + nt);
     double s = 0.0;
     int k = k0;
     for (; k + 4 <= k1; k += 4) {
@@ -1032,6 +1033,257 @@ __global__ __launch_bounds__(256) void wide_update(const InstDesc* __restrict__ 
   }
 }
 
+// ------------------------------------------------------------------------------------------------ work-group per instance
+// Schedule GS_SCHEDULE_GROUP: the W wavefronts of one work-group run one instance's whole Newton loop, for networks
+// whose elimination levels are wide enough to keep several waves busy but too many to pay a kernel each. Per step:
+// the Y cells and the buses in blocks of 64 spread over the waves (a work-group barrier between the two passes, which
+// make every injection term visible), the reductions of max |f|, max |J| and the Jacobian's finiteness through LDS;
+// then the refactor and both triangular solves by the plan's segments: a level of several chunks (passes) split into W
+// contiguous ranges, one per wave, each wave on its own LDS workspace, or a run of narrow levels walked by wave 0 in
+// stream order; a barrier after each segment publishes its factors (solve values) to the next. The permutations, the
+// refinement residual, max |dx| and the update spread over all threads. Every value is the batched kernel's: the same
+// device functions over the same streams, each reduction over the same set of values.
+constexpr int kGroupMaxWaves = 8;
+
+// The group's reductions: each wave's (wave-uniform) values into one of two LDS slot sets, a barrier, every thread
+// combines the W entries. Consecutive reductions alternate the sets, so a set is only rewritten after every wave has
+// passed the barrier that follows its last read.
+struct GroupReduce {
+  double a[2][kGroupMaxWaves], b[2][kGroupMaxWaves];
+  int c[2][kGroupMaxWaves];
+  int fail[3];   // the refactor segments' failure keys (three in rotation: see group_refactor)
+};
+
+__device__ __forceinline__ void group_reduce(GroupReduce& g, int& set, int wave, int lane, int W, double* a, double* b,
+                                             int* c, bool a_is_min) {
+  if (lane == 0) { g.a[set][wave] = *a; g.b[set][wave] = *b; g.c[set][wave] = *c; }
+  __syncthreads();
+  double ra = g.a[set][0], rb = g.b[set][0];
+  int rc = g.c[set][0];
+  for (int w = 1; w < W; w++) {
+    const double va = g.a[set][w];
+    ra = a_is_min ? (va < ra ? va : ra) : nan_max(ra, va);
+    rb = nan_max(rb, g.b[set][w]);
+    rc |= g.c[set][w];
+  }
+  *a = ra; *b = rb; *c = rc;
+  set ^= 1;
+}
+
+// The first and end chunk (pass) of this wave's share of segment sg.
+__device__ __forceinline__ void group_share(v4i sg, int wave, int W, int* first, int* end) {
+  if (sg.z) {
+    const int c = sg.y - sg.x;
+    *first = sg.x + (c * wave) / W;
+    *end = sg.x + (c * (wave + 1)) / W;
+  } else {
+    *first = sg.x;
+    *end = wave == 0 ? sg.y : sg.x;
+  }
+}
+
+// Refactor by segments. Returns the failing pivot step, or -1. A failure ends the refactor after its segment; the
+// failure reported is the first in stream order, as one wave walking the whole stream finds it (the waves' shares are
+// in stream order, so the lowest failing wave's first). The failure keys rotate over three slots: slot g % 3 is
+// written during segment g and read after its barrier, and thread 0 resets slot (g + 1) % 3 during segment g, after
+// every wave has read it (segment g - 2) and before any wave writes it (segment g + 1).
+template <typename T>
+__device__ __forceinline__ int group_refactor(const DevPlan& P, const Inst<T>& I, LDSP T* ws, GroupReduce& g,
+                                              double growth_tol, double abs_floor, double* min_ratio, int tid, int lane,
+                                              int wave, int W) {
+  GLOBAL const v4i* seg = globv<v4i>(P.gseg);
+  GLOBAL const int* chunk_row = glob(P.rs_chunk_row);
+  constexpr int kNone = 0x7FFFFFFF;
+  if (tid == 0) { g.fail[0] = kNone; g.fail[1] = kNone; }
+  __syncthreads();
+  int failed = -1;
+  for (int k = 0; k < P.n_gseg_rs; k++) {
+    if (tid == 0) g.fail[(k + 1) % 3] = kNone;
+    int c0, c1;
+    group_share(seg[k], wave, W, &c0, &c1);
+    if (c1 > c0) {
+      const int f = refactor<T, false, true>(P, I.vals, I.fval, I.bval, I.dval, ws, growth_tol, abs_floor, nullptr,
+                                             lane, min_ratio, chunk_row[c0], chunk_row[c1]);
+      if (f >= 0 && lane == 0) atomicMin(&g.fail[k % 3], (wave << 24) | f);
+    }
+    __syncthreads();
+    const int key = g.fail[k % 3];
+    if (key != kNone) { failed = key & 0xFFFFFF; break; }
+  }
+  return failed;
+}
+
+// x = A^{-1} (permuted rhs), out = the column-permuted x: the forward then back segments.
+template <typename T, bool XLDS, typename XP>
+__device__ __forceinline__ void group_solve(const DevPlan& P, const Inst<T>& I, GLOBAL const double* rhs,
+                                            GLOBAL double* out, XP x, LDSP T* scr, int tid, int nt, int lane,
+                                            int wave, int W) {
+  GLOBAL const int* pinv = glob(P.pinv);
+  GLOBAL const int* q = glob(P.q);
+  GLOBAL const v4i* seg = globv<v4i>(P.gseg);
+  for (int r = tid; r < P.m; r += nt) x[pinv[r]] = (T)rhs[r];
+  __syncthreads();
+  for (int k = 0; k < P.n_gseg_fs; k++) {
+    int p0, p1;
+    group_share(seg[P.n_gseg_rs + k], wave, W, &p0, &p1);
+    if (p1 > p0)
+      solve_stream<T, false, XLDS>(globv<v2i>(P.fs_rec), P.fs_pass_step[p0], P.fs_pass_step[p1], I.fval, I.dval, x,
+                                   scr, lane);
+    __syncthreads();
+  }
+  for (int k = 0; k < P.n_gseg_bs; k++) {
+    int p0, p1;
+    group_share(seg[P.n_gseg_rs + P.n_gseg_fs + k], wave, W, &p0, &p1);
+    if (p1 > p0)
+      solve_stream<T, true, XLDS>(globv<v2i>(P.bs_rec), P.bs_pass_step[p0], P.bs_pass_step[p1], I.bval, I.dval, x,
+                                  scr, lane);
+    __syncthreads();
+  }
+  for (int k = tid; k < P.m; k += nt) out[q[k]] = (double)x[k];
+  __syncthreads();
+}
+
+template <typename T, bool XLDS>
+__global__ __launch_bounds__(kWave * kGroupMaxWaves) void group_kernel(const InstDesc* __restrict__ descs,
+                                                                        const DevPlan P, const NrParams prm) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ GroupReduce g;
+  const int W = (int)blockDim.x / kWave, nt = (int)blockDim.x;
+  const int tid = (int)threadIdx.x, lane = tid % kWave, wave = tid / kWave;
+  // LDS: each wave's refactor workspace (also its solve scratch), then (when it fits) the solve vector
+  LDSP T* ws = lds(reinterpret_cast<T*>(smem)) + wave * ws_slots(P);
+  LDSP T* xl = lds(reinterpret_cast<T*>(smem)) + W * ws_slots(P);
+  const Inst<T> I = load_inst<T>(descs, P, blockIdx.x);
+  InstState* st = descs[blockIdx.x].state;
+  const int n = P.n, m = P.m;
+  int set = 0;
+
+  if (prm.init || prm.load_start) {
+    for (int i = tid; i < n; i += nt) { I.vm[i] = I.vm_start[i]; I.va[i] = I.va_start[i]; }
+    __syncthreads();
+  }
+  if (!prm.init && st->done) return;
+  int iter = prm.init ? 0 : st->iter;
+  double start_mis = prm.init ? __builtin_nan("") : st->start_mis;
+  double min_ratio = prm.init ? 1.0 : st->min_ratio;
+  double last_from = prm.init ? __builtin_nan("") : st->last_from;
+  int step_failures = prm.init ? 0 : st->step_failures;
+  int converged = 0, diverged = 0, paused = 0, fail_col = -1, steps = 0;
+  double max_mis;
+  uint64_t t_mis = prm.init ? 0 : st->t_mismatch, t_fac = prm.init ? 0 : st->t_refactor;
+  uint64_t t_sol = prm.init ? 0 : st->t_solve, t_upd = prm.init ? 0 : st->t_update;
+  uint64_t clk = wall_clock64();
+  auto tick = [&](uint64_t& acc) {
+    const uint64_t now = wall_clock64();
+    acc += now - clk;
+    clk = now;
+  };
+  auto solve = [&](GLOBAL const double* rhs, GLOBAL double* out) {
+    if constexpr (XLDS)
+      group_solve<T, true>(P, I, rhs, out, xl, ws, tid, nt, opaque_lane(lane), wave, W);
+    else
+      group_solve<T, false>(P, I, rhs, out, I.xg, ws, tid, nt, opaque_lane(lane), wave, W);
+  };
+  for (;;) {
+    double jc, jb;
+    int bad;
+    const int cbad = prm.refine_steps > 0 ? mismatch_cells<T, true>(P, I, opaque_lane(lane), wave, W, &jc)
+                                          : mismatch_cells<T, false>(P, I, opaque_lane(lane), wave, W, &jc);
+    __syncthreads();
+    max_mis = wave_max(prm.refine_steps > 0
+                           ? mismatch_buses<T, true>(P, I, opaque_lane(lane), wave, W, &bad, &jb)
+                           : mismatch_buses<T, false>(P, I, opaque_lane(lane), wave, W, &bad, &jb));
+    bad = wave_any(cbad | bad);
+    double jnorm = wave_max(jc > jb ? jc : jb);
+    group_reduce(g, set, wave, lane, W, &jnorm, &max_mis, &bad, false);   // (also publishes f, p, q, J)
+    tick(t_mis);
+    if (iter == 0) start_mis = max_mis;
+    if (max_mis < prm.tolerance) { converged = 1; break; }
+    if (max_mis > nan_max(prm.diverge_limit, start_mis)) { diverged = 1; break; }
+    if (iter >= prm.max_iterations) break;
+    if (steps >= prm.steps_this_launch) { paused = 1; break; }
+    if (bad || !isfinite(max_mis)) break;  // NewtonShell.SolveStep refuses a non-finite J or f
+    // (a plan the host made at this very state is the factorisation's own: its first step is not checked)
+    const bool trust = prm.trust_first && steps == 0;
+    double ratio = min_ratio;
+    fail_col = group_refactor<T>(P, I, ws, g, trust ? 0.0 : prm.growth_tol,
+                                 trust ? 0.0 : pivot_floor_rel<T>() * jnorm, &ratio, tid, opaque_lane(lane), wave, W);
+    {
+      double unused = 0.0;
+      int none = 0;
+      group_reduce(g, set, wave, lane, W, &ratio, &unused, &none, true);
+    }
+    min_ratio = ratio;
+    tick(t_fac);
+    if (fail_col >= 0) { paused = 2; break; }
+    solve(I.f, I.dx);
+    for (int it = 0; it < prm.refine_steps; it++) {
+      residual(P, I.jv, I.f, I.dx, I.r, tid, nt);
+      __syncthreads();
+      solve(I.r, I.r);
+      for (int k = tid; k < m; k += nt) I.dx[k] = I.dx[k] + I.r[k];
+      __syncthreads();
+    }
+    tick(t_sol);
+    double lmax = 0.0, unused = 0.0;
+    int lbad = 0;
+    for (int k = tid; k < m; k += nt) {
+      const double v = I.dx[k];
+      lbad |= !isfinite(v);
+      lmax = fabs(v) > lmax ? fabs(v) : lmax;
+    }
+    lbad = wave_any(lbad);
+    lmax = wave_max(lmax);
+    group_reduce(g, set, wave, lane, W, &unused, &lmax, &lbad, false);
+    if (lbad) { step_failures++; break; }
+    const double damp = lmax > 0.4 ? 0.4 / lmax : 1.0;
+    GLOBAL const int* ppos = glob(P.ppos);
+    GLOBAL const int* qpos = glob(P.qpos);
+    for (int i = tid; i < n; i += nt) {
+      const int pp = ppos[i], qq = qpos[i];
+      if (pp >= 0) I.va[i] = I.va[i] + damp * I.dx[pp];
+      if (qq >= 0) {
+        double v = I.vm[i] + damp * I.dx[qq];
+        I.vm[i] = v < 0.01 ? 0.01 : v;
+      }
+    }
+    __syncthreads();
+    tick(t_upd);
+    last_from = max_mis;
+    iter++;
+    steps++;
+  }
+  // final verdict (NewtonShell.AssembleResult precedence: NonFinite > Converged > Diverged > MaxIterations)
+  int lnf = 0;
+  for (int i = tid; i < n; i += nt)
+    lnf |= !(isfinite(I.vm[i]) && isfinite(I.va[i]) && isfinite(I.p[i] * P.base_mva) &&
+             isfinite(I.q[i] * P.base_mva));
+  lnf = wave_any(lnf);
+  {
+    double u0 = 0.0, u1 = 0.0;
+    group_reduce(g, set, wave, lane, W, &u0, &u1, &lnf, false);
+  }
+  const int nonfinite = lnf || !isfinite(max_mis);
+  if (tid == 0) {
+    st->iter = iter;
+    st->start_mis = start_mis;
+    st->max_mis = max_mis;
+    st->min_ratio = min_ratio;
+    st->step_failures = step_failures;
+    st->fail_column = fail_col;
+    st->last_from = last_from;
+    st->engine_stop = 0;
+    st->converged = converged;
+    st->diverged = diverged;
+    st->nonfinite = nonfinite;
+    st->t_mismatch = t_mis;
+    st->t_refactor = t_fac;
+    st->t_solve = t_sol;
+    st->t_update = t_upd;
+    st->done = paused == 1 ? 0 : paused == 2 ? 2 : 1;
+  }
+}
+
 // ------------------------------------------------------------------------------------------------ iterative step
 // NewtonShell.SolveStep's last resort (NewtonShell.cs:97-105): when the sparse LU refuses the Jacobian (the host's
 // fresh symbolic pass found it singular), the step is solved by BiCGSTAB with an ILU(0) preconditioner on the FP64
@@ -1377,6 +1629,26 @@ hipError_t launch_nr_batch(int lu_fp32, int x_in_lds, int pair, const InstDesc* 
       hipLaunchKernelGGL((nr_batch_kernel<double, true, false>), grid, block, lds_bytes, stream, descs, plan, prm);
     else
       hipLaunchKernelGGL((nr_batch_kernel<double, false, false>), grid, block, lds_bytes, stream, descs, plan, prm);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_group(int lu_fp32, int x_in_lds, int waves, const InstDesc* descs, const DevPlan& plan, int count,
+                        const NrParams& prm0, size_t lds_bytes, hipStream_t stream) {
+  NrParams prm = prm0;
+  prm.n_inst = count;
+  if (waves < 1 || waves > kGroupMaxWaves) return hipErrorInvalidValue;
+  const dim3 grid(count), block(kWave * waves);
+  if (lu_fp32) {
+    if (x_in_lds)
+      hipLaunchKernelGGL((group_kernel<float, true>), grid, block, lds_bytes, stream, descs, plan, prm);
+    else
+      hipLaunchKernelGGL((group_kernel<float, false>), grid, block, lds_bytes, stream, descs, plan, prm);
+  } else {
+    if (x_in_lds)
+      hipLaunchKernelGGL((group_kernel<double, true>), grid, block, lds_bytes, stream, descs, plan, prm);
+    else
+      hipLaunchKernelGGL((group_kernel<double, false>), grid, block, lds_bytes, stream, descs, plan, prm);
   }
   return hipGetLastError();
 }

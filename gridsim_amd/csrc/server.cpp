@@ -425,8 +425,9 @@ GS_API gs_status gs_server_add_network(gs_server* srv, gs_net* net, const gs_nr_
     auto nt = std::make_unique<gs_server::Net>();
     nt->net = net;
     nt->opts = normalise(opts);
-    if (nt->opts.schedule < 0)
-      fail(GS_INVALID_ARGUMENT, "the serving loop launches a wavefront per instance (schedule >= 0)");
+    if (nt->opts.schedule == GS_SCHEDULE_LEVELS)
+      fail(GS_INVALID_ARGUMENT, "the serving loop launches a wavefront or a work-group per instance, not the "
+                                "level-parallel schedule");
     if (nt->opts.flags & (GS_FLAG_Q_LIMITS | GS_FLAG_DC_RETRY))
       fail(GS_INVALID_ARGUMENT, "the serving loop runs neither the reactive-limit passes nor the DC-start retry");
     nt->run = plan_run(net->dev.h, nt->opts, false);

@@ -35,6 +35,12 @@ FLAG_SINGLE_WAVE = 4
 FLAG_Q_LIMITS = 2      # enforce generator reactive limits (the outer loop of PV -> PQ switching)
 FLAG_DC_RETRY = 8      # retry a stalled or diverged solve from the DC power-flow angle start (RetryFromDcStart)
 SCHEDULE_LEVELS = -1   # every instance spread over the whole GPU, level by level (GS_SCHEDULE_LEVELS)
+SCHEDULE_GROUP = -2    # a work-group of several wavefronts per instance (GS_SCHEDULE_GROUP)
+
+
+def group_waves(w: int) -> int:
+    """The flags bits that set the waves of a work-group per instance (GS_GROUP_WAVES)."""
+    return int(w) << 8
 STATUS_NAMES = {0: "Converged", 1: "MaxIterations", 2: "Diverged", 3: "NonFinite"}
 
 

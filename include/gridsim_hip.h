@@ -84,8 +84,10 @@ typedef struct gs_nr_options {
   double growth_tol;        /* a static pivot is kept while |pivot| >= growth_tol * max |candidate| */
   int32_t schedule;         /* 0 = a wavefront per instance, all instances of a call in one launch; k > 0 = the
                                same in launches of at most k instances one after another; GS_SCHEDULE_LEVELS = every
-                               instance spread over the whole GPU, level by level (gs_nr_autotune chooses) */
-  int32_t flags;            /* GS_FLAG_*: placement choices (gs_nr_autotune chooses) and reactive limits */
+                               instance spread over the whole GPU, level by level; GS_SCHEDULE_GROUP = a work-group of
+                               several wavefronts per instance (gs_nr_autotune chooses) */
+  int32_t flags;            /* GS_FLAG_*: placement choices and GS_GROUP_WAVES (gs_nr_autotune chooses), reactive
+                               limits, the DC-start retry */
 } gs_nr_options;
 
 /* gs_nr_options.flags: keep the triangular-solve vector in global memory even where it fits in LDS (smaller LDS per
@@ -119,6 +121,20 @@ enum { GS_FLAG_DC_RETRY = 8 };
  * of rows) of a level and per instance, replayed until every instance has finished. For large networks (tens of
  * thousands of buses), where one wavefront per instance leaves the GPU idle. gs_nr_solve_batch only. */
 enum { GS_SCHEDULE_LEVELS = -1 };
+
+/* gs_nr_options.schedule: a work-group per instance. The W wavefronts of one work-group (W = GS_GROUP_WAVES_OF(flags),
+ * set with GS_GROUP_WAVES(W), 2..16, default 8) share the instance's whole Newton loop in one launch: the mismatch and
+ * Jacobian over their slices of the cells and buses, then the refactor and both triangular solves level by level, the
+ * chunks (passes) of a wide level split between the waves and a work-group barrier between levels, runs of narrow
+ * levels walked by one wave. The same device functions over the same streams as the other schedules, so the same
+ * bits. Each wave takes its own refactor workspace in LDS (W is lowered until the W workspaces fit); the solve vector
+ * shares LDS with them when it fits too (GS_FLAG_X_GLOBAL: never). For networks of thousands to tens of thousands of
+ * buses, where one wavefront per instance leaves the levels' parallelism unused and a kernel per level costs a
+ * dispatch per level. Everywhere a wavefront schedule runs: gs_nr_solve_batch, gs_nr_screen_batch, gs_batch_* and the
+ * serving loop (instances resumed after a fresh host plan run one wavefront each). */
+enum { GS_SCHEDULE_GROUP = -2 };
+#define GS_GROUP_WAVES(w) ((int32_t)(w) << 8)
+#define GS_GROUP_WAVES_OF(flags) (((flags) >> 8) & 15)
 
 /* One solve of a batch: the uploaded network with some branches out of service and every load and generator P scaled
  * by load_scale, as ContingencyAnalysis.ScreenOutageCore solves WithoutMany(model, indices)
@@ -325,8 +341,10 @@ GS_API void gs_batch_free(gs_batch* batch);
  * launch or slices of 512 or 256; the solve vector in LDS or in global memory - and writes into *opts the fastest
  * whose verdicts equal the FP64 LU's, whose
  * converged instances meet the tolerance and whose converged voltages lie within 1e-9 (pu, rad) of the FP64 LU's.
- * With opts->schedule = GS_SCHEDULE_LEVELS on entry the level-parallel schedule (FP64 LU, or FP32 LU with one
- * refinement step) is a candidate too; otherwise it is not offered, as the serving loop does not run it.
+ * With opts->schedule = GS_SCHEDULE_GROUP on entry a work-group of 2, 4 or 8 waves per instance (FP64 LU, or FP32
+ * LU with one refinement step) is a candidate too, and with GS_SCHEDULE_LEVELS that and the level-parallel schedule
+ * (the same arithmetic; the serving loop does not run it). The candidates are timed alone:
+ * a caller that runs several groups at once (gs_batch) offers the wider schedules where they do not compete.
  * *best_ms: that candidate's kernel time on the sample. */
 GS_API gs_status gs_nr_autotune(gs_net* net, int32_t count, const gs_instance* sample, gs_nr_options* opts,
                                 double* best_ms);

@@ -2,8 +2,8 @@
 
 * Uneven batch: 300-, 2,000- and 10,000-bus grids (branches 1.35 n, 15 % PV), each warm-started from its base case,
   instances with a load multiplier U[0.8, 1.2] and one random non-bridge outage, all solved by one captured hipGraph
-  replay (the 10,000-bus group on the level-parallel schedule with its Newton steps unrolled), FP64 LU and FP32 LU with
-  one FP64 refinement step.
+  replay (the 10,000-bus group on the level-parallel schedule with its Newton steps unrolled, or on a work-group of
+  wavefronts per instance), FP64 LU and FP32 LU with one FP64 refinement step.
 * Large instance: the 70,000-bus grid from flat start (max 100 iterations), level-parallel, FP64 and FP32 + 1.
 
 Bar: the oracle's verdict and iteration count per instance, final mismatch < 1e-8 pu, |dV| <= 1e-9 (pu, rad).
@@ -43,13 +43,14 @@ def uneven(oracle):
     return groups
 
 
+@pytest.mark.parametrize("large_schedule", ["levels", "group"])
 @pytest.mark.parametrize("lu", ["fp64", "fp32+1"])
-def test_uneven_batch_in_one_graph_matches_the_oracle(ctx, uneven, lu):
+def test_uneven_batch_in_one_graph_matches_the_oracle(ctx, uneven, lu, large_schedule):
     from gridsim_amd import engine as eng
     prec, refine = (eng.LU_FP64, 0) if lu == "fp64" else (eng.LU_FP32, 1)
     dns, groups = [], []
     for k, (warm, outs, scales, _) in enumerate(uneven):
-        sched = eng.SCHEDULE_LEVELS if k == 2 else 0
+        sched = 0 if k < 2 else eng.SCHEDULE_LEVELS if large_schedule == "levels" else eng.SCHEDULE_GROUP
         o = eng.NrOptions(max_iterations=10, lu_precision=prec, refine_steps=refine, schedule=sched)
         dn = eng.DeviceNetwork(ctx, warm, eng.NrOptions(max_iterations=10))
         dns.append(dn)
@@ -57,7 +58,7 @@ def test_uneven_batch_in_one_graph_matches_the_oracle(ctx, uneven, lu):
     b = eng.Batch(ctx, groups, use_graph=True)
     for rep in range(2):
         ms = b.run()
-        print(f"uneven batch ({lu}) replay {rep}: {ms:.2f} ms", flush=True)
+        print(f"uneven batch ({lu}, 10,000-bus group {large_schedule}) replay {rep}: {ms:.2f} ms", flush=True)
         for g, (warm, outs, scales, refs) in enumerate(uneven):
             r = b.results(g, keep_state=True)
             for i, (rec, vm, va) in enumerate(refs):

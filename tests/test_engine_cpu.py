@@ -232,3 +232,15 @@ def test_fresh_symbolic_passes_are_thread_independent_and_share_orders(eng, orac
         oracle.lib().orc_lu_get(C.c_void_p(h), *(oracle._p(a) for a in arrs))
         oracle.lib().orc_lu_free(C.c_void_p(h))
         assert np.array_equal(one[j].q, arrs[7]) and np.array_equal(one[j].pivot_row, arrs[8]), j
+
+
+def test_schedule_and_group_wave_options_are_checked(eng):
+    """gs_nr_options.schedule and the work-group size in flags are validated where the options enter the library (the
+    host-only plan builder runs the same check as every device call)."""
+    net = matpower_case("case9")
+    eng.HostPlan(net, eng.NrOptions(schedule=eng.SCHEDULE_GROUP, flags=eng.group_waves(8)))
+    eng.HostPlan(net, eng.NrOptions(schedule=eng.SCHEDULE_GROUP))
+    for bad in (eng.NrOptions(schedule=eng.SCHEDULE_GROUP, flags=eng.group_waves(9)),
+                eng.NrOptions(schedule=-3), eng.NrOptions(flags=1 << 12 << 4)):
+        with pytest.raises(eng.GsError):
+            eng.HostPlan(net, bad)
