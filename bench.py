@@ -70,6 +70,113 @@ def _schedule_name(opts):
     return "one wavefront per instance" + (f", launches of {opts.schedule}" if opts.schedule > 0 else "")
 
 
+def run_seam(args):
+    """Per-call latency of the sparse-solve seam that replaces the reference's GPU linear solver
+    (ISparseLinearSolver.Solve, src/GridSim.Core/Numerics/ISparseLinearSolver.cs:12-16; the backend it replaces
+    allocates and frees its device buffers on every call, RocmSparseLinearSolver.cs:120-140): gs_sparse_solve (the
+    plan cached per pattern, the buffers resident), gs_sparse_solve_noalloc (the same in a static device array) and
+    gs_sparse_solve_batch (64 systems of one pattern per call), on the Jacobian patterns of MATPOWER case300 and the
+    2,000-bus grid at their flat starts. The values are synthetic on those patterns: off-diagonal entries N(0, 1),
+    each diagonal entry its row's or its column's absolute sum, the larger, plus one, so the reference's pivot search
+    keeps the diagonal and a cached plan replays for new values (the first call's time is the fresh factorisation). One
+    call = one host-to-device upload of the values and the right-hand side, the factorisation and both solves on the
+    device, and the solution back. CPU baseline: the oracle's sparse LU (ordering, factorisation and solves, as the
+    reference's CpuSparseLuSolver does per call) on the same systems, one thread."""
+    import ctypes as C
+    from gridsim_amd import engine, gridgen
+    from gridsim_amd.matpower import read_case
+    ctx = engine.Context(0)
+    info = ctx.device_info()
+    rng = np.random.default_rng(gridgen.BASE_SEED + 17)
+    golden = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "matpower", "case300.m")
+    out_rows = []
+    lib = engine.lib()
+    calls = max(50, 20 * args.steps)
+    for name, net in (("case300", read_case(golden)), ("2,000-bus grid", gridgen.config2_grid())):
+        plan = engine.HostPlan(net, engine.NrOptions(flat_start=True))
+        rp, ci = plan.j_rowptr.astype(np.int32), plan.j_colidx.astype(np.int32)
+        n = len(rp) - 1
+
+        rows = np.repeat(np.arange(n), np.diff(rp))
+        diag = np.nonzero(rows == ci)[0]
+
+        def values():
+            v = rng.standard_normal(len(ci))
+            v[diag] = 0.0
+            rsum, csum = np.zeros(n), np.zeros(n)
+            np.add.at(rsum, rows, np.abs(v))
+            np.add.at(csum, ci, np.abs(v))
+            v[diag] = np.maximum(rsum, csum) + 1.0   # dominant by rows and by columns: no pivot moves
+            return v
+
+        v = values()
+        b = rng.standard_normal(n)
+        x = np.empty(n)
+        solver = engine.HipSparseLinearSolver(ctx)
+        t0 = time.perf_counter()
+        x0 = solver.solve(rp, ci, v, b)
+        first_ms = 1000 * (time.perf_counter() - t0)
+        assert x0 is not None
+        resid = float(np.abs(_csr_mul(rp, ci, v, x0) - b).max())
+
+        def timed(fn):
+            out = []
+            for _ in range(calls):
+                ts = time.perf_counter()
+                fn()
+                out.append(1e6 * (time.perf_counter() - ts))
+            return out
+
+        args_c = (C.c_int32(n), engine._p(rp), engine._p(ci), engine._p(v), engine._p(b), engine._p(x))
+        lat = timed(lambda: lib.gs_sparse_solve(ctx.h, *args_c))
+        if lib.gs_sparse_solve_noalloc(ctx.h, *args_c) != engine.GS_OK:
+            raise RuntimeError("gs_sparse_solve_noalloc: " + ctx.error())
+        lat_na = timed(lambda: lib.gs_sparse_solve_noalloc(ctx.h, *args_c))
+        batch = engine.HipBatchedSparseLinearSolver(ctx)
+        mats = [(rp, ci, values()) for _ in range(64)]
+        rhs = [rng.standard_normal(n) for _ in range(64)]
+        errs = [None] * 64
+        batch.solve_batch(mats, rhs, errs)
+        tb = time.perf_counter()
+        reps = max(3, args.steps)
+        for _ in range(reps):
+            batch.solve_batch(mats, rhs, errs)
+        batch_us = 1e6 * (time.perf_counter() - tb) / (reps * 64)
+        row = {"system": f"{name} Jacobian pattern", "n": n, "nnz": int(len(ci)), "first_call_ms": round(first_ms, 3),
+               "p50_us": round(float(np.percentile(lat, 50)), 1), "p99_us": round(float(np.percentile(lat, 99)), 1),
+               "noalloc_p50_us": round(float(np.percentile(lat_na, 50)), 1),
+               "noalloc_p99_us": round(float(np.percentile(lat_na, 99)), 1),
+               "batch64_us_per_system": round(batch_us, 1), "max_abs_residual": resid, "calls": calls}
+        if not args.no_cpu:
+            from oracle import oracle as O
+            cpu = []
+            for _ in range(max(5, calls // 10)):
+                ts = time.perf_counter()
+                O.sparse_solve(rp, ci, v, b)
+                cpu.append(1e6 * (time.perf_counter() - ts))
+            row["cpu_oracle_p50_us"] = round(float(np.percentile(cpu, 50)), 1)
+        print(f"# seam {row}", file=sys.stderr, flush=True)
+        out_rows.append(row)
+    head = out_rows[-1]
+    line = {"metric": "sparse-solve seam: per-call latency of gs_sparse_solve (ISparseLinearSolver.Solve), 1 MI355X",
+            "value": head["p50_us"], "unit": "us per call (p50)", "n_gpus": 1, "steps": calls, "warmup": 1,
+            "ms_per_step": round(head["p50_us"] / 1000, 4), "higher_is_better": False, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic values on real Jacobian patterns",
+            "config": {"workload": "one linear system per call on the Jacobian pattern of the 2,000-bus grid "
+                                   "(case300 beside it)", "systems": out_rows, "device": info["name"] or info["arch"]},
+            "cpu_baseline": ({"value": head.get("cpu_oracle_p50_us"), "unit": "us per call (p50)", "cores": 1,
+                              "kind": "port", "sample": "the same system, ordering + factorisation + solves per call "
+                              "(C++ restatement of the reference's CpuSparseLuSolver)"} if not args.no_cpu else None)}
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+def _csr_mul(rp, ci, v, x):
+    y = np.zeros(len(rp) - 1)
+    np.add.at(y, np.repeat(np.arange(len(rp) - 1), np.diff(rp)), v * x[ci])
+    return y
+
+
 def run_config3(args):
     """512 instances, a third each on 300-, 2,000- and 10,000-bus grids (branches 1.35 n, 15 % PV), each with a load
     multiplier U[0.8, 1.2] and one random non-bridge branch outage, warm-started from its grid's base case; options
@@ -457,6 +564,9 @@ def main():
                     help="2: N-1 sweep of the 2,000-bus grid (the headline); 3: 512 uneven instances, hipGraph; "
                          "4: one 70,000-bus solve, FP64 vs FP32 + refinement; 5: serving, Poisson request stream "
                          "over 8 resident networks")
+    ap.add_argument("--seam", action="store_true",
+                    help="instead: per-call latency of the sparse-solve seam (gs_sparse_solve and its no-allocation "
+                         "and batched forms) on the case300 and 2,000-bus Jacobian patterns")
     ap.add_argument("--c4-variants", default="",
                     help="config 4: comma-separated subset of fp64-wave, fp64-levels, fp32+1-levels, fp32+2-levels")
     ap.add_argument("--serve-seconds", type=float, default=2.0,
@@ -482,6 +592,8 @@ def main():
         print(f"# copy probe: {args.copy_probe_mib} MiB read + {args.copy_probe_mib} MiB written", file=sys.stderr)
         del src, dst
 
+    if args.seam:
+        return run_seam(args)
     if args.config == 3:
         return run_config3(args)
     if args.config == 4:

@@ -33,6 +33,14 @@ static __device__ __align__(256) char scratch[kNoAllocScratchBytes];
 namespace {
 std::mutex g_scratch_mu;
 std::map<int, char*> g_scratch_addr;   // device -> device address of `scratch` there
+// The plan image each device's scratch array holds (the last one a call built there), so that the next call on the
+// same structure replays it instead of building and uploading it again. Its key is empty while none is valid.
+struct ScratchPlan {
+  std::vector<int> key;
+  gs::DevPlan h{};
+  size_t plan_bytes = 0;
+};
+std::map<int, ScratchPlan> g_scratch_plan;
 }  // namespace
 
 // The device address of `scratch` on `device`, looked up once per device with that device current.
@@ -333,19 +341,19 @@ Layout instance_layout(const DevPlan& h, bool fp32, bool x_lds) {
 
 // The outaged branches of an instance (outage_branch, then the list), checked: in range, each once, at most
 // kMaxOutages.
-std::vector<int> outage_set(const gs_instance& in, int n_branch) {
-  std::vector<int> out;
+OutageSet outage_set(const gs_instance& in, int n_branch) {
+  OutageSet out;
   if (in.n_outages < 0 || in.n_outages > kMaxOutages || (in.n_outages > 0 && !in.outages))
     fail(GS_INVALID_ARGUMENT, "n_outages must lie in 0..GS_MAX_OUTAGES, with the list given");
   if (in.outage_branch < -1 || in.outage_branch >= n_branch) fail(GS_INVALID_ARGUMENT, "outage_branch out of range");
-  if (in.outage_branch >= 0) out.push_back(in.outage_branch);
+  if (in.outage_branch >= 0) out.b[out.n++] = in.outage_branch;
   for (int k = 0; k < in.n_outages; k++) {
     const int b = in.outages[k];
     if (b < 0 || b >= n_branch) fail(GS_INVALID_ARGUMENT, "outaged branch out of range");
     if (std::find(out.begin(), out.end(), b) != out.end()) fail(GS_INVALID_ARGUMENT, "a branch is outaged twice");
-    out.push_back(b);
+    out.b[out.n++] = b;
   }
-  if ((int)out.size() > kMaxOutages) fail(GS_INVALID_ARGUMENT, "more than GS_MAX_OUTAGES outaged branches");
+  if (out.n > kMaxOutages) fail(GS_INVALID_ARGUMENT, "more than GS_MAX_OUTAGES outaged branches");
   return out;
 }
 
@@ -366,15 +374,15 @@ void check_instances(int n_branch, int count, const gs_instance* inst, const std
 
 // Every Y cell the outaged branches touch, re-summed in insertion order without them: bit-identical to the Y-bus
 // rebuilt without those branches (WithoutMany keeps model order and the builder sums from +0.0).
-std::vector<Patch> outage_patches(const YBus& y, const Network& net, const std::vector<int>& outaged) {
-  std::vector<Patch> out;
+int outage_patches(const YBus& y, const Network& net, const OutageSet& outaged, Patch* out) {
+  int n = 0;
   for (int br : outaged) {
     const int f = net.br_from[br], t = net.br_to[br];
     const int cells[4] = {y.find(f, f), y.find(t, t), y.find(f, t), y.find(t, f)};
     for (int k : cells) {
       if (k < 0) continue;
       bool dup = false;
-      for (const Patch& p : out) dup |= p.idx == k;
+      for (int q = 0; q < n; q++) dup |= out[q].idx == k;
       if (dup) continue;
       double g = 0.0, b = 0.0;
       bool any = false;
@@ -387,10 +395,10 @@ std::vector<Patch> outage_patches(const YBus& y, const Network& net, const std::
       p.idx = k;
       p.g = any ? g : 0.0;
       p.b = any ? b : 0.0;
-      out.push_back(p);
+      out[n++] = p;
     }
   }
-  return out;
+  return n;
 }
 
 // Instances of a batch whose outage touches more Y cells than the descriptor patches in place: they get Y value arrays
@@ -399,8 +407,9 @@ int own_y_count(const YBus* y, const Network* netw, int count, const gs_instance
   if (!y || !netw) return 0;
   int c = 0;
   for (int i = 0; i < count; i++) {
-    const std::vector<int> out = outage_set(inst[i], netw->nb);
-    if (out.size() > 2 && (int)outage_patches(*y, *netw, out).size() > kMaxPatch) c++;
+    const OutageSet out = outage_set(inst[i], netw->nb);
+    Patch patches[kMaxOutagePatches];
+    if (out.n > 2 && outage_patches(*y, *netw, out, patches) > kMaxPatch) c++;
   }
   return c;
 }
@@ -479,7 +488,7 @@ PreparedRun prepare_into(const DevPlan& h, const RunBuffers& rb, Staging& stg, i
   char* base = rb.inst;
   InstState* dstate = rb.state;
   std::vector<InstDesc>& descs = stg.descs;
-  descs.assign(count, InstDesc{});
+  descs.resize(count);   // (each record is cleared below)
   stg.stage.clear();
   stg.own_y.clear();
   int own_used = 0;
@@ -499,19 +508,23 @@ PreparedRun prepare_into(const DevPlan& h, const RunBuffers& rb, Staging& stg, i
     d.y_gb_own = nullptr;
     for (int t = 0; t < kMaxOutages; t++) d.out[t] = -1;
     if (y && netw) {
-      const std::vector<int> outaged = outage_set(in, netw->nb);
-      d.n_out = (int)outaged.size();
+      const OutageSet outaged = outage_set(in, netw->nb);
+      d.n_out = outaged.n;
       std::copy(outaged.begin(), outaged.end(), d.out);
-      const std::vector<Patch> patches = outage_patches(*y, *netw, outaged);
-      if ((int)patches.size() <= kMaxPatch) {
-        d.n_patch = (int)patches.size();
-        std::copy(patches.begin(), patches.end(), d.patch);
+      Patch patches[kMaxOutagePatches];
+      const int np = outage_patches(*y, *netw, outaged, patches);
+      if (np <= kMaxPatch) {
+        d.n_patch = np;
+        std::copy(patches, patches + np, d.patch);
       } else {
         // more cells than the descriptor holds: the instance's own Y values, the network's with the patches applied
         if (own_used >= rb.own_y_count) fail(GS_DEVICE_ERROR, "own Y-value arrays under-counted");
         std::vector<double> gb(2 * (size_t)y->rp[y->n]);
         for (int k = 0; k < y->rp[y->n]; k++) { gb[2 * (size_t)k] = y->g[k]; gb[2 * (size_t)k + 1] = y->b[k]; }
-        for (const Patch& p : patches) { gb[2 * (size_t)p.idx] = p.g; gb[2 * (size_t)p.idx + 1] = p.b; }
+        for (int q = 0; q < np; q++) {
+          gb[2 * (size_t)patches[q].idx] = patches[q].g;
+          gb[2 * (size_t)patches[q].idx + 1] = patches[q].b;
+        }
         double* dst = rb.own_y + (size_t)own_used * 2 * (size_t)y->rp[y->n];
         d.y_gb_own = dst;
         stg.own_y.emplace_back(dst, std::move(gb));
@@ -1510,6 +1523,47 @@ int seam_run(const DevPlan& h, char* base, const SeamLayout& L, int nnz, const d
   return status;
 }
 
+// `count` systems of one structure on plan h in one launch, a wave each, with their buffers one after another at
+// `base` (count x L.total bytes, then the descriptors and the status words): the values and right-hand sides go up,
+// the kernel replays the factorisation of each, the solutions come back into xt. status[i]: 0 solved, 1 refused.
+void seam_run_many(const DevPlan& h, char* base, const SeamLayout& L, int nnz, int count, const double* const* val,
+                   const double* const* b, std::vector<std::vector<double>>& xt, std::vector<int32_t>& status,
+                   hipStream_t s) {
+  const size_t m = (size_t)h.m;
+  char* desc_base = base + L.total * (size_t)count;
+  char* status_base = desc_base + ((sizeof(SeamDesc) * (size_t)count + 255) & ~size_t(255));
+  std::vector<SeamDesc> descs(count);
+  for (int i = 0; i < count; i++) {
+    char* p = base + L.total * (size_t)i;
+    descs[i] = SeamDesc{reinterpret_cast<double*>(p + L.a),    reinterpret_cast<double*>(p + L.b),
+                        reinterpret_cast<double*>(p + L.x),    reinterpret_cast<double*>(p + L.vals),
+                        reinterpret_cast<double*>(p + L.fval), reinterpret_cast<double*>(p + L.bval),
+                        reinterpret_cast<double*>(p + L.dval), reinterpret_cast<double*>(p + L.xs),
+                        reinterpret_cast<double*>(p + L.rf),   reinterpret_cast<int32_t*>(status_base) + i};
+    check(hipMemcpyAsync(p + L.a, val[i], 8 * (size_t)nnz, hipMemcpyHostToDevice, s), "seam a");
+    check(hipMemcpyAsync(p + L.b, b[i], 8 * m, hipMemcpyHostToDevice, s), "seam b");
+  }
+  status.assign(count, 1);
+  check(hipMemcpyAsync(status_base, status.data(), 4 * (size_t)count, hipMemcpyHostToDevice, s), "seam status");
+  check(hipMemcpyAsync(desc_base, descs.data(), sizeof(SeamDesc) * (size_t)count, hipMemcpyHostToDevice, s),
+        "seam desc");
+  check(launch_seam(reinterpret_cast<const SeamDesc*>(desc_base), h, count, 8 * (size_t)ws_slots(h), s),
+        "seam kernel");
+  check(hipMemcpyAsync(status.data(), status_base, 4 * (size_t)count, hipMemcpyDeviceToHost, s), "seam status");
+  xt.resize(count);
+  for (int i = 0; i < count; i++) {
+    xt[i].resize(m);
+    check(hipMemcpyAsync(xt[i].data(), base + L.total * (size_t)i + L.x, 8 * m, hipMemcpyDeviceToHost, s), "seam x");
+  }
+  // the host arguments and the descriptors are read by the stream: finish before they go
+  check(hipStreamSynchronize(s), "seam sync");
+}
+
+// Bytes seam_run_many needs for `count` systems of layout L.
+size_t seam_many_bytes(const SeamLayout& L, int count) {
+  return L.total * (size_t)count + ((sizeof(SeamDesc) * (size_t)count + 255) & ~size_t(255)) + 4 * (size_t)count;
+}
+
 // The seam's checks of its arguments (CpuSparseLuSolver.Solve and SparseLuFactorization.Factor): GS_OK, or the
 // status to return with ctx->err set; a malformed call throws GS_INVALID_ARGUMENT.
 gs_status seam_check(gs_ctx* ctx, int n, const int* rowptr, const int* colidx, const double* val, const double* b,
@@ -1556,14 +1610,10 @@ const std::vector<int>& seam_order(gs_ctx* ctx, int n, const int* rowptr, const 
 
 namespace {
 
-gs_status seam_solve_one(gs_ctx* ctx, int n, const int* rowptr, const int* colidx, const double* val,
-                         const double* b, double* x) {
-  if (gs_status st = seam_check(ctx, n, rowptr, colidx, val, b, x)) return st;
-  if (n == 0) return GS_OK;
+// A device plan is keyed by the whole structure it was made for: the pattern and the positions of the nonzero
+// values (the factorisation drops exact zeros, as SparseLu.ToCsc does).
+std::vector<int> seam_key(int n, const int* rowptr, const int* colidx, const double* val) {
   const int nnz = rowptr[n];
-  const std::vector<int>& order = seam_order(ctx, n, rowptr, colidx);
-  // a device plan is keyed by the whole structure it was made for: the pattern and the positions of the nonzero
-  // values (the factorisation drops exact zeros, as SparseLu.ToCsc does)
   std::vector<int> key;
   key.reserve(2 + (size_t)n + 2 * (size_t)nnz);
   key.push_back(n);
@@ -1571,6 +1621,16 @@ gs_status seam_solve_one(gs_ctx* ctx, int n, const int* rowptr, const int* colid
   key.insert(key.end(), colidx, colidx + nnz);
   key.push_back(-1);
   for (int k = 0; k < nnz; k++) if (val[k] != 0.0) key.push_back(k);
+  return key;
+}
+
+gs_status seam_solve_one(gs_ctx* ctx, int n, const int* rowptr, const int* colidx, const double* val,
+                         const double* b, double* x) {
+  if (gs_status st = seam_check(ctx, n, rowptr, colidx, val, b, x)) return st;
+  if (n == 0) return GS_OK;
+  const int nnz = rowptr[n];
+  const std::vector<int>& order = seam_order(ctx, n, rowptr, colidx);
+  const std::vector<int> key = seam_key(n, rowptr, colidx, val);
   hipStream_t s = ctx->stream;
   std::vector<double> xt;
   if (std::shared_ptr<SeamPlan>* hit = ctx->seam_plans.find(key)) {
@@ -1619,6 +1679,26 @@ gs_status seam_solve_noalloc(gs_ctx* ctx, int n, const int* rowptr, const int* c
   if (n == 0) return GS_OK;
   if (!ctx->scratch) fail(GS_DEVICE_ERROR, "the scratch array has no device address on this device");
   const int nnz = rowptr[n];
+  hipStream_t s = ctx->stream;
+  std::vector<int> key = seam_key(n, rowptr, colidx, val);
+  {
+    // the plan the array already holds, when it was made for this structure: replay it
+    ScratchPlan resident;
+    {
+      std::lock_guard<std::mutex> lock(g_scratch_mu);
+      auto it = g_scratch_plan.find(ctx->device);
+      if (it != g_scratch_plan.end() && it->second.key == key) resident = it->second;
+    }
+    if (!resident.key.empty()) {
+      const SeamLayout L = seam_layout(resident.h, nnz);
+      std::vector<double> xt;
+      if (seam_run(resident.h, ctx->scratch + resident.plan_bytes, L, nnz, val, b, xt, s) == 0) {
+        if (gs_status st = seam_rhs_check(ctx, n, b)) return st;
+        std::copy(xt.begin(), xt.end(), x);
+        return GS_OK;
+      }
+    }
+  }
   const std::vector<int>& order = seam_order(ctx, n, rowptr, colidx);
   Plan plan;
   std::string why;
@@ -1642,13 +1722,23 @@ gs_status seam_solve_noalloc(gs_ctx* ctx, int n, const int* rowptr, const int* c
     fail(GS_INVALID_ARGUMENT, "the system needs " + std::to_string(plan_bytes + L.total) +
                                   " bytes of device scratch; gs_sparse_solve_noalloc holds " +
                                   std::to_string(kNoAllocScratchBytes));
-  hipStream_t s = ctx->stream;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    g_scratch_plan[ctx->device].key.clear();   // the array's image is about to change
+  }
   check(hipMemcpyAsync(ctx->scratch, img.bytes.data(), img.bytes.size(), hipMemcpyHostToDevice, s), "scratch plan");
   img.bind(h, ctx->scratch);
   std::vector<double> xt;
   // (seam_run finishes the stream before it returns, so the image above is consumed before it goes)
   if (seam_run(h, ctx->scratch + plan_bytes, L, nnz, val, b, xt, s) != 0)
     fail(GS_DEVICE_ERROR, "the device refused the replay of its own factorisation");
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    ScratchPlan& r = g_scratch_plan[ctx->device];
+    r.key = std::move(key);
+    r.h = h;
+    r.plan_bytes = plan_bytes;
+  }
   std::copy(xt.begin(), xt.end(), x);
   return GS_OK;
 }
@@ -1848,13 +1938,66 @@ GS_API gs_status gs_sparse_solve_batch(gs_ctx* ctx, int32_t count, const gs_csr*
                                        double* const* x, gs_status* per_system) {
   return guarded(ctx, [&]() -> gs_status {
     if (count < 0 || (count > 0 && (!systems || !b || !x || !per_system))) fail(GS_INVALID_ARGUMENT, "null argument");
-    for (int s = 0; s < count; s++) {
+    // The systems whose structure has a cached device plan go to the device together, one launch per structure
+    // (a wave per system); the rest, and any whose pivots the cached plan cannot keep, one at a time as
+    // gs_sparse_solve solves them (a fresh host factorisation, then a plan cached for the systems after them).
+    std::map<std::vector<int>, std::vector<int>> by_plan;
+    std::vector<int> alone;
+    for (int i = 0; i < count; i++) {
+      const gs_csr& sy = systems[i];
+      per_system[i] = GS_OK;
       try {
-        per_system[s] = seam_solve_one(ctx, systems[s].n, systems[s].rowptr, systems[s].colidx, systems[s].val,
-                                       b[s], x[s]);
+        const gs_status st = seam_check(ctx, sy.n, sy.rowptr, sy.colidx, sy.val, b[i], x[i]);
+        if (st != GS_OK) { per_system[i] = st; continue; }
+        if (sy.n == 0) continue;
+      } catch (const HipError& e) {
+        per_system[i] = e.code;
+        continue;
+      }
+      bool finite_rhs = true;
+      for (int k = 0; k < sy.n && finite_rhs; k++) finite_rhs = std::isfinite(b[i][k]);
+      std::vector<int> key = seam_key(sy.n, sy.rowptr, sy.colidx, sy.val);
+      // (a non-finite right-hand side takes the single-system path, which orders its checks as the reference does)
+      if (finite_rhs && ctx->seam_plans.find(key)) by_plan[std::move(key)].push_back(i);
+      else alone.push_back(i);
+    }
+    hipStream_t s = ctx->stream;
+    for (auto& [key, members] : by_plan) {
+      std::shared_ptr<SeamPlan> sp = *ctx->seam_plans.find(key);
+      const DevPlan& h = sp->dev.h;
+      const int nnz = systems[members[0]].rowptr[systems[members[0]].n];
+      const SeamLayout L = seam_layout(h, nnz);
+      const int chunk = (int)std::max<size_t>(1, std::min<size_t>(members.size(), (size_t)(1u << 30) / L.total));
+      for (size_t c0 = 0; c0 < members.size(); c0 += chunk) {
+        const int cnt = (int)std::min<size_t>(chunk, members.size() - c0);
+        std::vector<const double*> vals(cnt), rhs(cnt);
+        for (int j = 0; j < cnt; j++) {
+          vals[j] = systems[members[c0 + j]].val;
+          rhs[j] = b[members[c0 + j]];
+        }
+        ctx->seam_buf.ensure(seam_many_bytes(L, cnt));
+        std::vector<std::vector<double>> xt;
+        std::vector<int32_t> status;
+        seam_run_many(h, ctx->seam_buf.as<char>(), L, nnz, cnt, vals.data(), rhs.data(), xt, status, s);
+        for (int j = 0; j < cnt; j++) {
+          const int i = members[c0 + j];
+          if (status[j] == 0) {
+            std::copy(xt[j].begin(), xt[j].end(), x[i]);
+            ctx->seam_plan_hits++;
+          } else {
+            alone.push_back(i);   // a pivot of the cached plan would move: afresh on the host
+          }
+        }
+      }
+    }
+    std::sort(alone.begin(), alone.end());
+    for (int i : alone) {
+      try {
+        per_system[i] = seam_solve_one(ctx, systems[i].n, systems[i].rowptr, systems[i].colidx, systems[i].val,
+                                       b[i], x[i]);
       } catch (const HipError& e) {
         if (e.code == GS_DEVICE_ERROR) throw;
-        per_system[s] = e.code;
+        per_system[i] = e.code;
       }
     }
     return GS_OK;

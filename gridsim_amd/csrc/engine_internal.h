@@ -299,9 +299,17 @@ int symbolic_threads();
 gs_nr_options normalise(const gs_nr_options* o);
 Layout instance_layout(const DevPlan& h, bool fp32, bool x_lds);
 BatchRun plan_run(const DevPlan& h, const gs_nr_options& o, bool force_fp64);
-std::vector<int> outage_set(const gs_instance& in, int n_branch);
+// An instance's outaged branches (no allocation: this runs per instance on every batch call).
+struct OutageSet {
+  int n = 0;
+  int b[kMaxOutages + 1];
+  const int* begin() const { return b; }
+  const int* end() const { return b + n; }
+};
+OutageSet outage_set(const gs_instance& in, int n_branch);
+constexpr int kMaxOutagePatches = 4 * (kMaxOutages + 1);   // the Y cells an outage set can touch
 void check_instances(int n_branch, int count, const gs_instance* inst, const std::string& where);
-std::vector<Patch> outage_patches(const YBus& y, const Network& net, const std::vector<int>& outaged);
+int outage_patches(const YBus& y, const Network& net, const OutageSet& outaged, Patch* out);
 int own_y_count(const YBus* y, const Network* netw, int count, const gs_instance* inst);
 void fill_result(const InstState& s, gs_nr_result& r, int refreshes, double us_per_tick);
 

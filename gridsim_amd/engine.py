@@ -24,7 +24,8 @@ import numpy as np
 from .network import SLACK, Network, bridges, splits
 
 PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG, "libgridsim_hip.so")
+# GRIDSIM_HIP_LIB: another build of the same library (the host-sanitizer build of the CPU tests)
+LIB_PATH = os.environ.get("GRIDSIM_HIP_LIB") or os.path.join(PKG, "libgridsim_hip.so")
 
 GS_OK, GS_SINGULAR, GS_NONFINITE_INPUT, GS_OUT_OF_MEMORY, GS_INVALID_ARGUMENT, GS_DEVICE_ERROR, GS_NO_DEVICE, \
     GS_BUSY, GS_PENDING = range(9)
@@ -440,7 +441,11 @@ class DeviceNetwork:
         arr = (GsInstance * max(count, 1))()
         view = np.frombuffer(arr, INSTANCE_DTYPE)[:count]    # filled column by column, not record by record
         keep = []
-        if outages is not None and any(np.ndim(o) > 0 for o in list(outages)[:count]):
+        flat = None
+        if outages is not None and not isinstance(outages, (list, tuple)):
+            a = np.asarray(outages)
+            flat = a if a.ndim == 1 and a.dtype != object else None   # one branch per instance: no per-item scan
+        if outages is not None and flat is None and any(np.ndim(o) > 0 for o in list(outages)[:count]):
             lists = [np.atleast_1d(np.asarray(o, np.int32)) for o in list(outages)[:count]]
             view["outage_branch"] = [int(o[0]) if len(o) else -1 for o in lists]
             rest = [np.ascontiguousarray(o[1:]) for o in lists]

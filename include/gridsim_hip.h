@@ -210,7 +210,8 @@ GS_API gs_status gs_sparse_solve(gs_ctx* ctx, int32_t n, const int32_t* rowptr, 
 /* gs_sparse_solve that allocates no device memory (no counterpart in the reference: for callers that cannot allocate,
  * such as code under HIP graph capture or a serving loop that must not call hipMalloc). The same contract, arithmetic
  * and results as gs_sparse_solve; the plan and the call's device buffers go into a fixed scratch array of 64 MiB that
- * each GPU holds until the process ends (no plan is cached between calls). A system needing more than that returns
+ * each GPU holds until the process ends. The array keeps the last plan built in it: a call on the same structure
+ * replays it (as long as its pivots hold) instead of building and uploading it again. A system needing more returns
  * GS_INVALID_ARGUMENT (gs_last_error says how much it needs). Calls on contexts of the same GPU share the array: they
  * must not overlap (calls on one context never do, the context serialises them). */
 GS_API gs_status gs_sparse_solve_noalloc(gs_ctx* ctx, int32_t n, const int32_t* rowptr, const int32_t* colidx,
@@ -225,7 +226,9 @@ typedef struct gs_csr {
 } gs_csr;
 
 /* One status per system in per_system (a failing system never aborts the others); returns GS_OK unless the call
- * itself is malformed (GS_INVALID_ARGUMENT) or the device failed. */
+ * itself is malformed (GS_INVALID_ARGUMENT) or the device failed. The systems whose structure (pattern and nonzero
+ * positions) has a cached device plan are solved together, one launch per structure and a wavefront per system; the
+ * others, and any whose pivots that plan cannot keep, one at a time as gs_sparse_solve solves them. */
 GS_API gs_status gs_sparse_solve_batch(gs_ctx* ctx, int32_t count, const gs_csr* systems, const double* const* b,
                                        double* const* x, gs_status* per_system);
 

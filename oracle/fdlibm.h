@@ -232,7 +232,7 @@ static inline int rem_pio2(double x, double* y) {
   // large arguments: |x| split into three 24-bit pieces scaled by 2^-e0
   double tx[3];
   int32_t e0 = (ix >> 20) - 1046;
-  double z = from_words(ix - (e0 << 20), lo_word(x));
+  double z = from_words(ix - e0 * (1 << 20), lo_word(x));   // (e0 may be negative: no left shift of it)
   for (int i = 0; i < 2; i++) { tx[i] = (double)((int32_t)z); z = (z - tx[i]) * two24; }
   tx[2] = z;
   int nx = 3;

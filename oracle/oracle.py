@@ -11,7 +11,8 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+# GRIDSIM_ORACLE_LIB: another build of the same library (the host-sanitizer build of the CPU tests)
+LIB_PATH = os.environ.get("GRIDSIM_ORACLE_LIB") or os.path.join(HERE, "_build", "liboracle.so")
 
 CONVERGED, MAX_ITERATIONS, DIVERGED, NONFINITE = 0, 1, 2, 3
 STATUS_NAMES = {0: "Converged", 1: "MaxIterations", 2: "Diverged", 3: "NonFinite"}

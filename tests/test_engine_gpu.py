@@ -334,8 +334,8 @@ def test_batch_voltages_of_instances_the_host_finished(ctx, eng, oracle, two_net
 
 
 def test_autotune_with_the_level_parallel_schedule_offered(ctx, eng, oracle, two_networks):
-    """Options asking for GS_SCHEDULE_LEVELS let the autotuner weigh the level-parallel schedule too; its choice keeps
-    the oracle's verdicts and voltages."""
+    """Options asking for GS_SCHEDULE_LEVELS let the autotuner weigh the level-parallel schedule and a work-group of
+    waves per instance too; its choice keeps the oracle's verdicts and voltages."""
     from gridsim_amd import gridgen
     net = two_networks[1]
     outs = gridgen.config2_outages(net)[::40][:8]
@@ -344,7 +344,7 @@ def test_autotune_with_the_level_parallel_schedule_offered(ctx, eng, oracle, two
     chosen, ms = eng.autotune(dn, outages=outs, options=eng.NrOptions(max_iterations=10, schedule=eng.SCHEDULE_LEVELS))
     print(f"autotune with levels offered: lu {chosen.lu_precision} refine {chosen.refine_steps} "
           f"schedule {chosen.schedule}, {ms:.3f} ms", flush=True)
-    assert chosen.schedule in (0, 256, 512, eng.SCHEDULE_LEVELS)
+    assert chosen.schedule in (0, 256, 512, eng.SCHEDULE_LEVELS, eng.SCHEDULE_GROUP)
     r = dn.solve(outages=outs, options=chosen)
     _compare(oracle, r.results, r.vm, r.va, rres, rvm, rva, "300-bus N-1 sample, autotuned with levels offered")
     dn.close()
@@ -601,3 +601,42 @@ def test_reactive_limits_synthetic_batch_matches_oracle(ctx, eng, oracle, n1_118
     print(f"{switched} of {len(outs)} outages ran limit passes", flush=True)
     assert switched > 0
     dn.close()
+
+
+def test_sparse_solve_batch_launches_shared_structures_together(ctx, eng, oracle):
+    """gs_sparse_solve_batch: systems of one structure with a cached plan go to the device in one launch; a new
+    structure, a singular system and a non-finite right-hand side take the single-system path. Every answer equals
+    gs_sparse_solve's and the oracle's, bit for bit, and the statuses are the single calls' statuses."""
+    net = matpower_case("case118")
+    t, vm, va, _, _ = oracle.bus_spec(net)
+    rp, ci, val = oracle.jacobian(net, vm, va)
+    rng = np.random.default_rng(5)
+    single = eng.HipSparseLinearSolver(ctx)
+    single.solve(rp, ci, val, np.ones(len(rp) - 1))   # caches the plan of this structure
+    mats, rhs = [], []
+    for k in range(12):
+        mats.append((rp, ci, val * (1 + 1e-7 * rng.standard_normal(len(val)))))
+        rhs.append(rng.standard_normal(len(rp) - 1))
+    rp9, ci9, val9 = oracle.jacobian(matpower_case("case9"), *oracle.bus_spec(matpower_case("case9"))[1:3])
+    mats.append((rp9, ci9, val9))                                # a structure with no cached plan yet
+    rhs.append(rng.standard_normal(len(rp9) - 1))
+    r2 = np.array([0, 2, 4], np.int32)
+    c2 = np.array([0, 1, 0, 1], np.int32)
+    mats.append((r2, c2, np.array([1.0, 2.0, 2.0, 4.0])))        # singular
+    rhs.append(np.ones(2))
+    mats.append((rp, ci, val))                                   # non-finite right-hand side
+    bad = np.ones(len(rp) - 1)
+    bad[3] = np.inf
+    rhs.append(bad)
+    errs = [None] * len(mats)
+    out = eng.HipBatchedSparseLinearSolver(ctx).solve_batch(mats, rhs, errs)
+    for k, ((r, c, v), b) in enumerate(zip(mats, rhs)):
+        if k == len(mats) - 1:
+            assert out[k] is None and isinstance(errs[k], ValueError)
+            continue
+        rc, xr = oracle.sparse_solve(r, c, v, b)
+        if rc != 0:
+            assert out[k] is None and errs[k] is None, k
+            continue
+        assert out[k] is not None and np.array_equal(out[k], xr), k
+        assert np.array_equal(single.solve(r, c, v, b), xr), k

@@ -55,6 +55,21 @@ def test_no_alloc_seam_equals_the_allocating_seam(ctx, oracle):
             assert rc == 0 and np.array_equal(x1, xr)
 
 
+def test_no_alloc_seam_replays_the_plan_its_array_holds(ctx, oracle):
+    """Calls on one structure in a row: the second and third replay the plan the scratch array holds (new values, the
+    same pivots), and still give gs_sparse_solve's and the oracle's bits."""
+    rng = np.random.default_rng(12)
+    rp, ci, val = _jacobian(oracle, "case118")
+    for rep in range(3):
+        v = val * (1 + 1e-6 * rng.standard_normal(len(val)))
+        b = rng.standard_normal(len(rp) - 1)
+        st0, x0 = _solve("gs_sparse_solve", ctx, rp, ci, v, b)
+        st1, x1 = _solve("gs_sparse_solve_noalloc", ctx, rp, ci, v, b)
+        assert st0 == st1 == 0 and np.array_equal(x0, x1), rep
+        rc, xr = oracle.sparse_solve(rp, ci, v, b)
+        assert rc == 0 and np.array_equal(x1, xr), rep
+
+
 def test_no_alloc_seam_keeps_the_seam_contract(ctx):
     from gridsim_amd import engine
     rp = np.array([0, 2, 4], np.int32)
