@@ -208,3 +208,39 @@ def test_server_n2_requests_match_the_oracle(ctx, warm_nets, oracle):
             assert np.abs(vm - rvm).max() <= TOL_DV and np.abs(va - rva).max() <= TOL_DV, r
     srv.close()
     dn.close()
+
+
+def test_server_runs_a_work_group_per_instance_network(ctx, warm_nets, oracle):
+    """A network served on the work-group-per-instance schedule (8 waves per instance): every request's answer is the
+    wavefront batch API's bit for bit, alone and mixed with the other networks' requests, and the oracle's."""
+    from gridsim_amd import engine
+    group = engine.NrOptions(max_iterations=10, lu_precision=engine.LU_FP64, refine_steps=0,
+                             schedule=engine.SCHEDULE_GROUP, flags=engine.group_waves(8))
+    plain = engine.NrOptions(max_iterations=10, lu_precision=engine.LU_FP32, refine_steps=1)
+    dns = [engine.DeviceNetwork(ctx, warm_nets[2], group), engine.DeviceNetwork(ctx, warm_nets[0], plain)]
+    srv = engine.Server(ctx, max_batch=16)
+    ids = [srv.add_network(dns[0], group), srv.add_network(dns[1], plain)]
+    rng = np.random.default_rng(9)
+    from gridsim_amd import gridgen
+    outs = gridgen.config2_outages(warm_nets[2])
+    reqs = [(int(rng.integers(2)), int(rng.choice(outs)) if rng.random() < 0.7 else -1, float(rng.uniform(0.8, 1.2)))
+            for _ in range(40)]
+    reqs = [(k, o if k == 0 else -1, s) for k, o, s in reqs]
+    tickets = [srv.submit(ids[k], o, s) for k, o, s in reqs]
+    mix = [srv.wait(t, ids[k]) for t, (k, _, _) in zip(tickets, reqs)]
+    sel = [i for i, r in enumerate(reqs) if r[0] == 0]
+    wave = dns[0].solve(outages=[reqs[i][1] for i in sel], load_scale=[reqs[i][2] for i in sel],
+                        options=engine.NrOptions(max_iterations=10, lu_precision=engine.LU_FP64, refine_steps=0))
+    for j, i in enumerate(sel):
+        assert wave.status[j] == mix[i][0]["status"] and wave.results["iterations"][j] == mix[i][0]["iterations"]
+        assert np.array_equal(wave.vm[j], mix[i][1]) and np.array_equal(wave.va[j], mix[i][2]), reqs[i]
+    for i in sel[:6]:
+        k, o, s = reqs[i]
+        rec, vm, va = oracle.solve(warm_nets[0 if k else 2].with_load_multiplier(s), oracle.options(max_iterations=10),
+                                   skip=o)
+        assert rec["status"] == mix[i][0]["status"]
+        if rec["status"] == 0:
+            assert np.abs(vm - mix[i][1]).max() <= TOL_DV and np.abs(va - mix[i][2]).max() <= TOL_DV
+    srv.close()
+    for dn in dns:
+        dn.close()
