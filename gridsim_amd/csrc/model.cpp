@@ -824,6 +824,8 @@ void compile_plan(const Plan& plan, DevicePlanHost& d, int scratch_cap) {
     steps = 0;
     pass_step.clear();
     level_pass.clear();
+    // the step at which each row's value ends in this stream (rows it never writes: long before it starts)
+    vector<int> end_step(m, -D::kStepXAhead);
     for (int lv = 0; lv < nlv; lv++) {
       level_pass.push_back((int)pass_step.size());
       int b1 = level_end[lv];
@@ -857,6 +859,9 @@ void compile_plan(const Plan& plan, DevicePlanHost& d, int scratch_cap) {
               ent_idx.push_back(ridx[rptr[rows[p0 + l]] + e]);
             }
           for (int e0 = 0; e0 < fill; e0 += 64) {
+            int fl = D::kStepProduct;
+            for (int e = e0; e < std::min(fill, e0 + 64); e++)
+              if (end_step[ent_col[e]] > steps - D::kStepXAhead) fl |= D::kProductLate;
             for (int lane = 0; lane < 64; lane++) {
               const int e = e0 + lane;
               int col = -1, idx = 0;
@@ -865,7 +870,7 @@ void compile_plan(const Plan& plan, DevicePlanHost& d, int scratch_cap) {
                 idx = n_val++;
                 pos[ent_idx[e]] = idx;
               }
-              rec.push_back(col); rec.push_back(idx | (D::kStepProduct << D::kRsMetaShift));
+              rec.push_back(col); rec.push_back(idx | (fl << D::kRsMetaShift));
             }
             steps++;
           }
@@ -877,7 +882,11 @@ void compile_plan(const Plan& plan, DevicePlanHost& d, int scratch_cap) {
               row = rows[p0 + lane];
               x0 = st[lane] | (take[lane] << 16);
               if (!begun[lane]) { fl |= D::kChainBegin; begun[lane] = 1; }
-              if (done[lane] + take[lane] == len[lane]) { fl |= D::kChainEnd; ended[lane] = 1; }
+              if (done[lane] + take[lane] == len[lane]) {
+                fl |= D::kChainEnd;
+                ended[lane] = 1;
+                end_step[row] = steps;
+              }
             }
             rec.push_back(x0); rec.push_back(row | (fl << D::kRsMetaShift));
           }
@@ -1263,7 +1272,7 @@ bool emulate_device_lu(const DevicePlanHost& d, int nnz_j, const double* jv, con
     using D = DevicePlanHost;
     const int cap = d.solve_scratch();
     vector<double> acc(64, 0.0), scratch(cap, std::nan(""));
-    vector<int> row(64, -1);
+    vector<int> row(64, -1), end_step(m, -D::kStepXAhead);
     int sbase = 0;
     if ((int)rec.size() != steps * 64 * 2) { why = "stream size"; return false; }
     for (int t = 0; t < steps; t++) {
@@ -1273,6 +1282,8 @@ bool emulate_device_lu(const DevicePlanHost& d, int nnz_j, const double* jv, con
         const int* r2 = &rec[((size_t)t * 64 + lane) * 2];
         const int meta = (int)((unsigned)r2[1] >> D::kRsMetaShift), field = r2[1] & D::kRsIndexMask;
         if ((meta & D::kStepKindMask) != kind) { why = "stream step kind differs across lanes"; return false; }
+        if (kind == D::kStepProduct && meta != (int)((unsigned)rec[((size_t)t * 64) * 2 + 1] >> D::kRsMetaShift))
+          { why = "product flags differ across lanes"; return false; }
         // the device loads the value array and (back) the pivot array at every step's field: they hold
         // max(values, m) and max(n_bval, m) entries
         if (field >= std::max({(int)vals.size(), m, 1}) || (back && field >= std::max({d.n_bval, m, 1})))
@@ -1281,6 +1292,9 @@ bool emulate_device_lu(const DevicePlanHost& d, int nnz_j, const double* jv, con
           if (sbase + 64 > cap) { why = "products pass the scratch area"; return false; }
           if (r2[0] < 0) continue;
           if (r2[0] >= m || !final_rows[r2[0]]) { why = "stream reads a row not yet final"; return false; }
+          // a device with x in global memory loads it kStepXAhead steps early unless the step says otherwise
+          if (!(meta & D::kProductLate) && end_step[r2[0]] > t - D::kStepXAhead)
+            { why = "a product loads x ahead of its column's last chain"; return false; }
           if (field >= (int)vals.size()) { why = "stream value index"; return false; }
           scratch[sbase + lane] = vals[field] * x[r2[0]];
         } else if (kind == D::kStepChain) {
@@ -1305,6 +1319,7 @@ bool emulate_device_lu(const DevicePlanHost& d, int nnz_j, const double* jv, con
           if (meta & D::kChainEnd) {
             x[row[lane]] = back ? acc[lane] / dval[row[lane]] : acc[lane];
             ended.push_back(row[lane]);
+            end_step[row[lane]] = t;
             row[lane] = -1;
           }
         } else if (kind != 0) {

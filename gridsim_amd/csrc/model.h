@@ -189,12 +189,16 @@ struct DevicePlanHost {
   //   CHAIN    (start | len << 16, row | (kStepChain | kChainBegin? | kChainEnd? | kChainIdle?) << 24): kChainBegin:
   //            acc = x[row]; then acc -= scratch[start + t] for t < len; kChainEnd: x[row] = acc (back: acc / pivot).
   //            kChainIdle: a lane with no row (its row field 0).
+  // A device whose solve vector sits in global memory loads x kStepXAhead steps ahead with the values: x[row] for a
+  // CHAIN step (only the row's own last CHAIN step writes it), x[col] for a PRODUCT step unless kProductLate marks it
+  // (a column of the step ends fewer than kStepXAhead steps before it in the stream): that step loads x at the step.
   // The factors are stored in the order the PRODUCT steps read them: fval holds L's off-diagonal entries in
   // forward-stream order, bval U's off-diagonal entries in back-stream order (the active lanes of a step being a
   // prefix, a step's values are contiguous), dval U's diagonal by pivot step. fpos / bpos map an L / U entry (CSC
   // index) to its position, -1 for a diagonal. The refactor keeps L in column order too (lx, CSC), for its own updates.
   static constexpr int kStepProduct = 1, kStepChain = 2, kStepKindMask = 15;
   static constexpr int kChainBegin = 16, kChainEnd = 32, kChainIdle = 64;
+  static constexpr int kProductLate = 16, kStepXAhead = 8;
   // scratch slots the solves may use: the refactor workspace rounded down to whole steps (or fewer, when the plan was
   // compiled with a smaller cap: scratch_cap, 0 = none; tests use it to make passes run in several segments)
   int scratch_cap = 0;

@@ -66,6 +66,7 @@ constexpr int kWriteU = 1, kWritePivot = 2, kWriteL = 3;
 // solve step kinds and CHAIN flags (model.h DevicePlanHost)
 constexpr int kStepProduct = 1, kStepChain = 2, kStepKindMask = 15;
 constexpr int kChainBegin = 16, kChainEnd = 32, kChainIdle = 64;
+constexpr int kProductLate = 16, kStepXAhead = 8;   // a PRODUCT step that loads x at the step, not ahead
 // load-ahead distance of the row and step streams, and the padding each stream carries; rows of the refactor load
 // their value kRsValueAhead rows ahead (the plan places its rows for that distance)
 constexpr int kStreamPad = 16;

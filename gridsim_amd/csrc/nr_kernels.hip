@@ -447,8 +447,10 @@ __device__ __forceinline__ int refactor(const DevPlan& P, GLOBAL T* vals, GLOBAL
 // instead of one step. Passes larger than the scratch area run in segments; a lane's running value stays in its
 // register across them. The step records are loaded kRing steps ahead and the factor values kHalf steps ahead (the
 // factors are final before a solve starts; the streams are padded by kRing steps); a PRODUCT step's values are
-// contiguous (its active lanes are a prefix), so each value load is one coalesced segment. x is read at the step
-// itself. XP: the solve vector (LDS or global); scr: the scratch area (LDS, the wave's refactor workspace).
+// contiguous (its active lanes are a prefix), so each value load is one coalesced segment. A solve vector in LDS is
+// read at the step itself; one in global memory is loaded kHalf steps ahead with the values (x[row] for a CHAIN step,
+// x[col] for a PRODUCT step), except at PRODUCT steps the plan marks kProductLate, whose columns end too shortly
+// before them. XP: the solve vector (LDS or global); scr: the scratch area (LDS, the wave's refactor workspace).
 // Steps [step0, step1) of the stream (the whole stream, or one pass of a level-parallel run): whole rings from step0,
 // the steps from step1 on doing nothing.
 // Maximum of a non-negative int over the wave, with DPP moves (no LDS traffic): shifts within rows of 16 lanes, then
@@ -483,6 +485,12 @@ __device__ __forceinline__ void solve_stream(GLOBAL const v2i* rec, int step0, i
     if constexpr (WIDE) return r.y & kRsIndexMask;
     return kind_of(r) == kStepChain ? (r.y & kRsIndexMask) : 0;
   };
+  // the entry of x a step reads: a CHAIN step's row, a PRODUCT step's column (an idle lane's -1, and the padding's,
+  // read as 0)
+  auto x_index = [&](const v2i& r) { return kind_of(r) == kStepChain ? (r.y & kRsIndexMask) : max(r.x, 0); };
+  static_assert(kHalf == kStepXAhead, "x loads ahead by the distance the plan marks late steps for");
+  constexpr bool XPRE = !XLDS;
+  T rx[kRing];
   // first loads in the loop's own order (see refactor): records of steps 0..kHalf-1, then per step the record of step
   // kHalf+h and the values of step h; the loop's first kHalf steps load the values of steps kHalf..kRing-1
 #pragma unroll
@@ -492,6 +500,7 @@ __device__ __forceinline__ void solve_stream(GLOBAL const v2i* rec, int step0, i
       const int h = s - (kRing - kHalf);
       rv[h] = vals[value_index(rr[h])];
       rd[h] = BACK ? dval[pivot_index(rr[h])] : T(1);
+      if constexpr (XPRE) rx[h] = x[x_index(rr[h])];
     }
     __builtin_amdgcn_sched_barrier(0);
   }
@@ -503,11 +512,20 @@ __device__ __forceinline__ void solve_stream(GLOBAL const v2i* rec, int step0, i
       const int t = base + s;
       const v2i r2 = rr[s];
       // the step's kind is the same in every lane: a scalar, taken at the step itself
-      int kind = (int)((unsigned)__builtin_amdgcn_readfirstlane(r2.y) >> kRsMetaShift) & kStepKindMask;
+      const int meta = (int)((unsigned)__builtin_amdgcn_readfirstlane(r2.y) >> kRsMetaShift);
+      int kind = meta & kStepKindMask;
       if (t >= steps) kind = 0;
       if (kind == kStepProduct) {
         const int c = r2.x;
-        const T xv = x[c >= 0 ? c : 0];
+        T xv;
+        if (XPRE && !(meta & kProductLate)) {
+          xv = rx[s];
+        } else {
+          xv = x[c >= 0 ? c : 0];
+          // the load completes inside this branch, so that where the branches join the other loads in flight are
+          // waited for only as far as the prefetched values need (vmcnt 0, expcnt and lgkmcnt left at their maxima)
+          if constexpr (XPRE) __builtin_amdgcn_s_waitcnt(0x0F70);
+        }
         scr[sbase + lane] = rv[s] * xv;   // (an idle lane's slot is never read)
         sbase += kWave;
         WAVE_SYNC();
@@ -516,7 +534,7 @@ __device__ __forceinline__ void solve_stream(GLOBAL const v2i* rec, int step0, i
         const bool active = !(fl & kChainIdle);
         const int st = r2.x & 0xFFFF, len = (int)((unsigned)r2.x >> 16);
         const int maxlen = wave_max_nonneg(len);
-        const T x0 = x[row];
+        const T x0 = XPRE ? rx[s] : x[row];
         if (fl & kChainBegin) acc = x0;
 #pragma unroll 4
         for (int e = 0; e < maxlen; e++) {
@@ -533,6 +551,7 @@ __device__ __forceinline__ void solve_stream(GLOBAL const v2i* rec, int step0, i
       const int hs = (s + kHalf) % kRing;
       rv[hs] = vals[value_index(rr[hs])];
       if (BACK) rd[hs] = dval[pivot_index(rr[hs])];
+      if constexpr (XPRE) rx[hs] = x[x_index(rr[hs])];
     }
   }
 }
