@@ -57,13 +57,43 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 
-// maximum over the tl lanes of one track (aligned groups of tl, a power of two the same in every lane)
+// A lane's value from another lane of its 16-lane row by a DPP pattern (every lane has a source; no LDS traffic):
+// quad_perm [1,0,3,2] and [2,3,0,1] swap neighbours and pairs within each quad, row_half_mirror (lane 7 - i of each
+// 8-lane half) and row_mirror (lane 15 - i of each row) reach the other half.
+constexpr int kDppQuadXor1 = 0xB1, kDppQuadXor2 = 0x4E, kDppHalfMirror = 0x141, kDppMirror = 0x140;
+template <int CTRL>
+__device__ __forceinline__ int dpp_i32(int v) { return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false); }
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)dpp_i32<CTRL>((int)b), hi = (unsigned)dpp_i32<CTRL>((int)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// maximum over the tl lanes of one track (aligned groups of tl, a power of two the same in every lane). Tracks of 8 and
+// 16 lanes (nearly every pivot row) reduce with DPP moves: after the two quad steps each quad holds its maximum, and
+// the mirrors pair each lane with one of the other quad (half) of its group. Wider tracks shuffle through LDS. The
+// maximum of non-negative values and NaN does not depend on the order it is taken in.
 __device__ __forceinline__ double track_max(double v, int tl) {
+  if (tl <= 16) {
+    v = nan_max(v, dpp_f64<kDppQuadXor1>(v));
+    v = nan_max(v, dpp_f64<kDppQuadXor2>(v));
+    v = nan_max(v, dpp_f64<kDppHalfMirror>(v));
+    if (tl == 16) v = nan_max(v, dpp_f64<kDppMirror>(v));
+    return v;
+  }
   for (int off = tl / 2; off > 0; off >>= 1) v = nan_max(v, __shfl_xor(v, off, kWave));
   return v;
 }
 
 __device__ __forceinline__ int track_any(int v, int tl) {
+  if (tl <= 16) {
+    v |= dpp_i32<kDppQuadXor1>(v);
+    v |= dpp_i32<kDppQuadXor2>(v);
+    v |= dpp_i32<kDppHalfMirror>(v);
+    if (tl == 16) v |= dpp_i32<kDppMirror>(v);
+    return v;
+  }
   for (int off = tl / 2; off > 0; off >>= 1) v |= __shfl_xor(v, off, kWave);
   return v;
 }

@@ -132,6 +132,32 @@ def test_n1_batch_2000_subset_matches_oracle(ctx, eng, oracle):
     _compare(oracle, r.results, r.vm, r.va, rres, rvm, rva, "N-1 2000 subset")
 
 
+@pytest.mark.parametrize("case", ["case118", "case300"])
+@pytest.mark.parametrize("lu", ["fp64", "fp32"])
+def test_n1_loop_transformer_outages_match_oracle(ctx, eng, oracle, case, lu):
+    """Every non-bridge outage of a MATPOWER grid whose off-nominal transformers sit inside loops (9 of case118's 177
+    outages, 40 of case300's 322), warm-started from the base solution: each outaged transformer's tap-scaled stamp
+    leaves the instance's Y exactly as the network rebuilt without it, and the verdicts and states match the oracle."""
+    from gridsim_amd import gridgen
+    net = matpower_case(case)
+    rec, vm, va = oracle.solve(net)
+    warm = net.with_start(vm, np.degrees(va))
+    outs = gridgen.config2_outages(warm)
+    tap = warm.br_tap[outs]
+    n_xf = int(((tap != 0) & (tap != 1)).sum())
+    assert n_xf >= 9
+    rres, rvm, rva = oracle.screen(warm, outs, oracle.options(max_iterations=10), threads=8)
+    opts = eng.NrOptions(max_iterations=10, lu_precision=eng.LU_FP64 if lu == "fp64" else eng.LU_FP32)
+    dn = eng.DeviceNetwork(ctx, warm, opts)
+    r = dn.solve(outages=outs)
+    print(f"{case} N-1 {lu}: {len(outs)} outages ({n_xf} transformers), statuses {np.bincount(r.status)}",
+          flush=True)
+    _compare(oracle, r.results, r.vm, r.va, rres, rvm, rva, f"N-1 {case} {lu}")
+    if lu == "fp64":
+        assert np.array_equal(r.results["iterations"], rres["iterations"])
+    dn.close()
+
+
 def test_two_instances_per_work_group_equal_one(ctx, eng, oracle):
     """Where the solve vector sits in LDS and two instances per work-group keep more of them resident, the batched
     kernel pairs them on one shared LDS solve vector (gs_network_launch_shape says so). The pairing changes no bit:
