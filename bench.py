@@ -53,7 +53,8 @@ def measured_traffic(stats: dict, lu: str, flags: int, alg: int):
     with open(path) as fh:
         rec = json.load(fh)
     fp = rec["fingerprint"]
-    mine = {k: stats[k] for k in ("m", "nnz_j", "nnz_l", "nnz_u", "refactor_rows", "forward_steps", "back_steps")}
+    mine = {k: stats.get(k) for k in ("m", "nnz_j", "nnz_l", "nnz_u", "refactor_rows", "forward_steps", "back_steps",
+                                      "factor_slots")}
     mine.update(lu=lu, algorithmic_bytes=int(alg))
     if any(fp.get(k) != v for k, v in mine.items()):
         return None, None

@@ -153,11 +153,10 @@ void build_plan_image(PlanImage& img, DevPlan& h, DevicePlanHost& d, const YBus*
       for (int c = 0; c < 2; c++)
         rs_rec[(((size_t)(r / 2) * 64 + l) * 2 + (r % 2)) * 2 + c] = d.rs_rec[((size_t)r * 64 + l) * 2 + c];
   img.put(&DevPlan::rs_rec, rs_rec); img.put(&DevPlan::ch_a_src, d.ch_a_src);
-  // the solves' step streams (padded with steps that touch nothing) and the factors' stream-order sizes
+  // the solves' step streams (padded with steps that touch nothing) and the factor slots (64 per WRITE row)
   h.fs_steps = ring_up(d.fs_steps);
   h.bs_steps = ring_up(d.bs_steps);
-  h.n_fval = d.n_fval;
-  h.n_bval = d.n_bval;
+  h.n_fac = d.n_fac;
   auto pad_steps = [](const std::vector<int>& rec, int steps) {
     std::vector<int> prec(rec);
     for (int t = (int)(rec.size() / 128); t < steps + 2 * kStreamPad; t++)
@@ -250,6 +249,7 @@ gs_plan_stats plan_stats(const DevicePlanHost& d, const YBus& y, const JacobianP
   st.forward_steps = ring_up(d.fs_steps);
   st.back_steps = ring_up(d.bs_steps);
   st.workspace_stride = d.wstride;
+  st.factor_slots = d.n_fac;
   const int64_t n = y.n, nnz_y = y.rp.empty() ? 0 : y.rp[y.n], m = jp.m, rec_row = 64 * 8;
   int64_t b = 4 * (n + 1) + 4 * nnz_y * 2 + 4 * n + 16 * nnz_y + 2 * 16 * nnz_y + 2 * 4 * n;  // Y-bus, slots, pos
   b += 4 * (m + 1) + 4 * (int64_t)st.nnz_j + 2 * 8 * n + 4 * 4 * m;                            // J pattern, start, pivots
@@ -329,8 +329,6 @@ Layout instance_layout(const DevPlan& h, bool fp32, bool x_lds) {
   L.jv = o; o += al(8 * (size_t)h.nnz_j);
   L.tpq = o; o += al(16 * (size_t)h.nnz_y);
   L.vals = o; o += al(t * (size_t)vals_slots(h));
-  L.fval = o; o += al(t * (size_t)fval_slots(h));
-  L.bval = o; o += al(t * (size_t)bval_slots(h));
   L.dval = o; o += al(t * (size_t)dval_slots(h));
   L.xg = o; o += x_lds ? 0 : al(t * (size_t)h.m);
   L.spec = o; o += al(16 * (size_t)h.n);
@@ -500,7 +498,7 @@ PreparedRun prepare_into(const DevPlan& h, const RunBuffers& rb, Staging& stg, i
     d.vm = (double*)(b + L.vm); d.va = (double*)(b + L.va); d.p = (double*)(b + L.p); d.q = (double*)(b + L.q);
     d.f = (double*)(b + L.f); d.dx = (double*)(b + L.dx); d.r = (double*)(b + L.r); d.jv = (double*)(b + L.jv);
     d.tpq = (double*)(b + L.tpq);
-    d.vals = b + L.vals; d.fval = b + L.fval; d.bval = b + L.bval; d.dval = b + L.dval;
+    d.vals = b + L.vals; d.dval = b + L.dval;
     d.xg = run.x_lds ? nullptr : b + L.xg;
     const gs_instance& in = inst[i];
     d.n_patch = 0;
@@ -1474,7 +1472,7 @@ struct SeamPlan {
 // matrix values and right-hand side as given, the solution, the factor arrays, the global solve vector, the row
 // floors, the status word and the kernel's descriptor.
 struct SeamLayout {
-  size_t a, b, x, vals, fval, bval, dval, xs, rf, status, desc, total;
+  size_t a, b, x, vals, dval, xs, rf, status, desc, total;
 };
 
 SeamLayout seam_layout(const DevPlan& h, int nnz) {
@@ -1486,8 +1484,6 @@ SeamLayout seam_layout(const DevPlan& h, int nnz) {
   L.b = o; o += al(8 * m);
   L.x = o; o += al(8 * m);
   L.vals = o; o += al(8 * (size_t)vals_slots(h));
-  L.fval = o; o += al(8 * (size_t)fval_slots(h));
-  L.bval = o; o += al(8 * (size_t)bval_slots(h));
   L.dval = o; o += al(8 * (size_t)dval_slots(h));
   L.xs = o; o += al(8 * m);
   L.rf = o; o += al(8 * m);
@@ -1505,7 +1501,6 @@ int seam_run(const DevPlan& h, char* base, const SeamLayout& L, int nnz, const d
   const size_t m = (size_t)h.m;
   SeamDesc d{reinterpret_cast<double*>(base + L.a),    reinterpret_cast<double*>(base + L.b),
              reinterpret_cast<double*>(base + L.x),    reinterpret_cast<double*>(base + L.vals),
-             reinterpret_cast<double*>(base + L.fval), reinterpret_cast<double*>(base + L.bval),
              reinterpret_cast<double*>(base + L.dval), reinterpret_cast<double*>(base + L.xs),
              reinterpret_cast<double*>(base + L.rf),   reinterpret_cast<int32_t*>(base + L.status)};
   const int32_t refused = 1;
@@ -1537,7 +1532,6 @@ void seam_run_many(const DevPlan& h, char* base, const SeamLayout& L, int nnz, i
     char* p = base + L.total * (size_t)i;
     descs[i] = SeamDesc{reinterpret_cast<double*>(p + L.a),    reinterpret_cast<double*>(p + L.b),
                         reinterpret_cast<double*>(p + L.x),    reinterpret_cast<double*>(p + L.vals),
-                        reinterpret_cast<double*>(p + L.fval), reinterpret_cast<double*>(p + L.bval),
                         reinterpret_cast<double*>(p + L.dval), reinterpret_cast<double*>(p + L.xs),
                         reinterpret_cast<double*>(p + L.rf),   reinterpret_cast<int32_t*>(status_base) + i};
     check(hipMemcpyAsync(p + L.a, val[i], 8 * (size_t)nnz, hipMemcpyHostToDevice, s), "seam a");

@@ -250,7 +250,7 @@ namespace gs {
 
 // Byte offsets of one instance's arrays inside its block of HBM (stride bytes per instance).
 struct Layout {
-  size_t vm, va, p, q, f, dx, r, jv, tpq, vals, fval, bval, dval, xg, spec, start, stride;
+  size_t vm, va, p, q, f, dx, r, jv, tpq, vals, dval, xg, spec, start, stride;
 };
 
 // Arithmetic and LDS placement of a launch.

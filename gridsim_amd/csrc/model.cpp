@@ -755,6 +755,27 @@ void compile_plan(const Plan& plan, DevicePlanHost& d, int scratch_cap) {
   }
   d.n_chunks = (int)d.ch_fence.size();
   d.n_rows = d.ch_row_ptr.back();
+  // Factor positions: chunk c's w-th WRITE row is the stream's (W_c + w)-th, and its lane g * TL + t % TL stores slot
+  // t = w * TL + t % TL of track g, at 64 * (W_c + w) + lane of the factor array.
+  d.fpos.assign(d.nnz_l, -1);
+  d.bpos.assign(d.nnz_u, -1);
+  d.n_wrows = 0;
+  for (int c = 0; c < d.n_chunks; c++) {
+    const int t0 = d.ch_track_ptr[c], ntr = d.ch_track_ptr[c + 1] - t0, TL = d.ch_tl[c];
+    int wrows = 0;
+    for (int g = 0; g < ntr; g++) {
+      const int k = d.tr_col[t0 + g], u0 = plan.up[k], nu = plan.up[k + 1] - u0, l0 = plan.lp[k];
+      const int nslot = nu + plan.lp[k + 1] - l0 - 1;
+      wrows = std::max(wrows, (nslot + TL - 1) / TL);
+      for (int t = 0; t < nslot; t++) {
+        const int pos = (d.n_wrows + t / TL) * 64 + g * TL + t % TL;
+        if (t < nu - 1) d.bpos[u0 + t] = pos;
+        else if (t > nu - 1) d.fpos[l0 + (t - (nu - 1))] = pos;
+      }
+    }
+    d.n_wrows += wrows;
+  }
+  d.n_fac = 64 * d.n_wrows;
   d.j2a.assign(plan.a_src.empty() ? 0 : (size_t)(*std::max_element(plan.a_src.begin(), plan.a_src.end()) + 1), -1);
   for (size_t e = 0; e < d.ch_a_src.size(); e++) d.j2a[d.ch_a_src[e]] = (int)e;
   // Forward solve rows: L(i, j) for j < i, columns ascending; level(i) = 1 + max level(j).
@@ -811,16 +832,13 @@ void compile_plan(const Plan& plan, DevicePlanHost& d, int scratch_cap) {
   }
   // Solve step streams: per level, rows by decreasing entry count, 64 rows per pass; per pass (segment) its PRODUCT
   // steps, the entries lane by lane (each lane's entries contiguous in the scratch area, in the reference's order),
-  // then one CHAIN step. Each entry's value gets the next position of its value array, step by step, so that a
-  // step's values are contiguous.
+  // then one CHAIN step. A PRODUCT lane's value index is its entry's factor position (pos, by CSC index).
   auto build_stream = [&](const vector<int>& order, const vector<int>& level_end, int nlv, const vector<int>& rptr,
                           const vector<int>& rcol, const vector<int>& ridx, bool back, vector<int>& rec,
-                          vector<int>& pos, int& n_val, int& steps, vector<int>& pass_step,
-                          vector<int>& level_pass) {
+                          const vector<int>& pos, int& steps, vector<int>& pass_step, vector<int>& level_pass) {
     using D = DevicePlanHost;
     const int cap = d.solve_scratch();
     int b0 = 0;
-    n_val = 0;
     steps = 0;
     pass_step.clear();
     level_pass.clear();
@@ -867,8 +885,8 @@ void compile_plan(const Plan& plan, DevicePlanHost& d, int scratch_cap) {
               int col = -1, idx = 0;
               if (e < fill) {
                 col = ent_col[e];
-                idx = n_val++;
-                pos[ent_idx[e]] = idx;
+                idx = pos[ent_idx[e]];
+                if (idx < 0) throw std::runtime_error("plan: a solve entry without a factor position");
               }
               rec.push_back(col); rec.push_back(idx | (fl << D::kRsMetaShift));
             }
@@ -904,15 +922,13 @@ void compile_plan(const Plan& plan, DevicePlanHost& d, int scratch_cap) {
     level_pass.push_back((int)pass_step.size());
     pass_step.push_back(steps);
   };
-  d.fpos.assign(d.nnz_l, -1);
-  d.bpos.assign(d.nnz_u, -1);
   build_stream(d.fw_order, d.fw_level_end, d.fw_levels, d.lr_ptr, d.lr_col, d.lr_idx, false, d.fs_rec, d.fpos,
-               d.n_fval, d.fs_steps, d.fs_pass_step, d.fs_level_pass);
+               d.fs_steps, d.fs_pass_step, d.fs_level_pass);
   build_stream(d.bw_order, d.bw_level_end, d.bw_levels, d.ur_ptr, d.ur_col, d.ur_idx, true, d.bs_rec, d.bpos,
-               d.n_bval, d.bs_steps, d.bs_pass_step, d.bs_level_pass);
+               d.bs_steps, d.bs_pass_step, d.bs_level_pass);
   // The refactor's unified row stream (see DevicePlanHost): per chunk its ASSIGN rows, its UPDATE rows (no-op rows
   // first where a row would read a level that ended fewer than kRsValueAhead rows before), a PIVOT row and its WRITE
-  // rows. The L values the updates read live in vals at n_a + CSC index.
+  // rows. The L values the updates read live in vals at n_a + their factor position.
   {
     using D = DevicePlanHost;
     d.n_a = (int)d.ch_a_src.size();
@@ -925,9 +941,8 @@ void compile_plan(const Plan& plan, DevicePlanHost& d, int scratch_cap) {
       d.rs_level.push_back(lv);
       d.rs_rec.insert(d.rs_rec.end(), rec2, rec2 + 2 * 64);
     };
-    if ((int64_t)d.n_a + d.nnz_l > D::kRsIndexMask) throw std::runtime_error("plan: too many values for the row stream");
-    if (d.n_fval > D::kRsIndexMask || d.n_bval > D::kRsIndexMask || m > D::kRsIndexMask)
-      throw std::runtime_error("plan: factor positions too large for the row stream");
+    if ((int64_t)d.n_a + d.n_fac > D::kRsIndexMask || m > D::kRsIndexMask)
+      throw std::runtime_error("plan: too many values for the row stream");
     vector<int> rec(2 * 64);
     auto nop = [&](int lv) {
       std::fill(rec.begin(), rec.end(), 0);
@@ -935,6 +950,7 @@ void compile_plan(const Plan& plan, DevicePlanHost& d, int scratch_cap) {
     };
     d.rs_chunk_row.clear();
     d.rs_level_chunk.assign(1, 0);
+    int n_wr = 0;   // WRITE rows emitted so far
     for (int c = 0; c < d.n_chunks; c++) {
       const int t0 = d.ch_track_ptr[c], ntr = d.ch_track_ptr[c + 1] - t0;
       const int lv = level[d.tr_col[t0]];
@@ -971,7 +987,7 @@ void compile_plan(const Plan& plan, DevicePlanHost& d, int scratch_cap) {
           const int rr = d.row_rec[(size_t)r * 64 + l];
           const bool in = (rr & 0xFFFF) != 0xFFFF;
           rec[2 * l] = in ? rr : d.spare(l) | (d.spare(l) << 16);
-          rec[2 * l + 1] = in ? d.n_a + d.row_lsrc[(size_t)r * 64 + l] : 0;
+          rec[2 * l + 1] = in ? d.n_a + d.fpos[d.row_lsrc[(size_t)r * 64 + l]] : 0;
         }
         emit(D::kRsUpdate | width, lv, rec.data());
       }
@@ -990,36 +1006,29 @@ void compile_plan(const Plan& plan, DevicePlanHost& d, int scratch_cap) {
         }
       }
       emit(D::kRsPivot | width, lv, rec.data());
-      // WRITE rows: (fval position | kind << 24, bval / dval / vals position); write row w of a chunk writes slot
-      // t = w * TL + lane % TL of the lane's track (TL lanes per track)
+      // WRITE rows: (kind << 24 | pivot step for the pivot's lane, vals position); write row w of a chunk writes slot
+      // t = w * TL + lane % TL of the lane's track (TL lanes per track) to the stream's next 64 factor positions, an
+      // idle lane included (its position is its own and never read)
       for (int w = 0; w < wrows; w++) {
         for (int l = 0; l < 64; l++) {
           const int g = l / TL, t = w * TL + l % TL;
           rec[2 * l] = 0;
-          rec[2 * l + 1] = 0;
+          rec[2 * l + 1] = d.n_a + 64 * n_wr + l;
           if (g >= ntr) continue;
-          const int k = d.tr_col[t0 + g], u0 = plan.up[k], nu = plan.up[k + 1] - u0, l0 = plan.lp[k];
-          const int nl = plan.lp[k + 1] - l0;
+          const int k = d.tr_col[t0 + g], nu = plan.up[k + 1] - plan.up[k];
+          const int nl = plan.lp[k + 1] - plan.lp[k];
           if (t >= nu + nl - 1) continue;
-          if (t < nu - 1) {
-            rec[2 * l] = D::kWriteU << 24;
-            rec[2 * l + 1] = d.bpos[u0 + t];
-          } else if (t == nu - 1) {
-            rec[2 * l] = D::kWritePivot << 24;
-            rec[2 * l + 1] = k;
-          } else {
-            const int p = l0 + (t - (nu - 1));
-            rec[2 * l] = d.fpos[p] | (D::kWriteL << 24);
-            rec[2 * l + 1] = d.n_a + p;
-          }
+          rec[2 * l] = t < nu - 1 ? D::kWriteU << 24 : t == nu - 1 ? (D::kWritePivot << 24) | k : D::kWriteL << 24;
         }
         emit(D::kRsWrite | width, lv, rec.data());
+        n_wr++;
       }
       if (d.ch_fence[c]) {
         d.rs_meta.back() |= D::kRsLevelEnd;
         level_end_row[lv] = (int)d.rs_meta.size() - 1;
       }
     }
+    if (n_wr != d.n_wrows) throw std::runtime_error("plan: WRITE rows differ from the factor layout");
     d.n_rs = (int)d.rs_meta.size();
     d.rs_chunk_row.push_back(d.n_rs);
     while ((int)d.rs_level_chunk.size() <= n_levels) d.rs_level_chunk.push_back(d.n_chunks);
@@ -1052,9 +1061,8 @@ bool check_level_schedule(const DevicePlanHost& d, int ring, int padded_rows, in
   if ((int)d.rs_chunk_row.size() != nc + 1 || (int)d.rs_level_chunk.size() != nl + 1) return bad("chunk tables size");
   if (d.rs_chunk_row[0] != 0 || d.rs_chunk_row[nc] != d.n_rs) return bad("chunk rows do not cover the stream");
   if (d.rs_level_chunk[0] != 0 || d.rs_level_chunk[nl] != nc) return bad("level chunks do not cover the chunks");
-  const int n_vals = d.n_a + d.nnz_l;
-  vector<int> val_level(n_vals, -1), bpos_chunk(std::max(1, d.n_bval), -1), dpos_chunk(d.m, -1);
-  vector<int> fpos_chunk(std::max(1, d.n_fval), -1);
+  const int n_vals = d.n_a + d.n_fac;
+  vector<int> val_level(n_vals, -1), val_owner(n_vals, -1), dpos_chunk(d.m, -1);
   for (int l = 0; l < nl; l++) {
     if (d.rs_level_chunk[l + 1] <= d.rs_level_chunk[l]) return bad("a level without chunks");
     // every read of the level first (the level's own writes must not be read by it)
@@ -1086,17 +1094,16 @@ bool check_level_schedule(const DevicePlanHost& d, int ring, int padded_rows, in
         for (int lane = 0; lane < 64; lane++) {
           const int* rr = &d.rs_rec[((size_t)r * 64 + lane) * 2];
           const int wk = (int)((unsigned)rr[0] >> 24), pos = rr[1] & D::kRsIndexMask;
-          if (wk == 0) continue;
-          int* owner = nullptr;
-          if (wk == D::kWriteU) owner = &bpos_chunk.at(pos);
-          else if (wk == D::kWritePivot) owner = &dpos_chunk.at(pos);
-          else {
-            if (val_level.at(pos) >= 0) return bad("an L value written twice");
-            val_level[pos] = l;
-            owner = &fpos_chunk.at(rr[0] & D::kRsIndexMask);
+          // every lane stores (an idle one too, at a position of its own)
+          if (pos < d.n_a || pos >= n_vals) return bad("a factor position outside the factor array");
+          if (val_owner[pos] >= 0) return bad("a factor position written twice");
+          val_owner[pos] = c;
+          if (wk == D::kWriteL) val_level[pos] = l;
+          if (wk == D::kWritePivot) {
+            int& owner = dpos_chunk.at(rr[0] & D::kRsIndexMask);
+            if (owner >= 0) return bad("a pivot written twice");
+            owner = c;
           }
-          if (*owner >= 0 && *owner != c) return bad("two chunks write one factor position");
-          *owner = c;
         }
       }
     }
@@ -1144,12 +1151,11 @@ bool emulate_device_lu(const DevicePlanHost& d, int nnz_j, const double* jv, con
                        std::string& why) {
   const int m = d.m, G = DevicePlanHost::kTracks;
   const int wsz = G * d.wstride;
-  vector<double> ws(wsz), fval(std::max(1, d.n_fval), std::nan("")), bval(std::max(1, d.n_bval), std::nan(""));
-  vector<double> dval(m, std::nan(""));
+  vector<double> ws(wsz), dval(m, std::nan(""));
   auto bad = [&](const std::string& w) { why = w; return false; };
   using D = DevicePlanHost;
-  // vals = [A values in record order, as the mismatch kernel writes them through j2a | L in column order]
-  const int n_a = d.n_a, n_vals = n_a + d.nnz_l;
+  // vals = [A values in record order, as the mismatch kernel writes them through j2a | the factors, 64 per WRITE row]
+  const int n_a = d.n_a, n_vals = n_a + d.n_fac;
   vector<double> vals(n_vals, std::nan(""));
   for (int s = 0; s < nnz_j && s < (int)d.j2a.size(); s++) {
     int e = d.j2a[s];
@@ -1208,8 +1214,8 @@ bool emulate_device_lu(const DevicePlanHost& d, int nnz_j, const double* jv, con
       if ((int)((unsigned)rr[1] >> D::kRsMetaShift) != meta) return bad("row kind differs across lanes");
       const int rc[2] = {rr[0], rr[1] & D::kRsIndexMask};
       if ((kind == D::kRsAssign || kind == D::kRsUpdate) && rc[1] >= n_vals) return bad("row value index");
-      // the device loads vals[field] for every row (rows without a value too): vals holds max(n_vals, m, n_bval)
-      if (rc[1] >= std::max({n_vals, m, d.n_bval})) return bad("row field outside the value array");
+      // the device loads vals[field] for every row (rows without a value too): vals holds max(n_vals, m)
+      if (rc[1] >= std::max(n_vals, m)) return bad("row field outside the value array");
       const int load_row = r - D::kRsValueAhead;  // the value is loaded at the end of this row
       if (kind == D::kRsAssign) {
         if (rc[0] == d.spare(lane)) continue;
@@ -1234,19 +1240,23 @@ bool emulate_device_lu(const DevicePlanHost& d, int nnz_j, const double* jv, con
       } else if (kind == D::kRsWrite) {
         const int wk = (int)((unsigned)rc[0] >> 24);
         const int slot = (lane / TL) * d.wstride + write_row * TL + lane % TL;
-        if (wk == 0) continue;
+        // every lane reads its slot and stores to its position (an idle lane's store is never read: NaN here)
+        if (slot >= wsz + D::kSpareSlots) return bad("write slot outside the workspace");
+        if (rc[1] < n_a || rc[1] >= n_vals) return bad("write position");
+        if (wk == 0) {
+          vals[rc[1]] = std::nan("");
+          continue;
+        }
         if (slot >= wsz || slot / d.wstride != lane / TL) return bad("write slot");
         if (wk == D::kWriteU) {
-          if (rc[1] >= d.n_bval) return bad("U write position");
-          bval[rc[1]] = ws[slot];
+          vals[rc[1]] = ws[slot];
         } else if (wk == D::kWritePivot) {
-          if (rc[1] >= m) return bad("pivot write position");
-          dval[rc[1]] = ws[slot];
+          const int k = rc[0] & D::kRsIndexMask;
+          if (k >= m) return bad("pivot write position");
+          vals[rc[1]] = dval[k] = ws[slot];
         } else if (wk == D::kWriteL) {
-          const int fp = rc[0] & D::kRsIndexMask;
-          if (rc[1] < n_a || rc[1] >= n_vals || fp >= d.n_fval) return bad("L write position");
           if (piv[lane] == 0.0) return bad("L write without a pivot");
-          vals[rc[1]] = fval[fp] = ws[slot] / piv[lane];
+          vals[rc[1]] = ws[slot] / piv[lane];
           written_row[rc[1]] = r;
           pending.push_back(rc[1]);
         } else {
@@ -1284,10 +1294,9 @@ bool emulate_device_lu(const DevicePlanHost& d, int nnz_j, const double* jv, con
         if ((meta & D::kStepKindMask) != kind) { why = "stream step kind differs across lanes"; return false; }
         if (kind == D::kStepProduct && meta != (int)((unsigned)rec[((size_t)t * 64) * 2 + 1] >> D::kRsMetaShift))
           { why = "product flags differ across lanes"; return false; }
-        // the device loads the value array and (back) the pivot array at every step's field: they hold
-        // max(values, m) and max(n_bval, m) entries
-        if (field >= std::max({(int)vals.size(), m, 1}) || (back && field >= std::max({d.n_bval, m, 1})))
-          { why = "step field outside the value or pivot array"; return false; }
+        // the device loads the value array at every step's field (it holds max(values, m)); the pivot array at a
+        // CHAIN step's row only
+        if (field >= std::max({(int)vals.size(), m, 1})) { why = "step field outside the value array"; return false; }
         if (kind == D::kStepProduct) {
           if (sbase + 64 > cap) { why = "products pass the scratch area"; return false; }
           if (r2[0] < 0) continue;
@@ -1340,9 +1349,10 @@ bool emulate_device_lu(const DevicePlanHost& d, int nnz_j, const double* jv, con
     // forward: rows without L entries are final from the start; back: a row is final once divided
     vector<char> fin(m, 0);
     for (int i = 0; i < m; i++) fin[i] = d.lr_ptr[i + 1] == d.lr_ptr[i];
-    if (!run_stream(d.fs_rec, d.fs_steps, fval, false, fin)) return false;
+    const vector<double> fac(vals.begin() + n_a, vals.end());
+    if (!run_stream(d.fs_rec, d.fs_steps, fac, false, fin)) return false;
     vector<char> fin_b(m, 0);
-    if (!run_stream(d.bs_rec, d.bs_steps, bval, true, fin_b)) return false;
+    if (!run_stream(d.bs_rec, d.bs_steps, fac, true, fin_b)) return false;
     for (int k = 0; k < m; k++) out[d.q[k]] = x[k];
     return true;
   }

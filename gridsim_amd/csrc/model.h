@@ -161,11 +161,15 @@ struct DevicePlanHost {
   //   workspaces, so the device runs these rows with every lane active and no branch)
   //   PIVOT   x = pivot slot ps | (candidates nc << 16) (the candidates are slots ps + 1 .. ps + nc), y = pivot step;
   //           x = -1: idle track
-  //   WRITE   slot: track base + w * kTrackLanes + lane % kTrackLanes for the chunk's w-th WRITE row; kind x >> 24:
-  //           U (bval[y] = ws), PIVOT (dval[y] = ws), L (vals[y] = fval[x & kRsIndexMask] = ws / pivot), 0: idle
-  // vals = [A values (n_a) | L in column order (nnz_l)], so y and the L destinations index one array. kRsLevelEnd marks
-  // the last row of a dependency level (the device fences there). No-op rows are inserted so that an UPDATE row stands
-  // at least kRsValueAhead rows after the end of every level it reads: its value, loaded that far ahead, is then final.
+  //   WRITE   slot: track base + w * TL + lane % TL for the chunk's w-th WRITE row (TL lanes per track); every lane
+  //           stores its slot to vals[y] (y = n_a + 64 * the stream's WRITE-row count + lane, so one WRITE row is one
+  //           contiguous 64-value store): an L slot divided by its column's pivot, a U slot or the pivot as it is, an
+  //           idle lane whatever its slot holds (a position nothing reads); kind x >> 24: U, PIVOT (also dval[x &
+  //           kRsIndexMask] = ws, by pivot step), L, 0: idle
+  // vals = [A values (n_a) | the factors, 64 per WRITE row (n_fac)], so y indexes one array for every row kind.
+  // kRsLevelEnd marks the last row of a dependency level (the device fences there). No-op rows are inserted so that an
+  // UPDATE row stands at least kRsValueAhead rows after the end of every level it reads: its value, loaded that far
+  // ahead, is then final.
   static constexpr int kRsNop = 0, kRsAssign = 1, kRsUpdate = 2, kRsPivot = 3, kRsWrite = 4;
   static constexpr int kRsKindMask = 7, kRsClear = 8, kRsLevelEnd = 16;
   static constexpr int kWriteU = 1, kWritePivot = 2, kWriteL = 3;
@@ -192,10 +196,10 @@ struct DevicePlanHost {
   // A device whose solve vector sits in global memory loads x kStepXAhead steps ahead with the values: x[row] for a
   // CHAIN step (only the row's own last CHAIN step writes it), x[col] for a PRODUCT step unless kProductLate marks it
   // (a column of the step ends fewer than kStepXAhead steps before it in the stream): that step loads x at the step.
-  // The factors are stored in the order the PRODUCT steps read them: fval holds L's off-diagonal entries in
-  // forward-stream order, bval U's off-diagonal entries in back-stream order (the active lanes of a step being a
-  // prefix, a step's values are contiguous), dval U's diagonal by pivot step. fpos / bpos map an L / U entry (CSC
-  // index) to its position, -1 for a diagonal. The refactor keeps L in column order too (lx, CSC), for its own updates.
+  // The factors stay where the refactor's WRITE rows put them (the factor part of vals, 64 per WRITE row): a PRODUCT
+  // step's value index is its entry's position there (fpos / bpos map an L / U entry, CSC index, to it; -1 for a
+  // diagonal), so the solves gather the values they need while the refactor stores each WRITE row in one contiguous
+  // piece; dval holds U's diagonal by pivot step.
   static constexpr int kStepProduct = 1, kStepChain = 2, kStepKindMask = 15;
   static constexpr int kChainBegin = 16, kChainEnd = 32, kChainIdle = 64;
   static constexpr int kProductLate = 16, kStepXAhead = 8;
@@ -206,7 +210,8 @@ struct DevicePlanHost {
     const int full = (kTracks * wstride + kSpareSlots) / 64 * 64;
     return scratch_cap > 0 && scratch_cap < full ? scratch_cap / 64 * 64 : full;
   }
-  int fs_steps = 0, bs_steps = 0, n_fval = 0, n_bval = 0;
+  int fs_steps = 0, bs_steps = 0;
+  int n_wrows = 0, n_fac = 0;                       // WRITE rows of the refactor stream; factor slots (64 per row)
   vector<int> fs_rec, bs_rec;                       // 2 ints per lane per step
   // pass p of a stream covers steps [pass_step[p], pass_step[p + 1]); level l's passes are [level_pass[l], [l + 1])
   vector<int> fs_pass_step, fs_level_pass, bs_pass_step, bs_level_pass;

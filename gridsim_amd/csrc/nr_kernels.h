@@ -34,10 +34,10 @@ struct DevPlan {
   const int4* rs_rec;
   const int32_t* ch_a_src;            // per A entry: J CSR position (the seam fills vals from its matrix)
   // Triangular-solve step streams: per lane and step (x, y): PRODUCT (col, value index | kind << 24) and CHAIN
-  // (start | len << 16, row | (kind | flags) << 24) steps (model.h DevicePlanHost). The factors live in the order the
-  // PRODUCT steps read them (fval: L off-diagonal, forward order; bval: U off-diagonal, back order; dval: U's
-  // diagonal by pivot step).
-  int32_t fs_steps, bs_steps, n_fval, n_bval;
+  // (start | len << 16, row | (kind | flags) << 24) steps (model.h DevicePlanHost). The factors live where the
+  // refactor's WRITE rows stored them, 64 per WRITE row (the factor part of vals, n_fac slots after the n_a A values;
+  // a PRODUCT step's value index is a position there), the pivots also in dval by pivot step.
+  int32_t fs_steps, bs_steps, n_fac, pad_fac;
   const int2 *fs_rec, *bs_rec;
   // Level-parallel schedules: first row of each refactor chunk (+ the end), first step of each solve pass (+ the end).
   // The streams carry two kStreamPad rings of padding, so one chunk's or pass's range rounded up to whole rings can
@@ -79,14 +79,12 @@ constexpr int kWsSpare = 64;
 // slots of the refactor workspace (the triangular-solve vector follows it in LDS)
 __host__ __device__ inline int ws_slots(const DevPlan& p) { return kTracks * p.wstride + kWsSpare; }
 // Entries of each factor array per instance. Every stream record's 24-bit field indexes the array its value load
-// reads (vals for the refactor rows, fval / bval and dval for the solve steps), also in the rows and steps that take no
-// value (pivot steps, factor positions, solve rows), so the arrays are sized to hold every such index and the loads
-// need no select.
+// reads (vals for the refactor rows, its factor part for the solve steps), also in the rows and steps that take no
+// value (pivot steps, solve rows: below m, and the factor part holds at least one slot per column), so the loads need
+// no select; the pivots' array is read at a CHAIN step's row only.
 __host__ __device__ inline int imax3(int a, int b, int c) { return a > b ? (a > c ? a : c) : (b > c ? b : c); }
-__host__ __device__ inline int vals_slots(const DevPlan& p) { return imax3(p.n_a + p.nnz_l, p.m, p.n_bval); }
-__host__ __device__ inline int fval_slots(const DevPlan& p) { return imax3(p.n_fval, p.m, 1); }
-__host__ __device__ inline int bval_slots(const DevPlan& p) { return imax3(p.n_bval, p.m, 1); }
-__host__ __device__ inline int dval_slots(const DevPlan& p) { return imax3(p.n_bval, p.m, 1); }
+__host__ __device__ inline int vals_slots(const DevPlan& p) { return imax3(p.n_a + p.n_fac, p.m, 1); }
+__host__ __device__ inline int dval_slots(const DevPlan& p) { return imax3(p.m, 1, 1); }
 
 // Level-parallel schedule: a level of at most this many chunks (solve passes) is cheaper walked in sequence by one
 // work-group, together with its narrow neighbours, than given a kernel of its own (measured on the 70,000-bus grid).
@@ -154,8 +152,8 @@ struct InstDesc {
   double *vm, *va, *p, *q;
   double *f, *dx, *r, *jv;            // f, dx, residual, FP64 Jacobian values (CSR order)
   double* tpq;                        // per Y cell: the cell's two injection terms (P, Q)
-  void* vals;                         // LU precision: [Jacobian in the refactor's record order | L, column order]
-  void *fval, *bval, *dval;           // the factors in the LU precision (stream order)
+  void* vals;                         // LU precision: [Jacobian in the refactor's record order | the factors]
+  void* dval;                         // the pivots in the LU precision, by pivot step
   void* xg;                           // global triangular-solve workspace (used when LDS cannot hold x)
 };
 
@@ -164,7 +162,7 @@ struct SeamDesc {
   const double* a;         // CSR values
   const double* b;
   double* x;
-  double *vals, *fval, *bval, *dval, *xs, *rowfloor;
+  double *vals, *dval, *xs, *rowfloor;
   int32_t* status;         // 0 solved, 1 refused (a pivot would move, or is at or below its row floor)
 };
 

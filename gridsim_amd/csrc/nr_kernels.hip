@@ -124,7 +124,7 @@ struct Inst {
   GLOBAL const double *pspec, *qspec, *vm_start, *va_start;
   GLOBAL double *vm, *va, *p, *q, *f, *dx, *r, *jv;
   GLOBAL v2d* tpq;
-  GLOBAL T *vals, *fval, *bval, *dval, *xg;
+  GLOBAL T *vals, *fac, *dval, *xg;   // fac: the factor part of vals (vals + n_a)
   GLOBAL const Patch* patch;
   GLOBAL const v2d* ygb;   // the Y values (g, b) per cell: the network's, or the instance's own
   int n_patch;
@@ -323,9 +323,10 @@ __device__ __forceinline__ double mismatch_jacobian(const DevPlan& P, const Inst
 // it is loaded and nothing is ever loaded twice. The ring registers are only rewritten at fixed points of the unrolled
 // loop, so the compiler's wait counts stay partial and the loads stay in flight. The stream is padded to whole rings.
 //
-// The factors are written where the triangular solves read them: L's off-diagonal entries to fval, U's off-diagonal
-// entries to bval, the pivots to dval; L also stays in column order in vals, for the later columns' updates (the 8
-// lanes of a track then read 8 consecutive values of one column, a single memory segment).
+// Each WRITE row stores its 64 lanes' slots to 64 consecutive factor positions of vals (L divided by the pivot, U and
+// the pivot as they are; an idle lane to a position nothing reads), one contiguous store; the later columns' updates
+// and the triangular solves read the factors there (a track's 8 lanes read 8 consecutive slots of one column, one or
+// two memory segments), and the pivots go to dval by pivot step as well.
 constexpr int kRing = kStreamPad, kHalf = kRing / 2, kAhead = kRsValueAhead;
 static_assert(kAhead < kRing, "the value of a row is loaded after its record");
 
@@ -333,8 +334,8 @@ static_assert(kAhead < kRing, "the value of a row is loaded after its record");
 // whole rings from row0's pair and, with CHUNK, treats the rows outside [row0, row1) as no-ops (the whole stream's
 // rounding rows are no-ops already).
 template <typename T, bool EXACT, bool CHUNK = false>
-__device__ __forceinline__ int refactor(const DevPlan& P, GLOBAL T* vals, GLOBAL T* fval, GLOBAL T* bval,
-                                        GLOBAL T* dval, LDSP T* ws, double growth_tol, double abs_floor,
+__device__ __forceinline__ int refactor(const DevPlan& P, GLOBAL T* vals, GLOBAL T* dval, LDSP T* ws,
+                                        double growth_tol, double abs_floor,
                                         GLOBAL const double* rowfloor, int lane, double* min_ratio, int row0,
                                         int row1) {
   // rows go in pairs (one 16-byte record per lane for two rows): start at the even row of row0's pair, the rows
@@ -437,17 +438,13 @@ __device__ __forceinline__ int refactor(const DevPlan& P, GLOBAL T* vals, GLOBAL
         if (fmask && failed < 0) failed = __shfl(fail, __ffsll((long long)fmask) - 1, kWave);
         lds_drained();
       } else if (kind == kRsWrite) {
-        // every lane reads its slot (an idle lane a slot inside the workspace) and forms the L quotient; the stores
-        // are masked by kind
+        // every lane reads its slot (an idle lane a slot inside the workspace), forms the L quotient and stores to
+        // its own factor position: the row's 64 stores are one contiguous piece
         const int wk = (int)((unsigned)r4.x >> 24), pos = r4.y & kRsIndexMask;
         const T v = ws[(lane >> wsh) * P.wstride + write_row * TLr + (lane & (TLr - 1))];
         const T lv = v / piv;
-        if (wk == kWriteU) bval[pos] = v;
-        if (wk == kWritePivot) dval[pos] = v;
-        if (wk == kWriteL) {
-          vals[pos] = lv;
-          fval[r4.x & kRsIndexMask] = lv;
-        }
+        vals[pos] = wk == kWriteL ? lv : v;
+        if (wk == kWritePivot) dval[r4.x & kRsIndexMask] = v;
         write_row++;
         lds_drained();
       }
@@ -476,8 +473,9 @@ __device__ __forceinline__ int refactor(const DevPlan& P, GLOBAL T* vals, GLOBAL
 // every entry is the reference's value, and a row costs one CHAIN iteration per entry (LDS reads and one subtraction)
 // instead of one step. Passes larger than the scratch area run in segments; a lane's running value stays in its
 // register across them. The step records are loaded kRing steps ahead and the factor values kHalf steps ahead (the
-// factors are final before a solve starts; the streams are padded by kRing steps); a PRODUCT step's values are
-// contiguous (its active lanes are a prefix), so each value load is one coalesced segment. A solve vector in LDS is
+// factors are final before a solve starts; the streams are padded by kRing steps); a PRODUCT step gathers its values
+// from where the refactor stored them (a WRITE row's stores are contiguous, the solves' loads scattered: a store
+// costs the vector-memory path more than a load). A solve vector in LDS is
 // read at the step itself; one in global memory is loaded kHalf steps ahead with the values (x[row] for a CHAIN step,
 // x[col] for a PRODUCT step), except at PRODUCT steps the plan marks kProductLate, whose columns end too shortly
 // before them. XP: the solve vector (LDS or global); scr: the scratch area (LDS, the wave's refactor workspace).
@@ -504,17 +502,14 @@ __device__ __forceinline__ void solve_stream(GLOBAL const v2i* rec, int step0, i
   T rv[kRing], rd[kRing];
   // the kind of a step (every lane holds it), and the value index / pivot row a lane's record names for it
   // a PRODUCT step's value index, a CHAIN step's row (its pivot); in a batch run the other steps load entry 0, one
-  // line for every lane, rather than 64 scattered ones; a level-parallel pass loads at the field unselected (every
-  // field indexes inside the value and pivot arrays: fval_slots, bval_slots, dval_slots)
+  // line for every lane, rather than 64 scattered ones; a level-parallel pass loads values at the field unselected
+  // (every field indexes inside the factors: vals_slots) and the pivot at a CHAIN step's row only
   auto kind_of = [](const v2i& r) { return (int)((unsigned)r.y >> kRsMetaShift) & kStepKindMask; };
   auto value_index = [&](const v2i& r) {
     if constexpr (WIDE) return r.y & kRsIndexMask;
     return kind_of(r) == kStepProduct ? (r.y & kRsIndexMask) : 0;
   };
-  auto pivot_index = [&](const v2i& r) {
-    if constexpr (WIDE) return r.y & kRsIndexMask;
-    return kind_of(r) == kStepChain ? (r.y & kRsIndexMask) : 0;
-  };
+  auto pivot_index = [&](const v2i& r) { return kind_of(r) == kStepChain ? (r.y & kRsIndexMask) : 0; };
   // the entry of x a step reads: a CHAIN step's row, a PRODUCT step's column (an idle lane's -1, and the padding's,
   // read as 0)
   auto x_index = [&](const v2i& r) { return kind_of(r) == kStepChain ? (r.y & kRsIndexMask) : max(r.x, 0); };
@@ -587,16 +582,16 @@ __device__ __forceinline__ void solve_stream(GLOBAL const v2i* rec, int step0, i
 }
 
 template <typename T, bool XLDS, typename XP>
-__device__ __forceinline__ void lu_solve(const DevPlan& P, GLOBAL const T* fval, GLOBAL const T* bval,
-                                         GLOBAL const T* dval, GLOBAL const double* rhs, GLOBAL double* out, XP x,
+__device__ __forceinline__ void lu_solve(const DevPlan& P, GLOBAL const T* fac, GLOBAL const T* dval,
+                                         GLOBAL const double* rhs, GLOBAL double* out, XP x,
                                          LDSP T* scr, int lane) {
   const int m = P.m;
   GLOBAL const int* pinv = glob(P.pinv);
   GLOBAL const int* q = glob(P.q);
   for (int r = lane; r < m; r += kWave) x[pinv[r]] = (T)rhs[r];
   if (XLDS) WAVE_SYNC(); else WG_SYNC();
-  solve_stream<T, false, XLDS>(globv<v2i>(P.fs_rec), 0, P.fs_steps, fval, dval, x, scr, lane);
-  solve_stream<T, true, XLDS>(globv<v2i>(P.bs_rec), 0, P.bs_steps, bval, dval, x, scr, lane);
+  solve_stream<T, false, XLDS>(globv<v2i>(P.fs_rec), 0, P.fs_steps, fac, dval, x, scr, lane);
+  solve_stream<T, true, XLDS>(globv<v2i>(P.bs_rec), 0, P.bs_steps, fac, dval, x, scr, lane);
   for (int k = lane; k < m; k += kWave) out[q[k]] = (double)x[k];
   WG_SYNC();
 }
@@ -682,7 +677,7 @@ __global__ __launch_bounds__(kWave * (PAIR ? kPairWaves : 1), 2) void nr_batch_k
   I.f = glob(d.f); I.dx = glob(d.dx); I.r = glob(d.r); I.jv = glob(d.jv);
   I.tpq = (GLOBAL v2d*)d.tpq;
   I.vals = glob(reinterpret_cast<T*>(d.vals));
-  I.fval = glob(reinterpret_cast<T*>(d.fval)); I.bval = glob(reinterpret_cast<T*>(d.bval));
+  I.fac = I.vals + P.n_a;
   I.dval = glob(reinterpret_cast<T*>(d.dval));
   I.xg = glob(reinterpret_cast<T*>(d.xg));
   I.patch = glob(&descs[inst].patch[0]);
@@ -717,12 +712,12 @@ __global__ __launch_bounds__(kWave * (PAIR ? kPairWaves : 1), 2) void nr_batch_k
   auto solve = [&](GLOBAL const double* rhs, GLOBAL double* out) {
     if constexpr (PAIR) {
       if (!take_lock(x_lock, lane)) { lock_timeout = 1; return; }
-      lu_solve<T, true>(P, I.fval, I.bval, I.dval, rhs, out, xl, ws, opaque_lane(lane));
+      lu_solve<T, true>(P, I.fac, I.dval, rhs, out, xl, ws, opaque_lane(lane));
       give_lock(x_lock, lane);
     } else if constexpr (XLDS) {
-      lu_solve<T, true>(P, I.fval, I.bval, I.dval, rhs, out, xl, ws, opaque_lane(lane));
+      lu_solve<T, true>(P, I.fac, I.dval, rhs, out, xl, ws, opaque_lane(lane));
     } else {
-      lu_solve<T, false>(P, I.fval, I.bval, I.dval, rhs, out, I.xg, ws, opaque_lane(lane));
+      lu_solve<T, false>(P, I.fac, I.dval, rhs, out, I.xg, ws, opaque_lane(lane));
     }
   };
   for (;;) {
@@ -740,7 +735,7 @@ __global__ __launch_bounds__(kWave * (PAIR ? kPairWaves : 1), 2) void nr_batch_k
     if (bad || !isfinite(max_mis)) break;  // NewtonShell.SolveStep refuses a non-finite J or f
     // (a plan the host made at this very state is the factorisation's own: its first step is not checked)
     const bool trust = prm.trust_first && steps == 0;
-    fail_col = refactor<T, false>(P, I.vals, I.fval, I.bval, I.dval, ws, trust ? 0.0 : prm.growth_tol,
+    fail_col = refactor<T, false>(P, I.vals, I.dval, ws, trust ? 0.0 : prm.growth_tol,
                                   trust ? 0.0 : pivot_floor_rel<T>() * jnorm, nullptr, opaque_lane(lane), &min_ratio, 0,
                                   P.n_rs);
     tick(t_fac);
@@ -825,7 +820,7 @@ __device__ __forceinline__ Inst<T> load_inst(const InstDesc* descs, const DevPla
   I.f = glob(d.f); I.dx = glob(d.dx); I.r = glob(d.r); I.jv = glob(d.jv);
   I.tpq = (GLOBAL v2d*)d.tpq;
   I.vals = glob(reinterpret_cast<T*>(d.vals));
-  I.fval = glob(reinterpret_cast<T*>(d.fval)); I.bval = glob(reinterpret_cast<T*>(d.bval));
+  I.fac = I.vals + P.n_a;
   I.dval = glob(reinterpret_cast<T*>(d.dval));
   I.xg = glob(reinterpret_cast<T*>(d.xg));
   I.patch = glob(&descs[b].patch[0]);
@@ -961,7 +956,7 @@ __global__ __launch_bounds__(kWave) void wide_refactor(const InstDesc* __restric
   const int c = first_chunk + blockIdx.x * per;
   const int row0 = P.rs_chunk_row[c], row1 = P.rs_chunk_row[c + per];
   double ratio = 1.0;
-  const int fail = refactor<T, false, true>(P, I.vals, I.fval, I.bval, I.dval, lds(reinterpret_cast<T*>(smem)),
+  const int fail = refactor<T, false, true>(P, I.vals, I.dval, lds(reinterpret_cast<T*>(smem)),
                                             prm.growth_tol, pivot_floor_rel<T>() * st->jnorm, nullptr,
                                             opaque_lane(threadIdx.x), &ratio, row0, row1);
   if (threadIdx.x == 0) {
@@ -998,7 +993,7 @@ __global__ __launch_bounds__(kWave) void wide_solve(const InstDesc* __restrict__
   GLOBAL T* x = glob(reinterpret_cast<T*>(d.xg));
   extern __shared__ __align__(16) unsigned char smem[];
   solve_stream<T, BACK, false, GLOBAL T*, true>(globv<v2i>(BACK ? P.bs_rec : P.fs_rec), pass_step[p], pass_step[p + per],
-                               glob(reinterpret_cast<const T*>(BACK ? d.bval : d.fval)),
+                               glob(reinterpret_cast<const T*>(d.vals)) + P.n_a,
                                glob(reinterpret_cast<const T*>(d.dval)), x, lds(reinterpret_cast<T*>(smem)),
                                opaque_lane(threadIdx.x));
 }
@@ -1151,7 +1146,7 @@ __device__ __forceinline__ int group_refactor(const DevPlan& P, const Inst<T>& I
     int c0, c1;
     group_share(seg[k], wave, W, &c0, &c1);
     if (c1 > c0) {
-      const int f = refactor<T, false, true>(P, I.vals, I.fval, I.bval, I.dval, ws, growth_tol, abs_floor, nullptr,
+      const int f = refactor<T, false, true>(P, I.vals, I.dval, ws, growth_tol, abs_floor, nullptr,
                                              lane, min_ratio, chunk_row[c0], chunk_row[c1]);
       if (f >= 0 && lane == 0) atomicMin(&g.fail[k % 3], (wave << 24) | f);
     }
@@ -1176,7 +1171,7 @@ __device__ __forceinline__ void group_solve(const DevPlan& P, const Inst<T>& I, 
     int p0, p1;
     group_share(seg[P.n_gseg_rs + k], wave, W, &p0, &p1);
     if (p1 > p0)
-      solve_stream<T, false, XLDS>(globv<v2i>(P.fs_rec), P.fs_pass_step[p0], P.fs_pass_step[p1], I.fval, I.dval, x,
+      solve_stream<T, false, XLDS>(globv<v2i>(P.fs_rec), P.fs_pass_step[p0], P.fs_pass_step[p1], I.fac, I.dval, x,
                                    scr, lane);
     __syncthreads();
   }
@@ -1184,7 +1179,7 @@ __device__ __forceinline__ void group_solve(const DevPlan& P, const Inst<T>& I, 
     int p0, p1;
     group_share(seg[P.n_gseg_rs + P.n_gseg_fs + k], wave, W, &p0, &p1);
     if (p1 > p0)
-      solve_stream<T, true, XLDS>(globv<v2i>(P.bs_rec), P.bs_pass_step[p0], P.bs_pass_step[p1], I.bval, I.dval, x,
+      solve_stream<T, true, XLDS>(globv<v2i>(P.bs_rec), P.bs_pass_step[p0], P.bs_pass_step[p1], I.fac, I.dval, x,
                                   scr, lane);
     __syncthreads();
   }
@@ -1631,13 +1626,12 @@ __global__ __launch_bounds__(kWave) void seam_solve_kernel(const SeamDesc* __res
   WG_SYNC();
   double ratio = 1.0;
   const int fail =
-      refactor<double, true>(P, vals, glob(d.fval), glob(d.bval), glob(d.dval), ws, 0.0, 0.0, rowfloor, lane, &ratio, 0,
-                             P.n_rs);
+      refactor<double, true>(P, vals, glob(d.dval), ws, 0.0, 0.0, rowfloor, lane, &ratio, 0, P.n_rs);
   if (fail >= 0) {
     if (lane == 0) *d.status = 1;
     return;
   }
-  lu_solve<double, false>(P, glob(d.fval), glob(d.bval), glob(d.dval), glob(d.b), glob(d.x), glob(d.xs), ws, lane);
+  lu_solve<double, false>(P, vals + P.n_a, glob(d.dval), glob(d.b), glob(d.x), glob(d.xs), ws, lane);
   if (lane == 0) *d.status = 0;
 }
 

@@ -98,7 +98,7 @@ class GsCsr(C.Structure):
 class GsPlanStats(C.Structure):
     _fields_ = [("m", C.c_int32), ("nnz_j", C.c_int32), ("nnz_l", C.c_int32), ("nnz_u", C.c_int32),
                 ("levels", C.c_int32), ("refactor_rows", C.c_int32), ("forward_steps", C.c_int32),
-                ("back_steps", C.c_int32), ("workspace_stride", C.c_int32), ("reserved", C.c_int32),
+                ("back_steps", C.c_int32), ("workspace_stride", C.c_int32), ("factor_slots", C.c_int32),
                 ("stream_bytes", C.c_int64), ("rows_by_kind", C.c_int32 * 5), ("update_lanes", C.c_int32),
                 ("forward_lanes", C.c_int32), ("back_lanes", C.c_int32)]
 

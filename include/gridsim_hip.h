@@ -282,7 +282,7 @@ typedef struct gs_plan_stats {
   int32_t refactor_rows;        /* rows of the refactor stream (64 lanes each, padding included) */
   int32_t forward_steps, back_steps;
   int32_t workspace_stride;     /* refactor workspace slots per track */
-  int32_t reserved;
+  int32_t factor_slots;         /* per-instance factor values: 64 per WRITE row of the refactor stream */
   int64_t stream_bytes;
   /* refactor rows by kind (no-op, assign, update, pivot, write; padding not counted) and the lanes of the update
    * rows that carry an update (of 64 per row) */
