@@ -105,12 +105,20 @@ WideSplit wide_split_of(const DevicePlanHost& d) {
   return sp;
 }
 
+// A Newton plan: the Jacobian values at the positions the mismatch pass writes them (Y-cell order).
+void compile_newton_plan(const Plan& plan, DevicePlanHost& d, const YBus& y, const JacobianPattern& jp) {
+  std::vector<int> pos;
+  const int slots = jacobian_value_positions(y, jp, pos);
+  compile_plan(plan, d, solve_scratch_cap(), &pos, slots);
+}
+
 // The host image of a plan: the network pattern (when y is given), the start point (when spec is given), the Jacobian
 // pattern and the compiled plan's streams, padded as the kernels expect; h's scalar fields are set, its pointer fields
 // are recorded in the image for bind().
 void build_plan_image(PlanImage& img, DevPlan& h, DevicePlanHost& d, const YBus* y, const JacobianPattern& jp,
                       const BusSpec* spec, const Plan& plan, double base_mva) {
-  compile_plan(plan, d, solve_scratch_cap());
+  if (y) compile_newton_plan(plan, d, *y, jp);
+  else compile_plan(plan, d, solve_scratch_cap());
   h = DevPlan{};
   img = PlanImage{};
   h.m = jp.m;
@@ -120,7 +128,7 @@ void build_plan_image(PlanImage& img, DevPlan& h, DevicePlanHost& d, const YBus*
   h.max_col = d.max_col;
   h.n_levels = d.n_levels;
   h.base_mva = base_mva;
-  // the refactor record of each Jacobian CSR entry (every structural cell is kept, so every entry has one)
+  // the A position of each Jacobian CSR entry (every structural cell is kept, so every entry has one)
   std::vector<int> j2a(d.j2a);
   j2a.resize(std::max<size_t>((size_t)h.nnz_j, j2a.size()), -1);
   if (y) {
@@ -2173,7 +2181,7 @@ GS_API gs_status gs_plan_build(const gs_network* net, const gs_nr_options* opts,
     p->jp = build_jacobian_pattern(p->y, p->spec.type);
     std::string why;
     if (!symbolic_pass(p->y, p->jp, p->spec.vm.data(), p->spec.va.data(), p->plan, why, o.growth_tol)) return GS_SINGULAR;
-    compile_plan(p->plan, p->dph, solve_scratch_cap());
+    compile_newton_plan(p->plan, p->dph, p->y, p->jp);
     *out = p.release();
     return GS_OK;
   } catch (const HipError& e) {
@@ -2222,7 +2230,7 @@ GS_API gs_status gs_plan_fresh_batch(const gs_network* net, const gs_nr_options*
       p->spec = std::move(fs[i].spec);
       p->jp = std::move(fs[i].jp);
       p->plan = std::move(fs[i].plan);
-      compile_plan(p->plan, p->dph, solve_scratch_cap());
+      compile_newton_plan(p->plan, p->dph, p->y, p->jp);
       plans[i] = p.release();
     }
     return worst;

@@ -586,7 +586,23 @@ double max_branch_angle_deg(const Network& net, const double* va) {
   return worst;
 }
 
-void compile_plan(const Plan& plan, DevicePlanHost& d, int scratch_cap) {
+int jacobian_value_positions(const YBus& y, const JacobianPattern& jp, vector<int>& pos) {
+  const int nnz = y.rp[y.n];
+  pos.assign(jp.rp[jp.m], -1);
+  for (int i = 0; i < y.n; i++)
+    for (int k = y.rp[i]; k < y.rp[i + 1]; k++) {
+      const int base = k == y.diag[i] ? 4 * (nnz + i) : 4 * k;
+      for (int b = 0; b < 4; b++) {
+        const int s = jp.slots[4 * (size_t)k + b];
+        if (s >= 0) pos[s] = base + b;
+      }
+    }
+  for (int p : pos)
+    if (p < 0) throw std::runtime_error("plan: a Jacobian entry without a Y cell");
+  return 4 * (nnz + y.n);
+}
+
+void compile_plan(const Plan& plan, DevicePlanHost& d, int scratch_cap, const vector<int>* a_pos, int a_slots) {
   int m = plan.m;
   d = DevicePlanHost();
   d.scratch_cap = scratch_cap;
@@ -777,7 +793,13 @@ void compile_plan(const Plan& plan, DevicePlanHost& d, int scratch_cap) {
   }
   d.n_fac = 64 * d.n_wrows;
   d.j2a.assign(plan.a_src.empty() ? 0 : (size_t)(*std::max_element(plan.a_src.begin(), plan.a_src.end()) + 1), -1);
-  for (size_t e = 0; e < d.ch_a_src.size(); e++) d.j2a[d.ch_a_src[e]] = (int)e;
+  d.a_pos.resize(d.ch_a_src.size());
+  for (size_t e = 0; e < d.ch_a_src.size(); e++) {
+    const int s = d.ch_a_src[e];
+    d.a_pos[e] = a_pos ? a_pos->at(s) : (int)e;
+    if (d.a_pos[e] < 0 || (a_pos && d.a_pos[e] >= a_slots)) throw std::runtime_error("plan: A position out of range");
+    d.j2a[s] = d.a_pos[e];
+  }
   // Forward solve rows: L(i, j) for j < i, columns ascending; level(i) = 1 + max level(j).
   {
     vector<vector<std::pair<int, int>>> rows(m);
@@ -931,7 +953,7 @@ void compile_plan(const Plan& plan, DevicePlanHost& d, int scratch_cap) {
   // rows. The L values the updates read live in vals at n_a + their factor position.
   {
     using D = DevicePlanHost;
-    d.n_a = (int)d.ch_a_src.size();
+    d.n_a = a_pos ? a_slots : (int)d.ch_a_src.size();
     vector<int> col_of_l(d.nnz_l);
     for (int j = 0; j < m; j++)
       for (int p = plan.lp[j]; p < plan.lp[j + 1]; p++) col_of_l[p] = j;
@@ -966,7 +988,7 @@ void compile_plan(const Plan& plan, DevicePlanHost& d, int scratch_cap) {
           const int e = e0 + l;
           const bool in = e < a1;
           rec[2 * l] = in ? d.ch_a_ws[e] : d.spare(l);
-          rec[2 * l + 1] = in ? e : 0;
+          rec[2 * l + 1] = in ? d.a_pos[e] : 0;
         }
         emit(D::kRsAssign | (e0 == a0 ? D::kRsClear : 0) | width, lv, rec.data());
       }
@@ -1154,7 +1176,7 @@ bool emulate_device_lu(const DevicePlanHost& d, int nnz_j, const double* jv, con
   vector<double> ws(wsz), dval(m, std::nan(""));
   auto bad = [&](const std::string& w) { why = w; return false; };
   using D = DevicePlanHost;
-  // vals = [A values in record order, as the mismatch kernel writes them through j2a | the factors, 64 per WRITE row]
+  // vals = [A values at their positions, as the mismatch kernel writes them (j2a) | the factors, 64 per WRITE row]
   const int n_a = d.n_a, n_vals = n_a + d.n_fac;
   vector<double> vals(n_vals, std::nan(""));
   for (int s = 0; s < nnz_j && s < (int)d.j2a.size(); s++) {

@@ -151,11 +151,13 @@ struct DevicePlanHost {
   static constexpr int kRowBegin = 1, kRowEnd = 2, kRowLevelEnd = 4;
   vector<int> row_flags, row_level;                 // [n_rows]
   vector<int> tr_u0, tr_nu, tr_l0, tr_nl;           // per track (chunk order)
-  vector<int> j2a;                                  // [nnz_j]: A record of each Jacobian CSR entry, or -1
+  vector<int> j2a;                                  // [nnz_j]: A position of each Jacobian CSR entry, or -1
+  vector<int> a_pos;                                // per A record: its position in the A part of vals
   // The refactor as the device runs it: one stream of rows, each row one operation of the wavefront with one record
   // (x, y: 2 ints) per lane, so the kernel loads every row's record and value a fixed number of rows ahead. The
   // record's y holds a 24-bit field (y & kRsIndexMask) and the row's kind and flags (the same in every lane) above it:
-  //   ASSIGN  ws[x] = vals[y]                     (a chunk's A entries; kRsClear on its first: zero the workspace)
+  //   ASSIGN  ws[x] = vals[y]                     (a chunk's A entries, y = a_pos; kRsClear on its first: zero the
+  //                                               workspace)
   //   UPDATE  ws[x & 0xFFFF] -= vals[y] * ws[x >> 16]
   //   (a lane with nothing to assign or update is pointed at its own spare slot spare(lane) past the tracks'
   //   workspaces, so the device runs these rows with every lane active and no branch)
@@ -166,7 +168,7 @@ struct DevicePlanHost {
   //           contiguous 64-value store): an L slot divided by its column's pivot, a U slot or the pivot as it is, an
   //           idle lane whatever its slot holds (a position nothing reads); kind x >> 24: U, PIVOT (also dval[x &
   //           kRsIndexMask] = ws, by pivot step), L, 0: idle
-  // vals = [A values (n_a) | the factors, 64 per WRITE row (n_fac)], so y indexes one array for every row kind.
+  // vals = [A values (n_a slots) | the factors, 64 per WRITE row (n_fac)], so y indexes one array for every row kind.
   // kRsLevelEnd marks the last row of a dependency level (the device fences there). No-op rows are inserted so that an
   // UPDATE row stands at least kRsValueAhead rows after the end of every level it reads: its value, loaded that far
   // ahead, is then final.
@@ -221,8 +223,17 @@ struct DevicePlanHost {
   vector<int> fw_order, fw_level_end, lr_ptr, lr_col, lr_idx;  // L rows: columns ascending
   vector<int> bw_order, bw_level_end, ur_ptr, ur_col, ur_idx;  // U rows: columns descending, diagonal excluded
 };
-// scratch_cap > 0 caps the solves' scratch area (slots), so that passes run in several segments (for tests)
-void compile_plan(const Plan& plan, DevicePlanHost& out, int scratch_cap = 0);
+// The Newton path's positions of the Jacobian values in the A part of vals (the order the mismatch pass writes them:
+// off-diagonal Y cell k's H, N, M, L entries at 4k .. 4k + 3, bus i's diagonal block at 4 (nnz_y + i) .. + 3, so each
+// cell's or bus's four values are one contiguous store), per Jacobian CSR entry; returns the A part's size (entries
+// absent from the Jacobian leave holes nothing reads).
+int jacobian_value_positions(const YBus& y, const JacobianPattern& jp, vector<int>& pos);
+
+// scratch_cap > 0 caps the solves' scratch area (slots), so that passes run in several segments (for tests).
+// a_pos: the A part's position of each Jacobian CSR entry and a_slots its size (jacobian_value_positions); null: the
+// A entries in the refactor's record order (what the sparse-solve seam fills from its matrix).
+void compile_plan(const Plan& plan, DevicePlanHost& out, int scratch_cap = 0, const vector<int>* a_pos = nullptr,
+                  int a_slots = 0);
 
 // Host emulation of the device refactor (level chunks, rows of 64 records) and level-scheduled solves, with every
 // index checked against its buffer: out = A^{-1} rhs for the CSR values jv. Returns false (and why) on an

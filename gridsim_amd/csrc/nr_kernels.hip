@@ -142,21 +142,40 @@ __device__ __forceinline__ void y_value(const DevPlan& P, const Inst<T>& I, int 
     if (I.pidx[t] == k) { *g = I.patch[t].g; *b = I.patch[t].b; }
 }
 
-template <typename T, bool KEEP>
-__device__ __forceinline__ void put_j(const Inst<T>& I, int s, int a, double v) {
-  if (KEEP && s >= 0) I.jv[s] = v;
-  if (a >= 0) I.vals[a] = (T)v;
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// The four Jacobian values of one Y cell (or one bus's diagonal block) in the LU precision, to four consecutive
+// positions of vals: one contiguous store per lane, consecutive lanes' stores adjacent.
+template <typename T>
+__device__ __forceinline__ void store4(GLOBAL T* p, double h, double nn, double mm, double l) {
+  if constexpr (sizeof(T) == 8) {
+    GLOBAL v2d* q = (GLOBAL v2d*)p;
+    q[0] = (v2d){h, nn};
+    q[1] = (v2d){mm, l};
+  } else {
+    *(GLOBAL v4f*)p = (v4f){(float)h, (float)nn, (float)mm, (float)l};
+  }
+}
+
+// The FP64 CSR copy of the Jacobian the refinement residual and the iterative fallback read (scattered 8-byte stores).
+__device__ __forceinline__ void put_csr(GLOBAL double* jv, const v4i& s, double h, double nn, double mm, double l) {
+  if (s.x >= 0) jv[s.x] = h;
+  if (s.y >= 0) jv[s.y] = nn;
+  if (s.z >= 0) jv[s.z] = mm;
+  if (s.w >= 0) jv[s.w] = l;
 }
 
 // ------------------------------------------------------------------------------------------------ mismatch + J
-// Each Jacobian value goes, in the LU precision, to its refactor record (vals) and, when the solves are refined
-// (KEEP), to the FP64 CSR copy the refinement residual reads (without it, the CSR positions are not even loaded).
+// VALS: the Jacobian goes, in the LU precision, to the A part of vals in Y-cell order (off-diagonal cell k's H, N, M,
+// L at 4k .. 4k + 3, bus i's diagonal block at 4 (nnz_y + i) ..; the refactor's ASSIGN rows gather them from there);
+// KEEP: also to the FP64 CSR copy (without it, the CSR positions are not even loaded). The cells' refactor positions
+// (aslots, -1 for an entry the Jacobian does not have) say which values count towards max |J|.
 // Every Y cell's pattern and base value are two loads: (row, column) and (g, b). Both passes walk blocks of 64 cells (buses): blocks
 // blk0, blk0 + stride, ... of the wave (one wave: 0, 1; a level-parallel grid: one block per work-group).
 
 // Every Y cell's two injection terms and its off-diagonal Jacobian entries, one cell per lane. Returns whether a
 // Jacobian value of this lane was not finite.
-template <typename T, bool KEEP>
+template <typename T, bool KEEP, bool VALS = true>
 __device__ __forceinline__ int mismatch_cells(const DevPlan& P, const Inst<T>& I, int lane, int blk0, int stride,
                                               double* jmax) {
   GLOBAL const v2i* y_ij = globv<v2i>(P.y_ij);
@@ -208,10 +227,8 @@ __device__ __forceinline__ int mismatch_cells(const DevPlan& P, const Inst<T>& I
         const double nn = vmi * (g * cs + b * sn);
         const double mm = -vmi * vmj * (g * cs + b * sn);
         const double l = vmi * (g * sn - b * cs);
-        put_j<T, KEEP>(I, s.x, a.x, h);
-        put_j<T, KEEP>(I, s.y, a.y, nn);
-        put_j<T, KEEP>(I, s.z, a.z, mm);
-        put_j<T, KEEP>(I, s.w, a.w, l);
+        if (KEEP) put_csr(I.jv, s, h, nn, mm, l);
+        if (VALS) store4<T>(I.vals + 4 * (size_t)k, h, nn, mm, l);
         lbad |= !(isfinite(h) && isfinite(nn) && isfinite(mm) && isfinite(l));
         jm = a.x >= 0 && fabs(h) > jm ? fabs(h) : jm;
         jm = a.y >= 0 && fabs(nn) > jm ? fabs(nn) : jm;
@@ -226,7 +243,7 @@ __device__ __forceinline__ int mismatch_cells(const DevPlan& P, const Inst<T>& I
 
 // Per bus the ordered sums of its row's terms, the diagonal Jacobian entries and the mismatch; the next bus's
 // scalars are loaded ahead. Returns this lane's max |f| (NaN-propagating); *bad: a Jacobian value was not finite.
-template <typename T, bool KEEP>
+template <typename T, bool KEEP, bool VALS = true>
 __device__ __forceinline__ double mismatch_buses(const DevPlan& P, const Inst<T>& I, int lane, int blk0, int stride,
                                                  int* bad, double* jmax) {
   GLOBAL const v2d* y_gb = I.ygb;
@@ -273,10 +290,8 @@ __device__ __forceinline__ double mismatch_buses(const DevPlan& P, const Inst<T>
     const double nn = P_i / vmi + g * vmi;
     const double mm = P_i - g * vmi * vmi;
     const double l = Q_i / vmi - b * vmi;
-    put_j<T, KEEP>(I, s.x, a.x, h);
-    put_j<T, KEEP>(I, s.y, a.y, nn);
-    put_j<T, KEEP>(I, s.z, a.z, mm);
-    put_j<T, KEEP>(I, s.w, a.w, l);
+    if (KEEP) put_csr(I.jv, s, h, nn, mm, l);
+    if (VALS) store4<T>(I.vals + 4 * ((size_t)P.nnz_y + i), h, nn, mm, l);
     lbad |= !(isfinite(h) && isfinite(nn) && isfinite(mm) && isfinite(l));
     jm = a.x >= 0 && fabs(h) > jm ? fabs(h) : jm;
     jm = a.y >= 0 && fabs(nn) > jm ? fabs(nn) : jm;
@@ -293,14 +308,14 @@ __device__ __forceinline__ double mismatch_buses(const DevPlan& P, const Inst<T>
 // The whole pass for one wave. Returns max |f| (NaN-propagating) over the wave; sets *bad when a Jacobian value is
 // not finite.
 // *jnorm: the largest |entry| of the Jacobian (entries that are finite).
-template <typename T, bool KEEP>
+template <typename T, bool KEEP, bool VALS = true>
 __device__ __forceinline__ double mismatch_jacobian(const DevPlan& P, const Inst<T>& I, int lane, int* bad,
                                                    double* jnorm) {
   double jc, jb;
-  const int cbad = mismatch_cells<T, KEEP>(P, I, lane, 0, 1, &jc);
+  const int cbad = mismatch_cells<T, KEEP, VALS>(P, I, lane, 0, 1, &jc);
   WG_SYNC();
   int bbad;
-  const double lmax = mismatch_buses<T, KEEP>(P, I, lane, 0, 1, &bbad, &jb);
+  const double lmax = mismatch_buses<T, KEEP, VALS>(P, I, lane, 0, 1, &bbad, &jb);
   *bad = wave_any(cbad | bbad);
   *jnorm = wave_max(jc > jb ? jc : jb);
   return wave_max(lmax);
@@ -1446,7 +1461,8 @@ __global__ __launch_bounds__(kWave) void iterative_step_kernel(const InstDesc* _
   }
   int bad;
   double jnorm;
-  const double max_mis = mismatch_jacobian<double, true>(P, I, opaque_lane(lane), &bad, &jnorm);
+  // (the FP64 CSR copy only: this instance has no LU values)
+  const double max_mis = mismatch_jacobian<double, true, false>(P, I, opaque_lane(lane), &bad, &jnorm);
   WG_SYNC();
   if (lane != 0) return;
   const int iter = st->iter;
