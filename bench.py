@@ -44,9 +44,10 @@ def algorithmic_bytes(plan: dict, n_bus: int, nnz_y: int, iterations: np.ndarray
     return per_step * steps + mis * len(iterations)
 
 
-def measured_traffic(stats: dict, lu: str, flags: int, alg: int):
+def measured_traffic(stats: dict, lu: str, flags: int, alg: int, kernel_ms: float):
     """HBM bytes of one launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this same workload, when
-    the plan, arithmetic and iteration counts here are the ones profiled; otherwise None (no stale figure)."""
+    the plan, arithmetic and iteration counts here are the ones profiled and this run's kernel time lies within 10 % of
+    the profiled one (a kernel changed since would move it); otherwise None (no stale figure)."""
     path = os.path.join(ROOT, "profiles", "nr_batch_kernel_n1_2000bus_traffic.json")
     if not os.path.exists(path) or flags != 0:
         return None, None
@@ -57,6 +58,8 @@ def measured_traffic(stats: dict, lu: str, flags: int, alg: int):
                                       "factor_slots")}
     mine.update(lu=lu, algorithmic_bytes=int(alg))
     if any(fp.get(k) != v for k, v in mine.items()):
+        return None, None
+    if not abs(kernel_ms / rec["kernel_ms"] - 1.0) <= 0.10:
         return None, None
     return int(rec["read_bytes"] + rec["write_bytes"]), rec["source"]
 
@@ -659,7 +662,7 @@ def main():
     stats = dn.plan_stats()
     shape = dn.launch_shape(opts)
     traffic, traffic_src = measured_traffic(stats, "fp32" if opts.lu_precision == engine.LU_FP32 else "fp64",
-                                            opts.flags, alg)
+                                            opts.flags, alg, kavg)
 
     # the other arithmetic on the same sweep: FP32 LU + one FP64 refinement step beside FP64 LU
     other = fp64 if opts.lu_precision == engine.LU_FP32 else fp32
