@@ -488,7 +488,7 @@ def run_config5(args):
         row = {"offered_rps": rate, "requests": n_req, "solves_per_s": round(conv / wall, 1),
                "p50_ms": round(st["latency_p50_ms"], 3), "p99_ms": round(st["latency_p99_ms"], 3),
                "max_ms": round(st["latency_max_ms"], 3), "mean_batch": round(st["mean_batch"], 2),
-               "launches": st["launches"], "converged": conv,
+               "launches": st["launches"], "joined_running": st["joined"], "converged": conv,
                "plain": {"requests": pl_n, "solves_per_s": round(pconv / pwall, 1),
                          "p50_ms": round(float(np.percentile(lat, 50)), 3),
                          "p99_ms": round(float(np.percentile(lat, 99)), 3)}}
@@ -825,7 +825,7 @@ def _serve_segment(engine, ctx, dn, opts, outages, args):
     return {"offered_rps": args.serve_rate, "requests": n_req, "served_rps": round(n_req / wall, 1),
             "converged": conv, "p50_ms": round(st["latency_p50_ms"], 3), "p99_ms": round(st["latency_p99_ms"], 3),
             "max_ms": round(st["latency_max_ms"], 3), "mean_batch": round(st["mean_batch"], 2),
-            "launches": st["launches"]}
+            "launches": st["launches"], "joined_running": st["joined"]}
 
 
 def gridgen_seed() -> int:

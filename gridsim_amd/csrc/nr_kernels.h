@@ -117,7 +117,8 @@ struct Patch {
 // Per-instance mutable state: lives in device memory across launches, so the host can resume an instance.
 struct InstState {
   int32_t iter;         // Newton steps taken so far
-  int32_t done;         // 1: finished (status valid); 2: paused for a fresh symbolic pass
+  int32_t done;         // 0: running (paused between launches); 1: finished (status valid); 2: paused for a fresh
+                        // symbolic pass; kDoneFresh: not started (the next launch starts it, whatever its init)
   int32_t converged, diverged;
   int32_t nonfinite, step_failures;
   int32_t fail_column;
@@ -133,6 +134,10 @@ struct InstState {
   uint64_t jn_bits;     // level-parallel: max |J entry| of the step being formed (bits), and of the current step
   double jnorm;
 };
+
+// InstState::done of an instance the next launch starts afresh (its start point, its counters reset) even when the
+// launch itself resumes the others: how the serving loop adds requests to a running batch between Newton steps.
+constexpr int kDoneFresh = 3;
 
 // Control block of a level-parallel run: how many of its instances have finished (or paused).
 struct WideCtl {

@@ -703,20 +703,24 @@ __global__ __launch_bounds__(kWave * (PAIR ? kPairWaves : 1), 2) void nr_batch_k
   InstState* st = d.state;
   const int n = P.n, m = P.m;
 
-  if (prm.init || prm.load_start) {
+  // a fresh instance (the launch starts every instance, or this one was added to a running batch) starts from its
+  // start point with its counters reset; any other finished or paused one returns at once
+  const int was = prm.init ? 0 : st->done;
+  const bool fresh = prm.init || was == kDoneFresh;
+  if (fresh || prm.load_start) {
     for (int i = lane; i < n; i += kWave) { I.vm[i] = I.vm_start[i]; I.va[i] = I.va_start[i]; }
     WG_SYNC();
   }
-  if (!prm.init && st->done) return;
-  int iter = prm.init ? 0 : st->iter;
-  double start_mis = prm.init ? __builtin_nan("") : st->start_mis;
-  double min_ratio = prm.init ? 1.0 : st->min_ratio;
-  double last_from = prm.init ? __builtin_nan("") : st->last_from;
-  int step_failures = prm.init ? 0 : st->step_failures;
+  if (!fresh && was) return;
+  int iter = fresh ? 0 : st->iter;
+  double start_mis = fresh ? __builtin_nan("") : st->start_mis;
+  double min_ratio = fresh ? 1.0 : st->min_ratio;
+  double last_from = fresh ? __builtin_nan("") : st->last_from;
+  int step_failures = fresh ? 0 : st->step_failures;
   int converged = 0, diverged = 0, paused = 0, fail_col = -1, steps = 0;
-  double max_mis;
-  uint64_t t_mis = prm.init ? 0 : st->t_mismatch, t_fac = prm.init ? 0 : st->t_refactor;
-  uint64_t t_sol = prm.init ? 0 : st->t_solve, t_upd = prm.init ? 0 : st->t_update;
+  double max_mis = __builtin_nan("");
+  uint64_t t_mis = fresh ? 0 : st->t_mismatch, t_fac = fresh ? 0 : st->t_refactor;
+  uint64_t t_sol = fresh ? 0 : st->t_solve, t_upd = fresh ? 0 : st->t_update;
   uint64_t clk = wall_clock64();
   auto tick = [&](uint64_t& acc) {
     const uint64_t now = wall_clock64();
@@ -736,6 +740,8 @@ __global__ __launch_bounds__(kWave * (PAIR ? kPairWaves : 1), 2) void nr_batch_k
     }
   };
   for (;;) {
+    // the launch's steps are taken: pause before the next mismatch (the next launch starts with it)
+    if (steps >= prm.steps_this_launch) { paused = 1; break; }
     int bad;
     double jnorm;
     max_mis = prm.refine_steps > 0 ? mismatch_jacobian<T, true>(P, I, opaque_lane(lane), &bad, &jnorm)
@@ -746,7 +752,6 @@ __global__ __launch_bounds__(kWave * (PAIR ? kPairWaves : 1), 2) void nr_batch_k
     if (max_mis < prm.tolerance) { converged = 1; break; }
     if (max_mis > nan_max(prm.diverge_limit, start_mis)) { diverged = 1; break; }
     if (iter >= prm.max_iterations) break;
-    if (steps >= prm.steps_this_launch) { paused = 1; break; }
     if (bad || !isfinite(max_mis)) break;  // NewtonShell.SolveStep refuses a non-finite J or f
     // (a plan the host made at this very state is the factorisation's own: its first step is not checked)
     const bool trust = prm.trust_first && steps == 0;
@@ -1217,20 +1222,23 @@ __global__ __launch_bounds__(kWave * kGroupMaxWaves) void group_kernel(const Ins
   const int n = P.n, m = P.m;
   int set = 0;
 
-  if (prm.init || prm.load_start) {
+  // (as the wavefront kernel: a fresh instance starts from its start point, any other finished one returns)
+  const int was = prm.init ? 0 : st->done;
+  const bool fresh = prm.init || was == kDoneFresh;
+  if (fresh || prm.load_start) {
     for (int i = tid; i < n; i += nt) { I.vm[i] = I.vm_start[i]; I.va[i] = I.va_start[i]; }
     __syncthreads();
   }
-  if (!prm.init && st->done) return;
-  int iter = prm.init ? 0 : st->iter;
-  double start_mis = prm.init ? __builtin_nan("") : st->start_mis;
-  double min_ratio = prm.init ? 1.0 : st->min_ratio;
-  double last_from = prm.init ? __builtin_nan("") : st->last_from;
-  int step_failures = prm.init ? 0 : st->step_failures;
+  if (!fresh && was) return;
+  int iter = fresh ? 0 : st->iter;
+  double start_mis = fresh ? __builtin_nan("") : st->start_mis;
+  double min_ratio = fresh ? 1.0 : st->min_ratio;
+  double last_from = fresh ? __builtin_nan("") : st->last_from;
+  int step_failures = fresh ? 0 : st->step_failures;
   int converged = 0, diverged = 0, paused = 0, fail_col = -1, steps = 0;
-  double max_mis;
-  uint64_t t_mis = prm.init ? 0 : st->t_mismatch, t_fac = prm.init ? 0 : st->t_refactor;
-  uint64_t t_sol = prm.init ? 0 : st->t_solve, t_upd = prm.init ? 0 : st->t_update;
+  double max_mis = __builtin_nan("");
+  uint64_t t_mis = fresh ? 0 : st->t_mismatch, t_fac = fresh ? 0 : st->t_refactor;
+  uint64_t t_sol = fresh ? 0 : st->t_solve, t_upd = fresh ? 0 : st->t_update;
   uint64_t clk = wall_clock64();
   auto tick = [&](uint64_t& acc) {
     const uint64_t now = wall_clock64();
@@ -1244,6 +1252,7 @@ __global__ __launch_bounds__(kWave * kGroupMaxWaves) void group_kernel(const Ins
       group_solve<T, false>(P, I, rhs, out, I.xg, ws, tid, nt, opaque_lane(lane), wave, W);
   };
   for (;;) {
+    if (steps >= prm.steps_this_launch) { paused = 1; break; }
     double jc, jb;
     int bad;
     const int cbad = prm.refine_steps > 0 ? mismatch_cells<T, true>(P, I, opaque_lane(lane), wave, W, &jc)
@@ -1260,7 +1269,6 @@ __global__ __launch_bounds__(kWave * kGroupMaxWaves) void group_kernel(const Ins
     if (max_mis < prm.tolerance) { converged = 1; break; }
     if (max_mis > nan_max(prm.diverge_limit, start_mis)) { diverged = 1; break; }
     if (iter >= prm.max_iterations) break;
-    if (steps >= prm.steps_this_launch) { paused = 1; break; }
     if (bad || !isfinite(max_mis)) break;  // NewtonShell.SolveStep refuses a non-finite J or f
     // (a plan the host made at this very state is the factorisation's own: its first step is not checked)
     const bool trust = prm.trust_first && steps == 0;

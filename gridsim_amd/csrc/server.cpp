@@ -2,13 +2,14 @@
 //
 // The reference answers gridsim-rpc/1 "solve" requests one after another on one thread (RpcCommand.cs:460-497 reads a
 // line, :668-700 solves it to completion, writes the response, reads the next). Here a dispatcher thread owns up to
-// four launch slots, each with its own stream. Whenever a slot is free it takes the network of the oldest waiting
-// request, gathers that network's waiting requests in arrival order (at most max_batch, and only as many as the HBM
-// state arena and the pinned result arena can hold), lays them out in the arenas and launches the batched Newton
-// kernel on the slot's stream, with the states and voltages copied back into pinned memory behind it. Requests that
-// arrive while launches run are gathered into the next free slot, so their work overlaps with what is already on the
-// device. Finished slots are reaped by polling their events; a result is published under its ticket and the arena
-// blocks return to the free lists.
+// four slots, each with its own stream. A slot is a running batch of one network: a block of instance positions in the
+// HBM state arena (taken by memory: the slot halves its size until the arena holds it), each holding one request or
+// nothing. Every launch of the batched Newton kernel on the slot runs a few Newton steps of each instance in it (as
+// many as keep a launch near a millisecond); between launches the dispatcher publishes the instances that finished,
+// and the waiting requests of the same network take the free positions and start afresh in the next launch. A request
+// therefore joins the GPU's work at the next Newton-step boundary of a running batch of its network, or in a free slot,
+// and leaves as soon as its own solve ends. Each instance is solved by its own wavefront (or work-group), so its result
+// does not depend on what runs beside it or when it joined.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -116,17 +117,27 @@ struct gs_server {
   };
   std::vector<std::unique_ptr<Net>> nets;
 
-  // One launch slot: its stream, completion event and the launch it holds.
+  // One slot: a running batch of one network. Its device block holds `cap` positions (instance blocks, then the
+  // descriptors, then the states), its pinned block their states and voltage spans. A free position's state says
+  // finished, so the kernel returns at once for it.
   struct Slot {
+    enum Phase { kIdle, kStepping, kCopying };
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
-    bool busy = false;
+    Phase phase = kIdle;
     int net = -1;
-    const Net* nt = nullptr;  // srv->nets[net], taken under the lock (the vector may grow meanwhile)
-    std::vector<Request> reqs;
+    const Net* nt = nullptr;   // srv->nets[net], taken under the lock (the vector may grow meanwhile)
+    int cap = 0, live = 0;
+    int steps = 1;             // Newton steps per launch
+    std::vector<Request> pos;  // per position: its request (ticket 0: free)
+    std::vector<Staging> stg;  // per position: the host copies its last upload reads
+    std::vector<int> joining;  // positions given new requests, not yet uploaded
+    std::vector<int> leaving;  // positions whose instances ended in the last launch
+    std::vector<InstDesc> free_desc;
+    std::vector<InstState> free_state;
     size_t dev_off = 0, dev_bytes = 0, host_off = 0, host_bytes = 0;
-    Staging stg;
     PreparedRun pr;
+    bool pr_ready = false;
     Clock::time_point launched;
   };
   std::vector<Slot> slots;
@@ -146,7 +157,7 @@ struct gs_server {
   std::thread worker;
 
   // statistics since the last reset
-  int64_t submitted = 0, completed = 0, rejected = 0, launches = 0, launched_instances = 0;
+  int64_t submitted = 0, completed = 0, rejected = 0, launches = 0, launched_instances = 0, joined = 0;
   std::vector<double> latencies;
   double busy_ms = 0;
   Clock::time_point busy_since;
@@ -164,8 +175,17 @@ struct gs_server {
 
 namespace {
 
-// Byte sizes of one launch of `count` instances of network nt: device block [instances | descriptors | states] and
-// pinned block [states | per instance the vm..va span].
+using Slot = gs_server::Slot;
+
+// A launch should take about this long: long enough that launching costs little beside it, short enough that a
+// waiting request soon finds a step boundary.
+constexpr double kStepTargetMs = 1.0;
+constexpr int kMaxStepsPerLaunch = 8;
+// a slot starts with room for at least this many requests (later arrivals of its network join into it)
+constexpr int kMinSlotCap = 16;
+
+// Byte sizes of a slot of `count` positions of network nt: device block [instances | descriptors | states] and
+// pinned block [states | per position the vm..q span].
 size_t device_block(const gs_server::Net& nt, int count) {
   return nt.L.stride * (size_t)count + align_up(sizeof(InstDesc) * (size_t)count) +
          align_up(sizeof(InstState) * (size_t)count);
@@ -174,196 +194,332 @@ size_t host_block(const gs_server::Net& nt, int count) {
   return align_up(sizeof(InstState) * (size_t)count) + nt.state_span * (size_t)count;
 }
 
-// Lays out and launches the requests of slot s (device and pinned blocks already reserved) on the slot's stream.
-void launch_slot(gs_server* srv, gs_server::Slot& s) {
-  const gs_server::Net& nt = *s.nt;
-  const DevPlan& h = nt.net->dev.h;
-  const int count = (int)s.reqs.size();
-  char* dbase = srv->dev_arena.as<char>() + s.dev_off;
-  RunBuffers rb;
-  rb.inst = dbase;
-  rb.desc = reinterpret_cast<InstDesc*>(dbase + nt.L.stride * (size_t)count);
-  rb.state = reinterpret_cast<InstState*>(dbase + nt.L.stride * (size_t)count +
-                                          align_up(sizeof(InstDesc) * (size_t)count));
-  rb.base_spec = nt.base_spec.as<double>();
-  std::vector<gs_instance> inst(count);
-  for (int i = 0; i < count; i++) {
-    inst[i] = s.reqs[i].inst;
-    inst[i].outages = s.reqs[i].outages.empty() ? nullptr : s.reqs[i].outages.data();
-    if (!s.reqs[i].start.empty()) {
-      inst[i].vm0 = s.reqs[i].start.data();
-      inst[i].va0 = s.reqs[i].start.data() + h.n;
-    } else {
-      inst[i].vm0 = inst[i].va0 = nullptr;
-    }
-  }
-  s.pr = prepare_into(h, rb, s.stg, count, inst.data(), &nt.net->y, &nt.net->net, nt.opts, false, nullptr, nullptr,
-                      nullptr, s.stream);
-  launch_prepared(h, s.pr, nt.opts.schedule, s.stream);
-  char* hbase = srv->pinned + s.host_off;
-  check(hipMemcpyAsync(hbase, rb.state, sizeof(InstState) * (size_t)count, hipMemcpyDeviceToHost, s.stream),
-        "server states");
-  check(hipMemcpy2DAsync(hbase + align_up(sizeof(InstState) * (size_t)count), nt.state_span, dbase + nt.L.vm,
-                         nt.L.stride, nt.state_span, (size_t)count, hipMemcpyDeviceToHost, s.stream),
-        "server voltages");
-  check(hipEventRecord(s.done, s.stream), "server event");
+char* slot_inst(gs_server* srv, const Slot& s) { return srv->dev_arena.as<char>() + s.dev_off; }
+InstDesc* slot_desc(gs_server* srv, const Slot& s) {
+  return reinterpret_cast<InstDesc*>(slot_inst(srv, s) + s.nt->L.stride * (size_t)s.cap);
+}
+InstState* slot_state(gs_server* srv, const Slot& s) {
+  return reinterpret_cast<InstState*>(reinterpret_cast<char*>(slot_desc(srv, s)) +
+                                      align_up(sizeof(InstDesc) * (size_t)s.cap));
+}
+const InstState* pinned_state(gs_server* srv, const Slot& s) {
+  return reinterpret_cast<const InstState*>(srv->pinned + s.host_off);
+}
+char* pinned_span(gs_server* srv, const Slot& s, int p) {
+  return srv->pinned + s.host_off + align_up(sizeof(InstState) * (size_t)s.cap) + s.nt->state_span * (size_t)p;
 }
 
-// Results of a finished slot (its event has completed): finished instances from the pinned copies, paused ones
-// through fresh symbolic passes. Called without the server lock.
-std::vector<std::pair<uint64_t, Finished>> reap_slot(gs_server* srv, gs_server::Slot& s, gs_status launch_status) {
-  std::vector<std::pair<uint64_t, Finished>> out;
-  const int count = (int)s.reqs.size();
-  if (launch_status != GS_OK) {
-    for (auto& r : s.reqs) {
-      Finished f;
-      f.status = launch_status;
-      out.emplace_back(r.ticket, std::move(f));
-    }
-    return out;
+// A new slot's device state: every position free (its descriptor names its state, the state says finished).
+void clear_slot(gs_server* srv, Slot& s) {
+  s.free_desc.assign(s.cap, InstDesc{});
+  s.free_state.assign(s.cap, InstState{});
+  InstState* st = slot_state(srv, s);
+  for (int p = 0; p < s.cap; p++) {
+    s.free_desc[p].state = st + p;
+    s.free_state[p].done = 1;
   }
+  check(hipMemcpyAsync(slot_desc(srv, s), s.free_desc.data(), sizeof(InstDesc) * (size_t)s.cap,
+                       hipMemcpyHostToDevice, s.stream), "server descriptors");
+  check(hipMemcpyAsync(st, s.free_state.data(), sizeof(InstState) * (size_t)s.cap, hipMemcpyHostToDevice, s.stream),
+        "server states");
+}
+
+// Lays the requests of s.joining out at their positions, each starting afresh at the next launch.
+void place_joining(gs_server* srv, Slot& s) {
+  const gs_server::Net& nt = *s.nt;
+  const DevPlan& h = nt.net->dev.h;
+  std::vector<InstState> fresh(1);
+  fresh[0].done = kDoneFresh;
+  for (int p : s.joining) {
+    const Request& r = s.pos[p];
+    RunBuffers rb;
+    rb.inst = slot_inst(srv, s) + nt.L.stride * (size_t)p;
+    rb.desc = slot_desc(srv, s) + p;
+    rb.state = slot_state(srv, s) + p;
+    rb.base_spec = nt.base_spec.as<double>();
+    gs_instance in = r.inst;
+    in.outages = r.outages.empty() ? nullptr : r.outages.data();
+    in.vm0 = r.start.empty() ? nullptr : r.start.data();
+    in.va0 = r.start.empty() ? nullptr : r.start.data() + h.n;
+    PreparedRun one = prepare_into(h, rb, s.stg[p], 1, &in, &nt.net->y, &nt.net->net, nt.opts, false, &fresh, nullptr,
+                                   nullptr, s.stream);
+    if (!s.pr_ready) {
+      // the launch over all positions: the same run and parameters, resuming (a fresh instance starts by its state)
+      s.pr = one;
+      s.pr.count = s.cap;
+      s.pr.desc = slot_desc(srv, s);
+      s.pr.prm.init = 0;
+      s.pr.prm.load_start = 0;
+      s.pr.prm.trust_first = 0;
+      s.pr_ready = true;
+    }
+  }
+  s.joining.clear();
+}
+
+// Uploads the joining requests and launches the slot's next steps, then copies every position's state back.
+void launch_steps(gs_server* srv, Slot& s) {
+  place_joining(srv, s);
+  s.pr.prm.steps_this_launch = s.steps;
+  launch_prepared(s.nt->net->dev.h, s.pr, s.nt->opts.schedule, s.stream);
+  check(hipMemcpyAsync(srv->pinned + s.host_off, slot_state(srv, s), sizeof(InstState) * (size_t)s.cap,
+                       hipMemcpyDeviceToHost, s.stream), "server states back");
+  check(hipEventRecord(s.done, s.stream), "server event");
+  s.launched = Clock::now();
+  s.phase = Slot::kStepping;
+}
+
+// After a launch: the positions whose instances ended (finished, or paused for a fresh symbolic pass) get their
+// voltages copied back. Returns whether there is anything to copy (the slot then waits for its event again).
+bool collect_leaving(gs_server* srv, Slot& s) {
+  const InstState* st = pinned_state(srv, s);
+  s.leaving.clear();
+  for (int p = 0; p < s.cap; p++)
+    if (s.pos[p].ticket && st[p].done != 0) s.leaving.push_back(p);
+  if (s.leaving.empty()) return false;
+  const Layout& L = s.nt->L;
+  for (int p : s.leaving)
+    check(hipMemcpyAsync(pinned_span(srv, s, p), slot_inst(srv, s) + L.stride * (size_t)p + L.vm, s.nt->state_span,
+                         hipMemcpyDeviceToHost, s.stream), "server voltages");
+  check(hipEventRecord(s.done, s.stream), "server event");
+  s.phase = Slot::kCopying;
+  return true;
+}
+
+// The results of the leaving positions (their voltages are in pinned memory): finished instances as they are, paused
+// ones through fresh symbolic passes. Called without the server lock.
+std::vector<std::pair<uint64_t, Finished>> results_of_leaving(gs_server* srv, Slot& s, gs_status status) {
+  std::vector<std::pair<uint64_t, Finished>> out;
   const gs_server::Net& nt = *s.nt;
   const int n = nt.net->net.n;
-  const char* hbase = srv->pinned + s.host_off;
-  const InstState* st = reinterpret_cast<const InstState*>(hbase);
-  const char* volts = hbase + align_up(sizeof(InstState) * (size_t)count);
-  auto span = [&](int i, size_t off) {
-    return reinterpret_cast<const double*>(volts + nt.state_span * (size_t)i + (off - nt.L.vm));
-  };
-  for (int i = 0; i < count; i++) {
+  const InstState* st = pinned_state(srv, s);
+  for (int p : s.leaving) {
     Finished f;
+    f.status = status;
     f.base_mva = nt.net->net.base;
-    const double *vm = span(i, nt.L.vm), *va = span(i, nt.L.va), *p = span(i, nt.L.p), *q = span(i, nt.L.q);
-    f.vm.assign(vm, vm + n);
-    f.va.assign(va, va + n);
-    f.p.assign(p, p + n);
-    f.q.assign(q, q + n);
-    if (st[i].done == 1) {
-      fill_result(st[i], f.result, 0, srv->ctx->us_per_tick);
-    } else {
-      try {
-        double kms = 0;
-        int nl = 0;
-        gs_instance in = s.reqs[i].inst;
-        in.vm0 = in.va0 = nullptr;
-        in.outages = s.reqs[i].outages.empty() ? nullptr : s.reqs[i].outages.data();
-        std::vector<double> pp, qq;
-        // the fresh symbolic passes and resumed launches run on the context's stream: under the context's lock, as
-        // every other call on the context, so they never land inside a capture a client has begun on that stream
-        std::lock_guard<std::mutex> ctx_lock(srv->ctx->mu);
-        finish_paused(nt.net, in, nt.opts, false, st[i], f.vm, f.va, f.result, kms, nl, &pp, &qq);
-        if ((int)pp.size() == n) f.p = pp;
-        if ((int)qq.size() == n) f.q = qq;
-      } catch (const HipError& e) {
-        f.status = e.code;
+    if (status == GS_OK) {
+      const char* span = pinned_span(srv, s, p);
+      auto arr = [&](size_t off) { return reinterpret_cast<const double*>(span + (off - nt.L.vm)); };
+      f.vm.assign(arr(nt.L.vm), arr(nt.L.vm) + n);
+      f.va.assign(arr(nt.L.va), arr(nt.L.va) + n);
+      f.p.assign(arr(nt.L.p), arr(nt.L.p) + n);
+      f.q.assign(arr(nt.L.q), arr(nt.L.q) + n);
+      if (st[p].done == 1) {
+        fill_result(st[p], f.result, 0, srv->ctx->us_per_tick);
+      } else {
+        try {
+          double kms = 0;
+          int nl = 0;
+          gs_instance in = s.pos[p].inst;
+          in.vm0 = in.va0 = nullptr;
+          in.outages = s.pos[p].outages.empty() ? nullptr : s.pos[p].outages.data();
+          std::vector<double> pp, qq;
+          // the fresh symbolic passes and resumed launches run on the context's stream: under the context's lock, as
+          // every other call on the context, so they never land inside a capture a client has begun on that stream
+          std::lock_guard<std::mutex> ctx_lock(srv->ctx->mu);
+          finish_paused(nt.net, in, nt.opts, false, st[p], f.vm, f.va, f.result, kms, nl, &pp, &qq);
+          if ((int)pp.size() == n) f.p = pp;
+          if ((int)qq.size() == n) f.q = qq;
+        } catch (const HipError& e) {
+          f.status = e.code;
+        }
       }
     }
-    out.emplace_back(s.reqs[i].ticket, std::move(f));
+    out.emplace_back(s.pos[p].ticket, std::move(f));
   }
   return out;
 }
 
-// Picks the next launch for a free slot under the lock: the oldest request's network and that network's waiting
-// requests in arrival order, as many as max_batch and the two arenas admit. False when nothing can be launched.
-bool admit(gs_server* srv, gs_server::Slot& s) {
+// Under the lock: publishes results, frees their positions.
+void publish(gs_server* srv, Slot& s, std::vector<std::pair<uint64_t, Finished>>& results) {
+  const auto now = Clock::now();
+  for (auto& tf : results) {
+    const uint64_t ticket = tf.first;
+    Finished& f = tf.second;
+    for (Request& r : s.pos) {
+      if (r.ticket != ticket) continue;
+      f.latency_ms = ms_between(r.submitted, now);
+      r = Request{};
+      s.live--;
+      break;
+    }
+    srv->latencies.push_back(f.latency_ms);
+    srv->in_flight_tickets.erase(ticket);
+    srv->finished[ticket] = std::move(f);
+    srv->completed++;
+  }
+  s.leaving.clear();
+  if (!results.empty()) srv->done_cv.notify_all();
+}
+
+// Under the lock: the waiting requests of the slot's network take its free positions (arrival order). A slot stops
+// taking its network's requests while an older request of another network waits for a slot and none is idle, so that
+// the oldest request is served first.
+void take_waiting(gs_server* srv, Slot& s) {
+  if (srv->queue.empty() || s.live == s.cap) return;
+  if (srv->queue.front().net != s.net) {
+    bool idle = false;
+    for (const Slot& o : srv->slots) idle |= o.phase == Slot::kIdle;
+    if (!idle) return;
+  }
+  const bool running = s.live > 0;   // the slot's other instances are between Newton steps
+  int p = 0;
+  for (auto it = srv->queue.begin(); it != srv->queue.end() && s.live < s.cap;) {
+    if (it->net != s.net) { ++it; continue; }
+    while (s.pos[p].ticket) p++;
+    s.pos[p] = std::move(*it);
+    it = srv->queue.erase(it);
+    srv->in_flight_tickets[s.pos[p].ticket] = (int)(&s - srv->slots.data());
+    s.joining.push_back(p);
+    s.live++;
+    srv->joined += running;
+  }
+}
+
+// Under the lock: an idle slot takes the network of the oldest waiting request, with room for that network's waiting
+// requests (at least kMinSlotCap, at most max_batch), halved until the two arenas hold it. False when nothing fits.
+bool open_slot(gs_server* srv, Slot& s) {
   if (srv->queue.empty()) return false;
   const int net = srv->queue.front().net;
   const gs_server::Net& nt = *srv->nets[net];
-  int avail = 0;
-  for (const Request& r : srv->queue)
-    if (r.net == net && ++avail >= srv->cfg.max_batch) break;
-  for (int count = avail; count >= 1; count = count > 1 ? std::max(1, count / 2) : 0) {
-    size_t db = device_block(nt, count), hb = host_block(nt, count);
-    size_t doff = srv->dev_alloc.alloc(db);
+  int waiting = 0;
+  for (const Request& r : srv->queue) waiting += r.net == net;
+  int cap = std::min(srv->cfg.max_batch, std::max(waiting, kMinSlotCap));
+  for (; cap >= 1; cap = cap > 1 ? cap / 2 : 0) {
+    const size_t db = device_block(nt, cap), hb = host_block(nt, cap);
+    const size_t doff = srv->dev_alloc.alloc(db);
     if (doff == SIZE_MAX) continue;
-    size_t hoff = srv->host_alloc.alloc(hb);
+    const size_t hoff = srv->host_alloc.alloc(hb);
     if (hoff == SIZE_MAX) {
       srv->dev_alloc.release(doff, db);
       continue;
     }
     s.net = net;
     s.nt = &nt;
+    s.cap = cap;
+    s.live = 0;
+    s.steps = 1;
+    s.pos.assign(cap, Request{});
+    s.stg.assign(cap, Staging{});
+    s.joining.clear();
+    s.leaving.clear();
+    s.pr_ready = false;
     s.dev_off = doff; s.dev_bytes = db;
     s.host_off = hoff; s.host_bytes = hb;
-    s.reqs.clear();
-    for (auto it = srv->queue.begin(); it != srv->queue.end() && (int)s.reqs.size() < count;) {
-      if (it->net == net) {
-        s.reqs.push_back(std::move(*it));
-        it = srv->queue.erase(it);
-      } else {
-        ++it;
-      }
-    }
     srv->state_peak = std::max(srv->state_peak, srv->dev_alloc.used());
     return true;
   }
-  return false;  // the arenas are full: wait for a slot to finish
+  return false;  // the arenas are full: wait for a slot to drain
+}
+
+void close_slot(gs_server* srv, Slot& s) {
+  srv->dev_alloc.release(s.dev_off, s.dev_bytes);
+  srv->host_alloc.release(s.host_off, s.host_bytes);
+  s.pos.clear();
+  s.stg.clear();
+  s.net = -1;
+  s.nt = nullptr;
+  s.phase = Slot::kIdle;
+  if (--srv->busy_slots == 0) srv->busy_ms += ms_between(srv->busy_since, Clock::now());
+}
+
+// Under the lock: the slot at a step boundary (its last launch and copies done, results published): it takes waiting
+// requests and launches its next steps, or closes when nothing is left in it. Unlocks around the launch.
+void next_steps(gs_server* srv, Slot& s, std::unique_lock<std::mutex>& lock) {
+  take_waiting(srv, s);
+  if (s.live == 0) {
+    close_slot(srv, s);
+    return;
+  }
+  srv->launches++;
+  srv->launched_instances += s.live;
+  lock.unlock();
+  gs_status st = GS_OK;
+  try {
+    launch_steps(srv, s);
+  } catch (const HipError& e) {
+    st = e.code;
+  }
+  lock.lock();
+  if (st != GS_OK) {
+    // the launch never reached the device: fail the slot's requests now
+    std::vector<std::pair<uint64_t, Finished>> failed;
+    for (const Request& r : s.pos)
+      if (r.ticket) {
+        Finished f;
+        f.status = st;
+        failed.emplace_back(r.ticket, std::move(f));
+      }
+    publish(srv, s, failed);
+    close_slot(srv, s);
+  }
 }
 
 void dispatcher(gs_server* srv) {
   (void)hipSetDevice(srv->ctx->device);
   std::unique_lock<std::mutex> lock(srv->mu);
   for (;;) {
-    // reap finished slots
-    for (gs_server::Slot& s : srv->slots) {
-      if (!s.busy) continue;
-      hipError_t q = hipEventQuery(s.done);
+    // slots whose launch or copies have completed
+    for (Slot& s : srv->slots) {
+      if (s.phase == Slot::kIdle) continue;
+      const hipError_t q = hipEventQuery(s.done);
       if (q == hipErrorNotReady) continue;
+      gs_status status = q == hipSuccess ? GS_OK : GS_DEVICE_ERROR;
       lock.unlock();
-      auto results = reap_slot(srv, s, q == hipSuccess ? GS_OK : GS_DEVICE_ERROR);
-      auto now = Clock::now();
-      lock.lock();
-      for (auto& tf : results) {
-        const uint64_t ticket = tf.first;
-        Finished& f = tf.second;
-        auto it = std::find_if(s.reqs.begin(), s.reqs.end(), [&](const Request& r) { return r.ticket == ticket; });
-        f.latency_ms = ms_between(it->submitted, now);
-        srv->latencies.push_back(f.latency_ms);
-        srv->in_flight_tickets.erase(ticket);
-        srv->finished[ticket] = std::move(f);
-        srv->completed++;
+      std::vector<std::pair<uint64_t, Finished>> results;
+      bool copying = false;
+      // a failed launch or copy fails every request the slot holds
+      auto fail_all = [&](gs_status st) {
+        status = st;
+        s.leaving.clear();
+        for (int p = 0; p < s.cap; p++)
+          if (s.pos[p].ticket) s.leaving.push_back(p);
+        results = results_of_leaving(srv, s, st);
+      };
+      try {
+        if (status != GS_OK) {
+          fail_all(status);
+        } else {
+          if (s.phase == Slot::kStepping) {
+            // adapt the steps per launch to keep launches near the target length
+            const double ms = ms_between(s.launched, Clock::now());
+            if (ms < 0.5 * kStepTargetMs && s.steps < kMaxStepsPerLaunch) s.steps *= 2;
+            else if (ms > 2.0 * kStepTargetMs && s.steps > 1) s.steps /= 2;
+            copying = collect_leaving(srv, s);
+          }
+          if (!copying) results = results_of_leaving(srv, s, GS_OK);
+        }
+      } catch (const HipError& e) {
+        copying = false;
+        fail_all(e.code);
       }
-      srv->dev_alloc.release(s.dev_off, s.dev_bytes);
-      srv->host_alloc.release(s.host_off, s.host_bytes);
-      s.reqs.clear();
-      s.busy = false;
-      if (--srv->busy_slots == 0) srv->busy_ms += ms_between(srv->busy_since, now);
-      srv->done_cv.notify_all();
+      lock.lock();
+      if (copying) continue;
+      publish(srv, s, results);
+      if (status != GS_OK) {
+        close_slot(srv, s);
+        continue;
+      }
+      next_steps(srv, s, lock);
     }
-    // fill free slots
-    for (gs_server::Slot& s : srv->slots) {
-      if (s.busy || !admit(srv, s)) continue;
-      for (const Request& r : s.reqs) srv->in_flight_tickets[r.ticket] = (int)(&s - srv->slots.data());
-      s.busy = true;
+    // idle slots take the oldest waiting request's network
+    for (Slot& s : srv->slots) {
+      if (s.phase != Slot::kIdle || !open_slot(srv, s)) continue;
       if (srv->busy_slots++ == 0) srv->busy_since = Clock::now();
-      srv->launches++;
-      srv->launched_instances += (int64_t)s.reqs.size();
+      s.phase = Slot::kStepping;   // (not idle while it is being set up)
       lock.unlock();
       gs_status st = GS_OK;
       try {
-        launch_slot(srv, s);
+        clear_slot(srv, s);
       } catch (const HipError& e) {
         st = e.code;
       }
       lock.lock();
       if (st != GS_OK) {
-        // the launch never reached the device: fail its requests now
-        auto now = Clock::now();
-        for (Request& r : s.reqs) {
-          Finished f;
-          f.status = st;
-          f.latency_ms = ms_between(r.submitted, now);
-          srv->in_flight_tickets.erase(r.ticket);
-          srv->finished[r.ticket] = std::move(f);
-          srv->completed++;
-        }
-        srv->dev_alloc.release(s.dev_off, s.dev_bytes);
-        srv->host_alloc.release(s.host_off, s.host_bytes);
-        s.reqs.clear();
-        s.busy = false;
-        if (--srv->busy_slots == 0) srv->busy_ms += ms_between(srv->busy_since, now);
-        srv->done_cv.notify_all();
+        close_slot(srv, s);
+        continue;
       }
+      next_steps(srv, s, lock);
     }
     if (srv->stopping && srv->queue.empty() && srv->busy_slots == 0) break;
     if (srv->busy_slots == 0) {
@@ -533,13 +689,14 @@ GS_API gs_status gs_server_stats_get(gs_server* srv, gs_server_stats* out) {
                                                                          srv->latencies.end());
   out->busy_ms = srv->busy_ms + (srv->busy_slots ? ms_between(srv->busy_since, Clock::now()) : 0.0);
   out->state_bytes_peak = (int64_t)srv->state_peak;
+  out->joined = srv->joined;
   return GS_OK;
 }
 
 GS_API gs_status gs_server_stats_reset(gs_server* srv) {
   if (!srv) return GS_INVALID_ARGUMENT;
   std::lock_guard<std::mutex> lock(srv->mu);
-  srv->submitted = srv->completed = srv->rejected = srv->launches = srv->launched_instances = 0;
+  srv->submitted = srv->completed = srv->rejected = srv->launches = srv->launched_instances = srv->joined = 0;
   srv->latencies.clear();
   srv->busy_ms = 0;
   if (srv->busy_slots) srv->busy_since = Clock::now();

@@ -122,7 +122,8 @@ class GsServerStats(C.Structure):
     _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("rejected", C.c_int64),
                 ("launches", C.c_int64), ("queued", C.c_int64), ("in_flight", C.c_int64),
                 ("mean_batch", C.c_double), ("latency_p50_ms", C.c_double), ("latency_p99_ms", C.c_double),
-                ("latency_max_ms", C.c_double), ("busy_ms", C.c_double), ("state_bytes_peak", C.c_int64)]
+                ("latency_max_ms", C.c_double), ("busy_ms", C.c_double), ("state_bytes_peak", C.c_int64),
+                ("joined", C.c_int64)]
 
 
 class GsDeviceInfo(C.Structure):
@@ -892,9 +893,9 @@ def autotune(dn: "DeviceNetwork", outages=None, load_scale=None, options: NrOpti
 
 class Server:
     """Continuous-batching request loop over resident networks (gs_server_*): submit() queues a power-flow or
-    single-contingency request and returns a ticket at once; a dispatcher thread in the library batches waiting
-    requests of one network into launches on up to four streams as slots and memory free up; wait() returns the
-    result of one ticket."""
+    single-contingency request and returns a ticket at once; a dispatcher thread in the library runs up to four slots,
+    each a batch of one network stepped a few Newton iterations per launch, and between launches finished requests
+    leave and waiting ones of the same network take their positions; wait() returns the result of one ticket."""
 
     def __init__(self, ctx: Context, max_batch: int = 0, streams: int = 0, queue_capacity: int = 0,
                  state_bytes: int = 0, pinned_bytes: int = 0):

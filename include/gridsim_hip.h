@@ -355,17 +355,21 @@ GS_API gs_status gs_nr_autotune(gs_net* net, int32_t count, const gs_instance* s
 /* ---- serving: continuous batching in front of the rpc "solve" contract ----------------------------------------
  * The reference serves gridsim-rpc/1 "solve" requests one at a time (src/GridSim.Cli/RpcCommand.cs:460-497,668-700:
  * each request runs SolverBackend.Solver(...).Solve(model, options) to completion before the next line is read). A
- * gs_server keeps resident networks (Y-bus, symbolic plan, base voltages) in HBM and runs a dispatcher thread that,
- * whenever a launch slot is free, takes the oldest waiting request's network, gathers that network's waiting requests
- * in arrival order (up to max_batch, and as many as the HBM and pinned-memory budgets admit) and launches them on the
- * slot's own stream. Requests that arrive while launches run wait only for the next free slot: up to `streams`
- * launches overlap on the device. Every instance is solved by its own wavefront, so a result does not depend on what
- * else shares its launch. Networks added to a server must outlive it. */
+ * gs_server keeps resident networks (Y-bus, symbolic plan, base voltages) in HBM and runs a dispatcher thread over up
+ * to `streams` slots, each a running batch of one network on its own stream: a slot opens for the oldest waiting
+ * request's network with room for that network's waiting requests (at least 16, at most max_batch positions, as the
+ * HBM and pinned-memory budgets admit), and each of its launches runs a few Newton steps of every instance in it (as
+ * many as keep a launch near a millisecond). Between launches, instances that finished leave and waiting requests of
+ * the same network take the free positions, starting in the next launch; a slot closes when it empties. A request thus
+ * joins the device's work at the next Newton-step boundary of a running batch of its network (or in a free slot), and
+ * a slot stops taking its own network's requests while an older request of another network waits with no slot free.
+ * Every instance is solved by its own wavefront (or work-group), so a result does not depend on what else shares its
+ * launch or when it joined. Networks added to a server must outlive it. */
 typedef struct gs_server gs_server;
 
 typedef struct gs_server_config {
-  int32_t max_batch;        /* instances per launch (0: 256) */
-  int32_t streams;          /* launches in flight at once, 1..4 (0: 4) */
+  int32_t max_batch;        /* instance positions per slot at most (0: 256) */
+  int32_t streams;          /* slots, each with its stream and launches in flight, 1..4 (0: 4) */
   int32_t queue_capacity;   /* waiting requests before gs_server_submit returns GS_BUSY (0: 65536) */
   int32_t reserved;
   size_t state_bytes;       /* HBM for the state of in-flight instances (0: 4 GiB) */
@@ -373,11 +377,14 @@ typedef struct gs_server_config {
 } gs_server_config;
 
 typedef struct gs_server_stats {
-  int64_t submitted, completed, rejected, launches, queued, in_flight;
+  int64_t submitted, completed, rejected;
+  int64_t launches;                                      /* kernel launches (each a few Newton steps of a slot) */
+  int64_t queued, in_flight;
   double mean_batch;                                     /* instances per launch */
   double latency_p50_ms, latency_p99_ms, latency_max_ms; /* submit -> result ready, since the last reset */
   double busy_ms;                                        /* wall time with at least one launch in flight */
   int64_t state_bytes_peak;
+  int64_t joined;   /* requests that took a position of a batch whose other instances were between Newton steps */
 } gs_server_stats;
 
 GS_API gs_status gs_server_create(gs_ctx* ctx, const gs_server_config* cfg, gs_server** out);

@@ -40,7 +40,7 @@ def test_struct_sizes_match_the_header(eng):
     assert C.sizeof(eng.GsCsr) == 32
     assert C.sizeof(eng.GsNetwork) == 4 * 3 + 4 + 8 + 24 * 8
     assert eng.SUMMARY_DTYPE.itemsize == 56
-    assert C.sizeof(eng.GsServerConfig) == 32 and C.sizeof(eng.GsServerStats) == 6 * 8 + 5 * 8 + 8
+    assert C.sizeof(eng.GsServerConfig) == 32 and C.sizeof(eng.GsServerStats) == 6 * 8 + 5 * 8 + 2 * 8
     assert C.sizeof(eng.GsPlanStats) == 10 * 4 + 8 + 8 * 4
 
 
