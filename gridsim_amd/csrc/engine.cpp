@@ -141,6 +141,15 @@ void build_plan_image(PlanImage& img, DevPlan& h, DevicePlanHost& d, const YBus*
     img.put(&DevPlan::y_rp, y->rp); img.put(&DevPlan::y_ij, y_ij); img.put(&DevPlan::y_diag, y->diag);
     img.put(&DevPlan::y_gb, y_gb); img.put(&DevPlan::slots, jp.slots); img.put(&DevPlan::aslots, aslots);
     img.put(&DevPlan::ppos, jp.ppos); img.put(&DevPlan::qpos, jp.qpos);
+    // the same unknowns in pivot-step order (where the wavefront schedule's dx keeps them)
+    std::vector<int> qinv(jp.m, -1), ppos_k(jp.ppos.size(), -1), qpos_k(jp.qpos.size(), -1), j_ci_k(jp.ci.size());
+    for (int k = 0; k < jp.m; k++) qinv[d.q[k]] = k;
+    for (size_t i = 0; i < jp.ppos.size(); i++) {
+      if (jp.ppos[i] >= 0) ppos_k[i] = qinv[jp.ppos[i]];
+      if (jp.qpos[i] >= 0) qpos_k[i] = qinv[jp.qpos[i]];
+    }
+    for (size_t e = 0; e < jp.ci.size(); e++) j_ci_k[e] = qinv[jp.ci[e]];
+    img.put(&DevPlan::ppos_k, ppos_k); img.put(&DevPlan::qpos_k, qpos_k); img.put(&DevPlan::j_ci_k, j_ci_k);
   }
   if (spec) { img.put(&DevPlan::vm0, spec->vm); img.put(&DevPlan::va0, spec->va); }
   img.put(&DevPlan::j_rp, jp.rp); img.put(&DevPlan::j_ci, jp.ci);

@@ -19,6 +19,10 @@ struct DevPlan {
   const int4* aslots;                 // per Y cell: refactor record (aval index) of the same four entries, or -1
   const int32_t *ppos, *qpos;         // equation numbering
   const int32_t *j_rp, *j_ci;         // Jacobian CSR pattern (for the FP64 refinement residual)
+  // The wavefront schedule keeps the Newton step dx in pivot-step order (the triangular solves' own order, so the
+  // solve stores it contiguously): each bus's P and Q unknowns' and each Jacobian column's positions there (qinv of
+  // ppos / qpos / j_ci; -1 where ppos / qpos are)
+  const int32_t *ppos_k, *qpos_k, *j_ci_k;
   // base bus specification
   const double *vm0, *va0;            // start point (BuildBusSpec under the upload options)
   double base_mva;

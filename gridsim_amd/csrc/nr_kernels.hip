@@ -145,15 +145,17 @@ __device__ __forceinline__ void y_value(const DevPlan& P, const Inst<T>& I, int 
 typedef float v4f __attribute__((ext_vector_type(4)));
 
 // The four Jacobian values of one Y cell (or one bus's diagonal block) in the LU precision, to four consecutive
-// positions of vals: one contiguous store per lane, consecutive lanes' stores adjacent.
+// positions of vals: one contiguous store per lane, consecutive lanes' stores adjacent. The per-instance values each
+// Newton step writes (Jacobian, injection terms, factors) are stored non-temporal: hundreds of instances per XCD far
+// exceed its L2, so keeping them there would only evict the plan's record streams, which every instance reads.
 template <typename T>
 __device__ __forceinline__ void store4(GLOBAL T* p, double h, double nn, double mm, double l) {
   if constexpr (sizeof(T) == 8) {
     GLOBAL v2d* q = (GLOBAL v2d*)p;
-    q[0] = (v2d){h, nn};
-    q[1] = (v2d){mm, l};
+    __builtin_nontemporal_store((v2d){h, nn}, q);
+    __builtin_nontemporal_store((v2d){mm, l}, q + 1);
   } else {
-    *(GLOBAL v4f*)p = (v4f){(float)h, (float)nn, (float)mm, (float)l};
+    __builtin_nontemporal_store((v4f){(float)h, (float)nn, (float)mm, (float)l}, (GLOBAL v4f*)p);
   }
 }
 
@@ -221,7 +223,7 @@ __device__ __forceinline__ int mismatch_cells(const DevPlan& P, const Inst<T>& I
       gs_fdlibm::sincos(vai - vaj, &sn, &cs);
       const double tp = vmj * (g * cs + b * sn);
       const double tq = vmj * (g * sn - b * cs);
-      I.tpq[k] = (v2d){tp, tq};
+      __builtin_nontemporal_store((v2d){tp, tq}, &I.tpq[k]);
       if (j != i) {
         const double h = vmi * vmj * (g * sn - b * cs);
         const double nn = vmi * (g * cs + b * sn);
@@ -458,7 +460,7 @@ __device__ __forceinline__ int refactor(const DevPlan& P, GLOBAL T* vals, GLOBAL
         const int wk = (int)((unsigned)r4.x >> 24), pos = r4.y & kRsIndexMask;
         const T v = ws[(lane >> wsh) * P.wstride + write_row * TLr + (lane & (TLr - 1))];
         const T lv = v / piv;
-        vals[pos] = wk == kWriteL ? lv : v;
+        __builtin_nontemporal_store(wk == kWriteL ? lv : v, &vals[pos]);
         if (wk == kWritePivot) dval[r4.x & kRsIndexMask] = v;
         write_row++;
         lds_drained();
@@ -596,7 +598,9 @@ __device__ __forceinline__ void solve_stream(GLOBAL const v2i* rec, int step0, i
   }
 }
 
-template <typename T, bool XLDS, typename XP>
+// OUTK: out in pivot-step order (out[k] = x[k], one contiguous store per 64 entries) instead of the unknowns' own order
+// (out[q[k]] = x[k]).
+template <typename T, bool XLDS, typename XP, bool OUTK = false>
 __device__ __forceinline__ void lu_solve(const DevPlan& P, GLOBAL const T* fac, GLOBAL const T* dval,
                                          GLOBAL const double* rhs, GLOBAL double* out, XP x,
                                          LDSP T* scr, int lane) {
@@ -607,16 +611,17 @@ __device__ __forceinline__ void lu_solve(const DevPlan& P, GLOBAL const T* fac, 
   if (XLDS) WAVE_SYNC(); else WG_SYNC();
   solve_stream<T, false, XLDS>(globv<v2i>(P.fs_rec), 0, P.fs_steps, fac, dval, x, scr, lane);
   solve_stream<T, true, XLDS>(globv<v2i>(P.bs_rec), 0, P.bs_steps, fac, dval, x, scr, lane);
-  for (int k = lane; k < m; k += kWave) out[q[k]] = (double)x[k];
+  for (int k = lane; k < m; k += kWave) out[OUTK ? k : q[k]] = (double)x[k];
   WG_SYNC();
 }
 
-// r = f - J dx (FP64, CSR rows; each row's products summed in column order).
+// r = f - J dx (FP64, CSR rows; each row's products summed in column order); KORDER: dx in pivot-step order.
 // Rows t, t + nt, ... (a wave: its lane and 64; a work-group: its thread and size).
+template <bool KORDER = false>
 __device__ __forceinline__ void residual(const DevPlan& P, GLOBAL const double* jv, GLOBAL const double* f,
                                          GLOBAL const double* dx, GLOBAL double* r, int t, int nt = kWave) {
   GLOBAL const int* j_rp = glob(P.j_rp);
-  GLOBAL const int* j_ci = glob(P.j_ci);
+  GLOBAL const int* j_ci = glob(KORDER ? P.j_ci_k : P.j_ci);
   const int m = P.m;
   int i = t;
   int nk0 = 0, nk1 = 0;
@@ -731,12 +736,12 @@ __global__ __launch_bounds__(kWave * (PAIR ? kPairWaves : 1), 2) void nr_batch_k
   auto solve = [&](GLOBAL const double* rhs, GLOBAL double* out) {
     if constexpr (PAIR) {
       if (!take_lock(x_lock, lane)) { lock_timeout = 1; return; }
-      lu_solve<T, true>(P, I.fac, I.dval, rhs, out, xl, ws, opaque_lane(lane));
+      lu_solve<T, true, LDSP T*, true>(P, I.fac, I.dval, rhs, out, xl, ws, opaque_lane(lane));
       give_lock(x_lock, lane);
     } else if constexpr (XLDS) {
-      lu_solve<T, true>(P, I.fac, I.dval, rhs, out, xl, ws, opaque_lane(lane));
+      lu_solve<T, true, LDSP T*, true>(P, I.fac, I.dval, rhs, out, xl, ws, opaque_lane(lane));
     } else {
-      lu_solve<T, false>(P, I.fac, I.dval, rhs, out, I.xg, ws, opaque_lane(lane));
+      lu_solve<T, false, GLOBAL T*, true>(P, I.fac, I.dval, rhs, out, I.xg, ws, opaque_lane(lane));
     }
   };
   for (;;) {
@@ -762,7 +767,7 @@ __global__ __launch_bounds__(kWave * (PAIR ? kPairWaves : 1), 2) void nr_batch_k
     if (fail_col >= 0) { paused = 2; break; }
     solve(I.f, I.dx);
     for (int it = 0; it < prm.refine_steps; it++) {
-      residual(P, I.jv, I.f, I.dx, I.r, opaque_lane(lane));
+      residual<true>(P, I.jv, I.f, I.dx, I.r, opaque_lane(lane));
       solve(I.r, I.r);
       for (int k = lane; k < m; k += kWave) I.dx[k] = I.dx[k] + I.r[k];
       WG_SYNC();
@@ -779,8 +784,9 @@ __global__ __launch_bounds__(kWave * (PAIR ? kPairWaves : 1), 2) void nr_batch_k
     if (wave_any(lbad)) { step_failures++; break; }
     const double max_abs = wave_max(lmax);
     const double damp = max_abs > 0.4 ? 0.4 / max_abs : 1.0;
-    GLOBAL const int* ppos = glob(P.ppos);
-    GLOBAL const int* qpos = glob(P.qpos);
+    // (dx in pivot-step order: each bus's unknowns found through ppos_k / qpos_k)
+    GLOBAL const int* ppos = glob(P.ppos_k);
+    GLOBAL const int* qpos = glob(P.qpos_k);
     for (int i = lane; i < n; i += kWave) {
       const int pp = ppos[i], qq = qpos[i];
       if (pp >= 0) I.va[i] = I.va[i] + damp * I.dx[pp];
