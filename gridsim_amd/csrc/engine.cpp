@@ -71,15 +71,31 @@ int solve_scratch_cap() {
 }
 
 // The Y cells as the Jacobian fill reads them, one 8-byte (row, column) and one 16-byte (g, b) record per cell.
+// Per Y cell its (row, column) and base value (g, b); gb then repeats each bus's diagonal cell, by bus, for the
+// mismatch pass's bus loop (one coalesced load where the cell's own place is one scattered line per bus).
 static void pack_cells(const YBus& y, std::vector<int>& ij, std::vector<double>& gb) {
   const int nnz = y.rp[y.n];
   ij.assign(2 * (size_t)nnz, 0);
-  gb.assign(2 * (size_t)nnz, 0.0);
+  gb.assign(2 * ((size_t)nnz + y.n), 0.0);
   for (int i = 0; i < y.n; i++)
     for (int k = y.rp[i]; k < y.rp[i + 1]; k++) {
       ij[2 * (size_t)k] = i; ij[2 * (size_t)k + 1] = y.ci[k];
       gb[2 * (size_t)k] = y.g[k]; gb[2 * (size_t)k + 1] = y.b[k];
     }
+  for (int i = 0; i < y.n; i++) {
+    gb[2 * ((size_t)nnz + i)] = y.g[y.diag[i]];
+    gb[2 * ((size_t)nnz + i) + 1] = y.b[y.diag[i]];
+  }
+}
+
+// Four slots per Y cell, then each bus's diagonal cell's four again, by bus (as pack_cells).
+static std::vector<int> with_diag_slots(const YBus& y, const std::vector<int>& cell_slots) {
+  const size_t nnz = y.rp[y.n];
+  std::vector<int> out(cell_slots);
+  out.resize(4 * (nnz + y.n));
+  for (int i = 0; i < y.n; i++)
+    for (int b = 0; b < 4; b++) out[4 * (nnz + i) + b] = cell_slots[4 * (size_t)y.diag[i] + b];
+  return out;
 }
 
 WideSplit wide_split_of(const DevicePlanHost& d) {
@@ -108,8 +124,10 @@ WideSplit wide_split_of(const DevicePlanHost& d) {
 // A Newton plan: the Jacobian values at the positions the mismatch pass writes them (Y-cell order).
 void compile_newton_plan(const Plan& plan, DevicePlanHost& d, const YBus& y, const JacobianPattern& jp) {
   std::vector<int> pos;
-  const int slots = jacobian_value_positions(y, jp, pos);
+  std::vector<int> tr = symmetric_cell_transpose(y);
+  const int slots = jacobian_value_positions(y, jp, tr, pos);
   compile_plan(plan, d, solve_scratch_cap(), &pos, slots);
+  d.y_tr = std::move(tr);
 }
 
 // The host image of a plan: the network pattern (when y is given), the start point (when spec is given), the Jacobian
@@ -134,12 +152,18 @@ void build_plan_image(PlanImage& img, DevPlan& h, DevicePlanHost& d, const YBus*
   if (y) {
     h.n = y->n;
     h.nnz_y = y->rp[y->n];
-    std::vector<int> y_ij, aslots(jp.slots.size());
+    std::vector<int> y_ij, aslots(jp.slots.size()), slots(jp.slots);
     std::vector<double> y_gb;
     pack_cells(*y, y_ij, y_gb);
-    for (size_t e = 0; e < jp.slots.size(); e++) aslots[e] = jp.slots[e] >= 0 ? j2a[jp.slots[e]] : -1;
+    // per Y cell's lane: the CSR positions of the values it forms (its own cell's, or its transpose's)
+    h.y_tr = d.y_tr.empty() ? 0 : 1;
+    if (h.y_tr)
+      for (size_t k = 0; k < d.y_tr.size(); k++)
+        for (int b = 0; b < 4; b++) slots[4 * k + b] = jp.slots[4 * (size_t)d.y_tr[k] + b];
+    for (size_t e = 0; e < slots.size(); e++) aslots[e] = slots[e] >= 0 ? j2a[slots[e]] : -1;
     img.put(&DevPlan::y_rp, y->rp); img.put(&DevPlan::y_ij, y_ij); img.put(&DevPlan::y_diag, y->diag);
-    img.put(&DevPlan::y_gb, y_gb); img.put(&DevPlan::slots, jp.slots); img.put(&DevPlan::aslots, aslots);
+    img.put(&DevPlan::y_gb, y_gb); img.put(&DevPlan::slots, with_diag_slots(*y, slots));
+    img.put(&DevPlan::aslots, with_diag_slots(*y, aslots));
     img.put(&DevPlan::ppos, jp.ppos); img.put(&DevPlan::qpos, jp.qpos);
     // the same unknowns in pivot-step order (where the wavefront schedule's dx keeps them)
     std::vector<int> qinv(jp.m, -1), ppos_k(jp.ppos.size(), -1), qpos_k(jp.qpos.size(), -1), j_ci_k(jp.ci.size());
@@ -267,8 +291,9 @@ gs_plan_stats plan_stats(const DevicePlanHost& d, const YBus& y, const JacobianP
   st.back_steps = ring_up(d.bs_steps);
   st.workspace_stride = d.wstride;
   st.factor_slots = d.n_fac;
+  st.jacobian_transposed = d.y_tr.empty() ? 0 : 1;
   const int64_t n = y.n, nnz_y = y.rp.empty() ? 0 : y.rp[y.n], m = jp.m, rec_row = 64 * 8;
-  int64_t b = 4 * (n + 1) + 4 * nnz_y * 2 + 4 * n + 16 * nnz_y + 2 * 16 * nnz_y + 2 * 4 * n;  // Y-bus, slots, pos
+  int64_t b = 4 * (n + 1) + 4 * nnz_y * 2 + 4 * n + 3 * 16 * (nnz_y + n) + 2 * 4 * n;  // Y-bus, slots, pos
   b += 4 * (m + 1) + 4 * (int64_t)st.nnz_j + 2 * 8 * n + 4 * 4 * m;                            // J pattern, start, pivots
   b += rec_row * (int64_t)(st.refactor_rows + 2 * kStreamPad) + 4 * (int64_t)d.ch_a_src.size();
   b += rec_row * (int64_t)(st.forward_steps + 2 * kStreamPad + st.back_steps + 2 * kStreamPad);
@@ -534,13 +559,19 @@ PreparedRun prepare_into(const DevPlan& h, const RunBuffers& rb, Staging& stg, i
       } else {
         // more cells than the descriptor holds: the instance's own Y values, the network's with the patches applied
         if (own_used >= rb.own_y_count) fail(GS_DEVICE_ERROR, "own Y-value arrays under-counted");
-        std::vector<double> gb(2 * (size_t)y->rp[y->n]);
-        for (int k = 0; k < y->rp[y->n]; k++) { gb[2 * (size_t)k] = y->g[k]; gb[2 * (size_t)k + 1] = y->b[k]; }
+        // (laid out as pack_cells: the cells, then the buses' diagonal cells; the kernel patches neither)
+        const size_t nnz = y->rp[y->n];
+        std::vector<double> gb(2 * (nnz + y->n));
+        for (size_t k = 0; k < nnz; k++) { gb[2 * k] = y->g[k]; gb[2 * k + 1] = y->b[k]; }
         for (int q = 0; q < np; q++) {
           gb[2 * (size_t)patches[q].idx] = patches[q].g;
           gb[2 * (size_t)patches[q].idx + 1] = patches[q].b;
         }
-        double* dst = rb.own_y + (size_t)own_used * 2 * (size_t)y->rp[y->n];
+        for (int b = 0; b < y->n; b++) {
+          gb[2 * (nnz + b)] = gb[2 * (size_t)y->diag[b]];
+          gb[2 * (nnz + b) + 1] = gb[2 * (size_t)y->diag[b] + 1];
+        }
+        double* dst = rb.own_y + (size_t)own_used * 2 * (nnz + y->n);
         d.y_gb_own = dst;
         stg.own_y.emplace_back(dst, std::move(gb));
         own_used++;
@@ -624,7 +655,7 @@ PreparedRun prepare_instances(gs_ctx* ctx, DevicePlan& dp, DevMem& inst_buf, Dev
   // the base specification, then the own Y-value arrays of the instances that need them
   const int own = own_y_count(y, netw, count, inst);
   const size_t spec_bytes = (16 * (size_t)h.n + 255) & ~size_t(255);
-  aux_buf.ensure(spec_bytes + 16 * (size_t)h.nnz_y * own);
+  aux_buf.ensure(spec_bytes + 16 * ((size_t)h.nnz_y + h.n) * own);
   stg.aux = base_spec.pspec;
   stg.aux.insert(stg.aux.end(), base_spec.qspec.begin(), base_spec.qspec.end());
   check(hipMemcpyAsync(aux_buf.p, stg.aux.data(), sizeof(double) * stg.aux.size(), hipMemcpyHostToDevice, s), "spec");
@@ -890,7 +921,8 @@ void upload_pattern(DevicePlan& dp, const YBus& y, const JacobianPattern& jp, do
       if (jp.ci[k] == r) jdiag[r] = k;
   PlanImage img;
   img.put(&DevPlan::y_rp, y.rp); img.put(&DevPlan::y_ij, y_ij); img.put(&DevPlan::y_diag, y.diag);
-  img.put(&DevPlan::y_gb, y_gb); img.put(&DevPlan::slots, jp.slots); img.put(&DevPlan::aslots, aslots);
+  img.put(&DevPlan::y_gb, y_gb); img.put(&DevPlan::slots, with_diag_slots(y, jp.slots));
+  img.put(&DevPlan::aslots, with_diag_slots(y, aslots));
   img.put(&DevPlan::ppos, jp.ppos); img.put(&DevPlan::qpos, jp.qpos); img.put(&DevPlan::j_rp, jp.rp);
   img.put(&DevPlan::j_ci, jp.ci); img.put(&DevPlan::j_diag, jdiag);
   dp.block.ensure(img.bytes.size());
@@ -2283,6 +2315,29 @@ GS_API gs_status gs_plan_selfcheck(const gs_plan* p, double* rel_residual, char*
     if (p->jp.qpos[i] >= 0) f[p->jp.qpos[i]] = p->spec.qspec[i] - qq[i];
   }
   std::string w;
+  // With a symmetric Y-bus, Y cell k's lane forms the Jacobian values of cell y_tr[k] (its transpose) from its own Y
+  // value, the two magnitudes swapped and the sine of the opposite angle difference, exactly as the mismatch kernel
+  // does: they equal that cell's entries bit for bit (compared with host_jacobian, which forms every cell's own).
+  const DevicePlanHost& dt = p->dph;
+  const double *vm = p->spec.vm.data(), *va = p->spec.va.data();
+  for (size_t k = 0; k < dt.y_tr.size(); k++) {
+    const int row = (int)(std::upper_bound(p->y.rp.begin(), p->y.rp.end(), (int)k) - p->y.rp.begin()) - 1;
+    const int col = p->y.ci[k];
+    if (row == col) continue;
+    double sn, cs;
+    gs_fdlibm::sincos(va[row] - va[col], &sn, &cs);
+    const double g = p->y.g[k], b = p->y.b[k];
+    const double vr = vm[col], vc = vm[row], st = sn == 0.0 ? va[col] - va[row] : 0.0 - sn;
+    const double v[4] = {vr * vc * (g * st - b * cs), vr * (g * cs + b * st), -vr * vc * (g * cs + b * st),
+                         vr * (g * st - b * cs)};
+    for (int e = 0; e < 4; e++) {
+      const int slot = p->jp.slots[4 * (size_t)dt.y_tr[k] + e];
+      if (slot >= 0 && std::memcmp(&v[e], &jv[slot], sizeof(double)) != 0) {
+        if (why) std::snprintf(why, 128, "cell %d's lane forms a transposed Jacobian value unlike the cell's own", (int)k);
+        return GS_INVALID_ARGUMENT;
+      }
+    }
+  }
   if (!emulate_device_lu(p->dph, p->jp.rp[m], jv.data(), f.data(), x.data(), w)) {
     if (why) std::snprintf(why, 128, "%s", w.c_str());
     return GS_INVALID_ARGUMENT;

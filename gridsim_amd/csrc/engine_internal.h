@@ -277,7 +277,7 @@ struct RunBuffers {
   InstDesc* desc = nullptr;
   InstState* state = nullptr;
   const double* base_spec = nullptr;
-  double* own_y = nullptr;    // room for own_y_count instances' own Y values (2 nnz_y doubles each)
+  double* own_y = nullptr;    // room for own_y_count instances' own Y values (2 (nnz_y + n) doubles each)
   int own_y_count = 0;
 };
 

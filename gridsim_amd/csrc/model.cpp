@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <limits>
 #include <stdexcept>
 
@@ -586,12 +587,32 @@ double max_branch_angle_deg(const Network& net, const double* va) {
   return worst;
 }
 
-int jacobian_value_positions(const YBus& y, const JacobianPattern& jp, vector<int>& pos) {
+vector<int> symmetric_cell_transpose(const YBus& y) {
+  const int nnz = y.rp[y.n];
+  vector<int> tr(nnz, -1);
+  auto same = [](double a, double b) { return std::memcmp(&a, &b, sizeof(double)) == 0; };
+  for (int i = 0; i < y.n; i++)
+    for (int k = y.rp[i]; k < y.rp[i + 1]; k++) {
+      const int t = y.find(y.ci[k], i);
+      if (t < 0 || !same(y.g[k], y.g[t]) || !same(y.b[k], y.b[t])) return {};
+      const int c0 = y.contrib_ptr[k], c1 = y.contrib_ptr[k + 1], d0 = y.contrib_ptr[t];
+      if (c1 - c0 != y.contrib_ptr[t + 1] - d0) return {};
+      for (int c = 0; c < c1 - c0; c++)
+        if (y.contrib_branch[c0 + c] != y.contrib_branch[d0 + c] || !same(y.contrib_g[c0 + c], y.contrib_g[d0 + c]) ||
+            !same(y.contrib_b[c0 + c], y.contrib_b[d0 + c]))
+          return {};
+      tr[k] = t;
+    }
+  return tr;
+}
+
+int jacobian_value_positions(const YBus& y, const JacobianPattern& jp, const vector<int>& tr, vector<int>& pos) {
   const int nnz = y.rp[y.n];
   pos.assign(jp.rp[jp.m], -1);
   for (int i = 0; i < y.n; i++)
     for (int k = y.rp[i]; k < y.rp[i + 1]; k++) {
-      const int base = k == y.diag[i] ? 4 * (nnz + i) : 4 * k;
+      // the lane of cell tr[k] forms cell k's values (tr is its own inverse; a diagonal cell is its own transpose)
+      const int base = 4 * (tr.empty() ? k : tr[k]);
       for (int b = 0; b < 4; b++) {
         const int s = jp.slots[4 * (size_t)k + b];
         if (s >= 0) pos[s] = base + b;
@@ -599,7 +620,7 @@ int jacobian_value_positions(const YBus& y, const JacobianPattern& jp, vector<in
     }
   for (int p : pos)
     if (p < 0) throw std::runtime_error("plan: a Jacobian entry without a Y cell");
-  return 4 * (nnz + y.n);
+  return 4 * nnz;
 }
 
 void compile_plan(const Plan& plan, DevicePlanHost& d, int scratch_cap, const vector<int>* a_pos, int a_slots) {

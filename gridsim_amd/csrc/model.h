@@ -153,6 +153,7 @@ struct DevicePlanHost {
   vector<int> tr_u0, tr_nu, tr_l0, tr_nl;           // per track (chunk order)
   vector<int> j2a;                                  // [nnz_j]: A position of each Jacobian CSR entry, or -1
   vector<int> a_pos;                                // per A record: its position in the A part of vals
+  vector<int> y_tr;                                 // Newton plans: the Y cells' transposes (symmetric Y), or empty
   // The refactor as the device runs it: one stream of rows, each row one operation of the wavefront with one record
   // (x, y: 2 ints) per lane, so the kernel loads every row's record and value a fixed number of rows ahead. The
   // record's y holds a 24-bit field (y & kRsIndexMask) and the row's kind and flags (the same in every lane) above it:
@@ -223,11 +224,18 @@ struct DevicePlanHost {
   vector<int> fw_order, fw_level_end, lr_ptr, lr_col, lr_idx;  // L rows: columns ascending
   vector<int> bw_order, bw_level_end, ur_ptr, ur_col, ur_idx;  // U rows: columns descending, diagonal excluded
 };
-// The Newton path's positions of the Jacobian values in the A part of vals (the order the mismatch pass writes them:
-// off-diagonal Y cell k's H, N, M, L entries at 4k .. 4k + 3, bus i's diagonal block at 4 (nnz_y + i) .. + 3, so each
-// cell's or bus's four values are one contiguous store), per Jacobian CSR entry; returns the A part's size (entries
-// absent from the Jacobian leave holes nothing reads).
-int jacobian_value_positions(const YBus& y, const JacobianPattern& jp, vector<int>& pos);
+// The Y cells' transposes (cell (i, j) -> cell (j, i)) when the Y-bus is symmetric bit for bit: every off-diagonal
+// cell's stamp list (branches, values, order) equal to its transpose's, so that every outage's re-summed value is
+// too. Empty otherwise (a phase shifter, or a stamp whose zero differs in sign).
+vector<int> symmetric_cell_transpose(const YBus& y);
+// The Newton path's positions of the Jacobian values in the A part of vals (the order the mismatch pass writes them),
+// per Jacobian CSR entry; returns the A part's size, 4 nnz_y (entries absent from the Jacobian leave holes nothing
+// reads). The pass's lane for Y cell k writes four values, H, N, M, L, to 4k .. 4k + 3 in one contiguous store: those
+// of cell k itself, or with a transpose map (tr non-empty) those of cell tr[k], which a symmetric Y-bus lets it form
+// from its own Y value and angle difference. Transposed, Jacobian column j's entries (Y column j's cells) sit in Y row
+// j's stretch, next to each other, where the refactor's ASSIGN rows gather a column's entries. Bus i's diagonal block
+// goes to its own cell's place, 4 y.diag[i] .. + 3, in the same stretch.
+int jacobian_value_positions(const YBus& y, const JacobianPattern& jp, const vector<int>& tr, vector<int>& pos);
 
 // scratch_cap > 0 caps the solves' scratch area (slots), so that passes run in several segments (for tests).
 // a_pos: the A part's position of each Jacobian CSR entry and a_slots its size (jacobian_value_positions); null: the

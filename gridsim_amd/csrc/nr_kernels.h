@@ -14,9 +14,12 @@ struct DevPlan {
   // network pattern
   const int32_t *y_rp, *y_diag;
   const int2* y_ij;                   // per Y cell: (row, column)
-  const double2* y_gb;                // per Y cell: base value (g, b) (instances patch at most kMaxPatch cells)
-  const int4* slots;                  // per Y cell: J CSR position of its H, N, M, L entries, or -1
-  const int4* aslots;                 // per Y cell: refactor record (aval index) of the same four entries, or -1
+  // per Y cell, then per bus again for its diagonal cell: base value (g, b) (instances patch at most kMaxPatch
+  // cells), and per Y cell's (bus's) lane the J CSR positions of the H, N, M, L entries it forms, and their A
+  // positions (-1: not in the Jacobian)
+  const double2* y_gb;
+  const int4* slots;
+  const int4* aslots;
   const int32_t *ppos, *qpos;         // equation numbering
   const int32_t *j_rp, *j_ci;         // Jacobian CSR pattern (for the FP64 refinement residual)
   // The wavefront schedule keeps the Newton step dx in pivot-step order (the triangular solves' own order, so the
@@ -32,7 +35,9 @@ struct DevPlan {
   // The refactor: one stream of rows, each one operation of the wavefront with one int4 record per lane (ASSIGN,
   // UPDATE, PIVOT, WRITE; model.h DevicePlanHost), padded with no-op rows to a multiple of kStreamPad rows plus two
   // more kStreamPad, so the kernel loads ahead without bounds checks. n_a: A entries (the first part of vals).
-  int32_t wstride, n_rs, n_a, pad_rs;
+  // y_tr: the lane of Y cell (i, j) forms the Jacobian values of cell (j, i) (a symmetric Y-bus; slots and aslots
+  // then name that cell's entries), so that a Jacobian column's values sit together.
+  int32_t wstride, n_rs, n_a, y_tr;
   // [(n_rs + kRsPadRings kStreamPad) / 2][64] pairs of rows: lane l of pair q holds (x, y) of rows 2q and 2q + 1;
   // y = 24-bit field | (kind | flags) << 24
   const int4* rs_rec;
@@ -156,7 +161,8 @@ struct InstDesc {
   int32_t n_patch, n_out;             // patched Y cells; outaged branches
   Patch patch[kMaxPatch];
   int32_t out[kMaxOutages];           // the outaged branches (their flows are not reported)
-  const double* y_gb_own;             // this instance's own Y values (g, b per cell), or null: the network's + patches
+  const double* y_gb_own;             // this instance's own Y values (laid out as DevPlan::y_gb), or null: the
+                                      // network's with the patches
   // per-instance buffers (device)
   double *vm, *va, *p, *q;
   double *f, *dx, *r, *jv;            // f, dx, residual, FP64 Jacobian values (CSR order)

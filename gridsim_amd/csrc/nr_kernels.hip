@@ -145,17 +145,25 @@ __device__ __forceinline__ void y_value(const DevPlan& P, const Inst<T>& I, int 
 typedef float v4f __attribute__((ext_vector_type(4)));
 
 // The four Jacobian values of one Y cell (or one bus's diagonal block) in the LU precision, to four consecutive
-// positions of vals: one contiguous store per lane, consecutive lanes' stores adjacent. The per-instance values each
-// Newton step writes (Jacobian, injection terms, factors) are stored non-temporal: hundreds of instances per XCD far
-// exceed its L2, so keeping them there would only evict the plan's record streams, which every instance reads.
-template <typename T>
+// positions of vals: one contiguous store per lane. The per-instance values each Newton step writes in whole lines
+// (Jacobian cells, injection terms, factors: consecutive lanes' stores adjacent) are stored non-temporal (NT):
+// hundreds of instances per XCD far exceed its L2, so keeping them there would only evict the plan's record streams,
+// which every instance reads.
+template <typename T, bool NT = true>
 __device__ __forceinline__ void store4(GLOBAL T* p, double h, double nn, double mm, double l) {
   if constexpr (sizeof(T) == 8) {
     GLOBAL v2d* q = (GLOBAL v2d*)p;
-    __builtin_nontemporal_store((v2d){h, nn}, q);
-    __builtin_nontemporal_store((v2d){mm, l}, q + 1);
+    if constexpr (NT) {
+      __builtin_nontemporal_store((v2d){h, nn}, q);
+      __builtin_nontemporal_store((v2d){mm, l}, q + 1);
+    } else {
+      q[0] = (v2d){h, nn};
+      q[1] = (v2d){mm, l};
+    }
   } else {
-    __builtin_nontemporal_store((v4f){(float)h, (float)nn, (float)mm, (float)l}, (GLOBAL v4f*)p);
+    const v4f v = {(float)h, (float)nn, (float)mm, (float)l};
+    if constexpr (NT) __builtin_nontemporal_store(v, (GLOBAL v4f*)p);
+    else *(GLOBAL v4f*)p = v;
   }
 }
 
@@ -168,10 +176,11 @@ __device__ __forceinline__ void put_csr(GLOBAL double* jv, const v4i& s, double 
 }
 
 // ------------------------------------------------------------------------------------------------ mismatch + J
-// VALS: the Jacobian goes, in the LU precision, to the A part of vals in Y-cell order (off-diagonal cell k's H, N, M,
-// L at 4k .. 4k + 3, bus i's diagonal block at 4 (nnz_y + i) ..; the refactor's ASSIGN rows gather them from there);
-// KEEP: also to the FP64 CSR copy (without it, the CSR positions are not even loaded). The cells' refactor positions
-// (aslots, -1 for an entry the Jacobian does not have) say which values count towards max |J|.
+// VALS: the Jacobian goes, in the LU precision, to the A part of vals in Y-cell order (cell k's lane writes H, N, M, L
+// to 4k .. 4k + 3: those of cell k, or with P.y_tr those of its transpose; bus i's diagonal block goes to its own
+// cell's place; the refactor's ASSIGN rows gather them from there); KEEP: also to the FP64 CSR copy (without it, the
+// CSR positions are not even loaded). The lanes' A positions (aslots, -1 for an entry the Jacobian does not have) say
+// which values count towards max |J|.
 // Every Y cell's pattern and base value are two loads: (row, column) and (g, b). Both passes walk blocks of 64 cells (buses): blocks
 // blk0, blk0 + stride, ... of the wave (one wave: 0, 1; a level-parallel grid: one block per work-group).
 
@@ -225,10 +234,16 @@ __device__ __forceinline__ int mismatch_cells(const DevPlan& P, const Inst<T>& I
       const double tq = vmj * (g * sn - b * cs);
       __builtin_nontemporal_store((v2d){tp, tq}, &I.tpq[k]);
       if (j != i) {
-        const double h = vmi * vmj * (g * sn - b * cs);
-        const double nn = vmi * (g * cs + b * sn);
-        const double mm = -vmi * vmj * (g * cs + b * sn);
-        const double l = vmi * (g * sn - b * cs);
+        // cell (r, c)'s entries from V_r, V_c and the sine of va_r - va_c: this cell's (r = i), or with y_tr its
+        // transpose's (r = j): the Y value is the same bit for bit, and sin(va_j - va_i) is -sin(va_i - va_j) exactly
+        // (an odd sine and an exact difference), or for a zero difference that difference itself, signed as it is
+        const bool tr = P.y_tr != 0;
+        const double vr = tr ? vmj : vmi, vc = tr ? vmi : vmj;
+        const double st = !tr ? sn : sn == 0.0 ? vaj - vai : 0.0 - sn;
+        const double h = vr * vc * (g * st - b * cs);
+        const double nn = vr * (g * cs + b * st);
+        const double mm = -vr * vc * (g * cs + b * st);
+        const double l = vr * (g * st - b * cs);
         if (KEEP) put_csr(I.jv, s, h, nn, mm, l);
         if (VALS) store4<T>(I.vals + 4 * (size_t)k, h, nn, mm, l);
         lbad |= !(isfinite(h) && isfinite(nn) && isfinite(mm) && isfinite(l));
@@ -271,9 +286,10 @@ __device__ __forceinline__ double mismatch_buses(const DevPlan& P, const Inst<T>
     const int k0 = nk0, k1 = nk1, kd = nkd, pp = npp, qq = nqq;
     const double vmi = nvm;
     load_bus(i + step);
-    const v2d gbd = y_gb[kd];
+    // (the diagonal cell's base value and slots repeated by bus after the cells': coalesced loads)
+    const v2d gbd = y_gb[P.nnz_y + i];
     const double gd0 = gbd.x, bd0 = gbd.y;
-    const v4i s = KEEP ? slots[kd] : (v4i)(-1), a = aslots[kd];
+    const v4i s = KEEP ? slots[P.nnz_y + i] : (v4i)(-1), a = aslots[P.nnz_y + i];
     const double ps = I.pspec[i], qs = I.qspec[i];
     double pi = 0.0, qi = 0.0;
     int k = k0;
@@ -293,7 +309,9 @@ __device__ __forceinline__ double mismatch_buses(const DevPlan& P, const Inst<T>
     const double mm = P_i - g * vmi * vmi;
     const double l = Q_i / vmi - b * vmi;
     if (KEEP) put_csr(I.jv, s, h, nn, mm, l);
-    if (VALS) store4<T>(I.vals + 4 * ((size_t)P.nnz_y + i), h, nn, mm, l);
+    // (plain stores: the lanes' diagonal blocks lie in their buses' stretches, one partial line each, which the L2
+    // merges with the neighbouring cells' values; streamed past it, each would cost a partial write of its own)
+    if (VALS) store4<T, false>(I.vals + 4 * (size_t)kd, h, nn, mm, l);
     lbad |= !(isfinite(h) && isfinite(nn) && isfinite(mm) && isfinite(l));
     jm = a.x >= 0 && fabs(h) > jm ? fabs(h) : jm;
     jm = a.y >= 0 && fabs(nn) > jm ? fabs(nn) : jm;

@@ -100,7 +100,7 @@ class GsPlanStats(C.Structure):
                 ("levels", C.c_int32), ("refactor_rows", C.c_int32), ("forward_steps", C.c_int32),
                 ("back_steps", C.c_int32), ("workspace_stride", C.c_int32), ("factor_slots", C.c_int32),
                 ("stream_bytes", C.c_int64), ("rows_by_kind", C.c_int32 * 5), ("update_lanes", C.c_int32),
-                ("forward_lanes", C.c_int32), ("back_lanes", C.c_int32)]
+                ("forward_lanes", C.c_int32), ("back_lanes", C.c_int32), ("jacobian_transposed", C.c_int32)]
 
 
 def _plan_stats_dict(st: "GsPlanStats") -> dict:

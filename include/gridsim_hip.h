@@ -290,6 +290,11 @@ typedef struct gs_plan_stats {
   int32_t update_lanes;
   /* solve steps' lanes that carry an entry (forward, back; of 64 per step) */
   int32_t forward_lanes, back_lanes;
+  /* 1: the Y-bus is symmetric bit for bit, so each Y cell's lane forms its transposed cell's Jacobian values and a
+   * Jacobian column's entries lie together where the refactor gathers them; 0: every lane forms its own cell's (a
+   * phase shifter, or a stamp whose zero is signed differently from its transpose's, as a reactance-only transformer's
+   * conductance can be) */
+  int32_t jacobian_transposed;
 } gs_plan_stats;
 GS_API gs_status gs_network_plan_stats(const gs_net* net, gs_plan_stats* out);
 

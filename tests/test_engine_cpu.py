@@ -3,6 +3,7 @@ plan) matches the oracle bit for bit. No GPU needed."""
 import ctypes as C
 import os
 import re
+from dataclasses import replace
 
 import numpy as np
 import pytest
@@ -41,7 +42,7 @@ def test_struct_sizes_match_the_header(eng):
     assert C.sizeof(eng.GsNetwork) == 4 * 3 + 4 + 8 + 24 * 8
     assert eng.SUMMARY_DTYPE.itemsize == 56
     assert C.sizeof(eng.GsServerConfig) == 32 and C.sizeof(eng.GsServerStats) == 6 * 8 + 5 * 8 + 2 * 8
-    assert C.sizeof(eng.GsPlanStats) == 10 * 4 + 8 + 8 * 4
+    assert C.sizeof(eng.GsPlanStats) == 10 * 4 + 8 + 9 * 4 + 4
 
 
 def test_no_device_status_without_a_gpu(eng):
@@ -148,6 +149,60 @@ def test_solve_passes_in_segments_emulated_on_host(eng, case, monkeypatch):
     st = plan.stats()
     assert st["forward_lanes"] == full["forward_lanes"] and st["back_lanes"] == full["back_lanes"]
     assert st["forward_steps"] + st["back_steps"] > full["forward_steps"] + full["back_steps"]
+    res = C.c_double()
+    why = C.create_string_buffer(128)
+    assert eng.lib().gs_plan_selfcheck(plan.h, C.byref(res), why) == eng.GS_OK, why.value.decode()
+    assert res.value < 1e-12
+
+
+def test_sine_is_odd_bit_for_bit(eng):
+    """The mismatch pass forms a transposed cell's Jacobian values with sin(va_j - va_i) taken as 0.0 - sin(va_i - va_j)
+    (and, for a zero difference, as that difference itself): FDLIBM's sine is odd and its cosine even bit for bit, and
+    IEEE subtraction is antisymmetric, including at huge arguments (Payne-Hanek reduction) and subnormals."""
+    rng = np.random.default_rng(7)
+    x = np.concatenate([rng.uniform(-4, 4, 20000), rng.uniform(-1e6, 1e6, 5000), 10.0 ** rng.uniform(-320, 300, 5000),
+                        [5e-324, 1e-300, np.pi, np.pi / 2, np.pi / 4, 1e22, 1.7e308]])
+    s1, c1 = eng.host_sincos(x)
+    s2, c2 = eng.host_sincos(-x)
+    assert np.array_equal((0.0 - s1).view(np.uint64), s2.view(np.uint64))
+    assert np.array_equal(c1.view(np.uint64), c2.view(np.uint64))
+    a, b = rng.uniform(-1, 1, 20000), rng.uniform(-1, 1, 20000)
+    assert np.array_equal((a - b).view(np.uint64), (0.0 - (b - a)).view(np.uint64))
+    z, _ = eng.host_sincos(np.array([0.0, -0.0]))
+    assert z.view(np.uint64).tolist() == np.array([0.0, -0.0]).view(np.uint64).tolist()
+
+
+def test_jacobian_transposed_only_for_a_bitwise_symmetric_ybus(eng):
+    """Cell lanes form their transposes' values only when every off-diagonal Y cell's stamps equal its transpose's bit
+    for bit: the synthetic grids (r > 0, shift 0) are; one phase shifter, or a reactance-only tapped branch (whose
+    conductance stamps are +0 one way and -0 the other), is not."""
+    from gridsim_amd import gridgen
+    net = gridgen.config1_grid()
+    assert eng.HostPlan(net).stats()["jacobian_transposed"] == 1
+    shifted = net.br_shift_deg.copy()
+    shifted[len(shifted) // 2] = 5.0
+    assert eng.HostPlan(replace(net, br_shift_deg=shifted)).stats()["jacobian_transposed"] == 0
+    r, tap = net.br_r.copy(), net.br_tap.copy()
+    r[3], tap[3] = 0.0, 0.98
+    assert eng.HostPlan(replace(net, br_r=r, br_tap=tap)).stats()["jacobian_transposed"] == 0
+    assert eng.HostPlan(matpower_case("case9")).stats()["jacobian_transposed"] == 0
+
+
+@pytest.mark.parametrize("transposed", [True, False])
+def test_transposed_fill_and_lu_of_a_warm_grid_emulated_on_host(eng, oracle, transposed):
+    """At a warm start point (nonzero angle differences) the self-check forms every lane's Jacobian values as the
+    mismatch kernel does and compares them bit for bit with each cell's own (transposed layout), then runs the device
+    LU emulation on them; with a phase shifter the grid keeps the untransposed layout and passes the same way."""
+    from gridsim_amd import gridgen
+    net = gridgen.config1_grid()
+    if not transposed:
+        shifted = net.br_shift_deg.copy()
+        shifted[7] = -3.0
+        net = replace(net, br_shift_deg=shifted)
+    rec, vm, va = oracle.solve(net, oracle.options(flat_start=True))
+    warm = net.with_start(vm, np.degrees(va))
+    plan = eng.HostPlan(warm, eng.NrOptions(flat_start=False))
+    assert plan.stats()["jacobian_transposed"] == int(transposed)
     res = C.c_double()
     why = C.create_string_buffer(128)
     assert eng.lib().gs_plan_selfcheck(plan.h, C.byref(res), why) == eng.GS_OK, why.value.decode()
