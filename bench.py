@@ -55,7 +55,7 @@ def measured_traffic(stats: dict, lu: str, flags: int, alg: int, kernel_ms: floa
         rec = json.load(fh)
     fp = rec["fingerprint"]
     mine = {k: stats.get(k) for k in ("m", "nnz_j", "nnz_l", "nnz_u", "refactor_rows", "forward_steps", "back_steps",
-                                      "factor_slots")}
+                                      "factor_slots", "jacobian_transposed")}
     mine.update(lu=lu, algorithmic_bytes=int(alg))
     if any(fp.get(k) != v for k, v in mine.items()):
         return None, None
