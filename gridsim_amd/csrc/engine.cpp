@@ -88,6 +88,19 @@ static void pack_cells(const YBus& y, std::vector<int>& ij, std::vector<double>&
   }
 }
 
+// Blocks of consecutive buses for the blocked mismatch pass: at most 64 buses (one per lane), whose rows hold at most
+// `cap` cells (the LDS scratch's two FP64 terms per cell); block b is buses [out[b], out[b + 1]). Empty (the two
+// passes run apart) when one row alone holds more.
+static std::vector<int> mismatch_blocks(const YBus& y, int cap) {
+  std::vector<int> out(1, 0);
+  for (int i = 0; i < y.n; i++) {
+    if (y.rp[i + 1] - y.rp[i] > cap) return {};
+    if (i - out.back() == 64 || y.rp[i + 1] - y.rp[out.back()] > cap) out.push_back(i);
+  }
+  out.push_back(y.n);
+  return out;
+}
+
 // Four slots per Y cell, then each bus's diagonal cell's four again, by bus (as pack_cells).
 static std::vector<int> with_diag_slots(const YBus& y, const std::vector<int>& cell_slots) {
   const size_t nnz = y.rp[y.n];
@@ -164,6 +177,14 @@ void build_plan_image(PlanImage& img, DevPlan& h, DevicePlanHost& d, const YBus*
     img.put(&DevPlan::y_rp, y->rp); img.put(&DevPlan::y_ij, y_ij); img.put(&DevPlan::y_diag, y->diag);
     img.put(&DevPlan::y_gb, y_gb); img.put(&DevPlan::slots, with_diag_slots(*y, slots));
     img.put(&DevPlan::aslots, with_diag_slots(*y, aslots));
+    // the blocked mismatch pass's bus blocks, for FP64 and FP32 workspaces (ws_slots x 8 or x 4 bytes, 16 per cell)
+    h.wstride = d.wstride;
+    const std::vector<int> mb8 = mismatch_blocks(*y, ws_slots(h) * 8 / 16);
+    const std::vector<int> mb4 = mismatch_blocks(*y, ws_slots(h) * 4 / 16);
+    h.n_mb = mb8.empty() ? 0 : (int)mb8.size() - 1;
+    h.n_mb4 = mb4.empty() ? 0 : (int)mb4.size() - 1;
+    if (h.n_mb) img.put(&DevPlan::mb_ptr, mb8);
+    if (h.n_mb4) img.put(&DevPlan::mb_ptr4, mb4);
     img.put(&DevPlan::ppos, jp.ppos); img.put(&DevPlan::qpos, jp.qpos);
     // the same unknowns in pivot-step order (where the wavefront schedule's dx keeps them)
     std::vector<int> qinv(jp.m, -1), ppos_k(jp.ppos.size(), -1), qpos_k(jp.qpos.size(), -1), j_ci_k(jp.ci.size());

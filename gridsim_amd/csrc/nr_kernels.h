@@ -54,11 +54,15 @@ struct DevPlan {
   const int32_t *rs_chunk_row, *fs_pass_step, *bs_pass_step;
   // Screening: per branch its end buses (from = -1: out of service), pi-model stamp (Yff, Yft, Ytf, Ytt as re, im)
   // and rate A; per bus its own voltage limits (NaN = the operational band).
-  int32_t nb, pad_nb;
+  int32_t nb, n_mb;
+  // The blocked mismatch pass's blocks of buses: block b is buses [mb_ptr[b], mb_ptr[b + 1]), at most 64, whose rows
+  // hold at most ws_slots x 8 / 16 cells (mb_ptr4, n_mb4: x 4 / 16, for FP32 workspaces); n_mb = 0: a row longer than
+  // that, or a network without the Newton pattern (the two passes then run apart, the terms through HBM)
+  const int32_t *mb_ptr, *mb_ptr4;
   // The group schedule's segments (x, y, z) = (first, end, split) of the refactor's chunks, then the forward and back
   // solves' passes: a level of two or more chunks (passes) split between the waves (z = 1), or a run of one-chunk levels
   // walked by one wave (z = 0).
-  int32_t n_gseg_rs, n_gseg_fs, n_gseg_bs, pad_gseg;
+  int32_t n_gseg_rs, n_gseg_fs, n_gseg_bs, n_mb4;
   const int4* gseg;
   const int32_t* j_diag;              // per Jacobian row: CSR position of its diagonal (the iterative fallback)
   const int2* br_ends;
@@ -86,7 +90,12 @@ constexpr int kTracks = 8, kTrackLanes = 8;
 // the refactor workspace: kTracks columns of wstride slots, then one spare slot per lane (model.h DevicePlanHost)
 constexpr int kWsSpare = 64;
 // slots of the refactor workspace (the triangular-solve vector follows it in LDS)
-__host__ __device__ inline int ws_slots(const DevPlan& p) { return kTracks * p.wstride + kWsSpare; }
+// (at least kFusedScratch slots: the blocked mismatch pass keeps a block's injection terms there, two FP64 per cell)
+constexpr int kFusedScratch = 512;
+__host__ __device__ inline int ws_slots(const DevPlan& p) {
+  const int w = kTracks * p.wstride + kWsSpare;   // even: wstride is odd
+  return w > kFusedScratch ? w : kFusedScratch;
+}
 // Entries of each factor array per instance. Every stream record's 24-bit field indexes the array its value load
 // reads (vals for the refactor rows, its factor part for the solve steps), also in the rows and steps that take no
 // value (pivot steps, solve rows: below m, and the factor part holds at least one slot per column), so the loads need

@@ -184,17 +184,32 @@ __device__ __forceinline__ void put_csr(GLOBAL double* jv, const v4i& s, double 
 // Every Y cell's pattern and base value are two loads: (row, column) and (g, b). Both passes walk blocks of 64 cells (buses): blocks
 // blk0, blk0 + stride, ... of the wave (one wave: 0, 1; a level-parallel grid: one block per work-group).
 
-// Every Y cell's two injection terms and its off-diagonal Jacobian entries, one cell per lane. Returns whether a
-// Jacobian value of this lane was not finite.
-template <typename T, bool KEEP, bool VALS = true>
-__device__ __forceinline__ int mismatch_cells(const DevPlan& P, const Inst<T>& I, int lane, int blk0, int stride,
-                                              double* jmax) {
+// Where the cells pass puts each Y cell's two injection terms and the bus pass reads them back: the instance's tpq
+// array in HBM (the two passes apart), or the wave's LDS scratch for one block of whole rows (the passes interleaved
+// block by block, mismatch_blocked).
+struct TermsGlobal {
+  GLOBAL v2d* t;
+  __device__ void put(int k, double tp, double tq) const { __builtin_nontemporal_store((v2d){tp, tq}, &t[k]); }
+  __device__ v2d get(int k) const { return t[k]; }
+};
+struct TermsLds {
+  LDSP double* p;
+  LDSP double* q;
+  int base;   // the block's first cell
+  __device__ void put(int k, double tp, double tq) const { p[k - base] = tp; q[k - base] = tq; }
+  __device__ v2d get(int k) const { return (v2d){p[k - base], q[k - base]}; }
+};
+
+// The two injection terms and off-diagonal Jacobian entries of the Y cells [k_lo, k_hi), one cell per lane. Returns
+// whether a Jacobian value of this lane was not finite.
+template <typename T, bool KEEP, bool VALS = true, typename TS = TermsGlobal>
+__device__ __forceinline__ int mismatch_cells(const DevPlan& P, const Inst<T>& I, int lane, int k_lo, int k_hi,
+                                              int blk0, int stride, const TS& terms, double* jmax) {
   GLOBAL const v2i* y_ij = globv<v2i>(P.y_ij);
   GLOBAL const v2d* y_gb = I.ygb;
   GLOBAL const v4i* slots = globv<v4i>(P.slots);
   GLOBAL const v4i* aslots = globv<v4i>(P.aslots);
-  const int nnz = P.nnz_y;
-  const int nblk = (nnz + kWave - 1) / kWave;
+  const int nblk = (k_hi - k_lo + kWave - 1) / kWave;
   int lbad = 0;
   double jm = 0.0;
   // stage 0 holds the pattern of block t + 2 (lane-strided), stage 1 the gathered state of block t + 1
@@ -204,8 +219,8 @@ __device__ __forceinline__ int mismatch_cells(const DevPlan& P, const Inst<T>& I
   v4i s1 = (v4i)(-1), a1 = s1, s2 = s1, a2 = s1;
   double vai1 = 0, vaj1 = 0, vmi1 = 0, vmj1 = 0;
   auto load_pattern = [&](int t, int& i, int& j, double& g, double& b, v4i& s, v4i& a) {
-    const int k = lane + (blk0 + t * stride) * kWave;
-    const int kk = k < nnz ? k : 0;
+    const int k = k_lo + lane + (blk0 + t * stride) * kWave;
+    const int kk = k < k_hi ? k : 0;
     const v2i ij = y_ij[kk];
     const v2d gb = y_gb[kk];
     i = ij.x; j = ij.y; g = gb.x; b = gb.y; a = aslots[kk];
@@ -217,7 +232,7 @@ __device__ __forceinline__ int mismatch_cells(const DevPlan& P, const Inst<T>& I
   vai1 = I.va[i1]; vaj1 = I.va[j1]; vmi1 = I.vm[i1]; vmj1 = I.vm[j1];
   for (int t = 0; t < iters; t++) {
     // this cell
-    const int k = lane + (blk0 + t * stride) * kWave;
+    const int k = k_lo + lane + (blk0 + t * stride) * kWave;
     const int i = i1, j = j1;
     const double g0 = g1, b0 = b1, vai = vai1, vaj = vaj1, vmi = vmi1, vmj = vmj1;
     const v4i s = s1, a = a1;
@@ -225,14 +240,14 @@ __device__ __forceinline__ int mismatch_cells(const DevPlan& P, const Inst<T>& I
     i1 = i2; j1 = j2; g1 = g2; b1 = b2; s1 = s2; a1 = a2;
     vai1 = I.va[i1]; vaj1 = I.va[j1]; vmi1 = I.vm[i1]; vmj1 = I.vm[j1];
     load_pattern(t + 2, i2, j2, g2, b2, s2, a2);
-    if (k < nnz) {
+    if (k < k_hi) {
       double g, b;
       y_value(P, I, k, g0, b0, &g, &b);
       double sn, cs;
       gs_fdlibm::sincos(vai - vaj, &sn, &cs);
       const double tp = vmj * (g * cs + b * sn);
       const double tq = vmj * (g * sn - b * cs);
-      __builtin_nontemporal_store((v2d){tp, tq}, &I.tpq[k]);
+      terms.put(k, tp, tq);
       if (j != i) {
         // cell (r, c)'s entries from V_r, V_c and the sine of va_r - va_c: this cell's (r = i), or with y_tr its
         // transpose's (r = j): the Y value is the same bit for bit, and sin(va_j - va_i) is -sin(va_i - va_j) exactly
@@ -259,10 +274,11 @@ __device__ __forceinline__ int mismatch_cells(const DevPlan& P, const Inst<T>& I
 }
 
 // Per bus the ordered sums of its row's terms, the diagonal Jacobian entries and the mismatch; the next bus's
-// scalars are loaded ahead. Returns this lane's max |f| (NaN-propagating); *bad: a Jacobian value was not finite.
-template <typename T, bool KEEP, bool VALS = true>
-__device__ __forceinline__ double mismatch_buses(const DevPlan& P, const Inst<T>& I, int lane, int blk0, int stride,
-                                                 int* bad, double* jmax) {
+// scalars are loaded ahead; buses [i_lo, i_hi). Returns this lane's max |f| (NaN-propagating); *bad: a Jacobian value
+// was not finite.
+template <typename T, bool KEEP, bool VALS = true, typename TS = TermsGlobal>
+__device__ __forceinline__ double mismatch_buses(const DevPlan& P, const Inst<T>& I, int lane, int i_lo, int i_hi,
+                                                 int blk0, int stride, const TS& terms, int* bad, double* jmax) {
   GLOBAL const v2d* y_gb = I.ygb;
   GLOBAL const v4i* slots = globv<v4i>(P.slots);
   GLOBAL const v4i* aslots = globv<v4i>(P.aslots);
@@ -270,19 +286,18 @@ __device__ __forceinline__ double mismatch_buses(const DevPlan& P, const Inst<T>
   GLOBAL const int* y_diag = glob(P.y_diag);
   GLOBAL const int* ppos = glob(P.ppos);
   GLOBAL const int* qpos = glob(P.qpos);
-  const int n = P.n;
   double lmax = 0.0, jm = 0.0;
   int lbad = 0;
-  int i = lane + blk0 * kWave;
+  int i = i_lo + lane + blk0 * kWave;
   const int step = stride * kWave;
   int nk0 = 0, nk1 = 0, nkd = 0, npp = -1, nqq = -1;
   double nvm = 0;
   auto load_bus = [&](int ii) {
-    const int c = ii < n ? ii : 0;
+    const int c = ii < i_hi ? ii : 0;
     nk0 = y_rp[c]; nk1 = y_rp[c + 1]; nkd = y_diag[c]; npp = ppos[c]; nqq = qpos[c]; nvm = I.vm[c];
   };
   load_bus(i);
-  for (; i < n; i += step) {
+  for (; i < i_hi; i += step) {
     const int k0 = nk0, k1 = nk1, kd = nkd, pp = npp, qq = nqq;
     const double vmi = nvm;
     load_bus(i + step);
@@ -294,11 +309,11 @@ __device__ __forceinline__ double mismatch_buses(const DevPlan& P, const Inst<T>
     double pi = 0.0, qi = 0.0;
     int k = k0;
     for (; k + 2 <= k1; k += 2) {
-      const v2d u = I.tpq[k], v = I.tpq[k + 1];
+      const v2d u = terms.get(k), v = terms.get(k + 1);
       pi += u.x; qi += u.y;
       pi += v.x; qi += v.y;
     }
-    if (k < k1) { const v2d u = I.tpq[k]; pi += u.x; qi += u.y; }
+    if (k < k1) { const v2d u = terms.get(k); pi += u.x; qi += u.y; }
     const double P_i = vmi * pi, Q_i = vmi * qi;
     I.p[i] = P_i;
     I.q[i] = Q_i;
@@ -325,17 +340,64 @@ __device__ __forceinline__ double mismatch_buses(const DevPlan& P, const Inst<T>
   return lmax;
 }
 
-// The whole pass for one wave. Returns max |f| (NaN-propagating) over the wave; sets *bad when a Jacobian value is
-// not finite.
+// Both passes interleaved over the plan's blocks of whole rows (P.mb_ptr for FP64, mb_ptr4 for FP32: consecutive
+// buses, at most 64, whose rows hold at most as many cells as the scratch takes; blocks blk0, blk0 + stride, ...):
+// the cells pass puts the block's injection terms in the wave's LDS scratch (scr: the refactor workspace, free here),
+// and the bus pass sums them from there, so the terms never leave the CU. Returns this lane's max |f|
+// (NaN-propagating); *bad: a Jacobian value was not finite; *jmax: this lane's max |J|.
+template <typename T, bool KEEP, bool VALS = true>
+__device__ __forceinline__ double mismatch_blocked(const DevPlan& P, const Inst<T>& I, int lane, int blk0, int stride,
+                                                   LDSP double* scr, int* bad, double* jmax) {
+  constexpr bool kF32 = sizeof(T) == 4;
+  GLOBAL const int* mb = glob(kF32 ? P.mb_ptr4 : P.mb_ptr);
+  const int nmb = kF32 ? P.n_mb4 : P.n_mb;
+  const int cap = ws_slots(P) * (int)sizeof(T) / 16;   // cells per block at most: two FP64 terms each
+  GLOBAL const int* y_rp = glob(P.y_rp);
+  int lbad = 0;
+  double jm = 0.0, lmax = 0.0;
+  for (int b = blk0; b < nmb; b += stride) {
+    const int i0 = mb[b], i1 = mb[b + 1];
+    const int c0 = y_rp[i0], c1 = y_rp[i1];
+    const TermsLds ts{scr, scr + cap, c0};
+    double jc, jb;
+    int bb;
+    lbad |= mismatch_cells<T, KEEP, VALS>(P, I, lane, c0, c1, 0, 1, ts, &jc);
+    WAVE_SYNC();
+    lmax = nan_max(lmax, mismatch_buses<T, KEEP, VALS>(P, I, lane, i0, i1, 0, 1, ts, &bb, &jb));
+    lbad |= bb;
+    jm = jc > jm ? jc : jm;
+    jm = jb > jm ? jb : jm;
+    WAVE_SYNC();   // (the next block rewrites the scratch: this block's reads first)
+  }
+  *bad = lbad;
+  *jmax = jm;
+  return lmax;
+}
+
+// Whether the blocked form runs: the plan has blocks for this precision (none: a row longer than the scratch).
+template <typename T>
+__device__ __forceinline__ bool blocked_mismatch(const DevPlan& P) { return (sizeof(T) == 4 ? P.n_mb4 : P.n_mb) > 0; }
+
+// The whole pass for one wave (blocked when the plan has blocks and the caller gives the LDS scratch). Returns max |f|
+// (NaN-propagating) over the wave; sets *bad when a Jacobian value is not finite.
 // *jnorm: the largest |entry| of the Jacobian (entries that are finite).
 template <typename T, bool KEEP, bool VALS = true>
 __device__ __forceinline__ double mismatch_jacobian(const DevPlan& P, const Inst<T>& I, int lane, int* bad,
-                                                   double* jnorm) {
+                                                   double* jnorm, LDSP double* scr = nullptr) {
+  if (scr != nullptr && blocked_mismatch<T>(P)) {
+    double jm;
+    int fbad;
+    const double lmax = mismatch_blocked<T, KEEP, VALS>(P, I, lane, 0, 1, scr, &fbad, &jm);
+    *bad = wave_any(fbad);
+    *jnorm = wave_max(jm);
+    return wave_max(lmax);
+  }
+  const TermsGlobal tg{I.tpq};
   double jc, jb;
-  const int cbad = mismatch_cells<T, KEEP, VALS>(P, I, lane, 0, 1, &jc);
+  const int cbad = mismatch_cells<T, KEEP, VALS>(P, I, lane, 0, P.nnz_y, 0, 1, tg, &jc);
   WG_SYNC();
   int bbad;
-  const double lmax = mismatch_buses<T, KEEP, VALS>(P, I, lane, 0, 1, &bbad, &jb);
+  const double lmax = mismatch_buses<T, KEEP, VALS>(P, I, lane, 0, P.n, 0, 1, tg, &bbad, &jb);
   *bad = wave_any(cbad | bbad);
   *jnorm = wave_max(jc > jb ? jc : jb);
   return wave_max(lmax);
@@ -767,8 +829,9 @@ __global__ __launch_bounds__(kWave * (PAIR ? kPairWaves : 1), 2) void nr_batch_k
     if (steps >= prm.steps_this_launch) { paused = 1; break; }
     int bad;
     double jnorm;
-    max_mis = prm.refine_steps > 0 ? mismatch_jacobian<T, true>(P, I, opaque_lane(lane), &bad, &jnorm)
-                                   : mismatch_jacobian<T, false>(P, I, opaque_lane(lane), &bad, &jnorm);
+    LDSP double* mscr = reinterpret_cast<LDSP double*>(ws);   // (the workspace is free between refactors)
+    max_mis = prm.refine_steps > 0 ? mismatch_jacobian<T, true>(P, I, opaque_lane(lane), &bad, &jnorm, mscr)
+                                   : mismatch_jacobian<T, false>(P, I, opaque_lane(lane), &bad, &jnorm, mscr);
     WG_SYNC();
     tick(t_mis);
     if (iter == 0) start_mis = max_mis;
@@ -928,8 +991,10 @@ __global__ __launch_bounds__(kWave) void wide_cells(const InstDesc* __restrict__
   if (st->done) return;
   const Inst<T> I = load_inst<T>(descs, P, blockIdx.y);
   double jm;
-  const int bad = prm.refine_steps > 0 ? mismatch_cells<T, true>(P, I, threadIdx.x, blockIdx.x, gridDim.x, &jm)
-                                       : mismatch_cells<T, false>(P, I, threadIdx.x, blockIdx.x, gridDim.x, &jm);
+  const TermsGlobal tg{I.tpq};
+  const int bad = prm.refine_steps > 0
+                      ? mismatch_cells<T, true>(P, I, threadIdx.x, 0, P.nnz_y, blockIdx.x, gridDim.x, tg, &jm)
+                      : mismatch_cells<T, false>(P, I, threadIdx.x, 0, P.nnz_y, blockIdx.x, gridDim.x, tg, &jm);
   const int anybad = wave_any(bad);
   jm = wave_max(jm);
   if (threadIdx.x == 0) {
@@ -947,8 +1012,10 @@ __global__ __launch_bounds__(kWave) void wide_buses(const InstDesc* __restrict__
   int bad;
   double jm;
   const double lmax = wave_max(prm.refine_steps > 0
-                                   ? mismatch_buses<T, true>(P, I, threadIdx.x, blockIdx.x, gridDim.x, &bad, &jm)
-                                   : mismatch_buses<T, false>(P, I, threadIdx.x, blockIdx.x, gridDim.x, &bad, &jm));
+                                   ? mismatch_buses<T, true>(P, I, threadIdx.x, 0, P.n, blockIdx.x, gridDim.x,
+                                                             TermsGlobal{I.tpq}, &bad, &jm)
+                                   : mismatch_buses<T, false>(P, I, threadIdx.x, 0, P.n, blockIdx.x, gridDim.x,
+                                                              TermsGlobal{I.tpq}, &bad, &jm));
   const int anybad = wave_any(bad);
   jm = wave_max(jm);
   if (threadIdx.x == 0) {
@@ -1279,14 +1346,27 @@ __global__ __launch_bounds__(kWave * kGroupMaxWaves) void group_kernel(const Ins
     if (steps >= prm.steps_this_launch) { paused = 1; break; }
     double jc, jb;
     int bad;
-    const int cbad = prm.refine_steps > 0 ? mismatch_cells<T, true>(P, I, opaque_lane(lane), wave, W, &jc)
-                                          : mismatch_cells<T, false>(P, I, opaque_lane(lane), wave, W, &jc);
-    __syncthreads();
-    max_mis = wave_max(prm.refine_steps > 0
-                           ? mismatch_buses<T, true>(P, I, opaque_lane(lane), wave, W, &bad, &jb)
-                           : mismatch_buses<T, false>(P, I, opaque_lane(lane), wave, W, &bad, &jb));
-    bad = wave_any(cbad | bad);
-    double jnorm = wave_max(jc > jb ? jc : jb);
+    double jnorm;
+    if (blocked_mismatch<T>(P)) {
+      // blocked: each wave its own blocks of whole rows and its own scratch (no barrier between the halves)
+      LDSP double* mscr = reinterpret_cast<LDSP double*>(ws);
+      max_mis = wave_max(prm.refine_steps > 0
+                             ? mismatch_blocked<T, true>(P, I, opaque_lane(lane), wave, W, mscr, &bad, &jc)
+                             : mismatch_blocked<T, false>(P, I, opaque_lane(lane), wave, W, mscr, &bad, &jc));
+      bad = wave_any(bad);
+      jnorm = wave_max(jc);
+    } else {
+      const TermsGlobal tg{I.tpq};
+      const int cbad = prm.refine_steps > 0
+                           ? mismatch_cells<T, true>(P, I, opaque_lane(lane), 0, P.nnz_y, wave, W, tg, &jc)
+                           : mismatch_cells<T, false>(P, I, opaque_lane(lane), 0, P.nnz_y, wave, W, tg, &jc);
+      __syncthreads();
+      max_mis = wave_max(prm.refine_steps > 0
+                             ? mismatch_buses<T, true>(P, I, opaque_lane(lane), 0, P.n, wave, W, tg, &bad, &jb)
+                             : mismatch_buses<T, false>(P, I, opaque_lane(lane), 0, P.n, wave, W, tg, &bad, &jb));
+      bad = wave_any(cbad | bad);
+      jnorm = wave_max(jc > jb ? jc : jb);
+    }
     group_reduce(g, set, wave, lane, W, &jnorm, &max_mis, &bad, false);   // (also publishes f, p, q, J)
     tick(t_mis);
     if (iter == 0) start_mis = max_mis;
