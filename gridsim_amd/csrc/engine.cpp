@@ -177,8 +177,12 @@ void build_plan_image(PlanImage& img, DevPlan& h, DevicePlanHost& d, const YBus*
     img.put(&DevPlan::y_rp, y->rp); img.put(&DevPlan::y_ij, y_ij); img.put(&DevPlan::y_diag, y->diag);
     img.put(&DevPlan::y_gb, y_gb); img.put(&DevPlan::slots, with_diag_slots(*y, slots));
     img.put(&DevPlan::aslots, with_diag_slots(*y, aslots));
-    // the blocked mismatch pass's bus blocks, for FP64 and FP32 workspaces (ws_slots x 8 or x 4 bytes, 16 per cell)
+    // the blocked mismatch pass's bus blocks, for FP64 and FP32 workspaces (ws_slots x 8 or x 4 bytes, 16 per cell),
+    // with the workspace grown where one row alone would not fit it
     h.wstride = d.wstride;
+    int longest = 0;
+    for (int i = 0; i < y->n; i++) longest = std::max(longest, y->rp[i + 1] - y->rp[i]);
+    h.ws_min = std::max(kFusedScratch, 4 * longest);
     const std::vector<int> mb8 = mismatch_blocks(*y, ws_slots(h) * 8 / 16);
     const std::vector<int> mb4 = mismatch_blocks(*y, ws_slots(h) * 4 / 16);
     h.n_mb = mb8.empty() ? 0 : (int)mb8.size() - 1;

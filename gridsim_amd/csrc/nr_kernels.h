@@ -46,7 +46,9 @@ struct DevPlan {
   // (start | len << 16, row | (kind | flags) << 24) steps (model.h DevicePlanHost). The factors live where the
   // refactor's WRITE rows stored them, 64 per WRITE row (the factor part of vals, n_fac slots after the n_a A values;
   // a PRODUCT step's value index is a position there), the pivots also in dval by pivot step.
-  int32_t fs_steps, bs_steps, n_fac, pad_fac;
+  // ws_min: the refactor workspace's slots at least (the blocked mismatch pass's scratch: four slots per cell of the
+  // longest Y row, at least kFusedScratch)
+  int32_t fs_steps, bs_steps, n_fac, ws_min;
   const int2 *fs_rec, *bs_rec;
   // Level-parallel schedules: first row of each refactor chunk (+ the end), first step of each solve pass (+ the end).
   // The streams carry two kStreamPad rings of padding, so one chunk's or pass's range rounded up to whole rings can
@@ -56,8 +58,8 @@ struct DevPlan {
   // and rate A; per bus its own voltage limits (NaN = the operational band).
   int32_t nb, n_mb;
   // The blocked mismatch pass's blocks of buses: block b is buses [mb_ptr[b], mb_ptr[b + 1]), at most 64, whose rows
-  // hold at most ws_slots x 8 / 16 cells (mb_ptr4, n_mb4: x 4 / 16, for FP32 workspaces); n_mb = 0: a row longer than
-  // that, or a network without the Newton pattern (the two passes then run apart, the terms through HBM)
+  // hold at most ws_slots x 8 / 16 cells (mb_ptr4, n_mb4: x 4 / 16, for FP32 workspaces); n_mb = 0: a network without
+  // the Newton pattern (the wavefront and work-group schedules do not run then)
   const int32_t *mb_ptr, *mb_ptr4;
   // The group schedule's segments (x, y, z) = (first, end, split) of the refactor's chunks, then the forward and back
   // solves' passes: a level of two or more chunks (passes) split between the waves (z = 1), or a run of one-chunk levels
@@ -90,11 +92,12 @@ constexpr int kTracks = 8, kTrackLanes = 8;
 // the refactor workspace: kTracks columns of wstride slots, then one spare slot per lane (model.h DevicePlanHost)
 constexpr int kWsSpare = 64;
 // slots of the refactor workspace (the triangular-solve vector follows it in LDS)
-// (at least kFusedScratch slots: the blocked mismatch pass keeps a block's injection terms there, two FP64 per cell)
+// (at least ws_min slots: the blocked mismatch pass keeps a block's injection terms there, two FP64 per cell, and
+// every Y row fits a block)
 constexpr int kFusedScratch = 512;
 __host__ __device__ inline int ws_slots(const DevPlan& p) {
-  const int w = kTracks * p.wstride + kWsSpare;   // even: wstride is odd
-  return w > kFusedScratch ? w : kFusedScratch;
+  const int w = kTracks * p.wstride + kWsSpare;   // even: wstride is odd; ws_min even
+  return w > p.ws_min ? w : p.ws_min;
 }
 // Entries of each factor array per instance. Every stream record's 24-bit field indexes the array its value load
 // reads (vals for the refactor rows, its factor part for the solve steps), also in the rows and steps that take no

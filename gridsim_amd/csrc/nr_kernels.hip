@@ -374,24 +374,12 @@ __device__ __forceinline__ double mismatch_blocked(const DevPlan& P, const Inst<
   return lmax;
 }
 
-// Whether the blocked form runs: the plan has blocks for this precision (none: a row longer than the scratch).
-template <typename T>
-__device__ __forceinline__ bool blocked_mismatch(const DevPlan& P) { return (sizeof(T) == 4 ? P.n_mb4 : P.n_mb) > 0; }
-
-// The whole pass for one wave (blocked when the plan has blocks and the caller gives the LDS scratch). Returns max |f|
-// (NaN-propagating) over the wave; sets *bad when a Jacobian value is not finite.
+// The two passes apart, the terms through HBM (the iterative fallback's kernel). Returns max |f| (NaN-propagating) over
+// the wave; sets *bad when a Jacobian value is not finite.
 // *jnorm: the largest |entry| of the Jacobian (entries that are finite).
 template <typename T, bool KEEP, bool VALS = true>
 __device__ __forceinline__ double mismatch_jacobian(const DevPlan& P, const Inst<T>& I, int lane, int* bad,
-                                                   double* jnorm, LDSP double* scr = nullptr) {
-  if (scr != nullptr && blocked_mismatch<T>(P)) {
-    double jm;
-    int fbad;
-    const double lmax = mismatch_blocked<T, KEEP, VALS>(P, I, lane, 0, 1, scr, &fbad, &jm);
-    *bad = wave_any(fbad);
-    *jnorm = wave_max(jm);
-    return wave_max(lmax);
-  }
+                                                   double* jnorm) {
   const TermsGlobal tg{I.tpq};
   double jc, jb;
   const int cbad = mismatch_cells<T, KEEP, VALS>(P, I, lane, 0, P.nnz_y, 0, 1, tg, &jc);
@@ -829,9 +817,16 @@ __global__ __launch_bounds__(kWave * (PAIR ? kPairWaves : 1), 2) void nr_batch_k
     if (steps >= prm.steps_this_launch) { paused = 1; break; }
     int bad;
     double jnorm;
-    LDSP double* mscr = reinterpret_cast<LDSP double*>(ws);   // (the workspace is free between refactors)
-    max_mis = prm.refine_steps > 0 ? mismatch_jacobian<T, true>(P, I, opaque_lane(lane), &bad, &jnorm, mscr)
-                                   : mismatch_jacobian<T, false>(P, I, opaque_lane(lane), &bad, &jnorm, mscr);
+    {
+      // blocked (the plan's blocks fit every row; the workspace is free between refactors)
+      LDSP double* mscr = reinterpret_cast<LDSP double*>(ws);
+      double jm;
+      max_mis = wave_max(prm.refine_steps > 0
+                             ? mismatch_blocked<T, true>(P, I, opaque_lane(lane), 0, 1, mscr, &bad, &jm)
+                             : mismatch_blocked<T, false>(P, I, opaque_lane(lane), 0, 1, mscr, &bad, &jm));
+      bad = wave_any(bad);
+      jnorm = wave_max(jm);
+    }
     WG_SYNC();
     tick(t_mis);
     if (iter == 0) start_mis = max_mis;
@@ -1344,10 +1339,10 @@ __global__ __launch_bounds__(kWave * kGroupMaxWaves) void group_kernel(const Ins
   };
   for (;;) {
     if (steps >= prm.steps_this_launch) { paused = 1; break; }
-    double jc, jb;
+    double jc;
     int bad;
     double jnorm;
-    if (blocked_mismatch<T>(P)) {
+    {
       // blocked: each wave its own blocks of whole rows and its own scratch (no barrier between the halves)
       LDSP double* mscr = reinterpret_cast<LDSP double*>(ws);
       max_mis = wave_max(prm.refine_steps > 0
@@ -1355,17 +1350,6 @@ __global__ __launch_bounds__(kWave * kGroupMaxWaves) void group_kernel(const Ins
                              : mismatch_blocked<T, false>(P, I, opaque_lane(lane), wave, W, mscr, &bad, &jc));
       bad = wave_any(bad);
       jnorm = wave_max(jc);
-    } else {
-      const TermsGlobal tg{I.tpq};
-      const int cbad = prm.refine_steps > 0
-                           ? mismatch_cells<T, true>(P, I, opaque_lane(lane), 0, P.nnz_y, wave, W, tg, &jc)
-                           : mismatch_cells<T, false>(P, I, opaque_lane(lane), 0, P.nnz_y, wave, W, tg, &jc);
-      __syncthreads();
-      max_mis = wave_max(prm.refine_steps > 0
-                             ? mismatch_buses<T, true>(P, I, opaque_lane(lane), 0, P.n, wave, W, tg, &bad, &jb)
-                             : mismatch_buses<T, false>(P, I, opaque_lane(lane), 0, P.n, wave, W, tg, &bad, &jb));
-      bad = wave_any(cbad | bad);
-      jnorm = wave_max(jc > jb ? jc : jb);
     }
     group_reduce(g, set, wave, lane, W, &jnorm, &max_mis, &bad, false);   // (also publishes f, p, q, J)
     tick(t_mis);
@@ -1785,6 +1769,7 @@ hipError_t launch_nr_batch(int lu_fp32, int x_in_lds, int pair, const InstDesc* 
   NrParams prm = prm0;
   prm.n_inst = count;
   if (pair && !x_in_lds) return hipErrorInvalidValue;
+  if ((lu_fp32 ? plan.n_mb4 : plan.n_mb) <= 0) return hipErrorInvalidValue;   // (the blocked mismatch pass's blocks)
   const dim3 grid(pair ? (count + kPairWaves - 1) / kPairWaves : count), block(pair ? kWave * kPairWaves : kWave);
   if (lu_fp32) {
     if (pair)
@@ -1809,6 +1794,7 @@ hipError_t launch_group(int lu_fp32, int x_in_lds, int waves, const InstDesc* de
   NrParams prm = prm0;
   prm.n_inst = count;
   if (waves < 1 || waves > kGroupMaxWaves) return hipErrorInvalidValue;
+  if ((lu_fp32 ? plan.n_mb4 : plan.n_mb) <= 0) return hipErrorInvalidValue;   // (the blocked mismatch pass's blocks)
   const dim3 grid(count), block(kWave * waves);
   if (lu_fp32) {
     if (x_in_lds)
