@@ -219,6 +219,9 @@ void build_plan_image(PlanImage& img, DevPlan& h, DevicePlanHost& d, const YBus*
       for (int c = 0; c < 2; c++)
         rs_rec[(((size_t)(r / 2) * 64 + l) * 2 + (r % 2)) * 2 + c] = d.rs_rec[((size_t)r * 64 + l) * 2 + c];
   img.put(&DevPlan::rs_rec, rs_rec); img.put(&DevPlan::ch_a_src, d.ch_a_src);
+  std::vector<int> wr0(d.rs_wr0);
+  wr0.resize(rs_rows, d.n_wrows);
+  img.put(&DevPlan::rs_wr0, wr0);
   // the solves' step streams (padded with steps that touch nothing) and the factor slots (64 per WRITE row)
   h.fs_steps = ring_up(d.fs_steps);
   h.bs_steps = ring_up(d.bs_steps);

@@ -1048,11 +1048,28 @@ void compile_plan(const Plan& plan, DevicePlanHost& d, int scratch_cap, const ve
           wrows = std::max(wrows, (nu + nl - 1 + TL - 1) / TL);
         }
       }
-      emit(D::kRsPivot | width, lv, rec.data());
+      // the PIVOT row also does the chunk's write row 0 (its lanes' kinds and positions derived from the PIVOT record:
+      // checked here against the WRITE record the row replaces)
+      for (int l = 0; l < 64; l++) {
+        const int g = l / TL, t = l % TL;
+        int want_wk = 0, want_k = 0;
+        if (g < ntr) {
+          const int k = d.tr_col[t0 + g], nu = plan.up[k + 1] - plan.up[k], nl = plan.lp[k + 1] - plan.lp[k];
+          if (t < nu + nl - 1) {
+            want_wk = t < nu - 1 ? D::kWriteU : t == nu - 1 ? D::kWritePivot : D::kWriteL;
+            want_k = k;
+          }
+        }
+        const D::WriteLane wl = d.pivot_write_lane(rec[2 * l], rec[2 * l + 1], l, TL, n_wr);
+        if (wl.wk != want_wk || (want_wk == D::kWritePivot && wl.k != want_k))
+          throw std::runtime_error("plan: a PIVOT row's write differs from its WRITE record");
+      }
+      emit(D::kRsPivot | D::kRsPivotWrite | width, lv, rec.data());
+      n_wr++;
       // WRITE rows: (kind << 24 | pivot step for the pivot's lane, vals position); write row w of a chunk writes slot
       // t = w * TL + lane % TL of the lane's track (TL lanes per track) to the stream's next 64 factor positions, an
-      // idle lane included (its position is its own and never read)
-      for (int w = 0; w < wrows; w++) {
+      // idle lane included (its position is its own and never read); row 0 is the PIVOT row's
+      for (int w = 1; w < wrows; w++) {
         for (int l = 0; l < 64; l++) {
           const int g = l / TL, t = w * TL + l % TL;
           rec[2 * l] = 0;
@@ -1073,6 +1090,12 @@ void compile_plan(const Plan& plan, DevicePlanHost& d, int scratch_cap, const ve
     }
     if (n_wr != d.n_wrows) throw std::runtime_error("plan: WRITE rows differ from the factor layout");
     d.n_rs = (int)d.rs_meta.size();
+    d.rs_wr0.assign(d.n_rs, 0);
+    for (int r = 0, c = 0; r < d.n_rs; r++) {
+      d.rs_wr0[r] = c;
+      const int k = d.rs_meta[r] & D::kRsKindMask;
+      c += k == D::kRsWrite || (k == D::kRsPivot && (d.rs_meta[r] & D::kRsPivotWrite));
+    }
     d.rs_chunk_row.push_back(d.n_rs);
     while ((int)d.rs_level_chunk.size() <= n_levels) d.rs_level_chunk.push_back(d.n_chunks);
     // fold each row's kind and flags into its records
@@ -1133,17 +1156,22 @@ bool check_level_schedule(const DevicePlanHost& d, int ring, int padded_rows, in
     // then the level's writes: disjoint between its chunks
     for (int c = d.rs_level_chunk[l]; c < d.rs_level_chunk[l + 1]; c++) {
       for (int r = d.rs_chunk_row[c]; r < d.rs_chunk_row[c + 1]; r++) {
-        if ((d.rs_meta[r] & D::kRsKindMask) != D::kRsWrite) continue;
+        const int rk = d.rs_meta[r] & D::kRsKindMask;
+        const bool pw = rk == D::kRsPivot && (d.rs_meta[r] & D::kRsPivotWrite);
+        if (rk != D::kRsWrite && !pw) continue;
         for (int lane = 0; lane < 64; lane++) {
           const int* rr = &d.rs_rec[((size_t)r * 64 + lane) * 2];
-          const int wk = (int)((unsigned)rr[0] >> 24), pos = rr[1] & D::kRsIndexMask;
+          const D::WriteLane wl = pw ? d.pivot_write_lane(rr[0], rr[1], lane, D::track_lanes(d.rs_meta[r]), d.rs_wr0[r])
+                                     : D::WriteLane{(int)((unsigned)rr[0] >> 24), rr[0] & D::kRsIndexMask,
+                                                    rr[1] & D::kRsIndexMask};
+          const int wk = wl.wk, pos = wl.pos;
           // every lane stores (an idle one too, at a position of its own)
           if (pos < d.n_a || pos >= n_vals) return bad("a factor position outside the factor array");
           if (val_owner[pos] >= 0) return bad("a factor position written twice");
           val_owner[pos] = c;
           if (wk == D::kWriteL) val_level[pos] = l;
           if (wk == D::kWritePivot) {
-            int& owner = dpos_chunk.at(rr[0] & D::kRsIndexMask);
+            int& owner = dpos_chunk.at(wl.k);
             if (owner >= 0) return bad("a pivot written twice");
             owner = c;
           }
@@ -1252,6 +1280,33 @@ bool emulate_device_lu(const DevicePlanHost& d, int nnz_j, const double* jv, con
     bool fail = false;
     const int TL = D::track_lanes(meta);   // lanes per track of the row's chunk
     if (kind == D::kRsPivot) write_row = 0;
+    // a lane's WRITE: its slot of write row write_row to vals[pos] (an idle lane's store is never read: NaN here)
+    std::string werr;
+    auto do_write = [&](int lane, int wk, int k, int pos) -> bool {
+      const int slot = (lane / TL) * d.wstride + write_row * TL + lane % TL;
+      if (slot >= wsz + D::kSpareSlots) { werr = "write slot outside the workspace"; return false; }
+      if (pos < n_a || pos >= n_vals) { werr = "write position"; return false; }
+      if (wk == 0) {
+        vals[pos] = std::nan("");
+        return true;
+      }
+      if (slot >= wsz || slot / d.wstride != lane / TL) { werr = "write slot"; return false; }
+      if (wk == D::kWriteU) {
+        vals[pos] = ws[slot];
+      } else if (wk == D::kWritePivot) {
+        if (k >= m) { werr = "pivot write position"; return false; }
+        vals[pos] = dval[k] = ws[slot];
+      } else if (wk == D::kWriteL) {
+        if (piv[lane] == 0.0) { werr = "L write without a pivot"; return false; }
+        vals[pos] = ws[slot] / piv[lane];
+        written_row[pos] = r;
+        pending.push_back(pos);
+      } else {
+        werr = "write kind";
+        return false;
+      }
+      return true;
+    };
     for (int lane = 0; lane < 64; lane++) {
       const int* rr = &d.rs_rec[((size_t)r * 64 + lane) * 2];
       if ((int)((unsigned)rr[1] >> D::kRsMetaShift) != meta) return bad("row kind differs across lanes");
@@ -1274,43 +1329,27 @@ bool emulate_device_lu(const DevicePlanHost& d, int nnz_j, const double* jv, con
         if (fence_row[rc[1]] < 0 || fence_row[rc[1]] > load_row) return bad("row value loaded before its fence");
         ws[dst] = ws[dst] - vals[rc[1]] * ws[xj];
       } else if (kind == D::kRsPivot) {
-        if (rc[0] == -1) continue;
-        const int ps = rc[0] & 0xFFFF, lo = ps + 1, hi = ps + 1 + (int)((unsigned)rc[0] >> 16);
-        if (ps < 0 || ps >= wsz || ps / d.wstride != lane / TL || lo != ps + 1 || hi < lo || hi > (lane / TL + 1) * d.wstride)
-          return bad("pivot record");
-        piv[lane] = ws[ps];
-        fail |= piv[lane] == 0.0;
+        if (rc[0] != -1) {
+          const int ps = rc[0] & 0xFFFF, lo = ps + 1, hi = ps + 1 + (int)((unsigned)rc[0] >> 16);
+          if (ps < 0 || ps >= wsz || ps / d.wstride != lane / TL || lo != ps + 1 || hi < lo ||
+              hi > (lane / TL + 1) * d.wstride)
+            return bad("pivot record");
+          piv[lane] = ws[ps];
+          fail |= piv[lane] == 0.0;
+        }
+        if (meta & D::kRsPivotWrite) {
+          const D::WriteLane wl = d.pivot_write_lane(rr[0], rr[1], lane, TL, d.rs_wr0[r]);
+          if (!do_write(lane, wl.wk, wl.k, wl.pos)) return bad(werr);
+        }
       } else if (kind == D::kRsWrite) {
-        const int wk = (int)((unsigned)rc[0] >> 24);
-        const int slot = (lane / TL) * d.wstride + write_row * TL + lane % TL;
-        // every lane reads its slot and stores to its position (an idle lane's store is never read: NaN here)
-        if (slot >= wsz + D::kSpareSlots) return bad("write slot outside the workspace");
-        if (rc[1] < n_a || rc[1] >= n_vals) return bad("write position");
-        if (wk == 0) {
-          vals[rc[1]] = std::nan("");
-          continue;
-        }
-        if (slot >= wsz || slot / d.wstride != lane / TL) return bad("write slot");
-        if (wk == D::kWriteU) {
-          vals[rc[1]] = ws[slot];
-        } else if (wk == D::kWritePivot) {
-          const int k = rc[0] & D::kRsIndexMask;
-          if (k >= m) return bad("pivot write position");
-          vals[rc[1]] = dval[k] = ws[slot];
-        } else if (wk == D::kWriteL) {
-          if (piv[lane] == 0.0) return bad("L write without a pivot");
-          vals[rc[1]] = ws[slot] / piv[lane];
-          written_row[rc[1]] = r;
-          pending.push_back(rc[1]);
-        } else {
-          return bad("write kind");
-        }
+        if (rc[1] != n_a + 64 * d.rs_wr0[r] + lane) return bad("write position out of the stream's order");
+        if (!do_write(lane, (int)((unsigned)rc[0] >> 24), rc[0] & D::kRsIndexMask, rc[1])) return bad(werr);
       } else if (kind != D::kRsNop) {
         return bad("row kind");
       }
     }
     if (fail) return bad("zero pivot");
-    if (kind == D::kRsWrite) write_row++;
+    if (kind == D::kRsWrite || (kind == D::kRsPivot && (meta & D::kRsPivotWrite))) write_row++;
     if (meta & D::kRsLevelEnd) {
       for (int v : pending) fence_row[v] = r;
       pending.clear();

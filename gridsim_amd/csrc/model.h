@@ -163,7 +163,8 @@ struct DevicePlanHost {
   //   (a lane with nothing to assign or update is pointed at its own spare slot spare(lane) past the tracks'
   //   workspaces, so the device runs these rows with every lane active and no branch)
   //   PIVOT   x = pivot slot ps | (candidates nc << 16) (the candidates are slots ps + 1 .. ps + nc), y = pivot step;
-  //           x = -1: idle track
+  //           x = -1: idle track. With kRsPivotWrite (every chunk's) it also does the chunk's write row 0, as a WRITE
+  //           row would (pivot_write_lane), so the chunk's WRITE rows start at w = 1
   //   WRITE   slot: track base + w * TL + lane % TL for the chunk's w-th WRITE row (TL lanes per track); every lane
   //           stores its slot to vals[y] (y = n_a + 64 * the stream's WRITE-row count + lane, so one WRITE row is one
   //           contiguous 64-value store): an L slot divided by its column's pivot, a U slot or the pivot as it is, an
@@ -175,11 +176,25 @@ struct DevicePlanHost {
   // ahead, is then final.
   static constexpr int kRsNop = 0, kRsAssign = 1, kRsUpdate = 2, kRsPivot = 3, kRsWrite = 4;
   static constexpr int kRsKindMask = 7, kRsClear = 8, kRsLevelEnd = 16;
+  // kRsPivotWrite (on a PIVOT row): the row also does the chunk's first WRITE row (write row 0), its lanes' kinds and
+  // positions derived (pivot_write_lane), so a chunk costs one row less
+  static constexpr int kRsPivotWrite = 128;
   static constexpr int kWriteU = 1, kWritePivot = 2, kWriteL = 3;
   static constexpr int kRsValueAhead = 8;
   static constexpr int kRsMetaShift = 24, kRsIndexMask = (1 << kRsMetaShift) - 1;
   int n_rs = 0, n_a = 0;
   vector<int> rs_rec, rs_meta, rs_level;            // [2 * 64 * n_rs] (meta folded into y), [n_rs], [n_rs]
+  vector<int> rs_wr0;                               // [n_rs]: rows that write factors (WRITE, PIVOT+WRITE) before r
+  // The WRITE a PIVOT+WRITE row's lane does: kind (kWrite*, 0: idle) and factor position, from its PIVOT record (x,
+  // y), the row's lanes per track and the stream's write rows before it.
+  struct WriteLane { int wk, k, pos; };
+  WriteLane pivot_write_lane(int x, int y, int lane, int tl, int wr0) const {
+    const int t = lane % tl;
+    if (x == -1) return {0, 0, n_a + 64 * wr0 + lane};
+    const int pt = (x & 0xFFFF) - (lane / tl) * wstride, nc = (int)((unsigned)x >> 16);
+    const int wk = t < pt ? kWriteU : t == pt ? kWritePivot : t <= pt + nc ? kWriteL : 0;
+    return {wk, y & kRsIndexMask, n_a + 64 * wr0 + lane};
+  }
   // For level-parallel schedules: chunk c's rows are [rs_chunk_row[c], rs_chunk_row[c + 1]) of the row stream, and
   // level l's chunks are [rs_level_chunk[l], rs_level_chunk[l + 1]).
   vector<int> rs_chunk_row, rs_level_chunk;

@@ -41,6 +41,7 @@ struct DevPlan {
   // [(n_rs + kRsPadRings kStreamPad) / 2][64] pairs of rows: lane l of pair q holds (x, y) of rows 2q and 2q + 1;
   // y = 24-bit field | (kind | flags) << 24
   const int4* rs_rec;
+  const int32_t* rs_wr0;              // per row: the rows before it that write factors (WRITE, PIVOT + WRITE)
   const int32_t* ch_a_src;            // per A entry: J CSR position (the seam fills vals from its matrix)
   // Triangular-solve step streams: per lane and step (x, y): PRODUCT (col, value index | kind << 24) and CHAIN
   // (start | len << 16, row | (kind | flags) << 24) steps (model.h DevicePlanHost). The factors live where the
@@ -74,7 +75,7 @@ struct DevPlan {
 
 // refactor row kinds and flags, write kinds (model.h DevicePlanHost)
 constexpr int kRsNop = 0, kRsAssign = 1, kRsUpdate = 2, kRsPivot = 3, kRsWrite = 4;
-constexpr int kRsKindMask = 7, kRsClear = 8, kRsLevelEnd = 16;
+constexpr int kRsKindMask = 7, kRsClear = 8, kRsLevelEnd = 16, kRsPivotWrite = 128;
 constexpr int kRsWidthShift = 5;   // a chunk's lanes per track: kTrackLanes << ((meta >> kRsWidthShift) & 3)
 constexpr int kRsMetaShift = 24, kRsIndexMask = (1 << kRsMetaShift) - 1;
 constexpr int kWriteU = 1, kWritePivot = 2, kWriteL = 3;

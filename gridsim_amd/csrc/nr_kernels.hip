@@ -411,7 +411,9 @@ __device__ __forceinline__ double mismatch_jacobian(const DevPlan& P, const Inst
 // Each WRITE row stores its 64 lanes' slots to 64 consecutive factor positions of vals (L divided by the pivot, U and
 // the pivot as they are; an idle lane to a position nothing reads), one contiguous store; the later columns' updates
 // and the triangular solves read the factors there (a track's 8 lanes read 8 consecutive slots of one column, one or
-// two memory segments), and the pivots go to dval by pivot step as well.
+// two memory segments), and the pivots go to dval by pivot step as well. A chunk's first write row is done by its
+// PIVOT row (kRsPivotWrite), each lane's kind taken from the pivot record and its position from the stream's count of
+// factor-writing rows (P.rs_wr0 at a level-parallel chunk's first row).
 constexpr int kRing = kStreamPad, kHalf = kRing / 2, kAhead = kRsValueAhead;
 static_assert(kAhead < kRing, "the value of a row is loaded after its record");
 
@@ -465,6 +467,7 @@ __device__ __forceinline__ int refactor(const DevPlan& P, GLOBAL T* vals, GLOBAL
   T piv = T(1);
   int failed = -1;
   int write_row = 0;   // WRITE rows since the chunk's PIVOT row (a WRITE row's slots follow from it)
+  int n_wr = CHUNK ? P.rs_wr0[row0] : 0;   // the stream's factor-writing rows before this one
   // (a failed pivot check ends the loop at the next ring boundary, not at once: an exit from inside the unrolled body
   // would make the compiler drain every load in flight at each row; the rows run meanwhile only write factors that
   // the caller discards)
@@ -521,6 +524,20 @@ __device__ __forceinline__ int refactor(const DevPlan& P, GLOBAL T* vals, GLOBAL
         // any failing track fails the factorisation (the first failure's lowest lane's column reported)
         const unsigned long long fmask = __ballot(fail >= 0);
         if (fmask && failed < 0) failed = __shfl(fail, __ffsll((long long)fmask) - 1, kWave);
+        if (meta & kRsPivotWrite) {
+          // the chunk's write row 0: slot t = lane % TLr of the lane's track; its kind from the pivot record (U before
+          // the pivot slot, the pivot, L up to the last candidate, idle past it or on an idle track), its position the
+          // stream's next 64
+          const int tb = (lane >> wsh) * P.wstride, t = lane & (TLr - 1);
+          const int pt = r4.x == -1 ? -1 : (r4.x & 0xFFFF) - tb, nc = (int)((unsigned)r4.x >> 16);
+          const int wk = pt < 0 ? 0 : t < pt ? kWriteU : t == pt ? kWritePivot : t <= pt + nc ? kWriteL : 0;
+          const T v = ws[tb + t];
+          const T lv = v / piv;
+          __builtin_nontemporal_store(wk == kWriteL ? lv : v, &vals[P.n_a + 64 * n_wr + lane]);
+          if (wk == kWritePivot) dval[r4.y & kRsIndexMask] = v;
+          write_row = 1;
+          n_wr++;
+        }
         lds_drained();
       } else if (kind == kRsWrite) {
         // every lane reads its slot (an idle lane a slot inside the workspace), forms the L quotient and stores to
@@ -531,6 +548,7 @@ __device__ __forceinline__ int refactor(const DevPlan& P, GLOBAL T* vals, GLOBAL
         __builtin_nontemporal_store(wk == kWriteL ? lv : v, &vals[pos]);
         if (wk == kWritePivot) dval[r4.x & kRsIndexMask] = v;
         write_row++;
+        n_wr++;
         lds_drained();
       }
       // the level's L columns are written: publish them before any later row loads them
