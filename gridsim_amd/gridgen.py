@@ -220,6 +220,26 @@ def synthetic_grid(n_bus: int, n_branch: int, n_pv: int, seed: int, *, xfmr_frac
 
 # ---- the five workloads -------------------------------------------------------------------------------------------
 
+def hub_grid(n_spokes: int = 100, seed: int = BASE_SEED + 11):
+    """The 118-bus synthetic grid with one bus tied to n_spokes others by long lines (x 2 pu): its Y row then holds
+    n_spokes + (its own lines) + 1 cells, far longer than any other row, for testing the paths sized by the longest row
+    (the mismatch pass's blocks and scratch)."""
+    from dataclasses import replace
+    net = config1_grid()
+    rng = np.random.default_rng(seed)
+    hub = int(np.argmax(np.bincount(np.concatenate([net.br_from, net.br_to]), minlength=len(net.bus_type))))
+    others = np.array([i for i in range(len(net.bus_type)) if i != hub])
+    to = np.sort(rng.choice(others, size=n_spokes, replace=False))
+    k = len(to)
+    cat = lambda a, b: np.concatenate([a, b])
+    return replace(net, br_from=cat(net.br_from, np.full(k, hub)), br_to=cat(net.br_to, to),
+                   br_r=cat(net.br_r, np.full(k, 0.2)), br_x=cat(net.br_x, np.full(k, 2.0)),
+                   br_b=cat(net.br_b, np.zeros(k)), br_tap=cat(net.br_tap, np.zeros(k)),
+                   br_shift_deg=cat(net.br_shift_deg, np.zeros(k)),
+                   br_in_service=cat(net.br_in_service, np.ones(k, dtype=np.uint8)),
+                   br_rate_a=cat(net.br_rate_a, np.zeros(k)), name=f"hub{n_spokes}")
+
+
 def config1_grid(seed: int = BASE_SEED + 1) -> Network:
     """118 buses (1 slack / 53 PV / 64 PQ), 186 branches, flat start."""
     return synthetic_grid(118, 186, 53, seed, name="synthetic-118")

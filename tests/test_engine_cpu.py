@@ -98,7 +98,7 @@ def test_product_plan_equals_oracle_plan(eng, oracle, case, flat):
     np.testing.assert_array_equal(plan.ui, lu.ui)
 
 
-@pytest.mark.parametrize("case", ["case9", "case118", "case300", "synthetic-2000", "synthetic-10000"])
+@pytest.mark.parametrize("case", ["case9", "case118", "case300", "synthetic-2000", "synthetic-10000", "hub"])
 def test_device_lu_indexing_emulated_on_host(eng, case):
     """The device refactor (level chunks, 64-record rows) and level-scheduled solves, emulated on the host with every
     index checked against its buffer, solve J dx = f of the start point to FP64 accuracy."""
@@ -107,6 +107,8 @@ def test_device_lu_indexing_emulated_on_host(eng, case):
         net = gridgen.config2_grid()
     elif case == "synthetic-10000":
         net = gridgen.sized_grid(10000, gridgen.BASE_SEED + 3)
+    elif case == "hub":
+        net = gridgen.hub_grid()
     else:
         net = matpower_case(case)
     plan = eng.HostPlan(net, eng.NrOptions(flat_start=True))

@@ -108,6 +108,25 @@ def test_n1_batch_118_matches_oracle(ctx, eng, oracle, n1_118, lu):
     _compare(oracle, r.results, r.vm, r.va, rres, rvm, rva, f"N-1 118 {lu}")
 
 
+@pytest.mark.parametrize("lu", ["fp64", "fp32"])
+def test_n1_hub_grid_matches_oracle(ctx, eng, oracle, lu):
+    """A grid with one Y row of 107 cells (a hub bus tied to 100 others): the mismatch pass's blocks of buses hold
+    that row alone or with few others (its cells more than a lane block), in FP64 and FP32 workspaces; every outage's
+    verdict and voltages against the oracle."""
+    from gridsim_amd import gridgen
+    net = gridgen.hub_grid()
+    rec, vm, va = oracle.solve(net, oracle.options(flat_start=True, max_iterations=60))
+    warm = net.with_start(vm, np.degrees(va))
+    outs = gridgen.config2_outages(warm)
+    rres, rvm, rva = oracle.screen(warm, outs, oracle.options(max_iterations=10), threads=8)
+    opts = eng.NrOptions(max_iterations=10, lu_precision=eng.LU_FP64 if lu == "fp64" else eng.LU_FP32,
+                         refine_steps=0 if lu == "fp64" else 1)
+    dn = eng.DeviceNetwork(ctx, warm, opts)
+    r = dn.solve(outages=outs)
+    print(f"hub N-1 {lu}: {len(outs)} outages, kernel {r.kernel_ms:.2f} ms", flush=True)
+    _compare(oracle, r.results, r.vm, r.va, rres, rvm, rva, f"N-1 hub {lu}")
+
+
 def test_plain_path_equals_batched_fp64(ctx, eng, n1_118):
     warm, outs, _ = n1_118
     opts = eng.NrOptions(max_iterations=10, lu_precision=eng.LU_FP64)
