@@ -684,6 +684,84 @@ __device__ __forceinline__ void solve_stream(GLOBAL const v2i* rec, int step0, i
   }
 }
 
+// Loops over a per-instance array with a dependent gather per entry take kBatch entries per thread at a time, every
+// load issued before any is used: one entry at a time each would pay the full memory latency. Thread t of nt takes
+// entries t, t + nt, ...
+
+// x[pinv[r]] = rhs[r] (the solve vector from the right-hand side, row-permuted)
+template <typename XP, typename T>
+__device__ __forceinline__ void permute_in(XP x, GLOBAL const int* pinv, GLOBAL const double* rhs, int m, int t,
+                                           int nt) {
+  constexpr int kBatch = 8;
+  for (int r0 = t; r0 < m; r0 += kBatch * nt) {
+    int pr[kBatch];
+    double rv[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; u++) {
+      const int r = r0 + u * nt < m ? r0 + u * nt : 0;
+      pr[u] = pinv[r];
+      rv[u] = rhs[r];
+    }
+#pragma unroll
+    for (int u = 0; u < kBatch; u++)
+      if (r0 + u * nt < m) x[pr[u]] = (T)rv[u];
+  }
+}
+
+// This thread's max |dx[k]| and whether one is not finite
+__device__ __forceinline__ double dx_max(GLOBAL const double* dx, int m, int t, int nt, int* bad) {
+  constexpr int kBatch = 8;
+  double lmax = 0.0;
+  int lbad = 0;
+  for (int k0 = t; k0 < m; k0 += kBatch * nt) {
+    double v[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; u++) v[u] = dx[k0 + u * nt < m ? k0 + u * nt : 0];
+#pragma unroll
+    for (int u = 0; u < kBatch; u++) {
+      if (k0 + u * nt >= m) continue;
+      lbad |= !isfinite(v[u]);
+      lmax = fabs(v[u]) > lmax ? fabs(v[u]) : lmax;
+    }
+  }
+  *bad = lbad;
+  return lmax;
+}
+
+// ApplyStep: va += damp dx(P), vm += damp dx(Q) floored at 0.01, each bus's unknowns at ppos / qpos of dx
+template <typename T>
+__device__ __forceinline__ void apply_step(const Inst<T>& I, GLOBAL const int* ppos, GLOBAL const int* qpos,
+                                           double damp, int n, int t, int nt) {
+  constexpr int kBatch = 8;
+  for (int i0 = t; i0 < n; i0 += kBatch * nt) {
+    int pp[kBatch], qq[kBatch];
+    double va[kBatch], vm[kBatch], dp[kBatch], dq[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; u++) {
+      const int i = i0 + u * nt < n ? i0 + u * nt : 0;
+      pp[u] = ppos[i];
+      qq[u] = qpos[i];
+      va[u] = I.va[i];
+      vm[u] = I.vm[i];
+    }
+#pragma unroll
+    for (int u = 0; u < kBatch; u++) {
+      dp[u] = I.dx[pp[u] >= 0 ? pp[u] : 0];
+      dq[u] = I.dx[qq[u] >= 0 ? qq[u] : 0];
+    }
+#pragma unroll
+    for (int u = 0; u < kBatch; u++) {
+      const int i = i0 + u * nt;
+      if (i >= n) continue;
+      if (pp[u] >= 0) I.va[i] = va[u] + damp * dp[u];
+      if (qq[u] >= 0) {
+        const double v = vm[u] + damp * dq[u];
+        I.vm[i] = v < 0.01 ? 0.01 : v;
+      }
+    }
+  }
+}
+
 // OUTK: out in pivot-step order (out[k] = x[k], one contiguous store per 64 entries) instead of the unknowns' own order
 // (out[q[k]] = x[k]).
 template <typename T, bool XLDS, typename XP, bool OUTK = false>
@@ -693,7 +771,7 @@ __device__ __forceinline__ void lu_solve(const DevPlan& P, GLOBAL const T* fac, 
   const int m = P.m;
   GLOBAL const int* pinv = glob(P.pinv);
   GLOBAL const int* q = glob(P.q);
-  for (int r = lane; r < m; r += kWave) x[pinv[r]] = (T)rhs[r];
+  permute_in<XP, T>(x, pinv, rhs, m, lane, kWave);
   if (XLDS) WAVE_SYNC(); else WG_SYNC();
   solve_stream<T, false, XLDS>(globv<v2i>(P.fs_rec), 0, P.fs_steps, fac, dval, x, scr, lane);
   solve_stream<T, true, XLDS>(globv<v2i>(P.bs_rec), 0, P.bs_steps, fac, dval, x, scr, lane);
@@ -799,7 +877,18 @@ __global__ __launch_bounds__(kWave * (PAIR ? kPairWaves : 1), 2) void nr_batch_k
   const int was = prm.init ? 0 : st->done;
   const bool fresh = prm.init || was == kDoneFresh;
   if (fresh || prm.load_start) {
-    for (int i = lane; i < n; i += kWave) { I.vm[i] = I.vm_start[i]; I.va[i] = I.va_start[i]; }
+    for (int i0 = lane; i0 < n; i0 += 4 * kWave) {
+      double a[4], b[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int i = i0 + u * kWave < n ? i0 + u * kWave : 0;
+        a[u] = I.vm_start[i];
+        b[u] = I.va_start[i];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        if (i0 + u * kWave < n) { I.vm[i0 + u * kWave] = a[u]; I.va[i0 + u * kWave] = b[u]; }
+    }
     WG_SYNC();
   }
   if (!fresh && was) return;
@@ -868,27 +957,13 @@ __global__ __launch_bounds__(kWave * (PAIR ? kPairWaves : 1), 2) void nr_batch_k
     }
     tick(t_sol);
     if (PAIR && lock_timeout) { step_failures++; break; }   // the solve vector never came free: the run ends
-    double lmax = 0.0;
-    int lbad = 0;
-    for (int k = lane; k < m; k += kWave) {
-      const double v = I.dx[k];
-      lbad |= !isfinite(v);
-      lmax = fabs(v) > lmax ? fabs(v) : lmax;
-    }
+    int lbad;
+    const double lmax = dx_max(I.dx, m, lane, kWave, &lbad);
     if (wave_any(lbad)) { step_failures++; break; }
     const double max_abs = wave_max(lmax);
     const double damp = max_abs > 0.4 ? 0.4 / max_abs : 1.0;
     // (dx in pivot-step order: each bus's unknowns found through ppos_k / qpos_k)
-    GLOBAL const int* ppos = glob(P.ppos_k);
-    GLOBAL const int* qpos = glob(P.qpos_k);
-    for (int i = lane; i < n; i += kWave) {
-      const int pp = ppos[i], qq = qpos[i];
-      if (pp >= 0) I.va[i] = I.va[i] + damp * I.dx[pp];
-      if (qq >= 0) {
-        double v = I.vm[i] + damp * I.dx[qq];
-        I.vm[i] = v < 0.01 ? 0.01 : v;
-      }
-    }
+    apply_step(I, glob(P.ppos_k), glob(P.qpos_k), damp, n, lane, kWave);
     WG_SYNC();
     tick(t_upd);
     last_from = max_mis;
@@ -897,9 +972,17 @@ __global__ __launch_bounds__(kWave * (PAIR ? kPairWaves : 1), 2) void nr_batch_k
   }
   // final verdict (NewtonShell.AssembleResult precedence: NonFinite > Converged > Diverged > MaxIterations)
   int lnf = 0;
-  for (int i = lane; i < n; i += kWave)
-    lnf |= !(isfinite(I.vm[i]) && isfinite(I.va[i]) && isfinite(I.p[i] * P.base_mva) &&
-             isfinite(I.q[i] * P.base_mva));
+  for (int i0 = lane; i0 < n; i0 += 4 * kWave) {
+    double a[4], b[4], c[4], e[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int i = i0 + u * kWave < n ? i0 + u * kWave : 0;
+      a[u] = I.vm[i]; b[u] = I.va[i]; c[u] = I.p[i]; e[u] = I.q[i];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      lnf |= !(isfinite(a[u]) && isfinite(b[u]) && isfinite(c[u] * P.base_mva) && isfinite(e[u] * P.base_mva));
+  }
   const int nonfinite = wave_any(lnf) || !isfinite(max_mis);
   if (lane == 0) {
     st->iter = iter;
@@ -1289,7 +1372,7 @@ __device__ __forceinline__ void group_solve(const DevPlan& P, const Inst<T>& I, 
   GLOBAL const int* pinv = glob(P.pinv);
   GLOBAL const int* q = glob(P.q);
   GLOBAL const v4i* seg = globv<v4i>(P.gseg);
-  for (int r = tid; r < P.m; r += nt) x[pinv[r]] = (T)rhs[r];
+  permute_in<XP, T>(x, pinv, rhs, P.m, tid, nt);
   __syncthreads();
   for (int k = 0; k < P.n_gseg_fs; k++) {
     int p0, p1;
@@ -1398,28 +1481,15 @@ __global__ __launch_bounds__(kWave * kGroupMaxWaves) void group_kernel(const Ins
       __syncthreads();
     }
     tick(t_sol);
-    double lmax = 0.0, unused = 0.0;
-    int lbad = 0;
-    for (int k = tid; k < m; k += nt) {
-      const double v = I.dx[k];
-      lbad |= !isfinite(v);
-      lmax = fabs(v) > lmax ? fabs(v) : lmax;
-    }
+    double unused = 0.0;
+    int lbad;
+    double lmax = dx_max(I.dx, m, tid, nt, &lbad);
     lbad = wave_any(lbad);
     lmax = wave_max(lmax);
     group_reduce(g, set, wave, lane, W, &unused, &lmax, &lbad, false);
     if (lbad) { step_failures++; break; }
     const double damp = lmax > 0.4 ? 0.4 / lmax : 1.0;
-    GLOBAL const int* ppos = glob(P.ppos);
-    GLOBAL const int* qpos = glob(P.qpos);
-    for (int i = tid; i < n; i += nt) {
-      const int pp = ppos[i], qq = qpos[i];
-      if (pp >= 0) I.va[i] = I.va[i] + damp * I.dx[pp];
-      if (qq >= 0) {
-        double v = I.vm[i] + damp * I.dx[qq];
-        I.vm[i] = v < 0.01 ? 0.01 : v;
-      }
-    }
+    apply_step(I, glob(P.ppos), glob(P.qpos), damp, n, tid, nt);
     __syncthreads();
     tick(t_upd);
     last_from = max_mis;
